@@ -1,0 +1,376 @@
+"""Python host of the cfdgpu C-ABI: the reference's pressure API on the MI355X.
+
+Mirrors crates/cfd-craft/src/pressure.rs and the parts of grid.rs it reads:
+
+=============================  =====================================================
+reference                      here
+=============================  =====================================================
+``Grid::build`` (grid.rs:246)  ``Grid(case)``
+``pressure::apply`` (:293)     ``Grid.apply(x)``
+``center_nullspaces`` (:316)   ``Grid.center_nullspaces(x)``
+``solve`` (:329)               ``Grid.solve(b, tol, max_iterations, control)``
+``solve_scaled`` (:342)        ``Grid.solve_scaled(b, scale, tol, max_iterations, control)``
+``project`` (:469)             ``Grid.project(u, p, dt, tol, max_iterations, control)``
+``multigrid_stats`` (:100)     ``Grid.multigrid_stats()``
+``LinearInfo`` (:29-36)        the ``dict`` returned beside the solution
+``Control`` (solver.rs:75)     ``Control(deadline_s, cancel=callable)``
+coded ``Err(String)``          ``CfdError`` (``.code`` 31..36, ``"cancelled"``, ...)
+=============================  =====================================================
+
+All arithmetic runs in hand-written HIP kernels (libcfdgpu.so, built for gfx950).
+There is no CPU fallback: importing this module without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+
+import numpy as np
+
+from .case import (  # noqa: F401  (re-exported)
+    BLOCKED,
+    FLOW_NEUTRAL_RANS,
+    FLOW_VECTOR,
+    GROUND_OPEN_TOP,
+    IC0,
+    INLET,
+    INTERIOR,
+    MULTIGRID,
+    OPEN,
+    OUTLET,
+    PERIODIC,
+    SLIP,
+    WALL,
+    Block,
+    Case,
+)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(_HERE, "libcfdgpu.so")
+
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+CANCELLED, WALL_TIME = 100, 101
+
+
+class CfdError(RuntimeError):
+    """A coded failure of the pressure solve (CFD-E031..E036, cancelled, wall_time_limit)."""
+
+    def __init__(self, code: int, text: str):
+        super().__init__(text)
+        self.code = code
+
+
+class LinearInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int64),
+        ("relative_residual", ctypes.c_double),
+        ("recurrent_residual", ctypes.c_double),
+        ("operator_applications", ctypes.c_int64),
+        ("preconditioner_applications", ctypes.c_int64),
+        ("true_residual_checks", ctypes.c_int64),
+        ("cycles", ctypes.c_int64),
+        ("smoothing_cell_updates", ctypes.c_int64),
+        ("residual_cell_updates", ctypes.c_int64),
+        ("coarse_solves", ctypes.c_int64),
+        ("hierarchy_builds", ctypes.c_int64),
+        ("seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MgStats(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("level_cells", ctypes.c_int64 * 32),
+        ("level_nonzeros", ctypes.c_int64 * 32),
+        ("level_dims", (ctypes.c_int64 * 3) * 32),
+        ("allocated_bytes", ctypes.c_int64),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("warm_start", ctypes.c_int32),
+        ("poll_every", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+
+_LIB = None
+
+
+def lib():
+    """The loaded libcfdgpu.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBRARY):
+            raise ImportError(f"{LIBRARY} is not built; run `make -C cfd-craft_amd` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIBRARY)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        D = ctypes.c_double
+        I64 = ctypes.c_int64
+        sig = {
+            "cfd_grid_create": (I, [P, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_grid_destroy": (None, [P]),
+            "cfd_grid_counts": (I, [P, P]),
+            "cfd_grid_codes": (I, [P, P]),
+            "cfd_grid_components": (I, [P, P, P]),
+            "cfd_grid_set_options": (I, [P, P]),
+            "cfd_ic0_diagonal": (I, [P, P]),
+            "cfd_multigrid_stats": (I, [P, P]),
+            "cfd_apply": (I, [P, P, P]),
+            "cfd_center_nullspaces": (I, [P, P]),
+            "cfd_solve_scaled": (I, [P, P, P, D, I64, POLL_FN, P, P, P]),
+            "cfd_project": (I, [P, P, P, D, D, I64, POLL_FN, P, P]),
+            "cfd_projection_rhs": (I, [P, P, D, P, P]),
+            "cfd_precondition": (I, [P, P, P]),
+            "cfd_apply_device": (I, [P, P, P]),
+            "cfd_solve_scaled_device": (I, [P, P, P, D, I64, POLL_FN, P, P, P]),
+            "cfd_precondition_device": (I, [P, P, P]),
+            "cfd_project_device": (I, [P, P, P, D, D, I64, POLL_FN, P, P]),
+            "cfd_device_alloc": (I, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_device_free": (I, [P]),
+            "cfd_copy_to_device": (I, [P, P, ctypes.c_size_t]),
+            "cfd_copy_to_host": (I, [P, P, ctypes.c_size_t]),
+            "cfd_device_synchronize": (I, []),
+            "cfd_last_error": (ctypes.c_char_p, []),
+            "cfd_case_topology": (I, [P, P, P, P, P, I64, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int):
+    if rc:
+        raise CfdError(rc, lib().cfd_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def case_topology(case: Case, ic0: bool = False):
+    """Host-only Grid::build topology (no device needed): dict with counts, codes,
+    component per cell, anchored per component and optionally the IC(0) diagonal."""
+    c, keep = case.to_c()
+    counts = np.zeros(8, np.int64)
+    _check(lib().cfd_case_topology(ctypes.byref(c), _ptr(counts), None, None, None, 0, None))
+    n = int(counts[0])
+    codes = np.zeros(n, np.uint8)
+    comp = np.zeros(n, np.int64)
+    anch = np.zeros(max(1, int(counts[3])), np.uint8)
+    diag = np.zeros(n) if ic0 else None
+    _check(lib().cfd_case_topology(ctypes.byref(c), _ptr(counts), _ptr(codes), _ptr(comp), _ptr(anch), anch.size,
+                                   _ptr(diag) if diag is not None else None))
+    return dict(counts=counts, codes=codes, component=comp, anchored=anch[: int(counts[3])].astype(bool), ic0_diagonal=diag)
+
+
+class Control:
+    """Control::check (solver.rs:122-139) as seen by the pressure solve: a wall-time
+    deadline and an optional cancel callback, polled every 8 PCG iterations."""
+
+    def __init__(self, deadline_s: float = float("inf"), cancel=None):
+        self.deadline = time.monotonic() + deadline_s
+        self.cancel = cancel
+        self.polls = 0
+
+    def check(self) -> int:
+        self.polls += 1
+        if self.cancel is not None and self.cancel():
+            return 1
+        if time.monotonic() > self.deadline:
+            return 2
+        return 0
+
+
+class DeviceArray:
+    """A float64 (or other dtype) buffer in HBM owned by the library's allocator."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(1, self.n * self.dtype.itemsize)
+        ptr = ctypes.c_void_p()
+        _check(lib().cfd_device_alloc(self.nbytes, ctypes.byref(ptr)))
+        self.ptr = ptr
+
+    @classmethod
+    def from_host(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        _check(lib().cfd_copy_to_device(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(lib().cfd_copy_to_device(self.ptr, _ptr(a), a.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        _check(lib().cfd_copy_to_host(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            lib().cfd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Grid:
+    """A device-resident pressure grid: topology, factor or hierarchy, work vectors."""
+
+    def __init__(self, case: Case):
+        self.case = case
+        c, keep = case.to_c()
+        h = ctypes.c_void_p()
+        _check(lib().cfd_grid_create(ctypes.byref(c), ctypes.byref(h)))
+        self.h = h
+        counts = np.zeros(8, np.int64)
+        _check(lib().cfd_grid_counts(self.h, _ptr(counts)))
+        self.count, self.face_count, self.fluid_count, self.n_components, self.n_anchored = (int(v) for v in counts[:5])
+        self.tolerance = case.tolerance
+        self.max_iterations = case.max_iterations
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().cfd_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- topology
+    def codes(self) -> np.ndarray:
+        out = np.zeros(self.count, np.uint8)
+        _check(lib().cfd_grid_codes(self.h, _ptr(out)))
+        return out
+
+    def components(self):
+        comp = np.zeros(self.count, np.int64)
+        anch = np.zeros(max(1, self.n_components), np.uint8)
+        _check(lib().cfd_grid_components(self.h, _ptr(comp), _ptr(anch)))
+        return comp, anch[: self.n_components].astype(bool)
+
+    def ic0_diagonal(self) -> np.ndarray:
+        out = np.zeros(self.count)
+        _check(lib().cfd_ic0_diagonal(self.h, _ptr(out)))
+        return out
+
+    def multigrid_stats(self) -> dict:
+        s = MgStats()
+        _check(lib().cfd_multigrid_stats(self.h, ctypes.byref(s)))
+        n = s.levels
+        return dict(
+            level_cells=list(s.level_cells[:n]),
+            level_nonzeros=list(s.level_nonzeros[:n]),
+            level_dimensions=[list(s.level_dims[i]) for i in range(n)],
+            allocated_bytes=s.allocated_bytes,
+        )
+
+    def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False):
+        o = Options(int(precision), int(bool(warm_start)), 8, 0)
+        _check(lib().cfd_grid_set_options(self.h, ctypes.byref(o)))
+
+    # ---- operator and gauges
+    def apply(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.count)
+        _check(lib().cfd_apply(self.h, _ptr(x), _ptr(y)))
+        return y
+
+    def center_nullspaces(self, x) -> np.ndarray:
+        x = np.array(x, dtype=np.float64)
+        _check(lib().cfd_center_nullspaces(self.h, _ptr(x)))
+        return x
+
+    def precondition(self, r) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty(self.count)
+        _check(lib().cfd_precondition(self.h, _ptr(r), _ptr(z)))
+        return z
+
+    # ---- solves
+    @staticmethod
+    def _poll(control):
+        if control is None:
+            return POLL_FN(0), None
+        fn = POLL_FN(lambda _user: control.check())
+        return fn, fn
+
+    def solve_scaled(self, b, scale=None, tol=None, max_iterations=None, control=None):
+        tol = self.tolerance if tol is None else tol
+        max_iterations = self.max_iterations if max_iterations is None else max_iterations
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        s = np.ascontiguousarray(scale, dtype=np.float64) if scale is not None else None
+        x = np.empty(self.count)
+        info = LinearInfo()
+        fn, keep = self._poll(control)
+        _check(
+            lib().cfd_solve_scaled(
+                self.h, _ptr(b), _ptr(s) if s is not None else None, float(tol), int(max_iterations), fn, None, _ptr(x),
+                ctypes.byref(info),
+            )
+        )
+        return x, info.as_dict()
+
+    def solve(self, b, tol=None, max_iterations=None, control=None):
+        return self.solve_scaled(b, None, tol, max_iterations, control)
+
+    def solve_device(self, d_b: DeviceArray, d_x: DeviceArray, tol=None, max_iterations=None, d_scale=None, control=None):
+        tol = self.tolerance if tol is None else tol
+        max_iterations = self.max_iterations if max_iterations is None else max_iterations
+        info = LinearInfo()
+        fn, keep = self._poll(control)
+        _check(
+            lib().cfd_solve_scaled_device(
+                self.h, d_b.ptr, d_scale.ptr if d_scale is not None else None, float(tol), int(max_iterations), fn, None,
+                d_x.ptr, ctypes.byref(info),
+            )
+        )
+        return info.as_dict()
+
+    def projection_rhs(self, u, dt):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        b = np.empty(self.count)
+        t = np.empty(self.count)
+        _check(lib().cfd_projection_rhs(self.h, _ptr(u), float(dt), _ptr(b), _ptr(t)))
+        return b, t
+
+    def project(self, u, p, dt, tol=None, max_iterations=None, control=None):
+        """Returns (u, p, info); the inputs are not modified (the reference mutates in place
+        and only on success, pressure.rs:468)."""
+        tol = self.tolerance if tol is None else tol
+        max_iterations = self.max_iterations if max_iterations is None else max_iterations
+        u = np.array(u, dtype=np.float64)
+        p = np.array(p, dtype=np.float64)
+        info = LinearInfo()
+        fn, keep = self._poll(control)
+        _check(lib().cfd_project(self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn, None,
+                                 ctypes.byref(info)))
+        return u, p, info.as_dict()
+
+
+def synchronize():
+    _check(lib().cfd_device_synchronize())

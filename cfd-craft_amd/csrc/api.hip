@@ -1,0 +1,694 @@
+// C-ABI of the cfdgpu library: grid contexts, the device-resident PCG driver
+// (pressure.rs:342-436), projection (pressure.rs:440-502) and device memory
+// helpers. One HIP stream per grid; the PCG scalars live on the device and the
+// host reads them once per segment of 8 iterations, when it also polls the
+// caller's control callback (pressure.rs:387-389). Each segment is a captured
+// hipGraph replayed until the device status leaves RUNNING.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cfdgpu.h"
+#include "device.hpp"
+#include "internal.hpp"
+#include "kernels.hpp"
+
+namespace cfdgpu {
+
+thread_local std::string g_error;
+void set_error(const std::string& text) { g_error = text; }
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);     \
+      return CFD_EDEVICE;                                                               \
+    }                                                                                   \
+  } while (0)
+
+constexpr int kSegment = 8;          // PCG iterations per host poll (pressure.rs:387)
+constexpr int64_t kCoarseMax = 512;  // coarsest multigrid level solved by the dense pseudo-inverse
+constexpr int kMaxReduceBlocks = 2048;
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    n = count;
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+      return CFD_ENOMEM;
+    }
+    return 0;
+  }
+  int upload(const T* h, size_t count) {
+    int rc = alloc(count);
+    if (rc) return rc;
+    if (count) HIP_TRY(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+  }
+};
+
+template <class T>
+struct DevHierarchy {
+  std::vector<DevLevel<T>> levels;
+  std::vector<std::unique_ptr<DevBuf<T>>> store;
+  T* x0 = nullptr;  // FP32 fine-level iterate (mixed precision only)
+  DevBuf<T> x0buf;
+};
+
+}  // namespace cfdgpu
+
+using namespace cfdgpu;
+
+struct cfd_grid {
+  HostGrid host;
+  HostHierarchy hier;
+  bool has_hier = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Geo geo;
+  FaceGeo fgeo;
+  int nb = 1;  // reduction grid
+  cfd_options opt{CFD_PRECISION_FP64, 0, kSegment, 0};
+  // topology
+  DevBuf<uint8_t> codes;
+  DevBuf<int> cidx;
+  DevBuf<double> D;
+  int n_closed = 0;
+  double inv_count1 = 0.0;
+  DevBuf<int> members, chunk_begin, comp_first_chunk;
+  DevBuf<double> inv_count;
+  int n_chunks = 0;
+  // PCG state and vectors
+  DevBuf<double> x, r, bc, z, p0, p1, ap, b, scale, xo, phi, part;
+  DevBuf<double> mean_b, mean_z, mean_x, sums;
+  DevBuf<unsigned int> tickets;
+  DevBuf<PcgState> st;
+  PcgState* h_st = nullptr;
+  // multigrid
+  DevHierarchy<double> h64;
+  DevHierarchy<float> h32;
+  DevBuf<double> G;
+  // captured iteration segment, per precision
+  hipGraphExec_t seg[2] = {nullptr, nullptr};
+  ~cfd_grid() {
+    for (auto& g : seg)
+      if (g) (void)hipGraphExecDestroy(g);
+    if (h_st) (void)hipHostFree(h_st);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int fill_geo(cfd_grid* g) {
+  const HostGrid& h = g->host;
+  Geo& G = g->geo;
+  for (int a = 0; a < 3; ++a) {
+    G.n[a] = (int)h.n[a];
+    G.per[a] = h.periodic[a] ? 1 : 0;
+    G.wi[a] = h.w_int[a];
+    G.wo[a] = h.w_out[a];
+  }
+  G.N = h.count;
+  G.sy = h.n[0];
+  G.sz = h.n[0] * h.n[1];
+  FaceGeo& F = g->fgeo;
+  long long off = 0;
+  for (int a = 0; a < 3; ++a) {
+    for (int d = 0; d < 3; ++d) F.dims[a][d] = (int)h.n[d];
+    if (!h.periodic[a]) F.dims[a][a] += 1;
+    F.off[a] = off;
+    off += (long long)F.dims[a][0] * F.dims[a][1] * F.dims[a][2];
+    F.dx[a] = h.dx[a];
+  }
+  F.total = off;
+  long long nb = (h.count + 255) / 256;
+  g->nb = (int)std::max<long long>(1, std::min<long long>(nb, kMaxReduceBlocks));
+  return 0;
+}
+
+template <class T>
+int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
+  const HostHierarchy& H = g->hier;
+  dh.levels.clear();
+  dh.store.clear();
+  auto buf = [&](const std::vector<double>& v) -> T* {
+    std::vector<T> t(v.begin(), v.end());
+    dh.store.emplace_back(new DevBuf<T>());
+    if (dh.store.back()->upload(t.data(), t.size())) return nullptr;
+    return dh.store.back()->p;
+  };
+  auto zeros = [&](size_t n) -> T* {
+    dh.store.emplace_back(new DevBuf<T>());
+    if (dh.store.back()->alloc(n)) return nullptr;
+    if (hipMemset(dh.store.back()->p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
+    return dh.store.back()->p;
+  };
+  for (const HostLevel& L : H.coarse) {
+    DevLevel<T> d;
+    std::memset(&d.g, 0, sizeof d.g);
+    for (int a = 0; a < 3; ++a) {
+      d.g.n[a] = (int)L.n[a];
+      d.g.per[a] = g->host.periodic[a] ? 1 : 0;
+    }
+    d.g.N = L.count;
+    d.g.sy = L.n[0];
+    d.g.sz = L.n[0] * L.n[1];
+    for (int a = 0; a < 3; ++a) {
+      d.wp[a] = buf(L.wp[a]);
+      if (!d.wp[a]) return CFD_ENOMEM;
+    }
+    d.diag = buf(L.diag);
+    d.x = zeros((size_t)L.count);
+    d.b = zeros((size_t)L.count);
+    if (!d.diag || !d.x || !d.b) return CFD_ENOMEM;
+    dh.levels.push_back(d);
+  }
+  return 0;
+}
+
+int setup_device(cfd_grid* g) {
+  HostGrid& h = g->host;
+  HIP_TRY(hipGetDevice(&g->device));
+  HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&g->ev0));
+  HIP_TRY(hipEventCreate(&g->ev1));
+  HIP_TRY(hipHostMalloc((void**)&g->h_st, sizeof(PcgState), hipHostMallocDefault));
+  fill_geo(g);
+  size_t N = (size_t)h.count;
+  int rc;
+  if ((rc = g->codes.upload(h.codes.data(), N))) return rc;
+  g->n_closed = h.n_closed();
+  if (g->n_closed == 1) g->inv_count1 = 1.0 / (double)h.closed_offsets[1];
+  if (g->n_closed > 1) {
+    if ((rc = g->cidx.upload(h.closed_index.data(), N))) return rc;
+    // chunks of at most max(2048, size/64) members, never spanning two components
+    std::vector<int> begin, first;
+    std::vector<double> inv;
+    for (int cl = 0; cl < g->n_closed; ++cl) {
+      int64_t b0 = h.closed_offsets[(size_t)cl], b1 = h.closed_offsets[(size_t)cl + 1];
+      int64_t size = b1 - b0;
+      int64_t ch = std::max<int64_t>(2048, (size + 63) / 64);
+      first.push_back((int)begin.size());
+      for (int64_t s = b0; s < b1; s += ch) begin.push_back((int)s);
+      inv.push_back(1.0 / (double)size);
+    }
+    first.push_back((int)begin.size());
+    g->n_chunks = (int)begin.size();
+    begin.push_back((int)h.closed_offsets.back());
+    if ((rc = g->members.upload(h.closed_members.data(), h.closed_members.size()))) return rc;
+    if ((rc = g->chunk_begin.upload(begin.data(), begin.size()))) return rc;
+    if ((rc = g->comp_first_chunk.upload(first.data(), first.size()))) return rc;
+    if ((rc = g->inv_count.upload(inv.data(), inv.size()))) return rc;
+  }
+  size_t nm = (size_t)std::max(3, g->n_closed);
+  for (DevBuf<double>* v : {&g->x, &g->r, &g->bc, &g->z, &g->p0, &g->p1, &g->ap, &g->b, &g->scale, &g->xo, &g->phi})
+    if ((rc = v->alloc(N))) return rc;
+  for (DevBuf<double>* v : {&g->mean_b, &g->mean_z, &g->mean_x})
+    if ((rc = v->alloc(nm))) return rc;
+  if ((rc = g->sums.alloc(2 * nm))) return rc;
+  if ((rc = g->part.alloc((size_t)std::max(2 * g->nb, 2 * g->n_chunks + 2)))) return rc;
+  if ((rc = g->tickets.alloc(8))) return rc;
+  HIP_TRY(hipMemset(g->tickets.p, 0, 8 * sizeof(unsigned int)));
+  if ((rc = g->st.alloc(1))) return rc;
+  HIP_TRY(hipMemset(g->st.p, 0, sizeof(PcgState)));
+  if (h.preconditioner == CFD_IC0) {
+    build_ic0_diagonal(h);
+    if ((rc = g->D.upload(h.ic0_diag.data(), N))) return rc;
+  } else {
+    rc = build_hierarchy(h, g->hier, kCoarseMax);
+    if (rc) return rc;
+    g->has_hier = true;
+    if ((rc = upload_hierarchy(g, g->h64))) return rc;
+    if ((rc = upload_hierarchy(g, g->h32))) return rc;
+    if ((rc = g->h32.x0buf.alloc(N))) return rc;
+    g->h32.x0 = g->h32.x0buf.p;
+    if ((rc = g->G.upload(g->hier.direct.data(), g->hier.direct.size()))) return rc;
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+// Per closed component: mean of src into `mean` (and the E031 test when scale is given).
+void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st,
+                   unsigned int* ticket) {
+  if (g->n_closed == 0) return;
+  if (g->n_closed == 1) {
+    launch_csum1(g->geo.N, g->nb, g->codes.p, src, scale, e031, g->inv_count1, mean, g->sums.p, st, g->part.p, ticket,
+                 g->stream);
+  } else {
+    launch_csumN(g->n_chunks, g->members.p, g->chunk_begin.p, g->comp_first_chunk.p, g->n_closed, g->inv_count.p, src,
+                 scale, e031, mean, g->sums.p, st, g->part.p, ticket, g->stream);
+  }
+}
+const double* gauge_means(cfd_grid*, double* mean) { return mean; }
+
+void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
+  if (g->host.preconditioner == CFD_IC0) {
+    launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+    return;
+  }
+  const int nlev = (int)g->h64.levels.size();
+  const int dn = (int)g->hier.direct_n;
+  if (g->opt.precision == CFD_PRECISION_MIXED) {
+    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->h32.x0, nlev, g->h32.levels.data(), dn, g->G.p, z, st,
+                                 g->stream);
+  } else {
+    launch_vcycle<double, double>(g->geo, g->codes.p, r, z, nlev, g->h64.levels.data(), dn, g->G.p, nullptr, st,
+                                  g->stream);
+  }
+}
+
+// One PCG iteration (pressure.rs:386-434). p alternates between p0 and p1.
+void enqueue_iteration(cfd_grid* g, int parity) {
+  PcgState* st = g->st.p;
+  double* pold = parity ? g->p1.p : g->p0.p;
+  double* pnew = parity ? g->p0.p : g->p1.p;
+  const double* mz = gauge_means(g, g->mean_z.p);
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  launch_pcg_op(g->geo, g->nb, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+  launch_pcg_axpy(g->geo.N, g->nb, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+  launch_pcg_true(g->geo, g->nb, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
+  enqueue_precondition(g, g->r.p, g->z.p, st);
+  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st, &st->ticket);
+  launch_pcg_rz(g->geo.N, g->nb, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, 0, g->stream);
+}
+
+int segment_graph(cfd_grid* g, hipGraphExec_t* out) {
+  int slot = g->opt.precision == CFD_PRECISION_MIXED ? 1 : 0;
+  if (g->seg[slot]) {
+    *out = g->seg[slot];
+    return 0;
+  }
+  hipGraph_t graph;
+  HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  for (int it = 0; it < kSegment; ++it) enqueue_iteration(g, it & 1);
+  HIP_TRY(hipStreamEndCapture(g->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&g->seg[slot], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return CFD_EDEVICE;
+  }
+  *out = g->seg[slot];
+  return 0;
+}
+
+int finish_status(cfd_grid* g, const PcgState& s, int64_t max_iterations) {
+  switch (s.status) {
+    case ST_CONVERGED:
+    case ST_ZERO:
+      return CFD_OK;
+    case ST_E031:
+      set_error("CFD-E031: incompatible flux in a closed fluid component");
+      return CFD_E031;
+    case ST_E032:
+      set_error("CFD-E032: pressure CG lost positive definiteness");
+      return CFD_E032;
+    case ST_E033: {
+      char buf[96];
+      std::snprintf(buf, sizeof buf, "CFD-E033: pressure iteration limit %lld", (long long)max_iterations);
+      set_error(buf);
+      return CFD_E033;
+    }
+    default:
+      set_error("pressure solve ended in an unknown device state");
+      return CFD_EDEVICE;
+  }
+  (void)g;
+}
+
+int read_state(cfd_grid* g) {
+  HIP_TRY(hipMemcpyAsync(g->h_st, g->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                 cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info) {
+  if (!(tol > 0.0) || max_iterations <= 0) {
+    set_error("pressure tolerance must be positive and the iteration limit nonzero (CFD-E008/E009)");
+    return CFD_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  PcgState* st = g->st.p;
+  cfd_linear_info local;
+  std::memset(&local, 0, sizeof local);
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  launch_state_init(st, tol, max_iterations, g->stream);
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  enqueue_gauge(g, d_b, d_scale, 1, g->mean_b.p, st, &st->ticket);
+  launch_pcg_init(g->geo.N, g->nb, g->codes.p, d_b, gauge_means(g, g->mean_b.p), cidx, g->r.p, g->bc.p, g->x.p, st,
+                  g->part.p, g->stream);
+  enqueue_precondition(g, g->r.p, g->z.p, st);
+  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st, &st->ticket);
+  launch_pcg_rz(g->geo.N, g->nb, g->codes.p, g->r.p, g->z.p, gauge_means(g, g->mean_z.p), cidx, st, g->part.p, 1,
+                g->stream);
+  int rc = read_state(g);
+  if (rc) return rc;
+  hipGraphExec_t seg = nullptr;
+  while (g->h_st->status == ST_RUNNING) {
+    if (poll) {
+      int p = poll(user);
+      if (p == 1) {
+        set_error("cancelled");
+        return CFD_CANCELLED;
+      }
+      if (p == 2) {
+        set_error("wall_time_limit");
+        return CFD_WALL_TIME;
+      }
+    }
+    if (!seg && (rc = segment_graph(g, &seg))) return rc;
+    HIP_TRY(hipGraphLaunch(seg, g->stream));
+    if ((rc = read_state(g))) return rc;
+  }
+  const PcgState s = *g->h_st;
+  local.iterations = s.status == ST_ZERO ? 0 : s.it;
+  local.relative_residual = s.status == ST_ZERO ? 0.0 : s.true_rel;
+  local.recurrent_residual = s.rel;
+  local.true_residual_checks = s.true_checks;
+  local.operator_applications = local.iterations + s.true_checks;
+  local.preconditioner_applications = s.status == ST_ZERO ? 0 : (s.status == ST_CONVERGED ? s.it : s.it + 1);
+  if (g->has_hier) {
+    int64_t fine = g->host.count;
+    int64_t cells = fine;
+    for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
+    local.cycles = local.preconditioner_applications;
+    local.smoothing_cell_updates = local.cycles * 4 * cells;
+    local.residual_cell_updates = local.cycles * cells;
+    local.coarse_solves = local.cycles;
+  }
+  rc = finish_status(g, s, max_iterations);
+  if (rc) {
+    if (info) *info = local;
+    return rc;
+  }
+  if (s.status == ST_ZERO) {
+    launch_fill(g->geo.N, d_x, 0.0, g->stream);
+  } else {
+    enqueue_gauge(g, g->x.p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
+    launch_center(g->geo.N, g->codes.p, g->x.p, gauge_means(g, g->mean_x.p), cidx, d_x, g->stream);
+  }
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  local.seconds = ms * 1e-3;
+  if (info) *info = local;
+  return CFD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* cfd_last_error(void) { return g_error.c_str(); }
+
+int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
+  if (!c || !out) {
+    set_error("cfd_grid_create: null argument");
+    return CFD_EINVAL;
+  }
+  *out = nullptr;
+  if (c->preconditioner != CFD_IC0 && c->preconditioner != CFD_MULTIGRID) {
+    set_error("numerics.pressure_preconditioner must be ic0 or multigrid");
+    return CFD_EINVAL;
+  }
+  std::unique_ptr<cfd_grid> g(new cfd_grid());
+  int rc = build_host_grid(*c, g->host);
+  if (rc) return rc;
+  if (c->preconditioner == CFD_IC0) {
+    bool fill = false;
+    for (int a = 0; a < 3; ++a) fill |= g->host.periodic[a] && g->host.n[a] == 3;
+    if (fill) {
+      set_error("ic0: a 3-cell periodic axis gives the IC(0) factor fill coupling, which the device sweeps do not carry");
+      return CFD_EUNSUPPORTED;
+    }
+  }
+  rc = setup_device(g.get());
+  if (rc) return rc;
+  *out = g.release();
+  return CFD_OK;
+}
+
+void cfd_grid_destroy(cfd_grid* g) { delete g; }
+
+static void host_counts(const HostGrid& h, int64_t counts[8]) {
+  int64_t anchored = 0;
+  for (uint8_t a : h.anchored) anchored += a;
+  counts[0] = h.count;
+  counts[1] = h.face_count[0] + h.face_count[1] + h.face_count[2];
+  counts[2] = h.fluid_count;
+  counts[3] = (int64_t)h.anchored.size();
+  counts[4] = anchored;
+  counts[5] = h.face_count[0];
+  counts[6] = h.face_count[1];
+  counts[7] = h.face_count[2];
+}
+
+int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
+                      int64_t anchored_cap, double* ic0_diagonal) {
+  if (!c) {
+    set_error("cfd_case_topology: null case");
+    return CFD_EINVAL;
+  }
+  HostGrid h;
+  int rc = build_host_grid(*c, h);
+  if (rc) return rc;
+  if (counts) host_counts(h, counts);
+  if (codes) std::memcpy(codes, h.codes.data(), h.codes.size());
+  if (component)
+    for (int64_t i = 0; i < h.count; ++i) component[i] = h.component[(size_t)i];
+  if (anchored)
+    for (size_t k = 0; k < h.anchored.size() && (int64_t)k < anchored_cap; ++k) anchored[k] = h.anchored[k];
+  if (ic0_diagonal) {
+    build_ic0_diagonal(h);
+    std::memcpy(ic0_diagonal, h.ic0_diag.data(), h.ic0_diag.size() * sizeof(double));
+  }
+  return CFD_OK;
+}
+
+int cfd_grid_counts(const cfd_grid* g, int64_t counts[8]) {
+  host_counts(g->host, counts);
+  return CFD_OK;
+}
+
+int cfd_grid_codes(const cfd_grid* g, uint8_t* codes) {
+  std::memcpy(codes, g->host.codes.data(), g->host.codes.size());
+  return CFD_OK;
+}
+
+int cfd_grid_components(const cfd_grid* g, int64_t* component, uint8_t* anchored) {
+  const HostGrid& h = g->host;
+  for (int64_t i = 0; i < h.count; ++i) component[i] = h.component[(size_t)i];
+  for (size_t c = 0; c < h.anchored.size(); ++c) anchored[c] = h.anchored[c];
+  return CFD_OK;
+}
+
+int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
+  if (o->precision != CFD_PRECISION_FP64 && o->precision != CFD_PRECISION_MIXED) {
+    set_error("cfd_options.precision must be FP64 or MIXED");
+    return CFD_EINVAL;
+  }
+  g->opt = *o;
+  return CFD_OK;
+}
+
+int cfd_ic0_diagonal(const cfd_grid* g, double* diag) {
+  if (g->host.ic0_diag.empty()) {
+    set_error("the grid's preconditioner is not ic0");
+    return CFD_EINVAL;
+  }
+  std::memcpy(diag, g->host.ic0_diag.data(), g->host.ic0_diag.size() * sizeof(double));
+  return CFD_OK;
+}
+
+int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
+  std::memset(s, 0, sizeof *s);
+  if (!g->has_hier) {
+    set_error("the grid's preconditioner is not multigrid");
+    return CFD_EINVAL;
+  }
+  const HostGrid& h = g->host;
+  int64_t fine_nnz = h.count;
+  for (uint8_t c : h.codes)
+    for (int b = 0; b < 6; ++b) fine_nnz += (c >> b) & 1;
+  s->level_cells[0] = h.count;
+  s->level_nonzeros[0] = fine_nnz;
+  for (int a = 0; a < 3; ++a) s->level_dims[0][a] = h.n[a];
+  int L = 1;
+  int64_t bytes = h.count * (int64_t)(sizeof(double) + sizeof(float));
+  for (const HostLevel& lv : g->hier.coarse) {
+    if (L >= 32) break;
+    int64_t nnz = lv.count;
+    for (int a = 0; a < 3; ++a)
+      for (double w : lv.wp[a]) nnz += w != 0.0 ? 2 : 0;
+    s->level_cells[L] = lv.count;
+    s->level_nonzeros[L] = nnz;
+    for (int a = 0; a < 3; ++a) s->level_dims[L][a] = lv.n[a];
+    bytes += lv.count * (int64_t)(6 * sizeof(double) + 6 * sizeof(float));
+    ++L;
+  }
+  s->levels = L;
+  s->allocated_bytes = bytes + (int64_t)g->hier.direct.size() * (int64_t)sizeof(double);
+  return CFD_OK;
+}
+
+int cfd_apply_device(cfd_grid* g, const double* d_x, double* d_y) {
+  HIP_TRY(hipSetDevice(g->device));
+  launch_apply(g->geo, g->codes.p, d_x, d_y, g->stream);
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_apply(cfd_grid* g, const double* x, double* y) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(g->b.p, x, bytes, hipMemcpyHostToDevice, g->stream));
+  launch_apply(g->geo, g->codes.p, g->b.p, g->xo.p, g->stream);
+  HIP_TRY(hipMemcpyAsync(y, g->xo.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_center_nullspaces(cfd_grid* g, double* x) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(g->b.p, x, bytes, hipMemcpyHostToDevice, g->stream));
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  enqueue_gauge(g, g->b.p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
+  launch_center(g->geo.N, g->codes.p, g->b.p, gauge_means(g, g->mean_x.p), cidx, g->xo.p, g->stream);
+  HIP_TRY(hipMemcpyAsync(x, g->xo.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_precondition_device(cfd_grid* g, const double* d_r, double* d_z) {
+  HIP_TRY(hipSetDevice(g->device));
+  launch_state_init(g->st.p, 1.0, 1, g->stream);
+  enqueue_precondition(g, d_r, d_z, g->st.p);
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_precondition(cfd_grid* g, const double* r, double* z) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(g->b.p, r, bytes, hipMemcpyHostToDevice, g->stream));
+  launch_state_init(g->st.p, 1.0, 1, g->stream);
+  enqueue_precondition(g, g->b.p, g->xo.p, g->st.p);
+  HIP_TRY(hipMemcpyAsync(z, g->xo.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_solve_scaled_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                            cfd_poll_fn poll, void* poll_user, double* d_x, cfd_linear_info* info) {
+  return solve_device(g, d_b, d_scale, tol, max_iterations, poll, poll_user, d_x, info);
+}
+
+int cfd_solve_scaled(cfd_grid* g, const double* b, const double* roundoff_scale, double tol, int64_t max_iterations,
+                     cfd_poll_fn poll, void* poll_user, double* x, cfd_linear_info* info) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  HIP_TRY(hipMemcpyAsync(g->b.p, b, bytes, hipMemcpyHostToDevice, g->stream));
+  if (roundoff_scale) HIP_TRY(hipMemcpyAsync(g->scale.p, roundoff_scale, bytes, hipMemcpyHostToDevice, g->stream));
+  int rc = solve_device(g, g->b.p, roundoff_scale ? g->scale.p : nullptr, tol, max_iterations, poll, poll_user, g->xo.p,
+                        info);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(x, g->xo.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_projection_rhs(cfd_grid* g, const double* u, double dt, double* b, double* terms) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  size_t fbytes = (size_t)g->fgeo.total * sizeof(double);
+  DevBuf<double> du;
+  int rc = du.alloc((size_t)g->fgeo.total);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(du.p, u, fbytes, hipMemcpyHostToDevice, g->stream));
+  launch_projection_rhs(g->geo, g->fgeo, g->codes.p, du.p, dt, g->b.p, g->scale.p, g->stream);
+  HIP_TRY(hipMemcpyAsync(b, g->b.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(terms, g->scale.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
+                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info) {
+  HIP_TRY(hipSetDevice(g->device));
+  launch_projection_rhs(g->geo, g->fgeo, g->codes.p, d_u, dt, g->b.p, g->scale.p, g->stream);
+  int rc = solve_device(g, g->b.p, g->scale.p, tol, max_iterations, poll, poll_user, g->phi.p, info);
+  if (rc) return rc;
+  launch_project_faces(g->geo, g->fgeo, g->codes.p, g->phi.p, dt, d_u, g->stream);
+  launch_add(g->geo.N, d_p, g->phi.p, g->stream);
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  enqueue_gauge(g, d_p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
+  launch_center(g->geo.N, g->codes.p, d_p, gauge_means(g, g->mean_x.p), cidx, d_p, g->stream);
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_project(cfd_grid* g, double* u, double* p, double dt, double tol, int64_t max_iterations, cfd_poll_fn poll,
+                void* poll_user, cfd_linear_info* info) {
+  HIP_TRY(hipSetDevice(g->device));
+  size_t bytes = (size_t)g->geo.N * sizeof(double);
+  size_t fbytes = (size_t)g->fgeo.total * sizeof(double);
+  DevBuf<double> du, dp;
+  int rc = du.alloc((size_t)g->fgeo.total);
+  if (!rc) rc = dp.alloc((size_t)g->geo.N);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(du.p, u, fbytes, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(dp.p, p, bytes, hipMemcpyHostToDevice, g->stream));
+  rc = cfd_project_device(g, du.p, dp.p, dt, tol, max_iterations, poll, poll_user, info);
+  if (rc) return rc;  // u and p untouched on error
+  HIP_TRY(hipMemcpyAsync(u, du.p, fbytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(p, dp.p, bytes, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+int cfd_device_alloc(size_t bytes, void** d_ptr) {
+  HIP_TRY(hipMalloc(d_ptr, bytes));
+  return CFD_OK;
+}
+int cfd_device_free(void* d_ptr) {
+  HIP_TRY(hipFree(d_ptr));
+  return CFD_OK;
+}
+int cfd_copy_to_device(void* d_dst, const void* h_src, size_t bytes) {
+  HIP_TRY(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+int cfd_copy_to_host(void* h_dst, const void* d_src, size_t bytes) {
+  HIP_TRY(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  return CFD_OK;
+}
+int cfd_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return CFD_OK;
+}
+
+}  // extern "C"

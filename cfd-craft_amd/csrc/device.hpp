@@ -1,0 +1,65 @@
+// Plain-old-data structures shared by the kernels and the host launchers.
+#pragma once
+#include <cstdint>
+
+namespace cfdgpu {
+
+// Geometry of a structured level as the kernels see it.
+struct Geo {
+  int n[3];
+  int per[3];  // periodic flags
+  long long N;
+  long long sy, sz;  // strides of y and z (x is 1)
+  double wi[3];      // interior face weight per axis (fine level)
+  double wo[3];      // outlet face weight per axis (fine level)
+};
+
+// Device-resident PCG scalars (FP64) and control words.
+enum : int {
+  ST_RUNNING = 0,
+  ST_CONVERGED = 1,
+  ST_ZERO = 2,
+  ST_E031 = 31,
+  ST_E032 = 32,
+  ST_E033 = 33,
+};
+struct PcgState {
+  double rz;        // r . z of the current iteration
+  double denom;     // p . Ap
+  double alpha, beta;
+  double norm;      // |b_c|
+  double rel;       // recurrent |r| / norm
+  double true_rel;  // recomputed |A x - b_c| / norm
+  double tol;
+  double accept;    // 10 tol + 1e-12
+  double spare[7];
+  long long it;     // completed iterations
+  long long max_it;
+  long long true_checks;
+  int status;
+  int check;  // recurrent residual crossed tol: recompute the true residual
+  int first;  // next p update is p = z_c
+  int spare_i;
+  unsigned int ticket;
+  unsigned int pad[3];
+};
+
+// The reference's face list (grid.rs:341-429) as three blocks of face_dims each.
+struct FaceGeo {
+  long long off[3];
+  int dims[3][3];
+  double dx[3];
+  long long total;
+};
+
+// One coarse level of the geometric hierarchy on the device.
+template <class T>
+struct DevLevel {
+  Geo g;
+  T* wp[3];   // coupling to the + neighbour per axis
+  T* diag;    // diagonal (0: inactive cell)
+  T* x;
+  T* b;
+};
+
+}  // namespace cfdgpu

@@ -1,0 +1,575 @@
+// Host set-up of the device grid: case -> per-cell coupling codes, fluid
+// components, the IC(0) factor diagonal and the geometric multigrid hierarchy.
+//
+// The topology must be the reference's (grid.rs:246-464), so the mask
+// rasterization (geometry.rs:113-193, case.rs:436-443) and the face
+// classification follow the same rules; they are stored per cell (six side
+// bits) instead of as a face list, which is all the device needs.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <deque>
+
+#include "internal.hpp"
+
+namespace cfdgpu {
+namespace {
+
+constexpr int64_t NONE = -1;
+
+// Polygon membership by even-odd crossings (geometry.rs:13-26).
+bool in_polygon(const double* p, int n, double x, double y) {
+  bool inside = false;
+  for (int i = 0; i < n; ++i) {
+    const double* a = p + 2 * i;
+    const double* b = p + 2 * ((i + 1) % n);
+    if ((a[1] > y) != (b[1] > y)) {
+      const double* lo = a[1] < b[1] ? a : b;
+      const double* hi = a[1] < b[1] ? b : a;
+      if (x < lo[0] + (y - lo[1]) * (hi[0] - lo[0]) / (hi[1] - lo[1])) inside = !inside;
+    }
+  }
+  return inside;
+}
+
+double center(int64_t q, double dx) { return ((double)q + 0.5) * dx; }
+
+// First index in [0, n) whose cell center is not below `bound` (geometry.rs:96-107,124-129).
+int64_t first_center_at_or_above(int64_t n, double dx, double bound, bool nan_means_all) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (center(mid, dx) < bound || (nan_means_all && std::isnan(bound)))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Marks the cells whose centers a block contains; returns how many (per block, overlaps counted).
+int64_t rasterize(const cfd_block& b, const int64_t n[3], const double dx[3], std::vector<uint8_t>& fluid) {
+  int64_t r0[3], r1[3];
+  for (int a = 0; a < 3; ++a) {
+    r0[a] = first_center_at_or_above(n[a], dx[a], b.min_m[a], true);
+    r1[a] = std::max(first_center_at_or_above(n[a], dx[a], b.max_m[a], false), r0[a]);
+    if (r0[a] == r1[a]) return 0;
+  }
+  // Every center of the z range lies within the z bounds: one test per column decides it.
+  double z = center(r0[2], dx[2]);
+  int64_t count = 0;
+  for (int64_t j = r0[1]; j < r1[1]; ++j)
+    for (int64_t i = r0[0]; i < r1[0]; ++i) {
+      double p[3] = {center(i, dx[0]), center(j, dx[1]), z};
+      bool inside = true;
+      for (int a = 0; a < 3; ++a) inside = inside && p[a] >= b.min_m[a] && p[a] < b.max_m[a];
+      if (inside && b.n_vertices > 0) inside = in_polygon(b.footprint_xy, b.n_vertices, p[0], p[1]);
+      if (!inside) continue;
+      count += r1[2] - r0[2];
+      for (int64_t k = r0[2]; k < r1[2]; ++k) fluid[(size_t)((k * n[1] + j) * n[0] + i)] = 0;
+    }
+  return count;
+}
+
+// Flow::vector (case.rs:345-367) at height z: only its sign classifies open faces.
+void flow_vector(const cfd_case& c, double z, double v[3]) {
+  if (c.flow_kind == CFD_FLOW_NEUTRAL_RANS) {
+    double z0 = c.roughness_m;
+    double u = c.speed_m_s * std::log((std::max(z, 0.0) + z0) / z0) / std::log((c.reference_height_m + z0) / z0);
+    double t = c.direction_deg_from * (M_PI / 180.0);
+    v[0] = -u * std::sin(t);
+    v[1] = -u * std::cos(t);
+    v[2] = 0.0;
+  } else {
+    for (int a = 0; a < 3; ++a) v[a] = c.velocity_m_s[a];
+  }
+}
+
+}  // namespace
+
+int build_host_grid(const cfd_case& c, HostGrid& g) {
+  int64_t total = 1;
+  for (int a = 0; a < 3; ++a) {
+    int64_t minimum = c.boundary[a] == CFD_PERIODIC ? 1 : 2;
+    if (c.cells[a] < minimum || c.cells[a] > 4096) {
+      set_error("CFD-E004: grid.cells needs 2..4096 cells per axis (1..4096 on a periodic axis)");
+      return CFD_EINVAL;
+    }
+    if (!(c.size_m[a] > 0.0) || !std::isfinite(c.size_m[a])) {
+      set_error("CFD-E003: domain.size_m must be finite and positive");
+      return CFD_EINVAL;
+    }
+    if (c.boundary[a] < CFD_PERIODIC || c.boundary[a] > CFD_GROUND_OPEN_TOP) {
+      set_error("cfd_case.boundary: unknown boundary code");
+      return CFD_EINVAL;
+    }
+    total *= c.cells[a];
+  }
+  if (total > 100000000) {
+    set_error("CFD-E006: at most 1e8 cells");
+    return CFD_EINVAL;
+  }
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = c.cells[a];
+    g.boundary[a] = c.boundary[a];
+    g.periodic[a] = c.boundary[a] == CFD_PERIODIC;
+    g.dx[a] = c.size_m[a] / (double)c.cells[a];
+  }
+  g.preconditioner = c.preconditioner;
+  g.count = total;
+  g.volume = g.dx[0] * g.dx[1] * g.dx[2];
+  for (int a = 0; a < 3; ++a) {
+    double area = g.volume / g.dx[a];
+    g.w_int[a] = area / (g.dx[a] * g.volume);
+    g.w_out[a] = area / ((0.5 * g.dx[a]) * g.volume);
+  }
+  const int64_t* n = g.n;
+  g.fluid.assign((size_t)total, 1);
+  for (int b = 0; b < c.n_blocks; ++b) {
+    if (rasterize(c.blocks[b], n, g.dx, g.fluid) == 0) {
+      char buf[200];
+      std::snprintf(buf, sizeof buf,
+                    "CFD-E028: geometry[%d] (%s) contains no cell center, so the grid does not represent it; enlarge it or "
+                    "refine the grid",
+                    b, c.blocks[b].n_vertices > 0 ? "prism" : "box");
+      set_error(buf);
+      return CFD_EINVAL;
+    }
+  }
+  g.fluid_count = 0;
+  for (uint8_t f : g.fluid) g.fluid_count += f;
+  if (g.fluid_count == 0) {
+    set_error("CFD-E030: geometry contains no fluid cells");
+    return CFD_EINVAL;
+  }
+  for (int a = 0; a < 3; ++a) g.face_count[a] = total / n[a] * (n[a] + (g.periodic[a] ? 0 : 1));
+
+  // Face classification (grid.rs:341-429), per cell side. A side couples the cell
+  // into the pressure operator when its face is Interior or Outlet. A one-cell
+  // periodic axis joins a cell to itself: that face carries no pressure difference.
+  g.codes.assign((size_t)total, 0);
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
+  for (int64_t k = 0; k < n[2]; ++k)
+    for (int64_t j = 0; j < n[1]; ++j)
+      for (int64_t i = 0; i < n[0]; ++i) {
+        int64_t c0 = (k * n[1] + j) * n[0] + i;
+        if (!g.fluid[(size_t)c0]) continue;
+        uint8_t code = kFluid;
+        int64_t q[3] = {i, j, k};
+        for (int a = 0; a < 3; ++a) {
+          int64_t t = q[a];
+          for (int side = 0; side < 2; ++side) {
+            int64_t other = NONE;
+            if (side == 0) {
+              if (t > 0)
+                other = c0 - stride[a];
+              else if (g.periodic[a])
+                other = c0 + (n[a] - 1) * stride[a];
+            } else {
+              if (t + 1 < n[a])
+                other = c0 + stride[a];
+              else if (g.periodic[a])
+                other = c0 - (n[a] - 1) * stride[a];
+            }
+            bool coupled = false;
+            if (other != NONE) {
+              coupled = other != c0 && g.fluid[(size_t)other];  // Interior (or Wall against a solid)
+            } else if (c.boundary[a] == CFD_OPEN) {
+              // Domain face of an open axis: Outlet where the case flow leaves.
+              double face_z = a == 2 ? (double)(side == 0 ? t : t + 1) * g.dx[2] : ((double)k + 0.5) * g.dx[2];
+              double v[3];
+              flow_vector(c, face_z, v);
+              double normal = side == 0 ? -1.0 : 1.0;
+              coupled = v[a] * normal > 1e-12;
+            }
+            if (coupled) code |= side_bit(a, side);
+          }
+        }
+        g.codes[(size_t)c0] = code;
+      }
+
+  // Fluid components by breadth-first search from the lowest unassigned cell
+  // (grid.rs:430-462): the same partition and numbering as the reference.
+  g.component.assign((size_t)total, -1);
+  g.anchored.clear();
+  g.comp_size.clear();
+  std::vector<int64_t> queue;
+  for (int64_t start = 0; start < total; ++start) {
+    if (!g.fluid[(size_t)start] || g.component[(size_t)start] != -1) continue;
+    int32_t id = (int32_t)g.anchored.size();
+    bool anchored = false;
+    int64_t size = 0;
+    queue.clear();
+    queue.push_back(start);
+    g.component[(size_t)start] = id;
+    for (size_t head = 0; head < queue.size(); ++head) {
+      int64_t c0 = queue[head];
+      ++size;
+      uint8_t code = g.codes[(size_t)c0];
+      int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+      for (int a = 0; a < 3; ++a)
+        for (int side = 0; side < 2; ++side) {
+          if (!(code & side_bit(a, side))) continue;
+          int64_t t = q[a];
+          int64_t other;
+          if (side == 0)
+            other = t > 0 ? c0 - stride[a] : (g.periodic[a] ? c0 + (n[a] - 1) * stride[a] : NONE);
+          else
+            other = t + 1 < n[a] ? c0 + stride[a] : (g.periodic[a] ? c0 - (n[a] - 1) * stride[a] : NONE);
+          if (other == NONE) {
+            anchored = true;  // a coupled domain face is an outlet
+            continue;
+          }
+          if (g.component[(size_t)other] == -1) {
+            g.component[(size_t)other] = id;
+            queue.push_back(other);
+          }
+        }
+    }
+    g.anchored.push_back(anchored ? 1 : 0);
+    g.comp_size.push_back(size);
+  }
+  // Closed components: gauge lists, sorted by cell index.
+  g.closed_of_comp.assign(g.anchored.size(), -1);
+  int32_t n_closed = 0;
+  for (size_t id = 0; id < g.anchored.size(); ++id)
+    if (!g.anchored[id]) g.closed_of_comp[id] = n_closed++;
+  g.closed_offsets.assign((size_t)n_closed + 1, 0);
+  g.closed_index.assign((size_t)total, -1);
+  for (int64_t c0 = 0; c0 < total; ++c0) {
+    int32_t id = g.component[(size_t)c0];
+    if (id < 0) continue;
+    int32_t cl = g.closed_of_comp[(size_t)id];
+    if (cl < 0) continue;
+    g.closed_index[(size_t)c0] = cl;
+    g.closed_offsets[(size_t)cl + 1] += 1;
+    g.codes[(size_t)c0] |= kClosed;
+  }
+  for (int32_t cl = 0; cl < n_closed; ++cl) g.closed_offsets[(size_t)cl + 1] += g.closed_offsets[(size_t)cl];
+  g.closed_members.assign((size_t)g.closed_offsets[(size_t)n_closed], 0);
+  {
+    std::vector<int64_t> fill(g.closed_offsets.begin(), g.closed_offsets.end() - 1);
+    for (int64_t c0 = 0; c0 < total; ++c0) {
+      int32_t cl = g.closed_index[(size_t)c0];
+      if (cl >= 0) g.closed_members[(size_t)fill[(size_t)cl]++] = (int32_t)c0;
+    }
+  }
+  return 0;
+}
+
+// Diagonal of the IC(0) factor of A + 1e-6*maxdiag*I (pressure.rs:118-222) in
+// the reference's arithmetic order. The diagonal of A accumulates the face
+// weights in face order (x-faces, then y-faces, then z-faces; within an axis
+// the lower face first); the lower entries of a row are l = a / D[col] (no
+// fill exists without a 3-cell periodic axis, so the reference's dot term is
+// exactly zero), summed as squares in ascending column order.
+void build_ic0_diagonal(HostGrid& g) {
+  const int64_t* n = g.n;
+  const int64_t total = g.count;
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
+  g.ic0_fill = false;
+  for (int a = 0; a < 3; ++a) g.ic0_fill |= g.periodic[a] && n[a] == 3;
+  std::vector<double> raw((size_t)total, 0.0);
+  double maxdiag = 1.0;
+  for (int64_t c0 = 0; c0 < total; ++c0) {
+    uint8_t code = g.codes[(size_t)c0];
+    double d = 0.0;
+    int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        if (!(code & side_bit(a, side))) continue;
+        bool boundary = side == 0 ? q[a] == 0 : q[a] + 1 == n[a];
+        d += (boundary && !g.periodic[a]) ? g.w_out[a] : g.w_int[a];
+      }
+    raw[(size_t)c0] = d;
+    maxdiag = std::fmax(maxdiag, d);
+  }
+  g.ic0_diag.assign((size_t)total, 0.0);
+  double* D = g.ic0_diag.data();
+  for (int64_t c0 = 0; c0 < total; ++c0) {
+    uint8_t code = g.codes[(size_t)c0];
+    int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+    double squared = -0.0;
+    // Lower neighbours in ascending index order: z wrap, z lo, y wrap, y lo, x wrap, x lo.
+    for (int a = 2; a >= 0; --a) {
+      if (!g.periodic[a] || n[a] == 1) {
+        if (q[a] > 0 && (code & side_bit(a, 0))) {
+          double l = -g.w_int[a] / D[(size_t)(c0 - stride[a])];
+          squared += l * l;
+        }
+        continue;
+      }
+      if (n[a] == 2) {
+        // Two faces join the same pair of cells; the reference merges them into one entry.
+        if (q[a] == 1 && (code & side_bit(a, 0))) {
+          double l = (-g.w_int[a] - g.w_int[a]) / D[(size_t)(c0 - stride[a])];
+          squared += l * l;
+        }
+        continue;
+      }
+      if (q[a] == n[a] - 1 && (code & side_bit(a, 1))) {
+        double l = -g.w_int[a] / D[(size_t)(c0 - (n[a] - 1) * stride[a])];
+        squared += l * l;
+      }
+      if (q[a] > 0 && (code & side_bit(a, 0))) {
+        double l = -g.w_int[a] / D[(size_t)(c0 - stride[a])];
+        squared += l * l;
+      }
+    }
+    D[(size_t)c0] = std::sqrt(std::fmax(raw[(size_t)c0] + 1e-6 * maxdiag - squared, 1e-12 * maxdiag));
+  }
+}
+
+namespace {
+
+// Coarsens level `f` (fine dims fn, weights from `wp_of`) into bins of 2x2x2.
+struct FineView {
+  int64_t n[3];
+  bool periodic[3];
+};
+
+int64_t half(int64_t v) { return v > 1 ? (v + 1) / 2 : 1; }
+
+// Dense Cholesky (lower, row-major) of an SPD matrix; false if not positive definite.
+bool cholesky(std::vector<double>& m, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j <= i; ++j) {
+      double v = m[(size_t)(i * n + j)];
+      for (int64_t k = 0; k < j; ++k) v -= m[(size_t)(i * n + k)] * m[(size_t)(j * n + k)];
+      if (i == j) {
+        if (!(v > 0.0) || !std::isfinite(v)) return false;
+        m[(size_t)(i * n + i)] = std::sqrt(v);
+      } else {
+        m[(size_t)(i * n + j)] = v / m[(size_t)(j * n + j)];
+      }
+    }
+  return true;
+}
+
+}  // namespace
+
+int build_hierarchy(const HostGrid& g, HostHierarchy& h, int64_t coarse_max) {
+  h.coarse.clear();
+  int64_t fn[3] = {g.n[0], g.n[1], g.n[2]};
+  const int64_t fine_stride[3] = {1, g.n[0], g.n[0] * g.n[1]};
+  // Level 1 from the fine grid's codes.
+  auto coarse_dims_ok = [&](const int64_t* dims) { return dims[0] * dims[1] * dims[2] > coarse_max; };
+  if (!coarse_dims_ok(fn) && g.count <= coarse_max) {
+    // The fine grid itself is the coarsest level: represent it as a weighted level.
+  }
+  // Build a weighted representation of the fine grid (level "0w") when needed as the coarsest.
+  auto fine_as_level = [&]() {
+    HostLevel L;
+    for (int a = 0; a < 3; ++a) L.n[a] = fn[a];
+    L.count = g.count;
+    for (int a = 0; a < 3; ++a) L.wp[a].assign((size_t)L.count, 0.0);
+    L.anchor.assign((size_t)L.count, 0.0);
+    for (int64_t c0 = 0; c0 < g.count; ++c0) {
+      uint8_t code = g.codes[(size_t)c0];
+      int64_t q[3] = {c0 % fn[0], c0 / fn[0] % fn[1], c0 / (fn[0] * fn[1])};
+      for (int a = 0; a < 3; ++a) {
+        for (int side = 0; side < 2; ++side) {
+          if (!(code & side_bit(a, side))) continue;
+          bool boundary = side == 0 ? q[a] == 0 : q[a] + 1 == fn[a];
+          if (boundary && !g.periodic[a]) L.anchor[(size_t)c0] += g.w_out[a];
+        }
+        if (code & side_bit(a, 1)) {
+          bool boundary = q[a] + 1 == fn[a];
+          if (!boundary || g.periodic[a]) L.wp[a][(size_t)c0] += g.w_int[a];
+        }
+      }
+    }
+    return L;
+  };
+  std::vector<HostLevel> levels;  // weighted levels: [fine-as-level?] then coarse
+  HostLevel current = fine_as_level();
+  bool current_is_fine = true;
+  while (current.count > coarse_max) {
+    int64_t cn[3];
+    bool can = false;
+    for (int a = 0; a < 3; ++a) {
+      cn[a] = half(current.n[a]);
+      can |= cn[a] != current.n[a];
+    }
+    if (!can) break;
+    HostLevel C;
+    for (int a = 0; a < 3; ++a) C.n[a] = cn[a];
+    C.count = cn[0] * cn[1] * cn[2];
+    for (int a = 0; a < 3; ++a) C.wp[a].assign((size_t)C.count, 0.0);
+    C.anchor.assign((size_t)C.count, 0.0);
+    const int64_t* n = current.n;
+    for (int64_t c0 = 0; c0 < current.count; ++c0) {
+      int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+      int64_t b[3] = {q[0] / 2, q[1] / 2, q[2] / 2};
+      int64_t B = (b[2] * cn[1] + b[1]) * cn[0] + b[0];
+      C.anchor[(size_t)B] += current.anchor[(size_t)c0];
+      for (int a = 0; a < 3; ++a) {
+        double w = current.wp[a][(size_t)c0];
+        if (w == 0.0) continue;
+        int64_t qn = q[a] + 1 == n[a] ? 0 : q[a] + 1;  // + neighbour (wraps only when periodic: w is 0 otherwise)
+        int64_t bn = qn / 2;
+        if (bn == b[a]) continue;  // internal to the bin
+        // bn is the + neighbour of b along a (possibly by wrap)
+        C.wp[a][(size_t)B] += w;
+      }
+    }
+    if (current_is_fine) {
+      current_is_fine = false;
+    } else {
+      levels.push_back(std::move(current));
+    }
+    current = std::move(C);
+  }
+  if (current_is_fine) {
+    // No coarsening at all: the fine grid is solved directly.
+    levels.clear();
+  } else {
+    levels.push_back(std::move(current));
+  }
+  for (auto& L : levels) {
+    L.diag.assign((size_t)L.count, 0.0);
+    for (int64_t c0 = 0; c0 < L.count; ++c0) {
+      int64_t q[3] = {c0 % L.n[0], c0 / L.n[0] % L.n[1], c0 / (L.n[0] * L.n[1])};
+      double d = L.anchor[(size_t)c0];
+      const int64_t st[3] = {1, L.n[0], L.n[0] * L.n[1]};
+      for (int a = 0; a < 3; ++a) {
+        d += L.wp[a][(size_t)c0];
+        int64_t lo = q[a] > 0 ? c0 - st[a] : c0 + (L.n[a] - 1) * st[a];
+        if (q[a] > 0 || g.periodic[a]) d += L.wp[a][(size_t)lo];
+      }
+      L.diag[(size_t)c0] = d;
+    }
+  }
+  (void)fine_stride;
+  // The coarsest weighted level (or the fine grid) is solved by a dense pseudo-inverse.
+  HostLevel finest_direct;
+  const HostLevel* last;
+  if (levels.empty()) {
+    finest_direct = fine_as_level();
+    finest_direct.diag.assign((size_t)finest_direct.count, 0.0);
+    for (int64_t c0 = 0; c0 < finest_direct.count; ++c0) {
+      uint8_t code = g.codes[(size_t)c0];
+      int64_t q[3] = {c0 % fn[0], c0 / fn[0] % fn[1], c0 / (fn[0] * fn[1])};
+      double d = 0.0;
+      for (int a = 0; a < 3; ++a)
+        for (int side = 0; side < 2; ++side) {
+          if (!(code & side_bit(a, side))) continue;
+          bool boundary = side == 0 ? q[a] == 0 : q[a] + 1 == fn[a];
+          d += (boundary && !g.periodic[a]) ? g.w_out[a] : g.w_int[a];
+        }
+      finest_direct.diag[(size_t)c0] = d;
+    }
+    last = &finest_direct;
+  } else {
+    last = &levels.back();
+  }
+  const int64_t N = last->count;
+  if (N > 4096) {
+    set_error("multigrid: coarsest level too large for the dense solve");
+    return CFD_EUNSUPPORTED;
+  }
+  for (int a = 0; a < 3; ++a) h.coarsest_dims[a] = last->n[a];
+  std::vector<double> A((size_t)(N * N), 0.0);
+  const int64_t st[3] = {1, last->n[0], last->n[0] * last->n[1]};
+  for (int64_t c0 = 0; c0 < N; ++c0) {
+    int64_t q[3] = {c0 % last->n[0], c0 / last->n[0] % last->n[1], c0 / (last->n[0] * last->n[1])};
+    A[(size_t)(c0 * N + c0)] += last->diag[(size_t)c0];
+    for (int a = 0; a < 3; ++a) {
+      double w = last->wp[a][(size_t)c0];
+      if (w == 0.0) continue;
+      int64_t nb = q[a] + 1 < last->n[a] ? c0 + st[a] : c0 - (last->n[a] - 1) * st[a];
+      if (nb == c0) continue;
+      A[(size_t)(c0 * N + nb)] -= w;
+      A[(size_t)(nb * N + c0)] -= w;
+    }
+  }
+  // Components of the coarsest graph; closed ones lose their first member (gauge).
+  std::vector<int64_t> comp((size_t)N, -1);
+  std::vector<std::vector<int64_t>> members;
+  std::vector<bool> anchored;
+  for (int64_t s = 0; s < N; ++s) {
+    if (comp[(size_t)s] != -1 || last->diag[(size_t)s] == 0.0) continue;
+    int64_t id = (int64_t)members.size();
+    members.emplace_back();
+    bool anch = false;
+    std::vector<int64_t> q{s};
+    comp[(size_t)s] = id;
+    for (size_t head = 0; head < q.size(); ++head) {
+      int64_t i = q[head];
+      members.back().push_back(i);
+      double offsum = 0.0;
+      for (int64_t j = 0; j < N; ++j) {
+        if (j == i) continue;
+        double v = A[(size_t)(i * N + j)];
+        if (v == 0.0) continue;
+        offsum -= v;
+        if (comp[(size_t)j] == -1) {
+          comp[(size_t)j] = id;
+          q.push_back(j);
+        }
+      }
+      if (A[(size_t)(i * N + i)] - offsum > 1e-12 * A[(size_t)(i * N + i)]) anch = true;
+    }
+    anchored.push_back(anch);
+  }
+  std::vector<int64_t> active;
+  for (size_t id = 0; id < members.size(); ++id) {
+    auto& m = members[id];
+    std::sort(m.begin(), m.end());
+    for (size_t k = anchored[id] ? 0 : 1; k < m.size(); ++k) active.push_back(m[k]);
+  }
+  const int64_t M = (int64_t)active.size();
+  std::vector<double> R((size_t)(M * M), 0.0);
+  for (int64_t i = 0; i < M; ++i)
+    for (int64_t j = 0; j < M; ++j) R[(size_t)(i * M + j)] = A[(size_t)(active[(size_t)i] * N + active[(size_t)j])];
+  if (!cholesky(R, M)) {
+    set_error("CFD-E035: numerics.pressure_preconditioner: multigrid coarse Cholesky lost positive definiteness");
+    return CFD_E035;
+  }
+  // Inverse of the reduced matrix, column by column.
+  std::vector<double> Ginv((size_t)(M * M), 0.0), e((size_t)M);
+  for (int64_t col = 0; col < M; ++col) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[(size_t)col] = 1.0;
+    for (int64_t i = 0; i < M; ++i) {
+      double v = e[(size_t)i];
+      for (int64_t k = 0; k < i; ++k) v -= R[(size_t)(i * M + k)] * e[(size_t)k];
+      e[(size_t)i] = v / R[(size_t)(i * M + i)];
+    }
+    for (int64_t i = M - 1; i >= 0; --i) {
+      double v = e[(size_t)i];
+      for (int64_t k = i + 1; k < M; ++k) v -= R[(size_t)(k * M + i)] * e[(size_t)k];
+      e[(size_t)i] = v / R[(size_t)(i * M + i)];
+    }
+    for (int64_t i = 0; i < M; ++i) Ginv[(size_t)(i * M + col)] = e[(size_t)i];
+  }
+  std::vector<double> G((size_t)(N * N), 0.0);
+  for (int64_t i = 0; i < M; ++i)
+    for (int64_t j = 0; j < M; ++j) G[(size_t)(active[(size_t)i] * N + active[(size_t)j])] = Ginv[(size_t)(i * M + j)];
+  // Symmetric gauge: G <- C G C with C the per-closed-component centering projector.
+  for (size_t id = 0; id < members.size(); ++id) {
+    if (anchored[id]) continue;
+    const auto& m = members[id];
+    double inv = 1.0 / (double)m.size();
+    for (int pass = 0; pass < 2; ++pass) {
+      // pass 0: rows (C G), pass 1: columns (G C)
+      for (int64_t other = 0; other < N; ++other) {
+        double mean = 0.0;
+        for (int64_t i : m) mean += pass == 0 ? G[(size_t)(i * N + other)] : G[(size_t)(other * N + i)];
+        mean *= inv;
+        for (int64_t i : m) {
+          if (pass == 0)
+            G[(size_t)(i * N + other)] -= mean;
+          else
+            G[(size_t)(other * N + i)] -= mean;
+        }
+      }
+    }
+  }
+  h.direct_n = N;
+  h.direct = std::move(G);
+  h.coarse = std::move(levels);
+  return 0;
+}
+
+}  // namespace cfdgpu

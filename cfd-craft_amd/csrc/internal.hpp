@@ -1,0 +1,73 @@
+// Internal host-side structures of the cfdgpu library.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cfdgpu.h"
+
+namespace cfdgpu {
+
+// Coupling-code bits (one byte per cell, the only per-cell topology on the device).
+constexpr uint8_t kFluid = 1u << 6;
+constexpr uint8_t kClosed = 1u << 7;
+inline uint8_t side_bit(int axis, int side) { return (uint8_t)(1u << (2 * axis + side)); }
+
+void set_error(const std::string& text);
+
+// Host description of a uniform staggered grid as the pressure solve sees it:
+// the restatement of Grid::build (grid.rs:246-464) in per-cell form.
+struct HostGrid {
+  int64_t n[3];
+  int64_t count;
+  double dx[3];
+  double volume;
+  int boundary[3];
+  bool periodic[3];
+  int preconditioner;
+  // Interior and outlet face weights area / (distance * volume) per axis (pressure.rs:257).
+  double w_int[3], w_out[3];
+  std::vector<uint8_t> fluid;
+  std::vector<uint8_t> codes;
+  int64_t fluid_count = 0;
+  int64_t face_count[3];
+  // Fluid components in the reference's numbering (BFS from the lowest index, grid.rs:430-462).
+  std::vector<int32_t> component;  // -1 for solid
+  std::vector<uint8_t> anchored;   // per component
+  std::vector<int64_t> comp_size;
+  // Closed (non-anchored) components: index into the closed list per component, and member lists.
+  std::vector<int32_t> closed_of_comp;           // -1 if anchored
+  std::vector<int32_t> closed_index;             // per cell: closed index or -1
+  std::vector<int64_t> closed_offsets;           // members CSR (cells sorted ascending)
+  std::vector<int32_t> closed_members;
+  int n_closed() const { return (int)closed_offsets.size() - 1; }
+  // IC(0): diagonal of the shifted factor (pressure.rs:191-214); ic0_fill marks a
+  // topology whose factor has fill coupling (a periodic axis of 3 cells).
+  std::vector<double> ic0_diag;
+  bool ic0_fill = false;
+};
+
+// Builds the grid; returns a CFD error code (0 on success) and sets the error text.
+int build_host_grid(const cfd_case& c, HostGrid& g);
+void build_ic0_diagonal(HostGrid& g);
+
+// Geometric hierarchy: 2x2x2 bins of the structured grid with Galerkin
+// piecewise-constant coarse operators (7-point with per-face weights).
+struct HostLevel {
+  int64_t n[3];
+  int64_t count;
+  // coupling of cell c to its + neighbour on axis a (wrapping on periodic axes)
+  std::vector<double> wp[3];
+  std::vector<double> anchor;
+  std::vector<double> diag;
+};
+struct HostHierarchy {
+  std::vector<HostLevel> coarse;  // levels 1.. (level 0 is the fine grid itself)
+  // Coarsest level: dense symmetric pseudo-inverse with per-closed-component gauge centering.
+  int64_t direct_n = 0;
+  std::vector<double> direct;  // direct_n x direct_n, row-major
+  int64_t coarsest_dims[3];
+};
+int build_hierarchy(const HostGrid& g, HostHierarchy& h, int64_t coarse_max);
+
+}  // namespace cfdgpu

@@ -1,0 +1,190 @@
+/*
+ * cfdgpu — MI355X-native pressure-projection solver for CFD-craft's staggered
+ * uniform Cartesian grids. C-ABI boundary: plain pointers and sizes, opaque
+ * handles, no exceptions across the ABI, error text through cfd_last_error().
+ *
+ * The entry points replace the reference's pressure seam
+ * (crates/cfd-craft/src/pressure.rs) and the grid topology it consumes
+ * (crates/cfd-craft/src/grid.rs). Each declaration cites the function it
+ * stands in for. Reference citations are path:line in the CFD-craft tree.
+ *
+ * Error codes returned by the solve entry points are the numbers of the
+ * reference's coded errors (CFD-E031 -> 31, ...), CFD_CANCELLED for
+ * "cancelled", CFD_WALL_TIME for "wall_time_limit" (solver.rs:118-139), or
+ * CFD_EINVAL / CFD_EDEVICE for misuse and device failures. On any error
+ * the outputs of solve are left untouched and project() leaves u and p
+ * unchanged (pressure.rs:468).
+ *
+ * Threading: distinct grids may be used from distinct threads at once; one
+ * grid is not re-entrant (the reference serializes solves on one grid through
+ * its multigrid cache mutex, pressure.rs:71-77).
+ */
+#ifndef CFDGPU_H
+#define CFDGPU_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ case */
+/* Boundary of one axis pair (case.rs:186-245 Boundary). */
+enum { CFD_PERIODIC = 0, CFD_OPEN = 1, CFD_WALL = 2, CFD_SLIP = 3, CFD_GROUND_OPEN_TOP = 4 };
+/* Flow model, as far as it classifies open faces (case.rs:345-367 Flow::vector):
+ * a constant vector (prescribed_uniform / laminar) or the neutral_rans log law. */
+enum { CFD_FLOW_VECTOR = 0, CFD_FLOW_NEUTRAL_RANS = 1 };
+/* numerics.pressure_preconditioner (case.rs:850-879). */
+enum { CFD_IC0 = 0, CFD_MULTIGRID = 1 };
+
+/* One geometry entry (case.rs:415-459 Block): an axis-aligned box, or a
+ * polygon prism when n_vertices > 0. */
+typedef struct {
+  double min_m[3];
+  double max_m[3];
+  int32_t n_vertices;
+  int32_t reserved;
+  const double* footprint_xy; /* 2*n_vertices doubles: x0 y0 x1 y1 ... */
+} cfd_block;
+
+/* The keys of a case that fix the pressure operator (case.rs:88-130):
+ * domain.size_m, grid.cells, boundaries, flow, geometry and
+ * numerics.pressure_preconditioner. */
+typedef struct {
+  int64_t cells[3];
+  double size_m[3];
+  int32_t boundary[3];
+  int32_t flow_kind;
+  double velocity_m_s[3];
+  double speed_m_s, reference_height_m, roughness_m, direction_deg_from;
+  int32_t preconditioner;
+  int32_t n_blocks;
+  const cfd_block* blocks;
+} cfd_case;
+
+/* ------------------------------------------------------------------ results */
+enum {
+  CFD_OK = 0,
+  CFD_E031 = 31, /* incompatible flux in a closed fluid component */
+  CFD_E032 = 32, /* pressure CG lost positive definiteness */
+  CFD_E033 = 33, /* pressure iteration limit */
+  CFD_E034 = 34, /* invalid multigrid topology or weights */
+  CFD_E035 = 35, /* coarse Cholesky lost positive definiteness */
+  CFD_CANCELLED = 100,
+  CFD_WALL_TIME = 101,
+  CFD_EINVAL = 200,
+  CFD_EUNSUPPORTED = 201,
+  CFD_EDEVICE = 202,
+  CFD_ENOMEM = 203
+};
+
+/* LinearInfo (pressure.rs:29-36) plus the Span work counters
+ * (profiling.rs:112-135) and MultigridWork (multigrid.rs:78-114). */
+typedef struct {
+  int64_t iterations;
+  double relative_residual; /* recomputed (true) residual / |b_c| */
+  double recurrent_residual;
+  int64_t operator_applications;
+  int64_t preconditioner_applications;
+  int64_t true_residual_checks;
+  int64_t cycles;
+  int64_t smoothing_cell_updates;
+  int64_t residual_cell_updates;
+  int64_t coarse_solves;
+  int64_t hierarchy_builds;
+  double seconds; /* device time of the solve, HIP events */
+} cfd_linear_info;
+
+/* MultigridStats (multigrid.rs:55-72). */
+typedef struct {
+  int32_t levels;
+  int32_t reserved;
+  int64_t level_cells[32];
+  int64_t level_nonzeros[32];
+  int64_t level_dims[32][3];
+  int64_t allocated_bytes;
+} cfd_mg_stats;
+
+/* Control::check as a callback (solver.rs:122-139): 0 continue, 1 cancelled,
+ * 2 wall_time_limit. Called before every 8th PCG iteration (pressure.rs:387-389). */
+typedef int (*cfd_poll_fn)(void* user);
+
+/* Solver options beyond the reference's case keys (serving and precision). */
+enum { CFD_PRECISION_FP64 = 0, CFD_PRECISION_MIXED = 1 };
+typedef struct {
+  int32_t precision;   /* FP64 (plain) or FP32 multigrid/IC(0)-apply inside FP64 PCG */
+  int32_t warm_start;  /* 1: start PCG from the grid's resident previous solution */
+  int32_t poll_every;  /* PCG iterations between host polls (multiple of 8; default 8) */
+  int32_t reserved;
+} cfd_options;
+
+typedef struct cfd_grid cfd_grid;
+
+/* ------------------------------------------------------------------ grid */
+/* Grid::build (grid.rs:246-464): masks the geometry, classifies every face,
+ * finds the fluid components and uploads the topology (1 byte per cell) to
+ * the current HIP device. Builds the IC(0) factor diagonal or the multigrid
+ * hierarchy for the selected preconditioner (pressure.rs:55-97). Errors:
+ * CFD-E028/E030 as CFD_EINVAL with the reference text. */
+int cfd_grid_create(const cfd_case* c, cfd_grid** out);
+void cfd_grid_destroy(cfd_grid* g);
+/* counts: cells, faces, fluid cells, components, anchored components, x/y/z face counts. */
+int cfd_grid_counts(const cfd_grid* g, int64_t counts[8]);
+/* Per cell coupling code: bit 2a = lower face on axis a couples (interior or
+ * outlet), bit 2a+1 = upper face, bit 6 = fluid, bit 7 = in a closed component. */
+int cfd_grid_codes(const cfd_grid* g, uint8_t* codes);
+/* Grid.component / Grid.anchored (grid.rs:129-138). */
+int cfd_grid_components(const cfd_grid* g, int64_t* component, uint8_t* anchored);
+int cfd_grid_set_options(cfd_grid* g, const cfd_options* o);
+/* The IC(0) factor diagonal of pressure.rs:191-214 (host copy, for parity checks). */
+int cfd_ic0_diagonal(const cfd_grid* g, double* diag);
+/* multigrid_stats (pressure.rs:100-106) of the device hierarchy. */
+int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s);
+
+/* Host-only inspection of Grid::build (grid.rs:246-464) without a device: the
+ * counts of cfd_grid_counts, and optionally the per-cell coupling codes, the
+ * component of each cell, the anchored flag of each component (at most
+ * anchored_cap) and the IC(0) factor diagonal (pressure.rs:191-214). Any
+ * output pointer may be NULL. */
+int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
+                      int64_t anchored_cap, double* ic0_diagonal);
+
+/* ------------------------------------------------------------------ host-buffer seam */
+/* pressure::apply (pressure.rs:293-312): y = A x. */
+int cfd_apply(cfd_grid* g, const double* x, double* y);
+/* pressure::center_nullspaces (pressure.rs:316-325). */
+int cfd_center_nullspaces(cfd_grid* g, double* x);
+/* pressure::solve_scaled (pressure.rs:342-436); scale may be NULL (= solve, 329-337). */
+int cfd_solve_scaled(cfd_grid* g, const double* b, const double* roundoff_scale, double tol, int64_t max_iterations,
+                     cfd_poll_fn poll, void* poll_user, double* x, cfd_linear_info* info);
+/* pressure::project (pressure.rs:469-502): u has one value per face in the
+ * reference's face order (x-faces, y-faces, z-faces, each x fastest). */
+int cfd_project(cfd_grid* g, double* u, double* p, double dt, double tol, int64_t max_iterations, cfd_poll_fn poll,
+                void* poll_user, cfd_linear_info* info);
+/* projection_rhs (pressure.rs:440-459). */
+int cfd_projection_rhs(cfd_grid* g, const double* u, double dt, double* b, double* terms);
+/* One preconditioner application z = M r: Factor::apply (pressure.rs:223-241)
+ * or one V-cycle (multigrid.rs:656-665). */
+int cfd_precondition(cfd_grid* g, const double* r, double* z);
+
+/* ------------------------------------------------------------------ device-resident seam */
+/* The same operations on device pointers (HBM), ordered on the grid's stream. */
+int cfd_apply_device(cfd_grid* g, const double* d_x, double* d_y);
+int cfd_solve_scaled_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                            cfd_poll_fn poll, void* poll_user, double* d_x, cfd_linear_info* info);
+int cfd_precondition_device(cfd_grid* g, const double* d_r, double* d_z);
+int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
+                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info);
+
+/* Device memory helpers for callers without their own allocator. */
+int cfd_device_alloc(size_t bytes, void** d_ptr);
+int cfd_device_free(void* d_ptr);
+int cfd_copy_to_device(void* d_dst, const void* h_src, size_t bytes);
+int cfd_copy_to_host(void* h_dst, const void* d_src, size_t bytes);
+int cfd_device_synchronize(void);
+
+const char* cfd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

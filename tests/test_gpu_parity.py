@@ -1,0 +1,170 @@
+"""GPU parity of the pressure path against the CPU oracle (the reference's serial
+FP64 arithmetic restated in oracle/), through the C-ABI of libcfdgpu.so.
+
+Bars: the operator and one IC(0) application are bitwise equal to the oracle
+(same per-cell arithmetic, FP contraction off); PCG-IC(0) solves take the same
+iteration count within +-1 and match the oracle's field to 1e-10 relative; the
+geometric multigrid preconditioner is a different (red-black) cycle, so its
+solves are held to the same field bound and to the acceptance gates.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import cfdgpu
+from cfdgpu import workloads as W
+from cfdgpu.case import IC0, MULTIGRID, OPEN, PERIODIC, SLIP, WALL, Block, Case
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def small_cases():
+    cases = [("cavity24", W.cavity([24, 2, 24])), ("cube12", W.cavity([12, 12, 12]))]
+    for n in ([9, 2, 7], [8, 5, 6], [2, 2, 2], [9, 1, 7], [6, 4, 5]):
+        for bnd in (WALL, PERIODIC, OPEN):
+            cases.append((f"lid{n}-{bnd}", Case(cells=n, size_m=[2.7, 1.9, 3.1], boundaries=[bnd, PERIODIC, WALL],
+                                                velocity_m_s=[1.0, 0.0, 0.0])))
+    cases.append(("barrier", Case(cells=[9, 2, 5], size_m=[9, 2, 5], boundaries=[OPEN, WALL, WALL],
+                                  velocity_m_s=[1.0, 0, 0], blocks=[Block([3.0, 0, 0], [4.0, 2, 5])])))
+    c, _, _ = W.urban_block(n=(32, 32, 16), boxes=6)
+    cases.append(("urban32", c))
+    return cases
+
+
+@pytest.mark.parametrize("name,case", small_cases(), ids=[n for n, _ in small_cases()])
+def test_operator_bitwise(gpu_available, name, case):
+    g = cfdgpu.Grid(case)
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(g.count)
+    y = g.apply(x)
+    assert np.array_equal(y, o.apply(x)), name
+    assert np.array_equal(y, o.operator_apply(x)), name
+
+
+@pytest.mark.parametrize("name,case", small_cases(), ids=[n for n, _ in small_cases()])
+def test_ic0_apply_bitwise(gpu_available, name, case):
+    case.preconditioner = IC0
+    g = cfdgpu.Grid(case)
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(11)
+    r = rng.standard_normal(g.count)
+    z = g.precondition(r)
+    assert np.array_equal(z, o.ic0_apply(r)), name
+
+
+def _manufactured(case):
+    o = oracle.Grid(case)
+    ex = W.manufactured(case, o.fluid())
+    ex = o.center_nullspaces(ex)
+    return o, ex, o.apply(ex)
+
+
+@pytest.mark.parametrize("cells,its", [([24, 2, 24], 53), ([48, 2, 48], 102), ([24, 24, 24], 61)])
+def test_ic0_pcg_matches_published_and_oracle(gpu_available, cells, its):
+    case = W.cavity(cells, IC0)
+    o, ex, b = _manufactured(case)
+    xo, io = o.solve(b, 1e-11, 10000)
+    g = cfdgpu.Grid(case)
+    x, info = g.solve(b, 1e-11, 10000)
+    assert abs(info["iterations"] - its) <= 1
+    assert abs(info["iterations"] - io["iterations"]) <= 1
+    assert info["relative_residual"] <= 1.01e-10
+    assert rel(x, xo) <= 1e-10
+    assert math.sqrt(np.mean((x - ex) ** 2)) <= 1e-8
+
+
+@pytest.mark.parametrize("name,case", small_cases(), ids=[n for n, _ in small_cases()])
+@pytest.mark.parametrize("pc", [IC0, MULTIGRID])
+def test_solve_field_parity_small(gpu_available, name, case, pc):
+    case.preconditioner = pc
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(5)
+    ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    xo, io = o.solve(b, 1e-13, 5000)
+    g = cfdgpu.Grid(case)
+    x, info = g.solve(b, 1e-13, 5000)
+    assert info["relative_residual"] <= 1e-12 + 1e-12
+    assert rel(x, xo) <= 1e-10, (name, info, io)
+    if pc == IC0:
+        assert abs(info["iterations"] - io["iterations"]) <= 1
+
+
+def test_multigrid_urban_block_field(gpu_available):
+    case, b, t = W.urban_block(n=(64, 64, 32), boxes=10)
+    o = oracle.Grid(case)
+    xo, io = o.solve_scaled(b, t, 1e-10, 2000)
+    for precision in (cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED):
+        g = cfdgpu.Grid(case)
+        g.set_options(precision)
+        x, info = g.solve_scaled(b, t, 1e-10, 2000)
+        assert info["relative_residual"] <= 1.01e-9
+        # two solves at residual 1e-10 differ by at most ~cond * 1e-10; the field bound is looser
+        assert rel(x, xo) <= 1e-7, (precision, info, io)
+
+
+def test_zero_rhs_returns_zero_iterations(gpu_available):
+    g = cfdgpu.Grid(W.cavity([8, 2, 8]))
+    x, info = g.solve(np.zeros(g.count), 1e-10, 100)
+    assert info["iterations"] == 0 and np.all(x == 0.0)
+
+
+def test_incompatible_flux_is_e031(gpu_available):
+    case = Case(cells=[9, 2, 7], size_m=[2.7, 1.9, 3.1], boundaries=[WALL, PERIODIC, WALL])
+    g = cfdgpu.Grid(case)
+    b = np.cos(0.3 * np.arange(g.count))
+    b = g.center_nullspaces(b) + 1e-11
+    g.solve(b, 1e-12, 2000)  # round-off incompatibility is accepted
+    with pytest.raises(cfdgpu.CfdError) as e:
+        g.solve(b + 0.1, 1e-12, 2000)
+    assert e.value.code == 31 and "CFD-E031" in str(e.value)
+
+
+def test_iteration_limit_is_e033_and_cancel(gpu_available):
+    case = W.cavity([24, 2, 24])
+    o, ex, b = _manufactured(case)
+    g = cfdgpu.Grid(case)
+    with pytest.raises(cfdgpu.CfdError) as e:
+        g.solve(b, 1e-11, 1)
+    assert e.value.code == 33 and "CFD-E033" in str(e.value)
+    calls = []
+    ctl = cfdgpu.Control(cancel=lambda: calls.append(1) or len(calls) > 2)
+    with pytest.raises(cfdgpu.CfdError) as e:
+        g.solve(b, 1e-11, 10000, control=ctl)
+    assert e.value.code == cfdgpu.CANCELLED and str(e.value) == "cancelled"
+
+
+def test_projection_matches_oracle(gpu_available):
+    case, b, t = W.urban_block(n=(32, 32, 16), boxes=6)
+    case.preconditioner = IC0
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(3)
+    fluid = o.fluid()
+    kinds, zc, offs = W.face_kinds(case, fluid)
+    u = W.enforce_velocity(case, kinds, zc, offs, rng.standard_normal(o.face_count))
+    p = o.center_nullspaces(rng.standard_normal(o.count))
+    g = cfdgpu.Grid(case)
+    bg, tg = g.projection_rhs(u, 0.25)
+    bo, to = o.projection_rhs(u, 0.25)
+    assert np.array_equal(bg, bo) and np.array_equal(tg, to)
+    uo, po, io = o.project(u, p, 0.25, 1e-12, 3000)
+    ug, pg, ig = g.project(u, p, 0.25, 1e-12, 3000)
+    assert abs(ig["iterations"] - io["iterations"]) <= 1
+    assert rel(ug, uo) <= 1e-10 and rel(pg, po) <= 1e-10
+    div = o.divergence(ug)
+    assert np.max(np.abs(div[fluid])) < 1e-8 * max(1.0, np.max(np.abs(bo)))
+
+
+def test_deterministic_repeat(gpu_available):
+    case, b, t = W.urban_block(n=(32, 32, 16), boxes=6)
+    g = cfdgpu.Grid(case)
+    x1, i1 = g.solve_scaled(b, t, 1e-10, 2000)
+    x2, i2 = g.solve_scaled(b, t, 1e-10, 2000)
+    assert np.array_equal(x1, x2) and i1["iterations"] == i2["iterations"]

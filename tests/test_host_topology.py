@@ -1,0 +1,114 @@
+"""The product's host set-up (cfd_case_topology, no device) against the oracle's
+Grid::build (grid.rs:246-464): per-cell coupling codes, components, anchoring and
+the IC(0) factor diagonal (pressure.rs:191-214) must be identical, bit for bit.
+Also checks that libcfdgpu.so exports every entry point include/cfdgpu.h declares,
+and that the numpy workload helpers classify faces exactly as the reference does.
+"""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import cfdgpu
+from cfdgpu import workloads as W
+from cfdgpu.case import INTERIOR, OPEN, OUTLET, PERIODIC, WALL, Block, Case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def codes_from_oracle(g):
+    f = g.faces()
+    codes = np.zeros(g.count, np.uint8)
+    codes[g.fluid()] |= 64
+    for ax, lo, hi, kind in zip(f["axis"], f["lo"], f["hi"], f["kind"]):
+        if kind == INTERIOR and lo != hi:
+            codes[lo] |= 1 << (2 * ax + 1)
+            codes[hi] |= 1 << (2 * ax)
+        elif kind == OUTLET:
+            if lo >= 0:
+                codes[lo] |= 1 << (2 * ax + 1)
+            else:
+                codes[hi] |= 1 << (2 * ax)
+    comp, anch = g.components()
+    for c in range(len(anch)):
+        if not anch[c]:
+            codes[comp == c] |= 128
+    return codes
+
+
+def cases():
+    out = [("cavity24", W.cavity([24, 2, 24])), ("cube12", W.cavity([12, 12, 12]))]
+    for n in ([9, 2, 7], [8, 5, 6], [2, 2, 2], [9, 1, 7], [6, 4, 5]):
+        for bnd in (WALL, PERIODIC, OPEN):
+            out.append((f"lid{n}-{bnd}", Case(cells=n, size_m=[2.7, 1.9, 3.1], boundaries=[bnd, PERIODIC, WALL],
+                                              velocity_m_s=[1.0, 0.0, 0.0])))
+    with open(os.path.join(ROOT, "tests", "golden", "canyon-93k.json")) as f:
+        out.append(("canyon93k", Case.from_json(json.load(f))))
+    out.append(("urban32", W.urban_block(n=(32, 32, 16), boxes=6)[0]))
+    out.append(("prisms48", W.urban_prisms(n=(48, 48, 24), prisms=12, radius=(3.0, 6.0), heights=(3, 12))[0]))
+    out.append(("barrier", Case(cells=[9, 2, 5], size_m=[9, 2, 5], boundaries=[OPEN, WALL, WALL], velocity_m_s=[1.0, 0, 0],
+                                blocks=[Block([3.0, 0, 0], [4.0, 2, 5])])))
+    out.append(("rotated-wind", Case(cells=[20, 16, 8], size_m=[20, 16, 8], boundaries=[OPEN, OPEN, 4], flow_kind=1,
+                                     speed_m_s=3.0, reference_height_m=5.0, roughness_m=0.05, direction_deg_from=225.0,
+                                     blocks=[Block([5.0, 4, 0], [9.0, 9, 5])])))
+    return out
+
+
+@pytest.mark.parametrize("name,case", cases(), ids=[n for n, _ in cases()])
+def test_topology_matches_oracle(name, case):
+    o = oracle.Grid(case)
+    t = cfdgpu.case_topology(case, ic0=True)
+    assert np.array_equal(t["codes"], codes_from_oracle(o))
+    comp, anch = o.components()
+    assert np.array_equal(t["component"], comp)
+    assert np.array_equal(t["anchored"], anch)
+    assert list(t["counts"][:5]) == [o.count, o.face_count, o.fluid_count, o.n_components, o.n_anchored]
+    assert np.array_equal(t["ic0_diagonal"].view(np.uint64), o.ic0_diagonal().view(np.uint64))
+
+
+@pytest.mark.parametrize("name,case", cases(), ids=[n for n, _ in cases()])
+def test_numpy_workload_helpers_match_oracle(name, case):
+    o = oracle.Grid(case)
+    fluid = W.fluid_mask(case)
+    assert np.array_equal(fluid, o.fluid())
+    kinds, zc, offs = W.face_kinds(case, fluid)
+    assert np.array_equal(kinds, o.faces()["kind"])
+    u = np.random.default_rng(0).standard_normal(o.face_count)
+    assert np.array_equal(W.enforce_velocity(case, kinds, zc, offs, u), o.enforce_velocity(u))
+    b1, t1 = o.projection_rhs(u, 0.5)
+    b2, t2 = W.projection_rhs(case, fluid, u, 0.5)
+    assert np.array_equal(b1, b2) and np.array_equal(t1, t2)
+
+
+def test_grid_build_errors_match_reference_codes():
+    with pytest.raises(cfdgpu.CfdError, match="CFD-E028"):
+        cfdgpu.case_topology(Case(cells=[8, 8, 8], size_m=[8, 8, 8], boundaries=[WALL] * 3,
+                                  blocks=[Block([1.1, 1.1, 1.1], [1.2, 1.2, 1.2])]))
+    with pytest.raises(cfdgpu.CfdError, match="CFD-E030"):
+        cfdgpu.case_topology(Case(cells=[4, 4, 4], size_m=[4, 4, 4], boundaries=[WALL] * 3,
+                                  blocks=[Block([0.0, 0, 0], [4.0, 4, 4])]))
+    with pytest.raises(cfdgpu.CfdError, match="CFD-E004"):
+        cfdgpu.case_topology(Case(cells=[1, 4, 4], size_m=[4, 4, 4], boundaries=[WALL] * 3))
+
+
+def test_library_exports_every_declared_entry_point():
+    with open(os.path.join(ROOT, "include", "cfdgpu.h")) as f:
+        header = f.read()
+    names = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(cfd_\w+)\s*\(", header, re.M))
+    assert len(names) >= 25
+    lib = ctypes.CDLL(cfdgpu.LIBRARY)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_case_json_keys():
+    with open(os.path.join(ROOT, "tests", "golden", "lid-cavity.json")) as f:
+        c = json.load(f)
+    case = Case.from_json(c)
+    assert case.preconditioner == cfdgpu.IC0 and case.tolerance == 1e-10 and case.max_iterations == 1200
+    c["numerics"]["pressure_preconditioner"] = "multigrid"
+    assert Case.from_json(c).preconditioner == cfdgpu.MULTIGRID
