@@ -143,6 +143,7 @@ def lib():
             "cfd_device_synchronize": (I, []),
             "cfd_last_error": (ctypes.c_char_p, []),
             "cfd_case_topology": (I, [P, P, P, P, P, I64, P]),
+            "cfd_profile_kernel": (I, [P, I, I, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -288,6 +289,14 @@ class Grid:
             level_dimensions=[list(s.level_dims[i]) for i in range(n)],
             allocated_bytes=s.allocated_bytes,
         )
+
+    def profile_kernel(self, kernel: int, reps: int = 20):
+        """(average seconds per launch, algorithmic bytes per launch) of a hot kernel:
+        0 operator, 1 fine smoother half-sweep, 2 x/r update, 3 IC(0) apply, 4 V-cycle."""
+        sec = ctypes.c_double(0.0)
+        nbytes = ctypes.c_double(0.0)
+        _check(lib().cfd_profile_kernel(self.h, int(kernel), int(reps), ctypes.byref(sec), ctypes.byref(nbytes)))
+        return sec.value, nbytes.value
 
     def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False):
         o = Options(int(precision), int(bool(warm_start)), 8, 0)

@@ -670,6 +670,81 @@ int cfd_project(cfd_grid* g, double* u, double* p, double dt, double tol, int64_
   return CFD_OK;
 }
 
+int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch) {
+  if (reps < 1 || kernel < 0 || kernel > 4) {
+    set_error("cfd_profile_kernel: kernel must be 0..4 and reps positive");
+    return CFD_EINVAL;
+  }
+  if ((kernel == 1 || kernel == 4) && !g->has_hier) {
+    set_error("cfd_profile_kernel: the grid has no multigrid hierarchy");
+    return CFD_EINVAL;
+  }
+  if (kernel == 3 && g->host.preconditioner != CFD_IC0) {
+    set_error("cfd_profile_kernel: the grid's preconditioner is not ic0");
+    return CFD_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  const long long N = g->geo.N;
+  PcgState* st = g->st.p;
+  // Work vectors with finite values; the state stays RUNNING with alpha = beta = 0.
+  launch_fill(N, g->z.p, 1.0, g->stream);
+  launch_fill(N, g->p0.p, 0.5, g->stream);
+  launch_fill(N, g->r.p, 0.25, g->stream);
+  launch_fill(N, g->x.p, 0.0, g->stream);
+  launch_state_init(st, 1.0, 1LL << 60, g->stream);
+  const double* mz = g->mean_z.p;
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  if (g->n_closed > 0) launch_fill(std::max(1, g->n_closed), g->mean_z.p, 0.0, g->stream);
+  const bool mixed = g->opt.precision == CFD_PRECISION_MIXED;
+  auto once = [&]() {
+    switch (kernel) {
+      case 0:
+        launch_pcg_op(g->geo, g->nb, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+        break;
+      case 1:
+        if (mixed)
+          launch_fine_gs<float, double>(g->geo, g->codes.p, g->r.p, g->h32.x0, 1, g->h32.levels[0].g, g->stream);
+        else
+          launch_fine_gs<double, double>(g->geo, g->codes.p, g->r.p, g->z.p, 1, g->h64.levels[0].g, g->stream);
+        break;
+      case 2:
+        launch_pcg_axpy(N, g->nb, g->p0.p, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+        break;
+      default:
+        enqueue_precondition(g, g->r.p, g->z.p, st);
+        break;
+    }
+  };
+  once();  // warm
+  HIP_TRY(hipEventRecord(g->ev0, g->stream));
+  for (int i = 0; i < reps; ++i) once();
+  HIP_TRY(hipEventRecord(g->ev1, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+  *avg_seconds = (double)ms * 1e-3 / reps;
+  const double n = (double)N;
+  switch (kernel) {
+    case 0:  // z, p_old, code in; p_new, Ap out
+      *bytes_per_launch = n * (8 + 8 + 1 + 8 + 8);
+      break;
+    case 1:  // half the cells: neighbour x, own b (FP64 r), own code in; own x out
+      *bytes_per_launch = mixed ? 0.5 * n * (4 + 8 + 1 + 4) : 0.5 * n * (8 + 8 + 1 + 8);
+      break;
+    case 2:  // x, p, r, Ap in; x, r out
+      *bytes_per_launch = n * 48;
+      break;
+    case 3:  // forward r, D, code in, y out; backward y, D, code in, z out
+      *bytes_per_launch = n * 50;
+      break;
+    default: {  // fine level: 8 half-sweeps + residual/restriction; coarse levels add about 1/7 more
+      double fine = mixed ? 8 * 0.5 * (4 + 8 + 1 + 4) + (4 + 8 + 1 + 0.5) : 8 * 0.5 * (8 + 8 + 1 + 8) + (8 + 8 + 1 + 1);
+      *bytes_per_launch = n * fine * 8.0 / 7.0;
+    }
+  }
+  return CFD_OK;
+}
+
 int cfd_device_alloc(size_t bytes, void** d_ptr) {
   HIP_TRY(hipMalloc(d_ptr, bytes));
   return CFD_OK;

@@ -961,6 +961,14 @@ void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int
                      0, st);
 }
 
+template <class T, class TB>
+void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s) {
+  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(color_blocks<T>(g0)), dim3(BS), 0, s, g0, codes, b0, x0, color, 0,
+                     (const T*)nullptr, g1, (double*)nullptr, 0, (const PcgState*)nullptr);
+}
+template void launch_fine_gs<double, double>(const Geo&, const uint8_t*, const double*, double*, int, const Geo&, hipStream_t);
+template void launch_fine_gs<float, double>(const Geo&, const uint8_t*, const double*, float*, int, const Geo&, hipStream_t);
+
 template void launch_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, int, DevLevel<double>*, int,
                                             const double*, double*, const PcgState*, hipStream_t);
 template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, int, DevLevel<float>*, int,

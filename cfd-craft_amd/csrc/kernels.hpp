@@ -35,6 +35,8 @@ void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const doubl
 template <class T, class TB>
 void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv, int direct_n,
                    const double* G, double* out64, const PcgState* st, hipStream_t s);
+template <class T, class TB>
+void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s);
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,
                            double* terms, hipStream_t s);
 void launch_project_faces(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* phi, double dt, double* u,

@@ -175,6 +175,15 @@ int cfd_precondition_device(cfd_grid* g, const double* d_r, double* d_z);
 int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
                        cfd_poll_fn poll, void* poll_user, cfd_linear_info* info);
 
+/* Measurement hook (the GPU counterpart of the reference's phase spans,
+ * profiling.rs:217-281): average device duration of one launch of a hot kernel
+ * on the grid's stream, from HIP events around `reps` back-to-back launches over
+ * the grid's own work vectors, and the kernel's algorithmic HBM bytes per
+ * launch (each array crossing HBM once). kernel: 0 fused p update + operator +
+ * p.Ap; 1 fine-level red-black half-sweep of the multigrid smoother; 2 x/r
+ * update with |r|^2; 3 one IC(0) application (both sweeps); 4 one V-cycle. */
+int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch);
+
 /* Device memory helpers for callers without their own allocator. */
 int cfd_device_alloc(size_t bytes, void** d_ptr);
 int cfd_device_free(void* d_ptr);
