@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -33,7 +34,17 @@ void set_error(const std::string& text) { g_error = text; }
   } while (0)
 
 constexpr int kSegment = 8;          // PCG iterations per host poll (pressure.rs:387)
-constexpr int64_t kCoarseMax = 512;  // coarsest multigrid level solved by the dense pseudo-inverse
+// Multigrid shape: the coarsest level (solved by the dense pseudo-inverse) holds at
+// most kCoarseMax cells; coarse levels of at most kTailCells run inside one
+// workgroup (0: none); the coarse-grid correction is scaled by kOmega. A 2x2x2
+// piecewise-constant Galerkin operator is about twice as stiff as the
+// rediscretized one in 3-D, so the plain correction falls short by about half;
+// omega = 1.9 cut PCG iterations on the urban block from 36 to 11 (2.0 is the
+// optimum on every tested grid; 1.9 keeps a margin below the loss of
+// definiteness past 2). CFD_MG_COARSE / CFD_MG_TAIL / CFD_MG_OMEGA override.
+constexpr int64_t kCoarseMax = 512;
+constexpr int64_t kTailCells = 0;
+constexpr double kOmega = 1.9;
 constexpr int kMaxReduceBlocks = 2048;
 
 template <class T>
@@ -65,6 +76,8 @@ struct DevBuf {
 template <class T>
 struct DevHierarchy {
   std::vector<DevLevel<T>> levels;
+  DevBuf<DevLevel<T>> dlevels;  // device copy of `levels` for the one-workgroup tail
+  int tail_first = 0;
   std::vector<std::unique_ptr<DevBuf<T>>> store;
   T* x0 = nullptr;  // FP32 fine-level iterate (mixed precision only)
   DevBuf<T> x0buf;
@@ -106,6 +119,10 @@ struct cfd_grid {
   DevBuf<double> G;
   // captured iteration segment, per precision
   hipGraphExec_t seg[2] = {nullptr, nullptr};
+  // CFD_GPU_GRAPHS=0 launches the segments kernel by kernel (for profilers that do not follow graphs).
+  bool use_graphs = true;
+  int64_t coarse_max = kCoarseMax, tail_cells = kTailCells;
+  double omega = kOmega;
   ~cfd_grid() {
     for (auto& g : seg)
       if (g) (void)hipGraphExecDestroy(g);
@@ -182,6 +199,16 @@ int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
     if (!d.diag || !d.x || !d.b) return CFD_ENOMEM;
     dh.levels.push_back(d);
   }
+  dh.tail_first = (int)dh.levels.size();
+  for (size_t l = 0; l < dh.levels.size(); ++l)
+    if (dh.levels[l].g.N <= g->tail_cells) {
+      dh.tail_first = (int)l;
+      break;
+    }
+  if (!dh.levels.empty()) {
+    int rc = dh.dlevels.upload(dh.levels.data(), dh.levels.size());
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -193,6 +220,10 @@ int setup_device(cfd_grid* g) {
   HIP_TRY(hipEventCreate(&g->ev1));
   HIP_TRY(hipHostMalloc((void**)&g->h_st, sizeof(PcgState), hipHostMallocDefault));
   fill_geo(g);
+  if (const char* env = std::getenv("CFD_GPU_GRAPHS")) g->use_graphs = std::string(env) != "0";
+  if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
+  if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
+  if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   size_t N = (size_t)h.count;
   int rc;
   if ((rc = g->codes.upload(h.codes.data(), N))) return rc;
@@ -225,7 +256,8 @@ int setup_device(cfd_grid* g) {
   for (DevBuf<double>* v : {&g->mean_b, &g->mean_z, &g->mean_x})
     if ((rc = v->alloc(nm))) return rc;
   if ((rc = g->sums.alloc(2 * nm))) return rc;
-  if ((rc = g->part.alloc((size_t)std::max(2 * g->nb, 2 * g->n_chunks + 2)))) return rc;
+  if ((rc = g->part.alloc((size_t)(P_CS + 2 * std::max(kReduceGrid, g->n_chunks + 1))))) return rc;
+  HIP_TRY(hipMemset(g->part.p, 0, g->part.n * sizeof(double)));
   if ((rc = g->tickets.alloc(8))) return rc;
   HIP_TRY(hipMemset(g->tickets.p, 0, 8 * sizeof(unsigned int)));
   if ((rc = g->st.alloc(1))) return rc;
@@ -234,7 +266,7 @@ int setup_device(cfd_grid* g) {
     build_ic0_diagonal(h);
     if ((rc = g->D.upload(h.ic0_diag.data(), N))) return rc;
   } else {
-    rc = build_hierarchy(h, g->hier, kCoarseMax);
+    rc = build_hierarchy(h, g->hier, g->coarse_max);
     if (rc) return rc;
     g->has_hier = true;
     if ((rc = upload_hierarchy(g, g->h64))) return rc;
@@ -247,20 +279,11 @@ int setup_device(cfd_grid* g) {
   return 0;
 }
 
-// Per closed component: mean of src into `mean` (and the E031 test when scale is given).
-void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st,
-                   unsigned int* ticket) {
-  if (g->n_closed == 0) return;
-  if (g->n_closed == 1) {
-    launch_csum1(g->geo.N, g->nb, g->codes.p, src, scale, e031, g->inv_count1, mean, g->sums.p, st, g->part.p, ticket,
-                 g->stream);
-  } else {
-    launch_csumN(g->n_chunks, g->members.p, g->chunk_begin.p, g->comp_first_chunk.p, g->n_closed, g->inv_count.p, src,
-                 scale, e031, mean, g->sums.p, st, g->part.p, ticket, g->stream);
-  }
+// Per closed component: mean of src into `mean` (and the E031 test when e031 is set).
+void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st) {
+  launch_gauge(g->geo.N, g->codes.p, g->n_closed, g->n_chunks, g->members.p, g->chunk_begin.p, g->comp_first_chunk.p,
+               g->inv_count.p, g->inv_count1, src, scale, e031, mean, st, g->part.p, g->stream);
 }
-const double* gauge_means(cfd_grid*, double* mean) { return mean; }
-
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
   if (g->host.preconditioner == CFD_IC0) {
     launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
@@ -269,11 +292,11 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
   const int nlev = (int)g->h64.levels.size();
   const int dn = (int)g->hier.direct_n;
   if (g->opt.precision == CFD_PRECISION_MIXED) {
-    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->h32.x0, nlev, g->h32.levels.data(), dn, g->G.p, z, st,
-                                 g->stream);
+    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->h32.x0, nlev, g->h32.levels.data(), g->h32.dlevels.p,
+                                 g->h32.tail_first, dn, g->G.p, z, st, g->omega, g->stream);
   } else {
-    launch_vcycle<double, double>(g->geo, g->codes.p, r, z, nlev, g->h64.levels.data(), dn, g->G.p, nullptr, st,
-                                  g->stream);
+    launch_vcycle<double, double>(g->geo, g->codes.p, r, z, nlev, g->h64.levels.data(), g->h64.dlevels.p,
+                                  g->h64.tail_first, dn, g->G.p, nullptr, st, g->omega, g->stream);
   }
 }
 
@@ -282,14 +305,14 @@ void enqueue_iteration(cfd_grid* g, int parity) {
   PcgState* st = g->st.p;
   double* pold = parity ? g->p1.p : g->p0.p;
   double* pnew = parity ? g->p0.p : g->p1.p;
-  const double* mz = gauge_means(g, g->mean_z.p);
+  const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  launch_pcg_op(g->geo, g->nb, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, st, g->part.p, g->stream);
-  launch_pcg_axpy(g->geo.N, g->nb, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
-  launch_pcg_true(g->geo, g->nb, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
+  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+  launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+  launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
-  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st, &st->ticket);
-  launch_pcg_rz(g->geo.N, g->nb, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, 0, g->stream);
+  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
+  launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
 }
 
 int segment_graph(cfd_grid* g, hipGraphExec_t* out) {
@@ -355,13 +378,11 @@ int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double t
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
   launch_state_init(st, tol, max_iterations, g->stream);
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  enqueue_gauge(g, d_b, d_scale, 1, g->mean_b.p, st, &st->ticket);
-  launch_pcg_init(g->geo.N, g->nb, g->codes.p, d_b, gauge_means(g, g->mean_b.p), cidx, g->r.p, g->bc.p, g->x.p, st,
-                  g->part.p, g->stream);
+  enqueue_gauge(g, d_b, d_scale, 1, g->mean_b.p, st);
+  launch_pcg_init(g->geo.N, g->codes.p, d_b, g->mean_b.p, cidx, g->r.p, g->bc.p, g->x.p, st, g->part.p, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
-  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st, &st->ticket);
-  launch_pcg_rz(g->geo.N, g->nb, g->codes.p, g->r.p, g->z.p, gauge_means(g, g->mean_z.p), cidx, st, g->part.p, 1,
-                g->stream);
+  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
+  launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);
   int rc = read_state(g);
   if (rc) return rc;
   hipGraphExec_t seg = nullptr;
@@ -377,8 +398,12 @@ int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double t
         return CFD_WALL_TIME;
       }
     }
-    if (!seg && (rc = segment_graph(g, &seg))) return rc;
-    HIP_TRY(hipGraphLaunch(seg, g->stream));
+    if (g->use_graphs) {
+      if (!seg && (rc = segment_graph(g, &seg))) return rc;
+      HIP_TRY(hipGraphLaunch(seg, g->stream));
+    } else {
+      for (int it = 0; it < kSegment; ++it) enqueue_iteration(g, it & 1);
+    }
     if ((rc = read_state(g))) return rc;
   }
   const PcgState s = *g->h_st;
@@ -405,8 +430,8 @@ int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double t
   if (s.status == ST_ZERO) {
     launch_fill(g->geo.N, d_x, 0.0, g->stream);
   } else {
-    enqueue_gauge(g, g->x.p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
-    launch_center(g->geo.N, g->codes.p, g->x.p, gauge_means(g, g->mean_x.p), cidx, d_x, g->stream);
+    enqueue_gauge(g, g->x.p, nullptr, 0, g->mean_x.p, nullptr);
+    launch_center(g->geo.N, g->codes.p, g->x.p, g->mean_x.p, cidx, d_x, g->stream);
   }
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
@@ -576,8 +601,8 @@ int cfd_center_nullspaces(cfd_grid* g, double* x) {
   size_t bytes = (size_t)g->geo.N * sizeof(double);
   HIP_TRY(hipMemcpyAsync(g->b.p, x, bytes, hipMemcpyHostToDevice, g->stream));
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  enqueue_gauge(g, g->b.p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
-  launch_center(g->geo.N, g->codes.p, g->b.p, gauge_means(g, g->mean_x.p), cidx, g->xo.p, g->stream);
+  enqueue_gauge(g, g->b.p, nullptr, 0, g->mean_x.p, nullptr);
+  launch_center(g->geo.N, g->codes.p, g->b.p, g->mean_x.p, cidx, g->xo.p, g->stream);
   HIP_TRY(hipMemcpyAsync(x, g->xo.p, bytes, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   return CFD_OK;
@@ -645,8 +670,8 @@ int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double 
   launch_project_faces(g->geo, g->fgeo, g->codes.p, g->phi.p, dt, d_u, g->stream);
   launch_add(g->geo.N, d_p, g->phi.p, g->stream);
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  enqueue_gauge(g, d_p, nullptr, 0, g->mean_x.p, nullptr, g->tickets.p);
-  launch_center(g->geo.N, g->codes.p, d_p, gauge_means(g, g->mean_x.p), cidx, d_p, g->stream);
+  enqueue_gauge(g, d_p, nullptr, 0, g->mean_x.p, nullptr);
+  launch_center(g->geo.N, g->codes.p, d_p, g->mean_x.p, cidx, d_p, g->stream);
   HIP_TRY(hipStreamSynchronize(g->stream));
   return CFD_OK;
 }
@@ -675,7 +700,7 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
     set_error("cfd_profile_kernel: kernel must be 0..4 and reps positive");
     return CFD_EINVAL;
   }
-  if ((kernel == 1 || kernel == 4) && !g->has_hier) {
+  if ((kernel == 1 || kernel == 4) && (!g->has_hier || g->h64.levels.empty())) {
     set_error("cfd_profile_kernel: the grid has no multigrid hierarchy");
     return CFD_EINVAL;
   }
@@ -699,7 +724,7 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
   auto once = [&]() {
     switch (kernel) {
       case 0:
-        launch_pcg_op(g->geo, g->nb, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+        launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, st, g->part.p, g->stream);
         break;
       case 1:
         if (mixed)
@@ -708,7 +733,7 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
           launch_fine_gs<double, double>(g->geo, g->codes.p, g->r.p, g->z.p, 1, g->h64.levels[0].g, g->stream);
         break;
       case 2:
-        launch_pcg_axpy(N, g->nb, g->p0.p, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+        launch_pcg_axpy(N, g->p0.p, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
         break;
       default:
         enqueue_precondition(g, g->r.p, g->z.p, st);

@@ -14,6 +14,16 @@ struct Geo {
   double wo[3];      // outlet face weight per axis (fine level)
 };
 
+// Reduction grids: a fixed number of blocks writes one partial each.
+constexpr int kReduceBlock = 1024;
+constexpr int kReduceGrid = 512;
+// Regions of a grid's partials buffer, one per reduction site of an iteration.
+constexpr int P_PAP = 0;                // p . Ap
+constexpr int P_RR = kReduceGrid;       // |r|^2
+constexpr int P_TR = 2 * kReduceGrid;   // |A x - b_c|^2
+constexpr int P_RZ = 3 * kReduceGrid;   // r . z
+constexpr int P_CS = 4 * kReduceGrid;   // component sums (2 per block or chunk)
+
 // Device-resident PCG scalars (FP64) and control words.
 enum : int {
   ST_RUNNING = 0,

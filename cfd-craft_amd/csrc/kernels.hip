@@ -14,48 +14,67 @@
 namespace cfdgpu {
 
 constexpr int BS = 256;
+constexpr int RB = kReduceBlock;  // threads of a reduction block
+constexpr int RP = kReduceGrid;   // blocks of a reduction grid
 constexpr uint8_t FLUID = 1u << 6;
 constexpr uint8_t CLOSED = 1u << 7;
 
 // ---------------------------------------------------------------- reductions
+// Device-wide sums without fences: a reduction kernel runs on a fixed grid of
+// RP blocks of RB threads and writes one partial per block; the kernel that
+// needs the total re-reduces the RP partials in a fixed order (the kernel
+// boundary makes them visible). Results are deterministic for a given grid.
+// (An agent-scope fence per block would write back and invalidate the L2 of
+// every XCD once per wave.)
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
-// Sum over a 256-thread block, valid in thread 0. Fixed association order.
+// Sum over a block of RB threads, returned to every thread (lane 0 of each wave,
+// then lane 0 of the first wave: fixed association order).
 __device__ double block_sum(double v) {
-  __shared__ double s[BS / 64];
+  __shared__ double s[RB / 64];
+  __shared__ double total;
   v = wave_sum(v);
-  __syncthreads();
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
   __syncthreads();
-  return (s[0] + s[1]) + (s[2] + s[3]);
-}
-// True in every thread of the last block to finish (release/acquire at agent scope).
-__device__ bool last_block(unsigned int* ticket) {
-  __shared__ unsigned int t;
-  __threadfence();
+  if (threadIdx.x < 64) {
+    double t = threadIdx.x < RB / 64 ? s[threadIdx.x] : 0.0;
+    t = wave_sum(t);
+    if (threadIdx.x == 0) total = t;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) t = atomicAdd(ticket, 1u);
+  const double r = total;
   __syncthreads();
-  bool last = t == gridDim.x - 1;
-  if (last) __threadfence();
-  return last;
+  return r;
 }
-// Deterministic sum of per-block partials inside the last block.
-__device__ double sum_partials(const double* part, int n) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += BS) v += __builtin_nontemporal_load(part + i);
+// Total of n <= RB partials, in every thread.
+__device__ double reduce_part(const double* part, int n, int stride = 1) {
+  double v = (int)threadIdx.x < n ? part[threadIdx.x * stride] : 0.0;
   return block_sum(v);
+}
+// The block's view of the solve status, read once so that every thread agrees.
+__device__ bool block_running(const PcgState* st) {
+  __shared__ int status;
+  if (threadIdx.x == 0) status = st->status;
+  __syncthreads();
+  const bool run = status == ST_RUNNING;
+  __syncthreads();
+  return run;
 }
 
 // ---------------------------------------------------------------- geometry helpers
+// Cell coordinates from a linear index; grids hold at most 1e8 cells, so 32-bit
+// unsigned division is exact (and far cheaper than 64-bit division).
 __device__ __forceinline__ void coords(const Geo& g, long long c, int q[3]) {
-  long long r = c / g.n[0];
-  q[0] = (int)(c - r * g.n[0]);
-  q[1] = (int)(r % g.n[1]);
-  q[2] = (int)(r / g.n[1]);
+  const unsigned u = (unsigned)c;
+  const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
+  const unsigned r = u / nx;
+  q[0] = (int)(u - r * nx);
+  const unsigned k = r / ny;
+  q[1] = (int)(r - k * ny);
+  q[2] = (int)k;
 }
 __device__ __forceinline__ long long stride_of(const Geo& g, int a) { return a == 0 ? 1 : (a == 1 ? g.sy : g.sz); }
 __device__ __forceinline__ long long lo_nb(const Geo& g, long long c, int a, int t) {
@@ -123,16 +142,16 @@ __global__ void __launch_bounds__(BS) k_apply(Geo g, const uint8_t* __restrict__
   y[c] = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
 }
 
-// p_new = z_c + beta p (or z_c on the first iteration), Ap = A p_new, and the
-// partial sums of p_new . Ap; the last block forms alpha = rz / (p . Ap) and
-// raises CFD-E032 (pressure.rs:391-397,426-432).
-__global__ void __launch_bounds__(BS)
+// p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
+// Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
+__global__ void __launch_bounds__(RB)
     k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ z, const double* __restrict__ pold,
              double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
              const int* __restrict__ cidx, PcgState* st, double* part) {
-  if (st->status != ST_RUNNING) return;
+  if (!block_running(st)) return;
   const bool first = st->first;
-  const double beta = st->beta;
+  double beta = 0.0;
+  if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   auto pv = [&](long long i) {
     uint8_t ci = codes[i];
     double zc = z[i];
@@ -140,7 +159,7 @@ __global__ void __launch_bounds__(BS)
     return first ? zc : zc + beta * pold[i];
   };
   double acc = 0.0;
-  for (long long c = (long long)blockIdx.x * BS + threadIdx.x; c < g.N; c += (long long)gridDim.x * BS) {
+  for (long long c = (long long)blockIdx.x * RB + threadIdx.x; c < g.N; c += (long long)gridDim.x * RB) {
     int q[3];
     coords(g, c, q);
     uint8_t code = codes[c];
@@ -151,53 +170,56 @@ __global__ void __launch_bounds__(BS)
     acc += pc * y;
   }
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (!last_block(&st->ticket)) return;
-  double denom = sum_partials(part, gridDim.x);
   if (threadIdx.x == 0) {
-    st->denom = denom;
-    if (denom <= 0.0 || !isfinite(denom)) {
-      st->status = ST_E032;
-    } else {
-      st->alpha = st->rz / denom;
-    }
-    st->first = 0;
-    st->ticket = 0;
+    part[P_PAP + blockIdx.x] = s;
+    if (blockIdx.x == 0 && !first) st->beta = beta;
   }
 }
 
-// x += alpha p; r -= alpha Ap; |r|^2 -> recurrent residual test (pressure.rs:397-403).
-__global__ void __launch_bounds__(BS) k_pcg_axpy(long long N, const double* __restrict__ p, const double* __restrict__ ap,
+// alpha = rz / (p . Ap) with CFD-E032 (pressure.rs:393-397); x += alpha p;
+// r -= alpha Ap; partials of |r|^2 (pressure.rs:398-402).
+__global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __restrict__ p, const double* __restrict__ ap,
                                                  double* __restrict__ x, double* __restrict__ r, PcgState* st,
                                                  double* part) {
-  if (st->status != ST_RUNNING) return;
-  const double alpha = st->alpha;
+  if (!block_running(st)) return;
+  const double rz = reduce_part(part + P_RZ, RP);
+  const double denom = reduce_part(part + P_PAP, RP);
+  if (denom <= 0.0 || !isfinite(denom)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->denom = denom;
+      st->status = ST_E032;
+    }
+    return;
+  }
+  const double alpha = rz / denom;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < N; i += (long long)gridDim.x * BS) {
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
     x[i] += alpha * p[i];
     double ri = r[i] - alpha * ap[i];
     r[i] = ri;
     acc += ri * ri;
   }
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (!last_block(&st->ticket)) return;
-  double rr = sum_partials(part, gridDim.x);
   if (threadIdx.x == 0) {
-    double rel = sqrt(rr) / st->norm;
-    st->rel = rel;
-    if (rel <= st->tol) st->check = 1;
-    st->ticket = 0;
+    part[P_RR + blockIdx.x] = s;
+    if (blockIdx.x == 0) {
+      st->rz = rz;
+      st->alpha = alpha;
+      st->denom = denom;
+      st->first = 0;
+    }
   }
 }
 
-// Recomputed residual |A x - b_c| / |b_c| and the acceptance gate 10 tol + 1e-12
-// (pressure.rs:403-425). Runs only after the recurrent residual crossed tol.
-__global__ void __launch_bounds__(BS) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
+// When the recurrent residual |r|/|b_c| reached tol: partials of |A x - b_c|^2,
+// the independent recomputed residual (pressure.rs:403-414).
+__global__ void __launch_bounds__(RB) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
                                                  const double* __restrict__ bc, PcgState* st, double* part) {
-  if (st->status != ST_RUNNING || !st->check) return;
+  if (!block_running(st)) return;
+  const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
+  if (!(rel <= st->tol)) return;
   double acc = 0.0;
-  for (long long c = (long long)blockIdx.x * BS + threadIdx.x; c < g.N; c += (long long)gridDim.x * BS) {
+  for (long long c = (long long)blockIdx.x * RB + threadIdx.x; c < g.N; c += (long long)gridDim.x * RB) {
     int q[3];
     coords(g, c, q);
     double y = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
@@ -205,65 +227,58 @@ __global__ void __launch_bounds__(BS) k_pcg_true(Geo g, const uint8_t* __restric
     acc += d * d;
   }
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (!last_block(&st->ticket)) return;
-  double ss = sum_partials(part, gridDim.x);
+  if (threadIdx.x == 0) part[P_TR + blockIdx.x] = s;
+}
+
+// One block: the iteration's decisions. Acceptance needs the recurrent and the
+// recomputed residual (<= 10 tol + 1e-12, pressure.rs:415); the limit raises
+// CFD-E033 (pressure.rs:435).
+__global__ void __launch_bounds__(RB) k_pcg_fin(PcgState* st, const double* part) {
+  if (!block_running(st)) return;
+  const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
+  double tr = 0.0;
+  const bool check = rel <= st->tol;
+  if (check) tr = sqrt(reduce_part(part + P_TR, RP)) / st->norm;
   if (threadIdx.x == 0) {
-    double tr = sqrt(ss) / st->norm;
-    st->true_rel = tr;
-    st->true_checks += 1;
-    if (tr <= st->accept) {
-      st->status = ST_CONVERGED;
-      st->it += 1;
-    } else {
-      st->check = 0;
+    st->rel = rel;
+    st->it += 1;
+    if (check) {
+      st->true_rel = tr;
+      st->true_checks += 1;
     }
-    st->ticket = 0;
+    if (check && tr <= st->accept)
+      st->status = ST_CONVERGED;
+    else if (st->it >= st->max_it)
+      st->status = ST_E033;
   }
 }
 
-// rz_next = r . z_c, beta = rz_next / rz; closes one PCG iteration and raises
-// CFD-E033 at the limit (pressure.rs:426-435). With init set it forms the
-// first r . z instead.
-__global__ void __launch_bounds__(BS) k_pcg_rz(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ r,
+// Partials of r . z_c (pressure.rs:428); the next iteration's kernels reduce them.
+__global__ void __launch_bounds__(RB) k_pcg_rz(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ r,
                                                const double* __restrict__ z, const double* __restrict__ mean,
-                                               const int* __restrict__ cidx, PcgState* st, double* part, int init) {
-  if (st->status != ST_RUNNING) return;
+                                               const int* __restrict__ cidx, PcgState* st, double* part) {
+  if (!block_running(st)) return;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < N; i += (long long)gridDim.x * BS) {
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
     uint8_t ci = codes[i];
     double zc = z[i];
     if (ci & CLOSED) zc -= mean[cidx ? cidx[i] : 0];
     acc += r[i] * zc;
   }
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (!last_block(&st->ticket)) return;
-  double next = sum_partials(part, gridDim.x);
-  if (threadIdx.x == 0) {
-    if (init) {
-      st->rz = next;
-      st->first = 1;
-    } else {
-      st->beta = next / st->rz;
-      st->rz = next;
-      st->it += 1;
-      if (st->it >= st->max_it) st->status = ST_E033;
-    }
-    st->ticket = 0;
-  }
+  if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------- gauges
 // Per closed component: sum of src (and of max(|src|, scale) for the CFD-E031
-// compatibility test, pressure.rs:351-362); mean = sum / members.
+// compatibility test, pressure.rs:351-362); then mean = sum / members.
 // One closed component: a grid-stride pass over the cells flagged closed.
-__global__ void __launch_bounds__(BS) k_csum1(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ src,
-                                              const double* __restrict__ scale, int e031, double inv_count, double* mean,
-                                              double* sums, PcgState* st, double* part, unsigned int* ticket) {
-  if (st && st->status != ST_RUNNING) return;
+__global__ void __launch_bounds__(RB) k_csum1(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ src,
+                                              const double* __restrict__ scale, int e031, const PcgState* st,
+                                              double* part) {
+  if (st && !block_running(st)) return;
   double acc = 0.0, acc2 = 0.0;
-  for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < N; i += (long long)gridDim.x * BS) {
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
     if (!(codes[i] & CLOSED)) continue;
     double v = src[i];
     acc += v;
@@ -272,38 +287,20 @@ __global__ void __launch_bounds__(BS) k_csum1(long long N, const uint8_t* __rest
   double s = block_sum(acc);
   double s2 = e031 ? block_sum(acc2) : 0.0;
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s;
-    part[2 * blockIdx.x + 1] = s2;
-  }
-  if (!last_block(ticket)) return;
-  double v = 0.0, v2 = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS) {
-    v += __builtin_nontemporal_load(part + 2 * i);
-    v2 += __builtin_nontemporal_load(part + 2 * i + 1);
-  }
-  double sum = block_sum(v);
-  double sum2 = e031 ? block_sum(v2) : 0.0;
-  if (threadIdx.x == 0) {
-    mean[0] = sum * inv_count;
-    sums[0] = sum;
-    sums[1] = sum2;
-    if (e031 && st && fabs(sum) > 1e-9 * sum2) st->status = ST_E031;
-    *ticket = 0;
+    part[P_CS + 2 * blockIdx.x] = s;
+    part[P_CS + 2 * blockIdx.x + 1] = s2;
   }
 }
 
-// Several closed components: one block per chunk of a component's member list,
-// then the last block sums each component's chunks in order.
-__global__ void __launch_bounds__(BS)
-    k_csumN(const int* __restrict__ members, const int* __restrict__ chunk_begin, const int* __restrict__ chunk_comp,
-            const int* __restrict__ comp_first_chunk, int n_comp, const double* __restrict__ inv_count,
-            const double* __restrict__ src, const double* __restrict__ scale, int e031, double* mean, double* sums,
-            PcgState* st, double* part, unsigned int* ticket) {
-  if (st && st->status != ST_RUNNING) return;
+// Several closed components: one block per chunk of a component's member list.
+__global__ void __launch_bounds__(RB) k_csumN(const int* __restrict__ members, const int* __restrict__ chunk_begin,
+                                              const double* __restrict__ src, const double* __restrict__ scale, int e031,
+                                              const PcgState* st, double* part) {
+  if (st && !block_running(st)) return;
   const int ch = blockIdx.x;
   const int b0 = chunk_begin[ch], b1 = chunk_begin[ch + 1];
   double acc = 0.0, acc2 = 0.0;
-  for (int k = b0 + threadIdx.x; k < b1; k += BS) {
+  for (int k = b0 + threadIdx.x; k < b1; k += RB) {
     int i = members[k];
     double v = src[i];
     acc += v;
@@ -312,45 +309,49 @@ __global__ void __launch_bounds__(BS)
   double s = block_sum(acc);
   double s2 = e031 ? block_sum(acc2) : 0.0;
   if (threadIdx.x == 0) {
-    part[2 * ch] = s;
-    part[2 * ch + 1] = s2;
-  }
-  if (!last_block(ticket)) return;
-  bool bad = false;
-  for (int comp = threadIdx.x; comp < n_comp; comp += BS) {
-    double v = 0.0, v2 = 0.0;
-    for (int k = comp_first_chunk[comp]; k < comp_first_chunk[comp + 1]; ++k) {
-      v += __builtin_nontemporal_load(part + 2 * k);
-      v2 += __builtin_nontemporal_load(part + 2 * k + 1);
-    }
-    mean[comp] = v * inv_count[comp];
-    if (sums) {
-      sums[2 * comp] = v;
-      sums[2 * comp + 1] = v2;
-    }
-    if (e031 && fabs(v) > 1e-9 * v2) bad = true;
-  }
-  (void)chunk_comp;
-  __shared__ int any_bad;
-  if (threadIdx.x == 0) any_bad = 0;
-  __syncthreads();
-  if (bad) atomicOr(&any_bad, 1);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (st && any_bad) st->status = ST_E031;
-    *ticket = 0;
+    part[P_CS + 2 * ch] = s;
+    part[P_CS + 2 * ch + 1] = s2;
   }
 }
 
-// r = b - mean (closed cells), b_c = r, x = 0, |r| -> norm; a zero norm ends the
-// solve with zero iterations (pressure.rs:365-375).
-__global__ void __launch_bounds__(BS) k_pcg_init(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ b,
+// One block: component means from the partials, and the CFD-E031 test.
+__global__ void __launch_bounds__(RB) k_csum_fin(int n_comp, const int* __restrict__ comp_first_chunk,
+                                                 const double* __restrict__ inv_count, double inv_count1, int e031,
+                                                 double* mean, PcgState* st, const double* part) {
+  if (st && !block_running(st)) return;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  if (n_comp == 1) {
+    const double sum = reduce_part(part + P_CS, RP, 2);
+    const double sum2 = e031 ? reduce_part(part + P_CS + 1, RP, 2) : 0.0;
+    if (threadIdx.x == 0) {
+      mean[0] = sum * inv_count1;
+      if (e031 && fabs(sum) > 1e-9 * sum2) bad = 1;
+    }
+  } else {
+    for (int comp = threadIdx.x; comp < n_comp; comp += RB) {
+      double v = 0.0, v2 = 0.0;
+      for (int k = comp_first_chunk[comp]; k < comp_first_chunk[comp + 1]; ++k) {
+        v += part[P_CS + 2 * k];
+        v2 += part[P_CS + 2 * k + 1];
+      }
+      mean[comp] = v * inv_count[comp];
+      if (e031 && fabs(v) > 1e-9 * v2) atomicOr(&bad, 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && st && bad) st->status = ST_E031;
+}
+
+// r = b - mean (closed cells), b_c = r, x = 0, partials of |r|^2 (pressure.rs:365-372).
+__global__ void __launch_bounds__(RB) k_pcg_init(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ b,
                                                  const double* __restrict__ mean, const int* __restrict__ cidx,
                                                  double* __restrict__ r, double* __restrict__ bc, double* __restrict__ x,
-                                                 const double* __restrict__ x0, PcgState* st, double* part) {
-  if (st->status != ST_RUNNING) return;
+                                                 PcgState* st, double* part) {
+  if (!block_running(st)) return;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < N; i += (long long)gridDim.x * BS) {
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
     double v = b[i];
     uint8_t ci = codes[i];
     if (ci & CLOSED) v -= mean[cidx ? cidx[i] : 0];
@@ -360,30 +361,17 @@ __global__ void __launch_bounds__(BS) k_pcg_init(long long N, const uint8_t* __r
     acc += v * v;
   }
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-  if (!last_block(&st->ticket)) return;
-  double rr = sum_partials(part, gridDim.x);
-  if (threadIdx.x == 0) {
-    double norm = sqrt(rr);
-    st->norm = norm;
-    if (norm < 1e-24) st->status = ST_ZERO;
-    st->ticket = 0;
-  }
-  (void)x0;
+  if (threadIdx.x == 0) part[P_RR + blockIdx.x] = s;
 }
 
-// Warm start: x = x0, r = b_c - A x0 (b_c already formed), |b_c| from the caller's norm pass.
-__global__ void __launch_bounds__(BS) k_warm_residual(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x0,
-                                                      const double* __restrict__ bc, double* __restrict__ x,
-                                                      double* __restrict__ r, PcgState* st) {
-  if (st->status != ST_RUNNING) return;
-  long long c = (long long)blockIdx.x * BS + threadIdx.x;
-  if (c >= g.N) return;
-  int q[3];
-  coords(g, c, q);
-  double y = apply_cell(g, codes[c], c, q, [&](long long i) { return x0[i]; });
-  x[c] = x0[c];
-  r[c] = bc[c] - y;
+// One block: |b_c|; a zero right-hand side ends with zero iterations (pressure.rs:372-375).
+__global__ void __launch_bounds__(RB) k_init_fin(PcgState* st, const double* part) {
+  if (!block_running(st)) return;
+  const double norm = sqrt(reduce_part(part + P_RR, RP));
+  if (threadIdx.x == 0) {
+    st->norm = norm;
+    if (norm < 1e-24) st->status = ST_ZERO;
+  }
 }
 
 // dst = src - mean on closed cells (center_nullspaces, pressure.rs:316-325).
@@ -528,7 +516,8 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 template <class T, class TB>
 __global__ void __launch_bounds__(BS)
     k_fine_gs(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b, T* __restrict__ x, int color, int first,
-              const T* __restrict__ xc, Geo gc, double* __restrict__ out64, int all_cells, const PcgState* st) {
+              const T* __restrict__ xc, Geo gc, double* __restrict__ out64, int all_cells, const PcgState* st,
+              T cscale = T(1)) {
   if (st && st->status != ST_RUNNING) return;
   long long t = (long long)blockIdx.x * BS + threadIdx.x;
   int i, j, k;
@@ -540,12 +529,14 @@ __global__ void __launch_bounds__(BS)
     j = q[1];
     k = q[2];
   } else {
-    const int hx = (g.n[0] + 1) >> 1;
-    long long row = t / hx;
-    if (row >= (long long)g.n[1] * g.n[2]) return;
-    int i2 = (int)(t - row * hx);
-    j = (int)(row % g.n[1]);
-    k = (int)(row / g.n[1]);
+    const unsigned hx = (unsigned)(g.n[0] + 1) >> 1;
+    const unsigned tu = (unsigned)t;
+    const unsigned row = tu / hx;
+    if (row >= (unsigned)g.n[1] * (unsigned)g.n[2]) return;
+    const int i2 = (int)(tu - row * hx);
+    const unsigned kk = row / (unsigned)g.n[1];
+    j = (int)(row - kk * (unsigned)g.n[1]);
+    k = (int)kk;
     i = 2 * i2 + ((j + k + color) & 1);
     if (i >= g.n[0]) return;
   }
@@ -581,7 +572,7 @@ __global__ void __launch_bounds__(BS)
       if (xc) {
         int qn[3] = {i, j, k};
         qn[a] = side == 0 ? (q[a] > 0 ? q[a] - 1 : g.n[a] - 1) : (q[a] < g.n[a] - 1 ? q[a] + 1 : 0);
-        xn += xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+        xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
       }
       value += w * xn;
     }
@@ -591,61 +582,62 @@ __global__ void __launch_bounds__(BS)
   if (out64) out64[c] = (double)v;
 }
 
-// Fine residual r = b - A x summed over each 2x2x2 bin: the coarse right-hand side P^T r.
+// Fine-level residual (b - A x)_c of one cell from the coupling codes.
 template <class T, class TB>
-__global__ void __launch_bounds__(BS) k_fine_restrict(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b,
-                                                      const T* __restrict__ x, Geo gc, T* __restrict__ bc,
-                                                      const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
-  long long C = (long long)blockIdx.x * BS + threadIdx.x;
-  if (C >= gc.N) return;
-  int Q[3];
-  coords(gc, C, Q);
-  T sum = T(0);
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx) {
-        const int q[3] = {2 * Q[0] + dx, 2 * Q[1] + dy, 2 * Q[2] + dz};
-        if (q[0] >= g.n[0] || q[1] >= g.n[1] || q[2] >= g.n[2]) continue;
-        const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
-        const uint8_t code = codes[c];
-        if (!(code & FLUID)) continue;
-        const T xv = x[c];
-        T ax = T(0), off = T(0);
+__device__ __forceinline__ T fine_residual(const Geo& g, const uint8_t* __restrict__ codes, const TB* __restrict__ b,
+                                           const T* __restrict__ x, const int q[3]) {
+  const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
+  const uint8_t code = codes[c];
+  if (!(code & FLUID)) return T(0);
+  const T xv = x[c];
+  T ax = T(0), off = T(0);
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int side = 0; side < 2; ++side) {
-            if (!(code & (1u << (2 * a + side)))) continue;
-            const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
-            if (boundary && !g.per[a]) {
-              ax += (T)g.wo[a];
-              continue;
-            }
-            ax += (T)g.wi[a];
-            long long nb = side == 0 ? lo_nb(g, c, a, q[a]) : hi_nb(g, c, a, q[a]);
-            off += (T)g.wi[a] * x[nb];
-          }
-        sum += ((T)b[c] - (ax * xv - off));
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      if (boundary && !g.per[a]) {
+        ax += (T)g.wo[a];
+        continue;
       }
-  bc[C] = sum;
+      ax += (T)g.wi[a];
+      long long nb = side == 0 ? lo_nb(g, c, a, q[a]) : hi_nb(g, c, a, q[a]);
+      off += (T)g.wi[a] * x[nb];
+    }
+  return (T)b[c] - (ax * xv - off);
 }
 
-// Coarse-level red-black half-sweep with explicit per-face weights.
+// Coarse-level residual of one cell from the explicit face weights.
 template <class T>
-__global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int first, const T* __restrict__ xc, Geo gc,
-                                                  const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
+__device__ __forceinline__ T coarse_residual(const DevLevel<T>& L, const int q[3]) {
   const Geo& g = L.g;
-  long long t = (long long)blockIdx.x * BS + threadIdx.x;
-  const int hx = (g.n[0] + 1) >> 1;
-  long long row = t / hx;
-  if (row >= (long long)g.n[1] * g.n[2]) return;
-  int i2 = (int)(t - row * hx);
-  int j = (int)(row % g.n[1]);
-  int k = (int)(row / g.n[1]);
-  int i = 2 * i2 + ((j + k + color) & 1);
-  if (i >= g.n[0]) return;
+  const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
+  const T d = L.diag[c];
+  if (!(d > T(0))) return T(0);
+  T off = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.n[a];
+    if (q[a] < n - 1 || g.per[a]) {
+      T w = L.wp[a][c];
+      if (w != T(0)) off += w * L.x[hi_nb(g, c, a, q[a])];
+    }
+    if (q[a] > 0 || g.per[a]) {
+      long long nb = lo_nb(g, c, a, q[a]);
+      T w = L.wp[a][nb];
+      if (w != T(0)) off += w * L.x[nb];
+    }
+  }
+  return L.b[c] - (d * L.x[c] - off);
+}
+
+// One Gauss-Seidel update of a coarse cell; `xc` fuses the prolongation of the
+// next level's correction into the neighbour reads.
+template <class T>
+__device__ __forceinline__ void coarse_gs_cell(const DevLevel<T>& L, int i, int j, int k, int first,
+                                               const T* __restrict__ xc, const Geo& gc, T cscale) {
+  const Geo& g = L.g;
   const long long c = (long long)i + j * g.sy + k * g.sz;
   const T d = L.diag[c];
   if (!(d > T(0))) {
@@ -666,7 +658,7 @@ __global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int 
           if (xc) {
             int qn[3] = {i, j, k};
             qn[a] = q[a] < n - 1 ? q[a] + 1 : 0;
-            xn += xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+            xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
           }
           value += w * xn;
         }
@@ -679,7 +671,7 @@ __global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int 
           if (xc) {
             int qn[3] = {i, j, k};
             qn[a] = q[a] > 0 ? q[a] - 1 : n - 1;
-            xn += xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+            xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
           }
           value += w * xn;
         }
@@ -689,40 +681,129 @@ __global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int 
   L.x[c] = value / d;
 }
 
+// Cell (i, j, k) of colour `color` for the u-th thread of a half-sweep; false past the end.
+__device__ __forceinline__ bool color_cell(const Geo& g, unsigned u, int color, int& i, int& j, int& k) {
+  const unsigned hx = (unsigned)(g.n[0] + 1) >> 1;
+  const unsigned row = u / hx;
+  if (row >= (unsigned)g.n[1] * (unsigned)g.n[2]) return false;
+  const int i2 = (int)(u - row * hx);
+  const unsigned kk = row / (unsigned)g.n[1];
+  j = (int)(row - kk * (unsigned)g.n[1]);
+  k = (int)kk;
+  i = 2 * i2 + ((j + k + color) & 1);
+  return i < g.n[0];
+}
+
+// Restriction P^T r: eight lanes per coarse cell each form one fine residual of
+// the 2x2x2 bin, and three lane exchanges sum them.
+template <class T, class TB>
+__global__ void __launch_bounds__(BS) k_fine_restrict(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b,
+                                                      const T* __restrict__ x, Geo gc, T* __restrict__ bc,
+                                                      const PcgState* st) {
+  if (st && st->status != ST_RUNNING) return;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  const unsigned C = t >> 3;
+  const int sub = (int)(t & 7);
+  T r = T(0);
+  if (C < (unsigned)gc.N) {
+    int Q[3];
+    coords(gc, C, Q);
+    const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
+    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) r = fine_residual<T, TB>(g, codes, b, x, q);
+  }
+  r += __shfl_xor(r, 1, 64);
+  r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 4, 64);
+  if (C < (unsigned)gc.N && sub == 0) bc[C] = r;
+}
+
 template <class T>
 __global__ void __launch_bounds__(BS) k_coarse_restrict(DevLevel<T> L, Geo gc, T* __restrict__ bc, const PcgState* st) {
   if (st && st->status != ST_RUNNING) return;
   const Geo& g = L.g;
-  long long C = (long long)blockIdx.x * BS + threadIdx.x;
-  if (C >= gc.N) return;
-  int Q[3];
-  coords(gc, C, Q);
-  T sum = T(0);
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx) {
-        const int q[3] = {2 * Q[0] + dx, 2 * Q[1] + dy, 2 * Q[2] + dz};
-        if (q[0] >= g.n[0] || q[1] >= g.n[1] || q[2] >= g.n[2]) continue;
-        const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
-        const T d = L.diag[c];
-        if (!(d > T(0))) continue;
-        T off = T(0);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const int n = g.n[a];
-          if (q[a] < n - 1 || g.per[a]) {
-            T w = L.wp[a][c];
-            if (w != T(0)) off += w * L.x[hi_nb(g, c, a, q[a])];
-          }
-          if (q[a] > 0 || g.per[a]) {
-            long long nb = lo_nb(g, c, a, q[a]);
-            T w = L.wp[a][nb];
-            if (w != T(0)) off += w * L.x[nb];
-          }
-        }
-        sum += L.b[c] - (d * L.x[c] - off);
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  const unsigned C = t >> 3;
+  const int sub = (int)(t & 7);
+  T r = T(0);
+  if (C < (unsigned)gc.N) {
+    int Q[3];
+    coords(gc, C, Q);
+    const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
+    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) r = coarse_residual<T>(L, q);
+  }
+  r += __shfl_xor(r, 1, 64);
+  r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 4, 64);
+  if (C < (unsigned)gc.N && sub == 0) bc[C] = r;
+}
+
+// Coarse-level red-black half-sweep with explicit per-face weights.
+template <class T>
+__global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int first, const T* __restrict__ xc, Geo gc,
+                                                  const PcgState* st, T cscale) {
+  if (st && st->status != ST_RUNNING) return;
+  int i, j, k;
+  if (!color_cell(L.g, blockIdx.x * BS + threadIdx.x, color, i, j, k)) return;
+  coarse_gs_cell<T>(L, i, j, k, first, xc, gc, cscale);
+}
+
+// The small coarse levels of a V-cycle in one workgroup: smoothing,
+// restriction, the dense coarsest solve and the corrections, separated by
+// workgroup barriers instead of kernel launches.
+constexpr int TB_THREADS = 1024;
+template <class T>
+__global__ void __launch_bounds__(TB_THREADS)
+    k_tail(const DevLevel<T>* __restrict__ lv, int first, int nlev, int direct_n, const double* __restrict__ G,
+           const PcgState* st, T cscale) {
+  if (st && !block_running(st)) return;
+  const unsigned tid = threadIdx.x;
+  auto sweep = [&](int l, int color, int first_sweep, const T* xc) {
+    const DevLevel<T> L = lv[l];
+    Geo gc;
+    if (xc) gc = lv[l + 1].g;
+    const unsigned n = (unsigned)((L.g.n[0] + 1) >> 1) * (unsigned)L.g.n[1] * (unsigned)L.g.n[2];
+    for (unsigned u = tid; u < n; u += TB_THREADS) {
+      int i, j, k;
+      if (color_cell(L.g, u, color, i, j, k)) coarse_gs_cell<T>(L, i, j, k, first_sweep, xc, gc, cscale);
+    }
+    __syncthreads();
+  };
+  for (int l = first; l + 1 < nlev; ++l) {
+    sweep(l, 0, 1, nullptr);
+    sweep(l, 1, 0, nullptr);
+    sweep(l, 0, 0, nullptr);
+    sweep(l, 1, 0, nullptr);
+    const DevLevel<T> L = lv[l];
+    const DevLevel<T> N = lv[l + 1];
+    for (unsigned C = tid; C < (unsigned)N.g.N; C += TB_THREADS) {
+      int Q[3];
+      coords(N.g, C, Q);
+      T sum = T(0);
+      for (int sub = 0; sub < 8; ++sub) {
+        const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
+        if (q[0] < L.g.n[0] && q[1] < L.g.n[1] && q[2] < L.g.n[2]) sum += coarse_residual<T>(L, q);
       }
-  bc[C] = sum;
+      N.b[C] = sum;
+    }
+    __syncthreads();
+  }
+  {
+    const DevLevel<T> D = lv[nlev - 1];
+    const int wave = (int)(tid >> 6), lane = (int)(tid & 63);
+    for (int row = wave; row < direct_n; row += TB_THREADS / 64) {
+      double acc = 0.0;
+      for (int j = lane; j < direct_n; j += 64) acc += G[(long long)row * direct_n + j] * (double)D.b[j];
+      acc = wave_sum(acc);
+      if (lane == 0) D.x[row] = (T)acc;
+    }
+    __syncthreads();
+  }
+  for (int l = nlev - 2; l >= first; --l) {
+    sweep(l, 1, 0, lv[l + 1].x);
+    sweep(l, 0, 0, nullptr);
+    sweep(l, 1, 0, nullptr);
+    sweep(l, 0, 0, nullptr);
+  }
 }
 
 // Coarsest level: x = G b with the dense symmetric pseudo-inverse (FP64), one wave per row.
@@ -819,45 +900,42 @@ static inline unsigned blocks_for(long long n) { return (unsigned)((n + BS - 1) 
 void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_apply, dim3(blocks_for(g.N)), dim3(BS), 0, s, g, codes, x, y);
 }
-void launch_pcg_op(const Geo& g, int nb, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
+void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, PcgState* st, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_op, dim3(nb), dim3(BS), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+  hipLaunchKernelGGL(k_pcg_op, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
 }
-void launch_pcg_axpy(long long N, int nb, const double* p, const double* ap, double* x, double* r, PcgState* st,
-                     double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_axpy, dim3(nb), dim3(BS), 0, s, N, p, ap, x, r, st, part);
+void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_axpy, dim3(RP), dim3(RB), 0, s, N, p, ap, x, r, st, part);
 }
-void launch_pcg_true(const Geo& g, int nb, const uint8_t* codes, const double* x, const double* bc, PcgState* st,
-                     double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_true, dim3(nb), dim3(BS), 0, s, g, codes, x, bc, st, part);
+void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_true, dim3(RP), dim3(RB), 0, s, g, codes, x, bc, st, part);
+  hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(RB), 0, s, st, (const double*)part);
 }
-void launch_pcg_rz(long long N, int nb, const uint8_t* codes, const double* r, const double* z, const double* mean,
-                   const int* cidx, PcgState* st, double* part, int init, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_rz, dim3(nb), dim3(BS), 0, s, N, codes, r, z, mean, cidx, st, part, init);
+void launch_pcg_rz(long long N, const uint8_t* codes, const double* r, const double* z, const double* mean,
+                   const int* cidx, PcgState* st, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_rz, dim3(RP), dim3(RB), 0, s, N, codes, r, z, mean, cidx, st, part);
 }
-void launch_pcg_init(long long N, int nb, const uint8_t* codes, const double* b, const double* mean, const int* cidx,
-                     double* r, double* bc, double* x, PcgState* st, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(BS), 0, s, N, codes, b, mean, cidx, r, bc, x, (const double*)nullptr, st,
-                     part);
-}
-void launch_warm_residual(const Geo& g, const uint8_t* codes, const double* x0, const double* bc, double* x, double* r,
-                          PcgState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_warm_residual, dim3(blocks_for(g.N)), dim3(BS), 0, s, g, codes, x0, bc, x, r, st);
+void launch_pcg_init(long long N, const uint8_t* codes, const double* b, const double* mean, const int* cidx, double* r,
+                     double* bc, double* x, PcgState* st, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_init, dim3(RP), dim3(RB), 0, s, N, codes, b, mean, cidx, r, bc, x, st, part);
+  hipLaunchKernelGGL(k_init_fin, dim3(1), dim3(RB), 0, s, st, (const double*)part);
 }
 void launch_state_init(PcgState* st, double tol, long long max_it, hipStream_t s) {
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, max_it);
 }
-void launch_csum1(long long N, int nb, const uint8_t* codes, const double* src, const double* scale, int e031,
-                  double inv_count, double* mean, double* sums, PcgState* st, double* part, unsigned int* ticket,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(k_csum1, dim3(nb), dim3(BS), 0, s, N, codes, src, scale, e031, inv_count, mean, sums, st, part,
-                     ticket);
-}
-void launch_csumN(int n_chunks, const int* members, const int* chunk_begin, const int* comp_first_chunk, int n_comp,
-                  const double* inv_count, const double* src, const double* scale, int e031, double* mean, double* sums,
-                  PcgState* st, double* part, unsigned int* ticket, hipStream_t s) {
-  hipLaunchKernelGGL(k_csumN, dim3(n_chunks), dim3(BS), 0, s, members, chunk_begin, (const int*)nullptr, comp_first_chunk,
-                     n_comp, inv_count, src, scale, e031, mean, sums, st, part, ticket);
+void launch_gauge(long long N, const uint8_t* codes, int n_comp, int n_chunks, const int* members, const int* chunk_begin,
+                  const int* comp_first_chunk, const double* inv_count, double inv_count1, const double* src,
+                  const double* scale, int e031, double* mean, PcgState* st, double* part, hipStream_t s) {
+  if (n_comp == 0) return;
+  if (n_comp == 1)
+    hipLaunchKernelGGL(k_csum1, dim3(RP), dim3(RB), 0, s, N, codes, src, scale, e031, (const PcgState*)st, part);
+  else
+    hipLaunchKernelGGL(k_csumN, dim3(n_chunks), dim3(RB), 0, s, members, chunk_begin, src, scale, e031, (const PcgState*)st,
+                       part);
+  hipLaunchKernelGGL(k_csum_fin, dim3(1), dim3(RB), 0, s, n_comp, comp_first_chunk, inv_count, inv_count1, e031, mean, st,
+                     (const double*)part);
 }
 void launch_center(long long N, const uint8_t* codes, const double* src, const double* mean, const int* cidx, double* dst,
                    hipStream_t s) {
@@ -910,55 +988,50 @@ static unsigned color_blocks(const Geo& g) {
 }
 
 template <class T, class TB>
-void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv, int direct_n,
-                   const double* G, double* out64, const PcgState* st, hipStream_t s) {
-  // Level 0 is the fine grid; lv[0..nlev) are the coarse levels, lv[nlev-1] solved directly.
+void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv,
+                   const DevLevel<T>* dlv, int tail_first, int direct_n, const double* G, double* out64,
+                   const PcgState* st, double omega, hipStream_t s) {
+  const T cs = (T)omega;
+  // Level 0 is the fine grid; lv[0..nlev) are the coarse levels, lv[nlev-1] solved
+  // directly; levels from tail_first on run inside one workgroup (k_tail).
   const unsigned fb = color_blocks<T>(g0);
   if (nlev == 0) {
-    // No coarse level: the fine grid itself is tiny and solved directly (T == double here).
+    // No coarse level: the fine grid itself is tiny and solved directly.
     hipLaunchKernelGGL((k_direct<TB, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, b0, x0, out64, st);
     return;
   }
   const Geo& g1 = lv[0].g;
-  // pre-smoothing on the fine level
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 0, 1, (const T*)nullptr, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 1, 0, (const T*)nullptr, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 0, 0, (const T*)nullptr, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 1, 0, (const T*)nullptr, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0, g1,
-                     lv[0].b, st);
-  for (int l = 0; l + 1 < nlev; ++l) {
+  const T* none = nullptr;
+  for (int pass = 0; pass < 4; ++pass)
+    hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, none,
+                       g1, (double*)nullptr, 0, st, cs);
+  hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(8 * g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0,
+                     g1, lv[0].b, st);
+  const int stop = tail_first < nlev ? tail_first : nlev - 1;
+  for (int l = 0; l < stop; ++l) {
     const unsigned cb = color_blocks<T>(lv[l].g);
     const Geo& gn = lv[l + 1].g;
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 0, 1, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 1, 0, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 0, 0, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 1, 0, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(blocks_for(gn.N)), dim3(BS), 0, s, lv[l], gn, lv[l + 1].b, st);
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], pass & 1, pass == 0 ? 1 : 0, none, gn, st, cs);
+    hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(blocks_for(8 * gn.N)), dim3(BS), 0, s, lv[l], gn, lv[l + 1].b, st);
   }
-  hipLaunchKernelGGL((k_direct<T, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, (const T*)lv[nlev - 1].b,
-                     lv[nlev - 1].x, (double*)nullptr, st);
-  for (int l = nlev - 2; l >= 0; --l) {
+  if (tail_first < nlev) {
+    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(TB_THREADS), 0, s, dlv, tail_first, nlev, direct_n, G, st, cs);
+  } else {
+    hipLaunchKernelGGL((k_direct<T, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G,
+                       (const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr, st);
+  }
+  for (int l = stop - 1; l >= 0; --l) {
     const unsigned cb = color_blocks<T>(lv[l].g);
     const Geo& gn = lv[l + 1].g;
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 1, 0, (const T*)lv[l + 1].x, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 0, 0, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 1, 0, (const T*)nullptr, gn, st);
-    hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], 0, 0, (const T*)nullptr, gn, st);
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], (pass + 1) & 1, 0,
+                         pass == 0 ? (const T*)lv[l + 1].x : none, gn, st, cs);
   }
   // post-smoothing on the fine level; in mixed precision the last two half-sweeps also write z in FP64
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 1, 0, (const T*)lv[0].x, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 0, 0, (const T*)nullptr, g1,
-                     (double*)nullptr, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 1, 0, (const T*)nullptr, g1,
-                     out64, 0, st);
-  hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, 0, 0, (const T*)nullptr, g1, out64,
-                     0, st);
+  for (int pass = 0; pass < 4; ++pass)
+    hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
+                       pass == 0 ? (const T*)lv[0].x : none, g1, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs);
 }
 
 template <class T, class TB>
@@ -969,10 +1042,12 @@ void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, in
 template void launch_fine_gs<double, double>(const Geo&, const uint8_t*, const double*, double*, int, const Geo&, hipStream_t);
 template void launch_fine_gs<float, double>(const Geo&, const uint8_t*, const double*, float*, int, const Geo&, hipStream_t);
 
-template void launch_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, int, DevLevel<double>*, int,
-                                            const double*, double*, const PcgState*, hipStream_t);
-template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, int, DevLevel<float>*, int,
-                                           const double*, double*, const PcgState*, hipStream_t);
+template void launch_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, int, DevLevel<double>*,
+                                            const DevLevel<double>*, int, int, const double*, double*, const PcgState*,
+                                            double, hipStream_t);
+template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, int, DevLevel<float>*,
+                                           const DevLevel<float>*, int, int, const double*, double*, const PcgState*,
+                                           double, hipStream_t);
 
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,
                            double* terms, hipStream_t s) {

@@ -7,34 +7,34 @@
 namespace cfdgpu {
 
 void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y, hipStream_t s);
-void launch_pcg_op(const Geo& g, int nb, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
+void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, PcgState* st, double* part, hipStream_t s);
-void launch_pcg_axpy(long long N, int nb, const double* p, const double* ap, double* x, double* r, PcgState* st,
-                     double* part, hipStream_t s);
-void launch_pcg_true(const Geo& g, int nb, const uint8_t* codes, const double* x, const double* bc, PcgState* st,
-                     double* part, hipStream_t s);
-void launch_pcg_rz(long long N, int nb, const uint8_t* codes, const double* r, const double* z, const double* mean,
-                   const int* cidx, PcgState* st, double* part, int init, hipStream_t s);
-void launch_pcg_init(long long N, int nb, const uint8_t* codes, const double* b, const double* mean, const int* cidx,
-                     double* r, double* bc, double* x, PcgState* st, double* part, hipStream_t s);
-void launch_warm_residual(const Geo& g, const uint8_t* codes, const double* x0, const double* bc, double* x, double* r,
-                          PcgState* st, hipStream_t s);
+void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
+                     hipStream_t s);
+// the recomputed-residual kernel and the one-block decision kernel of an iteration
+void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
+                     hipStream_t s);
+void launch_pcg_rz(long long N, const uint8_t* codes, const double* r, const double* z, const double* mean,
+                   const int* cidx, PcgState* st, double* part, hipStream_t s);
+void launch_pcg_init(long long N, const uint8_t* codes, const double* b, const double* mean, const int* cidx, double* r,
+                     double* bc, double* x, PcgState* st, double* part, hipStream_t s);
 void launch_state_init(PcgState* st, double tol, long long max_it, hipStream_t s);
-void launch_csum1(long long N, int nb, const uint8_t* codes, const double* src, const double* scale, int e031,
-                  double inv_count, double* mean, double* sums, PcgState* st, double* part, unsigned int* ticket,
-                  hipStream_t s);
-void launch_csumN(int n_chunks, const int* members, const int* chunk_begin, const int* comp_first_chunk, int n_comp,
-                  const double* inv_count, const double* src, const double* scale, int e031, double* mean, double* sums,
-                  PcgState* st, double* part, unsigned int* ticket, hipStream_t s);
+// component means of src (and the CFD-E031 test when e031 is set) into mean
+void launch_gauge(long long N, const uint8_t* codes, int n_comp, int n_chunks, const int* members, const int* chunk_begin,
+                  const int* comp_first_chunk, const double* inv_count, double inv_count1, const double* src,
+                  const double* scale, int e031, double* mean, PcgState* st, double* part, hipStream_t s);
 void launch_center(long long N, const uint8_t* codes, const double* src, const double* mean, const int* cidx, double* dst,
                    hipStream_t s);
 void launch_fill(long long N, double* dst, double v, hipStream_t s);
 void launch_add(long long N, double* p, const double* q, hipStream_t s);
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,
                 hipStream_t s);
+// One V-cycle: fine level from the codes, coarse levels lv[0..nlev) (device copy dlv),
+// levels from tail_first on inside one workgroup.
 template <class T, class TB>
-void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv, int direct_n,
-                   const double* G, double* out64, const PcgState* st, hipStream_t s);
+void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv,
+                   const DevLevel<T>* dlv, int tail_first, int direct_n, const double* G, double* out64,
+                   const PcgState* st, double omega, hipStream_t s);
 template <class T, class TB>
 void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s);
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,
