@@ -144,6 +144,17 @@ def lib():
             "cfd_last_error": (ctypes.c_char_p, []),
             "cfd_case_topology": (I, [P, P, P, P, P, I64, P]),
             "cfd_profile_kernel": (I, [P, I, I, P, P]),
+            "cfd_server_create": (I, [I, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_server_destroy": (None, [P]),
+            "cfd_server_admit": (I, [P, P, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_server_release": (I, [P, P]),
+            "cfd_server_usage": (I, [P, P, P]),
+            "cfd_grid_footprint": (ctypes.c_size_t, [P]),
+            "cfd_server_submit": (I, [P, P, P, P, D, I64, I, P, ctypes.POINTER(ctypes.c_uint64)]),
+            "cfd_server_progress": (I, [P]),
+            "cfd_server_wait": (I, [P, ctypes.c_uint64]),
+            "cfd_server_drain": (I, [P]),
+            "cfd_server_result": (I, [P, ctypes.c_uint64, P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -252,6 +263,8 @@ class Grid:
         self.max_iterations = case.max_iterations
 
     def close(self):
+        if getattr(self, "owner", None) is not None:
+            return  # released by its server
         if getattr(self, "h", None) is not None and self.h.value:
             lib().cfd_grid_destroy(self.h)
             self.h = None
@@ -379,6 +392,85 @@ class Grid:
         _check(lib().cfd_project(self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn, None,
                                  ctypes.byref(info)))
         return u, p, info.as_dict()
+
+
+class Server:
+    """Continuous batching of pressure solves on resident grids (cfd_server_*)."""
+
+    def __init__(self, max_active: int = 8, hbm_budget_bytes: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().cfd_server_create(int(max_active), int(hbm_budget_bytes), ctypes.byref(h)))
+        self.h = h
+        self.grids = []
+
+    def admit(self, case: Case) -> "Grid":
+        """Grid::build under the server's HBM budget (CfdError ENOMEM when it does not fit)."""
+        c, keep = case.to_c()
+        gh = ctypes.c_void_p()
+        _check(lib().cfd_server_admit(self.h, ctypes.byref(c), ctypes.byref(gh)))
+        g = Grid.__new__(Grid)
+        g.case = case
+        g.h = gh
+        g.owner = self
+        counts = np.zeros(8, np.int64)
+        _check(lib().cfd_grid_counts(g.h, _ptr(counts)))
+        g.count, g.face_count, g.fluid_count, g.n_components, g.n_anchored = (int(v) for v in counts[:5])
+        g.tolerance = case.tolerance
+        g.max_iterations = case.max_iterations
+        self.grids.append(g)
+        return g
+
+    def release(self, g: "Grid"):
+        _check(lib().cfd_server_release(self.h, g.h))
+        g.h = None
+        self.grids.remove(g)
+
+    def usage(self):
+        used = ctypes.c_size_t(0)
+        budget = ctypes.c_size_t(0)
+        _check(lib().cfd_server_usage(self.h, ctypes.byref(used), ctypes.byref(budget)))
+        return used.value, budget.value
+
+    def submit(self, g: "Grid", d_b: "DeviceArray", d_x: "DeviceArray", tol: float, max_iterations: int = 2000,
+               precision: int = PRECISION_MIXED, d_scale: "DeviceArray" = None) -> int:
+        t = ctypes.c_uint64(0)
+        _check(lib().cfd_server_submit(self.h, g.h, d_b.ptr, d_scale.ptr if d_scale is not None else None, float(tol),
+                                       int(max_iterations), int(precision), d_x.ptr, ctypes.byref(t)))
+        return t.value
+
+    def progress(self) -> int:
+        return lib().cfd_server_progress(self.h)
+
+    def wait(self, ticket: int):
+        _check(lib().cfd_server_wait(self.h, ctypes.c_uint64(ticket)))
+
+    def drain(self):
+        _check(lib().cfd_server_drain(self.h))
+
+    def result(self, ticket: int):
+        """(info dict, latency s, queueing s); raises CfdError if the solve failed."""
+        info = LinearInfo()
+        lat = ctypes.c_double(0.0)
+        q = ctypes.c_double(0.0)
+        _check(lib().cfd_server_result(self.h, ctypes.c_uint64(ticket), ctypes.byref(info), ctypes.byref(lat),
+                                       ctypes.byref(q)))
+        return info.as_dict(), lat.value, q.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().cfd_server_drain(self.h)
+            for g in list(self.grids):
+                lib().cfd_server_release(self.h, g.h)
+                g.h = None
+            self.grids = []
+            lib().cfd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def synchronize():

@@ -131,10 +131,12 @@ def face_kinds(case: Case, fluid: np.ndarray):
             kind[boundary & (tt != 0)] = SLIP_FACE
         elif bnd == OPEN:
             tt = np.broadcast_to(t.reshape([-1 if d == ax else 1 for d in range(3)]), shape)
-            normal = np.where(tt == 0, -1.0, 1.0)
-            v = np.vectorize(lambda z: case.flow_vector(z)[a])(zc) * normal
-            kb = np.where(v < -1e-12, INLET, np.where(v > 1e-12, OUTLET, SLIP_FACE))
-            kind[boundary] = kb[boundary]
+            normal = np.where(tt == 0, -1.0, 1.0)[boundary]
+            zb = np.ascontiguousarray(zc)[boundary]
+            uz, inv = np.unique(zb, return_inverse=True)
+            va = np.array([case.flow_vector(z)[a] for z in uz])[inv]
+            v = va * normal
+            kind[boundary] = np.where(v < -1e-12, INLET, np.where(v > 1e-12, OUTLET, SLIP_FACE))
         kinds.append(kind.reshape(-1))
         zs.append(np.ascontiguousarray(zc).reshape(-1))
         del idx
@@ -149,8 +151,9 @@ def enforce_velocity(case: Case, kinds, zc, offsets, u):
     for a in range(3):
         end = offsets[a + 1] if a < 2 else u.size
         sel = np.nonzero(kinds[offsets[a] : end] == INLET)[0] + offsets[a]
-        for f in sel:
-            u[f] = case.flow_vector(zc[f])[a]
+        if sel.size:
+            uz, inv = np.unique(zc[sel], return_inverse=True)
+            u[sel] = np.array([case.flow_vector(z)[a] for z in uz])[inv]
     return u
 
 
@@ -236,10 +239,10 @@ def _log_law_velocity(case: Case, kinds, zc, offs, rng, noise=0.05):
     for a in range(3):
         end = offs[a + 1] if a < 2 else kinds.size
         z = zc[offs[a] : end]
-        uz = np.array([case.flow_vector(v) for v in np.unique(z)])
-        lookup = {v: uz[k] for k, v in enumerate(np.unique(z))}
-        base = np.array([lookup[v][a] for v in z])
-        mag = np.array([math.hypot(lookup[v][0], lookup[v][1]) for v in z])
+        uzs, inv = np.unique(z, return_inverse=True)
+        vec = np.array([case.flow_vector(v) for v in uzs])
+        base = vec[inv, a]
+        mag = np.hypot(vec[:, 0], vec[:, 1])[inv]
         u[offs[a] : end] = base + noise * mag * rng.standard_normal(z.size)
     return enforce_velocity(case, kinds, zc, offs, u)
 

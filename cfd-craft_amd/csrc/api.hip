@@ -7,6 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <deque>
+#include <thread>
+#include <unordered_map>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -47,6 +51,9 @@ constexpr int64_t kTailCells = 0;
 constexpr double kOmega = 1.9;
 constexpr int kMaxReduceBlocks = 2048;
 
+// Device bytes allocated while a grid is being set up (its HBM footprint).
+thread_local size_t* tl_bytes = nullptr;
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -63,6 +70,7 @@ struct DevBuf {
       set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
       return CFD_ENOMEM;
     }
+    if (tl_bytes) *tl_bytes += count * sizeof(T);
     return 0;
   }
   int upload(const T* h, size_t count) {
@@ -81,6 +89,7 @@ struct DevHierarchy {
   std::vector<std::unique_ptr<DevBuf<T>>> store;
   T* x0 = nullptr;  // FP32 fine-level iterate (mixed precision only)
   DevBuf<T> x0buf;
+  DevBuf<T> xpre;  // fine-level iterate after pre-smoothing (fused cycle)
 };
 
 }  // namespace cfdgpu
@@ -119,9 +128,29 @@ struct cfd_grid {
   DevBuf<double> G;
   // captured iteration segment, per precision
   hipGraphExec_t seg[2] = {nullptr, nullptr};
+  // resident solution of the previous solve (warm start) and the HBM footprint
+  DevBuf<double> warm;
+  bool warm_valid = false;
+  size_t footprint = 0;
+  // the solve in flight (asynchronous state machine driven by job_poll)
+  struct Job {
+    int phase = 0;  // 0 idle, 1 initial kernels, 2 segment, 3 final kernels, 4 done
+    const double* b = nullptr;
+    const double* scale = nullptr;
+    double* x = nullptr;
+    double tol = 0.0;
+    int64_t max = 0;
+    int rc = 0;
+    std::string error;
+    cfd_linear_info info{};
+  } job;
+  hipEvent_t evs = nullptr;  // recorded after each state read-back
   // CFD_GPU_GRAPHS=0 launches the segments kernel by kernel (for profilers that do not follow graphs).
   bool use_graphs = true;
   int64_t coarse_max = kCoarseMax, tail_cells = kTailCells;
+  // CFD_MG_FUSED: 0 one kernel per half-sweep on every level; 1 coarse levels fused
+  // (one kernel per level and phase), the default; 2 every level fused.
+  int fused = 1;
   double omega = kOmega;
   ~cfd_grid() {
     for (auto& g : seg)
@@ -129,6 +158,7 @@ struct cfd_grid {
     if (h_st) (void)hipHostFree(h_st);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (evs) (void)hipEventDestroy(evs);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -196,8 +226,14 @@ int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
     d.diag = buf(L.diag);
     d.x = zeros((size_t)L.count);
     d.b = zeros((size_t)L.count);
-    if (!d.diag || !d.x || !d.b) return CFD_ENOMEM;
+    const bool coarsest = &L == &H.coarse.back();
+    d.xo = coarsest ? d.x : zeros((size_t)L.count);
+    if (!d.diag || !d.x || !d.b || !d.xo) return CFD_ENOMEM;
     dh.levels.push_back(d);
+  }
+  if (!H.coarse.empty()) {
+    int rc = dh.xpre.alloc((size_t)g->host.count);
+    if (rc) return rc;
   }
   dh.tail_first = (int)dh.levels.size();
   for (size_t l = 0; l < dh.levels.size(); ++l)
@@ -212,18 +248,29 @@ int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
   return 0;
 }
 
+int setup_device_inner(cfd_grid* g);
 int setup_device(cfd_grid* g) {
+  size_t bytes = 0;
+  tl_bytes = &bytes;
+  int rc = setup_device_inner(g);
+  tl_bytes = nullptr;
+  g->footprint = bytes;
+  return rc;
+}
+int setup_device_inner(cfd_grid* g) {
   HostGrid& h = g->host;
   HIP_TRY(hipGetDevice(&g->device));
   HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&g->ev0));
   HIP_TRY(hipEventCreate(&g->ev1));
+  HIP_TRY(hipEventCreateWithFlags(&g->evs, hipEventDisableTiming));
   HIP_TRY(hipHostMalloc((void**)&g->h_st, sizeof(PcgState), hipHostMallocDefault));
   fill_geo(g);
   if (const char* env = std::getenv("CFD_GPU_GRAPHS")) g->use_graphs = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
+  if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   size_t N = (size_t)h.count;
   int rc;
   if ((rc = g->codes.upload(h.codes.data(), N))) return rc;
@@ -251,7 +298,8 @@ int setup_device(cfd_grid* g) {
     if ((rc = g->inv_count.upload(inv.data(), inv.size()))) return rc;
   }
   size_t nm = (size_t)std::max(3, g->n_closed);
-  for (DevBuf<double>* v : {&g->x, &g->r, &g->bc, &g->z, &g->p0, &g->p1, &g->ap, &g->b, &g->scale, &g->xo, &g->phi})
+  for (DevBuf<double>* v :
+       {&g->x, &g->r, &g->bc, &g->z, &g->p0, &g->p1, &g->ap, &g->b, &g->scale, &g->xo, &g->phi, &g->warm})
     if ((rc = v->alloc(N))) return rc;
   for (DevBuf<double>* v : {&g->mean_b, &g->mean_z, &g->mean_x})
     if ((rc = v->alloc(nm))) return rc;
@@ -269,6 +317,17 @@ int setup_device(cfd_grid* g) {
     rc = build_hierarchy(h, g->hier, g->coarse_max);
     if (rc) return rc;
     g->has_hier = true;
+    // A periodic axis of odd length >= 3 on some level puts two cells of one colour
+    // next to each other across the seam; only the fused kernels (which update a
+    // colour from the values before the half-sweep) handle that deterministically.
+    auto odd = [&](const int64_t* n) {
+      for (int a = 0; a < 3; ++a)
+        if (h.periodic[a] && n[a] >= 3 && (n[a] & 1)) return true;
+      return false;
+    };
+    bool odd_periodic = odd(h.n);
+    for (const HostLevel& L : g->hier.coarse) odd_periodic |= odd(L.n);
+    if (odd_periodic) g->fused = 2;
     if ((rc = upload_hierarchy(g, g->h64))) return rc;
     if ((rc = upload_hierarchy(g, g->h32))) return rc;
     if ((rc = g->h32.x0buf.alloc(N))) return rc;
@@ -291,6 +350,15 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
   }
   const int nlev = (int)g->h64.levels.size();
   const int dn = (int)g->hier.direct_n;
+  if (g->fused) {
+    if (g->opt.precision == CFD_PRECISION_MIXED)
+      launch_vcycle_fused<float, double>(g->geo, g->codes.p, r, g->h32.x0, g->h32.xpre.p, nlev, g->h32.levels.data(), dn,
+                                         g->G.p, z, st, g->omega, g->fused == 2, g->stream);
+    else
+      launch_vcycle_fused<double, double>(g->geo, g->codes.p, r, z, g->h64.xpre.p, nlev, g->h64.levels.data(), dn, g->G.p,
+                                          nullptr, st, g->omega, g->fused == 2, g->stream);
+    return;
+  }
   if (g->opt.precision == CFD_PRECISION_MIXED) {
     launch_vcycle<float, double>(g->geo, g->codes.p, r, g->h32.x0, nlev, g->h32.levels.data(), g->h32.dlevels.p,
                                  g->h32.tail_first, dn, g->G.p, z, st, g->omega, g->stream);
@@ -365,48 +433,95 @@ int read_state(cfd_grid* g) {
   return 0;
 }
 
-int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
-                 cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info) {
+// Asynchronous solve: job_begin enqueues the set-up kernels; job_poll advances
+// the job whenever its last read-back of the device state has landed, so one
+// host thread can drive the solves of many grids (each on its own stream).
+enum { JOB_PENDING = 0, JOB_DONE = 1 };
+
+int enqueue_state_readback(cfd_grid* g) {
+  HIP_TRY(hipMemcpyAsync(g->h_st, g->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipEventRecord(g->evs, g->stream));
+  return 0;
+}
+
+int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations, double* d_x) {
   if (!(tol > 0.0) || max_iterations <= 0) {
     set_error("pressure tolerance must be positive and the iteration limit nonzero (CFD-E008/E009)");
     return CFD_EINVAL;
   }
   HIP_TRY(hipSetDevice(g->device));
+  cfd_grid::Job& J = g->job;
+  J = cfd_grid::Job();
+  J.b = d_b;
+  J.scale = d_scale;
+  J.x = d_x;
+  J.tol = tol;
+  J.max = max_iterations;
   PcgState* st = g->st.p;
-  cfd_linear_info local;
-  std::memset(&local, 0, sizeof local);
   HIP_TRY(hipEventRecord(g->ev0, g->stream));
   launch_state_init(st, tol, max_iterations, g->stream);
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
   enqueue_gauge(g, d_b, d_scale, 1, g->mean_b.p, st);
   launch_pcg_init(g->geo.N, g->codes.p, d_b, g->mean_b.p, cidx, g->r.p, g->bc.p, g->x.p, st, g->part.p, g->stream);
+  if (g->opt.warm_start && g->warm_valid)
+    launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);
-  int rc = read_state(g);
+  int rc = enqueue_state_readback(g);
   if (rc) return rc;
-  hipGraphExec_t seg = nullptr;
-  while (g->h_st->status == ST_RUNNING) {
+  J.phase = 1;
+  return 0;
+}
+
+// Advances the job; `block` waits for the pending read-back. Returns JOB_PENDING,
+// JOB_DONE (result in g->job.rc / info) or an error code of the device runtime.
+int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
+  cfd_grid::Job& J = g->job;
+  if (J.phase == 4 || J.phase == 0) return JOB_DONE;
+  if (block) {
+    HIP_TRY(hipEventSynchronize(g->evs));
+  } else {
+    hipError_t q = hipEventQuery(g->evs);
+    if (q == hipErrorNotReady) return JOB_PENDING;
+    if (q != hipSuccess) {
+      set_error(std::string("HIP error: ") + hipGetErrorString(q));
+      return CFD_EDEVICE;
+    }
+  }
+  if (J.phase == 3) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    J.info.seconds = ms * 1e-3;
+    J.phase = 4;
+    return JOB_DONE;
+  }
+  const PcgState s = *g->h_st;
+  if (s.status == ST_RUNNING) {
     if (poll) {
       int p = poll(user);
-      if (p == 1) {
-        set_error("cancelled");
-        return CFD_CANCELLED;
-      }
-      if (p == 2) {
-        set_error("wall_time_limit");
-        return CFD_WALL_TIME;
+      if (p == 1 || p == 2) {
+        J.rc = p == 1 ? CFD_CANCELLED : CFD_WALL_TIME;
+        J.error = p == 1 ? "cancelled" : "wall_time_limit";
+        J.phase = 4;
+        return JOB_DONE;
       }
     }
     if (g->use_graphs) {
-      if (!seg && (rc = segment_graph(g, &seg))) return rc;
+      hipGraphExec_t seg = nullptr;
+      int rc = segment_graph(g, &seg);
+      if (rc) return rc;
       HIP_TRY(hipGraphLaunch(seg, g->stream));
     } else {
       for (int it = 0; it < kSegment; ++it) enqueue_iteration(g, it & 1);
     }
-    if ((rc = read_state(g))) return rc;
+    int rc = enqueue_state_readback(g);
+    if (rc) return rc;
+    J.phase = 2;
+    return JOB_PENDING;
   }
-  const PcgState s = *g->h_st;
+  cfd_linear_info& local = J.info;
+  std::memset(&local, 0, sizeof local);
   local.iterations = s.status == ST_ZERO ? 0 : s.it;
   local.relative_residual = s.status == ST_ZERO ? 0.0 : s.true_rel;
   local.recurrent_residual = s.rel;
@@ -414,31 +529,53 @@ int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double t
   local.operator_applications = local.iterations + s.true_checks;
   local.preconditioner_applications = s.status == ST_ZERO ? 0 : (s.status == ST_CONVERGED ? s.it : s.it + 1);
   if (g->has_hier) {
-    int64_t fine = g->host.count;
-    int64_t cells = fine;
+    int64_t cells = g->host.count;
     for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
     local.cycles = local.preconditioner_applications;
     local.smoothing_cell_updates = local.cycles * 4 * cells;
     local.residual_cell_updates = local.cycles * cells;
     local.coarse_solves = local.cycles;
   }
-  rc = finish_status(g, s, max_iterations);
+  int rc = finish_status(g, s, J.max);
   if (rc) {
-    if (info) *info = local;
-    return rc;
+    J.rc = rc;
+    J.error = g_error;
+    J.phase = 4;
+    return JOB_DONE;
   }
+  const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
   if (s.status == ST_ZERO) {
-    launch_fill(g->geo.N, d_x, 0.0, g->stream);
+    launch_fill(g->geo.N, J.x, 0.0, g->stream);
   } else {
     enqueue_gauge(g, g->x.p, nullptr, 0, g->mean_x.p, nullptr);
-    launch_center(g->geo.N, g->codes.p, g->x.p, g->mean_x.p, cidx, d_x, g->stream);
+    launch_center(g->geo.N, g->codes.p, g->x.p, g->mean_x.p, cidx, J.x, g->stream);
+  }
+  if (g->opt.warm_start) {
+    HIP_TRY(hipMemcpyAsync(g->warm.p, J.x, (size_t)g->geo.N * sizeof(double), hipMemcpyDeviceToDevice, g->stream));
+    g->warm_valid = true;
   }
   HIP_TRY(hipEventRecord(g->ev1, g->stream));
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  float ms = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-  local.seconds = ms * 1e-3;
-  if (info) *info = local;
+  HIP_TRY(hipEventRecord(g->evs, g->stream));
+  J.phase = 3;
+  return JOB_PENDING;
+}
+
+int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                 cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info) {
+  int rc = job_begin(g, d_b, d_scale, tol, max_iterations, d_x);
+  if (rc) return rc;
+  for (;;) {
+    rc = job_poll(g, true, poll, user);
+    if (rc == JOB_DONE) break;
+    if (rc != JOB_PENDING) return rc;
+  }
+  if (info) *info = g->job.info;
+  if (g->job.rc) {
+    // a cancelled job may still have kernels queued: let them finish before returning
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    set_error(g->job.error);
+    return g->job.rc;
+  }
   return CFD_OK;
 }
 
@@ -789,6 +926,229 @@ int cfd_copy_to_host(void* h_dst, const void* d_src, size_t bytes) {
 int cfd_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return CFD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- serving
+// Continuous batching of independent pressure solves: each request names a
+// resident grid; up to max_active requests run at once, each on its grid's own
+// stream, and a request that arrives while others are on the GPU starts as soon
+// as a slot and its grid are free. Grids are admitted against an HBM budget by
+// their measured footprint.
+namespace {
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+struct cfd_server {
+  struct Req {
+    cfd_grid* g = nullptr;
+    const double* b = nullptr;
+    const double* scale = nullptr;
+    double* x = nullptr;
+    double tol = 0.0;
+    int64_t max = 0;
+    int precision = 0;
+    double t_submit = 0.0, t_start = 0.0, t_end = 0.0;
+    int state = 0;  // 0 queued, 1 running, 2 done
+    int rc = 0;
+    std::string error;
+    cfd_linear_info info{};
+  };
+  int max_active = 8;
+  size_t budget = 0, used = 0;
+  uint64_t next = 1;
+  std::unordered_map<uint64_t, Req> reqs;
+  std::deque<uint64_t> queued;
+  std::vector<uint64_t> running;
+  std::unordered_map<cfd_grid*, int> busy;
+  std::unordered_map<cfd_grid*, size_t> grids;  // admitted grids and their footprints
+};
+
+namespace {
+// Drives every running job one step and starts queued requests whose grid is idle.
+int server_progress(cfd_server* s) {
+  for (size_t i = 0; i < s->running.size();) {
+    cfd_server::Req& R = s->reqs[s->running[i]];
+    int rc = job_poll(R.g, false, nullptr, nullptr);
+    if (rc == JOB_PENDING) {
+      ++i;
+      continue;
+    }
+    R.t_end = now_s();
+    R.state = 2;
+    if (rc == JOB_DONE) {
+      R.rc = R.g->job.rc;
+      R.error = R.g->job.error;
+      R.info = R.g->job.info;
+    } else {
+      R.rc = rc;
+      R.error = g_error;
+    }
+    s->busy.erase(R.g);
+    s->running[i] = s->running.back();
+    s->running.pop_back();
+  }
+  for (size_t i = 0; i < s->queued.size() && (int)s->running.size() < s->max_active;) {
+    const uint64_t id = s->queued[i];
+    cfd_server::Req& R = s->reqs[id];
+    if (s->busy.count(R.g)) {
+      ++i;
+      continue;
+    }
+    R.g->opt.precision = R.precision;
+    int rc = job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
+    R.t_start = now_s();
+    s->queued.erase(s->queued.begin() + (long)i);
+    if (rc) {
+      R.state = 2;
+      R.rc = rc;
+      R.error = g_error;
+      R.t_end = R.t_start;
+      continue;
+    }
+    R.state = 1;
+    s->busy[R.g] = 1;
+    s->running.push_back(id);
+  }
+  return (int)(s->running.size() + s->queued.size());
+}
+}  // namespace
+
+extern "C" {
+
+int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out) {
+  if (!out || max_active < 1) {
+    set_error("cfd_server_create: max_active must be positive");
+    return CFD_EINVAL;
+  }
+  cfd_server* s = new cfd_server();
+  s->max_active = max_active;
+  if (hbm_budget_bytes == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) hbm_budget_bytes = free_b / 10 * 9;
+  }
+  s->budget = hbm_budget_bytes;
+  *out = s;
+  return CFD_OK;
+}
+
+void cfd_server_destroy(cfd_server* s) {
+  if (!s) return;
+  while (server_progress(s) > 0) std::this_thread::yield();
+  delete s;
+}
+
+int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out) {
+  cfd_grid* g = nullptr;
+  int rc = cfd_grid_create(c, &g);
+  if (rc) return rc;
+  if (s->used + g->footprint > s->budget) {
+    char buf[200];
+    std::snprintf(buf, sizeof buf, "HBM admission refused: the grid needs %zu bytes, %zu of %zu are in use", g->footprint,
+                  s->used, s->budget);
+    cfd_grid_destroy(g);
+    set_error(buf);
+    return CFD_ENOMEM;
+  }
+  s->used += g->footprint;
+  s->grids[g] = g->footprint;
+  *out = g;
+  return CFD_OK;
+}
+
+int cfd_server_release(cfd_server* s, cfd_grid* g) {
+  auto it = s->grids.find(g);
+  if (it == s->grids.end()) {
+    set_error("cfd_server_release: the grid was not admitted by this server");
+    return CFD_EINVAL;
+  }
+  while (s->busy.count(g)) {
+    server_progress(s);
+    std::this_thread::yield();
+  }
+  for (const uint64_t id : s->queued)
+    if (s->reqs[id].g == g) {
+      set_error("cfd_server_release: requests on this grid are still queued");
+      return CFD_EINVAL;
+    }
+  s->used -= it->second;
+  s->grids.erase(it);
+  cfd_grid_destroy(g);
+  return CFD_OK;
+}
+
+int cfd_server_usage(const cfd_server* s, size_t* used, size_t* budget) {
+  if (used) *used = s->used;
+  if (budget) *budget = s->budget;
+  return CFD_OK;
+}
+
+size_t cfd_grid_footprint(const cfd_grid* g) { return g->footprint; }
+
+int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                      int precision, double* d_x, uint64_t* ticket) {
+  if (!(tol > 0.0) || max_iterations <= 0 || (precision != CFD_PRECISION_FP64 && precision != CFD_PRECISION_MIXED)) {
+    set_error("cfd_server_submit: tolerance, iteration limit or precision out of range");
+    return CFD_EINVAL;
+  }
+  if (precision == CFD_PRECISION_MIXED && !g->has_hier) precision = CFD_PRECISION_FP64;
+  const uint64_t id = s->next++;
+  cfd_server::Req& R = s->reqs[id];
+  R.g = g;
+  R.b = d_b;
+  R.scale = d_scale;
+  R.x = d_x;
+  R.tol = tol;
+  R.max = max_iterations;
+  R.precision = precision;
+  R.t_submit = now_s();
+  s->queued.push_back(id);
+  server_progress(s);
+  if (ticket) *ticket = id;
+  return CFD_OK;
+}
+
+int cfd_server_progress(cfd_server* s) { return server_progress(s); }
+
+int cfd_server_wait(cfd_server* s, uint64_t ticket) {
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    set_error("cfd_server_wait: unknown ticket");
+    return CFD_EINVAL;
+  }
+  while (it->second.state != 2) {
+    server_progress(s);
+    if (it->second.state != 2) std::this_thread::yield();
+  }
+  return CFD_OK;
+}
+
+int cfd_server_drain(cfd_server* s) {
+  while (server_progress(s) > 0) std::this_thread::yield();
+  return CFD_OK;
+}
+
+int cfd_server_result(cfd_server* s, uint64_t ticket, cfd_linear_info* info, double* latency_s, double* queued_s) {
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    set_error("cfd_server_result: unknown ticket");
+    return CFD_EINVAL;
+  }
+  cfd_server::Req& R = it->second;
+  if (R.state != 2) {
+    set_error("cfd_server_result: the request has not finished");
+    return CFD_EINVAL;
+  }
+  if (info) *info = R.info;
+  if (latency_s) *latency_s = R.t_end - R.t_submit;
+  if (queued_s) *queued_s = R.t_start - R.t_submit;
+  const int rc = R.rc;
+  if (rc) set_error(R.error);
+  s->reqs.erase(it);
+  return rc;
 }
 
 }  // extern "C"

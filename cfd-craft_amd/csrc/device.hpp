@@ -68,8 +68,9 @@ struct DevLevel {
   Geo g;
   T* wp[3];   // coupling to the + neighbour per axis
   T* diag;    // diagonal (0: inactive cell)
-  T* x;
+  T* x;   // iterate after pre-smoothing (and the coarsest solution)
   T* b;
+  T* xo;  // iterate after post-smoothing (fused cycle; equal to x on the coarsest level)
 };
 
 }  // namespace cfdgpu

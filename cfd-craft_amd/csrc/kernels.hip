@@ -90,11 +90,28 @@ __device__ __forceinline__ long long hi_nb(const Geo& g, long long c, int a, int
 // face, except on the last cell of a periodic axis whose wrapping upper face
 // comes first in the face list. Interior: d = w (x_lo - x_hi), y_lo += d,
 // y_hi -= d; outlet: y += w x (pressure.rs:273-285).
+// Neighbour index of cell c on axis a: side 0 lower, 1 upper (wrapping on a
+// periodic axis); c itself where there is no neighbour, so that a load from the
+// returned index is always valid and can be issued before the coupling is known.
+__device__ __forceinline__ long long nb_or_self(const Geo& g, long long c, int a, int t, int side) {
+  const long long s = stride_of(g, a);
+  const int n = g.n[a];
+  if (side == 0) return t > 0 ? c - s : (g.per[a] ? c + (long long)(n - 1) * s : c);
+  return t < n - 1 ? c + s : (g.per[a] ? c - (long long)(n - 1) * s : c);
+}
+
 template <class XV>
 __device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long long c, const int q[3], XV xv) {
+  // every value first (independent loads), then the reference's operation order
+  const double xc = xv(c);
+  double nv[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nv[2 * a] = xv(nb_or_self(g, c, a, q[a], 0));
+    nv[2 * a + 1] = xv(nb_or_self(g, c, a, q[a], 1));
+  }
   double y = 0.0;
   if (!(code & 0x3f)) return y;
-  const double xc = xv(c);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const bool lo = code & (1u << (2 * a));
@@ -110,7 +127,7 @@ __device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long lo
         if (t == n - 1 && !P) {
           y += g.wo[a] * xc;
         } else {
-          double d = g.wi[a] * (xc - xv(hi_nb(g, c, a, t)));
+          double d = g.wi[a] * (xc - nv[2 * a + 1]);
           y += d;
         }
       } else {
@@ -118,7 +135,7 @@ __device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long lo
         if (t == 0 && !P) {
           y += g.wo[a] * xc;
         } else {
-          double d = g.wi[a] * (xv(lo_nb(g, c, a, t)) - xc);
+          double d = g.wi[a] * (nv[2 * a] - xc);
           y -= d;
         }
       }
@@ -374,6 +391,20 @@ __global__ void __launch_bounds__(RB) k_init_fin(PcgState* st, const double* par
   }
 }
 
+// Warm start: x = x0 and r = b_c - A x0 (b_c and |b_c| formed by k_pcg_init).
+__global__ void __launch_bounds__(BS) k_warm_residual(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x0,
+                                                      const double* __restrict__ bc, double* __restrict__ x,
+                                                      double* __restrict__ r, const PcgState* st) {
+  if (st->status != ST_RUNNING) return;
+  long long c = (long long)blockIdx.x * BS + threadIdx.x;
+  if (c >= g.N) return;
+  int q[3];
+  coords(g, c, q);
+  double y = apply_cell(g, codes[c], c, q, [&](long long i) { return x0[i]; });
+  x[c] = x0[c];
+  r[c] = bc[c] - y;
+}
+
 // dst = src - mean on closed cells (center_nullspaces, pressure.rs:316-325).
 __global__ void __launch_bounds__(BS) k_center(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ src,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
@@ -411,15 +442,63 @@ __global__ void k_state_init(PcgState* st, double tol, long long max_it) {
 }
 
 // ---------------------------------------------------------------- IC(0)
+// The IC(0) neighbours of a cell, in the order the reference's factor visits
+// them. Lower neighbours ascend in index (z wrap, z lo, y wrap, y lo, x wrap,
+// x lo); upper neighbours descend (z wrap, z hi, y wrap, y hi, x wrap, x hi). A
+// periodic axis of two cells joins the pair through two faces, merged into one
+// entry of weight 2w (the reference adds the second face to the first entry).
+// Slots that do not apply keep index c and weight 0, so every load is valid.
+struct Ic0Nbrs {
+  long long idx[6];
+  double a[6];  // factor numerator: -w or (-w - w); 0 for an unused slot
+};
+__device__ __forceinline__ Ic0Nbrs ic0_nbrs(const Geo& g, uint8_t code, long long c, const int q[3], bool lower) {
+  Ic0Nbrs N;
+  int k = 0;
+#pragma unroll
+  for (int a = 2; a >= 0; --a) {
+    const int n = g.n[a], t = q[a];
+    const long long s = stride_of(g, a);
+    const double w = g.wi[a];
+    long long i0 = c, i1 = c;
+    double a0 = 0.0, a1 = 0.0;
+    if (lower) {
+      if (!g.per[a] || n == 1) {
+        if (t > 0 && (code & (1u << (2 * a)))) i1 = c - s, a1 = -w;
+      } else if (n == 2) {
+        if (t == 1 && (code & (1u << (2 * a)))) i1 = c - s, a1 = -w - w;
+      } else {
+        if (t == n - 1 && (code & (2u << (2 * a)))) i0 = c - (long long)(n - 1) * s, a0 = -w;
+        if (t > 0 && (code & (1u << (2 * a)))) i1 = c - s, a1 = -w;
+      }
+    } else {
+      if (!g.per[a] || n == 1) {
+        if (t < n - 1 && (code & (2u << (2 * a)))) i1 = c + s, a1 = -w;
+      } else if (n == 2) {
+        if (t == 0 && (code & (2u << (2 * a)))) i1 = c + s, a1 = -w - w;
+      } else {
+        if (t == 0 && (code & (1u << (2 * a)))) i0 = c + (long long)(n - 1) * s, a0 = -w;
+        if (t < n - 1 && (code & (2u << (2 * a)))) i1 = c + s, a1 = -w;
+      }
+    }
+    N.idx[k] = i0;
+    N.a[k] = a0;
+    N.idx[k + 1] = i1;
+    N.a[k + 1] = a1;
+    k += 2;
+  }
+  return N;
+}
+
 // Forward sweep of the hyperplane i + j + k = L, gather form of Factor::apply
 // (pressure.rs:225-232): y = (((r - l y_zlo) - l y_ylo) - l y_xlo) / D with the
-// lower neighbours in ascending index order (periodic wraps included) and
-// l = a / D[col], the reference's factor entries (no fill without a 3-cell
-// periodic axis).
+// lower neighbours in ascending index order and l = a / D[col], the reference's
+// factor entries (no fill without a 3-cell periodic axis). All loads are issued
+// before the arithmetic.
 __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
                                                 const double* __restrict__ r, double* __restrict__ y, int L, int j0,
                                                 int k0, int nj, int nk, const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
+  const int status = st ? st->status : ST_RUNNING;
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= nj * nk) return;
   int j = j0 + t % nj, k = k0 + t / nj;
@@ -428,35 +507,23 @@ __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict
   long long c = (long long)i + j * g.sy + k * g.sz;
   const uint8_t code = codes[c];
   const int q[3] = {i, j, k};
-  double value = r[c];
+  const Ic0Nbrs N = ic0_nbrs(g, code, c, q, true);
+  double dn[6], yn[6];
 #pragma unroll
-  for (int a = 2; a >= 0; --a) {
-    const int n = g.n[a], tq = q[a];
-    const long long s = stride_of(g, a);
-    const double w = g.wi[a];
-    if (!g.per[a] || n == 1) {
-      if (tq > 0 && (code & (1u << (2 * a)))) {
-        double l = -w / D[c - s];
-        value -= l * y[c - s];
-      }
-    } else if (n == 2) {
-      if (tq == 1 && (code & (1u << (2 * a)))) {
-        double l = (-w - w) / D[c - s];
-        value -= l * y[c - s];
-      }
-    } else {
-      if (tq == n - 1 && (code & (2u << (2 * a)))) {
-        long long nb = c - (long long)(n - 1) * s;
-        double l = -w / D[nb];
-        value -= l * y[nb];
-      }
-      if (tq > 0 && (code & (1u << (2 * a)))) {
-        double l = -w / D[c - s];
-        value -= l * y[c - s];
-      }
-    }
+  for (int m = 0; m < 6; ++m) {
+    dn[m] = D[N.idx[m]];
+    yn[m] = y[N.idx[m]];
   }
-  y[c] = value / D[c];
+  double value = r[c];
+  const double dc = D[c];
+  if (status != ST_RUNNING) return;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    if (N.a[m] == 0.0) continue;
+    double l = N.a[m] / dn[m];
+    value -= l * yn[m];
+  }
+  y[c] = value / dc;
 }
 
 // Backward sweep (pressure.rs:233-240) in gather form: the reference's scatter
@@ -466,7 +533,7 @@ __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict
 __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
                                                 double* __restrict__ z, int L, int j0, int k0, int nj, int nk,
                                                 const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
+  const int status = st ? st->status : ST_RUNNING;
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= nj * nk) return;
   int j = j0 + t % nj, k = k0 + t / nj;
@@ -475,34 +542,18 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
   long long c = (long long)i + j * g.sy + k * g.sz;
   const uint8_t code = codes[c];
   const int q[3] = {i, j, k};
+  const Ic0Nbrs N = ic0_nbrs(g, code, c, q, false);
+  double zn[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) zn[m] = z[N.idx[m]];
   const double dc = D[c];
   double value = z[c];
+  if (status != ST_RUNNING) return;
 #pragma unroll
-  for (int a = 2; a >= 0; --a) {
-    const int n = g.n[a], tq = q[a];
-    const long long s = stride_of(g, a);
-    const double w = g.wi[a];
-    if (!g.per[a] || n == 1) {
-      if (tq < n - 1 && (code & (2u << (2 * a)))) {
-        double l = -w / dc;
-        value -= l * z[c + s];
-      }
-    } else if (n == 2) {
-      if (tq == 0 && (code & (2u << (2 * a)))) {
-        double l = (-w - w) / dc;
-        value -= l * z[c + s];
-      }
-    } else {
-      if (tq == 0 && (code & (1u << (2 * a)))) {
-        long long nb = c + (long long)(n - 1) * s;
-        double l = -w / dc;
-        value -= l * z[nb];
-      }
-      if (tq < n - 1 && (code & (2u << (2 * a)))) {
-        double l = -w / dc;
-        value -= l * z[c + s];
-      }
-    }
+  for (int m = 0; m < 6; ++m) {
+    if (N.a[m] == 0.0) continue;
+    double l = N.a[m] / dc;
+    value -= l * zn[m];
   }
   z[c] = value / dc;
 }
@@ -518,7 +569,6 @@ __global__ void __launch_bounds__(BS)
     k_fine_gs(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b, T* __restrict__ x, int color, int first,
               const T* __restrict__ xc, Geo gc, double* __restrict__ out64, int all_cells, const PcgState* st,
               T cscale = T(1)) {
-  if (st && st->status != ST_RUNNING) return;
   long long t = (long long)blockIdx.x * BS + threadIdx.x;
   int i, j, k;
   if (all_cells) {
@@ -541,19 +591,40 @@ __global__ void __launch_bounds__(BS)
     if (i >= g.n[0]) return;
   }
   const long long c = (long long)i + j * g.sy + k * g.sz;
+  const int status = st ? st->status : ST_RUNNING;
   if (((i + j + k) & 1) != color) {
-    if (out64) out64[c] = (double)x[c];
+    if (out64 && status == ST_RUNNING) out64[c] = (double)x[c];
     return;
   }
+  // all loads first (one memory round trip), then the update
   const uint8_t code = codes[c];
+  const T bv = (T)b[c];
+  const int q[3] = {i, j, k};
+  T xn[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      T v = T(0);
+      if (!first) {
+        const long long nb = nb_or_self(g, c, a, q[a], side);
+        v = x[nb];
+        if (xc) {
+          int qn[3] = {i, j, k};
+          qn[a] = side == 0 ? (q[a] > 0 ? q[a] - 1 : g.n[a] - 1) : (q[a] < g.n[a] - 1 ? q[a] + 1 : 0);
+          v += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+        }
+      }
+      xn[2 * a + side] = v;
+    }
+  if (status != ST_RUNNING) return;
   if (!(code & FLUID)) {
     x[c] = T(0);
     if (out64) out64[c] = 0.0;
     return;
   }
-  const int q[3] = {i, j, k};
   T diag = T(0);
-  T value = (T)b[c];
+  T value = bv;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
 #pragma unroll
@@ -567,14 +638,7 @@ __global__ void __launch_bounds__(BS)
       const T w = (T)g.wi[a];
       diag += w;
       if (first) continue;
-      long long nb = side == 0 ? lo_nb(g, c, a, q[a]) : hi_nb(g, c, a, q[a]);
-      T xn = x[nb];
-      if (xc) {
-        int qn[3] = {i, j, k};
-        qn[a] = side == 0 ? (q[a] > 0 ? q[a] - 1 : g.n[a] - 1) : (q[a] < g.n[a] - 1 ? q[a] + 1 : 0);
-        xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
-      }
-      value += w * xn;
+      value += w * xn[2 * a + side];
     }
   }
   T v = diag > T(0) ? value / diag : T(0);
@@ -588,8 +652,14 @@ __device__ __forceinline__ T fine_residual(const Geo& g, const uint8_t* __restri
                                            const T* __restrict__ x, const int q[3]) {
   const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
   const uint8_t code = codes[c];
-  if (!(code & FLUID)) return T(0);
+  const T bv = (T)b[c];
   const T xv = x[c];
+  T xn[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) xn[2 * a + side] = x[nb_or_self(g, c, a, q[a], side)];
+  if (!(code & FLUID)) return T(0);
   T ax = T(0), off = T(0);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -602,10 +672,54 @@ __device__ __forceinline__ T fine_residual(const Geo& g, const uint8_t* __restri
         continue;
       }
       ax += (T)g.wi[a];
-      long long nb = side == 0 ? lo_nb(g, c, a, q[a]) : hi_nb(g, c, a, q[a]);
-      off += (T)g.wi[a] * x[nb];
+      off += (T)g.wi[a] * xn[2 * a + side];
     }
-  return (T)b[c] - (ax * xv - off);
+  return bv - (ax * xv - off);
+}
+
+// Coefficients and neighbour values of a coarse cell, loaded together: the
+// coupling to the + neighbour (wp[a][c]) and to the - neighbour (wp[a][lo]) per
+// axis, zero where the axis ends.
+template <class T>
+struct CoarseStencil {
+  T d, bv, xv;
+  T wh[3], wl[3], xh[3], xl[3];
+};
+template <class T>
+__device__ __forceinline__ CoarseStencil<T> coarse_stencil(const DevLevel<T>& L, long long c, const int q[3], bool need_x,
+                                                           const T* __restrict__ xc, const Geo& gc, T cscale) {
+  const Geo& g = L.g;
+  CoarseStencil<T> S;
+  S.d = L.diag[c];
+  S.bv = L.b[c];
+  S.xv = need_x ? L.x[c] : T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.n[a];
+    const bool has_hi = q[a] < n - 1 || g.per[a];
+    const bool has_lo = q[a] > 0 || g.per[a];
+    const long long hn = nb_or_self(g, c, a, q[a], 1);
+    const long long ln = nb_or_self(g, c, a, q[a], 0);
+    const T wh = L.wp[a][c];
+    const T wl = L.wp[a][ln];
+    S.wh[a] = has_hi ? wh : T(0);
+    S.wl[a] = has_lo ? wl : T(0);
+    T xh = T(0), xl = T(0);
+    if (need_x) {
+      xh = L.x[hn];
+      xl = L.x[ln];
+      if (xc) {
+        int qh[3] = {q[0], q[1], q[2]}, ql[3] = {q[0], q[1], q[2]};
+        qh[a] = q[a] < n - 1 ? q[a] + 1 : 0;
+        ql[a] = q[a] > 0 ? q[a] - 1 : n - 1;
+        xh += cscale * xc[(qh[0] >> 1) + (qh[1] >> 1) * gc.sy + (qh[2] >> 1) * gc.sz];
+        xl += cscale * xc[(ql[0] >> 1) + (ql[1] >> 1) * gc.sy + (ql[2] >> 1) * gc.sz];
+      }
+    }
+    S.xh[a] = xh;
+    S.xl[a] = xl;
+  }
+  return S;
 }
 
 // Coarse-level residual of one cell from the explicit face weights.
@@ -613,72 +727,40 @@ template <class T>
 __device__ __forceinline__ T coarse_residual(const DevLevel<T>& L, const int q[3]) {
   const Geo& g = L.g;
   const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
-  const T d = L.diag[c];
-  if (!(d > T(0))) return T(0);
+  const CoarseStencil<T> S = coarse_stencil<T>(L, c, q, true, nullptr, g, T(0));
+  if (!(S.d > T(0))) return T(0);
   T off = T(0);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int n = g.n[a];
-    if (q[a] < n - 1 || g.per[a]) {
-      T w = L.wp[a][c];
-      if (w != T(0)) off += w * L.x[hi_nb(g, c, a, q[a])];
-    }
-    if (q[a] > 0 || g.per[a]) {
-      long long nb = lo_nb(g, c, a, q[a]);
-      T w = L.wp[a][nb];
-      if (w != T(0)) off += w * L.x[nb];
-    }
+    if (S.wh[a] != T(0)) off += S.wh[a] * S.xh[a];
+    if (S.wl[a] != T(0)) off += S.wl[a] * S.xl[a];
   }
-  return L.b[c] - (d * L.x[c] - off);
+  return S.bv - (S.d * S.xv - off);
 }
 
 // One Gauss-Seidel update of a coarse cell; `xc` fuses the prolongation of the
-// next level's correction into the neighbour reads.
+// next level's correction into the neighbour reads. Stores only while `run`.
 template <class T>
 __device__ __forceinline__ void coarse_gs_cell(const DevLevel<T>& L, int i, int j, int k, int first,
-                                               const T* __restrict__ xc, const Geo& gc, T cscale) {
+                                               const T* __restrict__ xc, const Geo& gc, T cscale, bool run = true) {
   const Geo& g = L.g;
   const long long c = (long long)i + j * g.sy + k * g.sz;
-  const T d = L.diag[c];
-  if (!(d > T(0))) {
+  const int q[3] = {i, j, k};
+  const CoarseStencil<T> S = coarse_stencil<T>(L, c, q, !first, xc, gc, cscale);
+  if (!run) return;
+  if (!(S.d > T(0))) {
     L.x[c] = T(0);
     return;
   }
-  T value = L.b[c];
+  T value = S.bv;
   if (!first) {
-    const int q[3] = {i, j, k};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const int n = g.n[a];
-      if (q[a] < n - 1 || g.per[a]) {
-        T w = L.wp[a][c];
-        if (w != T(0)) {
-          long long nb = hi_nb(g, c, a, q[a]);
-          T xn = L.x[nb];
-          if (xc) {
-            int qn[3] = {i, j, k};
-            qn[a] = q[a] < n - 1 ? q[a] + 1 : 0;
-            xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
-          }
-          value += w * xn;
-        }
-      }
-      if (q[a] > 0 || g.per[a]) {
-        long long nb = lo_nb(g, c, a, q[a]);
-        T w = L.wp[a][nb];
-        if (w != T(0)) {
-          T xn = L.x[nb];
-          if (xc) {
-            int qn[3] = {i, j, k};
-            qn[a] = q[a] > 0 ? q[a] - 1 : n - 1;
-            xn += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
-          }
-          value += w * xn;
-        }
-      }
+      if (S.wh[a] != T(0)) value += S.wh[a] * S.xh[a];
+      if (S.wl[a] != T(0)) value += S.wl[a] * S.xl[a];
     }
   }
-  L.x[c] = value / d;
+  L.x[c] = value / S.d;
 }
 
 // Cell (i, j, k) of colour `color` for the u-th thread of a half-sweep; false past the end.
@@ -741,10 +823,10 @@ __global__ void __launch_bounds__(BS) k_coarse_restrict(DevLevel<T> L, Geo gc, T
 template <class T>
 __global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int first, const T* __restrict__ xc, Geo gc,
                                                   const PcgState* st, T cscale) {
-  if (st && st->status != ST_RUNNING) return;
+  const int status = st ? st->status : ST_RUNNING;
   int i, j, k;
   if (!color_cell(L.g, blockIdx.x * BS + threadIdx.x, color, i, j, k)) return;
-  coarse_gs_cell<T>(L, i, j, k, first, xc, gc, cscale);
+  coarse_gs_cell<T>(L, i, j, k, first, xc, gc, cscale, status == ST_RUNNING);
 }
 
 // The small coarse levels of a V-cycle in one workgroup: smoothing,
@@ -803,6 +885,253 @@ __global__ void __launch_bounds__(TB_THREADS)
     sweep(l, 0, 0, nullptr);
     sweep(l, 1, 0, nullptr);
     sweep(l, 0, 0, nullptr);
+  }
+}
+
+// ---------------------------------------------------------------- fused tile smoother
+// Temporal blocking of a whole smoothing phase: one workgroup owns a tile of
+// TX x TY x TZ cells and loads the tile plus a halo of H cells into LDS. Half-sweep
+// s updates the cells of its colour at distance >= s from the region's edge, so
+// after four half-sweeps the tile holds exactly the values four separate global
+// half-sweeps would give (halo cells are recomputed redundantly by neighbouring
+// tiles, with the same arithmetic, hence bitwise the same numbers).
+//   PRE : x = 0, half-sweeps R B R B, then the tile's residual summed over its
+//         2x2x2 bins into the next level's right-hand side (H = 5);
+//   POST: x += omega P e over the region, half-sweeps B R B R (H = 4).
+// The tile's result goes to `xout`, never to the `x` other tiles read their halo
+// from (tiles that are not resident together would otherwise see finished values).
+// Within a half-sweep every update is computed from the values before the
+// half-sweep and written after a barrier (Jacobi within a colour), which is
+// Gauss-Seidel for a proper two-colouring and stays deterministic across the
+// seam of an odd periodic axis.
+constexpr int SM_THREADS = 1024;
+
+// Tile index for workgroup b of nb: each XCD (blocks b = x mod 8) gets a
+// contiguous range of tiles, so neighbouring tiles share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ bool wrap_coord(int& v, int n, int per) {
+  if (v >= 0 && v < n) return true;
+  if (!per) return false;
+  v %= n;
+  if (v < 0) v += n;
+  return true;
+}
+
+template <class T, class TB, bool FINE, bool PRE, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(SM_THREADS)
+    k_smooth(Geo g, const uint8_t* __restrict__ codes, DevLevel<T> L, const TB* __restrict__ b, const T* __restrict__ x,
+             T* __restrict__ xout, Geo gc, T* __restrict__ bc, const T* __restrict__ xc, T cscale, double* __restrict__ out64,
+             const PcgState* st) {
+  constexpr int H = PRE ? 5 : 4;
+  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, RN = RX * RY * RZ;
+  constexpr int SX = 1, SY = RX, SZ = RX * RY;
+  constexpr int PER = (RN + SM_THREADS - 1) / SM_THREADS;  // region cells owned by one thread
+  __shared__ T xs[RN];
+  __shared__ T rs[PRE ? TX * TY * TZ : 1];
+  const int ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + TY - 1) / TY, ntz = (g.n[2] + TZ - 1) / TZ;
+  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+  const int ox = tx * TX - H, oy = ty * TY - H, oz = tz * TZ - H;
+  const int lstride[3] = {SX, SY, SZ};
+  // Thread t owns region cells t, t + SM_THREADS, ...: their coordinates, colour,
+  // margin (distance to the region's edge) and every coefficient they need are
+  // loaded once, up front, in one batch of independent loads.
+  int own_c[PER];       // global index, -1 outside the domain or the region
+  uint8_t own_meta[PER];  // bit 0 colour, bits 1-3 margin (capped at 7), bit 4 fluid / active
+  uint8_t own_code[PER];
+  T own_b[PER], own_d[PER], own_wh[PER][3], own_wl[PER][3];
+  int own_q[PER][3];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int l = threadIdx.x + k * SM_THREADS;
+    own_c[k] = -1;
+    own_meta[k] = 0;
+    own_code[k] = 0;
+    own_b[k] = T(0);
+    own_d[k] = T(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      own_wh[k][a] = T(0);
+      own_wl[k][a] = T(0);
+      own_q[k][a] = 0;
+    }
+    T v = T(0);
+    if (l < RN) {
+      const int li = l % RX, lj = (l / RX) % RY, lk = l / (RX * RY);
+      int q[3] = {ox + li, oy + lj, oz + lk};
+      const bool inside = wrap_coord(q[0], g.n[0], g.per[0]) && wrap_coord(q[1], g.n[1], g.per[1]) &&
+                          wrap_coord(q[2], g.n[2], g.per[2]);
+      if (inside) {
+        const int c = q[0] + q[1] * (int)g.sy + q[2] * (int)g.sz;
+        int m = li < RX - 1 - li ? li : RX - 1 - li;
+        m = min(m, min(lj, RY - 1 - lj));
+        m = min(m, min(lk, RZ - 1 - lk));
+        own_c[k] = c;
+        own_meta[k] = (uint8_t)(((q[0] + q[1] + q[2]) & 1) | (min(m, 7) << 1));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) own_q[k][a] = q[a];
+        if constexpr (!PRE) {
+          v = x[c];
+          if (xc) v += cscale * xc[(q[0] >> 1) + (q[1] >> 1) * gc.sy + (q[2] >> 1) * gc.sz];
+        }
+        if (m >= 1) {
+          if constexpr (FINE) {
+            own_code[k] = codes[c];
+            own_b[k] = (T)b[c];
+          } else {
+            own_d[k] = L.diag[c];
+            own_b[k] = L.b[c];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const bool has_hi = q[a] < g.n[a] - 1 || g.per[a];
+              const bool has_lo = q[a] > 0 || g.per[a];
+              const T wh = L.wp[a][c];
+              const T wl = L.wp[a][lo_nb(g, c, a, q[a])];
+              own_wh[k][a] = has_hi ? wh : T(0);
+              own_wl[k][a] = has_lo ? wl : T(0);
+            }
+          }
+        }
+      }
+      xs[l] = v;
+    }
+  }
+  if (st && !block_running(st)) return;
+  __syncthreads();
+  for (int s = 1; s <= 4; ++s) {
+    const int color = PRE ? ((s - 1) & 1) : (s & 1);
+    const bool first = PRE && s == 1;
+    T newv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      newv[k] = T(0);
+      const int m = own_meta[k] >> 1;
+      if (own_c[k] < 0 || m < s || (own_meta[k] & 1) != color) continue;
+      const int l = threadIdx.x + k * SM_THREADS;
+      T v;
+      if constexpr (FINE) {
+        const uint8_t code = own_code[k];
+        if (!(code & FLUID)) {
+          v = T(0);
+        } else {
+          T diag = T(0);
+          T value = own_b[k];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+              if (!(code & (1u << (2 * a + side)))) continue;
+              const bool boundary = side == 0 ? own_q[k][a] == 0 : own_q[k][a] == g.n[a] - 1;
+              if (boundary && !g.per[a]) {
+                diag += (T)g.wo[a];
+                continue;
+              }
+              const T w = (T)g.wi[a];
+              diag += w;
+              if (first) continue;
+              value += w * xs[side == 0 ? l - lstride[a] : l + lstride[a]];
+            }
+          v = diag > T(0) ? value / diag : T(0);
+        }
+      } else {
+        const T d = own_d[k];
+        if (!(d > T(0))) {
+          v = T(0);
+        } else {
+          T value = own_b[k];
+          if (!first) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              if (own_wh[k][a] != T(0)) value += own_wh[k][a] * xs[l + lstride[a]];
+              if (own_wl[k][a] != T(0)) value += own_wl[k][a] * xs[l - lstride[a]];
+            }
+          }
+          v = value / d;
+        }
+      }
+      newv[k] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int m = own_meta[k] >> 1;
+      if (own_c[k] < 0 || m < s || (own_meta[k] & 1) != color) continue;
+      xs[threadIdx.x + k * SM_THREADS] = newv[k];
+    }
+    __syncthreads();
+  }
+  // the tile's final values, and for PRE the residual of each tile cell
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int l = threadIdx.x + k * SM_THREADS;
+    if (l >= RN) continue;
+    const int li = l % RX, lj = (l / RX) % RY, lk = l / (RX * RY);
+    if (li < H || li >= H + TX || lj < H || lj >= H + TY || lk < H || lk >= H + TZ) continue;
+    const int c = own_c[k];
+    const int u = (li - H) + (lj - H) * TX + (lk - H) * TX * TY;
+    const bool in_domain = ox + li < g.n[0] && oy + lj < g.n[1] && oz + lk < g.n[2] && c >= 0;
+    if (in_domain) {
+      xout[c] = xs[l];
+      if (out64) out64[c] = (double)xs[l];
+    }
+    if constexpr (PRE) {
+      T r = T(0);
+      if (in_domain) {
+        if constexpr (FINE) {
+          const uint8_t code = own_code[k];
+          if (code & FLUID) {
+            const T xv = xs[l];
+            T ax = T(0), off = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+              for (int side = 0; side < 2; ++side) {
+                if (!(code & (1u << (2 * a + side)))) continue;
+                const bool boundary = side == 0 ? own_q[k][a] == 0 : own_q[k][a] == g.n[a] - 1;
+                if (boundary && !g.per[a]) {
+                  ax += (T)g.wo[a];
+                  continue;
+                }
+                ax += (T)g.wi[a];
+                off += (T)g.wi[a] * xs[side == 0 ? l - lstride[a] : l + lstride[a]];
+              }
+            r = own_b[k] - (ax * xv - off);
+          }
+        } else {
+          const T d = own_d[k];
+          if (d > T(0)) {
+            T off = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              if (own_wh[k][a] != T(0)) off += own_wh[k][a] * xs[l + lstride[a]];
+              if (own_wl[k][a] != T(0)) off += own_wl[k][a] * xs[l - lstride[a]];
+            }
+            r = own_b[k] - (d * xs[l] - off);
+          }
+        }
+      }
+      rs[u] = r;
+    }
+  }
+  if constexpr (PRE) {
+    // P^T r over the tile's 2x2x2 bins, in the association order of the restriction kernels
+    __syncthreads();
+    constexpr int CX = TX / 2, CY = TY / 2, CZ = TZ / 2;
+    for (int u = threadIdx.x; u < CX * CY * CZ; u += SM_THREADS) {
+      const int ci = u % CX, cj = (u / CX) % CY, ck = u / (CX * CY);
+      const int I = tx * CX + ci, J = ty * CY + cj, K = tz * CZ + ck;
+      if (I >= gc.n[0] || J >= gc.n[1] || K >= gc.n[2]) continue;
+      T v[8];
+#pragma unroll
+      for (int sub = 0; sub < 8; ++sub)
+        v[sub] = rs[(2 * ci + (sub & 1)) + (2 * cj + ((sub >> 1) & 1)) * TX + (2 * ck + (sub >> 2)) * TX * TY];
+      const T sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+      bc[I + J * gc.sy + K * gc.sz] = sum;
+    }
   }
 }
 
@@ -922,6 +1251,10 @@ void launch_pcg_init(long long N, const uint8_t* codes, const double* b, const d
   hipLaunchKernelGGL(k_pcg_init, dim3(RP), dim3(RB), 0, s, N, codes, b, mean, cidx, r, bc, x, st, part);
   hipLaunchKernelGGL(k_init_fin, dim3(1), dim3(RB), 0, s, st, (const double*)part);
 }
+void launch_warm_residual(const Geo& g, const uint8_t* codes, const double* x0, const double* bc, double* x, double* r,
+                          const PcgState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_warm_residual, dim3(blocks_for(g.N)), dim3(BS), 0, s, g, codes, x0, bc, x, r, st);
+}
 void launch_state_init(PcgState* st, double tol, long long max_it, hipStream_t s) {
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, st, tol, max_it);
 }
@@ -1033,6 +1366,66 @@ void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int
     hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
                        pass == 0 ? (const T*)lv[0].x : none, g1, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs);
 }
+
+template <class T, class TB, bool FINE, bool PRE, int TX, int TY, int TZ>
+static void smooth(const Geo& g, const uint8_t* codes, const DevLevel<T>& L, const TB* b, const T* x, T* xout,
+                   const Geo& gc, T* bc, const T* xc, T cs, double* out64, const PcgState* st, hipStream_t s) {
+  const unsigned nt = (unsigned)(((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * ((g.n[2] + TZ - 1) / TZ));
+  hipLaunchKernelGGL((k_smooth<T, TB, FINE, PRE, TX, TY, TZ>), dim3(nt), dim3(SM_THREADS), 0, s, g, codes, L, b, x, xout,
+                     gc, bc, xc, cs, out64, st);
+}
+
+// The same V-cycle with one fused kernel per level and phase (pre-smoothing +
+// restriction, prolongation + post-smoothing); bitwise equal to launch_vcycle.
+// The fine pre-smoothed iterate goes to `xpre`; each coarse level keeps its
+// pre-smoothed iterate in x and its post-smoothed one in xo. With fuse_fine
+// false the fine level keeps its streaming per-half-sweep kernels (they move
+// the fewest bytes) and only the latency-bound coarse levels are fused.
+template <class T, class TB>
+void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, T* xpre, int nlev, DevLevel<T>* lv,
+                         int direct_n, const double* G, double* out64, const PcgState* st, double omega, bool fuse_fine,
+                         hipStream_t s) {
+  const T cs = (T)omega;
+  if (nlev == 0) {
+    hipLaunchKernelGGL((k_direct<TB, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, b0, x0, out64, st);
+    return;
+  }
+  DevLevel<T> none{};
+  constexpr int FX = 16, FY = 8, FZ = 4;
+  const Geo& g1 = lv[0].g;
+  const T* nothing = nullptr;
+  const unsigned fb = color_blocks<T>(g0);
+  if (fuse_fine) {
+    smooth<T, TB, true, true, FX, FY, FZ>(g0, codes, none, b0, nullptr, xpre, g1, lv[0].b, nullptr, cs, nullptr, st, s);
+  } else {
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0,
+                         nothing, g1, (double*)nullptr, 0, st, cs);
+    hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(8 * g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0,
+                       g1, lv[0].b, st);
+  }
+  for (int l = 0; l + 1 < nlev; ++l)
+    smooth<T, T, false, true, 8, 8, 4>(lv[l].g, nullptr, lv[l], lv[l].b, nullptr, lv[l].x, lv[l + 1].g, lv[l + 1].b,
+                                       nullptr, cs, nullptr, st, s);
+  hipLaunchKernelGGL((k_direct<T, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, (const T*)lv[nlev - 1].b,
+                     lv[nlev - 1].x, (double*)nullptr, st);
+  for (int l = nlev - 2; l >= 0; --l)
+    smooth<T, T, false, false, 8, 8, 8>(lv[l].g, nullptr, lv[l], lv[l].b, lv[l].x, lv[l].xo, lv[l + 1].g, nullptr,
+                                        lv[l + 1].xo, cs, nullptr, st, s);
+  if (fuse_fine) {
+    smooth<T, TB, true, false, FX, FY, FZ>(g0, codes, none, b0, xpre, x0, g1, nullptr, lv[0].xo, cs, out64, st, s);
+  } else {
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
+                         pass == 0 ? (const T*)lv[0].xo : nothing, g1, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs);
+  }
+}
+template void launch_vcycle_fused<double, double>(const Geo&, const uint8_t*, const double*, double*, double*, int,
+                                                  DevLevel<double>*, int, const double*, double*, const PcgState*, double,
+                                                  bool, hipStream_t);
+template void launch_vcycle_fused<float, double>(const Geo&, const uint8_t*, const double*, float*, float*, int,
+                                                 DevLevel<float>*, int, const double*, double*, const PcgState*, double,
+                                                 bool, hipStream_t);
 
 template <class T, class TB>
 void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s) {

@@ -18,6 +18,8 @@ void launch_pcg_rz(long long N, const uint8_t* codes, const double* r, const dou
                    const int* cidx, PcgState* st, double* part, hipStream_t s);
 void launch_pcg_init(long long N, const uint8_t* codes, const double* b, const double* mean, const int* cidx, double* r,
                      double* bc, double* x, PcgState* st, double* part, hipStream_t s);
+void launch_warm_residual(const Geo& g, const uint8_t* codes, const double* x0, const double* bc, double* x, double* r,
+                          const PcgState* st, hipStream_t s);
 void launch_state_init(PcgState* st, double tol, long long max_it, hipStream_t s);
 // component means of src (and the CFD-E031 test when e031 is set) into mean
 void launch_gauge(long long N, const uint8_t* codes, int n_comp, int n_chunks, const int* members, const int* chunk_begin,
@@ -35,6 +37,10 @@ template <class T, class TB>
 void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv,
                    const DevLevel<T>* dlv, int tail_first, int direct_n, const double* G, double* out64,
                    const PcgState* st, double omega, hipStream_t s);
+template <class T, class TB>
+void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, T* xpre, int nlev, DevLevel<T>* lv,
+                         int direct_n, const double* G, double* out64, const PcgState* st, double omega, bool fuse_fine,
+                         hipStream_t s);
 template <class T, class TB>
 void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s);
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,

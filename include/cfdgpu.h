@@ -184,6 +184,35 @@ int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double 
  * update with |r|^2; 3 one IC(0) application (both sweeps); 4 one V-cycle. */
 int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch);
 
+/* ------------------------------------------------------------------ serving */
+/* Continuous batching of independent pressure solves (wind-direction campaigns,
+ * optimisation jobs): no counterpart in the reference, whose concurrent solves
+ * are separate processes or threads (validation/urban_campaign.py:197-290).
+ * A server admits grids against an HBM budget by their measured footprint and
+ * runs up to max_active requests at once, each on its grid's own stream; a
+ * request that arrives while others are on the GPU starts as soon as a slot and
+ * its grid are free. Results do not depend on what else runs (fixed-shape
+ * reductions). One server is driven by one host thread. */
+typedef struct cfd_server cfd_server;
+/* hbm_budget_bytes 0: 90 % of the free device memory. */
+int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out);
+void cfd_server_destroy(cfd_server* s);
+/* cfd_grid_create under the budget; CFD_ENOMEM (nothing allocated) when it does not fit. */
+int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out);
+int cfd_server_release(cfd_server* s, cfd_grid* g);
+int cfd_server_usage(const cfd_server* s, size_t* used, size_t* budget);
+size_t cfd_grid_footprint(const cfd_grid* g);
+/* Queues a solve of grid g (device pointers); precision CFD_PRECISION_FP64 or MIXED. */
+int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const double* d_scale, double tol,
+                      int64_t max_iterations, int precision, double* d_x, uint64_t* ticket);
+/* Advances every running request without blocking; returns the number running or queued. */
+int cfd_server_progress(cfd_server* s);
+int cfd_server_wait(cfd_server* s, uint64_t ticket);
+int cfd_server_drain(cfd_server* s);
+/* Result of a finished request (then forgotten): the solve's code, info, latency
+ * (submit to finish) and queueing time (submit to start), in seconds. */
+int cfd_server_result(cfd_server* s, uint64_t ticket, cfd_linear_info* info, double* latency_s, double* queued_s);
+
 /* Device memory helpers for callers without their own allocator. */
 int cfd_device_alloc(size_t bytes, void** d_ptr);
 int cfd_device_free(void* d_ptr);

@@ -168,3 +168,58 @@ def test_deterministic_repeat(gpu_available):
     x1, i1 = g.solve_scaled(b, t, 1e-10, 2000)
     x2, i2 = g.solve_scaled(b, t, 1e-10, 2000)
     assert np.array_equal(x1, x2) and i1["iterations"] == i2["iterations"]
+
+
+def _vcycle(case, precision, fused, r):
+    import os
+
+    old = os.environ.get("CFD_MG_FUSED")
+    os.environ["CFD_MG_FUSED"] = str(fused)
+    try:
+        g = cfdgpu.Grid(case)
+    finally:
+        if old is None:
+            del os.environ["CFD_MG_FUSED"]
+        else:
+            os.environ["CFD_MG_FUSED"] = old
+    g.set_options(precision)
+    return g.precondition(r)
+
+
+@pytest.mark.parametrize("name", ["urban64", "cavity", "periodic-even", "barrier", "urban-many-tiles"])
+@pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
+def test_fused_vcycle_equals_per_sweep_vcycle(gpu_available, name, precision):
+    """One kernel per level and phase (temporal blocking in LDS) gives bitwise the
+    values of one kernel per red-black half-sweep. (Grids whose levels keep every
+    periodic axis even: an odd periodic axis always takes the fused path.)"""
+    if name == "urban64":
+        case = W.urban_block(n=(64, 48, 40), boxes=8)[0]
+    elif name == "urban-many-tiles":
+        # more tiles than can be resident at once: a tile must never read a halo
+        # value that another tile has already replaced
+        case = W.urban_block(n=(128, 96, 48), boxes=16)[0]
+    elif name == "cavity":
+        case = W.cavity([40, 32, 28], MULTIGRID)
+    elif name == "periodic-even":
+        case = Case(cells=[32, 16, 20], size_m=[3.2, 1.6, 2.0], boundaries=[PERIODIC, PERIODIC, WALL])
+    else:
+        case = Case(cells=[36, 20, 24], size_m=[36, 20, 24], boundaries=[OPEN, WALL, WALL], velocity_m_s=[1.0, 0, 0],
+                    blocks=[Block([10.0, 0, 0], [12.0, 20, 20]), Block([20.0, 5, 0], [26.0, 12, 9])])
+    case.preconditioner = MULTIGRID
+    r = np.random.default_rng(2).standard_normal(case.count)
+    b = _vcycle(case, precision, 0, r)
+    for fused in (1, 2):
+        a = _vcycle(case, precision, fused, r)
+        assert np.array_equal(a, b), (fused, float(np.max(np.abs(a - b))))
+
+
+def test_odd_periodic_multigrid_is_deterministic(gpu_available):
+    """Odd periodic coarse levels (36 -> 18 -> 9): repeated cycles are bitwise equal."""
+    case = W.cavity([40, 36, 28], MULTIGRID)
+    r = np.random.default_rng(4).standard_normal(case.count)
+    g = cfdgpu.Grid(case)
+    a = g.precondition(r)
+    b = g.precondition(r)
+    assert np.array_equal(a, b)
+    x, info = g.solve(g.center_nullspaces(r), 1e-10, 500)
+    assert info["relative_residual"] <= 1.01e-9
