@@ -63,17 +63,187 @@ def cpu_baseline(case, b, t, tol):
     )
 
 
+def served(requests=512, n_grids=16, max_active=8, seed=4):
+    """Served mix: resident grids of 64^3 .. 192x192x96 cells (seeded boxes, 5-25 %
+    solid), tolerances 1e-6/1e-8/1e-10. The same request list runs (1) plain:
+    one FP64 solve after another on the caller's thread, (2) served: FP64 and
+    mixed precision through the continuous-batching server, closed loop (all
+    requests queued at once: capacity) and (3) open loop: Poisson arrivals at
+    80 % of the served mixed capacity, for per-request latency p50/p99 measured
+    from arrival to completion. Grid set-up is excluded (resident grids)."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    pool = [W.served_request(seed, k) for k in range(n_grids)]
+    server = cfdgpu.Server(max_active=max_active)
+    grids = [server.admit(c) for c, _, _, _ in pool]
+    bufs = [(cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray.from_host(t), cfdgpu.DeviceArray(g.count))
+            for (_, b, t, _), g in zip(pool, grids)]
+    rng = np.random.default_rng(seed)
+    reqs = [(int(rng.integers(0, n_grids)), W.SERVED_TOLERANCES[int(rng.integers(0, 3))]) for _ in range(requests)]
+    cells = sum(grids[k].count for k, _ in reqs)
+
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, int(math.ceil(q * len(v))) - 1)]
+
+    # warm every grid in both precisions (hierarchy and graphs ready)
+    for prec in (cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED):
+        for k, g in enumerate(grids):
+            g.set_options(prec)
+            g.solve_device(bufs[k][0], bufs[k][2], 1e-8, 3000, d_scale=bufs[k][1])
+    cfdgpu.synchronize()
+    out = {"requests": requests, "grids": n_grids, "max_active": max_active,
+           "shapes": [list(s) for s in W.SERVED_SHAPES], "tolerances": W.SERVED_TOLERANCES,
+           "mean_cells": round(cells / requests)}
+    # (1) plain: sequential FP64 solves
+    for g in grids:
+        g.set_options(cfdgpu.PRECISION_FP64)
+    lat = []
+    t0 = time.perf_counter()
+    for k, tol in reqs:
+        s0 = time.perf_counter()
+        grids[k].solve_device(bufs[k][0], bufs[k][2], tol, 3000, d_scale=bufs[k][1])
+        lat.append(time.perf_counter() - s0)
+    el = time.perf_counter() - t0
+    out["plain_fp64"] = {"solves_per_s": round(requests / el, 1), "p50_ms": round(1e3 * pct(lat, 0.5), 3),
+                         "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
+    # (2) served, closed loop
+    cap = {}
+    for name, prec in (("served_fp64", cfdgpu.PRECISION_FP64), ("served_mixed", cfdgpu.PRECISION_MIXED)):
+        t0 = time.perf_counter()
+        tks = [server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, prec, d_scale=bufs[k][1]) for k, tol in reqs]
+        server.drain()
+        el = time.perf_counter() - t0
+        for tk in tks:
+            server.result(tk)
+        cap[name] = requests / el
+        out[name] = {"solves_per_s": round(cap[name], 1)}
+    # (3) open loop at 80 % of served mixed capacity
+    lam = 0.8 * cap["served_mixed"]
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+    tks = []
+    t0 = time.perf_counter()
+    i = 0
+    while i < requests:
+        now = time.perf_counter() - t0
+        while i < requests and arrivals[i] <= now:
+            k, tol = reqs[i]
+            tks.append(server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, cfdgpu.PRECISION_MIXED,
+                                     d_scale=bufs[k][1]))
+            i += 1
+        server.progress()
+    server.drain()
+    el = time.perf_counter() - t0
+    lat = [server.result(tk)[1] for tk in tks]
+    out["open_loop_mixed"] = {"lambda_per_s": round(lam, 1), "achieved_per_s": round(requests / el, 1),
+                              "p50_ms": round(1e3 * pct(lat, 0.5), 3), "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
+    out["served_mixed_over_plain"] = round(cap["served_mixed"] / out["plain_fp64"]["solves_per_s"], 3)
+    server.close()
+    return out
+
+
+def large(steps=3):
+    """The 256x256x128 prism city (8.4M cells, 200 convex 4-8-vertex prisms, about
+    20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10; each field is
+    compared with the FP64 solve at 1e-12 (itself held to the oracle by the parity
+    tests); fine smoother and operator bandwidth at a size beyond the 256 MB MALL."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.urban_prisms()
+    t0 = time.perf_counter()
+    g = cfdgpu.Grid(case)
+    setup = time.perf_counter() - t0
+    db = cfdgpu.DeviceArray.from_host(b)
+    ds = cfdgpu.DeviceArray.from_host(t)
+    dx = cfdgpu.DeviceArray(g.count)
+    g.set_options(cfdgpu.PRECISION_FP64)
+    ref_info = g.solve_device(db, dx, 1e-12, 3000, d_scale=ds)
+    ref = dx.numpy()
+    out = {"cells": g.count, "fluid_cells": g.fluid_count, "solid_fraction": round(1 - g.fluid_count / g.count, 4),
+           "cold_setup_s": round(setup, 3), "reference_iterations_1e-12": ref_info["iterations"]}
+    for tol in (1e-8, 1e-10):
+        for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
+            g.set_options(prec)
+            info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            el = time.perf_counter() - t0
+            x = dx.numpy()
+            out[f"{name}_tol_{tol:g}"] = {"solves_per_s": round(steps / el, 2), "iterations": info["iterations"],
+                                          "true_rel_residual": info["relative_residual"],
+                                          "field_rel_error": float(np.linalg.norm(x - ref) / np.linalg.norm(ref))}
+    g.set_options(cfdgpu.PRECISION_FP64)
+    for name, k in (("fine_gs_fp64", 1), ("operator", 0), ("axpy", 2)):
+        sec, nb = g.profile_kernel(k, reps=20)
+        out[name] = {"avg_us": round(sec * 1e6, 2), "algorithmic_GBps": round(nb / sec / 1e9, 1),
+                     "frac": round(nb / sec / 1e9 / HBM_PEAK_GBS, 4)}
+    g.set_options(cfdgpu.PRECISION_MIXED)
+    sec, nb = g.profile_kernel(1, reps=20)
+    out["fine_gs_mixed"] = {"avg_us": round(sec * 1e6, 2), "algorithmic_GBps": round(nb / sec / 1e9, 1),
+                            "frac": round(nb / sec / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_fine_gs_urban_block_128x128x64.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fine half-sweep from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel factor)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pmc_probe():
+    """The kernels the --pmc passes count: 20 fine half-sweeps of the urban block
+    (FP64) and 5 copies of 64 Mi floats (256 MiB read, 256 MiB written) that
+    calibrate FETCH_SIZE and WRITE_SIZE."""
+    import torch
+
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.urban_block()
+    g = cfdgpu.Grid(case)
+    g.set_options(cfdgpu.PRECISION_FP64)
+    sec, nb = g.profile_kernel(1, reps=20)
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(5):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"fine_gs_avg_us": sec * 1e6, "fine_gs_algorithmic_bytes": nb, "copy_bytes_each_way": 4 << 26}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the pressure solve of one grid is measured on one MI355X "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if args.pmc_probe:
+        pmc_probe()
+        return
 
     from cfdgpu import workloads as W
 
@@ -124,7 +294,7 @@ def main():
     sec, nbytes = g.profile_kernel(1, reps=50)
     achieved = nbytes / sec / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(),
                     kernel="k_fine_gs<double,double> (fine red-black half-sweep)",
                     avg_launch_us=round(sec * 1e6, 3), algorithmic_bytes_per_launch=nbytes)
     # the whole V-cycle and the operator for context
@@ -132,6 +302,24 @@ def main():
     so, bo = g.profile_kernel(0, reps=50)
     extra["vcycle"] = dict(avg_us=round(sv * 1e6, 2), algorithmic_GBps=round(bv / sv / 1e9, 1))
     extra["operator"] = dict(avg_us=round(so * 1e6, 2), algorithmic_GBps=round(bo / so / 1e9, 1))
+    # the same system with the IC(0) preconditioner (plain FP64, 1e-10)
+    from cfdgpu.case import IC0
+
+    case_ic = W.urban_block()[0]
+    case_ic.preconditioner = IC0
+    gi = cfdgpu.Grid(case_ic)
+    gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
+    cfdgpu.synchronize()
+    t0 = time.perf_counter()
+    ii = gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
+    cfdgpu.synchronize()
+    extra["ic0_tol_1e-10"] = dict(solves_per_s=round(1.0 / (time.perf_counter() - t0), 3), iterations=ii["iterations"],
+                                  true_rel_residual=ii["relative_residual"])
+    del gi
+    if not args.no_served:
+        extra["served"] = served()
+    if not args.no_large:
+        extra["prisms_256x256x128"] = large()
     lat_ms = sorted(1e3 * v for v in lat)
     p = lambda q: lat_ms[min(len(lat_ms) - 1, int(math.ceil(q * len(lat_ms))) - 1)]  # noqa: E731
     out = {
