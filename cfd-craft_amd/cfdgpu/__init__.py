@@ -50,6 +50,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(_HERE, "libcfdgpu.so")
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+CYCLE_RB, CYCLE_REFERENCE, CYCLE_GEOMETRIC = 0, 1, 2
 CANCELLED, WALL_TIME = 100, 101
 
 
@@ -97,7 +98,7 @@ class Options(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("warm_start", ctypes.c_int32),
         ("poll_every", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("cycle", ctypes.c_int32),
     ]
 
 
@@ -187,6 +188,15 @@ def case_topology(case: Case, ic0: bool = False):
     _check(lib().cfd_case_topology(ctypes.byref(c), _ptr(counts), _ptr(codes), _ptr(comp), _ptr(anch), anch.size,
                                    _ptr(diag) if diag is not None else None))
     return dict(counts=counts, codes=codes, component=comp, anchored=anch[: int(counts[3])].astype(bool), ic0_diagonal=diag)
+
+
+def case_multigrid_stats(case: Case, cycle: int = 0):
+    """Host-only level sizes of the multigrid hierarchy a cycle builds (no device)."""
+    c, keep = case.to_c()
+    s = MgStats()
+    _check(lib().cfd_case_multigrid_stats(ctypes.byref(c), int(cycle), ctypes.byref(s)))
+    n = s.levels
+    return dict(level_cells=list(s.level_cells[:n]), level_nonzeros=list(s.level_nonzeros[:n]))
 
 
 class Control:
@@ -311,8 +321,11 @@ class Grid:
         _check(lib().cfd_profile_kernel(self.h, int(kernel), int(reps), ctypes.byref(sec), ctypes.byref(nbytes)))
         return sec.value, nbytes.value
 
-    def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False):
-        o = Options(int(precision), int(bool(warm_start)), 8, 0)
+    def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False, cycle: int = 0):
+        """precision: PRECISION_FP64 or PRECISION_MIXED; cycle (multigrid): CYCLE_RB,
+        CYCLE_REFERENCE (the reference's hierarchy and lexicographic smoothing, FP64)
+        or CYCLE_GEOMETRIC."""
+        o = Options(int(precision), int(bool(warm_start)), 8, int(cycle))
         _check(lib().cfd_grid_set_options(self.h, ctypes.byref(o)))
 
     # ---- operator and gauges

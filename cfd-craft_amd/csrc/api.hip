@@ -49,6 +49,13 @@ constexpr int kSegment = 8;          // PCG iterations per host poll (pressure.r
 constexpr int64_t kCoarseMax = 512;
 constexpr int64_t kTailCells = 0;
 constexpr double kOmega = 1.9;
+// Aggregation hierarchy of the red-black cycle: coarsening stops once every
+// component has at most kAggDense nodes, and that level is solved exactly by a
+// dense inverse per component (one launch instead of the ten of each further
+// level; CFD_MG_DENSE overrides). Blocks above kHostBlock nodes are inverted
+// on the device at set-up.
+constexpr int64_t kAggDense = 2048;
+constexpr int64_t kHostBlock = 128;
 constexpr int kMaxReduceBlocks = 2048;
 
 // Device bytes allocated while a grid is being set up (its HBM footprint).
@@ -87,9 +94,29 @@ struct DevHierarchy {
   DevBuf<DevLevel<T>> dlevels;  // device copy of `levels` for the one-workgroup tail
   int tail_first = 0;
   std::vector<std::unique_ptr<DevBuf<T>>> store;
-  T* x0 = nullptr;  // FP32 fine-level iterate (mixed precision only)
-  DevBuf<T> x0buf;
   DevBuf<T> xpre;  // fine-level iterate after pre-smoothing (fused cycle)
+};
+
+// Aggregation hierarchy on the device. The integer arrays (CSR structure,
+// parents, children, coarsest blocks) serve both precisions; values, right-hand
+// sides and iterates exist per precision.
+struct DevAggShared {
+  std::vector<std::unique_ptr<DevBuf<int>>> ints;
+  DevBuf<int> fparent;  // per fine cell: node of the first coarse level (-1 solid)
+  DevBuf<int> bptr, rows;
+  DevBuf<long long> goff;
+  DevBuf<double> G;
+  AggCoarsest D{};
+  DevBuf<int> big_block, big_first, small_block;
+  int seams = 0;       // periodic axes of odd length on the fine level
+  int nlev = 0;        // coarse levels
+  std::vector<AggDevLevel<double>> shape;  // integer pointers and colour ranges per coarse level
+  bool built = false;
+};
+template <class T>
+struct DevAgg {
+  std::vector<AggDevLevel<T>> levels;
+  std::vector<std::unique_ptr<DevBuf<T>>> store;
 };
 
 }  // namespace cfdgpu
@@ -106,7 +133,7 @@ struct cfd_grid {
   Geo geo;
   FaceGeo fgeo;
   int nb = 1;  // reduction grid
-  cfd_options opt{CFD_PRECISION_FP64, 0, kSegment, 0};
+  cfd_options opt{CFD_PRECISION_FP64, 0, kSegment, CFD_CYCLE_RB};
   // topology
   DevBuf<uint8_t> codes;
   DevBuf<int> cidx;
@@ -126,6 +153,14 @@ struct cfd_grid {
   DevHierarchy<double> h64;
   DevHierarchy<float> h32;
   DevBuf<double> G;
+  bool has_geo = false;  // geometric hierarchy built (CFD_CYCLE_GEOMETRIC)
+  DevBuf<float> x0f;     // FP32 fine iterate of the mixed-precision cycles
+  // aggregation hierarchy (CFD_CYCLE_RB)
+  AggHierarchy agg;
+  DevAggShared ash;
+  DevAgg<double> a64;
+  DevAgg<float> a32;
+  int64_t agg_dense = kAggDense;
   // captured iteration segment, per precision
   hipGraphExec_t seg[2] = {nullptr, nullptr};
   // resident solution of the previous solve (warm start) and the HBM footprint
@@ -248,6 +283,207 @@ int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
   return 0;
 }
 
+template <class T>
+int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
+  const AggHierarchy& H = g->agg;
+  da.levels.clear();
+  da.store.clear();
+  auto up = [&](const std::vector<double>& v, T** out) -> int {
+    std::vector<T> t(v.begin(), v.end());
+    da.store.emplace_back(new DevBuf<T>());
+    int rc = da.store.back()->upload(t.data(), t.size());
+    *out = da.store.back()->p;
+    return rc;
+  };
+  for (int l = 0; l < g->ash.nlev; ++l) {
+    const AggLevel& L = H.levels[(size_t)l + 1];
+    const AggDevLevel<double>& sh = g->ash.shape[(size_t)l];
+    AggDevLevel<T> d{};
+    d.n = sh.n;
+    d.ncol = sh.ncol;
+    for (int c = 0; c <= kMaxColours; ++c) d.cptr[c] = sh.cptr[c];
+    d.ptr = sh.ptr;
+    d.col = sh.col;
+    d.parent = sh.parent;
+    d.child_ptr = sh.child_ptr;
+    d.child = sh.child;
+    T *val = nullptr, *diag = nullptr, *b = nullptr, *x = nullptr;
+    int rc;
+    if ((rc = up(L.val, &val)) || (rc = up(L.diag, &diag))) return rc;
+    std::vector<double> zero((size_t)L.n, 0.0);
+    if ((rc = up(zero, &b)) || (rc = up(zero, &x))) return rc;
+    d.val = val;
+    d.diag = diag;
+    d.b = b;
+    d.x = x;
+    da.levels.push_back(d);
+  }
+  return 0;
+}
+
+// Builds the aggregation hierarchy (host) and its device arrays.
+int build_agg(cfd_grid* g) {
+  if (g->ash.built) return 0;
+  const HostGrid& h = g->host;
+  int rc = build_agg_hierarchy(h, g->agg, g->agg_dense, kHostBlock);
+  if (rc) return rc;
+  colour_agg_hierarchy(g->agg);
+  AggHierarchy& H = g->agg;
+  DevAggShared& A = g->ash;
+  A.nlev = (int)H.levels.size() - 1;
+  A.seams = 0;
+  for (int a = 0; a < 3; ++a)
+    if (h.periodic[a] && h.n[a] >= 3 && (h.n[a] & 1)) A.seams |= 1 << a;
+  if (A.nlev > 0) {
+    std::vector<int> fp((size_t)h.count, -1);
+    for (int64_t c = 0; c < h.count; ++c) {
+      const int32_t node = H.node_of_cell[(size_t)c];
+      if (node >= 0) fp[(size_t)c] = H.levels[0].parent[(size_t)node];
+    }
+    if ((rc = A.fparent.upload(fp.data(), fp.size()))) return rc;
+  }
+  auto upi = [&](const int32_t* v, size_t n, const int** out) -> int {
+    A.ints.emplace_back(new DevBuf<int>());
+    int e = A.ints.back()->upload(v, n);
+    *out = A.ints.back()->p;
+    return e;
+  };
+  auto upi64 = [&](const std::vector<int64_t>& v, const int** out) -> int {
+    std::vector<int32_t> t(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+      if (v[i] > INT32_MAX) {
+        set_error("multigrid: a level has more than 2^31 entries");
+        return CFD_EUNSUPPORTED;
+      }
+      t[i] = (int32_t)v[i];
+    }
+    return upi(t.data(), t.size(), out);
+  };
+  A.shape.clear();
+  for (int l = 1; l <= A.nlev; ++l) {
+    const AggLevel& L = H.levels[(size_t)l];
+    AggDevLevel<double> sh{};
+    sh.n = (int)L.n;
+    sh.ncol = (int)L.colour_ptr.size() - 1;
+    if (sh.ncol > kMaxColours) {
+      set_error("multigrid: a coarse level needs more colour classes than supported");
+      return CFD_EUNSUPPORTED;
+    }
+    for (int c = 0; c <= sh.ncol; ++c) sh.cptr[c] = (int)L.colour_ptr[(size_t)c];
+    if ((rc = upi64(L.ptr, &sh.ptr)) || (rc = upi(L.col.data(), L.col.size(), &sh.col)) ||
+        (rc = upi64(L.child_ptr, &sh.child_ptr)) || (rc = upi(L.child.data(), L.child.size(), &sh.child)))
+      return rc;
+    if (!L.parent.empty() && (rc = upi(L.parent.data(), L.parent.size(), &sh.parent))) return rc;
+    A.shape.push_back(sh);
+  }
+  // coarsest blocks (on the fine level itself the rows are cells)
+  std::vector<int32_t> rows = H.block_rows;
+  if (A.nlev == 0)
+    for (auto& r : rows) r = (int32_t)H.cell_of_node[(size_t)r];
+  std::vector<int32_t> bptr(H.block_ptr.begin(), H.block_ptr.end());
+  std::vector<long long> goff(H.block_goff.begin(), H.block_goff.end());
+  if ((rc = A.bptr.upload(bptr.data(), bptr.size())) || (rc = A.rows.upload(rows.data(), rows.size())) ||
+      (rc = A.goff.upload(goff.data(), goff.size())) || (rc = A.G.upload(H.block_G.data(), H.block_G.size())))
+    return rc;
+  A.D.nblocks = (int)H.block_ptr.size() - 1;
+  A.D.bptr = A.bptr.p;
+  A.D.rows = A.rows.p;
+  A.D.goff = A.goff.p;
+  A.D.G = A.G.p;
+  {
+    std::vector<int> bb, bf{0}, sb;
+    for (int k = 0; k < A.D.nblocks; ++k) {
+      const int64_t m = H.block_ptr[(size_t)k + 1] - H.block_ptr[(size_t)k];
+      if (m > kSmallBlock) {
+        bb.push_back(k);
+        bf.push_back(bf.back() + (int)m);
+      } else {
+        sb.push_back(k);
+      }
+    }
+    A.D.nbig = (int)bb.size();
+    A.D.big_rows = bf.back();
+    A.D.nsmall = (int)sb.size();
+    if ((rc = A.big_first.upload(bf.data(), bf.size()))) return rc;
+    if (!bb.empty() && (rc = A.big_block.upload(bb.data(), bb.size()))) return rc;
+    if (!sb.empty() && (rc = A.small_block.upload(sb.data(), sb.size()))) return rc;
+    A.D.big_block = A.big_block.p;
+    A.D.big_first = A.big_first.p;
+    A.D.small_block = A.small_block.p;
+  }
+  if (!H.deferred.empty()) {
+    int64_t most = 0;
+    for (const auto& d : H.deferred) most = std::max(most, d.m);
+    // set-up scratch, not part of the grid's resident footprint
+    size_t* counting = tl_bytes;
+    tl_bytes = nullptr;
+    DevBuf<double> work;
+    DevBuf<int> bad;
+    rc = work.alloc((size_t)(most * most + 3 * most));
+    if (!rc) rc = bad.alloc(1);
+    tl_bytes = counting;
+    if (rc) return rc;
+    for (const auto& d : H.deferred) {
+      if (!invert_block(A.G.p + H.block_goff[(size_t)d.block], (int)d.m, d.anchored, d.K.data(), work.p, bad.p,
+                        g->stream)) {
+        set_error("CFD-E035: numerics.pressure_preconditioner: masked coarse Cholesky lost positive definiteness");
+        return CFD_E035;
+      }
+    }
+    H.deferred.clear();
+  }
+  if ((rc = upload_agg_values<double>(g, g->a64))) return rc;
+  if ((rc = upload_agg_values<float>(g, g->a32))) return rc;
+  // the host copies of the coarse matrices are no longer needed
+  for (auto& L : H.levels) {
+    std::vector<int32_t>().swap(L.col);
+    std::vector<double>().swap(L.val);
+    std::vector<int64_t>().swap(L.child_ptr);
+    std::vector<int32_t>().swap(L.child);
+  }
+  A.built = true;
+  return 0;
+}
+
+// Builds the geometric hierarchy of CFD_CYCLE_GEOMETRIC.
+int build_geometric(cfd_grid* g) {
+  if (g->has_geo) return 0;
+  HostGrid& h = g->host;
+  int rc = build_hierarchy(h, g->hier, g->coarse_max);
+  if (rc) return rc;
+  // A periodic axis of odd length >= 3 on some level puts two cells of one colour
+  // next to each other across the seam; only the fused kernels (which update a
+  // colour from the values before the half-sweep) handle that deterministically.
+  auto odd = [&](const int64_t* n) {
+    for (int a = 0; a < 3; ++a)
+      if (h.periodic[a] && n[a] >= 3 && (n[a] & 1)) return true;
+    return false;
+  };
+  bool odd_periodic = odd(h.n);
+  for (const HostLevel& L : g->hier.coarse) odd_periodic |= odd(L.n);
+  if (odd_periodic) g->fused = 2;
+  if ((rc = upload_hierarchy(g, g->h64))) return rc;
+  if ((rc = upload_hierarchy(g, g->h32))) return rc;
+  if ((rc = g->G.upload(g->hier.direct.data(), g->hier.direct.size()))) return rc;
+  g->has_geo = true;
+  return 0;
+}
+
+int ensure_cycle(cfd_grid* g, int cycle) {
+  switch (cycle) {
+    case CFD_CYCLE_RB:
+      return build_agg(g);
+    case CFD_CYCLE_GEOMETRIC:
+      return build_geometric(g);
+    case CFD_CYCLE_REFERENCE:
+      set_error("multigrid: the reference cycle is not available on this build");
+      return CFD_EUNSUPPORTED;
+    default:
+      set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_GEOMETRIC");
+      return CFD_EINVAL;
+  }
+}
+
 int setup_device_inner(cfd_grid* g);
 int setup_device(cfd_grid* g) {
   size_t bytes = 0;
@@ -269,6 +505,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_GPU_GRAPHS")) g->use_graphs = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
+  if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   size_t N = (size_t)h.count;
@@ -314,25 +551,12 @@ int setup_device_inner(cfd_grid* g) {
     build_ic0_diagonal(h);
     if ((rc = g->D.upload(h.ic0_diag.data(), N))) return rc;
   } else {
-    rc = build_hierarchy(h, g->hier, g->coarse_max);
-    if (rc) return rc;
     g->has_hier = true;
-    // A periodic axis of odd length >= 3 on some level puts two cells of one colour
-    // next to each other across the seam; only the fused kernels (which update a
-    // colour from the values before the half-sweep) handle that deterministically.
-    auto odd = [&](const int64_t* n) {
-      for (int a = 0; a < 3; ++a)
-        if (h.periodic[a] && n[a] >= 3 && (n[a] & 1)) return true;
-      return false;
-    };
-    bool odd_periodic = odd(h.n);
-    for (const HostLevel& L : g->hier.coarse) odd_periodic |= odd(L.n);
-    if (odd_periodic) g->fused = 2;
-    if ((rc = upload_hierarchy(g, g->h64))) return rc;
-    if ((rc = upload_hierarchy(g, g->h32))) return rc;
-    if ((rc = g->h32.x0buf.alloc(N))) return rc;
-    g->h32.x0 = g->h32.x0buf.p;
-    if ((rc = g->G.upload(g->hier.direct.data(), g->hier.direct.size()))) return rc;
+    if ((rc = g->x0f.alloc(N))) return rc;
+    int cycle = CFD_CYCLE_RB;
+    if (const char* env = std::getenv("CFD_MG_CYCLE")) cycle = std::atoi(env);
+    if ((rc = ensure_cycle(g, cycle))) return rc;
+    g->opt.cycle = cycle;
   }
   HIP_TRY(hipDeviceSynchronize());
   return 0;
@@ -348,11 +572,21 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
     launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
     return;
   }
+  if (g->opt.cycle == CFD_CYCLE_RB) {
+    const DevAggShared& A = g->ash;
+    if (g->opt.precision == CFD_PRECISION_MIXED)
+      launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
+                                       g->a32.levels.data(), A.D, z, st, g->omega, g->stream);
+    else
+      launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
+                                        A.D, nullptr, st, g->omega, g->stream);
+    return;
+  }
   const int nlev = (int)g->h64.levels.size();
   const int dn = (int)g->hier.direct_n;
   if (g->fused) {
     if (g->opt.precision == CFD_PRECISION_MIXED)
-      launch_vcycle_fused<float, double>(g->geo, g->codes.p, r, g->h32.x0, g->h32.xpre.p, nlev, g->h32.levels.data(), dn,
+      launch_vcycle_fused<float, double>(g->geo, g->codes.p, r, g->x0f.p, g->h32.xpre.p, nlev, g->h32.levels.data(), dn,
                                          g->G.p, z, st, g->omega, g->fused == 2, g->stream);
     else
       launch_vcycle_fused<double, double>(g->geo, g->codes.p, r, z, g->h64.xpre.p, nlev, g->h64.levels.data(), dn, g->G.p,
@@ -360,7 +594,7 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
     return;
   }
   if (g->opt.precision == CFD_PRECISION_MIXED) {
-    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->h32.x0, nlev, g->h32.levels.data(), g->h32.dlevels.p,
+    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, nlev, g->h32.levels.data(), g->h32.dlevels.p,
                                  g->h32.tail_first, dn, g->G.p, z, st, g->omega, g->stream);
   } else {
     launch_vcycle<double, double>(g->geo, g->codes.p, r, z, nlev, g->h64.levels.data(), g->h64.dlevels.p,
@@ -529,8 +763,13 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
   local.operator_applications = local.iterations + s.true_checks;
   local.preconditioner_applications = s.status == ST_ZERO ? 0 : (s.status == ST_CONVERGED ? s.it : s.it + 1);
   if (g->has_hier) {
-    int64_t cells = g->host.count;
-    for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
+    int64_t cells = 0;
+    if (g->opt.cycle == CFD_CYCLE_GEOMETRIC) {
+      cells = g->host.count;
+      for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
+    } else {
+      for (size_t l = 0; l + 1 < g->agg.levels.size(); ++l) cells += g->agg.levels[l].n;
+    }
     local.cycles = local.preconditioner_applications;
     local.smoothing_cell_updates = local.cycles * 4 * cells;
     local.residual_cell_updates = local.cycles * cells;
@@ -650,6 +889,32 @@ int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int6
   return CFD_OK;
 }
 
+int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) {
+  if (!c || !s) {
+    set_error("cfd_case_multigrid_stats: null argument");
+    return CFD_EINVAL;
+  }
+  std::memset(s, 0, sizeof *s);
+  if (cycle != CFD_CYCLE_RB) {
+    set_error("cfd_case_multigrid_stats: only CFD_CYCLE_RB is available without a device");
+    return CFD_EUNSUPPORTED;
+  }
+  HostGrid h;
+  int rc = build_host_grid(*c, h);
+  if (rc) return rc;
+  AggHierarchy H;
+  if ((rc = build_agg_hierarchy(h, H))) return rc;
+  int L = 0;
+  for (const AggLevel& lv : H.levels) {
+    if (L >= 32) break;
+    s->level_cells[L] = lv.n;
+    s->level_nonzeros[L] = L == 0 ? lv.n + H.level0_entries : (int64_t)lv.col.size() + lv.n;
+    ++L;
+  }
+  s->levels = L;
+  return CFD_OK;
+}
+
 int cfd_grid_counts(const cfd_grid* g, int64_t counts[8]) {
   host_counts(g->host, counts);
   return CFD_OK;
@@ -672,7 +937,24 @@ int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
     set_error("cfd_options.precision must be FP64 or MIXED");
     return CFD_EINVAL;
   }
+  if (g->has_hier && o->cycle != g->opt.cycle) {
+    size_t bytes = 0;
+    tl_bytes = &bytes;
+    HIP_TRY(hipSetDevice(g->device));
+    int rc = ensure_cycle(g, o->cycle);
+    tl_bytes = nullptr;
+    g->footprint += bytes;
+    if (rc) return rc;
+    // the captured iteration segments launch the previous cycle's kernels
+    for (auto& e : g->seg)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+  }
+  const int cycle = g->has_hier ? o->cycle : g->opt.cycle;
   g->opt = *o;
+  g->opt.cycle = cycle;
   return CFD_OK;
 }
 
@@ -692,6 +974,25 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
     return CFD_EINVAL;
   }
   const HostGrid& h = g->host;
+  if (g->opt.cycle != CFD_CYCLE_GEOMETRIC) {
+    // MultigridStats of the aggregation hierarchy (masked.rs:286-290): unknowns and
+    // stored entries (off-diagonal plus diagonal) per level
+    const AggHierarchy& H = g->agg;
+    int L = 0;
+    int64_t bytes = 0;
+    for (const AggLevel& lv : H.levels) {
+      if (L >= 32) break;
+      s->level_cells[L] = lv.n;
+      s->level_nonzeros[L] = L == 0 ? h.fluid_count + (int64_t)H.level0_entries : (int64_t)lv.ptr.back() + lv.n;
+      if (L > 0) bytes += lv.n * (int64_t)(3 * sizeof(int) + 4 * sizeof(double) + 4 * sizeof(float)) +
+                         (int64_t)lv.ptr.back() * (int64_t)(sizeof(int) + sizeof(double) + sizeof(float));
+      ++L;
+    }
+    s->levels = L;
+    s->allocated_bytes = bytes + h.count * (int64_t)(sizeof(int) + sizeof(float)) +
+                         (int64_t)H.block_G.size() * (int64_t)sizeof(double);
+    return CFD_OK;
+  }
   int64_t fine_nnz = h.count;
   for (uint8_t c : h.codes)
     for (int b = 0; b < 6; ++b) fine_nnz += (c >> b) & 1;
@@ -837,7 +1138,7 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
     set_error("cfd_profile_kernel: kernel must be 0..4 and reps positive");
     return CFD_EINVAL;
   }
-  if ((kernel == 1 || kernel == 4) && (!g->has_hier || g->h64.levels.empty())) {
+  if ((kernel == 1 || kernel == 4) && !g->has_hier) {
     set_error("cfd_profile_kernel: the grid has no multigrid hierarchy");
     return CFD_EINVAL;
   }
@@ -865,9 +1166,9 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
         break;
       case 1:
         if (mixed)
-          launch_fine_gs<float, double>(g->geo, g->codes.p, g->r.p, g->h32.x0, 1, g->h32.levels[0].g, g->stream);
+          launch_fine_gs<float, double>(g->geo, g->codes.p, g->r.p, g->x0f.p, 1, Geo{}, g->stream);
         else
-          launch_fine_gs<double, double>(g->geo, g->codes.p, g->r.p, g->z.p, 1, g->h64.levels[0].g, g->stream);
+          launch_fine_gs<double, double>(g->geo, g->codes.p, g->r.p, g->z.p, 1, Geo{}, g->stream);
         break;
       case 2:
         launch_pcg_axpy(N, g->p0.p, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);

@@ -73,4 +73,42 @@ struct DevLevel {
   T* xo;  // iterate after post-smoothing (fused cycle; equal to x on the coarsest level)
 };
 
+// One coarse level of the aggregation hierarchy on the device: a CSR matrix over
+// aggregates. In the colour-sorted numbering the rows of colour c are the range
+// [cptr[c], cptr[c+1]) and form an independent set.
+constexpr int kMaxColours = 16;
+template <class T>
+struct AggDevLevel {
+  int n;
+  int ncol;
+  int cptr[kMaxColours + 1];
+  const int* ptr;  // off-diagonal entries of each row
+  const int* col;
+  const T* val;
+  const T* diag;  // 0: a node without couplings (its value stays 0)
+  T* b;
+  T* x;
+  const int* parent;     // node of the next level (null at the coarsest)
+  const int* child_ptr;  // finer nodes of each node
+  const int* child;
+};
+
+// Coarsest level: one dense block per connected component. Blocks of at most
+// 64 nodes take one wavefront each; larger ones ("big", listed separately) one
+// wavefront per row.
+constexpr int kSmallBlock = 64;
+struct AggCoarsest {
+  int nblocks;
+  const int* bptr;        // members of block k: rows[bptr[k] .. bptr[k+1])
+  const int* rows;
+  const long long* goff;  // block k: G[goff[k] ..] row-major m x m
+  const double* G;
+  int nbig;
+  int big_rows;           // rows of all big blocks
+  const int* big_block;   // block index of big block b
+  const int* big_first;   // first row (over all big blocks) of big block b, nbig + 1 entries
+  int nsmall;
+  const int* small_block;  // block indices of the small blocks
+};
+
 }  // namespace cfdgpu

@@ -1,0 +1,536 @@
+// Host set-up of the aggregation multigrid hierarchy: the reference's masked
+// hierarchy (multigrid/masked.rs:181-291) built from the per-cell coupling
+// codes instead of a face list.
+//
+// Level 0 holds the fluid cells in index order. Each coarser level groups the
+// nodes of a 2x2x2 bin of coordinates into aggregates, one per connected group
+// inside the bin (masked.rs:127-156), numbered by their lowest member; the
+// coarse couplings sum the finer ones between distinct aggregates in the finer
+// level's edge order, and anchors (outlet weights) add up. Coarsening stops
+// when every connected component has at most 32 nodes; those are solved
+// densely (masked.rs:21-51).
+//
+// The device cycle uses this hierarchy in two layouts: the reference's node
+// numbering (the lexicographic cycle) and a colour-sorted numbering in which
+// the nodes of one independent set are contiguous (the red-black cycle).
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+
+#include "internal.hpp"
+
+namespace cfdgpu {
+namespace {
+
+
+struct UnionFind {
+  std::vector<int32_t> p;
+  explicit UnionFind(int64_t n) : p((size_t)n) { std::iota(p.begin(), p.end(), 0); }
+  int32_t find(int32_t x) {
+    while (p[(size_t)x] != x) {
+      p[(size_t)x] = p[(size_t)p[(size_t)x]];
+      x = p[(size_t)x];
+    }
+    return x;
+  }
+  void unite(int32_t a, int32_t b) {
+    a = find(a);
+    b = find(b);
+    if (a == b) return;
+    if (a < b)
+      p[(size_t)b] = a;
+    else
+      p[(size_t)a] = b;
+  }
+};
+
+// Visits the faces of the grid in the reference's order (x faces, then y, then
+// z; each block x-fastest over its face dimensions, grid.rs:341-429): `edge(lo,
+// hi, w)` for an interior face joining two distinct fluid cells, `outlet(c, w)`
+// for an outlet face of cell c.
+template <class E, class O>
+void for_each_face(const HostGrid& g, E edge, O outlet) {
+  const int64_t* n = g.n;
+  const int64_t st[3] = {1, n[0], n[0] * n[1]};
+  for (int a = 0; a < 3; ++a) {
+    int64_t fd[3] = {n[0], n[1], n[2]};
+    fd[a] = g.periodic[a] ? n[a] : n[a] + 1;
+    const uint8_t lo_bit = side_bit(a, 0), hi_bit = side_bit(a, 1);
+    for (int64_t k = 0; k < fd[2]; ++k)
+      for (int64_t j = 0; j < fd[1]; ++j)
+        for (int64_t i = 0; i < fd[0]; ++i) {
+          const int64_t q[3] = {i, j, k};
+          const int64_t f = q[a];
+          const int64_t base = i * (a == 0 ? 0 : 1) + j * st[1] * (a == 1 ? 0 : 1) + k * st[2] * (a == 2 ? 0 : 1);
+          const int64_t hi = f < n[a] ? base + f * st[a] : -1;
+          int64_t lo = -1;
+          if (f > 0)
+            lo = base + (f - 1) * st[a];
+          else if (g.periodic[a])
+            lo = base + (n[a] - 1) * st[a];
+          if (lo >= 0 && hi >= 0) {
+            if (lo != hi && (g.codes[(size_t)lo] & hi_bit)) edge(lo, hi, g.w_int[a]);
+          } else if (hi >= 0) {
+            if (g.codes[(size_t)hi] & lo_bit) outlet(hi, g.w_out[a]);
+          } else if (lo >= 0) {
+            if (g.codes[(size_t)lo] & hi_bit) outlet(lo, g.w_out[a]);
+          }
+        }
+  }
+}
+
+// Connected components of a level by breadth-first search from the lowest
+// unvisited node (masked.rs:90-126); members are stored in ascending order.
+void level_components(const AggLevel& L, std::vector<std::vector<int32_t>>& comps, std::vector<uint8_t>& anchored) {
+  std::vector<uint8_t> seen((size_t)L.n, 0);
+  std::vector<int32_t> queue;
+  for (int64_t s = 0; s < L.n; ++s) {
+    if (seen[(size_t)s]) continue;
+    seen[(size_t)s] = 1;
+    queue.assign(1, (int32_t)s);
+    bool fixed = false;
+    for (size_t head = 0; head < queue.size(); ++head) {
+      const int32_t i = queue[head];
+      fixed |= L.anchor[(size_t)i] > 0.0;
+      for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
+        const int32_t j = L.col[(size_t)e];
+        if (!seen[(size_t)j]) {
+          seen[(size_t)j] = 1;
+          queue.push_back(j);
+        }
+      }
+    }
+    std::sort(queue.begin(), queue.end());
+    comps.push_back(queue);
+    anchored.push_back(fixed ? 1 : 0);
+  }
+}
+
+// CSR of a level from its edge list (a < b, sorted by (a, b)) and anchors, in
+// the entry order Matrix::build gives (multigrid.rs:225-262): row r lists the
+// edges (x, r) by x, then (r, y) by y, so columns ascend; the diagonal is the
+// anchor plus the weights in that order.
+void csr_from_edges(AggLevel& L) {
+  const int64_t n = L.n;
+  L.ptr.assign((size_t)n + 1, 0);
+  for (size_t e = 0; e < L.ea.size(); ++e) {
+    L.ptr[(size_t)L.ea[e] + 1] += 1;
+    L.ptr[(size_t)L.eb[e] + 1] += 1;
+  }
+  for (int64_t i = 0; i < n; ++i) L.ptr[(size_t)i + 1] += L.ptr[(size_t)i];
+  L.col.assign((size_t)L.ptr[(size_t)n], 0);
+  L.val.assign((size_t)L.ptr[(size_t)n], 0.0);
+  std::vector<int64_t> next(L.ptr.begin(), L.ptr.end() - 1);
+  L.diag = L.anchor;
+  for (size_t e = 0; e < L.ea.size(); ++e) {
+    const int32_t a = L.ea[e], b = L.eb[e];
+    const double w = L.ew[e];
+    L.diag[(size_t)a] += w;
+    L.col[(size_t)next[(size_t)a]] = b;
+    L.val[(size_t)next[(size_t)a]++] = -w;
+    L.diag[(size_t)b] += w;
+    L.col[(size_t)next[(size_t)b]] = a;
+    L.val[(size_t)next[(size_t)b]++] = -w;
+  }
+}
+
+// CSR of level 0 with its entries in face order (each face adds an entry to
+// both of its rows, Matrix::build); the diagonal is already set.
+void csr_level0(AggLevel& L) {
+  const int64_t n = L.n;
+  L.ptr.assign((size_t)n + 1, 0);
+  for (size_t e = 0; e < L.ea.size(); ++e) {
+    L.ptr[(size_t)L.ea[e] + 1] += 1;
+    L.ptr[(size_t)L.eb[e] + 1] += 1;
+  }
+  for (int64_t i = 0; i < n; ++i) L.ptr[(size_t)i + 1] += L.ptr[(size_t)i];
+  L.col.assign((size_t)L.ptr[(size_t)n], 0);
+  L.val.assign((size_t)L.ptr[(size_t)n], 0.0);
+  std::vector<int64_t> next(L.ptr.begin(), L.ptr.end() - 1);
+  for (size_t e = 0; e < L.ea.size(); ++e) {
+    const int32_t rows[2] = {L.ea[e], L.eb[e]};
+    for (int s = 0; s < 2; ++s) {
+      const int32_t r = rows[s];
+      L.col[(size_t)next[(size_t)r]] = rows[1 - s];
+      L.val[(size_t)next[(size_t)r]++] = -L.ew[e];
+    }
+  }
+}
+
+// Children lists: the finer nodes of each aggregate, ascending.
+void build_children(AggHierarchy& h) {
+  for (size_t l = 1; l < h.levels.size(); ++l) {
+    AggLevel& C = h.levels[l];
+    const AggLevel& F = h.levels[l - 1];
+    C.child_ptr.assign((size_t)C.n + 1, 0);
+    for (int32_t p : F.parent) C.child_ptr[(size_t)p + 1] += 1;
+    for (int64_t i = 0; i < C.n; ++i) C.child_ptr[(size_t)i + 1] += C.child_ptr[(size_t)i];
+    C.child.assign((size_t)F.n, 0);
+    std::vector<int64_t> at(C.child_ptr.begin(), C.child_ptr.end() - 1);
+    for (int64_t i = 0; i < F.n; ++i) C.child[(size_t)at[(size_t)F.parent[(size_t)i]]++] = (int32_t)i;
+  }
+}
+
+// Sums fine couplings between distinct aggregates into the coarse edge list,
+// keyed by (min, max) and accumulated in the order the fine edges come.
+struct EdgeSums {
+  std::vector<std::vector<std::pair<int32_t, double>>> rows;
+  explicit EdgeSums(int64_t n) : rows((size_t)n) {}
+  void add(int32_t i, int32_t j, double w) {
+    if (i == j) return;
+    const int32_t a = std::min(i, j), b = std::max(i, j);
+    auto& r = rows[(size_t)a];
+    for (auto& e : r)
+      if (e.first == b) {
+        e.second += w;
+        return;
+      }
+    r.emplace_back(b, w);
+  }
+  void emit(AggLevel& C) {
+    for (size_t a = 0; a < rows.size(); ++a) {
+      auto& r = rows[a];
+      std::sort(r.begin(), r.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      for (auto& e : r) {
+        C.ea.push_back((int32_t)a);
+        C.eb.push_back(e.first);
+        C.ew.push_back(e.second);
+      }
+      std::vector<std::pair<int32_t, double>>().swap(r);
+    }
+  }
+};
+
+// Groups the nodes of level L into aggregates: bins of coordinates >> shift,
+// split into connected groups, numbered by their lowest member.
+int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, int shift,
+                  const std::function<void(const std::function<void(int32_t, int32_t)>&)>& edges,
+                  std::vector<int32_t>& parent, std::vector<int32_t>& coarse_coords) {
+  UnionFind uf(n);
+  auto same_bin = [&](int32_t i, int32_t j) {
+    for (int d = 0; d < 3; ++d)
+      if ((coords[3 * (size_t)i + d] >> shift) != (coords[3 * (size_t)j + d] >> shift)) return false;
+    return true;
+  };
+  edges([&](int32_t i, int32_t j) {
+    if (same_bin(i, j)) uf.unite(i, j);
+  });
+  parent.assign((size_t)n, -1);
+  std::vector<int32_t> id_of_root((size_t)n, -1);
+  coarse_coords.clear();
+  int64_t m = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t r = uf.find((int32_t)i);
+    if (id_of_root[(size_t)r] < 0) {
+      id_of_root[(size_t)r] = (int32_t)m++;
+      for (int d = 0; d < 3; ++d) coarse_coords.push_back(coords[3 * (size_t)i + d] >> shift);
+    }
+    parent[(size_t)i] = id_of_root[(size_t)r];
+  }
+  return m;
+}
+
+// The component's matrix without its first member when it is closed (the
+// gauge that Direct::build fixes, masked.rs:21-51): k x k, row-major.
+std::vector<double> assemble_block(const AggLevel& L, const std::vector<int32_t>& members, bool anchored) {
+  const int64_t m = (int64_t)members.size();
+  const int64_t skip = anchored ? 0 : 1;
+  const int64_t k = m - skip;
+  std::vector<double> K((size_t)(k * k), 0.0);
+  for (int64_t r = 0; r < k; ++r) {
+    const int32_t row = members[(size_t)(r + skip)];
+    K[(size_t)(r * k + r)] = L.diag[(size_t)row];
+    for (int64_t e = L.ptr[(size_t)row]; e < L.ptr[(size_t)row + 1]; ++e) {
+      auto it = std::lower_bound(members.begin() + skip, members.end(), L.col[(size_t)e]);
+      if (it != members.end() && *it == L.col[(size_t)e])
+        K[(size_t)(r * k + (it - members.begin() - skip))] += L.val[(size_t)e];
+    }
+  }
+  return K;
+}
+
+// Dense block of the coarsest level for one component: the inverse of the
+// component's matrix (anchored) or, for a closed component, the map
+// rhs -> centered solution with the first member held at zero (masked.rs:21-51,
+// 82-89, 330-336): G = C E K^-1 E^T C.
+int coarsest_block(const AggLevel& L, const std::vector<int32_t>& members, bool anchored, std::vector<double>& G) {
+  const int64_t m = (int64_t)members.size();
+  const int64_t skip = anchored ? 0 : 1;
+  const int64_t k = m - skip;
+  std::vector<double> K = assemble_block(L, members, anchored);
+  // Cholesky
+  for (int64_t i = 0; i < k; ++i)
+    for (int64_t j = 0; j <= i; ++j) {
+      double v = K[(size_t)(i * k + j)];
+      for (int64_t t = 0; t < j; ++t) v -= K[(size_t)(i * k + t)] * K[(size_t)(j * k + t)];
+      if (i == j) {
+        if (!std::isfinite(v) || v <= 0.0) {
+          set_error("CFD-E035: numerics.pressure_preconditioner: masked coarse Cholesky lost positive definiteness");
+          return CFD_E035;
+        }
+        K[(size_t)(i * k + i)] = std::sqrt(v);
+      } else {
+        K[(size_t)(i * k + j)] = v / K[(size_t)(j * k + j)];
+      }
+    }
+  G.assign((size_t)(m * m), 0.0);
+  std::vector<double> e((size_t)k);
+  for (int64_t c = 0; c < k; ++c) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[(size_t)c] = 1.0;
+    for (int64_t i = 0; i < k; ++i) {
+      double v = e[(size_t)i];
+      for (int64_t t = 0; t < i; ++t) v -= K[(size_t)(i * k + t)] * e[(size_t)t];
+      e[(size_t)i] = v / K[(size_t)(i * k + i)];
+    }
+    for (int64_t i = k - 1; i >= 0; --i) {
+      double v = e[(size_t)i];
+      for (int64_t t = i + 1; t < k; ++t) v -= K[(size_t)(t * k + i)] * e[(size_t)t];
+      e[(size_t)i] = v / K[(size_t)(i * k + i)];
+    }
+    for (int64_t i = 0; i < k; ++i) G[(size_t)((i + skip) * m + c + skip)] = e[(size_t)i];
+  }
+  if (!anchored) {
+    const double inv = 1.0 / (double)m;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int64_t o = 0; o < m; ++o) {
+        double mean = 0.0;
+        for (int64_t i = 0; i < m; ++i) mean += pass == 0 ? G[(size_t)(i * m + o)] : G[(size_t)(o * m + i)];
+        mean *= inv;
+        for (int64_t i = 0; i < m; ++i) (pass == 0 ? G[(size_t)(i * m + o)] : G[(size_t)(o * m + i)]) -= mean;
+      }
+  }
+  return 0;
+}
+
+}  // namespace
+
+int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells, int64_t host_block_max) {
+  h = AggHierarchy();
+  if (g.fluid_count == 0) {
+    set_error("CFD-E034: numerics.pressure_preconditioner: multigrid requires fluid cells");
+    return CFD_E034;
+  }
+  const int64_t* n = g.n;
+  // Level 0: fluid cells in index order.
+  h.node_of_cell.assign((size_t)g.count, -1);
+  AggLevel L0;
+  for (int64_t c = 0; c < g.count; ++c)
+    if (g.fluid[(size_t)c]) {
+      h.node_of_cell[(size_t)c] = (int32_t)L0.n;
+      h.cell_of_node.push_back(c);
+      L0.coords.push_back((int32_t)(c % n[0]));
+      L0.coords.push_back((int32_t)(c / n[0] % n[1]));
+      L0.coords.push_back((int32_t)(c / (n[0] * n[1])));
+      ++L0.n;
+    }
+  L0.anchor.assign((size_t)L0.n, 0.0);
+  const auto& node = h.node_of_cell;
+  for_each_face(
+      g, [&](int64_t lo, int64_t hi, double w) {
+        L0.ea.push_back(node[(size_t)lo]);
+        L0.eb.push_back(node[(size_t)hi]);
+        L0.ew.push_back(w);
+      },
+      [&](int64_t c, double w) { L0.anchor[(size_t)node[(size_t)c]] += w; });
+  h.level0_entries = 2 * (int64_t)L0.ea.size();
+  // Level 0's diagonal: anchors, then the face weights in face order.
+  L0.diag = L0.anchor;
+  for (size_t e = 0; e < L0.ea.size(); ++e) {
+    L0.diag[(size_t)L0.ea[e]] += L0.ew[e];
+    L0.diag[(size_t)L0.eb[e]] += L0.ew[e];
+  }
+  h.levels.push_back(std::move(L0));
+  for (;;) {
+    AggLevel& L = h.levels.back();
+    const bool fine = h.levels.size() == 1;
+    std::vector<std::vector<int32_t>> comps;
+    std::vector<uint8_t> anch;
+    if (fine) {
+      // the grid's own components (the same BFS numbering), in node numbers
+      comps.resize(g.anchored.size());
+      for (int64_t c = 0; c < g.count; ++c)
+        if (g.component[(size_t)c] >= 0) comps[(size_t)g.component[(size_t)c]].push_back(node[(size_t)c]);
+      anch.assign(g.anchored.begin(), g.anchored.end());
+      bool small = true;
+      for (auto& m : comps) small &= (int64_t)m.size() <= coarse_cells;
+      if (small) csr_level0(L);
+    } else {
+      csr_from_edges(L);
+      level_components(L, comps, anch);
+    }
+
+    L.closed_ptr.assign(1, 0);
+    bool small = true;
+    for (size_t r = 0; r < comps.size(); ++r) {
+      small &= (int64_t)comps[r].size() <= coarse_cells;
+      if (!anch[r]) {
+        L.closed_members.insert(L.closed_members.end(), comps[r].begin(), comps[r].end());
+        L.closed_ptr.push_back((int64_t)L.closed_members.size());
+      }
+    }
+    if (small) {
+      // Coarsest: one dense block per component.
+      h.block_ptr.assign(1, 0);
+      h.block_goff.assign(1, 0);
+      for (size_t r = 0; r < comps.size(); ++r) {
+        const int64_t m = (int64_t)comps[r].size();
+        if (m <= host_block_max) {
+          std::vector<double> G;
+          int e = coarsest_block(L, comps[r], anch[r] != 0, G);
+          if (e) return e;
+          h.block_G.insert(h.block_G.end(), G.begin(), G.end());
+        } else {
+          // inverted on the device; its slot stays zero until then
+          AggHierarchy::Deferred d;
+          d.block = (int64_t)r;
+          d.m = m;
+          d.anchored = anch[r] != 0;
+          d.K = assemble_block(L, comps[r], d.anchored);
+          h.deferred.push_back(std::move(d));
+          h.block_G.resize(h.block_G.size() + (size_t)(m * m), 0.0);
+        }
+        h.block_rows.insert(h.block_rows.end(), comps[r].begin(), comps[r].end());
+        h.block_ptr.push_back((int64_t)h.block_rows.size());
+        h.block_goff.push_back((int64_t)h.block_G.size());
+      }
+      break;
+    }
+    // Aggregate; grow the bins while nothing reduces (masked.rs:262-273).
+    std::vector<int32_t> parent, cc;
+    int64_t m = 0;
+    int shift = 1;
+    const auto edges = [&](const std::function<void(int32_t, int32_t)>& f) {
+      for (size_t e = 0; e < L.ea.size(); ++e) f(L.ea[e], L.eb[e]);
+    };
+    for (;;) {
+      m = aggregate(L.n, L.coords, shift, edges, parent, cc);
+      if (m < L.n) break;
+      bool all_zero = true;
+      for (int32_t v : L.coords) all_zero &= (v >> shift) == 0;
+      if (all_zero) {
+        set_error("CFD-E035: numerics.pressure_preconditioner: masked aggregation failed to reduce a connected component");
+        return CFD_E035;
+      }
+      ++shift;
+    }
+    AggLevel C;
+    C.n = m;
+    C.coords = std::move(cc);
+    C.anchor.assign((size_t)m, 0.0);
+    for (int64_t i = 0; i < L.n; ++i) C.anchor[(size_t)parent[(size_t)i]] += L.anchor[(size_t)i];
+    EdgeSums sums(m);
+    for (size_t e = 0; e < L.ea.size(); ++e) sums.add(parent[(size_t)L.ea[e]], parent[(size_t)L.eb[e]], L.ew[e]);
+    sums.emit(C);
+    L.parent = std::move(parent);
+    // the finer edge list is not needed once the next level exists
+    std::vector<int32_t>().swap(L.ea);
+    std::vector<int32_t>().swap(L.eb);
+    std::vector<double>().swap(L.ew);
+    h.levels.push_back(std::move(C));
+  }
+  std::vector<int32_t>().swap(h.levels.back().ea);
+  std::vector<int32_t>().swap(h.levels.back().eb);
+  std::vector<double>().swap(h.levels.back().ew);
+  build_children(h);
+  return 0;
+}
+
+void colour_agg_hierarchy(AggHierarchy& h) {
+  const size_t nl = h.levels.size();
+  h.permuted = true;
+  for (size_t l = 1; l < nl; ++l) {
+    AggLevel& L = h.levels[l];
+    const int64_t n = L.n;
+    // Parity of the bin coordinates separates coupled aggregates (they lie in
+    // face-adjacent bins) except across an odd periodic seam; fall back to a
+    // greedy colouring in node order where it fails.
+    std::vector<int32_t> colour((size_t)n);
+    bool ok = true;
+    for (int64_t i = 0; i < n; ++i)
+      colour[(size_t)i] = (L.coords[3 * (size_t)i] + L.coords[3 * (size_t)i + 1] + L.coords[3 * (size_t)i + 2]) & 1;
+    for (int64_t i = 0; i < n && ok; ++i)
+      for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
+        if (colour[(size_t)L.col[(size_t)e]] == colour[(size_t)i]) {
+          ok = false;
+          break;
+        }
+    int ncol = 2;
+    if (!ok) {
+      ncol = 1;
+      std::fill(colour.begin(), colour.end(), -1);
+      std::vector<uint8_t> used;
+      for (int64_t i = 0; i < n; ++i) {
+        used.assign((size_t)ncol + 1, 0);
+        for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
+          const int32_t c = colour[(size_t)L.col[(size_t)e]];
+          if (c >= 0) used[(size_t)c] = 1;
+        }
+        int c = 0;
+        while (used[(size_t)c]) ++c;
+        colour[(size_t)i] = c;
+        ncol = std::max(ncol, c + 1);
+      }
+    }
+    // new numbering: colour classes in order, ascending old index inside a class
+    std::vector<int32_t> perm((size_t)n);
+    L.colour_ptr.assign((size_t)ncol + 1, 0);
+    for (int64_t i = 0; i < n; ++i) L.colour_ptr[(size_t)colour[(size_t)i] + 1] += 1;
+    for (int c = 0; c < ncol; ++c) L.colour_ptr[(size_t)c + 1] += L.colour_ptr[(size_t)c];
+    {
+      std::vector<int64_t> at(L.colour_ptr.begin(), L.colour_ptr.end() - 1);
+      for (int64_t i = 0; i < n; ++i) perm[(size_t)i] = (int32_t)at[(size_t)colour[(size_t)i]]++;
+    }
+    // permute this level's rows, columns, diagonal, anchors, coordinates and parent
+    std::vector<int64_t> ptr((size_t)n + 1, 0);
+    std::vector<int32_t> inv((size_t)n);
+    for (int64_t i = 0; i < n; ++i) inv[(size_t)perm[(size_t)i]] = (int32_t)i;
+    for (int64_t r = 0; r < n; ++r) {
+      const int32_t o = inv[(size_t)r];
+      ptr[(size_t)r + 1] = ptr[(size_t)r] + (L.ptr[(size_t)o + 1] - L.ptr[(size_t)o]);
+    }
+    std::vector<int32_t> col((size_t)ptr[(size_t)n]);
+    std::vector<double> val((size_t)ptr[(size_t)n]), diag((size_t)n), anchor((size_t)n);
+    std::vector<int32_t> coords((size_t)(3 * n));
+    std::vector<int32_t> parent(L.parent.empty() ? 0 : (size_t)n);
+    for (int64_t r = 0; r < n; ++r) {
+      const int32_t o = inv[(size_t)r];
+      int64_t at = ptr[(size_t)r];
+      for (int64_t e = L.ptr[(size_t)o]; e < L.ptr[(size_t)o + 1]; ++e, ++at) {
+        col[(size_t)at] = perm[(size_t)L.col[(size_t)e]];
+        val[(size_t)at] = L.val[(size_t)e];
+      }
+      diag[(size_t)r] = L.diag[(size_t)o];
+      anchor[(size_t)r] = L.anchor[(size_t)o];
+      for (int d = 0; d < 3; ++d) coords[3 * (size_t)r + d] = L.coords[3 * (size_t)o + d];
+      if (!parent.empty()) parent[(size_t)r] = L.parent[(size_t)o];
+    }
+    L.ptr.swap(ptr);
+    L.col.swap(col);
+    L.val.swap(val);
+    L.diag.swap(diag);
+    L.anchor.swap(anchor);
+    L.coords.swap(coords);
+    L.parent.swap(parent);
+    for (auto& m : L.closed_members) m = perm[(size_t)m];
+    for (size_t c = 0; c + 1 < L.closed_ptr.size(); ++c)
+      std::sort(L.closed_members.begin() + L.closed_ptr[c], L.closed_members.begin() + L.closed_ptr[c + 1]);
+    // the finer level's parents point at the new numbers
+    for (auto& p : h.levels[l - 1].parent) p = perm[(size_t)p];
+    if (l + 1 == nl)
+      for (auto& r : h.block_rows) r = perm[(size_t)r];
+  }
+  build_children(h);
+}
+
+void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std::vector<int64_t>& nonzeros) {
+  cells.clear();
+  nonzeros.clear();
+  for (const auto& L : h.levels) {
+    cells.push_back(L.n);
+    nonzeros.push_back((int64_t)L.col.size() + L.n);
+  }
+}
+
+}  // namespace cfdgpu

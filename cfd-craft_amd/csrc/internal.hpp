@@ -70,4 +70,51 @@ struct HostHierarchy {
 };
 int build_hierarchy(const HostGrid& g, HostHierarchy& h, int64_t coarse_max);
 
+// Aggregation hierarchy (multigrid/masked.rs): level 0 = fluid cells in index
+// order, coarser levels = connected groups of 2x2x2 bins of coordinates.
+struct AggLevel {
+  int64_t n = 0;
+  std::vector<int64_t> ptr;  // off-diagonal entries per row (CSR)
+  std::vector<int32_t> col;
+  std::vector<double> val;     // -w
+  std::vector<double> diag;    // anchor + sum of w in entry order
+  std::vector<double> anchor;  // outlet weights
+  std::vector<int32_t> coords;  // 3 per node: bin coordinates
+  std::vector<int64_t> closed_ptr;  // closed components, members ascending
+  std::vector<int32_t> closed_members;
+  std::vector<int32_t> parent;  // node of the next level (empty at the coarsest)
+  std::vector<int64_t> child_ptr;  // finer nodes of each node (levels >= 1)
+  std::vector<int32_t> child;
+  std::vector<int64_t> colour_ptr;  // colour-sorted layout: independent sets
+  // edge list (a, b, w) while the next level is built
+  std::vector<int32_t> ea, eb;
+  std::vector<double> ew;
+};
+struct AggHierarchy {
+  std::vector<int32_t> node_of_cell;  // level-0 node of a cell, -1 for solid
+  std::vector<int64_t> cell_of_node;
+  int64_t level0_entries = 0;  // off-diagonal entries of level 0 (two per interior face)
+  std::vector<AggLevel> levels;
+  // coarsest level: per component, its members and a dense m x m block
+  std::vector<int64_t> block_ptr, block_goff;
+  std::vector<int32_t> block_rows;
+  std::vector<double> block_G;
+  // blocks too large to invert on the host: the component's matrix (without the
+  // first member of a closed component), inverted on the device
+  struct Deferred {
+    int64_t block = 0, m = 0;
+    bool anchored = true;
+    std::vector<double> K;
+  };
+  std::vector<Deferred> deferred;
+  bool permuted = false;
+};
+// Coarsening stops when every component has at most coarse_cells nodes (the
+// reference: 32); blocks above host_block_max nodes are left to the device.
+int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
+                        int64_t host_block_max = INT64_MAX);
+// Renumbers levels >= 1 so that each colour class (an independent set) is contiguous.
+void colour_agg_hierarchy(AggHierarchy& h);
+void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std::vector<int64_t>& nonzeros);
+
 }  // namespace cfdgpu

@@ -8,6 +8,8 @@
 // operation, so their results are bitwise equal to the serial CPU code.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -559,6 +561,26 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 }
 
 // ---------------------------------------------------------------- multigrid (geometric)
+// Colour of a fine cell: its parity, and with `seams` (a bit per periodic axis
+// of odd length) two more classes per seam for the cells on the last slab of
+// that axis, whose wrap neighbour has the same parity.
+__device__ __forceinline__ int fine_colour(const Geo& g, int i, int j, int k, int seams) {
+  int c = (i + j + k) & 1;
+  if (seams) {
+    // one bit per seam axis, packed in axis order
+    const int q[3] = {i, j, k};
+    int s = 0, bit = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if ((seams >> a) & 1) {
+        if (q[a] == g.n[a] - 1) s |= 1 << bit;
+        ++bit;
+      }
+    c += 2 * s;
+  }
+  return c;
+}
+
 // Red-black Gauss-Seidel half-sweep on the fine level from the coupling codes.
 // `first`: x starts at zero (neighbours contribute nothing). `xc`: fuse the
 // prolongation x += P e of the coarse correction into the neighbour reads (the
@@ -568,7 +590,7 @@ template <class T, class TB>
 __global__ void __launch_bounds__(BS)
     k_fine_gs(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b, T* __restrict__ x, int color, int first,
               const T* __restrict__ xc, Geo gc, double* __restrict__ out64, int all_cells, const PcgState* st,
-              T cscale = T(1)) {
+              T cscale = T(1), const int* __restrict__ fparent = nullptr, int seams = 0) {
   long long t = (long long)blockIdx.x * BS + threadIdx.x;
   int i, j, k;
   if (all_cells) {
@@ -592,7 +614,7 @@ __global__ void __launch_bounds__(BS)
   }
   const long long c = (long long)i + j * g.sy + k * g.sz;
   const int status = st ? st->status : ST_RUNNING;
-  if (((i + j + k) & 1) != color) {
+  if (fine_colour(g, i, j, k, seams) != color) {
     if (out64 && status == ST_RUNNING) out64[c] = (double)x[c];
     return;
   }
@@ -610,9 +632,14 @@ __global__ void __launch_bounds__(BS)
         const long long nb = nb_or_self(g, c, a, q[a], side);
         v = x[nb];
         if (xc) {
-          int qn[3] = {i, j, k};
-          qn[a] = side == 0 ? (q[a] > 0 ? q[a] - 1 : g.n[a] - 1) : (q[a] < g.n[a] - 1 ? q[a] + 1 : 0);
-          v += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+          if (fparent) {
+            const int pn = fparent[nb];
+            if (pn >= 0) v += cscale * xc[pn];
+          } else {
+            int qn[3] = {i, j, k};
+            qn[a] = side == 0 ? (q[a] > 0 ? q[a] - 1 : g.n[a] - 1) : (q[a] < g.n[a] - 1 ? q[a] + 1 : 0);
+            v += cscale * xc[(qn[0] >> 1) + (qn[1] >> 1) * gc.sy + (qn[2] >> 1) * gc.sz];
+          }
         }
       }
       xn[2 * a + side] = v;
@@ -1152,6 +1179,307 @@ __global__ void __launch_bounds__(BS) k_direct(int n, const double* __restrict__
   }
 }
 
+// ---------------------------------------------------------------- multigrid (aggregation)
+// Coarse levels are CSR matrices over aggregates (connected groups of 2x2x2
+// bins, multigrid/masked.rs:127-156). Rows are colour-sorted: one colour class
+// is a contiguous range and an independent set, so a class is relaxed in
+// parallel with the Gauss-Seidel formula x_i = (b_i - sum a_ij x_j) / a_ii. The
+// status word is read first and tested only before the stores, so its latency
+// overlaps the other loads.
+
+// a_ij x_j over row i's off-diagonal entries, in entry order; the first eight
+// entries are loaded together (predicated), so a row costs one load round trip.
+constexpr int kRowPre = 8;
+// With xc, each neighbour value carries the next level's correction
+// x_j + cs * xc[parent_j] (the prolongation riding on the first post-sweep).
+template <class T>
+__device__ __forceinline__ T agg_offdiag(const AggDevLevel<T>& L, int e0, int e1, T v, bool subtract,
+                                         const T* __restrict__ xc = nullptr, T cs = T(0)) {
+  int cols[kRowPre];
+  T vals[kRowPre], xv[kRowPre];
+#pragma unroll
+  for (int k = 0; k < kRowPre; ++k) {
+    cols[k] = -1;
+    vals[k] = T(0);
+    if (e0 + k < e1) {
+      cols[k] = L.col[e0 + k];
+      vals[k] = L.val[e0 + k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRowPre; ++k) {
+    xv[k] = T(0);
+    if (cols[k] >= 0) {
+      xv[k] = L.x[cols[k]];
+      if (xc) xv[k] += cs * xc[L.parent[cols[k]]];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kRowPre; ++k)
+    if (cols[k] >= 0) {
+      if (subtract)
+        v -= vals[k] * xv[k];
+      else
+        v += vals[k] * xv[k];
+    }
+  for (int e = e0 + kRowPre; e < e1; ++e) {
+    const int j = L.col[e];
+    T xj = L.x[j];
+    if (xc) xj += cs * xc[L.parent[j]];
+    if (subtract)
+      v -= L.val[e] * xj;
+    else
+      v += L.val[e] * xj;
+  }
+  return v;
+}
+
+// b_i - (a_ii x_i + sum a_ij x_j) (Matrix::product, multigrid.rs:263-271)
+template <class T>
+__device__ __forceinline__ T agg_row_residual(const AggDevLevel<T>& L, int i) {
+  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
+  const T d = L.diag[i], xi = L.x[i], bi = L.b[i];
+  return bi - agg_offdiag<T>(L, e0, e1, d * xi, false);
+}
+
+// (b_i - sum a_ij x_j) / a_ii, 0 for a node without couplings (masked.rs:158-172)
+template <class T>
+__device__ __forceinline__ T agg_row_relax(const AggDevLevel<T>& L, int i, const T* __restrict__ xc = nullptr,
+                                           T cs = T(0)) {
+  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
+  const T d = L.diag[i], bi = L.b[i];
+  const T v = agg_offdiag<T>(L, e0, e1, bi, true, xc, cs);
+  return d != T(0) ? v / d : T(0);
+}
+
+// One colour class of a half-sweep. With xc (two-colour levels only) the
+// prolongation of the next level's correction is fused into the neighbour
+// reads: this class is overwritten and the other one is recomputed from scratch
+// by the next half-sweep, so no node needs the correction stored.
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_gs(AggDevLevel<T> L, int colour, const PcgState* st,
+                                               const T* __restrict__ xc = nullptr, T cs = T(0)) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int i = L.cptr[colour] + (int)(blockIdx.x * BS + threadIdx.x);
+  if (i >= L.cptr[colour + 1]) return;
+  const T v = agg_row_relax<T>(L, i, xc, cs);
+  if (status != ST_RUNNING) return;
+  L.x[i] = v;
+}
+
+// Restriction into level C: eight lanes per aggregate each form the residual of
+// one finer node (lane k takes children k, k + 8, ...); the first lane adds the
+// eight partial sums in lane order (ascending children when there are at most
+// eight). The coarse iterate starts at zero.
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  const int J = (int)(t >> 3), sub = (int)(t & 7);
+  T r = T(0);
+  if (J < C.n) {
+    const int k0 = C.child_ptr[J], k1 = C.child_ptr[J + 1];
+    for (int k = k0 + sub; k < k1; k += 8) r += agg_row_residual<T>(F, C.child[k]);
+  }
+  const int base = (int)(threadIdx.x & 63) & ~7;
+  T rs[8];
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) rs[s2] = __shfl(r, base + s2, 64);
+  if (sub != 0 || J >= C.n || status != ST_RUNNING) return;
+  T sum = T(0);
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) sum += rs[s2];
+  C.b[J] = sum;
+  C.x[J] = T(0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_prolong(AggDevLevel<T> F, const T* __restrict__ xc, T cs,
+                                                    const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int i = (int)(blockIdx.x * BS + threadIdx.x);
+  if (i >= F.n) return;
+  const T v = F.x[i] + cs * xc[F.parent[i]];
+  if (status != ST_RUNNING) return;
+  F.x[i] = v;
+}
+
+// Coarsest level: x = G_k b per component block, one wavefront per block.
+template <class TI, class TO>
+__device__ __forceinline__ void agg_direct_block(const AggCoarsest& D, int k, int lane, const TI* __restrict__ b,
+                                                 TO* __restrict__ x, double* __restrict__ out64, bool run) {
+  const int r0 = D.bptr[k], m = D.bptr[k + 1] - r0;
+  if (m > kSmallBlock || lane >= m) return;
+  const double* G = D.G + D.goff[k] + (long long)lane * m;
+  double acc = 0.0;
+  for (int j = 0; j < m; ++j) acc += G[j] * (double)b[D.rows[r0 + j]];
+  if (!run) return;
+  const int row = D.rows[r0 + lane];
+  x[row] = (TO)acc;
+  if (out64) out64[row] = (double)(TO)acc;
+}
+
+// Coarsest solve in one launch: the first big_rows wavefronts take one row of a
+// big block each (lanes striding the columns), the rest one small block each.
+template <class TI, class TO>
+__global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __restrict__ b, TO* __restrict__ x,
+                                                   double* __restrict__ out64, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int w = (int)(blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (w >= D.big_rows) {
+    if (w - D.big_rows < D.nsmall)
+      agg_direct_block<TI, TO>(D, D.small_block[w - D.big_rows], lane, b, x, out64, status == ST_RUNNING);
+    return;
+  }
+  int q = 0;
+  while (D.big_first[q + 1] <= w) ++q;
+  const int k = D.big_block[q];
+  const int r0 = D.bptr[k], m = D.bptr[k + 1] - r0;
+  const int row = w - D.big_first[q];
+  const double* G = D.G + D.goff[k] + (long long)row * m;
+  double acc = 0.0;
+  for (int j = lane; j < m; j += 64) acc += G[j] * (double)b[D.rows[r0 + j]];
+  acc = wave_sum(acc);
+  if (lane != 0 || status != ST_RUNNING) return;
+  const int r = D.rows[r0 + row];
+  x[r] = (TO)acc;
+  if (out64) out64[r] = (double)(TO)acc;
+}
+
+// Set-up of a big coarsest block on the device: in-place Gauss-Jordan inversion
+// of the SPD k x k matrix A (no pivoting needed), one pivot per pair of
+// launches, then embedding into the m x m block (first row and column zero for
+// a closed component) and the symmetric centering C G C.
+__global__ void __launch_bounds__(BS) k_gj_pivot(const double* __restrict__ A, int k, int p, double* __restrict__ row,
+                                                 double* __restrict__ col, int* __restrict__ bad) {
+  const int t = (int)(blockIdx.x * BS + threadIdx.x);
+  if (t >= k) return;
+  row[t] = A[(long long)p * k + t];
+  col[t] = A[(long long)t * k + p];
+  if (t == p) {
+    const double d = A[(long long)p * k + p];
+    if (!(d > 0.0) || !isfinite(d)) *bad = 1;
+  }
+}
+__global__ void __launch_bounds__(BS) k_gj_update(double* __restrict__ A, int k, int p, const double* __restrict__ row,
+                                                  const double* __restrict__ col) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)k * k) return;
+  const int i = (int)(t / k), j = (int)(t - (long long)i * k);
+  const double d = row[p];
+  double v;
+  if (i == p && j == p)
+    v = 1.0 / d;
+  else if (i == p)
+    v = row[j] / d;
+  else if (j == p)
+    v = -col[i] / d;
+  else
+    v = A[t] - col[i] * row[j] / d;
+  A[t] = v;
+}
+__global__ void __launch_bounds__(BS) k_block_embed(double* __restrict__ G, int m, int skip, const double* __restrict__ A) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)m * m) return;
+  const int i = (int)(t / m), j = (int)(t - (long long)i * m);
+  const int k = m - skip;
+  G[t] = (i >= skip && j >= skip) ? A[(long long)(i - skip) * k + (j - skip)] : 0.0;
+}
+__global__ void __launch_bounds__(BS) k_block_colmean(const double* __restrict__ G, int m, double* __restrict__ mu) {
+  const int j = (int)(blockIdx.x * BS + threadIdx.x);
+  if (j >= m) return;
+  double s = 0.0;
+  for (int i = 0; i < m; ++i) s += G[(long long)i * m + j];
+  mu[j] = s / (double)m;
+}
+__global__ void __launch_bounds__(BS) k_block_center(double* __restrict__ G, int m, const double* __restrict__ mu) {
+  __shared__ double sh;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int j = 0; j < m; ++j) s += mu[j];
+    sh = s / (double)m;
+  }
+  __syncthreads();
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  if (t >= (long long)m * m) return;
+  const int i = (int)(t / m), j = (int)(t - (long long)i * m);
+  G[t] = G[t] - mu[i] - mu[j] + sh;
+}
+
+// Fine level -> first coarse level: eight lanes per 2x2x2 bin form the fine
+// residuals; the bin's first lane adds them up per aggregate (a bin holds one
+// aggregate per connected group of its fluid cells) in ascending cell order.
+template <class T, class TB>
+__global__ void __launch_bounds__(BS) k_agg_fine_restrict(Geo g, const uint8_t* __restrict__ codes,
+                                                          const TB* __restrict__ b, const T* __restrict__ x,
+                                                          const int* __restrict__ fparent, T* __restrict__ bc,
+                                                          T* __restrict__ xc, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
+  const unsigned B = t >> 3;
+  const int sub = (int)(t & 7);
+  T r = T(0);
+  int p = -1;
+  if (B < bx * by * bz) {
+    const unsigned Bx = B % bx, Byz = B / bx;
+    const unsigned By = Byz % by, Bz = Byz / by;
+    const int q[3] = {(int)(2 * Bx) + (sub & 1), (int)(2 * By) + ((sub >> 1) & 1), (int)(2 * Bz) + (sub >> 2)};
+    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) {
+      p = fparent[(long long)q[0] + q[1] * g.sy + q[2] * g.sz];
+      r = fine_residual<T, TB>(g, codes, b, x, q);
+    }
+  }
+  const int base = (int)(threadIdx.x & 63) & ~7;
+  T rs[8];
+  int ps[8];
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) {
+    rs[s2] = __shfl(r, base + s2, 64);
+    ps[s2] = __shfl(p, base + s2, 64);
+  }
+  if (sub != 0 || status != ST_RUNNING) return;
+  unsigned done = 0;
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) {
+    if (ps[s2] < 0 || (done & (1u << s2))) continue;
+    T sum = T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u >= s2 && ps[u] == ps[s2]) {
+        sum += rs[u];
+        done |= 1u << u;
+      }
+    bc[ps[s2]] = sum;
+    xc[ps[s2]] = T(0);
+  }
+}
+
+// x += cs * e[parent] on the fine level (used when the fine level has more than
+// two colour classes, so the prolongation cannot ride on the first half-sweep).
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_fine_prolong(long long N, const int* __restrict__ fparent,
+                                                         const T* __restrict__ xc, T cs, T* __restrict__ x,
+                                                         const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long c = (long long)blockIdx.x * BS + threadIdx.x;
+  if (c >= N) return;
+  const int p = fparent[c];
+  if (p < 0) return;
+  const T v = x[c] + cs * xc[p];
+  if (status != ST_RUNNING) return;
+  x[c] = v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(BS) k_fill_t(long long N, T* __restrict__ x, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long c = (long long)blockIdx.x * BS + threadIdx.x;
+  if (c >= N || status != ST_RUNNING) return;
+  x[c] = T(0);
+}
+
 // ---------------------------------------------------------------- projection
 // b = -div(u)/dt and the round-off scale sum(|u_lo| + |u_hi|)/dx/dt, in the
 // axis order of Grid::divergence (pressure.rs:440-459). u holds the x-, y- and
@@ -1340,7 +1668,7 @@ void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int
                        g1, (double*)nullptr, 0, st, cs);
   hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(8 * g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0,
                      g1, lv[0].b, st);
-  const int stop = tail_first < nlev ? tail_first : nlev - 1;
+  const int stop = nlev - 1;
   for (int l = 0; l < stop; ++l) {
     const unsigned cb = color_blocks<T>(lv[l].g);
     const Geo& gn = lv[l + 1].g;
@@ -1441,6 +1769,128 @@ template void launch_vcycle<double, double>(const Geo&, const uint8_t*, const do
 template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, int, DevLevel<float>*,
                                            const DevLevel<float>*, int, int, const double*, double*, const PcgState*,
                                            double, hipStream_t);
+
+// One V(2,2) cycle on the aggregation hierarchy: the fine level from the
+// coupling codes (red-black, or 2^(1+seams) colour classes across odd periodic
+// seams), lv[0..nlev) the coarse levels (lv[nlev-1] solved by the dense
+// blocks), levels from tail_first on inside one workgroup (dlv: device copy of
+// lv). The correction of each level is scaled by omega.
+template <class T, class TB>
+void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
+                       int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
+                       double omega, hipStream_t s) {
+  const T cs = (T)omega;
+  const unsigned ab = blocks_for(g0.N);
+  auto direct = [&](const auto* b, T* x, double* o64) {
+    using TI = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
+    hipLaunchKernelGGL((k_agg_direct<TI, T>), dim3((D.big_rows + D.nsmall + 3) / 4), dim3(BS), 0, s, D, b, x, o64, st);
+  };
+  if (nlev == 0) {
+    // the fine level is the coarsest: solid cells stay zero
+    hipLaunchKernelGGL((k_fill_t<T>), dim3(ab), dim3(BS), 0, s, g0.N, x0, st);
+    if (out64) hipLaunchKernelGGL((k_fill_t<double>), dim3(ab), dim3(BS), 0, s, g0.N, out64, st);
+    direct(b0, x0, out64);
+    return;
+  }
+  auto relax = [&](const AggDevLevel<T>& L, int c) {
+    const int rows = L.cptr[c + 1] - L.cptr[c];
+    if (rows > 0) hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, c, st);
+  };
+  const Geo none{};
+  const T* nox = nullptr;
+  const int fine_colours = 2 << __builtin_popcount((unsigned)seams);
+  const unsigned fb = color_blocks<T>(g0);
+  if (!seams) {
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
+                         none, (double*)nullptr, 0, st, cs, fparent, 0);
+  } else {
+    hipLaunchKernelGGL((k_fill_t<T>), dim3(ab), dim3(BS), 0, s, g0.N, x0, st);
+    for (int round = 0; round < 2; ++round)
+      for (int c = 0; c < fine_colours; ++c)
+        hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(ab), dim3(BS), 0, s, g0, codes, b0, x0, c, 0, nox, none,
+                           (double*)nullptr, 1, st, cs, fparent, seams);
+  }
+  {
+    const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
+    hipLaunchKernelGGL((k_agg_fine_restrict<T, TB>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, codes, b0,
+                       (const T*)x0, fparent, lv[0].b, lv[0].x, st);
+  }
+  const int stop = nlev - 1;
+  for (int l = 0; l < stop; ++l) {
+    const AggDevLevel<T>& L = lv[l];
+    for (int round = 0; round < 2; ++round)
+      for (int c = 0; c < L.ncol; ++c) relax(L, c);
+    hipLaunchKernelGGL((k_agg_restrict<T>), dim3(blocks_for(8LL * lv[l + 1].n)), dim3(BS), 0, s, L, lv[l + 1], st);
+  }
+  direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
+  for (int l = stop - 1; l >= 0; --l) {
+    const AggDevLevel<T>& L = lv[l];
+    const T* xc = lv[l + 1].x;
+    if (L.ncol == 2) {
+      // prolongation fused into the first half-sweep
+      const int rows = L.cptr[2] - L.cptr[1];
+      if (rows > 0)
+        hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, 1, st, xc, cs);
+      else
+        hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
+      relax(L, 0);
+      relax(L, 1);
+      relax(L, 0);
+    } else {
+      hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
+      for (int round = 0; round < 2; ++round)
+        for (int c = L.ncol - 1; c >= 0; --c) relax(L, c);
+    }
+  }
+  // fine post-smoothing; in mixed precision the final half-sweeps also write z in FP64
+  if (!seams) {
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
+                         pass == 0 ? (const T*)lv[0].x : nox, none, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs,
+                         fparent, 0);
+  } else {
+    hipLaunchKernelGGL((k_agg_fine_prolong<T>), dim3(ab), dim3(BS), 0, s, g0.N, fparent, (const T*)lv[0].x, cs, x0, st);
+    for (int round = 0; round < 2; ++round)
+      for (int c = fine_colours - 1; c >= 0; --c)
+        hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(ab), dim3(BS), 0, s, g0, codes, b0, x0, c, 0, nox, none,
+                           (round == 1 && c == 0) ? out64 : (double*)nullptr, 1, st, cs, fparent, seams);
+  }
+}
+template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
+                                                const AggDevLevel<double>*, const AggCoarsest&, double*,
+                                                const PcgState*, double, hipStream_t);
+template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
+                                               const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
+                                               double, hipStream_t);
+
+// Inverts the deferred coarsest blocks on the device (see k_gj_pivot); returns
+// false when a pivot is not positive (the block lost definiteness).
+bool invert_block(double* Gblock, int m, bool anchored, const double* K, double* work, int* bad, hipStream_t s) {
+  const int skip = anchored ? 0 : 1;
+  const int k = m - skip;
+  double* A = work;                        // k * k
+  double* row = work + (long long)k * k;   // k
+  double* col = row + k;                   // k
+  double* mu = col + k;                    // m
+  if (hipMemcpyAsync(A, K, (size_t)k * k * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) return false;
+  if (hipMemsetAsync(bad, 0, sizeof(int), s) != hipSuccess) return false;
+  for (int p = 0; p < k; ++p) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(k)), dim3(BS), 0, s, (const double*)A, k, p, row, col, bad);
+    hipLaunchKernelGGL(k_gj_update, dim3(blocks_for((long long)k * k)), dim3(BS), 0, s, A, k, p, (const double*)row,
+                       (const double*)col);
+  }
+  hipLaunchKernelGGL(k_block_embed, dim3(blocks_for((long long)m * m)), dim3(BS), 0, s, Gblock, m, skip,
+                     (const double*)A);
+  if (!anchored) {
+    hipLaunchKernelGGL(k_block_colmean, dim3(blocks_for(m)), dim3(BS), 0, s, (const double*)Gblock, m, mu);
+    hipLaunchKernelGGL(k_block_center, dim3(blocks_for((long long)m * m)), dim3(BS), 0, s, Gblock, m, (const double*)mu);
+  }
+  int h_bad = 0;
+  if (hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+  if (hipStreamSynchronize(s) != hipSuccess) return false;
+  return h_bad == 0;
+}
 
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,
                            double* terms, hipStream_t s) {

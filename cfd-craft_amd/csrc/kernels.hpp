@@ -42,6 +42,13 @@ void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x
                          int direct_n, const double* G, double* out64, const PcgState* st, double omega, bool fuse_fine,
                          hipStream_t s);
 template <class T, class TB>
+void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
+                       int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
+                       double omega, hipStream_t s);
+// Device inversion of a big coarsest block (host matrix K, k = m - !anchored);
+// work holds k * k + 2 k + m doubles. False if a pivot was not positive.
+bool invert_block(double* Gblock, int m, bool anchored, const double* K, double* work, int* bad, hipStream_t s);
+template <class T, class TB>
 void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s);
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,
                            double* terms, hipStream_t s);

@@ -110,11 +110,19 @@ typedef int (*cfd_poll_fn)(void* user);
 
 /* Solver options beyond the reference's case keys (serving and precision). */
 enum { CFD_PRECISION_FP64 = 0, CFD_PRECISION_MIXED = 1 };
+/* Multigrid cycle (numerics.pressure_preconditioner = multigrid):
+ *  CFD_CYCLE_RB         the reference's aggregation hierarchy (masked.rs:181-291)
+ *                       with red-black Gauss-Seidel and a scaled coarse correction
+ *                       (the throughput path, FP64 or mixed);
+ *  CFD_CYCLE_REFERENCE  the reference's own cycle: its hierarchy and lexicographic
+ *                       Gauss-Seidel (multigrid.rs:667-726, masked.rs:304-342), FP64;
+ *  CFD_CYCLE_GEOMETRIC  structured 2x2x2 bins without the connectivity split. */
+enum { CFD_CYCLE_RB = 0, CFD_CYCLE_REFERENCE = 1, CFD_CYCLE_GEOMETRIC = 2 };
 typedef struct {
   int32_t precision;   /* FP64 (plain) or FP32 multigrid/IC(0)-apply inside FP64 PCG */
   int32_t warm_start;  /* 1: start PCG from the grid's resident previous solution */
   int32_t poll_every;  /* PCG iterations between host polls (multiple of 8; default 8) */
-  int32_t reserved;
+  int32_t cycle;       /* CFD_CYCLE_* (multigrid only) */
 } cfd_options;
 
 typedef struct cfd_grid cfd_grid;
@@ -147,6 +155,11 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s);
  * output pointer may be NULL. */
 int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
                       int64_t anchored_cap, double* ic0_diagonal);
+
+/* Host-only MultigridStats (multigrid.rs:55-72) of the hierarchy that `cycle`
+ * builds for a case: unknowns and stored entries per level. For CFD_CYCLE_RB it is
+ * the reference's masked aggregation hierarchy (masked.rs:181-291). */
+int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s);
 
 /* ------------------------------------------------------------------ host-buffer seam */
 /* pressure::apply (pressure.rs:293-312): y = A x. */

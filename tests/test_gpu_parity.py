@@ -4,10 +4,12 @@ FP64 arithmetic restated in oracle/), through the C-ABI of libcfdgpu.so.
 Bars: the operator and one IC(0) application are bitwise equal to the oracle
 (same per-cell arithmetic, FP contraction off); PCG-IC(0) solves take the same
 iteration count within +-1 and match the oracle's field to 1e-10 relative; the
-geometric multigrid preconditioner is a different (red-black) cycle, so its
-solves are held to the same field bound and to the acceptance gates.
+red-black multigrid cycles differ from the reference's lexicographic one, so
+their solves are held to the same field bound and to the acceptance gates.
 """
 import math
+import os
+from contextlib import contextmanager
 
 import numpy as np
 import pytest
@@ -22,6 +24,21 @@ pytestmark = pytest.mark.gpu
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@contextmanager
+def env(**kv):
+    """Set-up knobs read when a grid is created (CFD_MG_DENSE, CFD_MG_FUSED, ...)."""
+    old = {k: os.environ.get(k) for k in kv}
+    os.environ.update({k: str(v) for k, v in kv.items()})
+    try:
+        yield
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def small_cases():
@@ -81,15 +98,20 @@ def test_ic0_pcg_matches_published_and_oracle(gpu_available, cells, its):
 
 
 @pytest.mark.parametrize("name,case", small_cases(), ids=[n for n, _ in small_cases()])
-@pytest.mark.parametrize("pc", [IC0, MULTIGRID])
+@pytest.mark.parametrize("pc", [IC0, MULTIGRID, "multigrid-levels"])
 def test_solve_field_parity_small(gpu_available, name, case, pc):
+    """multigrid: small grids fall under the dense coarsest size and are solved
+    directly; multigrid-levels forces coarse levels down to 8 nodes per component."""
+    dense = 8 if pc == "multigrid-levels" else 2048
+    pc = MULTIGRID if pc == "multigrid-levels" else pc
     case.preconditioner = pc
     o = oracle.Grid(case)
     rng = np.random.default_rng(5)
     ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
     b = o.apply(ex)
     xo, io = o.solve(b, 1e-13, 5000)
-    g = cfdgpu.Grid(case)
+    with env(CFD_MG_DENSE=dense):
+        g = cfdgpu.Grid(case)
     x, info = g.solve(b, 1e-13, 5000)
     assert info["relative_residual"] <= 1e-12 + 1e-12
     assert rel(x, xo) <= 1e-10, (name, info, io)
@@ -171,18 +193,9 @@ def test_deterministic_repeat(gpu_available):
 
 
 def _vcycle(case, precision, fused, r):
-    import os
-
-    old = os.environ.get("CFD_MG_FUSED")
-    os.environ["CFD_MG_FUSED"] = str(fused)
-    try:
+    with env(CFD_MG_FUSED=fused, CFD_MG_CYCLE=cfdgpu.CYCLE_GEOMETRIC):
         g = cfdgpu.Grid(case)
-    finally:
-        if old is None:
-            del os.environ["CFD_MG_FUSED"]
-        else:
-            os.environ["CFD_MG_FUSED"] = old
-    g.set_options(precision)
+    g.set_options(precision, cycle=cfdgpu.CYCLE_GEOMETRIC)
     return g.precondition(r)
 
 
@@ -213,13 +226,70 @@ def test_fused_vcycle_equals_per_sweep_vcycle(gpu_available, name, precision):
         assert np.array_equal(a, b), (fused, float(np.max(np.abs(a - b))))
 
 
-def test_odd_periodic_multigrid_is_deterministic(gpu_available):
+@pytest.mark.parametrize("cycle", [cfdgpu.CYCLE_RB, cfdgpu.CYCLE_GEOMETRIC])
+def test_odd_periodic_multigrid_is_deterministic(gpu_available, cycle):
     """Odd periodic coarse levels (36 -> 18 -> 9): repeated cycles are bitwise equal."""
     case = W.cavity([40, 36, 28], MULTIGRID)
     r = np.random.default_rng(4).standard_normal(case.count)
-    g = cfdgpu.Grid(case)
+    with env(CFD_MG_DENSE=8):
+        g = cfdgpu.Grid(case)
+    g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
     a = g.precondition(r)
     b = g.precondition(r)
     assert np.array_equal(a, b)
     x, info = g.solve(g.center_nullspaces(r), 1e-10, 500)
     assert info["relative_residual"] <= 1.01e-9
+
+
+@pytest.mark.parametrize("cells", [[27, 20, 18], [33, 21, 16]])
+def test_odd_periodic_fine_level_colours(gpu_available, cells):
+    """Periodic axes of odd length on the fine level (seam colour classes): the
+    cycle is deterministic, symmetric in effect (PCG converges) and the field
+    matches the oracle."""
+    case = Case(cells=cells, size_m=[2.7, 2.0, 1.8], boundaries=[PERIODIC, PERIODIC, WALL],
+                blocks=[Block([0.5, 0.5, 0.0], [1.1, 1.2, 0.9])])
+    case.preconditioner = MULTIGRID
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(9)
+    ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    xo, io = o.solve(b, 1e-13, 5000)
+    with env(CFD_MG_DENSE=8):
+        g = cfdgpu.Grid(case)
+    for precision in (cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED):
+        g.set_options(precision)
+        assert np.array_equal(g.precondition(b), g.precondition(b))
+        x, info = g.solve(b, 1e-13, 5000)
+        assert info["relative_residual"] <= 2e-12
+        assert rel(x, xo) <= 1e-10, (precision, info, io)
+
+
+def test_split_aggregation_converges_on_prisms(gpu_available):
+    """Polygon footprints leave thin solid corners: the aggregation splits each
+    2x2x2 bin into connected groups (masked.rs:127-156), which keeps the
+    red-black cycle fast where plain bins merge fluid across walls."""
+    case, b, t = W.urban_prisms(n=(96, 96, 48), prisms=60, radius=(5.0, 10.0), heights=(3, 22))
+    o = oracle.Grid(case)
+    xo, io = o.solve_scaled(b, t, 1e-11, 3000)
+    its = {}
+    for cycle in (cfdgpu.CYCLE_RB, cfdgpu.CYCLE_GEOMETRIC):
+        g = cfdgpu.Grid(case)
+        g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
+        x, info = g.solve_scaled(b, t, 1e-11, 3000)
+        assert info["relative_residual"] <= 1.01e-10
+        assert rel(x, xo) <= 1e-8, (cycle, info, io)
+        its[cycle] = info["iterations"]
+    assert its[cfdgpu.CYCLE_RB] <= 25 and its[cfdgpu.CYCLE_RB] < its[cfdgpu.CYCLE_GEOMETRIC], its
+
+
+def test_rb_cycle_deterministic_and_mixed_close(gpu_available):
+    case, b, t = W.urban_block(n=(64, 48, 40), boxes=8)
+    with env(CFD_MG_DENSE=64):
+        g = cfdgpu.Grid(case)
+    r = np.random.default_rng(3).standard_normal(case.count)
+    z1 = g.precondition(r)
+    assert np.array_equal(z1, g.precondition(r))
+    g.set_options(cfdgpu.PRECISION_MIXED)
+    z2 = g.precondition(r)
+    assert np.array_equal(z2, g.precondition(r))
+    assert rel(z2, z1) <= 1e-5

@@ -84,6 +84,25 @@ def test_numpy_workload_helpers_match_oracle(name, case):
     assert np.array_equal(b1, b2) and np.array_equal(t1, t2)
 
 
+@pytest.mark.parametrize("name,case", cases(), ids=[n for n, _ in cases()])
+def test_aggregation_hierarchy_matches_oracle(name, case):
+    """The masked aggregation hierarchy (masked.rs:181-291) the library builds on
+    the host: unknowns and stored entries per level equal the oracle's, which are
+    pinned by the published hierarchy sizes. (Grids without solids and with one
+    component take the reference's unmasked hierarchy, multigrid.rs:530-645.)"""
+    case.preconditioner = 1
+    o = oracle.Grid(case)
+    try:
+        ref = o.mg_stats()
+    except oracle.OracleError as e:
+        pytest.skip(f"oracle: {e}")
+    if not ref["masked"]:
+        pytest.skip("unmasked hierarchy")
+    got = cfdgpu.case_multigrid_stats(case)
+    assert got["level_cells"] == ref["level_cells"]
+    assert got["level_nonzeros"] == ref["level_nonzeros"]
+
+
 def test_grid_build_errors_match_reference_codes():
     with pytest.raises(cfdgpu.CfdError, match="CFD-E028"):
         cfdgpu.case_topology(Case(cells=[8, 8, 8], size_m=[8, 8, 8], boundaries=[WALL] * 3,
