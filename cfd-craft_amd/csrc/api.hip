@@ -113,6 +113,18 @@ struct DevAggShared {
   std::vector<AggDevLevel<double>> shape;  // integer pointers and colour ranges per coarse level
   bool built = false;
 };
+// The reference's cycle on the device (FP64).
+struct DevRef {
+  AggHierarchy h;
+  std::vector<std::unique_ptr<DevBuf<int>>> ints;
+  std::vector<std::unique_ptr<DevBuf<double>>> dbls;
+  DevBuf<int> bptr, rows, small_block, big_block, big_first;
+  DevBuf<long long> goff;
+  DevBuf<double> G;
+  RefCycle cyc;
+  bool built = false;
+};
+
 template <class T>
 struct DevAgg {
   std::vector<AggDevLevel<T>> levels;
@@ -161,6 +173,9 @@ struct cfd_grid {
   DevAgg<double> a64;
   DevAgg<float> a32;
   int64_t agg_dense = kAggDense;
+  // the reference's cycle (CFD_CYCLE_REFERENCE)
+  DevRef ref;
+  DevBuf<double> ref_mean;
   // captured iteration segment, per precision
   hipGraphExec_t seg[2] = {nullptr, nullptr};
   // resident solution of the previous solve (warm start) and the HBM footprint
@@ -469,6 +484,151 @@ int build_geometric(cfd_grid* g) {
   return 0;
 }
 
+void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st);
+
+// Builds the reference's hierarchy (host) and the device arrays of its cycle.
+int build_ref(cfd_grid* g) {
+  DevRef& R = g->ref;
+  if (R.built) return 0;
+  const HostGrid& hg = g->host;
+  AggHierarchy& H = R.h;
+  int rc = build_ref_hierarchy(hg, H);
+  if (rc) return rc;
+  auto upi = [&](const std::vector<int32_t>& v) -> const int* {
+    R.ints.emplace_back(new DevBuf<int>());
+    if (R.ints.back()->upload(v.data(), v.size())) return nullptr;
+    return R.ints.back()->p;
+  };
+  auto upi64 = [&](const std::vector<int64_t>& v) -> const int* {
+    std::vector<int32_t> t(v.begin(), v.end());
+    return upi(t);
+  };
+  auto upd = [&](const std::vector<double>& v) -> double* {
+    R.dbls.emplace_back(new DevBuf<double>());
+    if (R.dbls.back()->upload(v.data(), v.size())) return nullptr;
+    return R.dbls.back()->p;
+  };
+  const int64_t N = hg.count;
+  RefCycle& C = R.cyc;
+  C.g0 = g->geo;
+  C.codes = g->codes.p;
+  C.self_mask = 0;
+  if (!H.masked)
+    for (int a = 0; a < 3; ++a)
+      if (hg.periodic[a] && hg.n[a] == 1) C.self_mask |= 1 << a;
+  {
+    std::vector<double> d0((size_t)N, 0.0);
+    for (int64_t c = 0; c < N; ++c)
+      if (H.node_of_cell[(size_t)c] >= 0) d0[(size_t)c] = H.levels[0].diag[(size_t)H.node_of_cell[(size_t)c]];
+    C.diag0 = upd(d0);
+    std::vector<double> zero((size_t)N, 0.0);
+    C.rhs0 = upd(zero);
+    C.r0 = upd(zero);
+    if (!C.diag0 || !C.rhs0 || !C.r0) return CFD_ENOMEM;
+  }
+  const int nlev = (int)H.levels.size() - 1;
+  C.fparent = nullptr;
+  C.ftptr = nullptr;
+  C.ftcol = nullptr;
+  C.ftw = nullptr;
+  C.fchild = nullptr;
+  if (nlev > 0) {
+    const AggLevel& L0 = H.levels[0];
+    if (H.masked) {
+      std::vector<int32_t> fp((size_t)N, -1);
+      for (int64_t c = 0; c < N; ++c)
+        if (H.node_of_cell[(size_t)c] >= 0) fp[(size_t)c] = L0.parent[(size_t)H.node_of_cell[(size_t)c]];
+      C.fparent = upi(fp);
+    } else {
+      C.ftptr = upi64(L0.tptr);
+      C.ftcol = upi(L0.tcol);
+      C.ftw = upd(L0.tw);
+    }
+    std::vector<int32_t> fc(H.levels[1].child.size());
+    for (size_t k = 0; k < fc.size(); ++k) fc[k] = (int32_t)H.cell_of_node[(size_t)H.levels[1].child[k]];
+    C.fchild = upi(fc);
+  }
+  C.lv.clear();
+  for (int l = 1; l <= nlev; ++l) {
+    const AggLevel& L = H.levels[(size_t)l];
+    RefCoarse rc2{};
+    AggDevLevel<double>& d = rc2.L;
+    d.n = (int)L.n;
+    d.ncol = 0;
+    d.ptr = upi64(L.ptr);
+    d.col = upi(L.col);
+    d.val = upd(L.val);
+    d.diag = upd(L.diag);
+    std::vector<double> zero((size_t)L.n, 0.0);
+    d.b = upd(zero);
+    d.x = upd(zero);
+    rc2.r = upd(zero);
+    d.parent = L.parent.empty() ? nullptr : upi(L.parent);
+    d.child_ptr = upi64(L.child_ptr);
+    d.child = upi(L.child);
+    rc2.child_w = L.child_w.empty() ? nullptr : upd(L.child_w);
+    rc2.tptr = L.tptr.empty() ? nullptr : upi64(L.tptr);
+    rc2.tcol = L.tcol.empty() ? nullptr : upi(L.tcol);
+    rc2.tw = L.tw.empty() ? nullptr : upd(L.tw);
+    rc2.nclosed = (int)L.closed_ptr.size() - 1;
+    rc2.closed_ptr = rc2.nclosed > 0 ? upi64(L.closed_ptr) : nullptr;
+    rc2.closed_members = rc2.nclosed > 0 ? upi(L.closed_members) : nullptr;
+    rc2.fwd_rows = upi(L.fwd_rows);
+    rc2.bwd_rows = upi(L.bwd_rows);
+    rc2.fwd_ptr.assign(L.fwd_ptr.begin(), L.fwd_ptr.end());
+    rc2.bwd_ptr.assign(L.bwd_ptr.begin(), L.bwd_ptr.end());
+    if (!d.x || !d.b || !rc2.r || !d.diag) return CFD_ENOMEM;
+    C.lv.push_back(std::move(rc2));
+  }
+  // coarsest blocks (on the fine level itself the rows are cells)
+  std::vector<int32_t> rows = H.block_rows;
+  if (nlev == 0)
+    for (auto& r : rows) r = (int32_t)H.cell_of_node[(size_t)r];
+  std::vector<int32_t> bptr(H.block_ptr.begin(), H.block_ptr.end());
+  std::vector<long long> goff(H.block_goff.begin(), H.block_goff.end());
+  std::vector<int> bb, bf{0}, sb;
+  for (size_t k = 0; k + 1 < H.block_ptr.size(); ++k) {
+    const int64_t m = H.block_ptr[k + 1] - H.block_ptr[k];
+    if (m > kSmallBlock) {
+      bb.push_back((int)k);
+      bf.push_back(bf.back() + (int)m);
+    } else {
+      sb.push_back((int)k);
+    }
+  }
+  if ((rc = R.bptr.upload(bptr.data(), bptr.size())) || (rc = R.rows.upload(rows.data(), rows.size())) ||
+      (rc = R.goff.upload(goff.data(), goff.size())) || (rc = R.G.upload(H.block_G.data(), H.block_G.size())) ||
+      (rc = R.big_first.upload(bf.data(), bf.size())))
+    return rc;
+  if (!bb.empty() && (rc = R.big_block.upload(bb.data(), bb.size()))) return rc;
+  if (!sb.empty() && (rc = R.small_block.upload(sb.data(), sb.size()))) return rc;
+  AggCoarsest& D = C.D;
+  D.nblocks = (int)H.block_ptr.size() - 1;
+  D.bptr = R.bptr.p;
+  D.rows = R.rows.p;
+  D.goff = R.goff.p;
+  D.G = R.G.p;
+  D.nbig = (int)bb.size();
+  D.big_rows = bf.back();
+  D.big_block = R.big_block.p;
+  D.big_first = R.big_first.p;
+  D.nsmall = (int)sb.size();
+  D.small_block = R.small_block.p;
+  if ((rc = g->ref_mean.alloc((size_t)std::max(3, g->n_closed)))) return rc;
+  C.center0 = [g](const double* src, double* dst) {
+    if (g->n_closed == 0) {
+      if (src != dst)
+        (void)hipMemcpyAsync(dst, src, (size_t)g->geo.N * sizeof(double), hipMemcpyDeviceToDevice, g->stream);
+      return;
+    }
+    const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+    enqueue_gauge(g, src, nullptr, 0, g->ref_mean.p, nullptr);
+    launch_center(g->geo.N, g->codes.p, src, g->ref_mean.p, cidx, dst, g->stream);
+  };
+  R.built = true;
+  return 0;
+}
+
 int ensure_cycle(cfd_grid* g, int cycle) {
   switch (cycle) {
     case CFD_CYCLE_RB:
@@ -476,8 +636,7 @@ int ensure_cycle(cfd_grid* g, int cycle) {
     case CFD_CYCLE_GEOMETRIC:
       return build_geometric(g);
     case CFD_CYCLE_REFERENCE:
-      set_error("multigrid: the reference cycle is not available on this build");
-      return CFD_EUNSUPPORTED;
+      return build_ref(g);
     default:
       set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_GEOMETRIC");
       return CFD_EINVAL;
@@ -570,6 +729,10 @@ void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
   if (g->host.preconditioner == CFD_IC0) {
     launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+    return;
+  }
+  if (g->opt.cycle == CFD_CYCLE_REFERENCE) {
+    launch_ref_vcycle(g->ref.cyc, r, z, st, g->stream);
     return;
   }
   if (g->opt.cycle == CFD_CYCLE_RB) {
@@ -741,7 +904,7 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
         return JOB_DONE;
       }
     }
-    if (g->use_graphs) {
+    if (g->use_graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
       hipGraphExec_t seg = nullptr;
       int rc = segment_graph(g, &seg);
       if (rc) return rc;
@@ -768,7 +931,8 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
       cells = g->host.count;
       for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
     } else {
-      for (size_t l = 0; l + 1 < g->agg.levels.size(); ++l) cells += g->agg.levels[l].n;
+      const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
+      for (size_t l = 0; l + 1 < H.levels.size(); ++l) cells += H.levels[l].n;
     }
     local.cycles = local.preconditioner_applications;
     local.smoothing_cell_updates = local.cycles * 4 * cells;
@@ -895,15 +1059,19 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
     return CFD_EINVAL;
   }
   std::memset(s, 0, sizeof *s);
-  if (cycle != CFD_CYCLE_RB) {
-    set_error("cfd_case_multigrid_stats: only CFD_CYCLE_RB is available without a device");
-    return CFD_EUNSUPPORTED;
+  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_REFERENCE) {
+    set_error("cfd_case_multigrid_stats: cycle must be CFD_CYCLE_RB or CFD_CYCLE_REFERENCE");
+    return CFD_EINVAL;
   }
   HostGrid h;
   int rc = build_host_grid(*c, h);
   if (rc) return rc;
   AggHierarchy H;
-  if ((rc = build_agg_hierarchy(h, H))) return rc;
+  if (cycle == CFD_CYCLE_RB)
+    rc = build_agg_hierarchy(h, H, kAggDense);
+  else
+    rc = build_ref_hierarchy(h, H);
+  if (rc) return rc;
   int L = 0;
   for (const AggLevel& lv : H.levels) {
     if (L >= 32) break;
@@ -935,6 +1103,10 @@ int cfd_grid_components(const cfd_grid* g, int64_t* component, uint8_t* anchored
 int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
   if (o->precision != CFD_PRECISION_FP64 && o->precision != CFD_PRECISION_MIXED) {
     set_error("cfd_options.precision must be FP64 or MIXED");
+    return CFD_EINVAL;
+  }
+  if (g->has_hier && o->cycle == CFD_CYCLE_REFERENCE && o->precision != CFD_PRECISION_FP64) {
+    set_error("cfd_options: the reference cycle runs in FP64");
     return CFD_EINVAL;
   }
   if (g->has_hier && o->cycle != g->opt.cycle) {
@@ -977,7 +1149,7 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
   if (g->opt.cycle != CFD_CYCLE_GEOMETRIC) {
     // MultigridStats of the aggregation hierarchy (masked.rs:286-290): unknowns and
     // stored entries (off-diagonal plus diagonal) per level
-    const AggHierarchy& H = g->agg;
+    const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
     int L = 0;
     int64_t bytes = 0;
     for (const AggLevel& lv : H.levels) {

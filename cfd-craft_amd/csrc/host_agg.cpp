@@ -49,8 +49,11 @@ struct UnionFind {
 // z; each block x-fastest over its face dimensions, grid.rs:341-429): `edge(lo,
 // hi, w)` for an interior face joining two distinct fluid cells, `outlet(c, w)`
 // for an outlet face of cell c.
+// With self_loops, the face of a periodic axis of one cell, which joins a cell
+// to itself, is visited as an edge too (the unmasked hierarchy keeps it,
+// multigrid.rs:568-575; it cancels in products but not in a sweep).
 template <class E, class O>
-void for_each_face(const HostGrid& g, E edge, O outlet) {
+void for_each_face(const HostGrid& g, E edge, O outlet, bool self_loops = false) {
   const int64_t* n = g.n;
   const int64_t st[3] = {1, n[0], n[0] * n[1]};
   for (int a = 0; a < 3; ++a) {
@@ -71,6 +74,7 @@ void for_each_face(const HostGrid& g, E edge, O outlet) {
             lo = base + (n[a] - 1) * st[a];
           if (lo >= 0 && hi >= 0) {
             if (lo != hi && (g.codes[(size_t)lo] & hi_bit)) edge(lo, hi, g.w_int[a]);
+            if (lo == hi && self_loops && g.fluid[(size_t)lo]) edge(lo, hi, g.w_int[a]);
           } else if (hi >= 0) {
             if (g.codes[(size_t)hi] & lo_bit) outlet(hi, g.w_out[a]);
           } else if (lo >= 0) {
@@ -434,6 +438,334 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
   std::vector<int32_t>().swap(h.levels.back().eb);
   std::vector<double>().swap(h.levels.back().ew);
   build_children(h);
+  return 0;
+}
+
+namespace {
+
+// Interpolation weights of a fine cell centre x from the coarse centres along
+// one axis (multigrid.rs:296-320): the two bracketing centres, wrapped on a
+// periodic axis and clamped otherwise; zero weights are dropped.
+void axis_weights(double x, const std::vector<double>& centers, bool periodic, double extent,
+                  std::vector<std::pair<int64_t, double>>& out) {
+  out.clear();
+  const int64_t n = (int64_t)centers.size();
+  if (n == 1) {
+    out.emplace_back(0, 1.0);
+    return;
+  }
+  const int64_t hi0 = std::lower_bound(centers.begin(), centers.end(), x) - centers.begin();
+  int64_t lo, hi;
+  double left, right;
+  if (hi0 == 0) {
+    if (!periodic) {
+      out.emplace_back(0, 1.0);
+      return;
+    }
+    lo = n - 1;
+    hi = 0;
+    left = centers[(size_t)(n - 1)] - extent;
+    right = centers[0];
+  } else if (hi0 == n) {
+    if (!periodic) {
+      out.emplace_back(n - 1, 1.0);
+      return;
+    }
+    lo = n - 1;
+    hi = 0;
+    left = centers[(size_t)(n - 1)];
+    right = centers[0] + extent;
+  } else {
+    lo = hi0 - 1;
+    hi = hi0;
+    left = centers[(size_t)lo];
+    right = centers[(size_t)hi];
+  }
+  const double t = (x - left) / (right - left);
+  if (1.0 - t != 0.0) out.emplace_back(lo, 1.0 - t);
+  if (t != 0.0) out.emplace_back(hi, t);
+}
+
+// Trilinear interpolation rows of one level (multigrid.rs:330-371): axes of more
+// than two cells are paired, the coarse centre being the mean of the pair.
+void transfer_rows(const int64_t n[3], const std::vector<double> centers[3], const bool periodic[3],
+                   const double extent[3], std::vector<double> coarse[3], AggLevel& L) {
+  int64_t nc[3];
+  for (int a = 0; a < 3; ++a) {
+    coarse[a].clear();
+    const size_t step = n[a] > 2 ? 2 : 1;
+    for (size_t s0 = 0; s0 < centers[a].size(); s0 += step) {
+      const size_t e = std::min(centers[a].size(), s0 + step);
+      double sum = -0.0;
+      for (size_t k = s0; k < e; ++k) sum += centers[a][k];
+      coarse[a].push_back(sum / (double)(e - s0));
+    }
+    nc[a] = (int64_t)coarse[a].size();
+  }
+  std::vector<std::vector<std::pair<int64_t, double>>> w[3];
+  std::vector<std::pair<int64_t, double>> tmp;
+  for (int a = 0; a < 3; ++a)
+    for (double x : centers[a]) {
+      axis_weights(x, coarse[a], periodic[a], extent[a], tmp);
+      w[a].push_back(tmp);
+    }
+  const int64_t total = n[0] * n[1] * n[2];
+  L.tptr.assign(1, 0);
+  L.tcol.clear();
+  L.tw.clear();
+  std::vector<std::pair<int64_t, double>> row;
+  for (int64_t i = 0; i < total; ++i) {
+    const int64_t q[3] = {i % n[0], i / n[0] % n[1], i / (n[0] * n[1])};
+    row.clear();
+    for (auto& z : w[2][(size_t)q[2]])
+      for (auto& y : w[1][(size_t)q[1]])
+        for (auto& x : w[0][(size_t)q[0]])
+          row.emplace_back((z.first * nc[1] + y.first) * nc[0] + x.first, x.second * y.second * z.second);
+    std::sort(row.begin(), row.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (auto& e : row) {
+      L.tcol.push_back((int32_t)e.first);
+      L.tw.push_back(e.second);
+    }
+    L.tptr.push_back((int64_t)L.tcol.size());
+  }
+}
+
+// Galerkin coarse operator from edge energies (multigrid.rs:379-429): each fine
+// edge w (x_i - x_j)^2 with P rows p_i, p_j adds w d_a d_b to the coarse pair
+// (a, b) for the entries d of p_i - p_j; each anchor w x_i^2 adds w p_a p_b and
+// w p_a to the coarse anchor. Pairs are accumulated in that order and emitted
+// sorted, as edges (a, b, -sum).
+void galerkin(const AggLevel& F, int64_t nc, AggLevel& C) {
+  EdgeSums sums(nc);
+  std::vector<std::pair<int32_t, double>> delta;
+  auto pairs = [&](double w) {
+    for (size_t at = 0; at < delta.size(); ++at)
+      for (size_t k = at + 1; k < delta.size(); ++k) {
+        // keys ascend with position, so the pair is already (min, max)
+        auto& r = sums.rows[(size_t)delta[at].first];
+        const int32_t b = delta[k].first;
+        const double v = w * delta[at].second * delta[k].second;
+        bool found = false;
+        for (auto& e : r)
+          if (e.first == b) {
+            e.second += v;
+            found = true;
+            break;
+          }
+        if (!found) r.emplace_back(b, v);
+      }
+  };
+  for (size_t e = 0; e < F.ea.size(); ++e) {
+    delta.clear();
+    int64_t a0 = F.tptr[(size_t)F.ea[e]], a1 = F.tptr[(size_t)F.ea[e] + 1];
+    int64_t b0 = F.tptr[(size_t)F.eb[e]], b1 = F.tptr[(size_t)F.eb[e] + 1];
+    while (a0 < a1 || b0 < b1) {
+      const int64_t ca = a0 < a1 ? F.tcol[(size_t)a0] : INT64_MAX;
+      const int64_t cb = b0 < b1 ? F.tcol[(size_t)b0] : INT64_MAX;
+      int64_t col;
+      double value;
+      if (ca == cb) {
+        col = ca;
+        value = F.tw[(size_t)a0++] - F.tw[(size_t)b0++];
+      } else if (ca < cb) {
+        col = ca;
+        value = F.tw[(size_t)a0++];
+      } else {
+        col = cb;
+        value = -F.tw[(size_t)b0++];
+      }
+      if (value != 0.0) delta.emplace_back((int32_t)col, value);
+    }
+    pairs(F.ew[e]);
+  }
+  C.anchor.assign((size_t)nc, 0.0);
+  for (int64_t i = 0; i < F.n; ++i) {
+    const double w = F.anchor[(size_t)i];
+    if (w == 0.0) continue;
+    delta.clear();
+    for (int64_t at = F.tptr[(size_t)i]; at < F.tptr[(size_t)i + 1]; ++at)
+      delta.emplace_back(F.tcol[(size_t)at], F.tw[(size_t)at]);
+    for (size_t at = 0; at < delta.size(); ++at) C.anchor[(size_t)delta[at].first] += w * delta[at].second;
+    pairs(w);
+  }
+  C.ea.clear();
+  C.eb.clear();
+  C.ew.clear();
+  for (size_t a = 0; a < sums.rows.size(); ++a) {
+    auto& r = sums.rows[a];
+    std::sort(r.begin(), r.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (auto& e : r)
+      if (e.second != 0.0) {
+        C.ea.push_back((int32_t)a);
+        C.eb.push_back(e.first);
+        C.ew.push_back(-e.second);
+      }
+  }
+}
+
+// Anisotropic spacing: the reference switches to its line smoother when the
+// largest axis weight exceeds 25 times the smallest among axes of more than two
+// cells (multigrid.rs:33-47).
+bool needs_line_smoother(const HostGrid& g) {
+  double w[3];
+  for (int a = 0; a < 3; ++a) {
+    w[a] = 1.0 / (g.dx[a] * g.dx[a]);
+    if (!std::isfinite(w[a]) || w[a] <= 0.0) return false;
+  }
+  double lo = INFINITY, hi = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    if (g.n[a] > 2) lo = std::fmin(lo, w[a]);
+    hi = std::fmax(hi, w[a]);
+  }
+  return !(hi / lo <= 25.0);
+}
+
+int build_unmasked(const HostGrid& g, AggHierarchy& h) {
+  h.masked = false;
+  const int64_t N = g.count;
+  h.node_of_cell.resize((size_t)N);
+  h.cell_of_node.resize((size_t)N);
+  for (int64_t c = 0; c < N; ++c) {
+    h.node_of_cell[(size_t)c] = (int32_t)c;
+    h.cell_of_node[(size_t)c] = c;
+  }
+  const bool anchored = g.anchored[0] != 0;
+  AggLevel L0;
+  L0.n = N;
+  L0.anchor.assign((size_t)N, 0.0);
+  for_each_face(
+      g, [&](int64_t lo, int64_t hi, double w) {
+        L0.ea.push_back((int32_t)lo);
+        L0.eb.push_back((int32_t)hi);
+        L0.ew.push_back(w);
+      },
+      [&](int64_t c, double w) { L0.anchor[(size_t)c] += w; }, true);
+  h.level0_entries = 2 * (int64_t)L0.ea.size();
+  L0.diag = L0.anchor;
+  for (size_t e = 0; e < L0.ea.size(); ++e) {
+    L0.diag[(size_t)L0.ea[e]] += L0.ew[e];
+    L0.diag[(size_t)L0.eb[e]] += L0.ew[e];
+  }
+  csr_level0(L0);
+  int64_t n[3] = {g.n[0], g.n[1], g.n[2]};
+  std::vector<double> centers[3];
+  double extent[3];
+  bool periodic[3];
+  for (int a = 0; a < 3; ++a) {
+    for (int64_t i = 0; i < n[a]; ++i) centers[a].push_back((double)i + 0.5);
+    extent[a] = (double)n[a];
+    periodic[a] = g.periodic[a] && n[a] >= 2;
+  }
+  h.levels.push_back(std::move(L0));
+  for (;;) {
+    AggLevel& L = h.levels.back();
+    L.closed_ptr.assign(1, 0);
+    if (!anchored) {
+      for (int64_t i = 0; i < L.n; ++i) L.closed_members.push_back((int32_t)i);
+      L.closed_ptr.push_back(L.n);
+    }
+    if (L.n <= 32) {
+      std::vector<int32_t> members((size_t)L.n);
+      std::iota(members.begin(), members.end(), 0);
+      std::vector<double> G;
+      int e = coarsest_block(L, members, anchored, G);
+      if (e) {
+        set_error("CFD-E035: numerics.pressure_preconditioner: multigrid coarse Cholesky lost positive definiteness");
+        return CFD_E035;
+      }
+      h.block_ptr = {0, L.n};
+      h.block_goff = {0, (int64_t)G.size()};
+      h.block_rows = members;
+      h.block_G = std::move(G);
+      break;
+    }
+    std::vector<double> coarse[3];
+    transfer_rows(n, centers, periodic, extent, coarse, L);
+    for (int a = 0; a < 3; ++a) n[a] = (int64_t)coarse[a].size();
+    AggLevel C;
+    C.n = n[0] * n[1] * n[2];
+    galerkin(L, C.n, C);
+    csr_from_edges(C);
+    for (int a = 0; a < 3; ++a) centers[a] = coarse[a];
+    std::vector<int32_t>().swap(L.ea);
+    std::vector<int32_t>().swap(L.eb);
+    std::vector<double>().swap(L.ew);
+    h.levels.push_back(std::move(C));
+  }
+  // children of each coarse node: the finer nodes whose row has it, ascending, with weights
+  for (size_t l = 1; l < h.levels.size(); ++l) {
+    AggLevel& C = h.levels[l];
+    const AggLevel& F = h.levels[l - 1];
+    C.child_ptr.assign((size_t)C.n + 1, 0);
+    for (int32_t c : F.tcol) C.child_ptr[(size_t)c + 1] += 1;
+    for (int64_t i = 0; i < C.n; ++i) C.child_ptr[(size_t)i + 1] += C.child_ptr[(size_t)i];
+    C.child.assign(F.tcol.size(), 0);
+    C.child_w.assign(F.tcol.size(), 0.0);
+    std::vector<int64_t> at(C.child_ptr.begin(), C.child_ptr.end() - 1);
+    for (int64_t i = 0; i < F.n; ++i)
+      for (int64_t k = F.tptr[(size_t)i]; k < F.tptr[(size_t)i + 1]; ++k) {
+        const int64_t slot = at[(size_t)F.tcol[(size_t)k]]++;
+        C.child[(size_t)slot] = (int32_t)i;
+        C.child_w[(size_t)slot] = F.tw[(size_t)k];
+      }
+  }
+  std::vector<int32_t>().swap(h.levels.back().ea);
+  std::vector<int32_t>().swap(h.levels.back().eb);
+  std::vector<double>().swap(h.levels.back().ew);
+  return 0;
+}
+
+// Dependency levels of the lexicographic sweeps of one CSR level.
+void sweep_schedule(AggLevel& L) {
+  const int64_t n = L.n;
+  std::vector<int32_t> depth((size_t)n, 0);
+  auto group = [&](std::vector<int64_t>& ptr, std::vector<int32_t>& rows) {
+    int32_t most = 0;
+    for (int32_t d : depth) most = std::max(most, d);
+    ptr.assign((size_t)most + 2, 0);
+    for (int32_t d : depth) ptr[(size_t)d + 1] += 1;
+    for (int32_t d = 0; d <= most; ++d) ptr[(size_t)d + 1] += ptr[(size_t)d];
+    rows.assign((size_t)n, 0);
+    std::vector<int64_t> at(ptr.begin(), ptr.end() - 1);
+    for (int64_t i = 0; i < n; ++i) rows[(size_t)at[(size_t)depth[(size_t)i]]++] = (int32_t)i;
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t d = 0;
+    for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
+      if (L.col[(size_t)e] < i) d = std::max(d, depth[(size_t)L.col[(size_t)e]] + 1);
+    depth[(size_t)i] = d;
+  }
+  group(L.fwd_ptr, L.fwd_rows);
+  for (int64_t i = n - 1; i >= 0; --i) {
+    int32_t d = 0;
+    for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
+      if (L.col[(size_t)e] > i) d = std::max(d, depth[(size_t)L.col[(size_t)e]] + 1);
+    depth[(size_t)i] = d;
+  }
+  group(L.bwd_ptr, L.bwd_rows);
+}
+
+}  // namespace
+
+int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h) {
+  h = AggHierarchy();
+  if (g.fluid_count == 0) {
+    set_error("CFD-E034: numerics.pressure_preconditioner: multigrid requires fluid cells");
+    return CFD_E034;
+  }
+  int rc;
+  if (g.fluid_count < g.count || g.anchored.size() != 1) {
+    rc = build_agg_hierarchy(g, h, 32);
+  } else {
+    if (needs_line_smoother(g)) {
+      set_error("multigrid: the reference cycle on this grid needs the anisotropic line smoother (multigrid/line.rs), "
+                "which the device does not provide; use CFD_CYCLE_RB");
+      return CFD_EUNSUPPORTED;
+    }
+    rc = build_unmasked(g, h);
+  }
+  if (rc) return rc;
+  for (size_t l = 1; l < h.levels.size(); ++l) sweep_schedule(h.levels[l]);
   return 0;
 }
 

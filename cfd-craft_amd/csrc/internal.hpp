@@ -86,6 +86,16 @@ struct AggLevel {
   std::vector<int64_t> child_ptr;  // finer nodes of each node (levels >= 1)
   std::vector<int32_t> child;
   std::vector<int64_t> colour_ptr;  // colour-sorted layout: independent sets
+  // unmasked hierarchy (multigrid.rs:291-433): interpolation rows to the next
+  // level (columns ascending) and, on the next level, the weight of each child
+  std::vector<int64_t> tptr;
+  std::vector<int32_t> tcol;
+  std::vector<double> tw;
+  std::vector<double> child_w;
+  // lexicographic sweeps by dependency level: forward rows grouped by the
+  // longest chain of lower-numbered neighbours, backward by higher-numbered ones
+  std::vector<int64_t> fwd_ptr, bwd_ptr;
+  std::vector<int32_t> fwd_rows, bwd_rows;
   // edge list (a, b, w) while the next level is built
   std::vector<int32_t> ea, eb;
   std::vector<double> ew;
@@ -108,11 +118,18 @@ struct AggHierarchy {
   };
   std::vector<Deferred> deferred;
   bool permuted = false;
+  bool masked = true;  // false: the unmasked trilinear hierarchy of a solid-free grid
 };
 // Coarsening stops when every component has at most coarse_cells nodes (the
 // reference: 32); blocks above host_block_max nodes are left to the device.
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
                         int64_t host_block_max = INT64_MAX);
+// The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked
+// aggregation when the grid has solids or several components, else trilinear
+// interpolation with Galerkin coarse operators; with the dependency levels of
+// the lexicographic sweeps on levels >= 1. CFD_EUNSUPPORTED where the reference
+// would take its anisotropic line smoother (multigrid.rs:33-47).
+int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h);
 // Renumbers levels >= 1 so that each colour class (an independent set) is contiguous.
 void colour_agg_hierarchy(AggHierarchy& h);
 void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std::vector<int64_t>& nonzeros);

@@ -1480,6 +1480,179 @@ __global__ void __launch_bounds__(BS) k_fill_t(long long N, T* __restrict__ x, c
   x[c] = T(0);
 }
 
+// ---------------------------------------------------------------- multigrid (the reference's cycle)
+// The reference's V-cycle (multigrid.rs:667-726, masked.rs:304-342) with its
+// lexicographic Gauss-Seidel sweeps executed by dependency level: on the fine
+// grid the hyperplanes i + j + k (a row's lower-numbered neighbours lie on the
+// plane before it, its higher-numbered ones after it, wrap neighbours
+// included), on the CSR levels host-computed level sets. Each row repeats the
+// serial sweep's arithmetic in its entry order, so the sweeps produce the
+// serial values. FP64 only.
+
+// Fine-level row in the reference's entry order (faces in face order: per axis
+// the lower face first, the wrap face first on the last cell of a periodic
+// axis). acc + sign * sum w x_j over the interior faces; a periodic axis of one
+// cell (bit a of self_mask) contributes its self face twice.
+__device__ __forceinline__ double ref_fine_row(const Geo& g, uint8_t code, long long c, const int q[3],
+                                               const double* __restrict__ x, double acc, bool subtract, int self_mask) {
+  double nv[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nv[2 * a] = x[nb_or_self(g, c, a, q[a], 0)];
+    nv[2 * a + 1] = x[nb_or_self(g, c, a, q[a], 1)];
+  }
+  const double xc = x[c];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double w = g.wi[a];
+    if ((self_mask >> a) & 1) {
+      acc = subtract ? acc - w * xc : acc + w * xc;
+      acc = subtract ? acc - w * xc : acc + w * xc;
+      continue;
+    }
+    const int t = q[a], n = g.n[a];
+    const bool P = g.per[a];
+    const bool wrap_first = P && t == n - 1;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int side = (k == 0) == wrap_first ? 1 : 0;
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? t == 0 : t == n - 1;
+      if (boundary && !P) continue;  // outlet: in the diagonal only
+      acc = subtract ? acc - w * nv[2 * a + side] : acc + w * nv[2 * a + side];
+    }
+  }
+  return acc;
+}
+
+// Gauss-Seidel on hyperplane i + j + k = L: x = (rhs + sum w x_j) / a_ii, which
+// is the reference's rhs - sum a_ij x_j with a_ij = -w (Matrix::sweep).
+__global__ void __launch_bounds__(BS) k_ref_fine_gs(Geo g, const uint8_t* __restrict__ codes,
+                                                    const double* __restrict__ diag, const double* __restrict__ rhs,
+                                                    double* __restrict__ x, int L, int j0, int k0, int nj, int nk,
+                                                    int self_mask, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int t = blockIdx.x * BS + threadIdx.x;
+  if (t >= nj * nk) return;
+  const int j = j0 + t % nj, k = k0 + t / nj;
+  const int i = L - j - k;
+  if (i < 0 || i >= g.n[0]) return;
+  const long long c = (long long)i + j * g.sy + k * g.sz;
+  const uint8_t code = codes[c];
+  if (!(code & FLUID)) return;
+  const int q[3] = {i, j, k};
+  const double v = ref_fine_row(g, code, c, q, x, rhs[c], false, self_mask);
+  const double d = diag[c];
+  if (status != ST_RUNNING) return;
+  x[c] = v / d;
+}
+
+// r = rhs - (a_ii x + sum a_ij x_j) (Matrix::product); zero on solid cells.
+__global__ void __launch_bounds__(BS) k_ref_fine_residual(Geo g, const uint8_t* __restrict__ codes,
+                                                          const double* __restrict__ diag,
+                                                          const double* __restrict__ rhs, const double* __restrict__ x,
+                                                          double* __restrict__ r, int self_mask, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long c = (long long)blockIdx.x * BS + threadIdx.x;
+  if (c >= g.N) return;
+  const uint8_t code = codes[c];
+  double v = 0.0;
+  if (code & FLUID) {
+    int q[3];
+    coords(g, c, q);
+    v = rhs[c] - ref_fine_row(g, code, c, q, x, diag[c] * x[c], true, self_mask);
+  }
+  if (status != ST_RUNNING) return;
+  r[c] = v;
+}
+
+// Level l -> l + 1: sum over each coarse node's finer nodes in ascending order,
+// weighted on the unmasked hierarchy (next.rhs[col] += w r in fine order); the
+// coarse iterate starts at zero.
+__global__ void __launch_bounds__(BS) k_ref_restrict(int n, const int* __restrict__ child_ptr,
+                                                     const int* __restrict__ child, const double* __restrict__ child_w,
+                                                     const double* __restrict__ r, double* __restrict__ b,
+                                                     double* __restrict__ x, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int J = (int)(blockIdx.x * BS + threadIdx.x);
+  if (J >= n) return;
+  double sum = 0.0;
+  for (int k = child_ptr[J]; k < child_ptr[J + 1]; ++k)
+    sum += child_w ? child_w[k] * r[child[k]] : r[child[k]];
+  if (status != ST_RUNNING) return;
+  b[J] = sum;
+  x[J] = 0.0;
+}
+
+// Rows of one dependency level of a CSR level's sweep.
+__global__ void __launch_bounds__(BS) k_ref_gs(AggDevLevel<double> L, const int* __restrict__ rows, int count,
+                                               const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int t = (int)(blockIdx.x * BS + threadIdx.x);
+  if (t >= count) return;
+  const int i = rows[t];
+  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
+  const double d = L.diag[i];
+  double v = L.b[i];
+  for (int e = e0; e < e1; ++e) v -= L.val[e] * L.x[L.col[e]];
+  if (status != ST_RUNNING) return;
+  L.x[i] = d != 0.0 ? v / d : 0.0;
+}
+
+__global__ void __launch_bounds__(BS) k_ref_residual(AggDevLevel<double> L, double* __restrict__ r, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const int i = (int)(blockIdx.x * BS + threadIdx.x);
+  if (i >= L.n) return;
+  double v = L.diag[i] * L.x[i];
+  for (int e = L.ptr[i]; e < L.ptr[i + 1]; ++e) v += L.val[e] * L.x[L.col[e]];
+  const double out = L.b[i] - v;
+  if (status != ST_RUNNING) return;
+  r[i] = out;
+}
+
+// x_i += correction from the next level: its value at the parent (aggregation)
+// or -0.0 + sum w x_c over the interpolation row (trilinear).
+__global__ void __launch_bounds__(BS) k_ref_prolong(long long n, double* __restrict__ x, const int* __restrict__ parent,
+                                                    const int* __restrict__ tptr, const int* __restrict__ tcol,
+                                                    const double* __restrict__ tw, const double* __restrict__ xc,
+                                                    const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long i = (long long)blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  double add;
+  if (parent) {
+    const int p = parent[i];
+    if (p < 0) return;
+    add = xc[p];
+  } else {
+    add = -0.0;
+    for (int k = tptr[i]; k < tptr[i + 1]; ++k) add += tw[k] * xc[tcol[k]];
+  }
+  const double v = x[i] + add;
+  if (status != ST_RUNNING) return;
+  x[i] = v;
+}
+
+// Subtracts the mean of each closed component (one block per component).
+__global__ void __launch_bounds__(BS) k_center_lists(double* __restrict__ v, const int* __restrict__ cptr,
+                                                     const int* __restrict__ members, const PcgState* st) {
+  if (st && st->status != ST_RUNNING) return;
+  __shared__ double part[BS / 64];
+  const int comp = blockIdx.x;
+  const int m0 = cptr[comp], m1 = cptr[comp + 1];
+  double s = 0.0;
+  for (int k = m0 + (int)threadIdx.x; k < m1; k += BS) s += v[members[k]];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  double total = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) total += part[w];
+  const double mean = total / (double)(m1 - m0);
+  __syncthreads();
+  for (int k = m0 + (int)threadIdx.x; k < m1; k += BS) v[members[k]] -= mean;
+}
+
 // ---------------------------------------------------------------- projection
 // b = -div(u)/dt and the round-off scale sum(|u_lo| + |u_hi|)/dx/dt, in the
 // axis order of Grid::divergence (pressure.rs:440-459). u holds the x-, y- and
@@ -1890,6 +2063,83 @@ bool invert_block(double* Gblock, int m, bool anchored, const double* K, double*
   if (hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
   if (hipStreamSynchronize(s) != hipSuccess) return false;
   return h_bad == 0;
+}
+
+// One application of the reference's V-cycle: z = M r.
+void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgState* st, hipStream_t s) {
+  const Geo& g = R.g0;
+  const unsigned ab = blocks_for(g.N);
+  const int planes = g.n[0] + g.n[1] + g.n[2] - 2;
+  auto fine_sweep = [&](bool forward) {
+    for (int k = 0; k < planes; ++k) {
+      const int L = forward ? k : planes - 1 - k;
+      int j0, k0, nj, nk;
+      level_box(g, L, j0, k0, nj, nk);
+      if (nj * nk == 0) continue;
+      hipLaunchKernelGGL(k_ref_fine_gs, dim3(blocks_for((long long)nj * nk)), dim3(BS), 0, s, g, R.codes, R.diag0,
+                         (const double*)R.rhs0, z, L, j0, k0, nj, nk, R.self_mask, st);
+    }
+  };
+  auto center = [&](const RefCoarse& C, double* v) {
+    if (C.nclosed > 0)
+      hipLaunchKernelGGL(k_center_lists, dim3(C.nclosed), dim3(BS), 0, s, v, C.closed_ptr, C.closed_members, st);
+  };
+  auto coarse_sweep = [&](const RefCoarse& C, bool forward) {
+    const std::vector<int>& ptr = forward ? C.fwd_ptr : C.bwd_ptr;
+    const int* rows = forward ? C.fwd_rows : C.bwd_rows;
+    for (size_t d = 0; d + 1 < ptr.size(); ++d) {
+      const int count = ptr[d + 1] - ptr[d];
+      if (count > 0)
+        hipLaunchKernelGGL(k_ref_gs, dim3(blocks_for(count)), dim3(BS), 0, s, C.L, rows + ptr[d], count, st);
+    }
+  };
+  const int nlev = (int)R.lv.size();
+  // level 0: rhs centered, x = 0, two forward sweeps, centered x, centered residual
+  R.center0(r, R.rhs0);
+  hipLaunchKernelGGL(k_fill_t<double>, dim3(ab), dim3(BS), 0, s, g.N, z, st);
+  if (nlev == 0) {
+    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3((R.D.big_rows + R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
+                       (const double*)R.rhs0, z, (double*)nullptr, st);
+    return;
+  }
+  for (int pass = 0; pass < 2; ++pass) fine_sweep(true);
+  R.center0(z, z);
+  hipLaunchKernelGGL(k_ref_fine_residual, dim3(ab), dim3(BS), 0, s, g, R.codes, R.diag0, (const double*)R.rhs0,
+                     (const double*)z, R.r0, R.self_mask, st);
+  R.center0(R.r0, R.r0);
+  {
+    const RefCoarse& C = R.lv[0];
+    hipLaunchKernelGGL(k_ref_restrict, dim3(blocks_for(C.L.n)), dim3(BS), 0, s, C.L.n, C.L.child_ptr, R.fchild,
+                       C.child_w, (const double*)R.r0, C.L.b, C.L.x, st);
+  }
+  for (int l = 0; l + 1 < nlev; ++l) {
+    const RefCoarse& C = R.lv[l];
+    center(C, C.L.b);
+    for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, true);
+    center(C, C.L.x);
+    hipLaunchKernelGGL(k_ref_residual, dim3(blocks_for(C.L.n)), dim3(BS), 0, s, C.L, C.r, st);
+    center(C, C.r);
+    const RefCoarse& N = R.lv[l + 1];
+    hipLaunchKernelGGL(k_ref_restrict, dim3(blocks_for(N.L.n)), dim3(BS), 0, s, N.L.n, N.L.child_ptr, N.L.child,
+                       N.child_w, (const double*)C.r, N.L.b, N.L.x, st);
+  }
+  {
+    // coarsest: the dense blocks include the centering of rhs and solution
+    const RefCoarse& C = R.lv[nlev - 1];
+    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3((R.D.big_rows + R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
+                       (const double*)C.L.b, C.L.x, (double*)nullptr, st);
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const RefCoarse& C = R.lv[l];
+    hipLaunchKernelGGL(k_ref_prolong, dim3(blocks_for(C.L.n)), dim3(BS), 0, s, (long long)C.L.n, C.L.x, C.L.parent,
+                       C.tptr, C.tcol, C.tw, (const double*)R.lv[l + 1].L.x, st);
+    for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, false);
+    center(C, C.L.x);
+  }
+  hipLaunchKernelGGL(k_ref_prolong, dim3(ab), dim3(BS), 0, s, g.N, z, R.fparent, R.ftptr, R.ftcol, R.ftw,
+                     (const double*)R.lv[0].L.x, st);
+  for (int pass = 0; pass < 2; ++pass) fine_sweep(false);
+  R.center0(z, z);
 }
 
 void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* u, double dt, double* b,

@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <vector>
+
 #include "device.hpp"
 
 namespace cfdgpu {
@@ -45,6 +48,40 @@ template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s);
+// The reference's cycle: fine level from the codes (diagonal diag0 in the
+// reference's summation order), coarse CSR levels in the reference's numbering.
+struct RefCoarse {
+  AggDevLevel<double> L;  // n, CSR, b, x, parent (aggregation) and children (restriction into this level)
+  double* r;              // residual scratch
+  const double* child_w;  // trilinear restriction weights into this level (null: aggregation)
+  const int* tptr;        // trilinear interpolation rows to the next level (null: aggregation)
+  const int* tcol;
+  const double* tw;
+  const int* closed_ptr;  // closed components of this level (centering)
+  const int* closed_members;
+  int nclosed;
+  const int* fwd_rows;  // sweep schedules: rows of dependency level d at [ptr[d], ptr[d+1])
+  const int* bwd_rows;
+  std::vector<int> fwd_ptr, bwd_ptr;
+};
+struct RefCycle {
+  Geo g0;
+  const uint8_t* codes;
+  const double* diag0;
+  double* rhs0;  // centered copy of the fine right-hand side
+  double* r0;    // fine residual
+  int self_mask;  // periodic axes of one cell (self faces, trilinear hierarchy)
+  const int* fchild;   // children of the first coarse level as fine cells
+  const int* fparent;  // fine cell -> first coarse node (aggregation)
+  const int* ftptr;    // fine interpolation rows (trilinear)
+  const int* ftcol;
+  const double* ftw;
+  std::vector<RefCoarse> lv;
+  AggCoarsest D;
+  std::function<void(const double*, double*)> center0;  // fine-level centering (closed components)
+};
+void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgState* st, hipStream_t s);
+
 // Device inversion of a big coarsest block (host matrix K, k = m - !anchored);
 // work holds k * k + 2 k + m doubles. False if a pivot was not positive.
 bool invert_block(double* Gblock, int m, bool anchored, const double* K, double* work, int* bad, hipStream_t s);

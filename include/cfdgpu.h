@@ -157,8 +157,10 @@ int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int6
                       int64_t anchored_cap, double* ic0_diagonal);
 
 /* Host-only MultigridStats (multigrid.rs:55-72) of the hierarchy that `cycle`
- * builds for a case: unknowns and stored entries per level. For CFD_CYCLE_RB it is
- * the reference's masked aggregation hierarchy (masked.rs:181-291). */
+ * builds for a case: unknowns and stored entries per level. CFD_CYCLE_REFERENCE:
+ * the reference's own hierarchy (masked aggregation, masked.rs:181-291, or
+ * trilinear Galerkin, multigrid.rs:530-645); CFD_CYCLE_RB: the masked
+ * aggregation down to the dense coarsest level. */
 int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s);
 
 /* ------------------------------------------------------------------ host-buffer seam */

@@ -293,3 +293,59 @@ def test_rb_cycle_deterministic_and_mixed_close(gpu_available):
     z2 = g.precondition(r)
     assert np.array_equal(z2, g.precondition(r))
     assert rel(z2, z1) <= 1e-5
+
+
+# ---------------------------------------------------------------- the reference's cycle
+def _ref_grid(case):
+    g = cfdgpu.Grid(case)
+    g.set_options(cfdgpu.PRECISION_FP64, cycle=cfdgpu.CYCLE_REFERENCE)
+    return g
+
+
+def _ref_cases():
+    out = list(small_cases())
+    out.append(("prisms40", W.urban_prisms(n=(40, 36, 20), prisms=10, radius=(3.0, 6.0), heights=(3, 12))[0]))
+    out.append(("cube20", W.cavity([20, 20, 20])))
+    return out
+
+
+@pytest.mark.parametrize("name,case", _ref_cases(), ids=[n for n, _ in _ref_cases()])
+def test_reference_cycle_matches_oracle_vcycle(gpu_available, name, case):
+    """One application of the reference's V-cycle (lexicographic Gauss-Seidel by
+    dependency level, the reference's hierarchy and centering points) equals the
+    oracle's serial cycle up to the order of the centering sums."""
+    case.preconditioner = MULTIGRID
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(21)
+    r = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    zo = o.mg_apply(r)
+    g = _ref_grid(case)
+    z = g.precondition(r)
+    assert rel(z, zo) <= 1e-12, (name, rel(z, zo))
+
+
+@pytest.mark.parametrize("name,case", _ref_cases(), ids=[n for n, _ in _ref_cases()])
+def test_reference_cycle_iterations_match_oracle(gpu_available, name, case):
+    case.preconditioner = MULTIGRID
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(5)
+    ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    xo, io = o.solve(b, 1e-12, 3000)
+    g = _ref_grid(case)
+    x, info = g.solve(b, 1e-12, 3000)
+    assert abs(info["iterations"] - io["iterations"]) <= 1, (info, io)
+    assert rel(x, xo) <= 1e-10
+
+
+@pytest.mark.parametrize("cells,its", [([24, 2, 24], 6), ([48, 2, 48], 6), ([24, 24, 24], 7)])
+def test_reference_cycle_published_iterations(gpu_available, cells, its):
+    """Published multigrid PCG iteration counts of the perf-probe cavities
+    (pressure-multigrid.md:34-38) at tol 1e-11."""
+    case = W.cavity(cells, MULTIGRID)
+    o, ex, b = _manufactured(case)
+    g = _ref_grid(case)
+    x, info = g.solve(b, 1e-11, 10000)
+    assert abs(info["iterations"] - its) <= 1, info
+    assert info["relative_residual"] <= 1.01e-10
+    assert math.sqrt(np.mean((x - ex) ** 2)) <= 1e-8

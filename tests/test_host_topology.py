@@ -85,22 +85,31 @@ def test_numpy_workload_helpers_match_oracle(name, case):
 
 
 @pytest.mark.parametrize("name,case", cases(), ids=[n for n, _ in cases()])
-def test_aggregation_hierarchy_matches_oracle(name, case):
-    """The masked aggregation hierarchy (masked.rs:181-291) the library builds on
-    the host: unknowns and stored entries per level equal the oracle's, which are
-    pinned by the published hierarchy sizes. (Grids without solids and with one
-    component take the reference's unmasked hierarchy, multigrid.rs:530-645.)"""
+def test_reference_hierarchy_matches_oracle(name, case):
+    """The hierarchy of the reference cycle, built by the library on the host
+    (masked aggregation, masked.rs:181-291, or the unmasked trilinear Galerkin
+    hierarchy, multigrid.rs:530-645): unknowns and stored entries per level equal
+    the oracle's, which the published hierarchy sizes pin."""
     case.preconditioner = 1
     o = oracle.Grid(case)
     try:
         ref = o.mg_stats()
     except oracle.OracleError as e:
         pytest.skip(f"oracle: {e}")
-    if not ref["masked"]:
-        pytest.skip("unmasked hierarchy")
-    got = cfdgpu.case_multigrid_stats(case)
+    got = cfdgpu.case_multigrid_stats(case, cfdgpu.CYCLE_REFERENCE)
     assert got["level_cells"] == ref["level_cells"]
     assert got["level_nonzeros"] == ref["level_nonzeros"]
+
+
+def test_reference_hierarchy_published_sizes():
+    """48x2x48 cavity: 5 levels, 6138 cells and 78084 stored entries in all
+    (pressure-multigrid.md:141-142); operator complexity of cube48 2.601."""
+    s = cfdgpu.case_multigrid_stats(W.cavity([48, 2, 48]), cfdgpu.CYCLE_REFERENCE)
+    assert s["level_cells"] == [4608, 1152, 288, 72, 18]
+    assert sum(s["level_cells"]) == 6138 and sum(s["level_nonzeros"]) == 78084
+    s = cfdgpu.case_multigrid_stats(W.cavity([48, 48, 48]), cfdgpu.CYCLE_REFERENCE)
+    assert round(sum(s["level_nonzeros"]) / s["level_nonzeros"][0], 3) == 2.601
+    assert round(sum(s["level_cells"]) / s["level_cells"][0], 3) == 1.143
 
 
 def test_grid_build_errors_match_reference_codes():
