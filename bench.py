@@ -4,8 +4,9 @@
 Workload: the urban block (128 x 128 x 64 cells, h = 2 m, 24 seeded ground boxes,
 about 15 % solid, open x with a log-law inflow, outlet anchoring), right-hand
 side -div(u) of a seeded perturbed log-law face velocity. A step is one full
-pressure solve (PCG with the geometric multigrid V(2,2) red-black preconditioner,
-FP64) from the right-hand side resident in HBM to the solution in HBM.
+pressure solve (PCG with the red-black multigrid V(2,2) preconditioner on the
+reference's aggregation hierarchy, FP64) from the right-hand side resident in
+HBM to the solution in HBM.
 
 value = solves/s at relative residual 1e-8 (plain FP64 path); the same workload
 at 1e-10 and in mixed precision (FP32 V-cycle inside FP64 PCG) is reported
@@ -207,8 +208,8 @@ def measured_traffic():
 
 def pmc_probe():
     """The kernels the --pmc passes count: 20 fine half-sweeps of the urban block
-    (FP64) and 5 copies of 64 Mi floats (256 MiB read, 256 MiB written) that
-    calibrate FETCH_SIZE and WRITE_SIZE."""
+    (FP64) and 5 elementwise products c = 2 a over 64 Mi floats (256 MiB read,
+    256 MiB written), which calibrate FETCH_SIZE and WRITE_SIZE."""
     import torch
 
     import cfdgpu
@@ -221,7 +222,7 @@ def pmc_probe():
     a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     c = torch.empty_like(a)
     for _ in range(5):
-        c.copy_(a)
+        torch.mul(a, 2.0, out=c)
     torch.cuda.synchronize()
     print(json.dumps({"fine_gs_avg_us": sec * 1e6, "fine_gs_algorithmic_bytes": nb, "copy_bytes_each_way": 4 << 26}))
 
@@ -234,6 +235,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline solve and its roofline kernel")
     ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -283,9 +285,10 @@ def main():
     value = args.steps / el
     # beside the headline: 1e-10 plain, and mixed precision at 1e-8 and 1e-10
     extra = {}
-    for name, tol_, prec in (("plain_tol_1e-10", 1e-10, cfdgpu.PRECISION_FP64),
-                             ("mixed_tol_1e-8", 1e-8, cfdgpu.PRECISION_MIXED),
-                             ("mixed_tol_1e-10", 1e-10, cfdgpu.PRECISION_MIXED)):
+    variants = () if args.no_extra else (("plain_tol_1e-10", 1e-10, cfdgpu.PRECISION_FP64),
+                                         ("mixed_tol_1e-8", 1e-8, cfdgpu.PRECISION_MIXED),
+                                         ("mixed_tol_1e-10", 1e-10, cfdgpu.PRECISION_MIXED))
+    for name, tol_, prec in variants:
         k = max(3, args.steps // 2)
         e2, l2, i2 = run(k, 1, tol_, prec)
         extra[name] = dict(solves_per_s=k / e2, iterations=i2[-1]["iterations"],
@@ -297,6 +300,8 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(),
                     kernel="k_fine_gs<double,double> (fine red-black half-sweep)",
                     avg_launch_us=round(sec * 1e6, 3), algorithmic_bytes_per_launch=nbytes)
+    if args.no_extra:
+        args.no_served = args.no_large = True
     # the whole V-cycle and the operator for context
     sv, bv = g.profile_kernel(4, reps=20)
     so, bo = g.profile_kernel(0, reps=50)
@@ -305,17 +310,33 @@ def main():
     # the same system with the IC(0) preconditioner (plain FP64, 1e-10)
     from cfdgpu.case import IC0
 
+    cycles = () if args.no_extra else (("geometric_cycle", cfdgpu.CYCLE_GEOMETRIC, 5),
+                                       ("reference_cycle", cfdgpu.CYCLE_REFERENCE, 1))
     case_ic = W.urban_block()[0]
     case_ic.preconditioner = IC0
-    gi = cfdgpu.Grid(case_ic)
-    gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
-    cfdgpu.synchronize()
-    t0 = time.perf_counter()
-    ii = gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
-    cfdgpu.synchronize()
-    extra["ic0_tol_1e-10"] = dict(solves_per_s=round(1.0 / (time.perf_counter() - t0), 3), iterations=ii["iterations"],
-                                  true_rel_residual=ii["relative_residual"])
-    del gi
+    if not args.no_extra:
+        gi = cfdgpu.Grid(case_ic)
+        gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
+        cfdgpu.synchronize()
+        t0 = time.perf_counter()
+        ii = gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
+        cfdgpu.synchronize()
+        extra["ic0_tol_1e-10"] = dict(solves_per_s=round(1.0 / (time.perf_counter() - t0), 3),
+                                      iterations=ii["iterations"], true_rel_residual=ii["relative_residual"])
+        del gi
+    # the same system on the other multigrid cycles (plain FP64, 1e-8): the structured
+    # geometric one and the reference's own (its iteration count is the oracle's)
+    for name, cyc, k in cycles:
+        g.set_options(cfdgpu.PRECISION_FP64, cycle=cyc)
+        g.solve_device(db, dx, tol, 3000, d_scale=dt_)
+        cfdgpu.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            ic = g.solve_device(db, dx, tol, 3000, d_scale=dt_)
+        cfdgpu.synchronize()
+        extra[name] = dict(solves_per_s=round(k / (time.perf_counter() - t0), 3), iterations=ic["iterations"],
+                           true_rel_residual=ic["relative_residual"])
+    g.set_options(cfdgpu.PRECISION_FP64)
     if not args.no_served:
         extra["served"] = served()
     if not args.no_large:
@@ -337,7 +358,7 @@ def main():
         "data": "synthetic (seeded urban block and log-law velocity field)",
         "config": {
             "workload": "urban block 128x128x64, h=2 m, 24 seeded boxes (~15% solid), open x outlet, plain FP64 "
-            "PCG + geometric MG V(2,2) red-black, rel-res 1e-8",
+            "PCG + multigrid V(2,2) red-black on the split 2x2x2 aggregation hierarchy, rel-res 1e-8",
             "cells": case.count,
             "iterations": infos[-1]["iterations"],
             "true_rel_residual": infos[-1]["relative_residual"],

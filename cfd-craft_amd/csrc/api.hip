@@ -331,6 +331,18 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
     d.diag = diag;
     d.b = b;
     d.x = x;
+    d.K = sh.K;
+    d.ecol = sh.ecol;
+    d.eval = nullptr;
+    if (sh.ecol) {
+      std::vector<double> ev((size_t)sh.K * (size_t)L.n, 0.0);
+      for (int64_t i = 0; i < L.n; ++i)
+        for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
+          ev[(size_t)(e - L.ptr[(size_t)i]) * (size_t)L.n + (size_t)i] = L.val[(size_t)e];
+      T* evp = nullptr;
+      if ((rc = up(ev, &evp))) return rc;
+      d.eval = evp;
+    }
     da.levels.push_back(d);
   }
   return 0;
@@ -389,6 +401,21 @@ int build_agg(cfd_grid* g) {
         (rc = upi64(L.child_ptr, &sh.child_ptr)) || (rc = upi(L.child.data(), L.child.size(), &sh.child)))
       return rc;
     if (!L.parent.empty() && (rc = upi(L.parent.data(), L.parent.size(), &sh.parent))) return rc;
+    // padded rows when every row has at most kEllWidth entries
+    int64_t K = 0;
+    for (int64_t i = 0; i < L.n; ++i) K = std::max<int64_t>(K, L.ptr[(size_t)i + 1] - L.ptr[(size_t)i]);
+    sh.K = 0;
+    sh.ecol = nullptr;
+    if (K <= kEllWidth) {
+      std::vector<int32_t> ec((size_t)K * (size_t)L.n);
+      for (int64_t i = 0; i < L.n; ++i)
+        for (int64_t k = 0; k < K; ++k) {
+          const int64_t e = L.ptr[(size_t)i] + k;
+          ec[(size_t)k * (size_t)L.n + (size_t)i] = e < L.ptr[(size_t)i + 1] ? L.col[(size_t)e] : (int32_t)i;
+        }
+      sh.K = (int)K;
+      if (K > 0 && (rc = upi(ec.data(), ec.size(), &sh.ecol))) return rc;
+    }
     A.shape.push_back(sh);
   }
   // coarsest blocks (on the fine level itself the rows are cells)

@@ -91,7 +91,13 @@ struct AggDevLevel {
   const int* parent;     // node of the next level (null at the coarsest)
   const int* child_ptr;  // finer nodes of each node
   const int* child;
+  // the same rows padded to K <= kEllWidth entries (null when a row is longer):
+  // entry k of row i at [k * n + i], padding points at the row itself with value 0
+  int K;
+  const int* ecol;
+  const T* eval;
 };
+constexpr int kEllWidth = 8;
 
 // Coarsest level: one dense block per connected component. Blocks of at most
 // 64 nodes take one wavefront each; larger ones ("big", listed separately) one

@@ -1234,11 +1234,46 @@ __device__ __forceinline__ T agg_offdiag(const AggDevLevel<T>& L, int e0, int e1
   return v;
 }
 
+// The same sum over the padded rows: no row-pointer load, K is uniform.
+template <class T>
+__device__ __forceinline__ T ell_offdiag(const AggDevLevel<T>& L, int i, T v, bool subtract,
+                                         const T* __restrict__ xc = nullptr, T cs = T(0)) {
+  int cols[kEllWidth];
+  T vals[kEllWidth], xv[kEllWidth];
+#pragma unroll
+  for (int k = 0; k < kEllWidth; ++k) {
+    cols[k] = i;
+    vals[k] = T(0);
+    if (k < L.K) {
+      cols[k] = L.ecol[(long long)k * L.n + i];
+      vals[k] = L.eval[(long long)k * L.n + i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kEllWidth; ++k) {
+    xv[k] = T(0);
+    if (k < L.K) {
+      xv[k] = L.x[cols[k]];
+      if (xc) xv[k] += cs * xc[L.parent[cols[k]]];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kEllWidth; ++k)
+    if (k < L.K) {
+      if (subtract)
+        v -= vals[k] * xv[k];
+      else
+        v += vals[k] * xv[k];
+    }
+  return v;
+}
+
 // b_i - (a_ii x_i + sum a_ij x_j) (Matrix::product, multigrid.rs:263-271)
 template <class T>
 __device__ __forceinline__ T agg_row_residual(const AggDevLevel<T>& L, int i) {
-  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
   const T d = L.diag[i], xi = L.x[i], bi = L.b[i];
+  if (L.ecol) return bi - ell_offdiag<T>(L, i, d * xi, false);
+  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
   return bi - agg_offdiag<T>(L, e0, e1, d * xi, false);
 }
 
@@ -1246,9 +1281,14 @@ __device__ __forceinline__ T agg_row_residual(const AggDevLevel<T>& L, int i) {
 template <class T>
 __device__ __forceinline__ T agg_row_relax(const AggDevLevel<T>& L, int i, const T* __restrict__ xc = nullptr,
                                            T cs = T(0)) {
-  const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
   const T d = L.diag[i], bi = L.b[i];
-  const T v = agg_offdiag<T>(L, e0, e1, bi, true, xc, cs);
+  T v;
+  if (L.ecol) {
+    v = ell_offdiag<T>(L, i, bi, true, xc, cs);
+  } else {
+    const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
+    v = agg_offdiag<T>(L, e0, e1, bi, true, xc, cs);
+  }
   return d != T(0) ? v / d : T(0);
 }
 
@@ -1319,17 +1359,19 @@ __device__ __forceinline__ void agg_direct_block(const AggCoarsest& D, int k, in
   if (out64) out64[row] = (double)(TO)acc;
 }
 
-// Coarsest solve in one launch: the first big_rows wavefronts take one row of a
-// big block each (lanes striding the columns), the rest one small block each.
+// Coarsest solve in one launch: the first big_rows workgroups take one row of a
+// big block each (the four wavefronts split the columns), the following ones
+// four small blocks each (one wavefront per block).
 template <class TI, class TO>
 __global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __restrict__ b, TO* __restrict__ x,
                                                    double* __restrict__ out64, const PcgState* st) {
+  __shared__ double part[BS / 64];
   const int status = st ? st->status : ST_RUNNING;
-  const int w = (int)(blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int w = (int)blockIdx.x;
   if (w >= D.big_rows) {
-    if (w - D.big_rows < D.nsmall)
-      agg_direct_block<TI, TO>(D, D.small_block[w - D.big_rows], lane, b, x, out64, status == ST_RUNNING);
+    const int k = (w - D.big_rows) * (BS / 64) + wave;
+    if (k < D.nsmall) agg_direct_block<TI, TO>(D, D.small_block[k], lane, b, x, out64, status == ST_RUNNING);
     return;
   }
   int q = 0;
@@ -1339,12 +1381,18 @@ __global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __re
   const int row = w - D.big_first[q];
   const double* G = D.G + D.goff[k] + (long long)row * m;
   double acc = 0.0;
-  for (int j = lane; j < m; j += 64) acc += G[j] * (double)b[D.rows[r0 + j]];
+#pragma unroll 4
+  for (int j = threadIdx.x; j < m; j += BS) acc += G[j] * (double)b[D.rows[r0 + j]];
   acc = wave_sum(acc);
-  if (lane != 0 || status != ST_RUNNING) return;
+  if (lane == 0) part[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x != 0 || status != ST_RUNNING) return;
+  double total = 0.0;
+#pragma unroll
+  for (int v = 0; v < BS / 64; ++v) total += part[v];
   const int r = D.rows[r0 + row];
-  x[r] = (TO)acc;
-  if (out64) out64[r] = (double)(TO)acc;
+  x[r] = (TO)total;
+  if (out64) out64[r] = (double)(TO)total;
 }
 
 // Set-up of a big coarsest block on the device: in-place Gauss-Jordan inversion
@@ -1956,7 +2004,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const unsigned ab = blocks_for(g0.N);
   auto direct = [&](const auto* b, T* x, double* o64) {
     using TI = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
-    hipLaunchKernelGGL((k_agg_direct<TI, T>), dim3((D.big_rows + D.nsmall + 3) / 4), dim3(BS), 0, s, D, b, x, o64, st);
+    hipLaunchKernelGGL((k_agg_direct<TI, T>), dim3(D.big_rows + (D.nsmall + 3) / 4), dim3(BS), 0, s, D, b, x, o64, st);
   };
   if (nlev == 0) {
     // the fine level is the coarsest: solid cells stay zero
@@ -2098,7 +2146,7 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   R.center0(r, R.rhs0);
   hipLaunchKernelGGL(k_fill_t<double>, dim3(ab), dim3(BS), 0, s, g.N, z, st);
   if (nlev == 0) {
-    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3((R.D.big_rows + R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
+    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3(R.D.big_rows + (R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
                        (const double*)R.rhs0, z, (double*)nullptr, st);
     return;
   }
@@ -2126,7 +2174,7 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   {
     // coarsest: the dense blocks include the centering of rhs and solution
     const RefCoarse& C = R.lv[nlev - 1];
-    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3((R.D.big_rows + R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
+    hipLaunchKernelGGL((k_agg_direct<double, double>), dim3(R.D.big_rows + (R.D.nsmall + 3) / 4), dim3(BS), 0, s, R.D,
                        (const double*)C.L.b, C.L.x, (double*)nullptr, st);
   }
   for (int l = nlev - 2; l >= 0; --l) {
