@@ -108,6 +108,13 @@ struct DevAggShared {
   DevBuf<double> G;
   AggCoarsest D{};
   DevBuf<int> big_block, big_first, small_block;
+  // colour-separated fine level (grids without odd periodic seams)
+  DevBuf<uint8_t> rb_codes;
+  DevBuf<double> rb_b64, rb_x64;
+  DevBuf<float> rb_b32, rb_x32;
+  FineRB<double> rb64{};
+  FineRB<float> rb32{};
+  bool has_rb = false;
   int seams = 0;       // periodic axes of odd length on the fine level
   int nlev = 0;        // coarse levels
   std::vector<AggDevLevel<double>> shape;  // integer pointers and colour ranges per coarse level
@@ -173,6 +180,7 @@ struct cfd_grid {
   DevAgg<double> a64;
   DevAgg<float> a32;
   int64_t agg_dense = kAggDense;
+  bool rb_layout = true;  // CFD_MG_RB_LAYOUT=0: the fine level in the natural layout
   // the reference's cycle (CFD_CYCLE_REFERENCE)
   DevRef ref;
   DevBuf<double> ref_mean;
@@ -476,6 +484,27 @@ int build_agg(cfd_grid* g) {
   }
   if ((rc = upload_agg_values<double>(g, g->a64))) return rc;
   if ((rc = upload_agg_values<float>(g, g->a32))) return rc;
+  if (A.seams == 0 && A.nlev > 0 && g->rb_layout) {
+    const int hx = (int)((h.n[0] + 1) / 2);
+    const long long half = (long long)hx * h.n[1] * h.n[2];
+    std::vector<uint8_t> rc8((size_t)(2 * half), 0);
+    for (int64_t c = 0; c < h.count; ++c) {
+      const int64_t i = c % h.n[0], j = c / h.n[0] % h.n[1], k = c / (h.n[0] * h.n[1]);
+      const int colour = (int)((i + j + k) & 1);
+      rc8[(size_t)(colour * half + (i >> 1) + (long long)hx * (j + h.n[1] * k))] = h.codes[(size_t)c];
+    }
+    if ((rc = A.rb_codes.upload(rc8.data(), rc8.size())) || (rc = A.rb_b64.alloc((size_t)(2 * half))) ||
+        (rc = A.rb_x64.alloc((size_t)(2 * half))) || (rc = A.rb_b32.alloc((size_t)(2 * half))) ||
+        (rc = A.rb_x32.alloc((size_t)(2 * half))))
+      return rc;
+    HIP_TRY(hipMemset(A.rb_b64.p, 0, (size_t)(2 * half) * sizeof(double)));
+    HIP_TRY(hipMemset(A.rb_x64.p, 0, (size_t)(2 * half) * sizeof(double)));
+    HIP_TRY(hipMemset(A.rb_b32.p, 0, (size_t)(2 * half) * sizeof(float)));
+    HIP_TRY(hipMemset(A.rb_x32.p, 0, (size_t)(2 * half) * sizeof(float)));
+    A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p};
+    A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p};
+    A.has_rb = true;
+  }
   // the host copies of the coarse matrices are no longer needed
   for (auto& L : H.levels) {
     std::vector<int32_t>().swap(L.col);
@@ -692,6 +721,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
   if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
+  if (const char* env = std::getenv("CFD_MG_RB_LAYOUT")) g->rb_layout = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   size_t N = (size_t)h.count;
@@ -766,10 +796,12 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
     const DevAggShared& A = g->ash;
     if (g->opt.precision == CFD_PRECISION_MIXED)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
-                                       g->a32.levels.data(), A.D, z, st, g->omega, g->stream);
+                                       g->a32.levels.data(), A.D, z, st, g->omega, g->stream,
+                                       A.has_rb ? &A.rb32 : nullptr);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
-                                        A.D, nullptr, st, g->omega, g->stream);
+                                        A.D, A.has_rb ? z : nullptr, st, g->omega, g->stream,
+                                        A.has_rb ? &A.rb64 : nullptr);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -1364,10 +1396,16 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
         launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, st, g->part.p, g->stream);
         break;
       case 1:
-        if (mixed)
+        if (g->opt.cycle == CFD_CYCLE_RB && g->ash.has_rb) {
+          if (mixed)
+            launch_rb_gs<float>(g->geo, g->ash.rb32, 1, g->stream);
+          else
+            launch_rb_gs<double>(g->geo, g->ash.rb64, 1, g->stream);
+        } else if (mixed) {
           launch_fine_gs<float, double>(g->geo, g->codes.p, g->r.p, g->x0f.p, 1, Geo{}, g->stream);
-        else
+        } else {
           launch_fine_gs<double, double>(g->geo, g->codes.p, g->r.p, g->z.p, 1, Geo{}, g->stream);
+        }
         break;
       case 2:
         launch_pcg_axpy(N, g->p0.p, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);

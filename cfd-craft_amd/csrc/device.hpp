@@ -99,6 +99,19 @@ struct AggDevLevel {
 };
 constexpr int kEllWidth = 8;
 
+// Colour-separated copy of the fine level for the red-black cycle: the cells of
+// colour c (i + j + k = c mod 2) at [c * half + (i >> 1) + hx * (j + ny * k)],
+// hx = (nx + 1) / 2, half = hx * ny * nz. A half-sweep then reads only the
+// other colour's iterate and its own right-hand side and codes.
+template <class T>
+struct FineRB {
+  long long half;
+  int hx;
+  const uint8_t* codes;
+  T* b;
+  T* x;
+};
+
 // Coarsest level: one dense block per connected component. Blocks of at most
 // 64 nodes take one wavefront each; larger ones ("big", listed separately) one
 // wavefront per row.

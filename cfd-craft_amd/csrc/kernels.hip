@@ -1528,6 +1528,207 @@ __global__ void __launch_bounds__(BS) k_fill_t(long long N, T* __restrict__ x, c
   x[c] = T(0);
 }
 
+// ---------------------------------------------------------------- colour-separated fine level
+__device__ __forceinline__ long long rb_index(const Geo& g, int hx, int i, int j, int k) {
+  return (long long)(i >> 1) + (long long)hx * (j + (long long)g.n[1] * k);
+}
+
+// Diagonal of a fine cell from its code, accumulated in T as k_fine_gs does.
+template <class T>
+__device__ __forceinline__ T fine_diag(const Geo& g, uint8_t code, const int q[3]) {
+  T diag = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      diag += (boundary && !g.per[a]) ? (T)g.wo[a] : (T)g.wi[a];
+    }
+  return diag;
+}
+
+// The neighbour of q across side `side` of axis a (wrapping on periodic axes;
+// q itself where the axis ends, whose value is then never used).
+__device__ __forceinline__ void nb_coords(const Geo& g, const int q[3], int a, int side, int out[3]) {
+  out[0] = q[0];
+  out[1] = q[1];
+  out[2] = q[2];
+  const int n = g.n[a];
+  if (side == 0)
+    out[a] = q[a] > 0 ? q[a] - 1 : (g.per[a] ? n - 1 : q[a]);
+  else
+    out[a] = q[a] < n - 1 ? q[a] + 1 : (g.per[a] ? 0 : q[a]);
+}
+
+// First pre-smoothing half-sweep (x = 0): scatters the right-hand side into the
+// colour-separated layout and sets the red cells to b / a_ii.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ r,
+                                                 FineRB<T> F, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long c = (long long)blockIdx.x * BS + threadIdx.x;
+  if (c >= g.N) return;
+  int q[3];
+  coords(g, c, q);
+  const uint8_t code = codes[c];
+  const T bv = (T)r[c];
+  const int colour = (q[0] + q[1] + q[2]) & 1;
+  const long long at = colour * F.half + rb_index(g, F.hx, q[0], q[1], q[2]);
+  if (status != ST_RUNNING) return;
+  F.b[at] = bv;
+  if (colour == 0) {
+    const T d = fine_diag<T>(g, code, q);
+    F.x[at] = (code & FLUID) && d > T(0) ? bv / d : T(0);
+  }
+}
+
+// One half-sweep of colour `colour`. xc: the prolongation x += cs e[parent]
+// rides on the neighbour reads (first post-smoothing half-sweep). out64: also
+// store the cell's value, in the natural layout and FP64, there.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, const T* __restrict__ xc,
+                                              const int* __restrict__ fparent, T cs, double* __restrict__ out64,
+                                              const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long u = (long long)blockIdx.x * BS + threadIdx.x;
+  if (u >= F.half) return;
+  const unsigned hx = (unsigned)F.hx;
+  const unsigned row = (unsigned)(u / hx);
+  const int i2 = (int)(u - (long long)row * hx);
+  const int k = (int)(row / (unsigned)g.n[1]);
+  const int j = (int)(row - (unsigned)k * (unsigned)g.n[1]);
+  const int i = 2 * i2 + ((j + k + colour) & 1);
+  if (i >= g.n[0]) return;
+  const int q[3] = {i, j, k};
+  const long long own = colour * F.half + u;
+  const long long other = (1 - colour) * F.half;
+  const uint8_t code = F.codes[own];
+  const T bv = F.b[own];
+  T xn[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      int qn[3];
+      nb_coords(g, q, a, side, qn);
+      T v = F.x[other + rb_index(g, F.hx, qn[0], qn[1], qn[2])];
+      if (xc) {
+        const int p = fparent[(long long)qn[0] + qn[1] * g.sy + qn[2] * g.sz];
+        if (p >= 0) v += cs * xc[p];
+      }
+      xn[2 * a + side] = v;
+    }
+  if (status != ST_RUNNING) return;
+  const long long c = (long long)i + j * g.sy + k * g.sz;
+  if (!(code & FLUID)) {
+    F.x[own] = T(0);
+    if (out64) out64[c] = 0.0;
+    return;
+  }
+  T diag = T(0);
+  T value = bv;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      if (boundary && !g.per[a]) {
+        diag += (T)g.wo[a];
+        continue;
+      }
+      const T w = (T)g.wi[a];
+      diag += w;
+      value += w * xn[2 * a + side];
+    }
+  }
+  const T v = diag > T(0) ? value / diag : T(0);
+  F.x[own] = v;
+  if (out64) out64[c] = (double)v;
+}
+
+// Fine residual of cell q from the colour-separated layout.
+template <class T>
+__device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const int q[3]) {
+  const int colour = (q[0] + q[1] + q[2]) & 1;
+  const long long own = colour * F.half + rb_index(g, F.hx, q[0], q[1], q[2]);
+  const long long other = (1 - colour) * F.half;
+  const uint8_t code = F.codes[own];
+  const T bv = F.b[own];
+  const T xv = F.x[own];
+  T xn[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      int qn[3];
+      nb_coords(g, q, a, side, qn);
+      xn[2 * a + side] = F.x[other + rb_index(g, F.hx, qn[0], qn[1], qn[2])];
+    }
+  if (!(code & FLUID)) return T(0);
+  T ax = T(0), off = T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      if (boundary && !g.per[a]) {
+        ax += (T)g.wo[a];
+        continue;
+      }
+      ax += (T)g.wi[a];
+      off += (T)g.wi[a] * xn[2 * a + side];
+    }
+  return bv - (ax * xv - off);
+}
+
+// Restriction from the colour-separated fine level: as k_agg_fine_restrict.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const int* __restrict__ fparent,
+                                                    T* __restrict__ bc, T* __restrict__ xc, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned t = blockIdx.x * BS + threadIdx.x;
+  const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
+  const unsigned B = t >> 3;
+  const int sub = (int)(t & 7);
+  T r = T(0);
+  int p = -1;
+  if (B < bx * by * bz) {
+    const unsigned Bx = B % bx, Byz = B / bx;
+    const unsigned By = Byz % by, Bz = Byz / by;
+    const int q[3] = {(int)(2 * Bx) + (sub & 1), (int)(2 * By) + ((sub >> 1) & 1), (int)(2 * Bz) + (sub >> 2)};
+    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) {
+      p = fparent[(long long)q[0] + q[1] * g.sy + q[2] * g.sz];
+      r = rb_residual<T>(g, F, q);
+    }
+  }
+  const int base = (int)(threadIdx.x & 63) & ~7;
+  T rs[8];
+  int ps[8];
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) {
+    rs[s2] = __shfl(r, base + s2, 64);
+    ps[s2] = __shfl(p, base + s2, 64);
+  }
+  if (sub != 0 || status != ST_RUNNING) return;
+  unsigned done = 0;
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) {
+    if (ps[s2] < 0 || (done & (1u << s2))) continue;
+    T sum = T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u >= s2 && ps[u] == ps[s2]) {
+        sum += rs[u];
+        done |= 1u << u;
+      }
+    bc[ps[s2]] = sum;
+    xc[ps[s2]] = T(0);
+  }
+}
+
 // ---------------------------------------------------------------- multigrid (the reference's cycle)
 // The reference's V-cycle (multigrid.rs:667-726, masked.rs:304-342) with its
 // lexicographic Gauss-Seidel sweeps executed by dependency level: on the fine
@@ -1999,7 +2200,7 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
-                       double omega, hipStream_t s) {
+                       double omega, hipStream_t s, const FineRB<T>* rb) {
   const T cs = (T)omega;
   const unsigned ab = blocks_for(g0.N);
   auto direct = [&](const auto* b, T* x, double* o64) {
@@ -2021,7 +2222,17 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const T* nox = nullptr;
   const int fine_colours = 2 << __builtin_popcount((unsigned)seams);
   const unsigned fb = color_blocks<T>(g0);
-  if (!seams) {
+  const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
+  if (rb) {
+    // colour-separated fine level: scatter + red, black, red, black; restriction
+    const unsigned hb = blocks_for(rb->half);
+    hipLaunchKernelGGL((k_rb_first<T>), dim3(ab), dim3(BS), 0, s, g0, codes, (const double*)b0, *rb, st);
+    for (int pass = 1; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
+                         st);
+    hipLaunchKernelGGL((k_rb_restrict<T>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, *rb, fparent, lv[0].b,
+                       lv[0].x, st);
+  } else if (!seams) {
     for (int pass = 0; pass < 4; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
                          none, (double*)nullptr, 0, st, cs, fparent, 0);
@@ -2032,11 +2243,9 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(ab), dim3(BS), 0, s, g0, codes, b0, x0, c, 0, nox, none,
                            (double*)nullptr, 1, st, cs, fparent, seams);
   }
-  {
-    const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
+  if (!rb)
     hipLaunchKernelGGL((k_agg_fine_restrict<T, TB>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, codes, b0,
                        (const T*)x0, fparent, lv[0].b, lv[0].x, st);
-  }
   const int stop = nlev - 1;
   for (int l = 0; l < stop; ++l) {
     const AggDevLevel<T>& L = lv[l];
@@ -2064,8 +2273,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         for (int c = L.ncol - 1; c >= 0; --c) relax(L, c);
     }
   }
-  // fine post-smoothing; in mixed precision the final half-sweeps also write z in FP64
-  if (!seams) {
+  // fine post-smoothing; in mixed precision (and always in the colour-separated
+  // layout) the final half-sweeps also write z in FP64
+  if (rb) {
+    const unsigned hb = blocks_for(rb->half);
+    for (int pass = 0; pass < 4; ++pass)
+      hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, (pass + 1) & 1,
+                         pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 ? out64 : (double*)nullptr, st);
+  } else if (!seams) {
     for (int pass = 0; pass < 4; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
                          pass == 0 ? (const T*)lv[0].x : nox, none, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs,
@@ -2080,10 +2295,18 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 }
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
-                                                const PcgState*, double, hipStream_t);
+                                                const PcgState*, double, hipStream_t, const FineRB<double>*);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
-                                               double, hipStream_t);
+                                               double, hipStream_t, const FineRB<float>*);
+
+template <class T>
+void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {
+  hipLaunchKernelGGL((k_rb_gs<T>), dim3(blocks_for(rb.half)), dim3(BS), 0, s, g0, rb, colour, (const T*)nullptr,
+                     (const int*)nullptr, T(0), (double*)nullptr, (const PcgState*)nullptr);
+}
+template void launch_rb_gs<double>(const Geo&, const FineRB<double>&, int, hipStream_t);
+template void launch_rb_gs<float>(const Geo&, const FineRB<float>&, int, hipStream_t);
 
 // Inverts the deferred coarsest blocks on the device (see k_gj_pivot); returns
 // false when a pivot is not positive (the block lost definiteness).

@@ -47,7 +47,10 @@ void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
-                       double omega, hipStream_t s);
+                       double omega, hipStream_t s, const FineRB<T>* rb = nullptr);
+// One colour-separated fine half-sweep (timing hook).
+template <class T>
+void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
 // The reference's cycle: fine level from the codes (diagonal diag0 in the
 // reference's summation order), coarse CSR levels in the reference's numbering.
 struct RefCoarse {

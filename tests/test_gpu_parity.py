@@ -264,6 +264,23 @@ def test_odd_periodic_fine_level_colours(gpu_available, cells):
         assert rel(x, xo) <= 1e-10, (precision, info, io)
 
 
+@pytest.mark.parametrize("cells", [[64, 48, 40], [33, 27, 21]])
+@pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
+def test_colour_separated_fine_level_is_bitwise(gpu_available, cells, precision):
+    """The red-black cycle with the fine level stored colour by colour gives the
+    same bits as with the natural layout (same per-cell arithmetic), odd sizes
+    included."""
+    case = W.urban_block(n=tuple(cells), boxes=6)[0]
+    r = np.random.default_rng(8).standard_normal(case.count)
+    out = []
+    for layout in (1, 0):
+        with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_DENSE=64):
+            g = cfdgpu.Grid(case)
+        g.set_options(precision)
+        out.append(g.precondition(r))
+    assert np.array_equal(out[0], out[1]), float(np.max(np.abs(out[0] - out[1])))
+
+
 def test_split_aggregation_converges_on_prisms(gpu_available):
     """Polygon footprints leave thin solid corners: the aggregation splits each
     2x2x2 bin into connected groups (masked.rs:127-156), which keeps the
