@@ -1458,11 +1458,13 @@ __global__ void __launch_bounds__(BS) k_block_center(double* __restrict__ G, int
 // Fine level -> first coarse level: eight lanes per 2x2x2 bin form the fine
 // residuals; the bin's first lane adds them up per aggregate (a bin holds one
 // aggregate per connected group of its fluid cells) in ascending cell order.
+// zero_colour: the colour class relaxed last (its residuals vanish in exact
+// arithmetic and are taken as zero), -1 for none.
 template <class T, class TB>
 __global__ void __launch_bounds__(BS) k_agg_fine_restrict(Geo g, const uint8_t* __restrict__ codes,
                                                           const TB* __restrict__ b, const T* __restrict__ x,
                                                           const int* __restrict__ fparent, T* __restrict__ bc,
-                                                          T* __restrict__ xc, const PcgState* st) {
+                                                          T* __restrict__ xc, const PcgState* st, int zero_colour) {
   const int status = st ? st->status : ST_RUNNING;
   const unsigned t = blockIdx.x * BS + threadIdx.x;
   const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
@@ -1476,7 +1478,7 @@ __global__ void __launch_bounds__(BS) k_agg_fine_restrict(Geo g, const uint8_t* 
     const int q[3] = {(int)(2 * Bx) + (sub & 1), (int)(2 * By) + ((sub >> 1) & 1), (int)(2 * Bz) + (sub >> 2)};
     if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) {
       p = fparent[(long long)q[0] + q[1] * g.sy + q[2] * g.sz];
-      r = fine_residual<T, TB>(g, codes, b, x, q);
+      if (((q[0] + q[1] + q[2]) & 1) != zero_colour) r = fine_residual<T, TB>(g, codes, b, x, q);
     }
   }
   const int base = (int)(threadIdx.x & 63) & ~7;
@@ -1701,7 +1703,8 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
     const int q[3] = {(int)(2 * Bx) + (sub & 1), (int)(2 * By) + ((sub >> 1) & 1), (int)(2 * Bz) + (sub >> 2)};
     if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) {
       p = fparent[(long long)q[0] + q[1] * g.sy + q[2] * g.sz];
-      r = rb_residual<T>(g, F, q);
+      // the black cells were relaxed last: their residuals vanish (taken as zero)
+      if (((q[0] + q[1] + q[2]) & 1) == 0) r = rb_residual<T>(g, F, q);
     }
   }
   const int base = (int)(threadIdx.x & 63) & ~7;
@@ -2245,7 +2248,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   }
   if (!rb)
     hipLaunchKernelGGL((k_agg_fine_restrict<T, TB>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, codes, b0,
-                       (const T*)x0, fparent, lv[0].b, lv[0].x, st);
+                       (const T*)x0, fparent, lv[0].b, lv[0].x, st, seams ? -1 : 1);
   const int stop = nlev - 1;
   for (int l = 0; l < stop; ++l) {
     const AggDevLevel<T>& L = lv[l];
