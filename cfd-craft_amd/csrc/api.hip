@@ -181,6 +181,7 @@ struct cfd_grid {
   DevAgg<float> a32;
   int64_t agg_dense = kAggDense;
   bool rb_layout = true;  // CFD_MG_RB_LAYOUT=0: the fine level in the natural layout
+  bool ic0_tiles = true;  // CFD_IC0_TILES=0: one launch per hyperplane
   // the reference's cycle (CFD_CYCLE_REFERENCE)
   DevRef ref;
   DevBuf<double> ref_mean;
@@ -722,6 +723,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
   if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_RB_LAYOUT")) g->rb_layout = std::string(env) != "0";
+  if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   size_t N = (size_t)h.count;
@@ -785,7 +787,10 @@ void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031
 }
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
   if (g->host.preconditioner == CFD_IC0) {
-    launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+    if (g->ic0_tiles)
+      launch_ic0_tiles(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+    else
+      launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
     return;
   }
   if (g->opt.cycle == CFD_CYCLE_REFERENCE) {

@@ -560,6 +560,81 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
   z[c] = value / dc;
 }
 
+
+// Tiled wavefront of the same sweeps: 8x8x8 tiles; the tiles on one diagonal
+// I + J + K = P are independent (a face neighbour lies in the same tile or on
+// the neighbouring diagonal, wrap neighbours of a periodic axis on a lower
+// diagonal for the forward sweep and a higher one for the backward sweep), so
+// one launch per diagonal processes them, and inside a tile one wavefront walks
+// its 22 local hyperplanes with LDS between the steps. The tile's factor
+// entries, right-hand side and the neighbours outside the tile are loaded into
+// LDS first, so a step costs one LDS round trip. Each cell repeats
+// k_ic0_fwd / k_ic0_bwd's operations in the same order (bitwise equal).
+constexpr int IT = 8;
+constexpr int IT3 = IT * IT * IT;
+template <bool FWD>
+__global__ void __launch_bounds__(IT3) k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
+                                                  const double* __restrict__ r, double* __restrict__ y, int P, int J0,
+                                                  int K0, int nJ, int nK, int ntx, const PcgState* st) {
+  __shared__ double ys[IT3];
+  if (st && st->status != ST_RUNNING) return;
+  const int J = J0 + (int)blockIdx.x % nJ, K = K0 + (int)blockIdx.x / nJ;
+  const int I = P - J - K;
+  if (I < 0 || I >= ntx) return;
+  const int ox = I * IT, oy = J * IT, oz = K * IT;
+  // one thread per tile cell: its factor entries, right-hand side and outside
+  // neighbours are loaded by all threads at once, before the steps
+  const int li = threadIdx.x;
+  const int a = li & 7, b = (li >> 3) & 7, cz = li >> 6;
+  const int i = ox + a, j = oy + b, k = oz + cz;
+  const bool live = i < g.n[0] && j < g.n[1] && k < g.n[2];
+  double co[6], ext[6], rv = 0.0, dc = 1.0;
+  int lix[6];
+  long long c = 0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) lix[m] = -2;
+  if (live) {
+    c = (long long)i + j * g.sy + k * g.sz;
+    const uint8_t code = codes[c];
+    const int q[3] = {i, j, k};
+    const Ic0Nbrs N = ic0_nbrs(g, code, c, q, FWD);
+    dc = D[c];
+    rv = FWD ? r[c] : y[c];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      if (N.a[m] == 0.0) continue;
+      const unsigned nb = (unsigned)N.idx[m];
+      co[m] = FWD ? N.a[m] / D[nb] : N.a[m] / dc;
+      const unsigned ni = nb % (unsigned)g.n[0], nj = (nb / (unsigned)g.n[0]) % (unsigned)g.n[1];
+      const unsigned nk = nb / (unsigned)g.sz;
+      if ((int)ni >= ox && (int)ni < ox + IT && (int)nj >= oy && (int)nj < oy + IT && (int)nk >= oz && (int)nk < oz + IT) {
+        lix[m] = (int)(ni - ox) + IT * ((int)(nj - oy) + IT * (int)(nk - oz));
+      } else {
+        lix[m] = -1;
+        ext[m] = y[nb];
+      }
+    }
+  }
+  const int plane = a + b + cz;
+  constexpr int S = 3 * (IT - 1);
+  for (int step = 0; step <= S; ++step) {
+    const int sp = FWD ? step : S - step;
+    if (live && plane == sp) {
+      double value = rv;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        if (lix[m] == -2) continue;
+        const double ym = lix[m] >= 0 ? ys[lix[m]] : ext[m];
+        value -= co[m] * ym;
+      }
+      const double v = value / dc;
+      ys[li] = v;
+      y[c] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- multigrid (geometric)
 // Colour of a fine cell: its parity, and with `seams` (a bit per periodic axis
 // of odd length) two more classes per seam for the cells on the last slab of
@@ -1593,7 +1668,9 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
                                               const int* __restrict__ fparent, T cs, double* __restrict__ out64,
                                               const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
-  const long long u = (long long)blockIdx.x * BS + threadIdx.x;
+  // each XCD takes a contiguous range of rows, so the rows above and below a
+  // block's rows are in the same L2
+  const long long u = (long long)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (u >= F.half) return;
   const unsigned hx = (unsigned)F.hx;
   const unsigned row = (unsigned)(u / hx);
@@ -2046,6 +2123,28 @@ static void level_box(const Geo& g, int L, int& j0, int& k0, int& nj, int& nk) {
   k0 = klo;
   nj = jhi >= jlo ? jhi - jlo + 1 : 0;
   nk = khi >= klo ? khi - klo + 1 : 0;
+}
+
+void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
+                      const PcgState* st, hipStream_t s) {
+  Geo tg{};
+  tg.n[0] = (g.n[0] + IT - 1) / IT;
+  tg.n[1] = (g.n[1] + IT - 1) / IT;
+  tg.n[2] = (g.n[2] + IT - 1) / IT;
+  const int diagonals = tg.n[0] + tg.n[1] + tg.n[2] - 2;
+  for (int P = 0; P < diagonals; ++P) {
+    int j0, k0, nj, nk;
+    level_box(tg, P, j0, k0, nj, nk);
+    if (nj * nk == 0) continue;
+    hipLaunchKernelGGL(k_ic0_tile<true>, dim3(nj * nk), dim3(IT3), 0, s, g, codes, D, r, z, P, j0, k0, nj, nk, tg.n[0], st);
+  }
+  for (int P = diagonals - 1; P >= 0; --P) {
+    int j0, k0, nj, nk;
+    level_box(tg, P, j0, k0, nj, nk);
+    if (nj * nk == 0) continue;
+    hipLaunchKernelGGL(k_ic0_tile<false>, dim3(nj * nk), dim3(IT3), 0, s, g, codes, D, (const double*)z, z, P, j0, k0,
+                       nj, nk, tg.n[0], st);
+  }
 }
 
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,

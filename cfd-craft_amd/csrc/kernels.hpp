@@ -34,6 +34,9 @@ void launch_fill(long long N, double* dst, double v, hipStream_t s);
 void launch_add(long long N, double* p, const double* q, hipStream_t s);
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,
                 hipStream_t s);
+// The same sweeps as a tiled wavefront (one launch per tile diagonal).
+void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
+                      const PcgState* st, hipStream_t s);
 // One V-cycle: fine level from the codes, coarse levels lv[0..nlev) (device copy dlv),
 // levels from tail_first on inside one workgroup.
 template <class T, class TB>

@@ -76,6 +76,21 @@ def test_ic0_apply_bitwise(gpu_available, name, case):
     assert np.array_equal(z, o.ic0_apply(r)), name
 
 
+@pytest.mark.parametrize("name,case", small_cases() + [("lid20-odd", Case(cells=[20, 13, 9], size_m=[2.0, 1.3, 0.9],
+                                                                       boundaries=[PERIODIC, WALL, PERIODIC]))],
+                         ids=[n for n, _ in small_cases()] + ["lid20-odd"])
+def test_ic0_tiles_equal_hyperplanes(gpu_available, name, case):
+    """The tiled wavefront of the IC(0) sweeps gives the bits of one launch per hyperplane."""
+    case.preconditioner = IC0
+    r = np.random.default_rng(12).standard_normal(case.count)
+    out = []
+    for tiles in (1, 0):
+        with env(CFD_IC0_TILES=tiles):
+            g = cfdgpu.Grid(case)
+        out.append(g.precondition(r))
+    assert np.array_equal(out[0], out[1])
+
+
 def _manufactured(case):
     o = oracle.Grid(case)
     ex = W.manufactured(case, o.fluid())
