@@ -144,6 +144,29 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
     return out
 
 
+def cavity48(steps=5):
+    """The closed 48^3 lid cavity of the reference's CPU-runnable case: plain FP64
+    IC(0)-PCG at rel-res 1e-10, right-hand side -div(u) of seeded U(-1,1) face
+    velocities (the parity tests hold its iteration count to the oracle's)."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.lid_cavity_48()
+    g = cfdgpu.Grid(case)
+    db = cfdgpu.DeviceArray.from_host(b)
+    ds = cfdgpu.DeviceArray.from_host(t)
+    dx = cfdgpu.DeviceArray(g.count)
+    g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
+    cfdgpu.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
+    cfdgpu.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    return {"solves_per_s": round(1.0 / el, 2), "ms_per_solve": round(el * 1e3, 2), "iterations": info["iterations"],
+            "true_rel_residual": info["relative_residual"]}
+
+
 def large(steps=3):
     """The 256x256x128 prism city (8.4M cells, 200 convex 4-8-vertex prisms, about
     20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10; each field is
@@ -337,6 +360,8 @@ def main():
         extra[name] = dict(solves_per_s=round(k / (time.perf_counter() - t0), 3), iterations=ic["iterations"],
                            true_rel_residual=ic["relative_residual"])
     g.set_options(cfdgpu.PRECISION_FP64)
+    if not args.no_extra:
+        extra["cavity_48_ic0"] = cavity48()
     if not args.no_served:
         extra["served"] = served()
     if not args.no_large:

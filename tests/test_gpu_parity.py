@@ -381,3 +381,13 @@ def test_reference_cycle_published_iterations(gpu_available, cells, its):
     assert abs(info["iterations"] - its) <= 1, info
     assert info["relative_residual"] <= 1.01e-10
     assert math.sqrt(np.mean((x - ex) ** 2)) <= 1e-8
+
+
+def test_lid_cavity_48_ic0_matches_oracle(gpu_available):
+    """The reference's CPU-runnable case: closed 48^3 cavity, IC(0)-PCG at 1e-10,
+    right-hand side -div(u) of seeded face velocities."""
+    case, b, t = W.lid_cavity_48()
+    xo, io = oracle.Grid(case).solve_scaled(b, t, 1e-10, 1200)
+    x, info = cfdgpu.Grid(case).solve_scaled(b, t, 1e-10, 1200)
+    assert abs(info["iterations"] - io["iterations"]) <= 1, (info, io)
+    assert rel(x, xo) <= 1e-10
