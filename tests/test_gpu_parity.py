@@ -391,3 +391,35 @@ def test_lid_cavity_48_ic0_matches_oracle(gpu_available):
     x, info = cfdgpu.Grid(case).solve_scaled(b, t, 1e-10, 1200)
     assert abs(info["iterations"] - io["iterations"]) <= 1, (info, io)
     assert rel(x, xo) <= 1e-10
+
+
+# ---------------------------------------------------------------- anisotropy suite
+def _aniso_shapes():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "anisotropy_shapes.json")) as f:
+        return json.load(f)["shapes"]
+
+
+@pytest.mark.parametrize("name", list(_aniso_shapes()))
+def test_published_anisotropy_on_device(gpu_available, name):
+    """The 18 published shapes on the device: IC(0)-PCG iterations at tol 1e-14
+    (and, where the reference keeps point smoothing, the reference multigrid
+    cycle's), recovery RMS <= 1e-8 and true residual <= 1.1e-12."""
+    from test_oracle_goldens import aniso_case, aniso_rhs
+
+    spec = _aniso_shapes()[name]
+    for pc in ("ic0", "mg"):
+        want = spec["ic0_iterations"] if pc == "ic0" else spec["point_mg_iterations"]
+        if want is None:
+            continue
+        case = aniso_case(spec, pc)
+        g = cfdgpu.Grid(case)
+        if pc == "mg":
+            g.set_options(cfdgpu.PRECISION_FP64, cycle=cfdgpu.CYCLE_REFERENCE)
+        ex, b = aniso_rhs(spec, g.count, g.apply)
+        x, info = g.solve(b, 1e-14, 1200)
+        assert abs(info["iterations"] - want) <= 1, (pc, info["iterations"], want)
+        assert float(np.sqrt(np.mean((x - ex) ** 2))) <= 1e-8
+        assert info["relative_residual"] <= 1.1e-12

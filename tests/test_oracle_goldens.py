@@ -7,7 +7,10 @@ Goldens (no import of the reference needed):
   * multigrid grid / operator complexity and the 48x2x48 hierarchy sizes
     (pressure-multigrid.md:129-142);
   * canyon-93k: 166 IC(0) iterations, true residual 6.9540e-12, RMS 3.1094e-12
-    (docs/evidence/native-cpu-performance.md:210-211).
+    (docs/evidence/native-cpu-performance.md:210-211);
+  * the 18-shape anisotropy suite: IC(0) iterations at tol 1e-14, the point-multigrid
+    iterations of the three shapes below the line-smoother threshold, and the suite's
+    maximum recovery error and true residual (docs/evidence/anisotropic-pressure.md).
 """
 import json
 import math
@@ -79,3 +82,62 @@ def test_published_canyon_93k():
     assert info["iterations"] == 166
     assert f"{info['relative_residual']:.4e}" == "6.9540e-12"
     assert f"{rms:.4e}" == "3.1094e-12"
+
+
+# ---------------------------------------------------------------- anisotropy suite
+def _aniso():
+    with open(os.path.join(os.path.dirname(__file__), "golden", "anisotropy_shapes.json")) as f:
+        return json.load(f)
+
+
+def aniso_case(spec, preconditioner):
+    from cfdgpu.case import IC0, MULTIGRID, OPEN, PERIODIC, WALL, Case
+
+    bx = {"wall": WALL, "periodic": PERIODIC, "open": OPEN}[spec["x"]]
+    return Case(cells=spec["cells"], size_m=spec["size_m"], boundaries=[bx, PERIODIC, WALL],
+                velocity_m_s=[1.0, 0.0, 0.0] if bx == OPEN else [0.0, 0.0, 0.0],
+                preconditioner=IC0 if preconditioner == "ic0" else MULTIGRID)
+
+
+def aniso_rhs(spec, count, apply):
+    i = np.arange(count, dtype=float)
+    ex = np.sin(0.73 * i) + 0.3 * np.cos(0.17 * i)
+    if spec["x"] != "open":
+        ex -= ex.mean()
+    return ex, apply(ex)
+
+
+def test_published_anisotropy_ic0():
+    """The 18 shapes of the anisotropy suite: IC(0)-PCG iterations at tol 1e-14 equal
+    the published ones; the largest recovery error and true residual over the
+    suite equal the published maxima (anisotropic-pressure.md:27-30,43-60)."""
+    data = _aniso()
+    rms_max = res_max = 0.0
+    for name, spec in data["shapes"].items():
+        o = oracle.Grid(aniso_case(spec, "ic0"))
+        ex, b = aniso_rhs(spec, o.count, o.apply)
+        x, info = o.solve(b, 1e-14, 1200)
+        assert info["iterations"] == spec["ic0_iterations"], name
+        rms = float(np.sqrt(np.mean((x - ex) ** 2)))
+        assert rms <= data["gates"]["rms_max"] and info["relative_residual"] <= data["gates"]["true_residual_max"]
+        rms_max = max(rms_max, rms)
+        res_max = max(res_max, info["relative_residual"])
+    assert float(f"{rms_max:.3e}") in (1.844e-10, 1.843e-10, 1.845e-10), rms_max
+    assert abs(res_max - data["gates"]["published_max_true_residual"]) <= 0.005e-14, res_max
+
+
+def test_published_anisotropy_point_multigrid():
+    """The three shapes below the line-smoother threshold keep point Gauss-Seidel:
+    published multigrid iterations 8 / 13 / 24; the others need the line smoother,
+    which the oracle refuses (CFD-E036)."""
+    data = _aniso()
+    for name, spec in data["shapes"].items():
+        o = oracle.Grid(aniso_case(spec, "mg"))
+        ex, b = aniso_rhs(spec, o.count, o.apply)
+        if spec["point_mg_iterations"] is None:
+            if name in ("thin100", "thin400"):
+                with pytest.raises(oracle.OracleError):
+                    o.solve(b, 1e-14, 1200)
+            continue
+        x, info = o.solve(b, 1e-14, 1200)
+        assert info["iterations"] == spec["point_mg_iterations"], name
