@@ -13,9 +13,10 @@
 // iteration counts (docs/evidence/pressure-multigrid.md:34-38,
 // docs/evidence/native-cpu-performance.md:210-211), its hierarchy sizes
 // (pressure-multigrid.md:129-142), closed-form stencils, dense and pivoted
-// reference solves on tiny grids, and the Galerkin / symmetry / connectivity
-// invariants of its own unit tests. The anisotropic line smoother
-// (multigrid/line.rs) is not restated: a grid that selects it is refused.
+// reference solves on tiny grids, the Galerkin / symmetry / connectivity
+// invariants of its own unit tests, and the 18-shape anisotropy suite
+// (docs/evidence/anisotropic-pressure.md:43-60: IC(0) and multigrid iterations,
+// the latter with the line and small-plane block smoothers of multigrid/line.rs).
 #include "oracle.h"
 
 #include <algorithm>
@@ -238,11 +239,12 @@ vector<std::pair<size_t, double>> axis_weights(double x, const vector<double>& c
   if (t != 0.0) out.push_back({hi, t});
   return out;
 }
-// multigrid.rs:330-371
+// multigrid.rs:322-371 (Transfer::build: every axis of more than two cells;
+// build_selected: the axes the caller selects)
 Transfer transfer_build(const size_t n[3], const vector<double> centers[3], const bool periodic[3], const double extent[3],
-                        vector<double> coarse[3]) {
+                        vector<double> coarse[3], const bool* selected = nullptr) {
   bool coarsen[3];
-  for (int a = 0; a < 3; ++a) coarsen[a] = n[a] > 2;
+  for (int a = 0; a < 3; ++a) coarsen[a] = selected ? selected[a] : n[a] > 2;
   size_t nc[3];
   for (int a = 0; a < 3; ++a) {
     coarse[a].clear();
@@ -345,11 +347,124 @@ void center_plain(vector<double>& x, bool anchored) {  // multigrid.rs:446-453
   for (double& v : x) v -= mean;
 }
 
+// multigrid/line.rs:25-190: exact banded solves of lines (or small planes) of
+// one level, used as block Gauss-Seidel on anisotropic grids.
+struct BlockSmoother {
+  vector<int> axes;
+  size_t length = 0, bandwidth = 0;
+  vector<size_t> nodes, outside_offsets, outside_columns;
+  vector<double> outside_values, factor, scratch;
+
+  int build(const Matrix& m, const size_t n[3], const vector<int>& group, const bool periodic[3]) {
+    axes = group;
+    size_t count = m.diagonal.size();
+    length = 1;
+    for (int a : axes) length *= n[a];
+    nodes.clear();
+    for (size_t i = 0; i < count; ++i) {
+      size_t q[3] = {i % n[0], i / n[0] % n[1], i / (n[0] * n[1])};
+      bool base = true;
+      for (int a : axes) base &= q[a] == 0;
+      if (!base) continue;
+      for (size_t at = 0; at < length; ++at) {
+        // a periodic cycle is folded (0, n-1, 1, n-2, ...) to keep the band narrow
+        size_t remainder = at;
+        for (int axis : axes) {
+          size_t k = remainder % n[axis];
+          remainder /= n[axis];
+          q[axis] = !periodic[axis] ? k : (k % 2 == 0 ? k / 2 : n[axis] - 1 - k / 2);
+        }
+        nodes.push_back((q[2] * n[1] + q[1]) * n[0] + q[0]);
+      }
+    }
+    vector<size_t> position(count, 0);
+    for (size_t at = 0; at < nodes.size(); ++at) position[nodes[at]] = at;
+    bandwidth = 0;
+    for (size_t i = 0; i < count; ++i) {
+      size_t pi = position[i];
+      for (size_t at = m.offsets[i]; at < m.offsets[i + 1]; ++at) {
+        size_t pj = position[m.columns[at]];
+        if (pi / length == pj / length) bandwidth = std::max(bandwidth, pi > pj ? pi - pj : pj - pi);
+      }
+    }
+    size_t stride = bandwidth + 1;
+    outside_offsets.assign(1, 0);
+    outside_columns.clear();
+    outside_values.clear();
+    for (size_t at = 0; at < nodes.size(); ++at) {
+      size_t cell = nodes[at];
+      for (size_t e = m.offsets[cell]; e < m.offsets[cell + 1]; ++e) {
+        size_t col = m.columns[e];
+        if (position[col] / length != at / length) {
+          outside_columns.push_back(col);
+          outside_values.push_back(m.values[e]);
+        }
+      }
+      outside_offsets.push_back(outside_columns.size());
+    }
+    factor.assign(count * stride, 0.0);
+    for (size_t at = 0; at < nodes.size(); ++at) {
+      size_t cell = nodes[at];
+      factor[at * stride] = m.diagonal[cell];
+      for (size_t e = m.offsets[cell]; e < m.offsets[cell + 1]; ++e) {
+        size_t col = position[m.columns[e]];
+        if (col / length == at / length && col < at) factor[at * stride + at - col] += m.values[e];
+      }
+    }
+    for (size_t line = 0; line < count / length; ++line) {
+      size_t base = line * length * stride;
+      for (size_t i = 0; i < length; ++i)
+        for (size_t j = (i > bandwidth ? i - bandwidth : 0); j <= i; ++j) {
+          size_t row = base + i * stride, other = base + j * stride;
+          double value = factor[row + i - j];
+          for (size_t k = (i > bandwidth ? i - bandwidth : 0); k < j; ++k) value -= factor[row + i - k] * factor[other + j - k];
+          if (i == j) {
+            if (!std::isfinite(value) || value <= 0.0)
+              return fail(36, "CFD-E036: numerics.pressure_preconditioner: multigrid line Cholesky lost positive definiteness");
+            factor[row] = std::sqrt(value);
+          } else {
+            factor[row + i - j] = value / factor[other];
+          }
+        }
+    }
+    for (size_t row = 0; row < count; ++row) factor[row * stride] = 1.0 / factor[row * stride];
+    scratch.assign(length, 0.0);
+    return 0;
+  }
+  void sweep(const vector<double>& rhs, vector<double>& x, bool backward) {
+    size_t count = nodes.size() / length, stride = bandwidth + 1;
+    for (size_t at = 0; at < count; ++at) {
+      size_t line = backward ? count - 1 - at : at;
+      size_t start = line * length, base = start * stride;
+      for (size_t i = 0; i < length; ++i) {
+        double value = rhs[nodes[start + i]];
+        for (size_t e = outside_offsets[start + i]; e < outside_offsets[start + i + 1]; ++e)
+          value -= outside_values[e] * x[outside_columns[e]];
+        scratch[i] = value;
+      }
+      for (size_t i = 0; i < length; ++i) {
+        size_t row = base + i * stride;
+        double value = scratch[i];
+        for (size_t j = (i > bandwidth ? i - bandwidth : 0); j < i; ++j) value -= factor[row + i - j] * scratch[j];
+        scratch[i] = value * factor[row];
+      }
+      for (size_t i = length; i-- > 0;) {
+        double value = scratch[i];
+        for (size_t j = i + 1; j < std::min(length, i + bandwidth + 1); ++j)
+          value -= factor[base + j * stride + j - i] * scratch[j];
+        scratch[i] = value * factor[base + i * stride];
+      }
+      for (size_t i = 0; i < length; ++i) x[nodes[start + i]] = scratch[i];
+    }
+  }
+};
+
 struct Level {  // multigrid.rs:435-521
   Matrix matrix;
   Transfer transfer;
   vector<double> rhs, x, residual, cholesky;
   bool anchored = false;
+  vector<BlockSmoother> lines;
   Level(Matrix m, Transfer t, bool anch) : matrix(std::move(m)), transfer(std::move(t)), anchored(anch) {
     size_t n = matrix.diagonal.size();
     rhs.assign(n, 0.0);
@@ -500,7 +615,14 @@ void cycle_plain(vector<Level>& levels, size_t at, Work& work) {  // multigrid.r
     return;
   }
   std::fill(level.x.begin(), level.x.end(), 0.0);
-  for (int pass = 0; pass < SWEEPS; ++pass) level.matrix.sweep(level.rhs, level.x, false);
+  // multigrid.rs:676-688: one block sweep per side on a level with block smoothers
+  const int sweeps = level.lines.empty() ? SWEEPS : 1;
+  for (int pass = 0; pass < sweeps; ++pass) {
+    if (level.lines.empty())
+      level.matrix.sweep(level.rhs, level.x, false);
+    else
+      level.lines[(size_t)pass % level.lines.size()].sweep(level.rhs, level.x, false);
+  }
   center_plain(level.x, level.anchored);
   level.matrix.product(level.x, level.residual);
   for (size_t i = 0; i < level.residual.size(); ++i) level.residual[i] = level.rhs[i] - level.residual[i];
@@ -521,10 +643,16 @@ void cycle_plain(vector<Level>& levels, size_t at, Work& work) {  // multigrid.r
       s += lv.transfer.weights[k] * nx.x[lv.transfer.columns[k]];
     lv.x[i] += s;
   }
-  for (int pass = 0; pass < SWEEPS; ++pass) lv.matrix.sweep(lv.rhs, lv.x, true);
+  const int lsweeps = lv.lines.empty() ? SWEEPS : 1;
+  for (int pass = 0; pass < lsweeps; ++pass) {
+    if (lv.lines.empty())
+      lv.matrix.sweep(lv.rhs, lv.x, true);
+    else
+      lv.lines[(size_t)(lsweeps - 1 - pass) % lv.lines.size()].sweep(lv.rhs, lv.x, true);
+  }
   center_plain(lv.x, lv.anchored);
   int64_t cells = (int64_t)lv.x.size();
-  work.smoothing += 2 * SWEEPS * cells;
+  work.smoothing += 2 * lsweeps * cells;
   work.residual += cells;
 }
 
@@ -825,18 +953,24 @@ void factor_apply(const Factor& F, const double* r, double* z, size_t n) {
   }
 }
 
-// multigrid.rs:33-47
-bool needs_line_smoother(const orc_grid& g) {
+// multigrid.rs:33-47: the axes whose face weight is within a factor 4 of the
+// largest, when the weights spread by more than 25 over the axes of more than
+// two cells; empty otherwise.
+vector<int> line_axes(const orc_grid& g) {
   double weights[3];
   for (int a = 0; a < 3; ++a) weights[a] = 1.0 / (g.dx[a] * g.dx[a]);
   for (double w : weights)
-    if (!std::isfinite(w) || w <= 0.0) return false;
+    if (!std::isfinite(w) || w <= 0.0) return {};
   double minimum = INFINITY;
   for (int a = 0; a < 3; ++a)
     if (g.n[a] > 2) minimum = std::fmin(minimum, weights[a]);
   double maximum = 0.0;
   for (double w : weights) maximum = std::fmax(maximum, w);
-  return !(maximum / minimum <= 25.0);
+  if (maximum / minimum <= 25.0) return {};
+  vector<int> out;
+  for (int a = 0; a < 3; ++a)
+    if (weights[a] >= maximum / 4.0) out.push_back(a);
+  return out;
 }
 
 // masked.rs:90-126
@@ -989,7 +1123,7 @@ int hierarchy_build(const orc_grid& g, std::unique_ptr<Hierarchy>& out) {
     out = std::move(h);
     return 0;
   }
-  if (needs_line_smoother(g)) return fail(36, "CFD-E036: oracle: the anisotropic line smoother (multigrid/line.rs) is not restated");
+  const vector<int> axes = line_axes(g);
   size_t n[3] = {g.n[0], g.n[1], g.n[2]};
   vector<double> centers[3];
   double extent[3];
@@ -1025,14 +1159,46 @@ int hierarchy_build(const orc_grid& g, std::unique_ptr<Hierarchy>& out) {
       break;
     }
     vector<double> coarse[3];
-    Transfer transfer = transfer_build(n, centers, periodic, extent, coarse);
+    Transfer transfer;
+    vector<vector<int>> groups;
+    if (axes.empty()) {
+      transfer = transfer_build(n, centers, periodic, extent, coarse);
+    } else {
+      // multigrid.rs:582-611: keep the weak axes until the stiff ones have collapsed
+      bool stiff_can_coarsen = false;
+      for (int a : axes) stiff_can_coarsen |= n[a] > 2;
+      bool sel[3];
+      for (int a = 0; a < 3; ++a) {
+        bool in_axes = std::find(axes.begin(), axes.end(), a) != axes.end();
+        sel[a] = n[a] > 2 && (!stiff_can_coarsen || in_axes);
+      }
+      transfer = transfer_build(n, centers, periodic, extent, coarse, sel);
+      size_t span = 1;
+      for (int a : axes) span *= n[a];
+      if (axes.size() > 1 && span <= 16) {
+        groups.push_back(axes);
+      } else {
+        int best = axes[0];
+        for (int a : axes)
+          if (g.dx[a] < g.dx[best]) best = a;
+        groups.push_back({best});
+      }
+    }
+    Level level(std::move(matrix), Transfer(), anch);
+    for (auto& grp : groups) {
+      BlockSmoother b;
+      int e = b.build(level.matrix, n, grp, periodic);
+      if (e) return e;
+      level.lines.push_back(std::move(b));
+    }
     for (int a = 0; a < 3; ++a) n[a] = coarse[a].size();
     vector<Edge> ce;
     vector<double> ca;
     galerkin(transfer, edges, anchors, n[0] * n[1] * n[2], ce, ca);
     edges = std::move(ce);
     for (int a = 0; a < 3; ++a) centers[a] = coarse[a];
-    h->levels.emplace_back(std::move(matrix), std::move(transfer), anch);
+    level.transfer = std::move(transfer);
+    h->levels.push_back(std::move(level));
     anchors = std::move(ca);
   }
   for (auto& l : h->levels) {
