@@ -266,6 +266,26 @@ def urban_block(seed: int = 2, n=(128, 128, 64), h: float = 2.0, boxes: int = 24
     return _finish(case, rng)
 
 
+def wind_campaign(direction_deg: float, seed: int = 5, n=(128, 128, 64), h: float = 2.0, boxes: int = 24,
+                  preconditioner=MULTIGRID):
+    """One direction of a wind-direction campaign: the same seeded boxes (at least
+    4 cells from every side boundary), x and y open with the log-law wind blowing
+    from direction_deg, ground_open_top z. The direction decides which open faces
+    are outlets, so each direction is its own operator."""
+    n = [int(v) for v in n]
+    case = Case(cells=n, size_m=[n[0] * h, n[1] * h, n[2] * h], boundaries=[OPEN, OPEN, GROUND_OPEN_TOP],
+                preconditioner=preconditioner, tolerance=1e-8, max_iterations=2000)
+    _neutral_rans(case, direction=direction_deg)
+    rng = np.random.default_rng(seed)
+    for k in range(boxes):
+        wx, wy = (int(v) for v in rng.integers(8, 25, 2))
+        hz = min(int(rng.integers(10, 41)), n[2] - 1)
+        x0 = int(rng.integers(4, n[0] - 4 - wx + 1))
+        y0 = int(rng.integers(4, n[1] - 4 - wy + 1))
+        case.blocks.append(Block(min_m=[x0 * h, y0 * h, 0.0], max_m=[(x0 + wx) * h, (y0 + wy) * h, hz * h], id=f"b{k}"))
+    return _finish(case, rng)
+
+
 def _convex_polygon(rng, cx, cy, r, k):
     ang = np.sort(rng.uniform(0, 2 * math.pi, k))
     rad = r * rng.uniform(0.7, 1.0, k)

@@ -583,6 +583,31 @@ int build_ref(cfd_grid* g) {
     C.r0 = upd(zero);
     if (!C.diag0 || !C.rhs0 || !C.r0) return CFD_ENOMEM;
   }
+  // block smoothers: the level's lines in node numbers (cells on the fine level)
+  auto lines_of = [&](const AggLevel& L, bool fine, RefLines& B) -> int {
+    B = RefLines();
+    if (L.line_length == 0) return 0;
+    B.length = (int)L.line_length;
+    B.bandwidth = (int)L.line_bandwidth;
+    std::vector<int32_t> nodes = L.line_nodes, ocol = L.line_ocol;
+    if (fine) {
+      for (auto& v : nodes) v = (int32_t)H.cell_of_node[(size_t)v];
+      for (auto& v : ocol) v = (int32_t)H.cell_of_node[(size_t)v];
+    }
+    B.nodes = upi(nodes);
+    B.ocol = upi(ocol);
+    B.ooff = upi64(L.line_ooff);
+    B.oval = upd(L.line_oval);
+    B.factor = upd(L.line_factor);
+    std::vector<double> zero(L.line_nodes.size(), 0.0);
+    B.scratch = upd(zero);
+    B.fwd = upi(L.line_fwd);
+    B.bwd = upi(L.line_bwd);
+    B.fwd_ptr.assign(L.line_fwd_ptr.begin(), L.line_fwd_ptr.end());
+    B.bwd_ptr.assign(L.line_bwd_ptr.begin(), L.line_bwd_ptr.end());
+    return (!B.nodes || !B.ooff || !B.factor || !B.scratch || !B.fwd || !B.bwd) ? CFD_ENOMEM : 0;
+  };
+  if ((rc = lines_of(H.levels[0], true, C.fine_lines))) return rc;
   const int nlev = (int)H.levels.size() - 1;
   C.fparent = nullptr;
   C.ftptr = nullptr;
@@ -634,6 +659,7 @@ int build_ref(cfd_grid* g) {
     rc2.bwd_rows = upi(L.bwd_rows);
     rc2.fwd_ptr.assign(L.fwd_ptr.begin(), L.fwd_ptr.end());
     rc2.bwd_ptr.assign(L.bwd_ptr.begin(), L.bwd_ptr.end());
+    if ((rc = lines_of(L, false, rc2.lines))) return rc;
     if (!d.x || !d.b || !rc2.r || !d.diag) return CFD_ENOMEM;
     C.lv.push_back(std::move(rc2));
   }

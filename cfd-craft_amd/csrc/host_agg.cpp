@@ -489,11 +489,11 @@ void axis_weights(double x, const std::vector<double>& centers, bool periodic, d
 // Trilinear interpolation rows of one level (multigrid.rs:330-371): axes of more
 // than two cells are paired, the coarse centre being the mean of the pair.
 void transfer_rows(const int64_t n[3], const std::vector<double> centers[3], const bool periodic[3],
-                   const double extent[3], std::vector<double> coarse[3], AggLevel& L) {
+                   const double extent[3], std::vector<double> coarse[3], AggLevel& L, const bool coarsen[3]) {
   int64_t nc[3];
   for (int a = 0; a < 3; ++a) {
     coarse[a].clear();
-    const size_t step = n[a] > 2 ? 2 : 1;
+    const size_t step = coarsen[a] ? 2 : 1;
     for (size_t s0 = 0; s0 < centers[a].size(); s0 += step) {
       const size_t e = std::min(centers[a].size(), s0 + step);
       double sum = -0.0;
@@ -603,21 +603,137 @@ void galerkin(const AggLevel& F, int64_t nc, AggLevel& C) {
   }
 }
 
-// Anisotropic spacing: the reference switches to its line smoother when the
-// largest axis weight exceeds 25 times the smallest among axes of more than two
-// cells (multigrid.rs:33-47).
-bool needs_line_smoother(const HostGrid& g) {
+// Anisotropic spacing (multigrid.rs:33-47): when the axis weights 1/dx^2
+// spread by more than 25 over the axes of more than two cells, the axes within a
+// factor 4 of the largest weight are relaxed by line blocks; none otherwise.
+std::vector<int> line_axes(const HostGrid& g) {
   double w[3];
   for (int a = 0; a < 3; ++a) {
     w[a] = 1.0 / (g.dx[a] * g.dx[a]);
-    if (!std::isfinite(w[a]) || w[a] <= 0.0) return false;
+    if (!std::isfinite(w[a]) || w[a] <= 0.0) return {};
   }
   double lo = INFINITY, hi = 0.0;
   for (int a = 0; a < 3; ++a) {
     if (g.n[a] > 2) lo = std::fmin(lo, w[a]);
     hi = std::fmax(hi, w[a]);
   }
-  return !(hi / lo <= 25.0);
+  if (hi / lo <= 25.0) return {};
+  std::vector<int> out;
+  for (int a = 0; a < 3; ++a)
+    if (w[a] >= hi / 4.0) out.push_back(a);
+  return out;
+}
+
+// Block smoother of one level (multigrid/line.rs:38-154): the nodes of each
+// line in order (a periodic cycle folded as 0, n-1, 1, n-2, ... to keep the band
+// narrow), the entries leaving the block, and the banded Cholesky factor with
+// reciprocal diagonal; then the lines' dependency levels for parallel sweeps.
+int build_lines(AggLevel& L, const int64_t n[3], const std::vector<int>& axes, const bool periodic[3]) {
+  const int64_t count = L.n;
+  int64_t length = 1;
+  for (int a : axes) length *= n[a];
+  L.line_length = length;
+  L.line_nodes.clear();
+  for (int64_t i = 0; i < count; ++i) {
+    int64_t q[3] = {i % n[0], i / n[0] % n[1], i / (n[0] * n[1])};
+    bool base = true;
+    for (int a : axes) base &= q[a] == 0;
+    if (!base) continue;
+    for (int64_t at = 0; at < length; ++at) {
+      int64_t rest = at;
+      for (int a : axes) {
+        const int64_t k = rest % n[a];
+        rest /= n[a];
+        q[a] = !periodic[a] ? k : (k % 2 == 0 ? k / 2 : n[a] - 1 - k / 2);
+      }
+      L.line_nodes.push_back((int32_t)((q[2] * n[1] + q[1]) * n[0] + q[0]));
+    }
+  }
+  std::vector<int64_t> pos((size_t)count);
+  for (size_t at = 0; at < L.line_nodes.size(); ++at) pos[(size_t)L.line_nodes[at]] = (int64_t)at;
+  int64_t bw = 0;
+  for (int64_t i = 0; i < count; ++i)
+    for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
+      const int64_t pi = pos[(size_t)i], pj = pos[(size_t)L.col[(size_t)e]];
+      if (pi / length == pj / length) bw = std::max(bw, pi > pj ? pi - pj : pj - pi);
+    }
+  L.line_bandwidth = bw;
+  const int64_t stride = bw + 1;
+  L.line_ooff.assign(1, 0);
+  L.line_ocol.clear();
+  L.line_oval.clear();
+  for (int64_t at = 0; at < count; ++at) {
+    const int32_t cell = L.line_nodes[(size_t)at];
+    for (int64_t e = L.ptr[(size_t)cell]; e < L.ptr[(size_t)cell + 1]; ++e) {
+      const int32_t c = L.col[(size_t)e];
+      if (pos[(size_t)c] / length != at / length) {
+        L.line_ocol.push_back(c);
+        L.line_oval.push_back(L.val[(size_t)e]);
+      }
+    }
+    L.line_ooff.push_back((int64_t)L.line_ocol.size());
+  }
+  std::vector<double>& F = L.line_factor;
+  F.assign((size_t)(count * stride), 0.0);
+  for (int64_t at = 0; at < count; ++at) {
+    const int32_t cell = L.line_nodes[(size_t)at];
+    F[(size_t)(at * stride)] = L.diag[(size_t)cell];
+    for (int64_t e = L.ptr[(size_t)cell]; e < L.ptr[(size_t)cell + 1]; ++e) {
+      const int64_t c = pos[(size_t)L.col[(size_t)e]];
+      if (c / length == at / length && c < at) F[(size_t)(at * stride + at - c)] += L.val[(size_t)e];
+    }
+  }
+  for (int64_t line = 0; line < count / length; ++line) {
+    const int64_t base = line * length * stride;
+    for (int64_t i = 0; i < length; ++i)
+      for (int64_t j = std::max<int64_t>(0, i - bw); j <= i; ++j) {
+        const int64_t row = base + i * stride, other = base + j * stride;
+        double v = F[(size_t)(row + i - j)];
+        for (int64_t k = std::max<int64_t>(0, i - bw); k < j; ++k) v -= F[(size_t)(row + i - k)] * F[(size_t)(other + j - k)];
+        if (i == j) {
+          if (!std::isfinite(v) || v <= 0.0) {
+            set_error("CFD-E036: numerics.pressure_preconditioner: multigrid line Cholesky lost positive definiteness");
+            return CFD_E036;
+          }
+          F[(size_t)row] = std::sqrt(v);
+        } else {
+          F[(size_t)(row + i - j)] = v / F[(size_t)other];
+        }
+      }
+  }
+  for (int64_t row = 0; row < count; ++row) F[(size_t)(row * stride)] = 1.0 / F[(size_t)(row * stride)];
+  // dependency levels of the block Gauss-Seidel sweep over lines
+  const int64_t nl = count / length;
+  std::vector<int32_t> depth((size_t)nl, 0);
+  auto group = [&](std::vector<int64_t>& ptr, std::vector<int32_t>& lines) {
+    int32_t most = 0;
+    for (int32_t d : depth) most = std::max(most, d);
+    ptr.assign((size_t)most + 2, 0);
+    for (int32_t d : depth) ptr[(size_t)d + 1] += 1;
+    for (int32_t d = 0; d <= most; ++d) ptr[(size_t)d + 1] += ptr[(size_t)d];
+    lines.assign((size_t)nl, 0);
+    std::vector<int64_t> at(ptr.begin(), ptr.end() - 1);
+    for (int64_t l = 0; l < nl; ++l) lines[(size_t)at[(size_t)depth[(size_t)l]]++] = (int32_t)l;
+  };
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool forward = pass == 0;
+    std::fill(depth.begin(), depth.end(), 0);
+    for (int64_t k = 0; k < nl; ++k) {
+      const int64_t l = forward ? k : nl - 1 - k;
+      int32_t d = 0;
+      for (int64_t at = l * length; at < (l + 1) * length; ++at)
+        for (int64_t e = L.line_ooff[(size_t)at]; e < L.line_ooff[(size_t)at + 1]; ++e) {
+          const int64_t other = pos[(size_t)L.line_ocol[(size_t)e]] / length;
+          if (forward ? other < l : other > l) d = std::max(d, depth[(size_t)other] + 1);
+        }
+      depth[(size_t)l] = d;
+    }
+    if (forward)
+      group(L.line_fwd_ptr, L.line_fwd);
+    else
+      group(L.line_bwd_ptr, L.line_bwd);
+  }
+  return 0;
 }
 
 int build_unmasked(const HostGrid& g, AggHierarchy& h) {
@@ -657,6 +773,7 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
     periodic[a] = g.periodic[a] && n[a] >= 2;
   }
   h.levels.push_back(std::move(L0));
+  const std::vector<int> axes = line_axes(g);
   for (;;) {
     AggLevel& L = h.levels.back();
     L.closed_ptr.assign(1, 0);
@@ -680,7 +797,31 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
       break;
     }
     std::vector<double> coarse[3];
-    transfer_rows(n, centers, periodic, extent, coarse, L);
+    bool coarsen[3];
+    for (int a = 0; a < 3; ++a) coarsen[a] = n[a] > 2;
+    if (!axes.empty()) {
+      // multigrid.rs:582-611: the weak axes wait until the stiff ones have
+      // collapsed; one block group per level: the stiff axes together when their
+      // span is at most 16 nodes, else the stiffest axis alone
+      bool stiff_can_coarsen = false;
+      for (int a : axes) stiff_can_coarsen |= n[a] > 2;
+      for (int a = 0; a < 3; ++a)
+        coarsen[a] = n[a] > 2 && (!stiff_can_coarsen || std::find(axes.begin(), axes.end(), a) != axes.end());
+      int64_t span = 1;
+      for (int a : axes) span *= n[a];
+      std::vector<int> grp;
+      if (axes.size() > 1 && span <= 16) {
+        grp = axes;
+      } else {
+        int best = axes[0];
+        for (int a : axes)
+          if (g.dx[a] < g.dx[best]) best = a;
+        grp = {best};
+      }
+      int e = build_lines(L, n, grp, periodic);
+      if (e) return e;
+    }
+    transfer_rows(n, centers, periodic, extent, coarse, L, coarsen);
     for (int a = 0; a < 3; ++a) n[a] = (int64_t)coarse[a].size();
     AggLevel C;
     C.n = n[0] * n[1] * n[2];
@@ -757,11 +898,6 @@ int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h) {
   if (g.fluid_count < g.count || g.anchored.size() != 1) {
     rc = build_agg_hierarchy(g, h, 32);
   } else {
-    if (needs_line_smoother(g)) {
-      set_error("multigrid: the reference cycle on this grid needs the anisotropic line smoother (multigrid/line.rs), "
-                "which the device does not provide; use CFD_CYCLE_RB");
-      return CFD_EUNSUPPORTED;
-    }
     rc = build_unmasked(g, h);
   }
   if (rc) return rc;

@@ -96,6 +96,16 @@ struct AggLevel {
   // longest chain of lower-numbered neighbours, backward by higher-numbered ones
   std::vector<int64_t> fwd_ptr, bwd_ptr;
   std::vector<int32_t> fwd_rows, bwd_rows;
+  // block smoother of an anisotropic level (multigrid/line.rs): lines (or small
+  // planes) of `length` nodes in `nodes` order, banded factor of half-bandwidth
+  // `bandwidth` (reciprocal diagonal), the entries outside each node's block,
+  // and the lines grouped by dependency level for the forward and backward sweeps
+  int64_t line_length = 0, line_bandwidth = 0;
+  std::vector<int32_t> line_nodes, line_ocol;
+  std::vector<int64_t> line_ooff;
+  std::vector<double> line_oval, line_factor;
+  std::vector<int64_t> line_fwd_ptr, line_bwd_ptr;
+  std::vector<int32_t> line_fwd, line_bwd;
   // edge list (a, b, w) while the next level is built
   std::vector<int32_t> ea, eb;
   std::vector<double> ew;
@@ -126,9 +136,9 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
                         int64_t host_block_max = INT64_MAX);
 // The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked
 // aggregation when the grid has solids or several components, else trilinear
-// interpolation with Galerkin coarse operators; with the dependency levels of
-// the lexicographic sweeps on levels >= 1. CFD_EUNSUPPORTED where the reference
-// would take its anisotropic line smoother (multigrid.rs:33-47).
+// interpolation with Galerkin coarse operators (with line or small-plane block
+// smoothers and semicoarsening on anisotropic grids, multigrid.rs:33-47,573-611);
+// with the dependency levels of the lexicographic sweeps on levels >= 1.
 int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h);
 // Renumbers levels >= 1 so that each colour class (an independent set) is contiguous.
 void colour_agg_hierarchy(AggHierarchy& h);

@@ -1962,6 +1962,42 @@ __global__ void __launch_bounds__(BS) k_ref_prolong(long long n, double* __restr
   x[i] = v;
 }
 
+// Exact solve of each line of one dependency level (BlockSmoother::sweep,
+// multigrid/line.rs:155-190): rhs minus the entries leaving the block, banded
+// forward and backward substitution with the reciprocal diagonal, stored back.
+// One thread per line; the lines of one level are not coupled to each other.
+__global__ void __launch_bounds__(BS) k_ref_lines(int length, int bw, const int* __restrict__ nodes,
+                                                  const double* __restrict__ factor, const int* __restrict__ ooff,
+                                                  const int* __restrict__ ocol, const double* __restrict__ oval,
+                                                  double* __restrict__ scr, const int* __restrict__ lines, int count,
+                                                  const double* __restrict__ rhs, double* __restrict__ x,
+                                                  const PcgState* st) {
+  if (st && st->status != ST_RUNNING) return;
+  const int t = (int)(blockIdx.x * BS + threadIdx.x);
+  if (t >= count) return;
+  const long long start = (long long)lines[t] * length;
+  const int stride = bw + 1;
+  const long long base = start * stride;
+  for (int i = 0; i < length; ++i) {
+    double value = rhs[nodes[start + i]];
+    for (int e = ooff[start + i]; e < ooff[start + i + 1]; ++e) value -= oval[e] * x[ocol[e]];
+    scr[start + i] = value;
+  }
+  for (int i = 0; i < length; ++i) {
+    const long long row = base + (long long)i * stride;
+    double value = scr[start + i];
+    for (int j = i > bw ? i - bw : 0; j < i; ++j) value -= factor[row + i - j] * scr[start + j];
+    scr[start + i] = value * factor[row];
+  }
+  for (int i = length - 1; i >= 0; --i) {
+    double value = scr[start + i];
+    const int end = length < i + bw + 1 ? length : i + bw + 1;
+    for (int j = i + 1; j < end; ++j) value -= factor[base + (long long)j * stride + j - i] * scr[start + j];
+    scr[start + i] = value * factor[base + (long long)i * stride];
+  }
+  for (int i = 0; i < length; ++i) x[nodes[start + i]] = scr[start + i];
+}
+
 // Subtracts the mean of each closed component (one block per component).
 __global__ void __launch_bounds__(BS) k_center_lists(double* __restrict__ v, const int* __restrict__ cptr,
                                                      const int* __restrict__ members, const PcgState* st) {
@@ -2466,8 +2502,19 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
         hipLaunchKernelGGL(k_ref_gs, dim3(blocks_for(count)), dim3(BS), 0, s, C.L, rows + ptr[d], count, st);
     }
   };
+  auto line_sweep = [&](const RefLines& B, const double* rhs, double* x, bool forward) {
+    const std::vector<int>& ptr = forward ? B.fwd_ptr : B.bwd_ptr;
+    const int* lines = forward ? B.fwd : B.bwd;
+    for (size_t d = 0; d + 1 < ptr.size(); ++d) {
+      const int count = ptr[d + 1] - ptr[d];
+      if (count > 0)
+        hipLaunchKernelGGL(k_ref_lines, dim3(blocks_for(count)), dim3(BS), 0, s, B.length, B.bandwidth, B.nodes, B.factor,
+                           B.ooff, B.ocol, B.oval, B.scratch, lines + ptr[d], count, rhs, x, st);
+    }
+  };
   const int nlev = (int)R.lv.size();
-  // level 0: rhs centered, x = 0, two forward sweeps, centered x, centered residual
+  // level 0: rhs centered, x = 0, two forward sweeps (one block sweep on an
+  // anisotropic level, multigrid.rs:676-688), centered x, centered residual
   R.center0(r, R.rhs0);
   hipLaunchKernelGGL(k_fill_t<double>, dim3(ab), dim3(BS), 0, s, g.N, z, st);
   if (nlev == 0) {
@@ -2475,7 +2522,10 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
                        (const double*)R.rhs0, z, (double*)nullptr, st);
     return;
   }
-  for (int pass = 0; pass < 2; ++pass) fine_sweep(true);
+  if (R.fine_lines.length > 0)
+    line_sweep(R.fine_lines, R.rhs0, z, true);
+  else
+    for (int pass = 0; pass < 2; ++pass) fine_sweep(true);
   R.center0(z, z);
   hipLaunchKernelGGL(k_ref_fine_residual, dim3(ab), dim3(BS), 0, s, g, R.codes, R.diag0, (const double*)R.rhs0,
                      (const double*)z, R.r0, R.self_mask, st);
@@ -2488,7 +2538,10 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   for (int l = 0; l + 1 < nlev; ++l) {
     const RefCoarse& C = R.lv[l];
     center(C, C.L.b);
-    for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, true);
+    if (C.lines.length > 0)
+      line_sweep(C.lines, C.L.b, C.L.x, true);
+    else
+      for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, true);
     center(C, C.L.x);
     hipLaunchKernelGGL(k_ref_residual, dim3(blocks_for(C.L.n)), dim3(BS), 0, s, C.L, C.r, st);
     center(C, C.r);
@@ -2506,12 +2559,18 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
     const RefCoarse& C = R.lv[l];
     hipLaunchKernelGGL(k_ref_prolong, dim3(blocks_for(C.L.n)), dim3(BS), 0, s, (long long)C.L.n, C.L.x, C.L.parent,
                        C.tptr, C.tcol, C.tw, (const double*)R.lv[l + 1].L.x, st);
-    for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, false);
+    if (C.lines.length > 0)
+      line_sweep(C.lines, C.L.b, C.L.x, false);
+    else
+      for (int pass = 0; pass < 2; ++pass) coarse_sweep(C, false);
     center(C, C.L.x);
   }
   hipLaunchKernelGGL(k_ref_prolong, dim3(ab), dim3(BS), 0, s, g.N, z, R.fparent, R.ftptr, R.ftcol, R.ftw,
                      (const double*)R.lv[0].L.x, st);
-  for (int pass = 0; pass < 2; ++pass) fine_sweep(false);
+  if (R.fine_lines.length > 0)
+    line_sweep(R.fine_lines, R.rhs0, z, false);
+  else
+    for (int pass = 0; pass < 2; ++pass) fine_sweep(false);
   R.center0(z, z);
 }
 

@@ -56,8 +56,25 @@ template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
 // The reference's cycle: fine level from the codes (diagonal diag0 in the
 // reference's summation order), coarse CSR levels in the reference's numbering.
+// Block smoother of one level of the reference cycle (multigrid/line.rs): lines
+// of `length` nodes, banded factor (reciprocal diagonal) with half-bandwidth
+// `bandwidth`, entries leaving each node's block, and the lines grouped by
+// dependency level (host offsets) for the forward and backward sweeps.
+struct RefLines {
+  int length = 0, bandwidth = 0;
+  const int* nodes = nullptr;
+  const double* factor = nullptr;
+  const int* ooff = nullptr;
+  const int* ocol = nullptr;
+  const double* oval = nullptr;
+  double* scratch = nullptr;
+  const int* fwd = nullptr;
+  const int* bwd = nullptr;
+  std::vector<int> fwd_ptr, bwd_ptr;
+};
 struct RefCoarse {
   AggDevLevel<double> L;  // n, CSR, b, x, parent (aggregation) and children (restriction into this level)
+  RefLines lines;         // anisotropic levels: block smoother instead of point sweeps
   double* r;              // residual scratch
   const double* child_w;  // trilinear restriction weights into this level (null: aggregation)
   const int* tptr;        // trilinear interpolation rows to the next level (null: aggregation)
@@ -82,6 +99,7 @@ struct RefCycle {
   const int* ftptr;    // fine interpolation rows (trilinear)
   const int* ftcol;
   const double* ftw;
+  RefLines fine_lines;  // block smoother of the fine level (anisotropic grids)
   std::vector<RefCoarse> lv;
   AggCoarsest D;
   std::function<void(const double*, double*)> center0;  // fine-level centering (closed components)

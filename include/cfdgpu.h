@@ -69,6 +69,7 @@ enum {
   CFD_E033 = 33, /* pressure iteration limit */
   CFD_E034 = 34, /* invalid multigrid topology or weights */
   CFD_E035 = 35, /* coarse Cholesky lost positive definiteness */
+  CFD_E036 = 36, /* line (block smoother) Cholesky lost positive definiteness */
   CFD_CANCELLED = 100,
   CFD_WALL_TIME = 101,
   CFD_EINVAL = 200,

@@ -101,6 +101,22 @@ def test_reference_hierarchy_matches_oracle(name, case):
     assert got["level_nonzeros"] == ref["level_nonzeros"]
 
 
+def _aniso_cases():
+    from test_oracle_goldens import _aniso, aniso_case
+
+    return [(k, aniso_case(v, "mg")) for k, v in _aniso()["shapes"].items()]
+
+
+@pytest.mark.parametrize("name,case", _aniso_cases(), ids=[n for n, _ in _aniso_cases()])
+def test_anisotropic_hierarchy_matches_oracle(name, case):
+    """The 18 anisotropy shapes: semicoarsened levels (weak axes wait until the
+    stiff ones collapse, multigrid.rs:582-611) with the oracle's sizes and entries."""
+    ref = oracle.Grid(case).mg_stats()
+    got = cfdgpu.case_multigrid_stats(case, cfdgpu.CYCLE_REFERENCE)
+    assert got["level_cells"] == ref["level_cells"]
+    assert got["level_nonzeros"] == ref["level_nonzeros"]
+
+
 def test_reference_hierarchy_published_sizes():
     """48x2x48 cavity: 5 levels, 6138 cells and 78084 stored entries in all
     (pressure-multigrid.md:141-142); operator complexity of cube48 2.601."""
