@@ -13,7 +13,8 @@ reference                      here
 ``project`` (:469)             ``Grid.project(u, p, dt, tol, max_iterations, control)``
 ``multigrid_stats`` (:100)     ``Grid.multigrid_stats()``
 ``LinearInfo`` (:29-36)        the ``dict`` returned beside the solution
-``Control`` (solver.rs:75)     ``Control(deadline_s, cancel=callable)``
+``Control`` (solver.rs:75)     ``Control(deadline_s, cancel=callable, profile=True)``
+``Control::profile``            ``Control.profile()`` (phase profile, ``profiling.py``)
 coded ``Err(String)``          ``CfdError`` (``.code`` 31..36, ``"cancelled"``, ...)
 =============================  =====================================================
 
@@ -28,6 +29,7 @@ import time
 
 import numpy as np
 
+from . import profiling
 from .case import (  # noqa: F401  (re-exported)
     BLOCKED,
     FLOW_NEUTRAL_RANS,
@@ -76,6 +78,16 @@ class LinearInfo(ctypes.Structure):
         ("coarse_solves", ctypes.c_int64),
         ("hierarchy_builds", ctypes.c_int64),
         ("seconds", ctypes.c_double),
+        ("restriction_cell_updates", ctypes.c_int64),
+        ("prolongation_cell_updates", ctypes.c_int64),
+        ("block_smoothing_cell_updates", ctypes.c_int64),
+        ("block_solves", ctypes.c_int64),
+        ("block_level_cell_visits", ctypes.c_int64),
+        ("hierarchy_levels", ctypes.c_int64),
+        ("hierarchy_cells", ctypes.c_int64),
+        ("hierarchy_nonzeros", ctypes.c_int64),
+        ("hierarchy_bytes", ctypes.c_int64),
+        ("hierarchy_build_seconds", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -201,12 +213,22 @@ def case_multigrid_stats(case: Case, cycle: int = 0):
 
 class Control:
     """Control::check (solver.rs:122-139) as seen by the pressure solve: a wall-time
-    deadline and an optional cancel callback, polled every 8 PCG iterations."""
+    deadline and an optional cancel callback, polled every 8 PCG iterations. With
+    ``profile=True`` the pressure calls record phase spans (profiling.rs) that
+    ``profile()`` returns as a ``cfd-craft.phase-profile/0.1`` dict."""
 
-    def __init__(self, deadline_s: float = float("inf"), cancel=None):
+    def __init__(self, deadline_s: float = float("inf"), cancel=None, profile: bool = False):
         self.deadline = time.monotonic() + deadline_s
         self.cancel = cancel
         self.polls = 0
+        self.collector = profiling.Collector() if profile else None
+
+    def span(self, phase: str) -> profiling.Span:
+        return profiling.Span(self.collector, phase)
+
+    def profile(self):
+        """The phase profile so far, or None without profiling (solver.rs Control::profile)."""
+        return self.collector.snapshot() if self.collector is not None else None
 
     def check(self) -> int:
         self.polls += 1
@@ -264,6 +286,8 @@ class Grid:
         self.case = case
         c, keep = case.to_c()
         h = ctypes.c_void_p()
+        self._build_began = time.perf_counter()  # when the hierarchy build reported next began
+        self._cycles_seen = {int(os.environ.get("CFD_MG_CYCLE", "0"))}
         _check(lib().cfd_grid_create(ctypes.byref(c), ctypes.byref(h)))
         self.h = h
         counts = np.zeros(8, np.int64)
@@ -326,7 +350,11 @@ class Grid:
         CYCLE_REFERENCE (the reference's hierarchy and lexicographic smoothing, FP64)
         or CYCLE_GEOMETRIC."""
         o = Options(int(precision), int(bool(warm_start)), 8, int(cycle))
+        began = time.perf_counter()
         _check(lib().cfd_grid_set_options(self.h, ctypes.byref(o)))
+        if self.case.preconditioner == MULTIGRID and int(cycle) not in self._cycles_seen:
+            self._cycles_seen.add(int(cycle))
+            self._build_began = began
 
     # ---- operator and gauges
     def apply(self, x) -> np.ndarray:
@@ -354,21 +382,35 @@ class Grid:
         fn = POLL_FN(lambda _user: control.check())
         return fn, fn
 
+    def _measured(self, control, call):
+        """Run ``call(info)`` inside a pressure_solve span of ``control`` and record the
+        attempted work, also of a failed solve (pressure.rs:350-410)."""
+        info = LinearInfo()
+        span = control.span("pressure_solve") if control is not None else profiling.Span(None, "pressure_solve")
+        with span:
+            rc = call(info)
+            d = info.as_dict()
+            # the multigrid work exists once the solve passed its compatibility check
+            mg = self.case.preconditioner == MULTIGRID and d["hierarchy_levels"] > 0
+            profiling.record_solve(span, d, mg, self._build_began)
+        _check(rc)
+        return d
+
     def solve_scaled(self, b, scale=None, tol=None, max_iterations=None, control=None):
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
         b = np.ascontiguousarray(b, dtype=np.float64)
         s = np.ascontiguousarray(scale, dtype=np.float64) if scale is not None else None
         x = np.empty(self.count)
-        info = LinearInfo()
         fn, keep = self._poll(control)
-        _check(
-            lib().cfd_solve_scaled(
+        info = self._measured(
+            control,
+            lambda info: lib().cfd_solve_scaled(
                 self.h, _ptr(b), _ptr(s) if s is not None else None, float(tol), int(max_iterations), fn, None, _ptr(x),
                 ctypes.byref(info),
-            )
+            ),
         )
-        return x, info.as_dict()
+        return x, info
 
     def solve(self, b, tol=None, max_iterations=None, control=None):
         return self.solve_scaled(b, None, tol, max_iterations, control)
@@ -376,15 +418,14 @@ class Grid:
     def solve_device(self, d_b: DeviceArray, d_x: DeviceArray, tol=None, max_iterations=None, d_scale=None, control=None):
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
-        info = LinearInfo()
         fn, keep = self._poll(control)
-        _check(
-            lib().cfd_solve_scaled_device(
+        return self._measured(
+            control,
+            lambda info: lib().cfd_solve_scaled_device(
                 self.h, d_b.ptr, d_scale.ptr if d_scale is not None else None, float(tol), int(max_iterations), fn, None,
                 d_x.ptr, ctypes.byref(info),
-            )
+            ),
         )
-        return info.as_dict()
 
     def projection_rhs(self, u, dt):
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -400,11 +441,17 @@ class Grid:
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
         u = np.array(u, dtype=np.float64)
         p = np.array(p, dtype=np.float64)
-        info = LinearInfo()
         fn, keep = self._poll(control)
-        _check(lib().cfd_project(self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn, None,
-                                 ctypes.byref(info)))
-        return u, p, info.as_dict()
+        # pressure_projection encloses the solve (pressure.rs:478); one library call
+        # forms the right-hand side, solves and corrects, all inside pressure_solve
+        outer = control.span("pressure_projection") if control is not None else profiling.Span(None, "")
+        with outer:
+            info = self._measured(
+                control,
+                lambda info: lib().cfd_project(self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn,
+                                               None, ctypes.byref(info)),
+            )
+        return u, p, info
 
 
 class Server:

@@ -182,6 +182,10 @@ struct cfd_grid {
   int64_t agg_dense = kAggDense;
   bool rb_layout = true;  // CFD_MG_RB_LAYOUT=0: the fine level in the natural layout
   bool ic0_tiles = true;  // CFD_IC0_TILES=0: one launch per hyperplane
+  // a hierarchy built since the last solve of its cycle (MultigridWork::hierarchy_builds)
+  bool unreported_build[4] = {false, false, false, false};
+  bool cycle_built[4] = {false, false, false, false};
+  double build_seconds[4] = {0, 0, 0, 0};
   // the reference's cycle (CFD_CYCLE_REFERENCE)
   DevRef ref;
   DevBuf<double> ref_mean;
@@ -713,17 +717,19 @@ int build_ref(cfd_grid* g) {
 }
 
 int ensure_cycle(cfd_grid* g, int cycle) {
-  switch (cycle) {
-    case CFD_CYCLE_RB:
-      return build_agg(g);
-    case CFD_CYCLE_GEOMETRIC:
-      return build_geometric(g);
-    case CFD_CYCLE_REFERENCE:
-      return build_ref(g);
-    default:
-      set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_GEOMETRIC");
-      return CFD_EINVAL;
+  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_GEOMETRIC && cycle != CFD_CYCLE_REFERENCE) {
+    set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_GEOMETRIC");
+    return CFD_EINVAL;
   }
+  if (g->cycle_built[cycle]) return 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = cycle == CFD_CYCLE_RB ? build_agg(g) : cycle == CFD_CYCLE_GEOMETRIC ? build_geometric(g) : build_ref(g);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->build_seconds[cycle] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  g->cycle_built[cycle] = true;
+  g->unreported_build[cycle] = true;
+  return 0;
 }
 
 int setup_device_inner(cfd_grid* g);
@@ -961,6 +967,61 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   return 0;
 }
 
+// LinearInfo and the attempted work (pressure.rs:350-410, multigrid.rs:646-726) of
+// the job whose last read-back state is `s`.
+void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local) {
+  std::memset(&local, 0, sizeof local);
+  local.iterations = s.status == ST_ZERO ? 0 : s.it;
+  local.relative_residual = s.status == ST_ZERO ? 0.0 : s.true_rel;
+  local.recurrent_residual = s.rel;
+  local.true_residual_checks = s.true_checks;
+  local.operator_applications = local.iterations + s.true_checks;
+  local.preconditioner_applications = s.status == ST_ZERO ? 0 : (s.status == ST_CONVERGED ? s.it : s.it + 1);
+  if (g->has_hier) {
+    // MultigridWork (multigrid.rs:667-726): per cycle and nonterminal level, one
+    // residual, restriction and prolongation per cell and 2 x 2 point sweeps, or
+    // one block sweep each way on a block-smoothed level
+    int64_t cells = 0, block_cells = 0, block_lines = 0;
+    if (g->opt.cycle == CFD_CYCLE_GEOMETRIC) {
+      // without coarse levels the fine grid is the terminal (direct) level
+      cells = g->hier.coarse.empty() ? 0 : g->host.count;
+      for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
+    } else {
+      const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
+      for (size_t l = 0; l + 1 < H.levels.size(); ++l) {
+        cells += H.levels[l].n;
+        if (H.levels[l].line_length > 0) {
+          block_cells += H.levels[l].n;
+          block_lines += (int64_t)H.levels[l].line_nodes.size() / H.levels[l].line_length;
+        }
+      }
+    }
+    cfd_mg_stats ms;
+    if (cfd_multigrid_stats(g, &ms) == CFD_OK) {
+      local.hierarchy_levels = ms.levels;
+      for (int l = 0; l < ms.levels; ++l) {
+        local.hierarchy_cells += ms.level_cells[l];
+        local.hierarchy_nonzeros += ms.level_nonzeros[l];
+      }
+      local.hierarchy_bytes = ms.allocated_bytes;
+    }
+    local.cycles = local.preconditioner_applications;
+    local.smoothing_cell_updates = local.cycles * (4 * (cells - block_cells) + 2 * block_cells);
+    local.residual_cell_updates = local.cycles * cells;
+    local.restriction_cell_updates = local.residual_cell_updates;
+    local.prolongation_cell_updates = local.residual_cell_updates;
+    local.coarse_solves = local.cycles;
+    local.block_smoothing_cell_updates = local.cycles * 2 * block_cells;
+    local.block_solves = local.cycles * 2 * block_lines;
+    local.block_level_cell_visits = local.cycles * block_cells;
+    if (g->unreported_build[g->opt.cycle]) {
+      local.hierarchy_builds = 1;
+      local.hierarchy_build_seconds = g->build_seconds[g->opt.cycle];
+      g->unreported_build[g->opt.cycle] = false;
+    }
+  }
+}
+
 // Advances the job; `block` waits for the pending read-back. Returns JOB_PENDING,
 // JOB_DONE (result in g->job.rc / info) or an error code of the device runtime.
 int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
@@ -988,6 +1049,7 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
     if (poll) {
       int p = poll(user);
       if (p == 1 || p == 2) {
+        fill_info(g, s, J.info);  // the attempted work counts (pressure.rs:387-389)
         J.rc = p == 1 ? CFD_CANCELLED : CFD_WALL_TIME;
         J.error = p == 1 ? "cancelled" : "wall_time_limit";
         J.phase = 4;
@@ -1007,28 +1069,7 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
     J.phase = 2;
     return JOB_PENDING;
   }
-  cfd_linear_info& local = J.info;
-  std::memset(&local, 0, sizeof local);
-  local.iterations = s.status == ST_ZERO ? 0 : s.it;
-  local.relative_residual = s.status == ST_ZERO ? 0.0 : s.true_rel;
-  local.recurrent_residual = s.rel;
-  local.true_residual_checks = s.true_checks;
-  local.operator_applications = local.iterations + s.true_checks;
-  local.preconditioner_applications = s.status == ST_ZERO ? 0 : (s.status == ST_CONVERGED ? s.it : s.it + 1);
-  if (g->has_hier) {
-    int64_t cells = 0;
-    if (g->opt.cycle == CFD_CYCLE_GEOMETRIC) {
-      cells = g->host.count;
-      for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
-    } else {
-      const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
-      for (size_t l = 0; l + 1 < H.levels.size(); ++l) cells += H.levels[l].n;
-    }
-    local.cycles = local.preconditioner_applications;
-    local.smoothing_cell_updates = local.cycles * 4 * cells;
-    local.residual_cell_updates = local.cycles * cells;
-    local.coarse_solves = local.cycles;
-  }
+  fill_info(g, s, J.info);
   int rc = finish_status(g, s, J.max);
   if (rc) {
     J.rc = rc;
@@ -1200,9 +1241,9 @@ int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
     return CFD_EINVAL;
   }
   if (g->has_hier && o->cycle != g->opt.cycle) {
+    HIP_TRY(hipSetDevice(g->device));
     size_t bytes = 0;
     tl_bytes = &bytes;
-    HIP_TRY(hipSetDevice(g->device));
     int rc = ensure_cycle(g, o->cycle);
     tl_bytes = nullptr;
     g->footprint += bytes;

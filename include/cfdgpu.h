@@ -93,6 +93,21 @@ typedef struct {
   int64_t coarse_solves;
   int64_t hierarchy_builds;
   double seconds; /* device time of the solve, HIP events */
+  /* MultigridWork (multigrid.rs:74-116) beyond the fields above */
+  int64_t restriction_cell_updates;
+  int64_t prolongation_cell_updates;
+  int64_t block_smoothing_cell_updates;
+  int64_t block_solves;
+  int64_t block_level_cell_visits;
+  int64_t hierarchy_levels;
+  int64_t hierarchy_cells;
+  int64_t hierarchy_nonzeros;
+  int64_t hierarchy_bytes;
+  /* host wall seconds of the hierarchy build that hierarchy_builds reports (the
+   * hierarchy is built when the grid is created or a cycle is first selected;
+   * the first solve after the build reports it, as the reference's lazy build
+   * does, pressure.rs:71-93) */
+  double hierarchy_build_seconds;
 } cfd_linear_info;
 
 /* MultigridStats (multigrid.rs:55-72). */

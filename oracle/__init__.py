@@ -75,6 +75,11 @@ class Info(ctypes.Structure):
         ("residual_cell_updates", ctypes.c_int64),
         ("coarse_solves", ctypes.c_int64),
         ("hierarchy_builds", ctypes.c_int64),
+        ("restriction_cell_updates", ctypes.c_int64),
+        ("prolongation_cell_updates", ctypes.c_int64),
+        ("block_smoothing_cell_updates", ctypes.c_int64),
+        ("block_solves", ctypes.c_int64),
+        ("block_level_cell_visits", ctypes.c_int64),
     ]
 
     def as_dict(self):

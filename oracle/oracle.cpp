@@ -431,7 +431,8 @@ struct BlockSmoother {
     scratch.assign(length, 0.0);
     return 0;
   }
-  void sweep(const vector<double>& rhs, vector<double>& x, bool backward) {
+  // returns the blocks solved (line.rs:155-190)
+  int64_t sweep(const vector<double>& rhs, vector<double>& x, bool backward) {
     size_t count = nodes.size() / length, stride = bandwidth + 1;
     for (size_t at = 0; at < count; ++at) {
       size_t line = backward ? count - 1 - at : at;
@@ -456,6 +457,7 @@ struct BlockSmoother {
       }
       for (size_t i = 0; i < length; ++i) x[nodes[start + i]] = scratch[i];
     }
+    return (int64_t)count;
   }
 };
 
@@ -594,8 +596,10 @@ void center_components(vector<double>& x, const vector<vector<size_t>>& closed) 
   }
 }
 
+// MultigridWork (multigrid.rs:78-114), counted where the cycles increment it
 struct Work {
   int64_t cycles = 0, smoothing = 0, residual = 0, coarse = 0;
+  int64_t restriction = 0, prolongation = 0, block_smoothing = 0, block_solves = 0, block_visits = 0;
 };
 
 struct Hierarchy {
@@ -617,11 +621,14 @@ void cycle_plain(vector<Level>& levels, size_t at, Work& work) {  // multigrid.r
   std::fill(level.x.begin(), level.x.end(), 0.0);
   // multigrid.rs:676-688: one block sweep per side on a level with block smoothers
   const int sweeps = level.lines.empty() ? SWEEPS : 1;
+  if (!level.lines.empty()) work.block_visits += (int64_t)level.x.size();
   for (int pass = 0; pass < sweeps; ++pass) {
-    if (level.lines.empty())
+    if (level.lines.empty()) {
       level.matrix.sweep(level.rhs, level.x, false);
-    else
-      level.lines[(size_t)pass % level.lines.size()].sweep(level.rhs, level.x, false);
+    } else {
+      work.block_solves += level.lines[(size_t)pass % level.lines.size()].sweep(level.rhs, level.x, false);
+      work.block_smoothing += (int64_t)level.x.size();
+    }
   }
   center_plain(level.x, level.anchored);
   level.matrix.product(level.x, level.residual);
@@ -645,15 +652,19 @@ void cycle_plain(vector<Level>& levels, size_t at, Work& work) {  // multigrid.r
   }
   const int lsweeps = lv.lines.empty() ? SWEEPS : 1;
   for (int pass = 0; pass < lsweeps; ++pass) {
-    if (lv.lines.empty())
+    if (lv.lines.empty()) {
       lv.matrix.sweep(lv.rhs, lv.x, true);
-    else
-      lv.lines[(size_t)(lsweeps - 1 - pass) % lv.lines.size()].sweep(lv.rhs, lv.x, true);
+    } else {
+      work.block_solves += lv.lines[(size_t)(lsweeps - 1 - pass) % lv.lines.size()].sweep(lv.rhs, lv.x, true);
+      work.block_smoothing += (int64_t)lv.x.size();
+    }
   }
   center_plain(lv.x, lv.anchored);
   int64_t cells = (int64_t)lv.x.size();
   work.smoothing += 2 * lsweeps * cells;
   work.residual += cells;
+  work.restriction += cells;
+  work.prolongation += cells;
 }
 
 void cycle_masked(vector<MaskedLevel>& levels, size_t at, Work& work) {  // masked.rs:304-342
@@ -683,6 +694,8 @@ void cycle_masked(vector<MaskedLevel>& levels, size_t at, Work& work) {  // mask
   int64_t n = (int64_t)lv.x.size();
   work.smoothing += 2 * SWEEPS * n;
   work.residual += n;
+  work.restriction += n;
+  work.prolongation += n;
 }
 
 }  // namespace
@@ -1403,6 +1416,11 @@ int orc_solve_scaled(orc_grid* g, const double* b, const double* scale, double t
     local.smoothing_cell_updates = work.smoothing;
     local.residual_cell_updates = work.residual;
     local.coarse_solves = work.coarse;
+    local.restriction_cell_updates = work.restriction;
+    local.prolongation_cell_updates = work.prolongation;
+    local.block_smoothing_cell_updates = work.block_smoothing;
+    local.block_solves = work.block_solves;
+    local.block_level_cell_visits = work.block_visits;
     if (info) *info = local;
     return code;
   };

@@ -55,6 +55,11 @@ typedef struct {
   int64_t residual_cell_updates;
   int64_t coarse_solves;
   int64_t hierarchy_builds;
+  int64_t restriction_cell_updates;
+  int64_t prolongation_cell_updates;
+  int64_t block_smoothing_cell_updates;
+  int64_t block_solves;
+  int64_t block_level_cell_visits;
 } orc_info;
 
 typedef struct orc_grid orc_grid;
