@@ -94,6 +94,18 @@ class LinearInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BlockSmootherStats(ctypes.Structure):
+    _fields_ = [
+        ("level", ctypes.c_int64),
+        ("n_axes", ctypes.c_int64),
+        ("axes", ctypes.c_int64 * 3),
+        ("block_unknowns", ctypes.c_int64),
+        ("block_count", ctypes.c_int64),
+        ("half_bandwidth", ctypes.c_int64),
+        ("allocated_bytes", ctypes.c_int64),
+    ]
+
+
 class MgStats(ctypes.Structure):
     _fields_ = [
         ("levels", ctypes.c_int32),
@@ -102,7 +114,28 @@ class MgStats(ctypes.Structure):
         ("level_nonzeros", ctypes.c_int64 * 32),
         ("level_dims", (ctypes.c_int64 * 3) * 32),
         ("allocated_bytes", ctypes.c_int64),
+        ("n_block_smoothers", ctypes.c_int64),
+        ("block_smoothers", BlockSmootherStats * 32),
     ]
+
+    def as_dict(self):
+        """MultigridStats (multigrid.rs:55-72) as the reference serializes it: the
+        dimensions and block smoothers only where the hierarchy has them."""
+        n = self.levels
+        d = dict(level_cells=list(self.level_cells[:n]), level_nonzeros=list(self.level_nonzeros[:n]),
+                 allocated_bytes=int(self.allocated_bytes))
+        dims = [list(self.level_dims[i]) for i in range(n)]
+        if any(any(v) for v in dims):
+            d["level_dimensions"] = dims
+        blocks = []
+        for k in range(self.n_block_smoothers):
+            b = self.block_smoothers[k]
+            blocks.append(dict(level=int(b.level), axes=list(b.axes[: b.n_axes]), block_unknowns=int(b.block_unknowns),
+                               block_count=int(b.block_count), half_bandwidth=int(b.half_bandwidth),
+                               allocated_bytes=int(b.allocated_bytes)))
+        if blocks:
+            d["block_smoothers"] = blocks
+        return d
 
 
 class Options(ctypes.Structure):
@@ -207,8 +240,9 @@ def case_multigrid_stats(case: Case, cycle: int = 0):
     c, keep = case.to_c()
     s = MgStats()
     _check(lib().cfd_case_multigrid_stats(ctypes.byref(c), int(cycle), ctypes.byref(s)))
-    n = s.levels
-    return dict(level_cells=list(s.level_cells[:n]), level_nonzeros=list(s.level_nonzeros[:n]))
+    d = s.as_dict()
+    del d["allocated_bytes"]  # nothing is allocated on a device here
+    return d
 
 
 class Control:
@@ -329,13 +363,7 @@ class Grid:
     def multigrid_stats(self) -> dict:
         s = MgStats()
         _check(lib().cfd_multigrid_stats(self.h, ctypes.byref(s)))
-        n = s.levels
-        return dict(
-            level_cells=list(s.level_cells[:n]),
-            level_nonzeros=list(s.level_nonzeros[:n]),
-            level_dimensions=[list(s.level_dims[i]) for i in range(n)],
-            allocated_bytes=s.allocated_bytes,
-        )
+        return s.as_dict()
 
     def profile_kernel(self, kernel: int, reps: int = 20):
         """(average seconds per launch, algorithmic bytes per launch) of a hot kernel:

@@ -1211,6 +1211,7 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
     ++L;
   }
   s->levels = L;
+  agg_hierarchy_shape(H, s);
   return CFD_OK;
 }
 
@@ -1292,8 +1293,10 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
       ++L;
     }
     s->levels = L;
+    agg_hierarchy_shape(H, s);
     s->allocated_bytes = bytes + h.count * (int64_t)(sizeof(int) + sizeof(float)) +
                          (int64_t)H.block_G.size() * (int64_t)sizeof(double);
+    for (int64_t k = 0; k < s->n_block_smoothers; ++k) s->allocated_bytes += s->block_smoothers[k].allocated_bytes;
     return CFD_OK;
   }
   int64_t fine_nnz = h.count;

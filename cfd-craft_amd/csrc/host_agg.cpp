@@ -776,6 +776,7 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
   const std::vector<int> axes = line_axes(g);
   for (;;) {
     AggLevel& L = h.levels.back();
+    for (int a = 0; a < 3; ++a) L.dims[a] = n[a];
     L.closed_ptr.assign(1, 0);
     if (!anchored) {
       for (int64_t i = 0; i < L.n; ++i) L.closed_members.push_back((int32_t)i);
@@ -820,6 +821,8 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
       }
       int e = build_lines(L, n, grp, periodic);
       if (e) return e;
+      L.block_axes = grp;
+      std::sort(L.block_axes.begin(), L.block_axes.end());
     }
     transfer_rows(n, centers, periodic, extent, coarse, L, coarsen);
     for (int a = 0; a < 3; ++a) n[a] = (int64_t)coarse[a].size();
@@ -998,6 +1001,26 @@ void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std
   for (const auto& L : h.levels) {
     cells.push_back(L.n);
     nonzeros.push_back((int64_t)L.col.size() + L.n);
+  }
+}
+
+void agg_hierarchy_shape(const AggHierarchy& h, cfd_mg_stats* s) {
+  s->n_block_smoothers = 0;
+  for (size_t l = 0; l < h.levels.size() && l < 32; ++l) {
+    const AggLevel& L = h.levels[l];
+    for (int a = 0; a < 3; ++a) s->level_dims[l][a] = h.masked ? 0 : L.dims[a];
+    if (L.line_length <= 0 || s->n_block_smoothers >= 32) continue;
+    cfd_block_smoother_stats& b = s->block_smoothers[s->n_block_smoothers++];
+    b.level = (int64_t)l;
+    b.n_axes = (int64_t)L.block_axes.size();
+    for (size_t k = 0; k < L.block_axes.size() && k < 3; ++k) b.axes[k] = L.block_axes[k];
+    b.block_unknowns = L.line_length;
+    b.block_count = (int64_t)L.line_nodes.size() / L.line_length;
+    b.half_bandwidth = L.line_bandwidth;
+    b.allocated_bytes = (int64_t)(L.line_nodes.size() * 4 + L.line_ooff.size() * 8 + L.line_ocol.size() * 4 +
+                                  L.line_oval.size() * 8 + L.line_factor.size() * 8 +
+                                  (L.line_fwd_ptr.size() + L.line_bwd_ptr.size()) * 8 +
+                                  (L.line_fwd.size() + L.line_bwd.size()) * 4);
   }
 }
 

@@ -101,6 +101,8 @@ struct AggLevel {
   // `bandwidth` (reciprocal diagonal), the entries outside each node's block,
   // and the lines grouped by dependency level for the forward and backward sweeps
   int64_t line_length = 0, line_bandwidth = 0;
+  std::vector<int> block_axes;  // axes the blocks span, ascending
+  int64_t dims[3] = {0, 0, 0};  // cell dimensions (unmasked hierarchy)
   std::vector<int32_t> line_nodes, line_ocol;
   std::vector<int64_t> line_ooff;
   std::vector<double> line_oval, line_factor;
@@ -132,6 +134,9 @@ struct AggHierarchy {
 };
 // Coarsening stops when every component has at most coarse_cells nodes (the
 // reference: 32); blocks above host_block_max nodes are left to the device.
+// MultigridStats shape of an aggregation or unmasked hierarchy: level dimensions
+// (unmasked only) and the block smoothers (multigrid.rs:610-617, line.rs:200-209).
+void agg_hierarchy_shape(const AggHierarchy& h, cfd_mg_stats* s);
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
                         int64_t host_block_max = INT64_MAX);
 // The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked

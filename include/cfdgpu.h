@@ -110,7 +110,21 @@ typedef struct {
   double hierarchy_build_seconds;
 } cfd_linear_info;
 
-/* MultigridStats (multigrid.rs:55-72). */
+/* BlockSmootherStats (multigrid/line.rs:9-22): the line or small-plane block
+ * smoother of one anisotropic level of the unmasked hierarchy. */
+typedef struct {
+  int64_t level;
+  int64_t n_axes;
+  int64_t axes[3]; /* ascending; the first n_axes are used */
+  int64_t block_unknowns;
+  int64_t block_count;
+  int64_t half_bandwidth;
+  int64_t allocated_bytes; /* device storage of the blocks' order, factors and outside rows */
+} cfd_block_smoother_stats;
+
+/* MultigridStats (multigrid.rs:55-72). level_dims: cell dimensions per level of
+ * the structured hierarchies (trilinear Galerkin, geometric), zero for the
+ * aggregation hierarchy, whose levels are irregular (masked.rs:262). */
 typedef struct {
   int32_t levels;
   int32_t reserved;
@@ -118,6 +132,8 @@ typedef struct {
   int64_t level_nonzeros[32];
   int64_t level_dims[32][3];
   int64_t allocated_bytes;
+  int64_t n_block_smoothers;
+  cfd_block_smoother_stats block_smoothers[32];
 } cfd_mg_stats;
 
 /* Control::check as a callback (solver.rs:122-139): 0 continue, 1 cancelled,

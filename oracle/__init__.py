@@ -55,6 +55,7 @@ def lib():
         L.orc_ic0_diagonal.argtypes = [P, P]
         L.orc_mg_apply.argtypes = [P, P, P]
         L.orc_mg_stats.argtypes = [P, P, P, ctypes.c_int, P]
+        L.orc_mg_shape.argtypes = [P, P, P, ctypes.c_int, P]
         L.orc_divergence.argtypes = [P, P, P]
         L.orc_gradient.argtypes = [P, P, P]
         L.orc_enforce_velocity.argtypes = [P, P]
@@ -211,6 +212,22 @@ class Grid:
         if n < 0:
             raise OracleError(-n, lib().orc_last_error().decode())
         return dict(level_cells=cells[:n].tolist(), level_nonzeros=nnz[:n].tolist(), masked=bool(masked.value))
+
+    def mg_shape(self):
+        """Level dimensions (unmasked hierarchy) and block smoothers
+        (level, axes, block_unknowns, block_count, half_bandwidth)."""
+        dims = np.zeros(64 * 3, np.int64)
+        blocks = np.zeros(64 * 6, np.int64)
+        nb = ctypes.c_int(0)
+        n = lib().orc_mg_shape(self.h, _ptr(dims), _ptr(blocks), 64, ctypes.byref(nb))
+        if n < 0:
+            raise OracleError(-n, lib().orc_last_error().decode())
+        out = []
+        for k in range(nb.value):
+            lvl, mask, unknowns, count, bw, _ = (int(v) for v in blocks[6 * k: 6 * k + 6])
+            out.append(dict(level=lvl, axes=[a for a in range(3) if mask >> a & 1], block_unknowns=unknowns,
+                            block_count=count, half_bandwidth=bw))
+        return dict(level_dimensions=dims[: 3 * n].reshape(n, 3).tolist(), block_smoothers=out)
 
     def divergence(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -608,6 +608,7 @@ struct Hierarchy {
   vector<MaskedLevel> mlevels;
   vector<size_t> fluid_indices;
   vector<int64_t> level_cells, level_nonzeros;
+  vector<std::array<int64_t, 3>> level_dimensions;  // unmasked only (multigrid.rs:562-563)
 };
 
 void cycle_plain(vector<Level>& levels, size_t at, Work& work) {  // multigrid.rs:667-726
@@ -1162,6 +1163,7 @@ int hierarchy_build(const orc_grid& g, std::unique_ptr<Hierarchy>& out) {
       anchors[f.lo != NONE ? f.lo : f.hi] += w;
   }
   for (;;) {
+    h->level_dimensions.push_back({(int64_t)n[0], (int64_t)n[1], (int64_t)n[2]});
     Matrix matrix = Matrix::build(edges, anchors);
     size_t count = anchors.size();
     if (count <= COARSE_CELLS) {
@@ -1369,6 +1371,33 @@ int orc_mg_stats(orc_grid* g, int64_t* level_cells, int64_t* level_nonzeros, int
     level_nonzeros[i] = h.level_nonzeros[i];
   }
   if (masked) *masked = h.masked ? 1 : 0;
+  return n;
+}
+
+int orc_mg_shape(orc_grid* g, int64_t* dims, int64_t* blocks, int cap, int* n_blocks) {
+  if (!g->hierarchy) {
+    int e = hierarchy_build(*g, g->hierarchy);
+    if (e) return -e;
+  }
+  const Hierarchy& h = *g->hierarchy;
+  int n = (int)h.level_dimensions.size();
+  for (int i = 0; i < n && i < cap; ++i)
+    for (int a = 0; a < 3; ++a) dims[3 * i + a] = h.level_dimensions[(size_t)i][(size_t)a];
+  // BlockSmootherStats (line.rs:200-209) without the storage bytes
+  int k = 0;
+  for (size_t l = 0; l < h.levels.size(); ++l)
+    for (const BlockSmoother& b : h.levels[l].lines) {
+      if (k >= cap) break;
+      int64_t* o = blocks + 6 * k++;
+      o[0] = (int64_t)l;
+      o[1] = 0;
+      for (int a : b.axes) o[1] |= int64_t(1) << a;
+      o[2] = (int64_t)b.length;
+      o[3] = (int64_t)(b.nodes.size() / b.length);
+      o[4] = (int64_t)b.bandwidth;
+      o[5] = (int64_t)b.axes.size();
+    }
+  *n_blocks = k;
   return n;
 }
 

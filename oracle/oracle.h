@@ -99,6 +99,10 @@ void orc_ic0_diagonal(orc_grid* g, double* diag);
 int orc_mg_apply(orc_grid* g, const double* r, double* z);
 /* MultigridStats (multigrid.rs:55-72): writes up to cap level sizes, returns the level count (<0 on error). */
 int orc_mg_stats(orc_grid* g, int64_t* level_cells, int64_t* level_nonzeros, int cap, int* masked);
+/* Level dimensions of the unmasked hierarchy (none for masked) and, 6 values per
+ * block smoother: level, axes bitmask, unknowns per block, block count,
+ * half-bandwidth, axis count (line.rs:200-209). Returns the dimension rows. */
+int orc_mg_shape(orc_grid* g, int64_t* dims, int64_t* blocks, int cap, int* n_blocks);
 
 /* Grid::divergence / gradient (grid.rs:612-665) */
 void orc_divergence(const orc_grid* g, const double* u, double* div);

@@ -111,10 +111,43 @@ def _aniso_cases():
 def test_anisotropic_hierarchy_matches_oracle(name, case):
     """The 18 anisotropy shapes: semicoarsened levels (weak axes wait until the
     stiff ones collapse, multigrid.rs:582-611) with the oracle's sizes and entries."""
-    ref = oracle.Grid(case).mg_stats()
+    o = oracle.Grid(case)
+    ref = o.mg_stats()
     got = cfdgpu.case_multigrid_stats(case, cfdgpu.CYCLE_REFERENCE)
     assert got["level_cells"] == ref["level_cells"]
     assert got["level_nonzeros"] == ref["level_nonzeros"]
+    # MultigridStats shape: level dimensions and block smoothers (line.rs:200-209)
+    shape = o.mg_shape()
+    assert got.get("level_dimensions", []) == shape["level_dimensions"]
+    blocks = [{k: v for k, v in b.items() if k != "allocated_bytes"} for b in got.get("block_smoothers", [])]
+    assert blocks == shape["block_smoothers"]
+
+
+def _lid(n, size):
+    return Case(cells=list(n), size_m=list(size), boundaries=[WALL, PERIODIC, WALL], velocity_m_s=[1.0, 0.0, 0.0],
+                preconditioner=1)
+
+
+def test_block_smoother_stats_of_the_reference_tests():
+    """tests/multigrid.rs:297-377: the 12x4x12 cavity of size [1, 1, 0.1] takes z
+    lines on every smoothed level; a stiff two-cell y axis (16x2x16, y 0.001 m)
+    takes two-unknown y blocks; 8x3x8 takes blocks only once dz shrinks 100-fold.
+    multigrid.rs:940-960: while a stiff axis still coarsens, the weak axes keep
+    their dimensions."""
+    s = cfdgpu.case_multigrid_stats(_lid([12, 4, 12], [1.0, 1.0, 0.1]), cfdgpu.CYCLE_REFERENCE)
+    assert s["block_smoothers"] and all(b["axes"] == [2] for b in s["block_smoothers"])
+    dims = s["level_dimensions"]
+    for n, nxt in zip(dims, dims[1:]):
+        if n[2] > 2:
+            assert (n[0], n[1]) == (nxt[0], nxt[1])
+    s = cfdgpu.case_multigrid_stats(_lid([16, 2, 16], [1.0, 0.001, 1.0]), cfdgpu.CYCLE_REFERENCE)
+    assert s["block_smoothers"]
+    assert all(b["axes"] == [1] and b["block_unknowns"] == 2 for b in s["block_smoothers"])
+    assert "block_smoothers" not in cfdgpu.case_multigrid_stats(_lid([8, 3, 8], [1.0, 1.0, 1.0]), cfdgpu.CYCLE_REFERENCE)
+    assert cfdgpu.case_multigrid_stats(_lid([8, 3, 8], [1.0, 1.0, 0.01]), cfdgpu.CYCLE_REFERENCE)["block_smoothers"]
+    # the aggregation hierarchy has irregular levels: no dimensions (masked.rs:262)
+    s = cfdgpu.case_multigrid_stats(_lid([12, 4, 12], [1.0, 1.0, 0.1]), cfdgpu.CYCLE_RB)
+    assert "level_dimensions" not in s and "block_smoothers" not in s
 
 
 def test_reference_hierarchy_published_sizes():

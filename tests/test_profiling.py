@@ -214,6 +214,12 @@ def test_device_reference_cycle_work_equals_oracle(gpu_available, spec_name):
     assert got["hierarchy_levels"] == len(stats["level_cells"])
     assert got["hierarchy_cells"] == sum(stats["level_cells"])
     assert got["hierarchy_nonzeros"] == sum(stats["level_nonzeros"])
+    shape = o.mg_shape()
+    dev = g.multigrid_stats()
+    assert dev.get("level_dimensions", []) == shape["level_dimensions"]
+    assert [{k: v for k, v in b.items() if k != "allocated_bytes"} for b in dev.get("block_smoothers", [])] == \
+        shape["block_smoothers"]
+    assert dev["allocated_bytes"] == got["hierarchy_bytes"] > 0
     profiling.check_pressure(ctl.profile(), "multigrid")
 
 
