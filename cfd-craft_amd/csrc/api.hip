@@ -8,7 +8,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <deque>
+#include <map>
+#include <tuple>
 #include <thread>
 #include <unordered_map>
 #include <cstdlib>
@@ -38,6 +41,8 @@ void set_error(const std::string& text) { g_error = text; }
   } while (0)
 
 constexpr int kSegment = 8;          // PCG iterations per host poll (pressure.rs:387)
+constexpr int kRing = 4;             // host slots for state read-backs
+constexpr int kAhead = 2;            // segments queued beyond the last read-back
 // Multigrid shape: the coarsest level (solved by the dense pseudo-inverse) holds at
 // most kCoarseMax cells; coarse levels of at most kTailCells run inside one
 // workgroup (0: none); the coarse-grid correction is scaled by kOmega. A 2x2x2
@@ -167,7 +172,8 @@ struct cfd_grid {
   DevBuf<double> mean_b, mean_z, mean_x, sums;
   DevBuf<unsigned int> tickets;
   DevBuf<PcgState> st;
-  PcgState* h_st = nullptr;
+  PcgState* h_st = nullptr;  // kRing read-back slots
+  hipEvent_t evr[kRing] = {};  // recorded after the read-back into each slot
   // multigrid
   DevHierarchy<double> h64;
   DevHierarchy<float> h32;
@@ -189,15 +195,21 @@ struct cfd_grid {
   // the reference's cycle (CFD_CYCLE_REFERENCE)
   DevRef ref;
   DevBuf<double> ref_mean;
-  // captured iteration segment, per precision
-  hipGraphExec_t seg[2] = {nullptr, nullptr};
+  // captured iteration segments, by (length, parity of the first iteration, precision)
+  std::map<int, hipGraphExec_t> seg;
+  // iterations of the last converged solve, by (tol, precision, cycle, warm start):
+  // the next such solve queues that many iterations without waiting for a read-back
+  std::map<std::tuple<double, int, int, int>, int64_t> learned;
   // resident solution of the previous solve (warm start) and the HBM footprint
   DevBuf<double> warm;
   bool warm_valid = false;
   size_t footprint = 0;
   // the solve in flight (asynchronous state machine driven by job_poll)
   struct Job {
-    int phase = 0;  // 0 idle, 1 initial kernels, 2 segment, 3 final kernels, 4 done
+    int phase = 0;  // 0 idle, 2 iterating, 3 final kernels, 4 done
+    int64_t enq = 0;  // iterations queued
+    std::deque<std::pair<int, int64_t>> inflight;  // (slot, iterations queued) of pending read-backs
+    int slot = 0;
     const double* b = nullptr;
     const double* scale = nullptr;
     double* x = nullptr;
@@ -215,10 +227,15 @@ struct cfd_grid {
   // (one kernel per level and phase), the default; 2 every level fused.
   int fused = 1;
   double omega = kOmega;
+  // red-black cycle: smoothing rounds (sweeps over all colours) before and after
+  // the correction on the fine level and on the coarse levels (CFD_MG_SWEEPS=f,c)
+  int fine_rounds = 2, coarse_rounds = 2;
   ~cfd_grid() {
-    for (auto& g : seg)
-      if (g) (void)hipGraphExecDestroy(g);
+    for (auto& kv : seg)
+      if (kv.second) (void)hipGraphExecDestroy(kv.second);
     if (h_st) (void)hipHostFree(h_st);
+    for (hipEvent_t e : evr)
+      if (e) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (evs) (void)hipEventDestroy(evs);
@@ -748,7 +765,8 @@ int setup_device_inner(cfd_grid* g) {
   HIP_TRY(hipEventCreate(&g->ev0));
   HIP_TRY(hipEventCreate(&g->ev1));
   HIP_TRY(hipEventCreateWithFlags(&g->evs, hipEventDisableTiming));
-  HIP_TRY(hipHostMalloc((void**)&g->h_st, sizeof(PcgState), hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&g->h_st, kRing * sizeof(PcgState), hipHostMallocDefault));
+  for (hipEvent_t& e : g->evr) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   fill_geo(g);
   if (const char* env = std::getenv("CFD_GPU_GRAPHS")) g->use_graphs = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
@@ -758,6 +776,13 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
+  if (const char* env = std::getenv("CFD_MG_SWEEPS")) {
+    int f = 2, c = 2;
+    if (std::sscanf(env, "%d,%d", &f, &c) == 2 && f >= 1 && f <= 8 && c >= 1 && c <= 8) {
+      g->fine_rounds = f;
+      g->coarse_rounds = c;
+    }
+  }
   size_t N = (size_t)h.count;
   int rc;
   if ((rc = g->codes.upload(h.codes.data(), N))) return rc;
@@ -834,11 +859,11 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
     if (g->opt.precision == CFD_PRECISION_MIXED)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
                                        g->a32.levels.data(), A.D, z, st, g->omega, g->stream,
-                                       A.has_rb ? &A.rb32 : nullptr);
+                                       A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb ? z : nullptr, st, g->omega, g->stream,
-                                        A.has_rb ? &A.rb64 : nullptr);
+                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -876,24 +901,32 @@ void enqueue_iteration(cfd_grid* g, int parity) {
   launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
 }
 
-int segment_graph(cfd_grid* g, hipGraphExec_t* out) {
-  int slot = g->opt.precision == CFD_PRECISION_MIXED ? 1 : 0;
-  if (g->seg[slot]) {
-    *out = g->seg[slot];
+// The captured graph of `len` iterations whose first one has parity `parity`.
+int segment_graph(cfd_grid* g, int len, int parity, hipGraphExec_t* out) {
+  const int key = (len << 2) | (parity << 1) | (g->opt.precision == CFD_PRECISION_MIXED ? 1 : 0);
+  auto it = g->seg.find(key);
+  if (it != g->seg.end()) {
+    *out = it->second;
     return 0;
   }
   hipGraph_t graph;
   HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-  for (int it = 0; it < kSegment; ++it) enqueue_iteration(g, it & 1);
+  for (int i = 0; i < len; ++i) enqueue_iteration(g, (parity + i) & 1);
   HIP_TRY(hipStreamEndCapture(g->stream, &graph));
-  hipError_t e = hipGraphInstantiate(&g->seg[slot], graph, nullptr, nullptr, 0);
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
     set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     return CFD_EDEVICE;
   }
-  *out = g->seg[slot];
+  g->seg[key] = exec;
+  *out = exec;
   return 0;
+}
+
+std::tuple<double, int, int, int> learn_key(const cfd_grid* g, double tol) {
+  return {tol, g->opt.precision, g->opt.cycle, g->opt.warm_start ? 1 : 0};
 }
 
 int finish_status(cfd_grid* g, const PcgState& s, int64_t max_iterations) {
@@ -931,13 +964,57 @@ int read_state(cfd_grid* g) {
 // host thread can drive the solves of many grids (each on its own stream).
 enum { JOB_PENDING = 0, JOB_DONE = 1 };
 
-int enqueue_state_readback(cfd_grid* g) {
-  HIP_TRY(hipMemcpyAsync(g->h_st, g->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipEventRecord(g->evs, g->stream));
+void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local);
+
+// Queues `len` iterations and a read-back of the state after them into the next
+// host slot.
+int enqueue_chunk(cfd_grid* g, int len) {
+  cfd_grid::Job& J = g->job;
+  const int parity = (int)(J.enq & 1);
+  if (g->use_graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
+    hipGraphExec_t seg = nullptr;
+    int rc = segment_graph(g, len, parity, &seg);
+    if (rc) return rc;
+    HIP_TRY(hipGraphLaunch(seg, g->stream));
+  } else {
+    for (int i = 0; i < len; ++i) enqueue_iteration(g, (parity + i) & 1);
+  }
+  J.enq += len;
+  const int slot = J.slot;
+  J.slot = (J.slot + 1) % kRing;
+  HIP_TRY(hipMemcpyAsync(g->h_st + slot, g->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipEventRecord(g->evr[slot], g->stream));
+  J.inflight.emplace_back(slot, J.enq);
   return 0;
 }
 
-int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations, double* d_x) {
+// Queues segments of at most kSegment iterations until `target` iterations are
+// queued (at most the iteration limit, and at most kAhead segments past the last
+// read-back); at least one segment when none is pending.
+int queue_ahead(cfd_grid* g, int64_t target) {
+  cfd_grid::Job& J = g->job;
+  target = std::min(target, J.max);
+  if (target <= J.enq && J.inflight.empty()) target = std::min(J.enq + 1, J.max);
+  while (J.enq < target && (int)J.inflight.size() < kAhead) {
+    int rc = enqueue_chunk(g, (int)std::min<int64_t>(kSegment, target - J.enq));
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// The iterations a running solve is expected to need, from a read-back state: the
+// mean reduction of the recurrent residual per iteration so far, extrapolated to
+// the tolerance.
+int64_t expected_iterations(const PcgState& s, double tol, int64_t queued) {
+  if (s.it <= 0 || !(s.rel > 0.0) || !(s.rel < 1.0)) return queued + kSegment;
+  if (s.rel <= tol) return s.it + 2;  // the recomputed residual missed its gate
+  const double need = std::log(tol / s.rel) / (std::log(s.rel) / (double)s.it);
+  if (!(need < 1e6)) return queued + kSegment;
+  return s.it + (int64_t)std::ceil(need);
+}
+
+int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations, double* d_x,
+              cfd_poll_fn poll = nullptr, void* user = nullptr) {
   if (!(tol > 0.0) || max_iterations <= 0) {
     set_error("pressure tolerance must be positive and the iteration limit nonzero (CFD-E008/E009)");
     return CFD_EINVAL;
@@ -961,9 +1038,25 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   enqueue_precondition(g, g->r.p, g->z.p, st);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);
-  int rc = enqueue_state_readback(g);
+  // Control::check before the first iteration (pressure.rs:387-389)
+  // (a zero or incompatible right-hand side ends the solve before that check)
+  if (int p = poll ? poll(user) : 0; p == 1 || p == 2) {
+    int rc = read_state(g);
+    if (rc) return rc;
+    if (g->h_st[0].status == ST_RUNNING) {
+      fill_info(g, g->h_st[0], J.info);
+      J.rc = p == 1 ? CFD_CANCELLED : CFD_WALL_TIME;
+      J.error = p == 1 ? "cancelled" : "wall_time_limit";
+      J.phase = 4;
+      return 0;
+    }
+  }
+  // the first iterations go into the queue at once: as many as the last such solve
+  // took, else one segment; kernels queued past the end of a solve do nothing
+  auto known = g->learned.find(learn_key(g, tol));
+  int rc = queue_ahead(g, known != g->learned.end() ? known->second : kSegment);
   if (rc) return rc;
-  J.phase = 1;
+  J.phase = 2;
   return 0;
 }
 
@@ -1007,6 +1100,11 @@ void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local) {
     }
     local.cycles = local.preconditioner_applications;
     local.smoothing_cell_updates = local.cycles * (4 * (cells - block_cells) + 2 * block_cells);
+    if (g->opt.cycle == CFD_CYCLE_RB && cells > 0) {
+      // the red-black cycle's rounds per side: fine level and coarse levels
+      const int64_t n0 = g->agg.levels[0].n;
+      local.smoothing_cell_updates = local.cycles * 2 * (g->fine_rounds * n0 + g->coarse_rounds * (cells - n0));
+    }
     local.residual_cell_updates = local.cycles * cells;
     local.restriction_cell_updates = local.residual_cell_updates;
     local.prolongation_cell_updates = local.residual_cell_updates;
@@ -1027,10 +1125,15 @@ void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local) {
 int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
   cfd_grid::Job& J = g->job;
   if (J.phase == 4 || J.phase == 0) return JOB_DONE;
+  if (J.phase == 2 && J.inflight.empty()) {
+    set_error("pressure solve: no pending read-back of the solver state");
+    return CFD_EDEVICE;
+  }
+  hipEvent_t ev = J.phase == 3 ? g->evs : g->evr[J.inflight.front().first];
   if (block) {
-    HIP_TRY(hipEventSynchronize(g->evs));
+    HIP_TRY(hipEventSynchronize(ev));
   } else {
-    hipError_t q = hipEventQuery(g->evs);
+    hipError_t q = hipEventQuery(ev);
     if (q == hipErrorNotReady) return JOB_PENDING;
     if (q != hipSuccess) {
       set_error(std::string("HIP error: ") + hipGetErrorString(q));
@@ -1044,7 +1147,8 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
     J.phase = 4;
     return JOB_DONE;
   }
-  const PcgState s = *g->h_st;
+  const PcgState s = g->h_st[J.inflight.front().first];
+  J.inflight.pop_front();
   if (s.status == ST_RUNNING) {
     if (poll) {
       int p = poll(user);
@@ -1056,19 +1160,11 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
         return JOB_DONE;
       }
     }
-    if (g->use_graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
-      hipGraphExec_t seg = nullptr;
-      int rc = segment_graph(g, &seg);
-      if (rc) return rc;
-      HIP_TRY(hipGraphLaunch(seg, g->stream));
-    } else {
-      for (int it = 0; it < kSegment; ++it) enqueue_iteration(g, it & 1);
-    }
-    int rc = enqueue_state_readback(g);
+    int rc = queue_ahead(g, expected_iterations(s, J.tol, J.enq));
     if (rc) return rc;
-    J.phase = 2;
     return JOB_PENDING;
   }
+  if (s.status == ST_CONVERGED) g->learned[learn_key(g, J.tol)] = s.it;
   fill_info(g, s, J.info);
   int rc = finish_status(g, s, J.max);
   if (rc) {
@@ -1096,7 +1192,7 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
 
 int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
                  cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info) {
-  int rc = job_begin(g, d_b, d_scale, tol, max_iterations, d_x);
+  int rc = job_begin(g, d_b, d_scale, tol, max_iterations, d_x, poll, user);
   if (rc) return rc;
   for (;;) {
     rc = job_poll(g, true, poll, user);
@@ -1250,11 +1346,9 @@ int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
     g->footprint += bytes;
     if (rc) return rc;
     // the captured iteration segments launch the previous cycle's kernels
-    for (auto& e : g->seg)
-      if (e) {
-        (void)hipGraphExecDestroy(e);
-        e = nullptr;
-      }
+    for (auto& kv : g->seg)
+      if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    g->seg.clear();
   }
   const int cycle = g->has_hier ? o->cycle : g->opt.cycle;
   g->opt = *o;

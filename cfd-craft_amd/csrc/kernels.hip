@@ -2338,7 +2338,10 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
-                       double omega, hipStream_t s, const FineRB<T>* rb) {
+                       double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds) {
+  // a round is one sweep over every colour class; pre-smoothing runs the classes in
+  // ascending order, post-smoothing in descending order (the adjoint)
+  const int fr = fine_rounds, cr = coarse_rounds;
   const T cs = (T)omega;
   const unsigned ab = blocks_for(g0.N);
   auto direct = [&](const auto* b, T* x, double* o64) {
@@ -2365,18 +2368,18 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // colour-separated fine level: scatter + red, black, red, black; restriction
     const unsigned hb = blocks_for(rb->half);
     hipLaunchKernelGGL((k_rb_first<T>), dim3(ab), dim3(BS), 0, s, g0, codes, (const double*)b0, *rb, st);
-    for (int pass = 1; pass < 4; ++pass)
+    for (int pass = 1; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
                          st);
     hipLaunchKernelGGL((k_rb_restrict<T>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, *rb, fparent, lv[0].b,
                        lv[0].x, st);
   } else if (!seams) {
-    for (int pass = 0; pass < 4; ++pass)
+    for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
                          none, (double*)nullptr, 0, st, cs, fparent, 0);
   } else {
     hipLaunchKernelGGL((k_fill_t<T>), dim3(ab), dim3(BS), 0, s, g0.N, x0, st);
-    for (int round = 0; round < 2; ++round)
+    for (int round = 0; round < fr; ++round)
       for (int c = 0; c < fine_colours; ++c)
         hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(ab), dim3(BS), 0, s, g0, codes, b0, x0, c, 0, nox, none,
                            (double*)nullptr, 1, st, cs, fparent, seams);
@@ -2387,7 +2390,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const int stop = nlev - 1;
   for (int l = 0; l < stop; ++l) {
     const AggDevLevel<T>& L = lv[l];
-    for (int round = 0; round < 2; ++round)
+    for (int round = 0; round < cr; ++round)
       for (int c = 0; c < L.ncol; ++c) relax(L, c);
     hipLaunchKernelGGL((k_agg_restrict<T>), dim3(blocks_for(8LL * lv[l + 1].n)), dim3(BS), 0, s, L, lv[l + 1], st);
   }
@@ -2402,12 +2405,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, 1, st, xc, cs);
       else
         hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
-      relax(L, 0);
-      relax(L, 1);
-      relax(L, 0);
+      for (int pass = 1; pass < 2 * cr; ++pass) relax(L, (pass + 1) & 1);
     } else {
       hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
-      for (int round = 0; round < 2; ++round)
+      for (int round = 0; round < cr; ++round)
         for (int c = L.ncol - 1; c >= 0; --c) relax(L, c);
     }
   }
@@ -2415,28 +2416,29 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // layout) the final half-sweeps also write z in FP64
   if (rb) {
     const unsigned hb = blocks_for(rb->half);
-    for (int pass = 0; pass < 4; ++pass)
+    for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, (pass + 1) & 1,
-                         pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 ? out64 : (double*)nullptr, st);
+                         pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr,
+                         st);
   } else if (!seams) {
-    for (int pass = 0; pass < 4; ++pass)
+    for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
-                         pass == 0 ? (const T*)lv[0].x : nox, none, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs,
-                         fparent, 0);
+                         pass == 0 ? (const T*)lv[0].x : nox, none, pass >= 2 * fr - 2 ? out64 : (double*)nullptr, 0,
+                         st, cs, fparent, 0);
   } else {
     hipLaunchKernelGGL((k_agg_fine_prolong<T>), dim3(ab), dim3(BS), 0, s, g0.N, fparent, (const T*)lv[0].x, cs, x0, st);
-    for (int round = 0; round < 2; ++round)
+    for (int round = 0; round < fr; ++round)
       for (int c = fine_colours - 1; c >= 0; --c)
         hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(ab), dim3(BS), 0, s, g0, codes, b0, x0, c, 0, nox, none,
-                           (round == 1 && c == 0) ? out64 : (double*)nullptr, 1, st, cs, fparent, seams);
+                           (round == fr - 1 && c == 0) ? out64 : (double*)nullptr, 1, st, cs, fparent, seams);
   }
 }
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
-                                                const PcgState*, double, hipStream_t, const FineRB<double>*);
+                                                const PcgState*, double, hipStream_t, const FineRB<double>*, int, int);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
-                                               double, hipStream_t, const FineRB<float>*);
+                                               double, hipStream_t, const FineRB<float>*, int, int);
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {

@@ -137,7 +137,10 @@ typedef struct {
 } cfd_mg_stats;
 
 /* Control::check as a callback (solver.rs:122-139): 0 continue, 1 cancelled,
- * 2 wall_time_limit. Called before every 8th PCG iteration (pressure.rs:387-389). */
+ * 2 wall_time_limit. Called before the first PCG iteration and then at every
+ * read-back of the solver state, at most 8 iterations apart (pressure.rs:387-389);
+ * up to 16 further iterations may be queued on the device by then, and they end
+ * without effect. */
 typedef int (*cfd_poll_fn)(void* user);
 
 /* Solver options beyond the reference's case keys (serving and precision). */
