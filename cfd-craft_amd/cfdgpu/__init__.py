@@ -320,10 +320,15 @@ class Grid:
         self.case = case
         c, keep = case.to_c()
         h = ctypes.c_void_p()
-        self._build_began = time.perf_counter()  # when the hierarchy build reported next began
-        self._cycles_seen = {int(os.environ.get("CFD_MG_CYCLE", "0"))}
+        began = time.perf_counter()
         _check(lib().cfd_grid_create(ctypes.byref(c), ctypes.byref(h)))
+        self._attach(h, case, began)
+
+    def _attach(self, h, case: Case, began: float):
+        self.case = case
         self.h = h
+        self._build_began = began  # when the hierarchy build reported next began
+        self._cycles_seen = {int(os.environ.get("CFD_MG_CYCLE", "0"))}
         counts = np.zeros(8, np.int64)
         _check(lib().cfd_grid_counts(self.h, _ptr(counts)))
         self.count, self.face_count, self.fluid_count, self.n_components, self.n_anchored = (int(v) for v in counts[:5])
@@ -495,16 +500,11 @@ class Server:
         """Grid::build under the server's HBM budget (CfdError ENOMEM when it does not fit)."""
         c, keep = case.to_c()
         gh = ctypes.c_void_p()
+        began = time.perf_counter()
         _check(lib().cfd_server_admit(self.h, ctypes.byref(c), ctypes.byref(gh)))
         g = Grid.__new__(Grid)
-        g.case = case
-        g.h = gh
+        g._attach(gh, case, began)
         g.owner = self
-        counts = np.zeros(8, np.int64)
-        _check(lib().cfd_grid_counts(g.h, _ptr(counts)))
-        g.count, g.face_count, g.fluid_count, g.n_components, g.n_anchored = (int(v) for v in counts[:5])
-        g.tolerance = case.tolerance
-        g.max_iterations = case.max_iterations
         self.grids.append(g)
         return g
 
