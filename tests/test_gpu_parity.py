@@ -279,19 +279,29 @@ def test_odd_periodic_fine_level_colours(gpu_available, cells):
         assert rel(x, xo) <= 1e-10, (precision, info, io)
 
 
-@pytest.mark.parametrize("cells", [[64, 48, 40], [33, 27, 21]])
+LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130, 20, 9]),
+                ("cavity", [70, 18, 22]), ("cavity", [40, 2, 30]), ("cavity", [30, 1, 20])]
+
+
+@pytest.mark.parametrize("kind,cells", LAYOUT_CASES, ids=[f"{k}{'x'.join(map(str, c))}" for k, c in LAYOUT_CASES])
 @pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
-def test_colour_separated_fine_level_is_bitwise(gpu_available, cells, precision):
+def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, precision):
     """The red-black cycle with the fine level stored colour by colour gives the
     same bits as with the natural layout (same per-cell arithmetic), odd sizes
-    included."""
-    case = W.urban_block(n=tuple(cells), boxes=6)[0]
+    and periodic axes of 1 and 2 cells included."""
+    if kind == "urban":
+        case = W.urban_block(n=tuple(cells), boxes=6)[0]
+    else:
+        case = W.cavity(cells, MULTIGRID)
     r = np.random.default_rng(8).standard_normal(case.count)
+    if kind == "cavity":
+        r -= r.mean()  # one closed component
     out = []
     for layout in (1, 0):
         with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_DENSE=64):
             g = cfdgpu.Grid(case)
         g.set_options(precision)
+        print(f"[layout] {kind} {cells} precision={precision} layout={layout}", flush=True)
         out.append(g.precondition(r))
     assert np.array_equal(out[0], out[1]), float(np.max(np.abs(out[0] - out[1])))
 
