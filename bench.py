@@ -230,6 +230,22 @@ def large(steps=3):
     from cfdgpu import workloads as W
 
     case, b, t = W.urban_prisms()
+    # Grid::build's topology (rasterization, face codes, components) on the host
+    # and on the device, and a whole cold set-up each way
+    t0 = time.perf_counter()
+    cfdgpu.case_topology(case)
+    topo_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cfdgpu.case_topology_device(case)
+    topo_dev = time.perf_counter() - t0
+    os.environ["CFD_GPU_SETUP"] = "0"
+    try:
+        t0 = time.perf_counter()
+        gh = cfdgpu.Grid(case)
+        setup_host = time.perf_counter() - t0
+    finally:
+        os.environ.pop("CFD_GPU_SETUP", None)
+    del gh
     t0 = time.perf_counter()
     g = cfdgpu.Grid(case)
     setup = time.perf_counter() - t0
@@ -240,7 +256,9 @@ def large(steps=3):
     ref_info = g.solve_device(db, dx, 1e-12, 3000, d_scale=ds)
     ref = dx.numpy()
     out = {"cells": g.count, "fluid_cells": g.fluid_count, "solid_fraction": round(1 - g.fluid_count / g.count, 4),
-           "cold_setup_s": round(setup, 3), "reference_iterations_1e-12": ref_info["iterations"]}
+           "cold_setup_s": round(setup, 3), "cold_setup_host_topology_s": round(setup_host, 3),
+           "topology_host_s": round(topo_host, 3), "topology_device_s": round(topo_dev, 3),
+           "reference_iterations_1e-12": ref_info["iterations"]}
     for tol in (1e-8, 1e-10):
         for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
             g.set_options(prec)

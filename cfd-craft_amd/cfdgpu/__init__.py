@@ -189,6 +189,7 @@ def lib():
             "cfd_device_synchronize": (I, []),
             "cfd_last_error": (ctypes.c_char_p, []),
             "cfd_case_topology": (I, [P, P, P, P, P, I64, P]),
+            "cfd_case_topology_device": (I, [P, P, P, P, P, I64]),
             "cfd_profile_kernel": (I, [P, I, I, P, P]),
             "cfd_server_create": (I, [I, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_server_destroy": (None, [P]),
@@ -233,6 +234,19 @@ def case_topology(case: Case, ic0: bool = False):
     _check(lib().cfd_case_topology(ctypes.byref(c), _ptr(counts), _ptr(codes), _ptr(comp), _ptr(anch), anch.size,
                                    _ptr(diag) if diag is not None else None))
     return dict(counts=counts, codes=codes, component=comp, anchored=anch[: int(counts[3])].astype(bool), ic0_diagonal=diag)
+
+
+def case_topology_device(case: Case):
+    """The topology of case_topology built on the GPU (rasterization, face codes,
+    union-find components), as a dict of the same fields."""
+    c, keep = case.to_c()
+    n = int(np.prod(case.cells))
+    counts = np.zeros(8, np.int64)
+    codes = np.zeros(n, np.uint8)
+    comp = np.zeros(n, np.int64)
+    anch = np.zeros(n, np.uint8)
+    _check(lib().cfd_case_topology_device(ctypes.byref(c), _ptr(counts), _ptr(codes), _ptr(comp), _ptr(anch), anch.size))
+    return dict(counts=counts, codes=codes, component=comp, anchored=anch[: int(counts[3])].astype(bool))
 
 
 def case_multigrid_stats(case: Case, cycle: int = 0):

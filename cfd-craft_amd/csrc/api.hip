@@ -1227,7 +1227,9 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
     return CFD_EINVAL;
   }
   std::unique_ptr<cfd_grid> g(new cfd_grid());
-  int rc = build_host_grid(*c, g->host);
+  // rasterization, face codes and components on the device (CFD_GPU_SETUP=0: on the host)
+  const char* env = std::getenv("CFD_GPU_SETUP");
+  int rc = env && std::string(env) == "0" ? build_host_grid(*c, g->host) : build_device_grid(*c, g->host);
   if (rc) return rc;
   if (c->preconditioner == CFD_IC0) {
     bool fill = false;
@@ -1258,14 +1260,15 @@ static void host_counts(const HostGrid& h, int64_t counts[8]) {
   counts[7] = h.face_count[2];
 }
 
-int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
-                      int64_t anchored_cap, double* ic0_diagonal) {
+// cfd_case_topology on the host or the device.
+static int case_topology(const cfd_case* c, bool device, int64_t counts[8], uint8_t* codes, int64_t* component,
+                  uint8_t* anchored, int64_t anchored_cap, double* ic0_diagonal) {
   if (!c) {
     set_error("cfd_case_topology: null case");
     return CFD_EINVAL;
   }
   HostGrid h;
-  int rc = build_host_grid(*c, h);
+  int rc = device ? build_device_grid(*c, h) : build_host_grid(*c, h);
   if (rc) return rc;
   if (counts) host_counts(h, counts);
   if (codes) std::memcpy(codes, h.codes.data(), h.codes.size());
@@ -1278,6 +1281,16 @@ int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int6
     std::memcpy(ic0_diagonal, h.ic0_diag.data(), h.ic0_diag.size() * sizeof(double));
   }
   return CFD_OK;
+}
+
+int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
+                      int64_t anchored_cap, double* ic0_diagonal) {
+  return case_topology(c, false, counts, codes, component, anchored, anchored_cap, ic0_diagonal);
+}
+
+int cfd_case_topology_device(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component,
+                             uint8_t* anchored, int64_t anchored_cap) {
+  return case_topology(c, true, counts, codes, component, anchored, anchored_cap, nullptr);
 }
 
 int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) {
@@ -1295,7 +1308,7 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
   if (rc) return rc;
   AggHierarchy H;
   if (cycle == CFD_CYCLE_RB)
-    rc = build_agg_hierarchy(h, H, kAggDense);
+    rc = build_agg_hierarchy(h, H, kAggDense, kHostBlock);  // sizes only: no large block is inverted
   else
     rc = build_ref_hierarchy(h, H);
   if (rc) return rc;

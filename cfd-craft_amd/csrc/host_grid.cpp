@@ -47,14 +47,23 @@ int64_t first_center_at_or_above(int64_t n, double dx, double bound, bool nan_me
   return lo;
 }
 
-// Marks the cells whose centers a block contains; returns how many (per block, overlaps counted).
-int64_t rasterize(const cfd_block& b, const int64_t n[3], const double dx[3], std::vector<uint8_t>& fluid) {
-  int64_t r0[3], r1[3];
+}  // namespace
+
+bool block_ranges(const cfd_block& b, const int64_t n[3], const double dx[3], int64_t r0[3], int64_t r1[3]) {
   for (int a = 0; a < 3; ++a) {
     r0[a] = first_center_at_or_above(n[a], dx[a], b.min_m[a], true);
     r1[a] = std::max(first_center_at_or_above(n[a], dx[a], b.max_m[a], false), r0[a]);
-    if (r0[a] == r1[a]) return 0;
+    if (r0[a] == r1[a]) return false;
   }
+  return true;
+}
+
+namespace {
+
+// Marks the cells whose centers a block contains; returns how many (per block, overlaps counted).
+int64_t rasterize(const cfd_block& b, const int64_t n[3], const double dx[3], std::vector<uint8_t>& fluid) {
+  int64_t r0[3], r1[3];
+  if (!block_ranges(b, n, dx, r0, r1)) return 0;
   // Every center of the z range lies within the z bounds: one test per column decides it.
   double z = center(r0[2], dx[2]);
   int64_t count = 0;
@@ -87,7 +96,24 @@ void flow_vector(const cfd_case& c, double z, double v[3]) {
 
 }  // namespace
 
-int build_host_grid(const cfd_case& c, HostGrid& g) {
+void open_outlets(const cfd_case& c, const HostGrid& g, std::vector<uint8_t>& out) {
+  const int64_t nz = g.n[2];
+  out.assign((size_t)(6 * nz), 0);
+  for (int a = 0; a < 3; ++a) {
+    if (c.boundary[a] != CFD_OPEN) continue;
+    for (int side = 0; side < 2; ++side)
+      for (int64_t k = 0; k < nz; ++k) {
+        // the face's height: the cell center, or the bottom or top face of a z boundary
+        double face_z = a == 2 ? (double)(side == 0 ? 0 : nz) * g.dx[2] : ((double)k + 0.5) * g.dx[2];
+        double v[3];
+        flow_vector(c, face_z, v);
+        double normal = side == 0 ? -1.0 : 1.0;
+        out[(size_t)((2 * a + side) * nz + k)] = v[a] * normal > 1e-12 ? 1 : 0;
+      }
+  }
+}
+
+int grid_header(const cfd_case& c, HostGrid& g) {
   int64_t total = 1;
   for (int a = 0; a < 3; ++a) {
     int64_t minimum = c.boundary[a] == CFD_PERIODIC ? 1 : 2;
@@ -123,26 +149,39 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
     g.w_int[a] = area / (g.dx[a] * g.volume);
     g.w_out[a] = area / ((0.5 * g.dx[a]) * g.volume);
   }
-  const int64_t* n = g.n;
-  g.fluid.assign((size_t)total, 1);
-  for (int b = 0; b < c.n_blocks; ++b) {
-    if (rasterize(c.blocks[b], n, g.dx, g.fluid) == 0) {
-      char buf[200];
-      std::snprintf(buf, sizeof buf,
-                    "CFD-E028: geometry[%d] (%s) contains no cell center, so the grid does not represent it; enlarge it or "
-                    "refine the grid",
-                    b, c.blocks[b].n_vertices > 0 ? "prism" : "box");
-      set_error(buf);
-      return CFD_EINVAL;
-    }
-  }
+  for (int a = 0; a < 3; ++a) g.face_count[a] = total / g.n[a] * (g.n[a] + (g.periodic[a] ? 0 : 1));
+  return 0;
+}
+
+int block_error(const cfd_case& c, int b) {
+  char buf[200];
+  std::snprintf(buf, sizeof buf,
+                "CFD-E028: geometry[%d] (%s) contains no cell center, so the grid does not represent it; enlarge it or "
+                "refine the grid",
+                b, c.blocks[b].n_vertices > 0 ? "prism" : "box");
+  set_error(buf);
+  return CFD_EINVAL;
+}
+
+int count_fluid(HostGrid& g) {
   g.fluid_count = 0;
   for (uint8_t f : g.fluid) g.fluid_count += f;
   if (g.fluid_count == 0) {
     set_error("CFD-E030: geometry contains no fluid cells");
     return CFD_EINVAL;
   }
-  for (int a = 0; a < 3; ++a) g.face_count[a] = total / n[a] * (n[a] + (g.periodic[a] ? 0 : 1));
+  return 0;
+}
+
+int build_host_grid(const cfd_case& c, HostGrid& g) {
+  int rc = grid_header(c, g);
+  if (rc) return rc;
+  const int64_t total = g.count;
+  const int64_t* n = g.n;
+  g.fluid.assign((size_t)total, 1);
+  for (int b = 0; b < c.n_blocks; ++b)
+    if (rasterize(c.blocks[b], n, g.dx, g.fluid) == 0) return block_error(c, b);
+  if ((rc = count_fluid(g))) return rc;
 
   // Face classification (grid.rs:341-429), per cell side. A side couples the cell
   // into the pressure operator when its face is Interior or Outlet. A one-cell
@@ -192,19 +231,16 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
   // (grid.rs:430-462): the same partition and numbering as the reference.
   g.component.assign((size_t)total, -1);
   g.anchored.clear();
-  g.comp_size.clear();
   std::vector<int64_t> queue;
   for (int64_t start = 0; start < total; ++start) {
     if (!g.fluid[(size_t)start] || g.component[(size_t)start] != -1) continue;
     int32_t id = (int32_t)g.anchored.size();
     bool anchored = false;
-    int64_t size = 0;
     queue.clear();
     queue.push_back(start);
     g.component[(size_t)start] = id;
     for (size_t head = 0; head < queue.size(); ++head) {
       int64_t c0 = queue[head];
-      ++size;
       uint8_t code = g.codes[(size_t)c0];
       int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
       for (int a = 0; a < 3; ++a)
@@ -227,8 +263,13 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
         }
     }
     g.anchored.push_back(anchored ? 1 : 0);
-    g.comp_size.push_back(size);
   }
+  closed_lists(g);
+  return 0;
+}
+
+void closed_lists(HostGrid& g) {
+  const int64_t total = g.count;
   // Closed components: gauge lists, sorted by cell index.
   g.closed_of_comp.assign(g.anchored.size(), -1);
   int32_t n_closed = 0;
@@ -254,7 +295,6 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
       if (cl >= 0) g.closed_members[(size_t)fill[(size_t)cl]++] = (int32_t)c0;
     }
   }
-  return 0;
 }
 
 // Diagonal of the IC(0) factor of A + 1e-6*maxdiag*I (pressure.rs:118-222) in

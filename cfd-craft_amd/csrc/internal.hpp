@@ -34,7 +34,6 @@ struct HostGrid {
   // Fluid components in the reference's numbering (BFS from the lowest index, grid.rs:430-462).
   std::vector<int32_t> component;  // -1 for solid
   std::vector<uint8_t> anchored;   // per component
-  std::vector<int64_t> comp_size;
   // Closed (non-anchored) components: index into the closed list per component, and member lists.
   std::vector<int32_t> closed_of_comp;           // -1 if anchored
   std::vector<int32_t> closed_index;             // per cell: closed index or -1
@@ -49,6 +48,18 @@ struct HostGrid {
 
 // Builds the grid; returns a CFD error code (0 on success) and sets the error text.
 int build_host_grid(const cfd_case& c, HostGrid& g);
+// The same grid with rasterization, face codes and components on the device.
+int build_device_grid(const cfd_case& c, HostGrid& g);
+// Stages both builds share: validation, sizes and weights; the cell ranges a
+// block covers (false when empty); the CFD-E028 error of block b; the fluid
+// count with CFD-E030; per (axis, side, k) whether an open domain face is an
+// outlet (Flow::vector, case.rs:345-367); the closed-component lists.
+int grid_header(const cfd_case& c, HostGrid& g);
+bool block_ranges(const cfd_block& b, const int64_t n[3], const double dx[3], int64_t r0[3], int64_t r1[3]);
+int block_error(const cfd_case& c, int b);
+int count_fluid(HostGrid& g);
+void open_outlets(const cfd_case& c, const HostGrid& g, std::vector<uint8_t>& out);
+void closed_lists(HostGrid& g);
 void build_ic0_diagonal(HostGrid& g);
 
 // Geometric hierarchy: 2x2x2 bins of the structured grid with Galerkin

@@ -191,6 +191,13 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s);
 int cfd_case_topology(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component, uint8_t* anchored,
                       int64_t anchored_cap, double* ic0_diagonal);
 
+/* The same topology built on the current device (Grid::build, grid.rs:246-464):
+ * mask rasterization, face codes and union-find components in HIP kernels, the
+ * result copied back. cfd_grid_create builds its grids this way unless
+ * CFD_GPU_SETUP=0. */
+int cfd_case_topology_device(const cfd_case* c, int64_t counts[8], uint8_t* codes, int64_t* component,
+                             uint8_t* anchored, int64_t anchored_cap);
+
 /* Host-only MultigridStats (multigrid.rs:55-72) of the hierarchy that `cycle`
  * builds for a case: unknowns and stored entries per level. CFD_CYCLE_REFERENCE:
  * the reference's own hierarchy (masked aggregation, masked.rs:181-291, or
