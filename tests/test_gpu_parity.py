@@ -207,6 +207,33 @@ def test_deterministic_repeat(gpu_available):
     assert np.array_equal(x1, x2) and i1["iterations"] == i2["iterations"]
 
 
+@pytest.mark.parametrize("pc", [MULTIGRID, IC0], ids=["multigrid", "ic0"])
+def test_queued_iterations_do_not_change_results(gpu_available, pc):
+    """The solver queues as many iterations as the last such solve needed and
+    extrapolates the rest; iterations queued past convergence do nothing. A grid
+    that learned a long solve (tight tolerance) and then solves an easier system,
+    or a harder one after an easy one, gives the bits and counts of a fresh grid,
+    for every tolerance in turn and with an iteration limit inside a queued
+    segment (CFD-E033 at the same count)."""
+    case, b, t = W.urban_block(n=(32, 32, 16), boxes=6)
+    case.preconditioner = pc
+    seq = [(b, 1e-12), (0.5 * b, 1e-6), (b, 1e-10), (2.0 * b, 1e-12), (b, 1e-8)]
+    g = cfdgpu.Grid(case)
+    for k, (rhs, tol) in enumerate(seq):
+        print(f"[queue] {pc} step {k} tol={tol}", flush=True)
+        x, info = g.solve_scaled(rhs, t, tol, 3000)
+        fresh = cfdgpu.Grid(case)
+        xf, inf = fresh.solve_scaled(rhs, t, tol, 3000)
+        assert info["iterations"] == inf["iterations"], (k, info, inf)
+        assert np.array_equal(x, xf), k
+    # a limit that falls inside the learned segments
+    n = info["iterations"]
+    for limit in (n - 1, 3):
+        with pytest.raises(cfdgpu.CfdError) as e:
+            g.solve_scaled(b, t, 1e-8, limit)
+        assert e.value.code == 33
+
+
 def _vcycle(case, precision, fused, r):
     with env(CFD_MG_FUSED=fused, CFD_MG_CYCLE=cfdgpu.CYCLE_GEOMETRIC):
         g = cfdgpu.Grid(case)
