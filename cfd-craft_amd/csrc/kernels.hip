@@ -1382,6 +1382,16 @@ __global__ void __launch_bounds__(BS) k_agg_gs(AggDevLevel<T> L, int colour, con
   L.x[i] = v;
 }
 
+// The first pre-smoothing half-sweep of a coarse level starts from a zero
+// iterate, so its colour-0 nodes take b / a_ii without reading neighbours; the
+// restriction stores that value at once (x of the other colours stays 0).
+template <class T>
+__device__ __forceinline__ T first_relax(int node, int red_end, T b, const T* __restrict__ diag) {
+  if (node >= red_end) return T(0);
+  const T d = diag[node];
+  return d != T(0) ? b / d : T(0);
+}
+
 // Restriction into level C: eight lanes per aggregate each form the residual of
 // one finer node (lane k takes children k, k + 8, ...); the first lane adds the
 // eight partial sums in lane order (ascending children when there are at most
@@ -1405,7 +1415,7 @@ __global__ void __launch_bounds__(BS) k_agg_restrict(AggDevLevel<T> F, AggDevLev
 #pragma unroll
   for (int s2 = 0; s2 < 8; ++s2) sum += rs[s2];
   C.b[J] = sum;
-  C.x[J] = T(0);
+  C.x[J] = first_relax<T>(J, C.cptr[1], sum, C.diag);
 }
 
 template <class T>
@@ -1539,7 +1549,8 @@ template <class T, class TB>
 __global__ void __launch_bounds__(BS) k_agg_fine_restrict(Geo g, const uint8_t* __restrict__ codes,
                                                           const TB* __restrict__ b, const T* __restrict__ x,
                                                           const int* __restrict__ fparent, T* __restrict__ bc,
-                                                          T* __restrict__ xc, const PcgState* st, int zero_colour) {
+                                                          T* __restrict__ xc, const T* __restrict__ cdiag, int red_end,
+                                                          const PcgState* st, int zero_colour) {
   const int status = st ? st->status : ST_RUNNING;
   const unsigned t = blockIdx.x * BS + threadIdx.x;
   const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
@@ -1577,7 +1588,7 @@ __global__ void __launch_bounds__(BS) k_agg_fine_restrict(Geo g, const uint8_t* 
         done |= 1u << u;
       }
     bc[ps[s2]] = sum;
-    xc[ps[s2]] = T(0);
+    xc[ps[s2]] = first_relax<T>(ps[s2], red_end, sum, cdiag);
   }
 }
 
@@ -1766,7 +1777,8 @@ __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const
 // Restriction from the colour-separated fine level: as k_agg_fine_restrict.
 template <class T>
 __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const int* __restrict__ fparent,
-                                                    T* __restrict__ bc, T* __restrict__ xc, const PcgState* st) {
+                                                    T* __restrict__ bc, T* __restrict__ xc,
+                                                    const T* __restrict__ cdiag, int red_end, const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
   const unsigned t = blockIdx.x * BS + threadIdx.x;
   const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
@@ -1805,7 +1817,7 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
         done |= 1u << u;
       }
     bc[ps[s2]] = sum;
-    xc[ps[s2]] = T(0);
+    xc[ps[s2]] = first_relax<T>(ps[s2], red_end, sum, cdiag);
   }
 }
 
@@ -2372,7 +2384,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
                          st);
     hipLaunchKernelGGL((k_rb_restrict<T>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, *rb, fparent, lv[0].b,
-                       lv[0].x, st);
+                       lv[0].x, lv[0].diag, lv[0].cptr[1], st);
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
@@ -2386,12 +2398,13 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   }
   if (!rb)
     hipLaunchKernelGGL((k_agg_fine_restrict<T, TB>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, codes, b0,
-                       (const T*)x0, fparent, lv[0].b, lv[0].x, st, seams ? -1 : 1);
+                       (const T*)x0, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st, seams ? -1 : 1);
   const int stop = nlev - 1;
   for (int l = 0; l < stop; ++l) {
     const AggDevLevel<T>& L = lv[l];
+    // the first half-sweep (colour 0 from zero) happened in the restriction
     for (int round = 0; round < cr; ++round)
-      for (int c = 0; c < L.ncol; ++c) relax(L, c);
+      for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) relax(L, c);
     hipLaunchKernelGGL((k_agg_restrict<T>), dim3(blocks_for(8LL * lv[l + 1].n)), dim3(BS), 0, s, L, lv[l + 1], st);
   }
   direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
