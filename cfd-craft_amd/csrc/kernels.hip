@@ -1679,20 +1679,24 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
                                               const int* __restrict__ fparent, T cs, double* __restrict__ out64,
                                               const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
-  // each XCD takes a contiguous range of rows, so the rows above and below a
-  // block's rows are in the same L2
-  const long long u = (long long)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-  if (u >= F.half) return;
-  const unsigned hx = (unsigned)F.hx;
-  const unsigned row = (unsigned)(u / hx);
-  const int i2 = (int)(u - (long long)row * hx);
-  const int k = (int)(row / (unsigned)g.n[1]);
-  const int j = (int)(row - (unsigned)k * (unsigned)g.n[1]);
+  // a workgroup covers 64 cells of each of 4 rows (x and y); tiles are numbered
+  // x, then y, then z, and each XCD takes a contiguous range of them, so the rows
+  // above and below a workgroup's rows are in the same L2. The tile's position
+  // is block-uniform (scalar arithmetic); no per-cell division.
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned gx = (hx + 63) >> 6, gy = (ny + 3) >> 2;
+  const unsigned tile = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned txy = tile % (gx * gy);
+  const int i2 = (int)((txy % gx) * 64 + (threadIdx.x & 63));
+  const int j = (int)((txy / gx) * 4 + (threadIdx.x >> 6));
+  const int k = (int)(tile / (gx * gy));
+  if (i2 >= (int)hx || j >= (int)ny) return;
   const int i = 2 * i2 + ((j + k + colour) & 1);
   if (i >= g.n[0]) return;
   const int q[3] = {i, j, k};
+  const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
+  const T* __restrict__ xo = F.x + (1 - colour) * F.half;
   const long long own = colour * F.half + u;
-  const long long other = (1 - colour) * F.half;
   const uint8_t code = F.codes[own];
   const T bv = F.b[own];
   T xn[6];
@@ -1702,9 +1706,9 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
     for (int side = 0; side < 2; ++side) {
       int qn[3];
       nb_coords(g, q, a, side, qn);
-      T v = F.x[other + rb_index(g, F.hx, qn[0], qn[1], qn[2])];
+      T v = xo[(unsigned)(qn[0] >> 1) + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2])];
       if (xc) {
-        const int p = fparent[(long long)qn[0] + qn[1] * g.sy + qn[2] * g.sz];
+        const int p = fparent[(unsigned)qn[0] + (unsigned)g.sy * (unsigned)qn[1] + (unsigned)g.sz * (unsigned)qn[2]];
         if (p >= 0) v += cs * xc[p];
       }
       xn[2 * a + side] = v;
@@ -1736,6 +1740,12 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
   const T v = diag > T(0) ? value / diag : T(0);
   F.x[own] = v;
   if (out64) out64[c] = (double)v;
+}
+
+// Workgroups of a half-sweep: tiles of 64 x 4 cells of a colour (x, y) per z layer.
+template <class T>
+unsigned rb_blocks(const Geo& g, const FineRB<T>& F) {
+  return (unsigned)(((F.hx + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[2]);
 }
 
 // Fine residual of cell q from the colour-separated layout.
@@ -2378,7 +2388,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
-    const unsigned hb = blocks_for(rb->half);
+    const unsigned hb = rb_blocks(g0, *rb);
     hipLaunchKernelGGL((k_rb_first<T>), dim3(ab), dim3(BS), 0, s, g0, codes, (const double*)b0, *rb, st);
     for (int pass = 1; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
@@ -2428,7 +2438,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // fine post-smoothing; in mixed precision (and always in the colour-separated
   // layout) the final half-sweeps also write z in FP64
   if (rb) {
-    const unsigned hb = blocks_for(rb->half);
+    const unsigned hb = rb_blocks(g0, *rb);
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, (pass + 1) & 1,
                          pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr,
@@ -2455,7 +2465,7 @@ template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {
-  hipLaunchKernelGGL((k_rb_gs<T>), dim3(blocks_for(rb.half)), dim3(BS), 0, s, g0, rb, colour, (const T*)nullptr,
+  hipLaunchKernelGGL((k_rb_gs<T>), dim3(rb_blocks(g0, rb)), dim3(BS), 0, s, g0, rb, colour, (const T*)nullptr,
                      (const int*)nullptr, T(0), (double*)nullptr, (const PcgState*)nullptr);
 }
 template void launch_rb_gs<double>(const Geo&, const FineRB<double>&, int, hipStream_t);
