@@ -1655,14 +1655,19 @@ template <class T>
 __global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ r,
                                                  FineRB<T> F, const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
-  const long long c = (long long)blockIdx.x * BS + threadIdx.x;
-  if (c >= g.N) return;
-  int q[3];
-  coords(g, c, q);
+  // tiles of 64 x 4 cells (x, y) per z layer, located by block-uniform arithmetic
+  const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
+  const unsigned gx = (nx + 63) >> 6, gy = (ny + 3) >> 2;
+  const unsigned tile = blockIdx.x, txy = tile % (gx * gy);
+  const int q[3] = {(int)((txy % gx) * 64 + (threadIdx.x & 63)), (int)((txy / gx) * 4 + (threadIdx.x >> 6)),
+                    (int)(tile / (gx * gy))};
+  if (q[0] >= (int)nx || q[1] >= (int)ny) return;
+  const unsigned c = (unsigned)q[0] + nx * ((unsigned)q[1] + ny * (unsigned)q[2]);
   const uint8_t code = codes[c];
   const T bv = (T)r[c];
   const int colour = (q[0] + q[1] + q[2]) & 1;
-  const long long at = colour * F.half + rb_index(g, F.hx, q[0], q[1], q[2]);
+  const long long at =
+      colour * F.half + (long long)((unsigned)(q[0] >> 1) + (unsigned)F.hx * ((unsigned)q[1] + ny * (unsigned)q[2]));
   if (status != ST_RUNNING) return;
   F.b[at] = bv;
   if (colour == 0) {
@@ -1752,8 +1757,10 @@ unsigned rb_blocks(const Geo& g, const FineRB<T>& F) {
 template <class T>
 __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const int q[3]) {
   const int colour = (q[0] + q[1] + q[2]) & 1;
-  const long long own = colour * F.half + rb_index(g, F.hx, q[0], q[1], q[2]);
-  const long long other = (1 - colour) * F.half;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  auto at = [&](const int v[3]) { return (unsigned)(v[0] >> 1) + hx * ((unsigned)v[1] + ny * (unsigned)v[2]); };
+  const long long own = colour * F.half + at(q);
+  const T* __restrict__ xo = F.x + (1 - colour) * F.half;
   const uint8_t code = F.codes[own];
   const T bv = F.b[own];
   const T xv = F.x[own];
@@ -1764,7 +1771,7 @@ __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const
     for (int side = 0; side < 2; ++side) {
       int qn[3];
       nb_coords(g, q, a, side, qn);
-      xn[2 * a + side] = F.x[other + rb_index(g, F.hx, qn[0], qn[1], qn[2])];
+      xn[2 * a + side] = xo[at(qn)];
     }
   if (!(code & FLUID)) return T(0);
   T ax = T(0), off = T(0);
@@ -1790,30 +1797,42 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
                                                     T* __restrict__ bc, T* __restrict__ xc,
                                                     const T* __restrict__ cdiag, int red_end, const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;
-  const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1, bz = (unsigned)(g.n[2] + 1) >> 1;
-  const unsigned B = t >> 3;
-  const int sub = (int)(t & 7);
+  // four lanes per 2x2x2 bin, 64 bins of one bin row per workgroup (block-uniform
+  // position); a bin at even coordinates holds its red cells at subs 0, 3, 5, 6
+  // and its black ones at 1, 2, 4, 7 (sub = dx + 2 dy + 4 dz): lane l takes red
+  // sub kRed[l] and the parent of black sub kBlack[l]
+  const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1;
+  const unsigned gbx = (bx + 63) >> 6;
+  const unsigned tile = blockIdx.x, rest = tile / gbx;
+  const unsigned Bx = (tile % gbx) * 64 + (threadIdx.x >> 2), By = rest % by, Bz = rest / by;
+  const int lane = (int)(threadIdx.x & 3);
+  const int red_sub = lane == 0 ? 0 : lane == 1 ? 3 : lane == 2 ? 5 : 6;
+  const int black_sub = lane == 0 ? 1 : lane == 1 ? 2 : lane == 2 ? 4 : 7;
+  const int sub = lane;  // the group's first lane sums
   T r = T(0);
-  int p = -1;
-  if (B < bx * by * bz) {
-    const unsigned Bx = B % bx, Byz = B / bx;
-    const unsigned By = Byz % by, Bz = Byz / by;
-    const int q[3] = {(int)(2 * Bx) + (sub & 1), (int)(2 * By) + ((sub >> 1) & 1), (int)(2 * Bz) + (sub >> 2)};
-    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) {
-      p = fparent[(long long)q[0] + q[1] * g.sy + q[2] * g.sz];
-      // the black cells were relaxed last: their residuals vanish (taken as zero)
-      if (((q[0] + q[1] + q[2]) & 1) == 0) r = rb_residual<T>(g, F, q);
+  int p = -1, pb = -1;
+  if (Bx < bx) {
+    const int qr[3] = {(int)(2 * Bx) + (red_sub & 1), (int)(2 * By) + ((red_sub >> 1) & 1), (int)(2 * Bz) + (red_sub >> 2)};
+    const int qb[3] = {(int)(2 * Bx) + (black_sub & 1), (int)(2 * By) + ((black_sub >> 1) & 1),
+                       (int)(2 * Bz) + (black_sub >> 2)};
+    if (qr[0] < g.n[0] && qr[1] < g.n[1] && qr[2] < g.n[2]) {
+      p = fparent[(unsigned)qr[0] + (unsigned)g.sy * (unsigned)qr[1] + (unsigned)g.sz * (unsigned)qr[2]];
+      r = rb_residual<T>(g, F, qr);
     }
+    // the black cells were relaxed last: their residuals vanish (taken as zero)
+    if (qb[0] < g.n[0] && qb[1] < g.n[1] && qb[2] < g.n[2])
+      pb = fparent[(unsigned)qb[0] + (unsigned)g.sy * (unsigned)qb[1] + (unsigned)g.sz * (unsigned)qb[2]];
   }
-  const int base = (int)(threadIdx.x & 63) & ~7;
-  T rs[8];
-  int ps[8];
-#pragma unroll
-  for (int s2 = 0; s2 < 8; ++s2) {
-    rs[s2] = __shfl(r, base + s2, 64);
-    ps[s2] = __shfl(p, base + s2, 64);
-  }
+  // the bin's eight (residual, parent) pairs in sub order
+  const int base = (int)(threadIdx.x & 63) & ~3;
+  const T r0 = __shfl(r, base, 64), r3 = __shfl(r, base + 1, 64), r5 = __shfl(r, base + 2, 64),
+          r6 = __shfl(r, base + 3, 64);
+  const int p0 = __shfl(p, base, 64), p3 = __shfl(p, base + 1, 64), p5 = __shfl(p, base + 2, 64),
+            p6 = __shfl(p, base + 3, 64);
+  const int p1 = __shfl(pb, base, 64), p2 = __shfl(pb, base + 1, 64), p4 = __shfl(pb, base + 2, 64),
+            p7 = __shfl(pb, base + 3, 64);
+  const T rs[8] = {r0, T(0), T(0), r3, T(0), r5, r6, T(0)};
+  const int ps[8] = {p0, p1, p2, p3, p4, p5, p6, p7};
   if (sub != 0 || status != ST_RUNNING) return;
   unsigned done = 0;
 #pragma unroll
@@ -2389,12 +2408,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
     const unsigned hb = rb_blocks(g0, *rb);
-    hipLaunchKernelGGL((k_rb_first<T>), dim3(ab), dim3(BS), 0, s, g0, codes, (const double*)b0, *rb, st);
+    hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
+                       g0, codes, (const double*)b0, *rb, st);
     for (int pass = 1; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
                          st);
-    hipLaunchKernelGGL((k_rb_restrict<T>), dim3(blocks_for(8 * bins)), dim3(BS), 0, s, g0, *rb, fparent, lv[0].b,
-                       lv[0].x, lv[0].diag, lv[0].cptr[1], st);
+    hipLaunchKernelGGL((k_rb_restrict<T>),
+                       dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)), dim3(BS), 0, s,
+                       g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
