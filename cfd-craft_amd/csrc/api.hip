@@ -893,7 +893,7 @@ void enqueue_iteration(cfd_grid* g, int parity) {
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p, g->stream);
   launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
   launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
@@ -1575,7 +1575,8 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
   auto once = [&]() {
     switch (kernel) {
       case 0:
-        launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, st, g->part.p, g->stream);
+        launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,
+                      g->stream);
         break;
       case 1:
         if (g->opt.cycle == CFD_CYCLE_RB && g->ash.has_rb) {

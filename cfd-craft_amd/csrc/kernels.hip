@@ -161,8 +161,29 @@ __global__ void __launch_bounds__(BS) k_apply(Geo g, const uint8_t* __restrict__
   y[c] = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
 }
 
+// Every cell of the grid, once, over a fixed reduction grid: each wave takes
+// segments of 64 cells of one x row in turn; a segment's row is wave-uniform
+// (scalar arithmetic), so no cell needs a division.
+template <class F>
+__device__ __forceinline__ void for_cells(const Geo& g, F f) {
+  const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
+  const unsigned sx = (nx + 63) >> 6, segs = sx * ny * (unsigned)g.n[2];
+  const unsigned lane = threadIdx.x & 63;
+  const unsigned waves = gridDim.x * (blockDim.x >> 6);
+  for (unsigned sg = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); sg < segs;
+       sg += waves) {
+    const unsigned row = sg / sx;
+    const unsigned i = (sg - row * sx) * 64 + lane;
+    if (i >= nx) continue;
+    const unsigned k = row / ny;
+    const int q[3] = {(int)i, (int)(row - k * ny), (int)k};
+    f((long long)i + (long long)nx * row, q);
+  }
+}
+
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
+template <bool HAS_CLOSED>
 __global__ void __launch_bounds__(RB)
     k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ z, const double* __restrict__ pold,
              double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
@@ -172,22 +193,20 @@ __global__ void __launch_bounds__(RB)
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   auto pv = [&](long long i) {
-    uint8_t ci = codes[i];
     double zc = z[i];
-    if (ci & CLOSED) zc -= mean[cidx ? cidx[i] : 0];
+    // closed components: z centered (without any, the code load is skipped)
+    if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
     return first ? zc : zc + beta * pold[i];
   };
   double acc = 0.0;
-  for (long long c = (long long)blockIdx.x * RB + threadIdx.x; c < g.N; c += (long long)gridDim.x * RB) {
-    int q[3];
-    coords(g, c, q);
+  for_cells(g, [&](long long c, const int q[3]) {
     uint8_t code = codes[c];
     double pc = pv(c);
     double y = apply_cell(g, code, c, q, pv);
     pnew[c] = pc;
     ap[c] = y;
     acc += pc * y;
-  }
+  });
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -238,13 +257,11 @@ __global__ void __launch_bounds__(RB) k_pcg_true(Geo g, const uint8_t* __restric
   const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
   if (!(rel <= st->tol)) return;
   double acc = 0.0;
-  for (long long c = (long long)blockIdx.x * RB + threadIdx.x; c < g.N; c += (long long)gridDim.x * RB) {
-    int q[3];
-    coords(g, c, q);
+  for_cells(g, [&](long long c, const int q[3]) {
     double y = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
     double d = y - bc[c];
     acc += d * d;
-  }
+  });
   double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_TR + blockIdx.x] = s;
 }
@@ -2137,8 +2154,11 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
   hipLaunchKernelGGL(k_apply, dim3(blocks_for(g.N)), dim3(BS), 0, s, g, codes, x, y);
 }
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
-                   const double* mean, const int* cidx, PcgState* st, double* part, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_op, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+                   const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s) {
+  if (closed)
+    hipLaunchKernelGGL(k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+  else
+    hipLaunchKernelGGL(k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
 }
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s) {

@@ -11,7 +11,7 @@ namespace cfdgpu {
 
 void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y, hipStream_t s);
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
-                   const double* mean, const int* cidx, PcgState* st, double* part, hipStream_t s);
+                   const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s);
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s);
 // the recomputed-residual kernel and the one-block decision kernel of an iteration
