@@ -842,6 +842,13 @@ void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031
   launch_gauge(g->geo.N, g->codes.p, g->n_closed, g->n_chunks, g->members.p, g->chunk_begin.p, g->comp_first_chunk.p,
                g->inv_count.p, g->inv_count1, src, scale, e031, mean, st, g->part.p, g->stream);
 }
+// The red-black cycle's last fine half-sweep computes the PCG's r . z partials
+// (colour-separated fine level, no closed component to center z in).
+bool rz_fused(const cfd_grid* g) {
+  return g->host.preconditioner != CFD_IC0 && g->opt.cycle == CFD_CYCLE_RB && g->ash.has_rb && g->n_closed == 0 &&
+         g->fine_rounds >= 1;
+}
+
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
   if (g->host.preconditioner == CFD_IC0) {
     if (g->ic0_tiles)
@@ -856,14 +863,19 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
   }
   if (g->opt.cycle == CFD_CYCLE_RB) {
     const DevAggShared& A = g->ash;
+    // without closed components z needs no centering: the last fine half-sweep
+    // forms the r . z partials as well (no separate k_pcg_rz)
+    const double* rz_r = rz_fused(g) ? r : nullptr;
     if (g->opt.precision == CFD_PRECISION_MIXED)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
                                        g->a32.levels.data(), A.D, z, st, g->omega, g->stream,
-                                       A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds);
+                                       A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
+                                       g->part.p);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb ? z : nullptr, st, g->omega, g->stream,
-                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds);
+                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
+                                        g->part.p);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -898,7 +910,7 @@ void enqueue_iteration(cfd_grid* g, int parity) {
   launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
-  launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
+  if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
 }
 
 // The captured graph of `len` iterations whose first one has parity `parity`.
@@ -1037,7 +1049,8 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
-  launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);
+  if (!rz_fused(g))
+    launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);
   // Control::check before the first iteration (pressure.rs:387-389)
   // (a zero or incompatible right-hand side ends the solve before that check)
   if (int p = poll ? poll(user) : 0; p == 1 || p == 2) {

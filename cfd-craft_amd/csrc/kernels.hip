@@ -1693,9 +1693,54 @@ __global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restric
   }
 }
 
-// One half-sweep of colour `colour`. xc: the prolongation x += cs e[parent]
-// rides on the neighbour reads (first post-smoothing half-sweep). out64: also
-// store the cell's value, in the natural layout and FP64, there.
+// The new value of the colour-`colour` cell at q (half-row index i2; u its offset
+// in its colour's array): Gauss-Seidel from the other colour's values, with the
+// prolongation x += cs e[parent] riding on the neighbour reads when xc is set.
+template <class T>
+__device__ __forceinline__ T rb_update(const Geo& g, const FineRB<T>& F, int colour, const int q[3], unsigned u,
+                                       const T* __restrict__ xc, const int* __restrict__ fparent, T cs) {
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const T* __restrict__ xo = F.x + (1 - colour) * F.half;
+  const long long own = colour * F.half + u;
+  const uint8_t code = F.codes[own];
+  const T bv = F.b[own];
+  T xn[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      int qn[3];
+      nb_coords(g, q, a, side, qn);
+      T v = xo[(unsigned)(qn[0] >> 1) + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2])];
+      if (xc) {
+        const int p = fparent[(unsigned)qn[0] + (unsigned)g.sy * (unsigned)qn[1] + (unsigned)g.sz * (unsigned)qn[2]];
+        if (p >= 0) v += cs * xc[p];
+      }
+      xn[2 * a + side] = v;
+    }
+  if (!(code & FLUID)) return T(0);
+  T diag = T(0);
+  T value = bv;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      if (boundary && !g.per[a]) {
+        diag += (T)g.wo[a];
+        continue;
+      }
+      const T w = (T)g.wi[a];
+      diag += w;
+      value += w * xn[2 * a + side];
+    }
+  }
+  return diag > T(0) ? value / diag : T(0);
+}
+
+// One half-sweep of colour `colour` (rb_update per cell). out64: also store the
+// cell's value, in the natural layout and FP64, there.
 template <class T>
 __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, const T* __restrict__ xc,
                                               const int* __restrict__ fparent, T cs, double* __restrict__ out64,
@@ -1717,51 +1762,45 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
   if (i >= g.n[0]) return;
   const int q[3] = {i, j, k};
   const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
-  const T* __restrict__ xo = F.x + (1 - colour) * F.half;
-  const long long own = colour * F.half + u;
-  const uint8_t code = F.codes[own];
-  const T bv = F.b[own];
-  T xn[6];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      int qn[3];
-      nb_coords(g, q, a, side, qn);
-      T v = xo[(unsigned)(qn[0] >> 1) + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2])];
-      if (xc) {
-        const int p = fparent[(unsigned)qn[0] + (unsigned)g.sy * (unsigned)qn[1] + (unsigned)g.sz * (unsigned)qn[2]];
-        if (p >= 0) v += cs * xc[p];
-      }
-      xn[2 * a + side] = v;
-    }
+  const T v = rb_update<T>(g, F, colour, q, u, xc, fparent, cs);
   if (status != ST_RUNNING) return;
-  const long long c = (long long)i + j * g.sy + k * g.sz;
-  if (!(code & FLUID)) {
-    F.x[own] = T(0);
-    if (out64) out64[c] = 0.0;
-    return;
-  }
-  T diag = T(0);
-  T value = bv;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      if (!(code & (1u << (2 * a + side)))) continue;
-      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
-      if (boundary && !g.per[a]) {
-        diag += (T)g.wo[a];
-        continue;
-      }
-      const T w = (T)g.wi[a];
-      diag += w;
-      value += w * xn[2 * a + side];
+  F.x[colour * F.half + u] = v;
+  if (out64) out64[(long long)i + j * g.sy + k * g.sz] = (double)v;
+}
+
+// The last post-smoothing half-sweep (colour 0, z into out64) together with the
+// PCG's r . z (pressure.rs:428) for a grid without closed components, whose z
+// needs no centering: each thread adds its red cell and the black cell beside
+// it (final since the previous half-sweep), and the fixed reduction grid
+// (RP blocks of RB threads, tiles of 64 x 16 cells taken in turn) writes the
+// partials that k_pcg_rz would.
+template <class T>
+__global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __restrict__ out64,
+                                                 const double* __restrict__ r, const PcgState* st, double* part) {
+  if (!block_running(st)) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1], nx = (unsigned)g.n[0];
+  const unsigned gx = (hx + 63) >> 6, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
+  double acc = 0.0;
+  for (unsigned tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const unsigned txy = tile % (gx * gy);
+    const int i2 = (int)((txy % gx) * 64 + (threadIdx.x & 63));
+    const int j = (int)((txy / gx) * 16 + (threadIdx.x >> 6));
+    const int k = (int)(tile / (gx * gy));
+    if (i2 >= (int)hx || j >= (int)ny) continue;
+    const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
+    const long long row = (long long)j * g.sy + k * g.sz;
+    const int ir = 2 * i2 + ((j + k) & 1), ib = 2 * i2 + ((j + k + 1) & 1);
+    if (ir < (int)nx) {
+      const int q[3] = {ir, j, k};
+      const T v = rb_update<T>(g, F, 0, q, u, (const T*)nullptr, (const int*)nullptr, T(0));
+      F.x[u] = v;
+      out64[row + ir] = (double)v;
+      acc += r[row + ir] * (double)v;
     }
+    if (ib < (int)nx) acc += r[row + ib] * (double)F.x[F.half + u];
   }
-  const T v = diag > T(0) ? value / diag : T(0);
-  F.x[own] = v;
-  if (out64) out64[c] = (double)v;
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
 }
 
 // Workgroups of a half-sweep: tiles of 64 x 4 cells of a colour (x, y) per z layer.
@@ -2399,7 +2438,8 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
-                       double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds) {
+                       double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds,
+                       const double* rz_r, double* rz_part) {
   // a round is one sweep over every colour class; pre-smoothing runs the classes in
   // ascending order, post-smoothing in descending order (the adjoint)
   const int fr = fine_rounds, cr = coarse_rounds;
@@ -2480,10 +2520,16 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // layout) the final half-sweeps also write z in FP64
   if (rb) {
     const unsigned hb = rb_blocks(g0, *rb);
-    for (int pass = 0; pass < 2 * fr; ++pass)
+    for (int pass = 0; pass < 2 * fr; ++pass) {
+      if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
+        // the last half-sweep also forms the PCG's r . z partials
+        hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
+        continue;
+      }
       hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, (pass + 1) & 1,
                          pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr,
                          st);
+    }
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
@@ -2499,10 +2545,12 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 }
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
-                                                const PcgState*, double, hipStream_t, const FineRB<double>*, int, int);
+                                                const PcgState*, double, hipStream_t, const FineRB<double>*, int, int,
+                                                const double*, double*);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
-                                               double, hipStream_t, const FineRB<float>*, int, int);
+                                               double, hipStream_t, const FineRB<float>*, int, int, const double*,
+                                               double*);
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {

@@ -51,7 +51,7 @@ template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb = nullptr, int fine_rounds = 2,
-                       int coarse_rounds = 2);
+                       int coarse_rounds = 2, const double* rz_r = nullptr, double* rz_part = nullptr);
 // One colour-separated fine half-sweep (timing hook).
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
