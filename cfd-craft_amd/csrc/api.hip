@@ -192,8 +192,8 @@ struct cfd_grid {
   bool unreported_build[4] = {false, false, false, false};
   bool cycle_built[4] = {false, false, false, false};
   double build_seconds[4] = {0, 0, 0, 0};
-  // the reference's cycle (CFD_CYCLE_REFERENCE)
-  DevRef ref;
+  // the reference's cycle (CFD_CYCLE_REFERENCE) and its multi-colour variant (CFD_CYCLE_ZEBRA)
+  DevRef ref, zeb;
   DevBuf<double> ref_mean;
   // captured iteration segments, by (length, parity of the first iteration, precision)
   std::map<int, hipGraphExec_t> seg;
@@ -565,12 +565,11 @@ int build_geometric(cfd_grid* g) {
 void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st);
 
 // Builds the reference's hierarchy (host) and the device arrays of its cycle.
-int build_ref(cfd_grid* g) {
-  DevRef& R = g->ref;
+int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
   if (R.built) return 0;
   const HostGrid& hg = g->host;
   AggHierarchy& H = R.h;
-  int rc = build_ref_hierarchy(hg, H);
+  int rc = build_ref_hierarchy(hg, H, zebra);
   if (rc) return rc;
   auto upi = [&](const std::vector<int32_t>& v) -> const int* {
     R.ints.emplace_back(new DevBuf<int>());
@@ -718,7 +717,7 @@ int build_ref(cfd_grid* g) {
   D.big_first = R.big_first.p;
   D.nsmall = (int)sb.size();
   D.small_block = R.small_block.p;
-  if ((rc = g->ref_mean.alloc((size_t)std::max(3, g->n_closed)))) return rc;
+  if (!g->ref_mean.p && (rc = g->ref_mean.alloc((size_t)std::max(3, g->n_closed)))) return rc;
   C.center0 = [g](const double* src, double* dst) {
     if (g->n_closed == 0) {
       if (src != dst)
@@ -733,14 +732,21 @@ int build_ref(cfd_grid* g) {
   return 0;
 }
 
+// The aggregation or unmasked hierarchy of the grid's current (non-geometric) cycle.
+const AggHierarchy& hierarchy_of(const cfd_grid* g) {
+  return g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->opt.cycle == CFD_CYCLE_ZEBRA ? g->zeb.h : g->agg;
+}
+
 int ensure_cycle(cfd_grid* g, int cycle) {
-  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_GEOMETRIC && cycle != CFD_CYCLE_REFERENCE) {
-    set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_GEOMETRIC");
+  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_GEOMETRIC && cycle != CFD_CYCLE_REFERENCE && cycle != CFD_CYCLE_ZEBRA) {
+    set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE, CFD_CYCLE_GEOMETRIC or CFD_CYCLE_ZEBRA");
     return CFD_EINVAL;
   }
   if (g->cycle_built[cycle]) return 0;
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = cycle == CFD_CYCLE_RB ? build_agg(g) : cycle == CFD_CYCLE_GEOMETRIC ? build_geometric(g) : build_ref(g);
+  int rc = cycle == CFD_CYCLE_RB          ? build_agg(g)
+           : cycle == CFD_CYCLE_GEOMETRIC ? build_geometric(g)
+                                          : build_ref(g, cycle == CFD_CYCLE_ZEBRA ? g->zeb : g->ref, cycle == CFD_CYCLE_ZEBRA);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->build_seconds[cycle] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -857,8 +863,8 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
       launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
     return;
   }
-  if (g->opt.cycle == CFD_CYCLE_REFERENCE) {
-    launch_ref_vcycle(g->ref.cyc, r, z, st, g->stream);
+  if (g->opt.cycle == CFD_CYCLE_REFERENCE || g->opt.cycle == CFD_CYCLE_ZEBRA) {
+    launch_ref_vcycle((g->opt.cycle == CFD_CYCLE_ZEBRA ? g->zeb : g->ref).cyc, r, z, st, g->stream);
     return;
   }
   if (g->opt.cycle == CFD_CYCLE_RB) {
@@ -1093,7 +1099,7 @@ void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local) {
       cells = g->hier.coarse.empty() ? 0 : g->host.count;
       for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
     } else {
-      const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
+      const AggHierarchy& H = hierarchy_of(g);
       for (size_t l = 0; l + 1 < H.levels.size(); ++l) {
         cells += H.levels[l].n;
         if (H.levels[l].line_length > 0) {
@@ -1312,8 +1318,8 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
     return CFD_EINVAL;
   }
   std::memset(s, 0, sizeof *s);
-  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_REFERENCE) {
-    set_error("cfd_case_multigrid_stats: cycle must be CFD_CYCLE_RB or CFD_CYCLE_REFERENCE");
+  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_REFERENCE && cycle != CFD_CYCLE_ZEBRA) {
+    set_error("cfd_case_multigrid_stats: cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_ZEBRA");
     return CFD_EINVAL;
   }
   HostGrid h;
@@ -1323,7 +1329,7 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
   if (cycle == CFD_CYCLE_RB)
     rc = build_agg_hierarchy(h, H, kAggDense, kHostBlock);  // sizes only: no large block is inverted
   else
-    rc = build_ref_hierarchy(h, H);
+    rc = build_ref_hierarchy(h, H, cycle == CFD_CYCLE_ZEBRA);
   if (rc) return rc;
   int L = 0;
   for (const AggLevel& lv : H.levels) {
@@ -1359,8 +1365,9 @@ int cfd_grid_set_options(cfd_grid* g, const cfd_options* o) {
     set_error("cfd_options.precision must be FP64 or MIXED");
     return CFD_EINVAL;
   }
-  if (g->has_hier && o->cycle == CFD_CYCLE_REFERENCE && o->precision != CFD_PRECISION_FP64) {
-    set_error("cfd_options: the reference cycle runs in FP64");
+  if (g->has_hier && (o->cycle == CFD_CYCLE_REFERENCE || o->cycle == CFD_CYCLE_ZEBRA) &&
+      o->precision != CFD_PRECISION_FP64) {
+    set_error("cfd_options: the reference and zebra cycles run in FP64");
     return CFD_EINVAL;
   }
   if (g->has_hier && o->cycle != g->opt.cycle) {
@@ -1401,7 +1408,7 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
   if (g->opt.cycle != CFD_CYCLE_GEOMETRIC) {
     // MultigridStats of the aggregation hierarchy (masked.rs:286-290): unknowns and
     // stored entries (off-diagonal plus diagonal) per level
-    const AggHierarchy& H = g->opt.cycle == CFD_CYCLE_REFERENCE ? g->ref.h : g->agg;
+    const AggHierarchy& H = hierarchy_of(g);
     int L = 0;
     int64_t bytes = 0;
     for (const AggLevel& lv : H.levels) {

@@ -628,7 +628,7 @@ std::vector<int> line_axes(const HostGrid& g) {
 // line in order (a periodic cycle folded as 0, n-1, 1, n-2, ... to keep the band
 // narrow), the entries leaving the block, and the banded Cholesky factor with
 // reciprocal diagonal; then the lines' dependency levels for parallel sweeps.
-int build_lines(AggLevel& L, const int64_t n[3], const std::vector<int>& axes, const bool periodic[3]) {
+int build_lines(AggLevel& L, const int64_t n[3], const std::vector<int>& axes, const bool periodic[3], bool zebra) {
   const int64_t count = L.n;
   int64_t length = 1;
   for (int a : axes) length *= n[a];
@@ -715,6 +715,33 @@ int build_lines(AggLevel& L, const int64_t n[3], const std::vector<int>& axes, c
     std::vector<int64_t> at(ptr.begin(), ptr.end() - 1);
     for (int64_t l = 0; l < nl; ++l) lines[(size_t)at[(size_t)depth[(size_t)l]]++] = (int32_t)l;
   };
+  if (zebra) {
+    // colour classes instead: lines coupled by an outside entry differ in colour
+    // (greedy in line order); forward sweeps run the classes in ascending order,
+    // backward sweeps in descending order
+    std::vector<int32_t> colour((size_t)nl, -1);
+    std::vector<char> used;
+    for (int64_t l = 0; l < nl; ++l) {
+      used.assign(used.size(), 0);
+      for (int64_t at = l * length; at < (l + 1) * length; ++at)
+        for (int64_t e = L.line_ooff[(size_t)at]; e < L.line_ooff[(size_t)at + 1]; ++e) {
+          const int32_t c = colour[(size_t)(pos[(size_t)L.line_ocol[(size_t)e]] / length)];
+          if (c < 0) continue;
+          if ((size_t)c >= used.size()) used.resize((size_t)c + 1, 0);
+          used[(size_t)c] = 1;
+        }
+      int32_t c = 0;
+      while ((size_t)c < used.size() && used[(size_t)c]) ++c;
+      colour[(size_t)l] = c;
+    }
+    depth = colour;
+    group(L.line_fwd_ptr, L.line_fwd);
+    int32_t most = 0;
+    for (int32_t c : colour) most = std::max(most, c);
+    for (int64_t l = 0; l < nl; ++l) depth[(size_t)l] = most - colour[(size_t)l];
+    group(L.line_bwd_ptr, L.line_bwd);
+    return 0;
+  }
   for (int pass = 0; pass < 2; ++pass) {
     const bool forward = pass == 0;
     std::fill(depth.begin(), depth.end(), 0);
@@ -736,7 +763,7 @@ int build_lines(AggLevel& L, const int64_t n[3], const std::vector<int>& axes, c
   return 0;
 }
 
-int build_unmasked(const HostGrid& g, AggHierarchy& h) {
+int build_unmasked(const HostGrid& g, AggHierarchy& h, bool zebra) {
   h.masked = false;
   const int64_t N = g.count;
   h.node_of_cell.resize((size_t)N);
@@ -819,7 +846,7 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
           if (g.dx[a] < g.dx[best]) best = a;
         grp = {best};
       }
-      int e = build_lines(L, n, grp, periodic);
+      int e = build_lines(L, n, grp, periodic, zebra);
       if (e) return e;
       L.block_axes = grp;
       std::sort(L.block_axes.begin(), L.block_axes.end());
@@ -860,7 +887,7 @@ int build_unmasked(const HostGrid& g, AggHierarchy& h) {
 }
 
 // Dependency levels of the lexicographic sweeps of one CSR level.
-void sweep_schedule(AggLevel& L) {
+void sweep_schedule(AggLevel& L, bool zebra) {
   const int64_t n = L.n;
   std::vector<int32_t> depth((size_t)n, 0);
   auto group = [&](std::vector<int64_t>& ptr, std::vector<int32_t>& rows) {
@@ -873,6 +900,29 @@ void sweep_schedule(AggLevel& L) {
     std::vector<int64_t> at(ptr.begin(), ptr.end() - 1);
     for (int64_t i = 0; i < n; ++i) rows[(size_t)at[(size_t)depth[(size_t)i]]++] = (int32_t)i;
   };
+  if (zebra) {
+    // colour classes (greedy in row order): coupled rows differ in colour
+    std::vector<char> used;
+    std::fill(depth.begin(), depth.end(), -1);
+    int32_t most = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      used.assign(used.size(), 0);
+      for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
+        const int32_t c = depth[(size_t)L.col[(size_t)e]];
+        if (c < 0) continue;
+        if ((size_t)c >= used.size()) used.resize((size_t)c + 1, 0);
+        used[(size_t)c] = 1;
+      }
+      int32_t c = 0;
+      while ((size_t)c < used.size() && used[(size_t)c]) ++c;
+      depth[(size_t)i] = c;
+      most = std::max(most, c);
+    }
+    group(L.fwd_ptr, L.fwd_rows);
+    for (int64_t i = 0; i < n; ++i) depth[(size_t)i] = most - depth[(size_t)i];
+    group(L.bwd_ptr, L.bwd_rows);
+    return;
+  }
   for (int64_t i = 0; i < n; ++i) {
     int32_t d = 0;
     for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
@@ -891,7 +941,7 @@ void sweep_schedule(AggLevel& L) {
 
 }  // namespace
 
-int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h) {
+int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h, bool zebra) {
   h = AggHierarchy();
   if (g.fluid_count == 0) {
     set_error("CFD-E034: numerics.pressure_preconditioner: multigrid requires fluid cells");
@@ -901,10 +951,10 @@ int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h) {
   if (g.fluid_count < g.count || g.anchored.size() != 1) {
     rc = build_agg_hierarchy(g, h, 32);
   } else {
-    rc = build_unmasked(g, h);
+    rc = build_unmasked(g, h, zebra);
   }
   if (rc) return rc;
-  for (size_t l = 1; l < h.levels.size(); ++l) sweep_schedule(h.levels[l]);
+  for (size_t l = 1; l < h.levels.size(); ++l) sweep_schedule(h.levels[l], zebra);
   return 0;
 }
 

@@ -105,6 +105,7 @@ struct AggLevel {
   std::vector<double> child_w;
   // lexicographic sweeps by dependency level: forward rows grouped by the
   // longest chain of lower-numbered neighbours, backward by higher-numbered ones
+  // (or, in a zebra hierarchy, by colour class: ascending, then descending)
   std::vector<int64_t> fwd_ptr, bwd_ptr;
   std::vector<int32_t> fwd_rows, bwd_rows;
   // block smoother of an anisotropic level (multigrid/line.rs): lines (or small
@@ -143,11 +144,11 @@ struct AggHierarchy {
   bool permuted = false;
   bool masked = true;  // false: the unmasked trilinear hierarchy of a solid-free grid
 };
-// Coarsening stops when every component has at most coarse_cells nodes (the
-// reference: 32); blocks above host_block_max nodes are left to the device.
 // MultigridStats shape of an aggregation or unmasked hierarchy: level dimensions
 // (unmasked only) and the block smoothers (multigrid.rs:610-617, line.rs:200-209).
 void agg_hierarchy_shape(const AggHierarchy& h, cfd_mg_stats* s);
+// Coarsening stops when every component has at most coarse_cells nodes (the
+// reference: 32); blocks above host_block_max nodes are left to the device.
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
                         int64_t host_block_max = INT64_MAX);
 // The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked
@@ -155,7 +156,10 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
 // interpolation with Galerkin coarse operators (with line or small-plane block
 // smoothers and semicoarsening on anisotropic grids, multigrid.rs:33-47,573-611);
 // with the dependency levels of the lexicographic sweeps on levels >= 1.
-int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h);
+// zebra: colour classes instead of dependency levels (coupled rows, and coupled
+// lines or planes, differ in colour), for a multi-colour smoother that needs a
+// few launches per sweep instead of one per dependency level.
+int build_ref_hierarchy(const HostGrid& g, AggHierarchy& h, bool zebra = false);
 // Renumbers levels >= 1 so that each colour class (an independent set) is contiguous.
 void colour_agg_hierarchy(AggHierarchy& h);
 void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std::vector<int64_t>& nonzeros);

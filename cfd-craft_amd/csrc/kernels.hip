@@ -2063,6 +2063,48 @@ __global__ void __launch_bounds__(BS) k_ref_prolong(long long n, double* __restr
 // multigrid/line.rs:155-190): rhs minus the entries leaving the block, banded
 // forward and backward substitution with the reciprocal diagonal, stored back.
 // One thread per line; the lines of one level are not coupled to each other.
+// k_ref_lines with one wavefront per block: the lanes gather the block's
+// right-hand side (rhs minus its outside entries) and its banded factor into LDS
+// at once, one lane runs the forward and backward substitutions from LDS in the
+// same order as k_ref_lines (bit for bit), and the lanes write the block back.
+// LDS: length * (bw + 2) doubles (dynamic).
+__global__ void __launch_bounds__(64) k_ref_lines_lds(int length, int bw, const int* __restrict__ nodes,
+                                                      const double* __restrict__ factor, const int* __restrict__ ooff,
+                                                      const int* __restrict__ ocol, const double* __restrict__ oval,
+                                                      const int* __restrict__ lines, const double* __restrict__ rhs,
+                                                      double* __restrict__ x, const PcgState* st) {
+  extern __shared__ double lds[];
+  if (st && st->status != ST_RUNNING) return;
+  const int stride = bw + 1;
+  double* sv = lds;            // length values
+  double* fa = lds + length;   // length * stride factor entries
+  const long long start = (long long)lines[blockIdx.x] * length;
+  const long long base = start * stride;
+  for (int i = threadIdx.x; i < length; i += 64) {
+    double value = rhs[nodes[start + i]];
+    for (int e = ooff[start + i]; e < ooff[start + i + 1]; ++e) value -= oval[e] * x[ocol[e]];
+    sv[i] = value;
+  }
+  for (int i = threadIdx.x; i < length * stride; i += 64) fa[i] = factor[base + i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < length; ++i) {
+      const int row = i * stride;
+      double value = sv[i];
+      for (int j = i > bw ? i - bw : 0; j < i; ++j) value -= fa[row + i - j] * sv[j];
+      sv[i] = value * fa[row];
+    }
+    for (int i = length - 1; i >= 0; --i) {
+      double value = sv[i];
+      const int end = length < i + bw + 1 ? length : i + bw + 1;
+      for (int j = i + 1; j < end; ++j) value -= fa[j * stride + j - i] * sv[j];
+      sv[i] = value * fa[i * stride];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < length; i += 64) x[nodes[start + i]] = sv[i];
+}
+
 __global__ void __launch_bounds__(BS) k_ref_lines(int length, int bw, const int* __restrict__ nodes,
                                                   const double* __restrict__ factor, const int* __restrict__ ooff,
                                                   const int* __restrict__ ocol, const double* __restrict__ oval,
@@ -2621,7 +2663,12 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
     const int* lines = forward ? B.fwd : B.bwd;
     for (size_t d = 0; d + 1 < ptr.size(); ++d) {
       const int count = ptr[d + 1] - ptr[d];
-      if (count > 0)
+      if (count <= 0) continue;
+      const size_t lds = (size_t)B.length * (B.bandwidth + 2) * sizeof(double);
+      if (lds <= 48 * 1024)
+        hipLaunchKernelGGL(k_ref_lines_lds, dim3(count), dim3(64), lds, s, B.length, B.bandwidth, B.nodes, B.factor,
+                           B.ooff, B.ocol, B.oval, lines + ptr[d], rhs, x, st);
+      else
         hipLaunchKernelGGL(k_ref_lines, dim3(blocks_for(count)), dim3(BS), 0, s, B.length, B.bandwidth, B.nodes, B.factor,
                            B.ooff, B.ocol, B.oval, B.scratch, lines + ptr[d], count, rhs, x, st);
     }
