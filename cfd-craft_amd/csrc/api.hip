@@ -675,6 +675,22 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
     rc2.nclosed = (int)L.closed_ptr.size() - 1;
     rc2.closed_ptr = rc2.nclosed > 0 ? upi64(L.closed_ptr) : nullptr;
     rc2.closed_members = rc2.nclosed > 0 ? upi(L.closed_members) : nullptr;
+    {
+      std::vector<int32_t> cb, cc, kc{0};
+      for (int k = 0; k < rc2.nclosed; ++k) {
+        for (int64_t m = L.closed_ptr[(size_t)k]; m < L.closed_ptr[(size_t)k + 1]; m += 2048) {
+          cb.push_back((int32_t)m);
+          cc.push_back(k);
+        }
+        kc.push_back((int32_t)cb.size());
+      }
+      rc2.nchunks = (int)cb.size();
+      cb.push_back(rc2.nclosed > 0 ? (int32_t)L.closed_ptr.back() : 0);
+      rc2.chunk_begin = rc2.nchunks > 0 ? upi(cb) : nullptr;
+      rc2.chunk_comp = rc2.nchunks > 0 ? upi(cc) : nullptr;
+      rc2.comp_chunk = rc2.nchunks > 0 ? upi(kc) : nullptr;
+      rc2.chunk_sum = rc2.nchunks > 0 ? upd(std::vector<double>((size_t)rc2.nchunks, 0.0)) : nullptr;
+    }
     rc2.fwd_rows = upi(L.fwd_rows);
     rc2.bwd_rows = upi(L.bwd_rows);
     rc2.fwd_ptr.assign(L.fwd_ptr.begin(), L.fwd_ptr.end());

@@ -2137,24 +2137,37 @@ __global__ void __launch_bounds__(BS) k_ref_lines(int length, int bw, const int*
   for (int i = 0; i < length; ++i) x[nodes[start + i]] = scr[start + i];
 }
 
-// Subtracts the mean of each closed component (one block per component).
-__global__ void __launch_bounds__(BS) k_center_lists(double* __restrict__ v, const int* __restrict__ cptr,
-                                                     const int* __restrict__ members, const PcgState* st) {
+// Centering of a coarse level's closed components (masked.rs:16-34) over chunks
+// of their member lists: the sum of each chunk, then every chunk's block adds
+// its component's chunk sums in order and subtracts the mean from its members.
+__global__ void __launch_bounds__(BS) k_center_sum(const double* __restrict__ v, const int* __restrict__ cb,
+                                                   const int* __restrict__ members, double* __restrict__ sums,
+                                                   const PcgState* st) {
   if (st && st->status != ST_RUNNING) return;
   __shared__ double part[BS / 64];
-  const int comp = blockIdx.x;
-  const int m0 = cptr[comp], m1 = cptr[comp + 1];
+  const int c = blockIdx.x;
   double s = 0.0;
-  for (int k = m0 + (int)threadIdx.x; k < m1; k += BS) s += v[members[k]];
+  for (int k = cb[c] + (int)threadIdx.x; k < cb[c + 1]; k += BS) s += v[members[k]];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
   __syncthreads();
-  double total = 0.0;
+  if (threadIdx.x == 0) {
+    double total = 0.0;
 #pragma unroll
-  for (int w = 0; w < BS / 64; ++w) total += part[w];
-  const double mean = total / (double)(m1 - m0);
-  __syncthreads();
-  for (int k = m0 + (int)threadIdx.x; k < m1; k += BS) v[members[k]] -= mean;
+    for (int w = 0; w < BS / 64; ++w) total += part[w];
+    sums[c] = total;
+  }
+}
+__global__ void __launch_bounds__(BS) k_center_sub(double* __restrict__ v, const int* __restrict__ cb,
+                                                   const int* __restrict__ chunk_comp, const int* __restrict__ comp_chunk,
+                                                   const int* __restrict__ cptr, const int* __restrict__ members,
+                                                   const double* __restrict__ sums, const PcgState* st) {
+  if (st && st->status != ST_RUNNING) return;
+  const int c = blockIdx.x, comp = chunk_comp[c];
+  double total = 0.0;
+  for (int k = comp_chunk[comp]; k < comp_chunk[comp + 1]; ++k) total += sums[k];
+  const double mean = total / (double)(cptr[comp + 1] - cptr[comp]);
+  for (int k = cb[c] + (int)threadIdx.x; k < cb[c + 1]; k += BS) v[members[k]] -= mean;
 }
 
 // ---------------------------------------------------------------- projection
@@ -2646,8 +2659,11 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
     }
   };
   auto center = [&](const RefCoarse& C, double* v) {
-    if (C.nclosed > 0)
-      hipLaunchKernelGGL(k_center_lists, dim3(C.nclosed), dim3(BS), 0, s, v, C.closed_ptr, C.closed_members, st);
+    if (C.nchunks == 0) return;
+    hipLaunchKernelGGL(k_center_sum, dim3(C.nchunks), dim3(BS), 0, s, (const double*)v, C.chunk_begin,
+                       C.closed_members, C.chunk_sum, st);
+    hipLaunchKernelGGL(k_center_sub, dim3(C.nchunks), dim3(BS), 0, s, v, C.chunk_begin, C.chunk_comp, C.comp_chunk,
+                       C.closed_ptr, C.closed_members, (const double*)C.chunk_sum, st);
   };
   auto coarse_sweep = [&](const RefCoarse& C, bool forward) {
     const std::vector<int>& ptr = forward ? C.fwd_ptr : C.bwd_ptr;

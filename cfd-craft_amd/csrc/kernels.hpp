@@ -84,6 +84,14 @@ struct RefCoarse {
   const int* closed_ptr;  // closed components of this level (centering)
   const int* closed_members;
   int nclosed;
+  // the closed members in chunks of at most 2048, never spanning two components:
+  // chunk c covers closed_members[chunk_begin[c] .. chunk_begin[c + 1]) of
+  // component chunk_comp[c]; component k owns chunks comp_chunk[k] .. comp_chunk[k + 1]
+  const int* chunk_begin;
+  const int* chunk_comp;
+  const int* comp_chunk;
+  int nchunks;
+  double* chunk_sum;
   const int* fwd_rows;  // sweep schedules: rows of dependency level d at [ptr[d], ptr[d+1])
   const int* bwd_rows;
   std::vector<int> fwd_ptr, bwd_ptr;
