@@ -2063,6 +2063,44 @@ __global__ void __launch_bounds__(BS) k_ref_prolong(long long n, double* __restr
 // multigrid/line.rs:155-190): rhs minus the entries leaving the block, banded
 // forward and backward substitution with the reciprocal diagonal, stored back.
 // One thread per line; the lines of one level are not coupled to each other.
+// The forward and backward substitutions of one banded block (half-bandwidth
+// BW, reciprocal diagonal) in the order of k_ref_lines, with the last BW
+// solved values kept in registers, so the dependent chain does not wait on LDS.
+template <int BW>
+__device__ __forceinline__ void line_substitute(int length, double* sv, const double* fa) {
+  constexpr int stride = BW + 1;
+  double w[BW];  // w[k]: the value k + 1 places back in the sweep direction
+#pragma unroll
+  for (int k = 0; k < BW; ++k) w[k] = 0.0;
+  for (int i = 0; i < length; ++i) {
+    const int row = i * stride;
+    double value = sv[i];
+    // j ascending from max(0, i - BW) to i - 1: w[i - 1 - j]
+#pragma unroll
+    for (int k = BW - 1; k >= 0; --k)
+      if (i - 1 - k >= 0) value -= fa[row + 1 + k] * w[k];
+    value *= fa[row];
+#pragma unroll
+    for (int k = BW - 1; k > 0; --k) w[k] = w[k - 1];
+    w[0] = value;
+    sv[i] = value;
+  }
+#pragma unroll
+  for (int k = 0; k < BW; ++k) w[k] = 0.0;
+  for (int i = length - 1; i >= 0; --i) {
+    double value = sv[i];
+    // j ascending from i + 1 to min(length, i + BW + 1) - 1: w[j - i - 1]
+#pragma unroll
+    for (int k = 0; k < BW; ++k)
+      if (i + 1 + k < length) value -= fa[(i + 1 + k) * stride + 1 + k] * w[k];
+    value *= fa[i * stride];
+#pragma unroll
+    for (int k = BW - 1; k > 0; --k) w[k] = w[k - 1];
+    w[0] = value;
+    sv[i] = value;
+  }
+}
+
 // k_ref_lines with one wavefront per block: the lanes gather the block's
 // right-hand side (rhs minus its outside entries) and its banded factor into LDS
 // at once, one lane runs the forward and backward substitutions from LDS in the
@@ -2088,17 +2126,24 @@ __global__ void __launch_bounds__(64) k_ref_lines_lds(int length, int bw, const 
   for (int i = threadIdx.x; i < length * stride; i += 64) fa[i] = factor[base + i];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 0; i < length; ++i) {
-      const int row = i * stride;
-      double value = sv[i];
-      for (int j = i > bw ? i - bw : 0; j < i; ++j) value -= fa[row + i - j] * sv[j];
-      sv[i] = value * fa[row];
-    }
-    for (int i = length - 1; i >= 0; --i) {
-      double value = sv[i];
-      const int end = length < i + bw + 1 ? length : i + bw + 1;
-      for (int j = i + 1; j < end; ++j) value -= fa[j * stride + j - i] * sv[j];
-      sv[i] = value * fa[i * stride];
+    switch (bw) {
+      case 1: line_substitute<1>(length, sv, fa); break;
+      case 2: line_substitute<2>(length, sv, fa); break;
+      case 3: line_substitute<3>(length, sv, fa); break;
+      case 4: line_substitute<4>(length, sv, fa); break;
+      default:
+        for (int i = 0; i < length; ++i) {
+          const int row = i * stride;
+          double value = sv[i];
+          for (int j = i > bw ? i - bw : 0; j < i; ++j) value -= fa[row + i - j] * sv[j];
+          sv[i] = value * fa[row];
+        }
+        for (int i = length - 1; i >= 0; --i) {
+          double value = sv[i];
+          const int end = length < i + bw + 1 ? length : i + bw + 1;
+          for (int j = i + 1; j < end; ++j) value -= fa[j * stride + j - i] * sv[j];
+          sv[i] = value * fa[i * stride];
+        }
     }
   }
   __syncthreads();
