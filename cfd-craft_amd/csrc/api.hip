@@ -491,7 +491,7 @@ int build_agg(cfd_grid* g) {
     tl_bytes = nullptr;
     DevBuf<double> work;
     DevBuf<int> bad;
-    rc = work.alloc((size_t)(most * most + 3 * most));
+    rc = work.alloc((size_t)(most * most + 64 * 64 + 129 * most));  // matrix, pivot block, panels, means
     if (!rc) rc = bad.alloc(1);
     tl_bytes = counting;
     if (rc) return rc;

@@ -8,6 +8,7 @@
 // operation, so their results are bitwise equal to the serial CPU code.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device.hpp"
@@ -1497,37 +1498,127 @@ __global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __re
   if (out64) out64[r] = (double)(TO)total;
 }
 
-// Set-up of a big coarsest block on the device: in-place Gauss-Jordan inversion
-// of the SPD k x k matrix A (no pivoting needed), one pivot per pair of
-// launches, then embedding into the m x m block (first row and column zero for
-// a closed component) and the symmetric centering C G C.
-__global__ void __launch_bounds__(BS) k_gj_pivot(const double* __restrict__ A, int k, int p, double* __restrict__ row,
-                                                 double* __restrict__ col, int* __restrict__ bad) {
-  const int t = (int)(blockIdx.x * BS + threadIdx.x);
-  if (t >= k) return;
-  row[t] = A[(long long)p * k + t];
-  col[t] = A[(long long)t * k + p];
-  if (t == p) {
-    const double d = A[(long long)p * k + p];
-    if (!(d > 0.0) || !isfinite(d)) *bad = 1;
+// Set-up of a big coarsest block on the device: in-place blocked Gauss-Jordan
+// inversion of the SPD k x k matrix A (no pivoting needed), then embedding into
+// the m x m block (first row and column zero for a closed component) and the
+// symmetric centering C G C. A step takes the next nb <= kGJ pivots P:
+//   A_PP <- A_PP^-1 (scalar Gauss-Jordan in LDS; a pivot that is not positive
+//                    flags the block as no longer definite),
+//   A_Pj <- A_PP^-1 A_Pj, A_iP <- -A_iP A_PP^-1, A_ij <- A_ij - A_iP A_PP^-1 A_Pj
+// for i, j outside P: three launches and one rank-nb update per step.
+constexpr int kGJ = 64;
+
+constexpr int kGJThreads = 1024;
+__global__ void __launch_bounds__(kGJThreads) k_gj_panel(const double* __restrict__ A, int k, int p0, int nb,
+                                                         double* __restrict__ pinv, int* __restrict__ bad) {
+  static_assert(kGJ == 64, "entries are located by shifts");
+  __shared__ double M[kGJ * kGJ];
+  __shared__ double rowq[kGJ], colq[kGJ];
+  const int t = (int)threadIdx.x;
+  for (int e = t; e < kGJ * kGJ; e += kGJThreads) {
+    const int i = e >> 6, j = e & 63;
+    M[e] = (i < nb && j < nb) ? A[(long long)(p0 + i) * k + p0 + j] : 0.0;
+  }
+  __syncthreads();
+  for (int q = 0; q < nb; ++q) {
+    if (t < nb) {
+      rowq[t] = M[q * kGJ + t];
+      colq[t] = M[t * kGJ + q];
+    }
+    __syncthreads();
+    const double d = rowq[q];
+    if (t == 0 && (!(d > 0.0) || !isfinite(d))) *bad = 1;
+    const double inv = 1.0 / d;
+    for (int e = t; e < kGJ * kGJ; e += kGJThreads) {
+      const int i = e >> 6, j = e & 63;
+      if (i >= nb || j >= nb) continue;
+      double v;
+      if (i == q && j == q)
+        v = inv;
+      else if (i == q)
+        v = rowq[j] * inv;
+      else if (j == q)
+        v = -colq[i] * inv;
+      else
+        v = M[e] - colq[i] * (rowq[j] * inv);
+      M[e] = v;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < kGJ * kGJ; e += kGJThreads) pinv[e] = M[e];
+}
+
+// rp[s][j] = (A_PP^-1 A_Pj)[s][j] for j outside P, A_PP^-1[s][j - p0] inside;
+// cold[i][s] = A[i][p0 + s] (the column panel before the step).
+__global__ void __launch_bounds__(BS) k_gj_rows(const double* __restrict__ A, int k, int p0, int nb,
+                                                const double* __restrict__ pinv, double* __restrict__ rp,
+                                                double* __restrict__ cold) {
+  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long nrow = (long long)nb * k;
+  if (t < nrow) {
+    const int s = (int)(t / k), j = (int)(t - (long long)s * k);
+    double v;
+    if (j >= p0 && j < p0 + nb) {
+      v = pinv[s * kGJ + (j - p0)];
+    } else {
+      v = 0.0;
+      for (int q = 0; q < nb; ++q) v += pinv[s * kGJ + q] * A[(long long)(p0 + q) * k + j];
+    }
+    rp[(long long)s * k + j] = v;
+  } else if (t < 2 * nrow) {
+    const long long e = t - nrow;
+    const int i = (int)(e / nb), s = (int)(e - (long long)i * nb);
+    cold[(long long)i * kGJ + s] = A[(long long)i * k + p0 + s];
   }
 }
-__global__ void __launch_bounds__(BS) k_gj_update(double* __restrict__ A, int k, int p, const double* __restrict__ row,
-                                                  const double* __restrict__ col) {
-  const long long t = (long long)blockIdx.x * BS + threadIdx.x;
-  if (t >= (long long)k * k) return;
-  const int i = (int)(t / k), j = (int)(t - (long long)i * k);
-  const double d = row[p];
-  double v;
-  if (i == p && j == p)
-    v = 1.0 / d;
-  else if (i == p)
-    v = row[j] / d;
-  else if (j == p)
-    v = -col[i] / d;
-  else
-    v = A[t] - col[i] * row[j] / d;
-  A[t] = v;
+
+// The rank-nb update over 64 x 64 tiles of A (each thread 4 x 4 entries).
+__global__ void __launch_bounds__(BS) k_gj_update(double* __restrict__ A, int k, int p0, int nb,
+                                                  const double* __restrict__ rp, const double* __restrict__ cold) {
+  __shared__ double Cs[64 * kGJ];  // Cs[r * kGJ + s] = cold[i0 + r][s]
+  __shared__ double Rs[kGJ * 64];  // Rs[s * 64 + c] = rp[s][j0 + c]
+  const int tilesx = (k + 63) / 64;
+  const int i0 = (int)(blockIdx.x / tilesx) * 64, j0 = (int)(blockIdx.x % tilesx) * 64;
+  for (int e = (int)threadIdx.x; e < 64 * kGJ; e += BS) {
+    const int r = e / kGJ, s2 = e - r * kGJ;
+    Cs[e] = (i0 + r < k && s2 < nb) ? cold[(long long)(i0 + r) * kGJ + s2] : 0.0;
+    const int s3 = e / 64, c = e - s3 * 64;
+    Rs[e] = (j0 + c < k && s3 < nb) ? rp[(long long)s3 * k + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int tx = (int)(threadIdx.x & 15), ty = (int)(threadIdx.x >> 4);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    double cv[4], rv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) cv[a] = Cs[(ty + 16 * a) * kGJ + q];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) rv[b] = Rs[q * 64 + tx + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += cv[a] * rv[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ty + 16 * a;
+    if (i >= k) continue;
+    const bool irow = i >= p0 && i < p0 + nb;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tx + 16 * b;
+      if (j >= k) continue;
+      const long long at = (long long)i * k + j;
+      if (irow)
+        A[at] = rp[(long long)(i - p0) * k + j];
+      else
+        A[at] = ((j >= p0 && j < p0 + nb) ? 0.0 : A[at]) - acc[a][b];
+    }
+  }
 }
 __global__ void __launch_bounds__(BS) k_block_embed(double* __restrict__ G, int m, int skip, const double* __restrict__ A) {
   const long long t = (long long)blockIdx.x * BS + threadIdx.x;
@@ -2660,21 +2751,25 @@ void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s)
 template void launch_rb_gs<double>(const Geo&, const FineRB<double>&, int, hipStream_t);
 template void launch_rb_gs<float>(const Geo&, const FineRB<float>&, int, hipStream_t);
 
-// Inverts the deferred coarsest blocks on the device (see k_gj_pivot); returns
+// Inverts the deferred coarsest blocks on the device (see k_gj_panel); returns
 // false when a pivot is not positive (the block lost definiteness).
 bool invert_block(double* Gblock, int m, bool anchored, const double* K, double* work, int* bad, hipStream_t s) {
   const int skip = anchored ? 0 : 1;
   const int k = m - skip;
-  double* A = work;                        // k * k
-  double* row = work + (long long)k * k;   // k
-  double* col = row + k;                   // k
-  double* mu = col + k;                    // m
+  double* A = work;                              // k * k
+  double* pinv = work + (long long)k * k;        // kGJ * kGJ
+  double* rp = pinv + kGJ * kGJ;                 // kGJ * k
+  double* cold = rp + (long long)kGJ * k;        // k * kGJ
+  double* mu = cold + (long long)k * kGJ;        // m
   if (hipMemcpyAsync(A, K, (size_t)k * k * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) return false;
   if (hipMemsetAsync(bad, 0, sizeof(int), s) != hipSuccess) return false;
-  for (int p = 0; p < k; ++p) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(blocks_for(k)), dim3(BS), 0, s, (const double*)A, k, p, row, col, bad);
-    hipLaunchKernelGGL(k_gj_update, dim3(blocks_for((long long)k * k)), dim3(BS), 0, s, A, k, p, (const double*)row,
-                       (const double*)col);
+  const int tiles = ((k + 63) / 64) * ((k + 63) / 64);
+  for (int p0 = 0; p0 < k; p0 += kGJ) {
+    const int nb = std::min(kGJ, k - p0);
+    hipLaunchKernelGGL(k_gj_panel, dim3(1), dim3(kGJThreads), 0, s, (const double*)A, k, p0, nb, pinv, bad);
+    hipLaunchKernelGGL(k_gj_rows, dim3(blocks_for(2LL * nb * k)), dim3(BS), 0, s, (const double*)A, k, p0, nb,
+                       (const double*)pinv, rp, cold);
+    hipLaunchKernelGGL(k_gj_update, dim3(tiles), dim3(BS), 0, s, A, k, p0, nb, (const double*)rp, (const double*)cold);
   }
   hipLaunchKernelGGL(k_block_embed, dim3(blocks_for((long long)m * m)), dim3(BS), 0, s, Gblock, m, skip,
                      (const double*)A);

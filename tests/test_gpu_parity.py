@@ -523,3 +523,29 @@ def test_zebra_cycle_on_a_masked_grid_matches_oracle(gpu_available):
     assert rel(x, xo) <= 1e-9, (info, io)
     with pytest.raises(cfdgpu.CfdError):
         g.set_options(cfdgpu.PRECISION_MIXED, cycle=cfdgpu.CYCLE_ZEBRA)
+
+
+@pytest.mark.parametrize("kind", ["anchored", "closed"])
+def test_device_inverted_block_is_the_exact_inverse(gpu_available, kind):
+    """With the dense limit above the cell count the fine level is the coarsest
+    level and one dense block per component solves it. Blocks above 128 nodes
+    are inverted on the device by blocked Gauss-Jordan (pivot blocks of 64,
+    here with a partial last block; a closed component's block centered), so
+    one preconditioner application is A^-1 b itself: the oracle's manufactured
+    field to rounding."""
+    if kind == "anchored":
+        case = Case(cells=[17, 13, 9], size_m=[17, 13, 9], boundaries=[OPEN, WALL, WALL], velocity_m_s=[1.0, 0, 0],
+                    blocks=[Block([5.0, 4, 0], [8.0, 9, 6])], preconditioner=MULTIGRID)
+    else:
+        case = W.cavity([13, 11, 9], MULTIGRID)
+    o = oracle.Grid(case)
+    ex = o.center_nullspaces(np.where(o.fluid(), np.random.default_rng(2).standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    with env(CFD_MG_DENSE=4096):
+        g = cfdgpu.Grid(case)
+    print(f"[inverse] {kind} cells={case.cells}", flush=True)
+    z = g.precondition(b)
+    assert rel(z, ex) <= 1e-10, rel(z, ex)
+    x, info = g.solve(b, 1e-12, 50)
+    assert info["iterations"] <= 2, info
+    assert rel(x, ex) <= 1e-10
