@@ -110,6 +110,7 @@ struct FineRB {
   const uint8_t* codes;
   T* b;
   T* x;
+  int two_cells;  // half-sweeps take two cells per thread (hx even; chosen at set-up)
 };
 
 // Coarsest level: one dense block per connected component. Blocks of at most

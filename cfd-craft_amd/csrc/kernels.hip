@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "device.hpp"
@@ -1784,6 +1785,32 @@ __global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restric
   }
 }
 
+// The Gauss-Seidel value of a cell from its code, right-hand side and the six
+// neighbour values xn (axis-major, lower side first; entries of uncoupled or
+// boundary faces are not read).
+template <class T>
+__device__ __forceinline__ T rb_relax(const Geo& g, uint8_t code, T bv, const int q[3], const T xn[6]) {
+  if (!(code & FLUID)) return T(0);
+  T diag = T(0);
+  T value = bv;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (!(code & (1u << (2 * a + side)))) continue;
+      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
+      if (boundary && !g.per[a]) {
+        diag += (T)g.wo[a];
+        continue;
+      }
+      const T w = (T)g.wi[a];
+      diag += w;
+      value += w * xn[2 * a + side];
+    }
+  }
+  return diag > T(0) ? value / diag : T(0);
+}
+
 // The new value of the colour-`colour` cell at q (half-row index i2; u its offset
 // in its colour's array): Gauss-Seidel from the other colour's values, with the
 // prolongation x += cs e[parent] riding on the neighbour reads when xc is set.
@@ -1809,25 +1836,7 @@ __device__ __forceinline__ T rb_update(const Geo& g, const FineRB<T>& F, int col
       }
       xn[2 * a + side] = v;
     }
-  if (!(code & FLUID)) return T(0);
-  T diag = T(0);
-  T value = bv;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      if (!(code & (1u << (2 * a + side)))) continue;
-      const bool boundary = side == 0 ? q[a] == 0 : q[a] == g.n[a] - 1;
-      if (boundary && !g.per[a]) {
-        diag += (T)g.wo[a];
-        continue;
-      }
-      const T w = (T)g.wi[a];
-      diag += w;
-      value += w * xn[2 * a + side];
-    }
-  }
-  return diag > T(0) ? value / diag : T(0);
+  return rb_relax<T>(g, code, bv, q, xn);
 }
 
 // One half-sweep of colour `colour` (rb_update per cell). out64: also store the
@@ -1857,6 +1866,84 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
   if (status != ST_RUNNING) return;
   F.x[colour * F.half + u] = v;
   if (out64) out64[(long long)i + j * g.sy + k * g.sz] = (double)v;
+}
+
+// One half-sweep with two cells of the colour per thread (entries i2 and i2 + 1
+// of a colour row, tiles of 128 x 4): the pair's right-hand sides, codes'
+// neighbours in y and z and its new values move as 16-byte (FP64) or 8-byte
+// (FP32) pairs. Needs hx even, so that every pair is aligned. Each cell's
+// arithmetic is rb_update's. With xc (the prolongation riding on the reads) each
+// cell takes rb_update itself.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, const T* __restrict__ xc,
+                                               const int* __restrict__ fparent, T cs, double* __restrict__ out64,
+                                               const PcgState* st) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned gx = (hx + 127) >> 7, gy = (ny + 3) >> 2;
+  const unsigned tile = (unsigned)xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned txy = tile % (gx * gy);
+  const int i2 = (int)((txy % gx) * 128 + 2 * (threadIdx.x & 63));
+  const int j = (int)((txy / gx) * 4 + (threadIdx.x >> 6));
+  const int k = (int)(tile / (gx * gy));
+  if (i2 >= (int)hx || j >= (int)ny) return;
+  const int par = (j + k + colour) & 1;
+  const int ia = 2 * i2 + par, ib = ia + 2;  // ib < nx: hx even makes nx even
+  const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
+  T va, vb;
+  if (xc) {
+    const int qa[3] = {ia, j, k}, qb[3] = {ib, j, k};
+    va = rb_update<T>(g, F, colour, qa, u, xc, fparent, cs);
+    vb = rb_update<T>(g, F, colour, qb, u + 1, xc, fparent, cs);
+  } else {
+    const T* __restrict__ xo = F.x + (1 - colour) * F.half;
+    const long long own = colour * F.half + u;
+    const T2 bv = *reinterpret_cast<const T2*>(F.b + own);
+    const uint8_t ca = F.codes[own], cbc = F.codes[own + 1];
+    // x neighbours: entries i2 - 1 + par .. i2 + 1 + par of the other colour's row
+    const unsigned row = hx * ((unsigned)j + ny * (unsigned)k);
+    const T2 mid = *reinterpret_cast<const T2*>(xo + row + (unsigned)i2);
+    int e = par ? i2 + 2 : i2 - 1;
+    if (e < 0) e = g.per[0] ? (int)hx - 1 : 0;
+    if (e >= (int)hx) e = g.per[0] ? 0 : (int)hx - 1;
+    const T edge = xo[row + (unsigned)e];
+    T yz[4][2];
+#pragma unroll
+    for (int a = 1; a < 3; ++a)
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int qa[3] = {ia, j, k};
+        int qn[3];
+        nb_coords(g, qa, a, side, qn);
+        const T2 v = *reinterpret_cast<const T2*>(xo + (unsigned)i2 + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2]));
+        yz[2 * (a - 1) + side][0] = v.x;
+        yz[2 * (a - 1) + side][1] = v.y;
+      }
+    T xa[6], xb[6];
+    xa[0] = par ? mid.x : edge;
+    xa[1] = par ? mid.y : mid.x;
+    xb[0] = par ? mid.y : mid.x;
+    xb[1] = par ? edge : mid.y;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      xa[2 + f] = yz[f][0];
+      xb[2 + f] = yz[f][1];
+    }
+    const int qa[3] = {ia, j, k}, qb[3] = {ib, j, k};
+    va = rb_relax<T>(g, ca, bv.x, qa, xa);
+    vb = rb_relax<T>(g, cbc, bv.y, qb, xb);
+  }
+  if (status != ST_RUNNING) return;
+  T2 out;
+  out.x = va;
+  out.y = vb;
+  *reinterpret_cast<T2*>(F.x + colour * F.half + u) = out;
+  if (out64) {
+    const long long c = (long long)ia + j * g.sy + k * g.sz;
+    out64[c] = (double)va;
+    out64[c + 2] = (double)vb;
+  }
 }
 
 // The last post-smoothing half-sweep (colour 0, z into out64) together with the
@@ -1898,6 +1985,18 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __r
 template <class T>
 unsigned rb_blocks(const Geo& g, const FineRB<T>& F) {
   return (unsigned)(((F.hx + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[2]);
+}
+
+// One fine half-sweep: two cells per thread when the grid chose so at set-up.
+template <class T>
+void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, const int* fparent, T cs,
+                    double* out64, const PcgState* st, hipStream_t s) {
+  if (F.two_cells && F.hx % 2 == 0)
+    hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
+                       F, colour, xc, fparent, cs, out64, st);
+  else
+    hipLaunchKernelGGL((k_rb_gs<T>), dim3(rb_blocks(g0, F)), dim3(BS), 0, s, g0, F, colour, xc, fparent, cs, out64,
+                       st);
 }
 
 // Fine residual of cell q from the colour-separated layout.
@@ -2658,12 +2757,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
-    const unsigned hb = rb_blocks(g0, *rb);
     hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
                        g0, codes, (const double*)b0, *rb, st);
     for (int pass = 1; pass < 2 * fr; ++pass)
-      hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr,
-                         st);
+      launch_rb_half<T>(g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr, st, s);
     hipLaunchKernelGGL((k_rb_restrict<T>),
                        dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)), dim3(BS), 0, s,
                        g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
@@ -2710,16 +2807,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // fine post-smoothing; in mixed precision (and always in the colour-separated
   // layout) the final half-sweeps also write z in FP64
   if (rb) {
-    const unsigned hb = rb_blocks(g0, *rb);
     for (int pass = 0; pass < 2 * fr; ++pass) {
       if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
         // the last half-sweep also forms the PCG's r . z partials
         hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         continue;
       }
-      hipLaunchKernelGGL((k_rb_gs<T>), dim3(hb), dim3(BS), 0, s, g0, *rb, (pass + 1) & 1,
-                         pass == 0 ? (const T*)lv[0].x : nox, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr,
-                         st);
+      launch_rb_half<T>(g0, *rb, (pass + 1) & 1, pass == 0 ? (const T*)lv[0].x : nox, fparent, cs,
+                        pass >= 2 * fr - 2 ? out64 : (double*)nullptr, st, s);
     }
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
@@ -2745,8 +2840,8 @@ template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {
-  hipLaunchKernelGGL((k_rb_gs<T>), dim3(rb_blocks(g0, rb)), dim3(BS), 0, s, g0, rb, colour, (const T*)nullptr,
-                     (const int*)nullptr, T(0), (double*)nullptr, (const PcgState*)nullptr);
+  launch_rb_half<T>(g0, rb, colour, (const T*)nullptr, (const int*)nullptr, T(0), (double*)nullptr,
+                    (const PcgState*)nullptr, s);
 }
 template void launch_rb_gs<double>(const Geo&, const FineRB<double>&, int, hipStream_t);
 template void launch_rb_gs<float>(const Geo&, const FineRB<float>&, int, hipStream_t);

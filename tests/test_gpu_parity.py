@@ -306,16 +306,17 @@ def test_odd_periodic_fine_level_colours(gpu_available, cells):
         assert rel(x, xo) <= 1e-10, (precision, info, io)
 
 
-LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130, 20, 9]),
-                ("cavity", [70, 18, 22]), ("cavity", [40, 2, 30]), ("cavity", [30, 1, 20])]
+LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130, 20, 9]), ("urban", [260, 22, 7]),
+                ("cavity", [70, 18, 22]), ("cavity", [40, 2, 30]), ("cavity", [30, 1, 20]), ("cavity", [36, 4, 12])]
 
 
 @pytest.mark.parametrize("kind,cells", LAYOUT_CASES, ids=[f"{k}{'x'.join(map(str, c))}" for k, c in LAYOUT_CASES])
 @pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
 def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, precision):
-    """The red-black cycle with the fine level stored colour by colour gives the
-    same bits as with the natural layout (same per-cell arithmetic), odd sizes
-    and periodic axes of 1 and 2 cells included."""
+    """The red-black cycle with the fine level stored colour by colour, with one
+    or (even colour rows) two cells per thread, gives the same bits as with the
+    natural layout (same per-cell arithmetic), odd sizes and periodic axes of 1
+    and 2 cells included."""
     if kind == "urban":
         case = W.urban_block(n=tuple(cells), boxes=6)[0]
     else:
@@ -324,13 +325,14 @@ def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, prec
     if kind == "cavity":
         r -= r.mean()  # one closed component
     out = []
-    for layout in (1, 0):
-        with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_DENSE=64):
+    for layout, pairs in ((1, 0), (1, 1), (0, 0)):
+        with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_RB_PAIRS=pairs, CFD_MG_DENSE=64):
             g = cfdgpu.Grid(case)
         g.set_options(precision)
-        print(f"[layout] {kind} {cells} precision={precision} layout={layout}", flush=True)
+        print(f"[layout] {kind} {cells} precision={precision} layout={layout} pairs={pairs}", flush=True)
         out.append(g.precondition(r))
-    assert np.array_equal(out[0], out[1]), float(np.max(np.abs(out[0] - out[1])))
+    for k in (1, 2):
+        assert np.array_equal(out[0], out[k]), (k, float(np.max(np.abs(out[0] - out[k]))))
 
 
 def test_split_aggregation_converges_on_prisms(gpu_available):
