@@ -1869,11 +1869,11 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
 }
 
 // One half-sweep with two cells of the colour per thread (entries i2 and i2 + 1
-// of a colour row, tiles of 128 x 4): the pair's right-hand sides, codes'
+// of a colour row, tiles of 128 x 4): the pair's right-hand sides, its
 // neighbours in y and z and its new values move as 16-byte (FP64) or 8-byte
 // (FP32) pairs. Needs hx even, so that every pair is aligned. Each cell's
 // arithmetic is rb_update's. With xc (the prolongation riding on the reads) each
-// cell takes rb_update itself.
+// cell takes rb_update itself; the V-cycle launches that pass one cell per thread.
 template <class T>
 __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, const T* __restrict__ xc,
                                                const int* __restrict__ fparent, T cs, double* __restrict__ out64,
@@ -1991,7 +1991,9 @@ unsigned rb_blocks(const Geo& g, const FineRB<T>& F) {
 template <class T>
 void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, const int* fparent, T cs,
                     double* out64, const PcgState* st, hipStream_t s) {
-  if (F.two_cells && F.hx % 2 == 0)
+  // the pass carrying the prolongation keeps one cell per thread (its neighbour
+  // reads gather the coarse correction cell by cell)
+  if (F.two_cells && F.hx % 2 == 0 && !xc)
     hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
                        F, colour, xc, fparent, cs, out64, st);
   else
