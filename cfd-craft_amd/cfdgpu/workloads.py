@@ -341,6 +341,50 @@ def urban_prisms(seed: int = 3, n=(256, 256, 128), h: float = 1.0, prisms: int =
     return _finish(case, rng)
 
 
+def street_canyon(direction_deg: float, seed: int = 5, n=(384, 384, 160), h: float = 1.0, lots: int = 20,
+                  street: float = 3.0, heights=(30, 155), margin=4, preconditioner=MULTIGRID):
+    """One direction of the street-canyon wind campaign: lots x lots building lots
+    (400 by default) between streets of `street` metres, each holding one prism
+    with a random convex 4-8-vertex footprint that nearly fills the lot and a
+    height of heights cells (about 25 % solid cells at 384x384x160); x and y open
+    with the log-law wind blowing from direction_deg, ground_open_top z. The same
+    seed gives the same city for every direction; the direction decides which
+    open faces are outlets, so each direction is its own operator."""
+    n = [int(v) for v in n]
+    case = Case(cells=n, size_m=[n[0] * h, n[1] * h, n[2] * h], boundaries=[OPEN, OPEN, GROUND_OPEN_TOP],
+                preconditioner=preconditioner, tolerance=1e-8, max_iterations=2000)
+    _neutral_rans(case, direction=direction_deg)
+    rng = np.random.default_rng(seed)
+    lx = (n[0] - 2 * margin) * h / lots
+    ly = (n[1] - 2 * margin) * h / lots
+    cxs = _centers(n[0], h)
+    cys = _centers(n[1], h)
+    for a in range(lots):
+        for b in range(lots):
+            cx = margin * h + (a + 0.5) * lx
+            cy = margin * h + (b + 0.5) * ly
+            hx, hy = 0.5 * (lx - street), 0.5 * (ly - street)
+            while True:
+                k = int(rng.integers(4, 9))
+                # one vertex per sector of 2 pi / k, on the lot's rectangle (less the
+                # street), pulled in by up to 15 %
+                ang = (rng.uniform(0, 2 * math.pi / k) + 2 * math.pi * (np.arange(k) + rng.uniform(0.15, 0.85, k)) / k)
+                rad = 1.0 / np.maximum(np.abs(np.cos(ang)) / hx, np.abs(np.sin(ang)) / hy)
+                rad = rad * rng.uniform(0.85, 1.0, k)
+                poly = [[cx + rad[i] * math.cos(ang[i]), cy + rad[i] * math.sin(ang[i])] for i in range(k)]
+                if not _valid_convex(poly):
+                    continue
+                xs = [p[0] for p in poly]
+                ys = [p[1] for p in poly]
+                X, Y = np.meshgrid(cxs[(cxs >= min(xs)) & (cxs < max(xs))], cys[(cys >= min(ys)) & (cys < max(ys))])
+                if X.size and _in_polygon(poly, X, Y).any():
+                    break
+            hz = min(int(rng.integers(heights[0], heights[1] + 1)), n[2] - 1)
+            case.blocks.append(Block(min_m=[min(xs), min(ys), 0.0], max_m=[max(xs), max(ys), hz * h], footprint=poly,
+                                     id=f"c{a}_{b}"))
+    return _finish(case, rng)
+
+
 def _finish(case: Case, rng):
     fluid = fluid_mask(case)
     kinds, zc, offs = face_kinds(case, fluid)

@@ -50,6 +50,7 @@ def cases():
         out.append(("canyon93k", Case.from_json(json.load(f))))
     out.append(("urban32", W.urban_block(n=(32, 32, 16), boxes=6)[0]))
     out.append(("prisms48", W.urban_prisms(n=(48, 48, 24), prisms=12, radius=(3.0, 6.0), heights=(3, 12))[0]))
+    out.append(("canyon-streets", W.street_canyon(200.0, n=(72, 72, 30), lots=6, street=2.0, heights=(5, 28))[0]))
     out.append(("barrier", Case(cells=[9, 2, 5], size_m=[9, 2, 5], boundaries=[OPEN, WALL, WALL], velocity_m_s=[1.0, 0, 0],
                                 blocks=[Block([3.0, 0, 0], [4.0, 2, 5])])))
     out.append(("rotated-wind", Case(cells=[20, 16, 8], size_m=[20, 16, 8], boundaries=[OPEN, OPEN, 4], flow_kind=1,
@@ -189,3 +190,23 @@ def test_case_json_keys():
     assert case.preconditioner == cfdgpu.IC0 and case.tolerance == 1e-10 and case.max_iterations == 1200
     c["numerics"]["pressure_preconditioner"] = "multigrid"
     assert Case.from_json(c).preconditioner == cfdgpu.MULTIGRID
+
+
+def test_street_canyon_city_is_the_same_for_every_direction():
+    """The campaign's city (lots x lots convex prisms between streets) depends on
+    the seed only; the wind direction changes the open faces' kinds and the
+    right-hand side."""
+    a, ba, _ = W.street_canyon(0.0, n=(96, 96, 40), lots=5, heights=(8, 36))
+    b, bb, _ = W.street_canyon(90.0, n=(96, 96, 40), lots=5, heights=(8, 36))
+    assert len(a.blocks) == len(b.blocks) == 25
+    for p, q in zip(a.blocks, b.blocks):
+        assert p.footprint == q.footprint and p.max_m == q.max_m
+        xs = [v[0] for v in p.footprint]
+        ys = [v[1] for v in p.footprint]
+        assert 4.0 <= min(xs) and max(xs) <= 92.0 and 4.0 <= min(ys) and max(ys) <= 92.0
+        assert 4 <= len(p.footprint) <= 8
+    fluid = W.fluid_mask(a)
+    assert np.array_equal(fluid, W.fluid_mask(b))
+    assert 0.1 < 1.0 - fluid.mean() < 0.4
+    assert not np.array_equal(ba, bb)
+
