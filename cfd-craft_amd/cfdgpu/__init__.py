@@ -166,6 +166,7 @@ def lib():
         sig = {
             "cfd_grid_create": (I, [P, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_grid_destroy": (None, [P]),
+            "cfd_grid_matches": (I, [P, P, P]),
             "cfd_grid_counts": (I, [P, P]),
             "cfd_grid_codes": (I, [P, P]),
             "cfd_grid_components": (I, [P, P, P]),
@@ -416,6 +417,15 @@ class Grid:
         _check(lib().cfd_center_nullspaces(self.h, _ptr(x)))
         return x
 
+    def matches(self, case: Case) -> bool:
+        """Signature::matches (multigrid.rs:158-222): whether `case` gives this
+        grid's operator exactly (counts, spacing and face-weight bits, fluid mask,
+        anchoring, interior and outlet faces, periodic axes, preconditioner)."""
+        c, keep = case.to_c()
+        same = ctypes.c_int(0)
+        _check(lib().cfd_grid_matches(self.h, ctypes.byref(c), ctypes.byref(same)))
+        return bool(same.value)
+
     def precondition(self, r) -> np.ndarray:
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.empty(self.count)
@@ -500,6 +510,24 @@ class Grid:
                                                None, ctypes.byref(info)),
             )
         return u, p, info
+
+
+class GridCache:
+    """The reference's per-grid preconditioner cache (pressure.rs:55-97): the
+    device context is reused while the case gives the same operator
+    (Grid.matches, the Signature check) and rebuilt when it changes; the first
+    solve after a rebuild reports hierarchy_builds = 1."""
+
+    def __init__(self):
+        self.grid = None
+
+    def get(self, case: Case) -> Grid:
+        if self.grid is None or not self.grid.matches(case):
+            if self.grid is not None:
+                self.grid.close()
+            self.grid = None
+            self.grid = Grid(case)
+        return self.grid
 
 
 class Server:

@@ -1289,6 +1289,32 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
 
 void cfd_grid_destroy(cfd_grid* g) { delete g; }
 
+int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same) {
+  if (!g || !c || !same) {
+    set_error("cfd_grid_matches: null argument");
+    return CFD_EINVAL;
+  }
+  *same = 0;
+  HostGrid h;
+  if (int rc = build_host_grid(*c, h)) return rc;
+  const HostGrid& o = g->host;
+  auto bits = [](double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    return u;
+  };
+  bool eq = h.preconditioner == o.preconditioner;
+  for (int a = 0; a < 3 && eq; ++a)
+    eq = h.n[a] == o.n[a] && h.periodic[a] == o.periodic[a] && bits(h.dx[a]) == bits(o.dx[a]) &&
+         bits(h.w_int[a]) == bits(o.w_int[a]) && bits(h.w_out[a]) == bits(o.w_out[a]);
+  // the codes carry every interior and outlet face (a coupled side) and, with the
+  // fluid bit and the closed bit, the fluid mask and the anchoring of each cell's
+  // component; components are numbered by their lowest cell in both builds
+  eq = eq && h.fluid == o.fluid && h.anchored == o.anchored && h.codes == o.codes;
+  *same = eq ? 1 : 0;
+  return CFD_OK;
+}
+
 static void host_counts(const HostGrid& h, int64_t counts[8]) {
   int64_t anchored = 0;
   for (uint8_t a : h.anchored) anchored += a;

@@ -175,6 +175,16 @@ typedef struct cfd_grid cfd_grid;
  * CFD-E028/E030 as CFD_EINVAL with the reference text. */
 int cfd_grid_create(const cfd_case* c, cfd_grid** out);
 void cfd_grid_destroy(cfd_grid* g);
+/* The multigrid cache's Signature::matches (multigrid.rs:158-222, checked before
+ * every multigrid solve at pressure.rs:69-92): *same = 1 when the case gives the
+ * grid's operator exactly (cell counts, spacing bits, fluid mask, anchored
+ * components, every interior and outlet face with its weight bits, periodic
+ * axes, preconditioner), else 0. A caller that keeps a context per grid and
+ * lets the grid change calls this before a solve and recreates the context on
+ * 0; the next solve then reports hierarchy_builds = 1, as the reference's does.
+ * Builds the case's topology on the host (O(cells)); errors of Grid::build are
+ * returned as by cfd_grid_create. */
+int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same);
 /* counts: cells, faces, fluid cells, components, anchored components, x/y/z face counts. */
 int cfd_grid_counts(const cfd_grid* g, int64_t counts[8]);
 /* Per cell coupling code: bit 2a = lower face on axis a couples (interior or

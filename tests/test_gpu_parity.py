@@ -551,3 +551,37 @@ def test_device_inverted_block_is_the_exact_inverse(gpu_available, kind):
     x, info = g.solve(b, 1e-12, 50)
     assert info["iterations"] <= 2, info
     assert rel(x, ex) <= 1e-10
+
+
+def test_grid_signature_matches_like_the_reference(gpu_available):
+    """Signature::matches (multigrid.rs:158-222): a context is reused for a case
+    with the same operator and rebuilt when faces, weights, the mask or the
+    preconditioner change (pressure.rs:69-92); the first solve after a rebuild
+    reports hierarchy_builds = 1 and the next one 0."""
+    import copy
+
+    case, b, t = W.wind_campaign(200.0, n=(32, 32, 16), boxes=4)
+    g = cfdgpu.Grid(case)
+    print("[signature] wind 200", flush=True)
+    assert g.matches(case)
+    assert g.matches(W.wind_campaign(200.0, n=(32, 32, 16), boxes=4)[0])
+    assert not g.matches(W.wind_campaign(90.0, n=(32, 32, 16), boxes=4)[0])  # other outlets
+    fewer = copy.deepcopy(case)
+    fewer.blocks = fewer.blocks[1:]
+    assert not g.matches(fewer)  # another fluid mask
+    stretched = copy.deepcopy(case)
+    stretched.size_m = [v * 1.5 for v in case.size_m]
+    assert not g.matches(stretched)  # same codes, other face weights
+    ic0 = copy.deepcopy(case)
+    ic0.preconditioner = IC0
+    assert not g.matches(ic0)
+    cache = cfdgpu.GridCache()
+    first = cache.get(case)
+    _, i1 = first.solve_scaled(b, t, 1e-8, 300)
+    assert cache.get(case) is first
+    _, i2 = first.solve_scaled(b, t, 1e-8, 300)
+    other, bo, to = W.wind_campaign(90.0, n=(32, 32, 16), boxes=4)
+    second = cache.get(other)
+    assert second is not first
+    _, i3 = second.solve_scaled(bo, to, 1e-8, 300)
+    assert (i1["hierarchy_builds"], i2["hierarchy_builds"], i3["hierarchy_builds"]) == (1, 0, 1)
