@@ -167,6 +167,73 @@ def cavity48(steps=5):
             "true_rel_residual": info["relative_residual"]}
 
 
+# The reference's published single-thread CPU times (ms, reused solve; AMD EPYC 9V74):
+# docs/evidence/pressure-multigrid.md:34-38 (manufactured kernels, tol 1e-11, IC(0) / MG)
+# and docs/evidence/native-cpu-performance.md:87 (canyon-93k IC(0) solve, median warm).
+PUBLISHED_CPU_MS = {
+    "cavity24": ([24, 2, 24], 1.292, 0.928),
+    "cavity48": ([48, 2, 48], 13.007, 3.646),
+    "cavity96": ([96, 2, 96], 102.410, 16.864),
+    "cube24": ([24, 24, 24], 19.565, 13.059),
+    "cube48": ([48, 48, 48], 390.576, 152.088),
+}
+CANYON93K_IC0_CPU_MS = 393.146
+
+
+def published_fixtures(reps=3):
+    """The fixtures whose reference CPU times are published, on the GPU with the
+    reference's protocol: the manufactured field sin(2 pi x/Lx) + 0.3 cos(4 pi z/Lz)
+    (centered), b = A x built on the device, tol 1e-11, one cold solve, then the
+    median of `reps` reused solves (inputs resident in HBM). IC(0), the reference's
+    multigrid cycle (its iteration counts) and the red-black cycle; canyon-93k
+    with IC(0) (its published 166 iterations)."""
+    import json as _json
+
+    import cfdgpu
+    from cfdgpu import workloads as W
+    from cfdgpu.case import IC0, MULTIGRID, Case
+
+    def timed(case, cycle=None):
+        g = cfdgpu.Grid(case)
+        if cycle is not None:
+            g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
+        ex = g.center_nullspaces(W.manufactured(case, W.fluid_mask(case)))
+        db = cfdgpu.DeviceArray.from_host(g.apply(ex))
+        dx = cfdgpu.DeviceArray(g.count)
+        g.solve_device(db, dx, 1e-11, 10000)  # cold
+        cfdgpu.synchronize()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            info = g.solve_device(db, dx, 1e-11, 10000)
+            cfdgpu.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        x = dx.numpy()
+        return {"gpu_ms": round(float(np.median(times)), 3), "iterations": info["iterations"],
+                "rms_error": float(np.sqrt(np.mean((x - ex) ** 2)))}
+
+    out = {}
+    for name, (cells, ic0_ms, mg_ms) in PUBLISHED_CPU_MS.items():
+        row = {}
+        r = timed(W.cavity(cells, IC0))
+        r.update(reference_cpu_ms=ic0_ms, speedup=round(ic0_ms / r["gpu_ms"], 1))
+        row["ic0"] = r
+        r = timed(W.cavity(cells, MULTIGRID), cfdgpu.CYCLE_REFERENCE)
+        r.update(reference_cpu_ms=mg_ms, speedup=round(mg_ms / r["gpu_ms"], 1))
+        row["mg_reference_cycle"] = r
+        r = timed(W.cavity(cells, MULTIGRID), cfdgpu.CYCLE_RB)
+        r.update(reference_cpu_ms=mg_ms, speedup=round(mg_ms / r["gpu_ms"], 1))
+        row["mg_rb_cycle"] = r
+        out[name] = row
+    with open(os.path.join(ROOT, "tests", "golden", "canyon-93k.json")) as f:
+        case = Case.from_json(_json.load(f))
+    case.preconditioner = IC0
+    r = timed(case)
+    r.update(reference_cpu_ms=CANYON93K_IC0_CPU_MS, speedup=round(CANYON93K_IC0_CPU_MS / r["gpu_ms"], 1))
+    out["canyon93k_ic0"] = r
+    return out
+
+
 def anisotropic(steps=3):
     """A flat solid-free lid cavity (96x16x96 cells over 1 x 1 x 0.1 m: z couplings
     100x stiffer) at rel-res 1e-10 with the reference's own multigrid cycle (z-line
@@ -491,6 +558,7 @@ def main():
     if not args.no_extra:
         extra["cavity_48_ic0"] = cavity48()
         extra["anisotropic_96x16x96"] = anisotropic()
+        extra["published_fixtures"] = published_fixtures()
         extra["wind_campaign"] = campaign()
     if not args.no_served:
         extra["served"] = served()

@@ -383,8 +383,14 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
 int build_agg(cfd_grid* g) {
   if (g->ash.built) return 0;
   const HostGrid& h = g->host;
-  int rc = build_agg_hierarchy(h, g->agg, g->agg_dense, kHostBlock);
+  int rc = build_agg_hierarchy(h, g->agg, g->agg_dense, kHostBlock, true);
   if (rc) return rc;
+  if (g->agg.fine_shift > 1) {
+    // the fine-level restrictions sum each 2x2x2 bin's residuals per aggregate
+    set_error("multigrid: no fluid cells of the grid share a 2x2x2 bin with a face neighbour; the red-black "
+              "cycle cannot aggregate the fine level (CFD_CYCLE_REFERENCE can)");
+    return CFD_EUNSUPPORTED;
+  }
   colour_agg_hierarchy(g->agg);
   AggHierarchy& H = g->agg;
   DevAggShared& A = g->ash;
@@ -1376,7 +1382,7 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
   if (rc) return rc;
   AggHierarchy H;
   if (cycle == CFD_CYCLE_RB)
-    rc = build_agg_hierarchy(h, H, kAggDense, kHostBlock);  // sizes only: no large block is inverted
+    rc = build_agg_hierarchy(h, H, kAggDense, kHostBlock, true);  // sizes only: no large block is inverted
   else
     rc = build_ref_hierarchy(h, H, cycle == CFD_CYCLE_ZEBRA);
   if (rc) return rc;

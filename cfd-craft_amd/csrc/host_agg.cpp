@@ -206,15 +206,15 @@ struct EdgeSums {
   }
 };
 
-// Groups the nodes of level L into aggregates: bins of coordinates >> shift,
-// split into connected groups, numbered by their lowest member.
-int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, int shift,
+// Groups the nodes of level L into aggregates: bins of coordinates >> shift[d]
+// per axis, split into connected groups, numbered by their lowest member.
+int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, const int shift[3],
                   const std::function<void(const std::function<void(int32_t, int32_t)>&)>& edges,
                   std::vector<int32_t>& parent, std::vector<int32_t>& coarse_coords) {
   UnionFind uf(n);
   auto same_bin = [&](int32_t i, int32_t j) {
     for (int d = 0; d < 3; ++d)
-      if ((coords[3 * (size_t)i + d] >> shift) != (coords[3 * (size_t)j + d] >> shift)) return false;
+      if ((coords[3 * (size_t)i + d] >> shift[d]) != (coords[3 * (size_t)j + d] >> shift[d])) return false;
     return true;
   };
   edges([&](int32_t i, int32_t j) {
@@ -228,7 +228,7 @@ int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, int shift,
     const int32_t r = uf.find((int32_t)i);
     if (id_of_root[(size_t)r] < 0) {
       id_of_root[(size_t)r] = (int32_t)m++;
-      for (int d = 0; d < 3; ++d) coarse_coords.push_back(coords[3 * (size_t)i + d] >> shift);
+      for (int d = 0; d < 3; ++d) coarse_coords.push_back(coords[3 * (size_t)i + d] >> shift[d]);
     }
     parent[(size_t)i] = id_of_root[(size_t)r];
   }
@@ -310,7 +310,8 @@ int coarsest_block(const AggLevel& L, const std::vector<int32_t>& members, bool 
 
 }  // namespace
 
-int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells, int64_t host_block_max) {
+int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells, int64_t host_block_max,
+                        bool semicoarsen) {
   h = AggHierarchy();
   if (g.fluid_count == 0) {
     set_error("CFD-E034: numerics.pressure_preconditioner: multigrid requires fluid cells");
@@ -346,6 +347,9 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     L0.diag[(size_t)L0.eb[e]] += L0.ew[e];
   }
   h.levels.push_back(std::move(L0));
+  // coupling strength per axis of the current level's matrix: a coarse face sums
+  // the fine faces of its bin, so w_a grows by the bin's extent on the other axes
+  double w[3] = {g.w_int[0], g.w_int[1], g.w_int[2]};
   for (;;) {
     AggLevel& L = h.levels.back();
     const bool fine = h.levels.size() == 1;
@@ -401,23 +405,52 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
       }
       break;
     }
-    // Aggregate; grow the bins while nothing reduces (masked.rs:262-273).
+    // Aggregate; grow the bins while nothing reduces (masked.rs:262-273). With
+    // semicoarsening an axis is binned only while its coupling is within
+    // kSemiRatio of the strongest axis that still spans more than one node, so
+    // weakly coupled axes (thin or stretched cells) keep their nodes apart.
+    bool coarsen[3] = {true, true, true};
+    if (semicoarsen) {
+      int32_t ext[3] = {0, 0, 0};
+      for (int64_t i = 0; i < L.n; ++i)
+        for (int d = 0; d < 3; ++d) ext[d] = std::max(ext[d], L.coords[3 * (size_t)i + d] + 1);
+      double wmax = 0.0;
+      for (int d = 0; d < 3; ++d)
+        if (ext[d] > 1) wmax = std::max(wmax, w[d]);
+      for (int d = 0; d < 3; ++d) coarsen[d] = ext[d] > 1 && w[d] * kSemiRatio >= wmax;
+      if (!coarsen[0] && !coarsen[1] && !coarsen[2]) coarsen[0] = coarsen[1] = coarsen[2] = true;
+    }
     std::vector<int32_t> parent, cc;
     int64_t m = 0;
-    int shift = 1;
+    int s = 1;
+    int shift[3];
     const auto edges = [&](const std::function<void(int32_t, int32_t)>& f) {
       for (size_t e = 0; e < L.ea.size(); ++e) f(L.ea[e], L.eb[e]);
     };
     for (;;) {
+      for (int d = 0; d < 3; ++d) shift[d] = coarsen[d] ? s : 0;
       m = aggregate(L.n, L.coords, shift, edges, parent, cc);
       if (m < L.n) break;
       bool all_zero = true;
-      for (int32_t v : L.coords) all_zero &= (v >> shift) == 0;
+      for (size_t e = 0; e < L.coords.size(); ++e) all_zero &= (L.coords[e] >> shift[e % 3]) == 0;
       if (all_zero) {
+        if (!(coarsen[0] && coarsen[1] && coarsen[2])) {
+          // the binned axes are exhausted: bin every axis
+          coarsen[0] = coarsen[1] = coarsen[2] = true;
+          s = 1;
+          continue;
+        }
         set_error("CFD-E035: numerics.pressure_preconditioner: masked aggregation failed to reduce a connected component");
         return CFD_E035;
       }
-      ++shift;
+      ++s;
+    }
+    if (fine) h.fine_shift = std::max(shift[0], std::max(shift[1], shift[2]));
+    {
+      double f[3], wn[3];
+      for (int d = 0; d < 3; ++d) f[d] = (double)(1 << shift[d]);
+      for (int d = 0; d < 3; ++d) wn[d] = w[d] * f[(d + 1) % 3] * f[(d + 2) % 3];
+      for (int d = 0; d < 3; ++d) w[d] = wn[d];
     }
     AggLevel C;
     C.n = m;

@@ -128,6 +128,7 @@ struct AggHierarchy {
   std::vector<int32_t> node_of_cell;  // level-0 node of a cell, -1 for solid
   std::vector<int64_t> cell_of_node;
   int64_t level0_entries = 0;  // off-diagonal entries of level 0 (two per interior face)
+  int fine_shift = 1;          // largest bin shift of the first aggregation (1: bins within 2x2x2)
   std::vector<AggLevel> levels;
   // coarsest level: per component, its members and a dense m x m block
   std::vector<int64_t> block_ptr, block_goff;
@@ -149,8 +150,11 @@ struct AggHierarchy {
 void agg_hierarchy_shape(const AggHierarchy& h, cfd_mg_stats* s);
 // Coarsening stops when every component has at most coarse_cells nodes (the
 // reference: 32); blocks above host_block_max nodes are left to the device.
+// semicoarsen: bin an axis only while its coupling is within kSemiRatio of the
+// strongest axis (the red-black cycle; the reference bins every axis).
+constexpr double kSemiRatio = 25.0;
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
-                        int64_t host_block_max = INT64_MAX);
+                        int64_t host_block_max = INT64_MAX, bool semicoarsen = false);
 // The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked
 // aggregation when the grid has solids or several components, else trilinear
 // interpolation with Galerkin coarse operators (with line or small-plane block
