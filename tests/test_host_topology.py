@@ -210,3 +210,22 @@ def test_street_canyon_city_is_the_same_for_every_direction():
     assert 0.1 < 1.0 - fluid.mean() < 0.4
     assert not np.array_equal(ba, bb)
 
+
+
+def test_red_black_hierarchy_semicoarsens_weak_axes():
+    """The red-black cycle's aggregation bins an axis only while its coupling is
+    within 25x of the strongest axis that still spans several nodes: a lid cavity
+    of 2 or 4 cells over the unit y extent keeps its y layers apart (its y
+    couplings are 2300x weaker), a 100:1 stiff z axis is binned alone first;
+    cubic cells bin 2x2x2 as the reference's masked aggregation does."""
+    from cfdgpu.case import MULTIGRID, PERIODIC
+
+    rb = lambda case: cfdgpu.case_multigrid_stats(case, cfdgpu.CYCLE_RB)["level_cells"]
+    assert rb(W.cavity([48, 2, 48], MULTIGRID)) == [4608, 1152]
+    assert rb(W.cavity([96, 4, 96], MULTIGRID)) == [36864, 9216, 2304, 576]
+    assert rb(W.cavity([48, 48, 48], MULTIGRID)) == [110592, 13824, 1728]
+    aniso = Case(cells=[96, 16, 96], size_m=[1, 1, 0.1], boundaries=[WALL, PERIODIC, WALL], preconditioner=MULTIGRID)
+    assert rb(aniso)[:3] == [147456, 73728, 36864]
+    # the reference cycle keeps the reference's hierarchy
+    assert cfdgpu.case_multigrid_stats(W.cavity([48, 2, 48], MULTIGRID), cfdgpu.CYCLE_REFERENCE)["level_cells"] == \
+        [4608, 1152, 288, 72, 18]
