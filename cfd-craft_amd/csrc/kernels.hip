@@ -589,22 +589,23 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 // entries, right-hand side and the neighbours outside the tile are loaded into
 // LDS first, so a step costs one LDS round trip. Each cell repeats
 // k_ic0_fwd / k_ic0_bwd's operations in the same order (bitwise equal).
-constexpr int IT = 8;
-constexpr int IT3 = IT * IT * IT;
-template <bool FWD>
-__global__ void __launch_bounds__(IT3) k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
-                                                  const double* __restrict__ r, double* __restrict__ y, int P, int J0,
-                                                  int K0, int nJ, int nK, int ntx, const PcgState* st) {
-  __shared__ double ys[IT3];
+// Tiles of TX x TY x TZ cells (TX TY TZ = 512 threads, one per cell): 8 x 8 x 8,
+// or 16 x 2 x 16 (and its rotations) on grids with an axis of at most 2 cells.
+template <bool FWD, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX * TY * TZ)
+    k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D, const double* __restrict__ r,
+               double* __restrict__ y, int P, int J0, int K0, int nJ, int nK, int ntx, const PcgState* st) {
+  constexpr int NT = TX * TY * TZ;
+  __shared__ double ys[NT];
   if (st && st->status != ST_RUNNING) return;
   const int J = J0 + (int)blockIdx.x % nJ, K = K0 + (int)blockIdx.x / nJ;
   const int I = P - J - K;
   if (I < 0 || I >= ntx) return;
-  const int ox = I * IT, oy = J * IT, oz = K * IT;
+  const int ox = I * TX, oy = J * TY, oz = K * TZ;
   // one thread per tile cell: its factor entries, right-hand side and outside
   // neighbours are loaded by all threads at once, before the steps
   const int li = threadIdx.x;
-  const int a = li & 7, b = (li >> 3) & 7, cz = li >> 6;
+  const int a = li % TX, b = (li / TX) % TY, cz = li / (TX * TY);
   const int i = ox + a, j = oy + b, k = oz + cz;
   const bool live = i < g.n[0] && j < g.n[1] && k < g.n[2];
   double co[6], ext[6], rv = 0.0, dc = 1.0;
@@ -626,8 +627,8 @@ __global__ void __launch_bounds__(IT3) k_ic0_tile(Geo g, const uint8_t* __restri
       co[m] = FWD ? N.a[m] / D[nb] : N.a[m] / dc;
       const unsigned ni = nb % (unsigned)g.n[0], nj = (nb / (unsigned)g.n[0]) % (unsigned)g.n[1];
       const unsigned nk = nb / (unsigned)g.sz;
-      if ((int)ni >= ox && (int)ni < ox + IT && (int)nj >= oy && (int)nj < oy + IT && (int)nk >= oz && (int)nk < oz + IT) {
-        lix[m] = (int)(ni - ox) + IT * ((int)(nj - oy) + IT * (int)(nk - oz));
+      if ((int)ni >= ox && (int)ni < ox + TX && (int)nj >= oy && (int)nj < oy + TY && (int)nk >= oz && (int)nk < oz + TZ) {
+        lix[m] = (int)(ni - ox) + TX * ((int)(nj - oy) + TY * (int)(nk - oz));
       } else {
         lix[m] = -1;
         ext[m] = y[nb];
@@ -635,7 +636,9 @@ __global__ void __launch_bounds__(IT3) k_ic0_tile(Geo g, const uint8_t* __restri
     }
   }
   const int plane = a + b + cz;
-  constexpr int S = 3 * (IT - 1);
+  // the tile's last local hyperplane holding a cell of the grid (partial tiles
+  // at the grid's far faces walk fewer steps)
+  const int S = min(TX, g.n[0] - ox) + min(TY, g.n[1] - oy) + min(TZ, g.n[2] - oz) - 3;
   for (int step = 0; step <= S; ++step) {
     const int sp = FWD ? step : S - step;
     if (live && plane == sp) {
@@ -2553,26 +2556,42 @@ static void level_box(const Geo& g, int L, int& j0, int& k0, int& nj, int& nk) {
   nk = khi >= klo ? khi - klo + 1 : 0;
 }
 
-void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
-                      const PcgState* st, hipStream_t s) {
+template <int TX, int TY, int TZ>
+void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
+                             const PcgState* st, hipStream_t s) {
+  constexpr int NT = TX * TY * TZ;
   Geo tg{};
-  tg.n[0] = (g.n[0] + IT - 1) / IT;
-  tg.n[1] = (g.n[1] + IT - 1) / IT;
-  tg.n[2] = (g.n[2] + IT - 1) / IT;
+  tg.n[0] = (g.n[0] + TX - 1) / TX;
+  tg.n[1] = (g.n[1] + TY - 1) / TY;
+  tg.n[2] = (g.n[2] + TZ - 1) / TZ;
   const int diagonals = tg.n[0] + tg.n[1] + tg.n[2] - 2;
   for (int P = 0; P < diagonals; ++P) {
     int j0, k0, nj, nk;
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
-    hipLaunchKernelGGL(k_ic0_tile<true>, dim3(nj * nk), dim3(IT3), 0, s, g, codes, D, r, z, P, j0, k0, nj, nk, tg.n[0], st);
+    hipLaunchKernelGGL((k_ic0_tile<true, TX, TY, TZ>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, r, z, P, j0, k0, nj,
+                       nk, tg.n[0], st);
   }
   for (int P = diagonals - 1; P >= 0; --P) {
     int j0, k0, nj, nk;
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
-    hipLaunchKernelGGL(k_ic0_tile<false>, dim3(nj * nk), dim3(IT3), 0, s, g, codes, D, (const double*)z, z, P, j0, k0,
-                       nj, nk, tg.n[0], st);
+    hipLaunchKernelGGL((k_ic0_tile<false, TX, TY, TZ>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, (const double*)z, z,
+                       P, j0, k0, nj, nk, tg.n[0], st);
   }
+}
+
+void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
+                      const PcgState* st, hipStream_t s) {
+  // an axis of at most 2 cells: flat 16 x 16 tiles across the other two axes
+  if (g.n[1] <= 2)
+    launch_ic0_tiles_shaped<16, 2, 16>(g, codes, D, r, z, st, s);
+  else if (g.n[2] <= 2)
+    launch_ic0_tiles_shaped<16, 16, 2>(g, codes, D, r, z, st, s);
+  else if (g.n[0] <= 2)
+    launch_ic0_tiles_shaped<2, 16, 16>(g, codes, D, r, z, st, s);
+  else
+    launch_ic0_tiles_shaped<8, 8, 8>(g, codes, D, r, z, st, s);
 }
 
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,
