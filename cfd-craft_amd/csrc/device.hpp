@@ -110,7 +110,8 @@ struct FineRB {
   const uint8_t* codes;
   T* b;
   T* x;
-  int two_cells;  // half-sweeps take two cells per thread (hx even; chosen at set-up)
+  int two_cells;     // half-sweeps take two cells per thread (hx even; chosen at set-up)
+  const int* rpar;   // per red cell (red layout): node of the first coarse level, -1 without one
 };
 
 // Coarsest level: one dense block per connected component. Blocks of at most

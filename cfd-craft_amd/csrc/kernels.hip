@@ -1949,6 +1949,24 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
   }
 }
 
+// The coarse correction of the red cells, x += cs e[parent] (the value that the
+// black half-sweep after the prolongation would otherwise form at each of its
+// neighbour reads, with the same arithmetic): the red cells are recomputed from
+// scratch by the next red half-sweep, so only that black half-sweep reads these
+// values.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
+                                                   const T* __restrict__ xc, T cs, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const long long u = (long long)blockIdx.x * BS + threadIdx.x;
+  if (u >= half) return;
+  const int p = rpar[u];
+  if (p < 0 || status != ST_RUNNING) return;
+  T v = xr[u];
+  v += cs * xc[p];
+  xr[u] = v;
+}
+
 // The last post-smoothing half-sweep (colour 0, z into out64) together with the
 // PCG's r . z (pressure.rs:428) for a grid without closed components, whose z
 // needs no centering: each thread adds its red cell and the black cell beside
@@ -2834,8 +2852,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         continue;
       }
-      launch_rb_half<T>(g0, *rb, (pass + 1) & 1, pass == 0 ? (const T*)lv[0].x : nox, fparent, cs,
-                        pass >= 2 * fr - 2 ? out64 : (double*)nullptr, st, s);
+      const T* xcp = pass == 0 ? (const T*)lv[0].x : nox;
+      if (xcp && rb->rpar) {
+        // the correction added to the red cells first, then a plain half-sweep
+        hipLaunchKernelGGL((k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0, s, rb->half, rb->x, rb->rpar,
+                           xcp, cs, st);
+        xcp = nox;
+      }
+      launch_rb_half<T>(g0, *rb, (pass + 1) & 1, xcp, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr, st, s);
     }
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
