@@ -1967,12 +1967,13 @@ __global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict
   xr[u] = v;
 }
 
-// The last post-smoothing half-sweep (colour 0, z into out64) together with the
-// PCG's r . z (pressure.rs:428) for a grid without closed components, whose z
-// needs no centering: each thread adds its red cell and the black cell beside
-// it (final since the previous half-sweep), and the fixed reduction grid
-// (RP blocks of RB threads, tiles of 64 x 16 cells taken in turn) writes the
-// partials that k_pcg_rz would.
+// The last post-smoothing half-sweep (colour 0) together with the PCG's r . z
+// (pressure.rs:428) for a grid without closed components, whose z needs no
+// centering: each thread relaxes its red cell and takes the black cell beside it
+// (final since the previous half-sweep), writes both into z (out64, natural
+// layout: each wave fills whole cache lines) and adds both to r . z; the fixed
+// reduction grid (RP blocks of RB threads, tiles of 64 x 16 cells taken in turn)
+// writes the partials that k_pcg_rz would.
 template <class T>
 __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                  const double* __restrict__ r, const PcgState* st, double* part) {
@@ -1996,7 +1997,11 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __r
       out64[row + ir] = (double)v;
       acc += r[row + ir] * (double)v;
     }
-    if (ib < (int)nx) acc += r[row + ib] * (double)F.x[F.half + u];
+    if (ib < (int)nx) {
+      const T vb = F.x[F.half + u];
+      out64[row + ib] = (double)vb;
+      acc += r[row + ib] * (double)vb;
+    }
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -2859,7 +2864,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
                            xcp, cs, st);
         xcp = nox;
       }
-      launch_rb_half<T>(g0, *rb, (pass + 1) & 1, xcp, fparent, cs, pass >= 2 * fr - 2 ? out64 : (double*)nullptr, st, s);
+      // z in the natural layout from the last two half-sweeps (the r . z kernel
+      // writes both colours itself)
+      const bool z_out = pass == 2 * fr - 1 || (pass == 2 * fr - 2 && !(rz_r && fr >= 1));
+      launch_rb_half<T>(g0, *rb, (pass + 1) & 1, xcp, fparent, cs, z_out ? out64 : (double*)nullptr, st, s);
     }
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
