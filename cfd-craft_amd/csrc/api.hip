@@ -536,12 +536,13 @@ int build_agg(cfd_grid* g) {
     HIP_TRY(hipMemset(A.rb_x64.p, 0, (size_t)(2 * half) * sizeof(double)));
     HIP_TRY(hipMemset(A.rb_b32.p, 0, (size_t)(2 * half) * sizeof(float)));
     HIP_TRY(hipMemset(A.rb_x32.p, 0, (size_t)(2 * half) * sizeof(float)));
-    // two cells per thread when a colour holds at least 3 * 2^18 cells: below that
+    // two cells per thread (nx a multiple of 4: every pair of a colour row is
+    // aligned and complete) when a colour holds at least 3 * 2^18 cells: below that
     // one cell per thread fills the GPU in one round of workgroups and is faster
     // (1M cells: 4.9 against 6.4 us per half-sweep); above it the pairs keep more
     // bytes in flight (2M cells: 10.5 against 8.8 us, 3.5M: 14.2 against 11.4,
     // 8.4M: 23.0 against 18.2). CFD_MG_RB_PAIRS=0 / 1 forces.
-    const int two = hx % 2 == 0 && (g->rb_pairs == 1 || (g->rb_pairs < 0 && half >= 3 * (1LL << 18)));
+    const int two = h.n[0] % 4 == 0 && (g->rb_pairs == 1 || (g->rb_pairs < 0 && half >= 3 * (1LL << 18)));
     A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p, two, A.rb_rpar.p};
     A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p, two, A.rb_rpar.p};
     A.has_rb = true;

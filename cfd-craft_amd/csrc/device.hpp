@@ -110,7 +110,7 @@ struct FineRB {
   const uint8_t* codes;
   T* b;
   T* x;
-  int two_cells;     // half-sweeps take two cells per thread (hx even; chosen at set-up)
+  int two_cells;     // half-sweeps take two cells per thread (nx a multiple of 4; chosen at set-up)
   const int* rpar;   // per red cell (red layout): node of the first coarse level, -1 without one
 };
 

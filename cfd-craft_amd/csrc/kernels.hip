@@ -1874,7 +1874,8 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
 // One half-sweep with two cells of the colour per thread (entries i2 and i2 + 1
 // of a colour row, tiles of 128 x 4): the pair's right-hand sides, its
 // neighbours in y and z and its new values move as 16-byte (FP64) or 8-byte
-// (FP32) pairs. Needs hx even, so that every pair is aligned. Each cell's
+// (FP32) pairs. Needs nx a multiple of 4, so that every pair is aligned and
+// complete. Each cell's
 // arithmetic is rb_update's. With xc (the prolongation riding on the reads) each
 // cell takes rb_update itself; the V-cycle launches that pass one cell per thread.
 template <class T>
@@ -1892,7 +1893,7 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
   const int k = (int)(tile / (gx * gy));
   if (i2 >= (int)hx || j >= (int)ny) return;
   const int par = (j + k + colour) & 1;
-  const int ia = 2 * i2 + par, ib = ia + 2;  // ib < nx: hx even makes nx even
+  const int ia = 2 * i2 + par, ib = ia + 2;  // ib < nx: nx is a multiple of 4
   const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
   T va, vb;
   if (xc) {
@@ -2007,6 +2008,65 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __r
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
 }
 
+// k_rb_gs_rz with two red cells per thread (entries i2 and i2 + 1 of a colour
+// row, tiles of 128 x 16 taken in turn by the fixed reduction grid; needs nx a
+// multiple of 4): the four natural cells 2 i2 .. 2 i2 + 3 (red and black alternating)
+// have r read and z written as two 16-byte pairs. Each red cell's arithmetic is
+// rb_update's; r . z adds, per thread, red a, black a, red b, black b.
+template <class T>
+__global__ void __launch_bounds__(RB) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
+                                                  const double* __restrict__ r, const PcgState* st, double* part) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  if (!block_running(st)) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned gx = (hx + 127) >> 7, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
+  double acc = 0.0;
+  for (unsigned tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const unsigned txy = tile % (gx * gy);
+    const int i2 = (int)((txy % gx) * 128 + 2 * (threadIdx.x & 63));
+    const int j = (int)((txy / gx) * 16 + (threadIdx.x >> 6));
+    const int k = (int)(tile / (gx * gy));
+    if (i2 >= (int)hx || j >= (int)ny) continue;
+    const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
+    const long long row = (long long)j * g.sy + k * g.sz;
+    const int par = (j + k) & 1;  // red at 2 i2 + par and 2 i2 + 2 + par
+    const int qa[3] = {2 * i2 + par, j, k}, qb[3] = {2 * i2 + 2 + par, j, k};
+    const T va = rb_update<T>(g, F, 0, qa, u, (const T*)nullptr, (const int*)nullptr, T(0));
+    const T vb = rb_update<T>(g, F, 0, qb, u + 1, (const T*)nullptr, (const int*)nullptr, T(0));
+    T2 red;
+    red.x = va;
+    red.y = vb;
+    *reinterpret_cast<T2*>(F.x + u) = red;
+    const T2 black = *reinterpret_cast<const T2*>(F.x + F.half + u);
+    const double2 r01 = *reinterpret_cast<const double2*>(r + row + 2 * i2);
+    const double2 r23 = *reinterpret_cast<const double2*>(r + row + 2 * i2 + 2);
+    double2 z01, z23;
+    if (par == 0) {
+      z01.x = (double)va;
+      z01.y = (double)black.x;
+      z23.x = (double)vb;
+      z23.y = (double)black.y;
+      acc += r01.x * z01.x;
+      acc += r01.y * z01.y;
+      acc += r23.x * z23.x;
+      acc += r23.y * z23.y;
+    } else {
+      z01.x = (double)black.x;
+      z01.y = (double)va;
+      z23.x = (double)black.y;
+      z23.y = (double)vb;
+      acc += r01.y * z01.y;
+      acc += r01.x * z01.x;
+      acc += r23.y * z23.y;
+      acc += r23.x * z23.x;
+    }
+    *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
+    *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
+}
+
 // Workgroups of a half-sweep: tiles of 64 x 4 cells of a colour (x, y) per z layer.
 template <class T>
 unsigned rb_blocks(const Geo& g, const FineRB<T>& F) {
@@ -2019,7 +2079,7 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
                     double* out64, const PcgState* st, hipStream_t s) {
   // the pass carrying the prolongation keeps one cell per thread (its neighbour
   // reads gather the coarse correction cell by cell)
-  if (F.two_cells && F.hx % 2 == 0 && !xc)
+  if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
     hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
                        F, colour, xc, fparent, cs, out64, st);
   else
@@ -2854,7 +2914,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     for (int pass = 0; pass < 2 * fr; ++pass) {
       if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
         // the last half-sweep also forms the PCG's r . z partials
-        hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
+        if (rb->two_cells && g0.n[0] % 4 == 0)
+          hipLaunchKernelGGL((k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
+        else
+          hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         continue;
       }
       const T* xcp = pass == 0 ? (const T*)lv[0].x : nox;

@@ -307,6 +307,7 @@ def test_odd_periodic_fine_level_colours(gpu_available, cells):
 
 
 LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130, 20, 9]), ("urban", [260, 22, 7]),
+                ("urban", [63, 20, 9]),
                 ("cavity", [70, 18, 22]), ("cavity", [40, 2, 30]), ("cavity", [30, 1, 20]), ("cavity", [36, 4, 12])]
 
 
@@ -314,7 +315,7 @@ LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130
 @pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
 def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, precision):
     """The red-black cycle with the fine level stored colour by colour, with one
-    or (even colour rows) two cells per thread, gives the same bits as with the
+    or (nx a multiple of 4) two cells per thread, gives the same bits as with the
     natural layout (same per-cell arithmetic), odd sizes and periodic axes of 1
     and 2 cells included."""
     if kind == "urban":
@@ -605,3 +606,24 @@ def test_red_black_cycle_on_stretched_cells(gpu_available, cells, size, limit):
     assert info["iterations"] <= limit, info
     assert info["relative_residual"] <= 1.01e-10
     assert rel(x, ex) <= 1e-8
+
+
+@pytest.mark.parametrize("cells", [[64, 48, 40], [63, 20, 9], [128, 40, 12]])
+@pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
+def test_two_cell_kernels_solve_like_one_cell(gpu_available, cells, precision):
+    """Full solves with the fine half-sweeps and the final r . z half-sweep taking
+    two cells per thread (forced; nx a multiple of 4, else one cell) take the
+    iterations of the one-cell kernels and give the same field to rounding (only
+    the association of the r . z partials differs)."""
+    case, b, t = W.urban_block(n=tuple(cells), boxes=4)
+    res = []
+    for pairs in (1, 0):
+        with env(CFD_MG_RB_PAIRS=pairs):
+            g = cfdgpu.Grid(case)
+        g.set_options(precision)
+        print(f"[pairs] {cells} precision={precision} pairs={pairs}", flush=True)
+        res.append(g.solve_scaled(b, t, 1e-10, 500))
+    (x1, i1), (x0, i0) = res
+    assert i1["iterations"] == i0["iterations"], (i1, i0)
+    assert i1["relative_residual"] <= 1e-10 and i0["relative_residual"] <= 1e-10
+    assert rel(x1, x0) <= 1e-12
