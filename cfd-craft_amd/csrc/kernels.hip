@@ -1788,6 +1788,36 @@ __global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restric
   }
 }
 
+// k_rb_first with one thread per pair of natural cells (2 i2, 2 i2 + 1 of a row:
+// one red, one black; nx even): r and the codes are read as pairs and each
+// colour array is written by whole wavefronts (tiles of 64 pairs x 4 rows).
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first2(Geo g, const uint8_t* __restrict__ codes,
+                                                  const double* __restrict__ r, FineRB<T> F, const PcgState* st) {
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned gx = (hx + 63) >> 6, gy = (ny + 3) >> 2;
+  const unsigned tile = blockIdx.x, txy = tile % (gx * gy);
+  const int i2 = (int)((txy % gx) * 64 + (threadIdx.x & 63));
+  const int j = (int)((txy / gx) * 4 + (threadIdx.x >> 6));
+  const int k = (int)(tile / (gx * gy));
+  if (i2 >= (int)hx || j >= (int)ny) return;
+  const unsigned c = 2u * (unsigned)i2 + (unsigned)g.sy * (unsigned)j + (unsigned)g.sz * (unsigned)k;
+  const double2 rv = *reinterpret_cast<const double2*>(r + c);
+  const uint8_t c0 = codes[c], c1 = codes[c + 1];
+  const int par = (j + k) & 1;  // the red cell of the pair is 2 i2 + par
+  const uint8_t code = par ? c1 : c0;
+  const T bred = (T)(par ? rv.y : rv.x), bblack = (T)(par ? rv.x : rv.y);
+  const int q[3] = {2 * i2 + par, j, k};
+  const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
+  const T d = fine_diag<T>(g, code, q);
+  const T v = (code & FLUID) && d > T(0) ? bred / d : T(0);
+  if (status != ST_RUNNING) return;
+  F.b[u] = bred;
+  F.b[F.half + u] = bblack;
+  F.x[u] = v;
+}
+
 // The Gauss-Seidel value of a cell from its code, right-hand side and the six
 // neighbour values xn (axis-major, lower side first; entries of uncoupled or
 // boundary faces are not read).
@@ -2861,8 +2891,13 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const long long bins = (long long)((g0.n[0] + 1) >> 1) * ((g0.n[1] + 1) >> 1) * ((g0.n[2] + 1) >> 1);
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
-    hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
-                       g0, codes, (const double*)b0, *rb, st);
+    // pairs of natural cells when every row starts a pair and r allows 16-byte loads
+    if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
+      hipLaunchKernelGGL((k_rb_first2<T>), dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
+                         g0, codes, (const double*)b0, *rb, st);
+    else
+      hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
+                         g0, codes, (const double*)b0, *rb, st);
     for (int pass = 1; pass < 2 * fr; ++pass)
       launch_rb_half<T>(g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr, st, s);
     hipLaunchKernelGGL((k_rb_restrict<T>),
