@@ -2155,49 +2155,52 @@ __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const
   return bv - (ax * xv - off);
 }
 
-// Restriction from the colour-separated fine level: as k_agg_fine_restrict.
+// Restriction from the colour-separated fine level, as k_agg_fine_restrict: a
+// workgroup takes one row of 64 2x2x2 bins and wavefront w the cells at
+// (dy, dz) = (w & 1, w >> 1) of every bin (the bin's red cell there and the black
+// one beside it), so its loads are as coalesced as a half-sweep's. A bin at even
+// coordinates holds its red cells at subs 0, 3, 5, 6 and its black ones at 1, 2,
+// 4, 7 (sub = dx + 2 dy + 4 dz); the four red residuals and eight parents meet in
+// LDS and the first wavefront sums each parent's share in sub order.
 template <class T>
 __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const int* __restrict__ fparent,
-                                                    T* __restrict__ bc, T* __restrict__ xc,
-                                                    const T* __restrict__ cdiag, int red_end, const PcgState* st) {
+                                                     T* __restrict__ bc, T* __restrict__ xc,
+                                                     const T* __restrict__ cdiag, int red_end, const PcgState* st) {
+  __shared__ T sr[4][64];
+  __shared__ int sp[4][64], spb[4][64];
   const int status = st ? st->status : ST_RUNNING;
-  // four lanes per 2x2x2 bin, 64 bins of one bin row per workgroup (block-uniform
-  // position); a bin at even coordinates holds its red cells at subs 0, 3, 5, 6
-  // and its black ones at 1, 2, 4, 7 (sub = dx + 2 dy + 4 dz): lane l takes red
-  // sub kRed[l] and the parent of black sub kBlack[l]
   const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1;
   const unsigned gbx = (bx + 63) >> 6;
   const unsigned tile = blockIdx.x, rest = tile / gbx;
-  const unsigned Bx = (tile % gbx) * 64 + (threadIdx.x >> 2), By = rest % by, Bz = rest / by;
-  const int lane = (int)(threadIdx.x & 3);
-  const int red_sub = lane == 0 ? 0 : lane == 1 ? 3 : lane == 2 ? 5 : 6;
-  const int black_sub = lane == 0 ? 1 : lane == 1 ? 2 : lane == 2 ? 4 : 7;
-  const int sub = lane;  // the group's first lane sums
+  const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned Bx = (tile % gbx) * 64 + lane, By = rest % by, Bz = rest / by;
+  const int dy = (int)(w & 1), dz = (int)(w >> 1);
+  // red cell of the sub-row: i + j + k even with j = 2 By + dy, k = 2 Bz + dz
+  const int dxr = (dy + dz) & 1;
+  const int j = (int)(2 * By) + dy, k = (int)(2 * Bz) + dz;
   T r = T(0);
   int p = -1, pb = -1;
-  if (Bx < bx) {
-    const int qr[3] = {(int)(2 * Bx) + (red_sub & 1), (int)(2 * By) + ((red_sub >> 1) & 1), (int)(2 * Bz) + (red_sub >> 2)};
-    const int qb[3] = {(int)(2 * Bx) + (black_sub & 1), (int)(2 * By) + ((black_sub >> 1) & 1),
-                       (int)(2 * Bz) + (black_sub >> 2)};
-    if (qr[0] < g.n[0] && qr[1] < g.n[1] && qr[2] < g.n[2]) {
-      p = fparent[(unsigned)qr[0] + (unsigned)g.sy * (unsigned)qr[1] + (unsigned)g.sz * (unsigned)qr[2]];
-      r = rb_residual<T>(g, F, qr);
+  if (Bx < bx && j < g.n[1] && k < g.n[2]) {
+    const int ir = (int)(2 * Bx) + dxr, ib = (int)(2 * Bx) + (1 - dxr);
+    const unsigned row = (unsigned)g.sy * (unsigned)j + (unsigned)g.sz * (unsigned)k;
+    if (ir < g.n[0]) {
+      const int q[3] = {ir, j, k};
+      p = fparent[row + (unsigned)ir];
+      r = rb_residual<T>(g, F, q);
     }
     // the black cells were relaxed last: their residuals vanish (taken as zero)
-    if (qb[0] < g.n[0] && qb[1] < g.n[1] && qb[2] < g.n[2])
-      pb = fparent[(unsigned)qb[0] + (unsigned)g.sy * (unsigned)qb[1] + (unsigned)g.sz * (unsigned)qb[2]];
+    if (ib < g.n[0]) pb = fparent[row + (unsigned)ib];
   }
-  // the bin's eight (residual, parent) pairs in sub order
-  const int base = (int)(threadIdx.x & 63) & ~3;
-  const T r0 = __shfl(r, base, 64), r3 = __shfl(r, base + 1, 64), r5 = __shfl(r, base + 2, 64),
-          r6 = __shfl(r, base + 3, 64);
-  const int p0 = __shfl(p, base, 64), p3 = __shfl(p, base + 1, 64), p5 = __shfl(p, base + 2, 64),
-            p6 = __shfl(p, base + 3, 64);
-  const int p1 = __shfl(pb, base, 64), p2 = __shfl(pb, base + 1, 64), p4 = __shfl(pb, base + 2, 64),
-            p7 = __shfl(pb, base + 3, 64);
-  const T rs[8] = {r0, T(0), T(0), r3, T(0), r5, r6, T(0)};
-  const int ps[8] = {p0, p1, p2, p3, p4, p5, p6, p7};
-  if (sub != 0 || status != ST_RUNNING) return;
+  sr[w][lane] = r;
+  sp[w][lane] = p;
+  spb[w][lane] = pb;
+  __syncthreads();
+  if (w != 0 || Bx >= bx || status != ST_RUNNING) return;
+  // sub = dx + 2 dy + 4 dz: red subs 0, 3, 5, 6 are sub-rows 0, 1, 2, 3; black
+  // subs 1, 2, 4, 7 likewise
+  const T rs[8] = {sr[0][lane], T(0), T(0), sr[1][lane], T(0), sr[2][lane], sr[3][lane], T(0)};
+  const int ps[8] = {sp[0][lane], spb[0][lane], spb[1][lane], sp[1][lane], spb[2][lane], sp[2][lane], sp[3][lane],
+                     spb[3][lane]};
   unsigned done = 0;
 #pragma unroll
   for (int s2 = 0; s2 < 8; ++s2) {
