@@ -1414,28 +1414,31 @@ __device__ __forceinline__ T first_relax(int node, int red_end, T b, const T* __
   return d != T(0) ? b / d : T(0);
 }
 
-// Restriction into level C: eight lanes per aggregate each form the residual of
-// one finer node (lane k takes children k, k + 8, ...); the first lane adds the
-// eight partial sums in lane order (ascending children when there are at most
-// eight). The coarse iterate starts at zero.
+// Restriction into level C: wavefront s of a workgroup forms the residuals of
+// child slot s (children s, s + 8, ...) of 64 consecutive aggregates, so the
+// lanes of a load read the same slot of neighbouring aggregates; the eight
+// partial sums meet in LDS and the first wavefront adds them in slot order
+// (ascending children when there are at most eight). The coarse iterate starts
+// at zero.
+constexpr int kRestrictThreads = 512;
 template <class T>
-__global__ void __launch_bounds__(BS) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C, const PcgState* st) {
+__global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C,
+                                                                   const PcgState* st) {
+  __shared__ T part[8][64];
   const int status = st ? st->status : ST_RUNNING;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;
-  const int J = (int)(t >> 3), sub = (int)(t & 7);
+  const int lane = (int)(threadIdx.x & 63), sub = (int)(threadIdx.x >> 6);
+  const int J = (int)blockIdx.x * 64 + lane;
   T r = T(0);
   if (J < C.n) {
     const int k0 = C.child_ptr[J], k1 = C.child_ptr[J + 1];
     for (int k = k0 + sub; k < k1; k += 8) r += agg_row_residual<T>(F, C.child[k]);
   }
-  const int base = (int)(threadIdx.x & 63) & ~7;
-  T rs[8];
-#pragma unroll
-  for (int s2 = 0; s2 < 8; ++s2) rs[s2] = __shfl(r, base + s2, 64);
+  part[sub][lane] = r;
+  __syncthreads();
   if (sub != 0 || J >= C.n || status != ST_RUNNING) return;
   T sum = T(0);
 #pragma unroll
-  for (int s2 = 0; s2 < 8; ++s2) sum += rs[s2];
+  for (int s2 = 0; s2 < 8; ++s2) sum += part[s2][lane];
   C.b[J] = sum;
   C.x[J] = first_relax<T>(J, C.cptr[1], sum, C.diag);
 }
@@ -2926,7 +2929,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // the first half-sweep (colour 0 from zero) happened in the restriction
     for (int round = 0; round < cr; ++round)
       for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) relax(L, c);
-    hipLaunchKernelGGL((k_agg_restrict<T>), dim3(blocks_for(8LL * lv[l + 1].n)), dim3(BS), 0, s, L, lv[l + 1], st);
+    hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st);
   }
   direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
   for (int l = stop - 1; l >= 0; --l) {
