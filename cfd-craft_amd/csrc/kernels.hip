@@ -1349,7 +1349,8 @@ __device__ __forceinline__ T ell_offdiag(const AggDevLevel<T>& L, int i, T v, bo
 #pragma unroll
   for (int k = 0; k < kEllWidth; ++k) {
     xv[k] = T(0);
-    if (k < L.K) {
+    // padding (the row itself, value 0) needs no gather
+    if (k < L.K && cols[k] != i) {
       xv[k] = L.x[cols[k]];
       if (xc) xv[k] += cs * xc[L.parent[cols[k]]];
     }
