@@ -455,8 +455,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the pressure solve of one grid is measured on one MI355X "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the pressure solve of one grid is measured on one MI355X; "
+              "several GPUs would each hold their own grids (one wind direction or served grid per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.pmc_probe:
