@@ -1419,12 +1419,14 @@ __device__ __forceinline__ T first_relax(int node, int red_end, T b, const T* __
 // child slot s (children s, s + 8, ...) of 64 consecutive aggregates, so the
 // lanes of a load read the same slot of neighbouring aggregates; the eight
 // partial sums meet in LDS and the first wavefront adds them in slot order
-// (ascending children when there are at most eight). The coarse iterate starts
-// at zero.
+// (ascending children when there are at most eight). Children from zero_from
+// on (the colour class relaxed last) have a residual that is zero to rounding
+// and are taken as zero, except rows without couplings (their residual is b).
+// The coarse iterate starts at zero.
 constexpr int kRestrictThreads = 512;
 template <class T>
 __global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C,
-                                                                   const PcgState* st) {
+                                                                   const PcgState* st, int zero_from) {
   __shared__ T part[8][64];
   const int status = st ? st->status : ST_RUNNING;
   const int lane = (int)(threadIdx.x & 63), sub = (int)(threadIdx.x >> 6);
@@ -1432,7 +1434,10 @@ __global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T
   T r = T(0);
   if (J < C.n) {
     const int k0 = C.child_ptr[J], k1 = C.child_ptr[J + 1];
-    for (int k = k0 + sub; k < k1; k += 8) r += agg_row_residual<T>(F, C.child[k]);
+    for (int k = k0 + sub; k < k1; k += 8) {
+      const int c = C.child[k];
+      if (c < zero_from || F.diag[c] == T(0)) r += agg_row_residual<T>(F, c);
+    }
   }
   part[sub][lane] = r;
   __syncthreads();
@@ -2930,7 +2935,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // the first half-sweep (colour 0 from zero) happened in the restriction
     for (int round = 0; round < cr; ++round)
       for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) relax(L, c);
-    hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st);
+    // pre-smoothing ended with the last colour class (without rounds nothing is taken as zero)
+    const int zero_from = cr >= 1 ? L.cptr[L.ncol - 1] : L.n;
+    hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st,
+                       zero_from);
   }
   direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
   for (int l = stop - 1; l >= 0; --l) {
