@@ -116,6 +116,7 @@ struct DevAggShared {
   // colour-separated fine level (grids without odd periodic seams)
   DevBuf<uint8_t> rb_codes;
   DevBuf<int> rb_rpar;  // parent of each red cell in the colour-separated order
+  DevBuf<int> rb_bpar;  // common parent of each 2x2x2 block of cells (kMixedParent: several)
   DevBuf<double> rb_b64, rb_x64;
   DevBuf<float> rb_b32, rb_x32;
   FineRB<double> rb64{};
@@ -519,6 +520,9 @@ int build_agg(cfd_grid* g) {
     const long long half = (long long)hx * h.n[1] * h.n[2];
     std::vector<uint8_t> rc8((size_t)(2 * half), 0);
     std::vector<int> rpar((size_t)half, -1);
+    const int64_t bx = (h.n[0] + 1) / 2, by = (h.n[1] + 1) / 2, bz = (h.n[2] + 1) / 2;
+    std::vector<int> bpar((size_t)(bx * by * bz), 0);
+    std::vector<uint8_t> bseen(bpar.size(), 0);
     for (int64_t c = 0; c < h.count; ++c) {
       const int64_t i = c % h.n[0], j = c / h.n[0] % h.n[1], k = c / (h.n[0] * h.n[1]);
       const int colour = (int)((i + j + k) & 1);
@@ -526,8 +530,17 @@ int build_agg(cfd_grid* g) {
       rc8[(size_t)colour * (size_t)half + u] = h.codes[(size_t)c];
       const int32_t node = H.node_of_cell[(size_t)c];
       if (colour == 0 && node >= 0) rpar[u] = H.levels[0].parent[(size_t)node];
+      const int par = node >= 0 ? H.levels[0].parent[(size_t)node] : -1;
+      const size_t blk = (size_t)((i >> 1) + bx * ((j >> 1) + by * (k >> 1)));
+      if (!bseen[blk]) {
+        bseen[blk] = 1;
+        bpar[blk] = par;
+      } else if (bpar[blk] != par) {
+        bpar[blk] = kMixedParent;
+      }
     }
     if ((rc = A.rb_codes.upload(rc8.data(), rc8.size())) || (rc = A.rb_rpar.upload(rpar.data(), rpar.size())) ||
+        (rc = A.rb_bpar.upload(bpar.data(), bpar.size())) ||
         (rc = A.rb_b64.alloc((size_t)(2 * half))) ||
         (rc = A.rb_x64.alloc((size_t)(2 * half))) || (rc = A.rb_b32.alloc((size_t)(2 * half))) ||
         (rc = A.rb_x32.alloc((size_t)(2 * half))))
@@ -543,8 +556,8 @@ int build_agg(cfd_grid* g) {
     // bytes in flight (2M cells: 10.5 against 8.8 us, 3.5M: 14.2 against 11.4,
     // 8.4M: 23.0 against 18.2). CFD_MG_RB_PAIRS=0 / 1 forces.
     const int two = h.n[0] % 4 == 0 && (g->rb_pairs == 1 || (g->rb_pairs < 0 && half >= 3 * (1LL << 18)));
-    A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p, two, A.rb_rpar.p};
-    A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p, two, A.rb_rpar.p};
+    A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p, two, A.rb_rpar.p, A.rb_bpar.p};
+    A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p, two, A.rb_rpar.p, A.rb_bpar.p};
     A.has_rb = true;
   }
   // the host copies of the coarse matrices are no longer needed

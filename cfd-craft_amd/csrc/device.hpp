@@ -112,7 +112,10 @@ struct FineRB {
   T* x;
   int two_cells;     // half-sweeps take two cells per thread (nx a multiple of 4; chosen at set-up)
   const int* rpar;   // per red cell (red layout): node of the first coarse level, -1 without one
+  const int* bpar;   // per 2x2x2 block of cells (x fastest): the common node of the first coarse
+                     // level of its cells in the grid (-1: all solid), kMixedParent when they differ
 };
+constexpr int kMixedParent = -2147483647 - 1;
 
 // Coarsest level: one dense block per connected component. Blocks of at most
 // 64 nodes take one wavefront each; larger ones ("big", listed separately) one

@@ -2189,16 +2189,18 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
   const int j = (int)(2 * By) + dy, k = (int)(2 * Bz) + dz;
   T r = T(0);
   int p = -1, pb = -1;
+  // a block of one parent needs no per-cell parent load
+  const int bp = Bx < bx ? F.bpar[Bx + bx * (By + by * Bz)] : kMixedParent;
   if (Bx < bx && j < g.n[1] && k < g.n[2]) {
     const int ir = (int)(2 * Bx) + dxr, ib = (int)(2 * Bx) + (1 - dxr);
     const unsigned row = (unsigned)g.sy * (unsigned)j + (unsigned)g.sz * (unsigned)k;
     if (ir < g.n[0]) {
       const int q[3] = {ir, j, k};
-      p = fparent[row + (unsigned)ir];
+      p = bp != kMixedParent ? bp : fparent[row + (unsigned)ir];
       r = rb_residual<T>(g, F, q);
     }
     // the black cells were relaxed last: their residuals vanish (taken as zero)
-    if (ib < g.n[0]) pb = fparent[row + (unsigned)ib];
+    if (ib < g.n[0]) pb = bp != kMixedParent ? bp : fparent[row + (unsigned)ib];
   }
   sr[w][lane] = r;
   sp[w][lane] = p;
