@@ -60,6 +60,12 @@ def lib():
         L.orc_gradient.argtypes = [P, P, P]
         L.orc_enforce_velocity.argtypes = [P, P]
         L.orc_flow_vector.argtypes = [P, ctypes.c_double, D]
+        L.orc_poly_contains.argtypes = [P, ctypes.c_int32, ctypes.c_double, ctypes.c_double]
+        L.orc_poly_contains.restype = ctypes.c_int
+        L.orc_masked_level_sizes.argtypes = [P, ctypes.c_int, P]
+        L.orc_masked_level_sizes.restype = ctypes.c_int
+        L.orc_masked_level.argtypes = [P, ctypes.c_int] + [P] * 8
+        L.orc_masked_level.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -95,6 +101,12 @@ class OracleError(RuntimeError):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def poly_contains(polygon, p) -> bool:
+    """geometry.rs:13-26 contains(polygon, p) for one point."""
+    xy = np.ascontiguousarray(np.asarray(polygon, dtype=np.float64).reshape(-1))
+    return bool(lib().orc_poly_contains(_ptr(xy), len(xy) // 2, float(p[0]), float(p[1])))
 
 
 class Grid:
@@ -228,6 +240,37 @@ class Grid:
             out.append(dict(level=lvl, axes=[a for a in range(3) if mask >> a & 1], block_unknowns=unknowns,
                             block_count=count, half_bandwidth=bw))
         return dict(level_dimensions=dims[: 3 * n].reshape(n, 3).tolist(), block_smoothers=out)
+
+    def masked_levels(self):
+        """The masked aggregation hierarchy (masked.rs:181-291) level by level: dicts
+        with n, parent (None at the coarsest level), CSR offsets / columns / values,
+        diagonal, closed (list of member arrays in the level's order), and, on
+        level 0, fluid (the cell of each unknown)."""
+        sizes = np.zeros(6, np.int64)
+        nlev = lib().orc_masked_level_sizes(self.h, -1, _ptr(sizes))
+        if nlev < 0:
+            raise OracleError(-nlev, lib().orc_last_error().decode())
+        out = []
+        for l in range(nlev):
+            lib().orc_masked_level_sizes(self.h, l, _ptr(sizes))
+            n, nnz, ncl, nmem, has_parent, nblocks = (int(v) for v in sizes)
+            parent = np.zeros(max(1, n), np.int64)
+            offsets = np.zeros(n + 1, np.int64)
+            columns = np.zeros(max(1, nnz), np.int64)
+            values = np.zeros(max(1, nnz))
+            diag = np.zeros(max(1, n))
+            cptr = np.zeros(ncl + 1, np.int64)
+            cmem = np.zeros(max(1, nmem), np.int64)
+            fluid = np.zeros(max(1, self.fluid_count), np.int64) if l == 0 else None
+            code = lib().orc_masked_level(self.h, l, _ptr(parent), _ptr(offsets), _ptr(columns), _ptr(values), _ptr(diag),
+                                          _ptr(cptr), _ptr(cmem), _ptr(fluid) if fluid is not None else None)
+            if code:
+                raise OracleError(-code, lib().orc_last_error().decode())
+            out.append(dict(n=n, parent=parent[:n] if has_parent else None, offsets=offsets, columns=columns[:nnz],
+                            values=values[:nnz], diagonal=diag[:n],
+                            closed=[cmem[cptr[c]:cptr[c + 1]] for c in range(ncl)], blocks=nblocks,
+                            fluid=fluid[: self.fluid_count] if fluid is not None else None))
+        return out
 
     def divergence(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

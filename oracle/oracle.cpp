@@ -719,6 +719,7 @@ struct orc_grid {
   orc_case flow;  // flow description for inlet values (pointers cleared)
   std::unique_ptr<Factor> factor;
   std::unique_ptr<Hierarchy> hierarchy;
+  std::unique_ptr<Hierarchy> masked;  // masked.rs hierarchy of any grid (inspection)
   double weight(const Face& f) const { return f.area / (f.distance * volume); }
 };
 
@@ -1399,6 +1400,63 @@ int orc_mg_shape(orc_grid* g, int64_t* dims, int64_t* blocks, int cap, int* n_bl
     }
   *n_blocks = k;
   return n;
+}
+
+int orc_poly_contains(const double* xy, int32_t n, double x, double y) { return poly_contains(xy, n, x, y) ? 1 : 0; }
+
+namespace {
+int masked_of(orc_grid* g, const Hierarchy** out) {
+  if (!g->masked) {
+    auto h = std::make_unique<Hierarchy>();
+    int e = masked_build(*g, *h);
+    if (e) return e;
+    g->masked = std::move(h);
+  }
+  *out = g->masked.get();
+  return 0;
+}
+}  // namespace
+
+int orc_masked_level_sizes(orc_grid* g, int level, int64_t* sizes) {
+  const Hierarchy* h = nullptr;
+  if (int e = masked_of(g, &h)) return -e;
+  const int levels = (int)h->mlevels.size();
+  if (level < 0 || level >= levels) return levels;
+  const MaskedLevel& L = h->mlevels[(size_t)level];
+  int64_t members = 0;
+  for (const auto& c : L.closed) members += (int64_t)c.size();
+  sizes[0] = (int64_t)L.x.size();
+  sizes[1] = (int64_t)L.a.columns.size();
+  sizes[2] = (int64_t)L.closed.size();
+  sizes[3] = members;
+  sizes[4] = L.parent.empty() ? 0 : 1;
+  sizes[5] = (int64_t)L.direct.size();
+  return levels;
+}
+
+int orc_masked_level(orc_grid* g, int level, int64_t* parent, int64_t* offsets, int64_t* columns, double* values,
+                     double* diagonal, int64_t* closed_ptr, int64_t* closed_members, int64_t* fluid_indices) {
+  const Hierarchy* h = nullptr;
+  if (int e = masked_of(g, &h)) return -e;
+  if (level < 0 || level >= (int)h->mlevels.size()) return -34;
+  const MaskedLevel& L = h->mlevels[(size_t)level];
+  const size_t n = L.x.size();
+  for (size_t i = 0; i < L.parent.size(); ++i) parent[i] = (int64_t)L.parent[i];
+  for (size_t i = 0; i <= n; ++i) offsets[i] = (int64_t)L.a.offsets[i];
+  for (size_t k = 0; k < L.a.columns.size(); ++k) {
+    columns[k] = (int64_t)L.a.columns[k];
+    values[k] = L.a.values[k];
+  }
+  for (size_t i = 0; i < n; ++i) diagonal[i] = L.a.diagonal[i];
+  int64_t at = 0;
+  closed_ptr[0] = 0;
+  for (size_t c = 0; c < L.closed.size(); ++c) {
+    for (size_t i : L.closed[c]) closed_members[at++] = (int64_t)i;
+    closed_ptr[c + 1] = at;
+  }
+  if (fluid_indices)
+    for (size_t j = 0; j < h->fluid_indices.size(); ++j) fluid_indices[j] = (int64_t)h->fluid_indices[j];
+  return 0;
 }
 
 int orc_solve_scaled(orc_grid* g, const double* b, const double* scale, double tol, int64_t max, double* x_out,

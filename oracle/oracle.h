@@ -104,6 +104,22 @@ int orc_mg_stats(orc_grid* g, int64_t* level_cells, int64_t* level_nonzeros, int
  * half-bandwidth, axis count (line.rs:200-209). Returns the dimension rows. */
 int orc_mg_shape(orc_grid* g, int64_t* dims, int64_t* blocks, int cap, int* n_blocks);
 
+/* geometry.rs:13-26 contains(): the even-odd crossing test of one point
+ * against a footprint of n vertices (x0 y0 x1 y1 ...). */
+int orc_poly_contains(const double* xy, int32_t n, double x, double y);
+/* The masked aggregation hierarchy (masked.rs:181-291) of the grid, built on
+ * first use for any grid (also one the reference would give the unmasked
+ * hierarchy). sizes: unknowns, stored off-diagonal entries, closed components,
+ * closed members, whether the level has parents, dense blocks. Returns the
+ * level count, or minus the error code. */
+int orc_masked_level_sizes(orc_grid* g, int level, int64_t* sizes);
+/* One level's arrays: parent (unknowns, when the level has parents), CSR
+ * offsets (unknowns + 1), columns and values, diagonal, closed component
+ * offsets (closed + 1) and members in the level's order, and (may be NULL)
+ * the fluid cell of each level-0 unknown. Returns 0 or minus the error code. */
+int orc_masked_level(orc_grid* g, int level, int64_t* parent, int64_t* offsets, int64_t* columns, double* values,
+                     double* diagonal, int64_t* closed_ptr, int64_t* closed_members, int64_t* fluid_indices);
+
 /* Grid::divergence / gradient (grid.rs:612-665) */
 void orc_divergence(const orc_grid* g, const double* u, double* div);
 void orc_gradient(const orc_grid* g, const double* p, double* grad);
