@@ -221,6 +221,30 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _host_vector(a, n: int, name: str, copy: bool = False) -> np.ndarray:
+    """A contiguous float64 copy or view of `a`, which must hold exactly n values:
+    the library reads and writes n doubles through the pointer."""
+    v = np.array(a, dtype=np.float64, copy=True) if copy else np.ascontiguousarray(a, dtype=np.float64)
+    if v.ndim != 1 or v.size != n:
+        raise ValueError(f"{name} must be a vector of {n} values, got shape {np.shape(a)}")
+    return v
+
+
+def _device_vector(d, n: int, name: str, optional: bool = False):
+    """The device pointer of a DeviceArray of exactly n float64 values (the
+    kernels index n doubles from it)."""
+    if d is None:
+        if optional:
+            return None
+        raise ValueError(f"{name} is required")
+    if not isinstance(d, DeviceArray) or d.dtype != np.float64 or d.n != n:
+        got = f"{getattr(d, 'n', '?')} x {getattr(d, 'dtype', type(d).__name__)}"
+        raise ValueError(f"{name} must be a DeviceArray of {n} float64 values, got {got}")
+    if d.ptr is None:
+        raise ValueError(f"{name} has been freed")
+    return d.ptr
+
+
 def case_topology(case: Case, ic0: bool = False):
     """Host-only Grid::build topology (no device needed): dict with counts, codes,
     component per cell, anchored per component and optionally the IC(0) diagonal."""
@@ -407,13 +431,13 @@ class Grid:
 
     # ---- operator and gauges
     def apply(self, x) -> np.ndarray:
-        x = np.ascontiguousarray(x, dtype=np.float64)
+        x = _host_vector(x, self.count, "x")
         y = np.empty(self.count)
         _check(lib().cfd_apply(self.h, _ptr(x), _ptr(y)))
         return y
 
     def center_nullspaces(self, x) -> np.ndarray:
-        x = np.array(x, dtype=np.float64)
+        x = _host_vector(x, self.count, "x", copy=True)
         _check(lib().cfd_center_nullspaces(self.h, _ptr(x)))
         return x
 
@@ -427,7 +451,7 @@ class Grid:
         return bool(same.value)
 
     def precondition(self, r) -> np.ndarray:
-        r = np.ascontiguousarray(r, dtype=np.float64)
+        r = _host_vector(r, self.count, "r")
         z = np.empty(self.count)
         _check(lib().cfd_precondition(self.h, _ptr(r), _ptr(z)))
         return z
@@ -457,8 +481,8 @@ class Grid:
     def solve_scaled(self, b, scale=None, tol=None, max_iterations=None, control=None):
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        s = np.ascontiguousarray(scale, dtype=np.float64) if scale is not None else None
+        b = _host_vector(b, self.count, "b")
+        s = _host_vector(scale, self.count, "scale") if scale is not None else None
         x = np.empty(self.count)
         fn, keep = self._poll(control)
         info = self._measured(
@@ -476,17 +500,18 @@ class Grid:
     def solve_device(self, d_b: DeviceArray, d_x: DeviceArray, tol=None, max_iterations=None, d_scale=None, control=None):
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
+        pb = _device_vector(d_b, self.count, "d_b")
+        px = _device_vector(d_x, self.count, "d_x")
+        ps = _device_vector(d_scale, self.count, "d_scale", optional=True)
         fn, keep = self._poll(control)
         return self._measured(
             control,
-            lambda info: lib().cfd_solve_scaled_device(
-                self.h, d_b.ptr, d_scale.ptr if d_scale is not None else None, float(tol), int(max_iterations), fn, None,
-                d_x.ptr, ctypes.byref(info),
-            ),
+            lambda info: lib().cfd_solve_scaled_device(self.h, pb, ps, float(tol), int(max_iterations), fn, None, px,
+                                                       ctypes.byref(info)),
         )
 
     def projection_rhs(self, u, dt):
-        u = np.ascontiguousarray(u, dtype=np.float64)
+        u = _host_vector(u, self.face_count, "u")
         b = np.empty(self.count)
         t = np.empty(self.count)
         _check(lib().cfd_projection_rhs(self.h, _ptr(u), float(dt), _ptr(b), _ptr(t)))
@@ -497,8 +522,8 @@ class Grid:
         and only on success, pressure.rs:468)."""
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
-        u = np.array(u, dtype=np.float64)
-        p = np.array(p, dtype=np.float64)
+        u = _host_vector(u, self.face_count, "u", copy=True)
+        p = _host_vector(p, self.count, "p", copy=True)
         fn, keep = self._poll(control)
         # pressure_projection encloses the solve (pressure.rs:478); one library call
         # forms the right-hand side, solves and corrects, all inside pressure_solve
@@ -564,9 +589,12 @@ class Server:
 
     def submit(self, g: "Grid", d_b: "DeviceArray", d_x: "DeviceArray", tol: float, max_iterations: int = 2000,
                precision: int = PRECISION_MIXED, d_scale: "DeviceArray" = None) -> int:
+        pb = _device_vector(d_b, g.count, "d_b")
+        px = _device_vector(d_x, g.count, "d_x")
+        ps = _device_vector(d_scale, g.count, "d_scale", optional=True)
         t = ctypes.c_uint64(0)
-        _check(lib().cfd_server_submit(self.h, g.h, d_b.ptr, d_scale.ptr if d_scale is not None else None, float(tol),
-                                       int(max_iterations), int(precision), d_x.ptr, ctypes.byref(t)))
+        _check(lib().cfd_server_submit(self.h, g.h, pb, ps, float(tol), int(max_iterations), int(precision), px,
+                                       ctypes.byref(t)))
         return t.value
 
     def progress(self) -> int:

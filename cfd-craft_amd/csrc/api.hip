@@ -182,6 +182,7 @@ struct cfd_grid {
   DevBuf<double> G;
   bool has_geo = false;  // geometric hierarchy built (CFD_CYCLE_GEOMETRIC)
   DevBuf<float> x0f;     // FP32 fine iterate of the mixed-precision cycles
+  DevBuf<float> ic0_y32;  // FP32 forward result of the mixed-precision IC(0) application
   // aggregation hierarchy (CFD_CYCLE_RB)
   AggHierarchy agg;
   DevAggShared ash;
@@ -881,6 +882,7 @@ int setup_device_inner(cfd_grid* g) {
   if (h.preconditioner == CFD_IC0) {
     build_ic0_diagonal(h);
     if ((rc = g->D.upload(h.ic0_diag.data(), N))) return rc;
+    if ((rc = g->ic0_y32.alloc(N))) return rc;
   } else {
     g->has_hier = true;
     if ((rc = g->x0f.alloc(N))) return rc;
@@ -907,7 +909,10 @@ bool rz_fused(const cfd_grid* g) {
 
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
   if (g->host.preconditioner == CFD_IC0) {
-    if (g->ic0_tiles)
+    // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
+    if (g->opt.precision == CFD_PRECISION_MIXED)
+      launch_ic0_tiles_mixed(g->geo, g->codes.p, g->D.p, r, g->ic0_y32.p, z, st, g->stream);
+    else if (g->ic0_tiles)
       launch_ic0_tiles(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
     else
       launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
@@ -1768,6 +1773,7 @@ struct cfd_server {
     double tol = 0.0;
     int64_t max = 0;
     int precision = 0;
+    int grid_precision = 0;  // the grid's own setting, restored when the request ends
     double t_submit = 0.0, t_start = 0.0, t_end = 0.0;
     int state = 0;  // 0 queued, 1 running, 2 done
     int rc = 0;
@@ -1796,6 +1802,7 @@ int server_progress(cfd_server* s) {
     }
     R.t_end = now_s();
     R.state = 2;
+    R.g->opt.precision = R.grid_precision;
     if (rc == JOB_DONE) {
       R.rc = R.g->job.rc;
       R.error = R.g->job.error;
@@ -1815,11 +1822,15 @@ int server_progress(cfd_server* s) {
       ++i;
       continue;
     }
+    // the request's precision for the duration of the job (the iteration graphs
+    // and learned counts are keyed by it); the grid's own setting comes back after
+    R.grid_precision = R.g->opt.precision;
     R.g->opt.precision = R.precision;
     int rc = job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
     R.t_start = now_s();
     s->queued.erase(s->queued.begin() + (long)i);
     if (rc) {
+      R.g->opt.precision = R.grid_precision;
       R.state = 2;
       R.rc = rc;
       R.error = g_error;
@@ -1911,7 +1922,11 @@ int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const doubl
     set_error("cfd_server_submit: tolerance, iteration limit or precision out of range");
     return CFD_EINVAL;
   }
-  if (precision == CFD_PRECISION_MIXED && !g->has_hier) precision = CFD_PRECISION_FP64;
+  if (precision == CFD_PRECISION_MIXED && g->has_hier &&
+      (g->opt.cycle == CFD_CYCLE_REFERENCE || g->opt.cycle == CFD_CYCLE_ZEBRA)) {
+    set_error("cfd_server_submit: the reference and zebra cycles run in FP64 only");
+    return CFD_EINVAL;
+  }
   const uint64_t id = s->next++;
   cfd_server::Req& R = s->reqs[id];
   R.g = g;

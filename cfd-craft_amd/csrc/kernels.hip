@@ -591,12 +591,19 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 // k_ic0_fwd / k_ic0_bwd's operations in the same order (bitwise equal).
 // Tiles of TX x TY x TZ cells (TX TY TZ = 512 threads, one per cell): 8 x 8 x 8,
 // or 16 x 2 x 16 (and its rotations) on grids with an axis of at most 2 cells.
-template <bool FWD, int TX, int TY, int TZ>
+// T is the arithmetic type of the sweep: double repeats the reference's
+// operations bit for bit; float is the mixed-precision application (the factor
+// pivots D stay FP64: each entry a / D is formed in FP64 and rounded once).
+// own: the value the cell starts from (r forward, the forward result backward);
+// out: where the sweep writes, also read for the neighbours solved by earlier
+// tile diagonals (the forward sweep's lower and the backward sweep's upper
+// neighbours).
+template <bool FWD, int TX, int TY, int TZ, class T, class TOWN, class TOUT>
 __global__ void __launch_bounds__(TX * TY * TZ)
-    k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D, const double* __restrict__ r,
-               double* __restrict__ y, int P, int J0, int K0, int nJ, int nK, int ntx, const PcgState* st) {
+    k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D, const TOWN* own, TOUT* out,
+               int P, int J0, int K0, int nJ, int nK, int ntx, const PcgState* st) {
   constexpr int NT = TX * TY * TZ;
-  __shared__ double ys[NT];
+  __shared__ T ys[NT];
   if (st && st->status != ST_RUNNING) return;
   const int J = J0 + (int)blockIdx.x % nJ, K = K0 + (int)blockIdx.x / nJ;
   const int I = P - J - K;
@@ -608,7 +615,7 @@ __global__ void __launch_bounds__(TX * TY * TZ)
   const int a = li % TX, b = (li / TX) % TY, cz = li / (TX * TY);
   const int i = ox + a, j = oy + b, k = oz + cz;
   const bool live = i < g.n[0] && j < g.n[1] && k < g.n[2];
-  double co[6], ext[6], rv = 0.0, dc = 1.0;
+  T co[6], ext[6], rv = 0, dc = 1;
   int lix[6];
   long long c = 0;
 #pragma unroll
@@ -618,20 +625,21 @@ __global__ void __launch_bounds__(TX * TY * TZ)
     const uint8_t code = codes[c];
     const int q[3] = {i, j, k};
     const Ic0Nbrs N = ic0_nbrs(g, code, c, q, FWD);
-    dc = D[c];
-    rv = FWD ? r[c] : y[c];
+    const double d64 = D[c];
+    dc = (T)d64;
+    rv = (T)own[c];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
       if (N.a[m] == 0.0) continue;
       const unsigned nb = (unsigned)N.idx[m];
-      co[m] = FWD ? N.a[m] / D[nb] : N.a[m] / dc;
+      co[m] = (T)(FWD ? N.a[m] / D[nb] : N.a[m] / d64);
       const unsigned ni = nb % (unsigned)g.n[0], nj = (nb / (unsigned)g.n[0]) % (unsigned)g.n[1];
       const unsigned nk = nb / (unsigned)g.sz;
       if ((int)ni >= ox && (int)ni < ox + TX && (int)nj >= oy && (int)nj < oy + TY && (int)nk >= oz && (int)nk < oz + TZ) {
         lix[m] = (int)(ni - ox) + TX * ((int)(nj - oy) + TY * (int)(nk - oz));
       } else {
         lix[m] = -1;
-        ext[m] = y[nb];
+        ext[m] = (T)out[nb];
       }
     }
   }
@@ -642,16 +650,16 @@ __global__ void __launch_bounds__(TX * TY * TZ)
   for (int step = 0; step <= S; ++step) {
     const int sp = FWD ? step : S - step;
     if (live && plane == sp) {
-      double value = rv;
+      T value = rv;
 #pragma unroll
       for (int m = 0; m < 6; ++m) {
         if (lix[m] == -2) continue;
-        const double ym = lix[m] >= 0 ? ys[lix[m]] : ext[m];
+        const T ym = lix[m] >= 0 ? ys[lix[m]] : ext[m];
         value -= co[m] * ym;
       }
-      const double v = value / dc;
+      const T v = value / dc;
       ys[li] = v;
-      y[c] = v;
+      out[c] = (TOUT)v;
     }
     __syncthreads();
   }
@@ -2274,7 +2282,8 @@ __device__ __forceinline__ double ref_fine_row(const Geo& g, uint8_t code, long 
 }
 
 // Gauss-Seidel on hyperplane i + j + k = L: x = (rhs + sum w x_j) / a_ii, which
-// is the reference's rhs - sum a_ij x_j with a_ij = -w (Matrix::sweep).
+// is the reference's rhs - sum a_ij x_j with a_ij = -w (Matrix::sweep,
+// masked.rs:158-172).
 __global__ void __launch_bounds__(BS) k_ref_fine_gs(Geo g, const uint8_t* __restrict__ codes,
                                                     const double* __restrict__ diag, const double* __restrict__ rhs,
                                                     double* __restrict__ x, int L, int j0, int k0, int nj, int nk,
@@ -2292,7 +2301,9 @@ __global__ void __launch_bounds__(BS) k_ref_fine_gs(Geo g, const uint8_t* __rest
   const double v = ref_fine_row(g, code, c, q, x, rhs[c], false, self_mask);
   const double d = diag[c];
   if (status != ST_RUNNING) return;
-  x[c] = v / d;
+  // a sealed singleton (no couplings, no outlet) has no unknown: it stays zero
+  // (masked.rs:161-166)
+  x[c] = d != 0.0 ? v / d : 0.0;
 }
 
 // r = rhs - (a_ii x + sum a_ij x_j) (Matrix::product); zero on solid cells.
@@ -2683,8 +2694,9 @@ static void level_box(const Geo& g, int L, int& j0, int& k0, int& nj, int& nk) {
   nk = khi >= klo ? khi - klo + 1 : 0;
 }
 
-template <int TX, int TY, int TZ>
-void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
+// y: the forward result (z itself in FP64; an FP32 buffer for the mixed sweeps).
+template <int TX, int TY, int TZ, class T>
+void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D, const double* r, T* y, double* z,
                              const PcgState* st, hipStream_t s) {
   constexpr int NT = TX * TY * TZ;
   Geo tg{};
@@ -2696,29 +2708,40 @@ void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D
     int j0, k0, nj, nk;
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
-    hipLaunchKernelGGL((k_ic0_tile<true, TX, TY, TZ>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, r, z, P, j0, k0, nj,
-                       nk, tg.n[0], st);
+    hipLaunchKernelGGL((k_ic0_tile<true, TX, TY, TZ, T, double, T>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, r, y,
+                       P, j0, k0, nj, nk, tg.n[0], st);
   }
   for (int P = diagonals - 1; P >= 0; --P) {
     int j0, k0, nj, nk;
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
-    hipLaunchKernelGGL((k_ic0_tile<false, TX, TY, TZ>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, (const double*)z, z,
-                       P, j0, k0, nj, nk, tg.n[0], st);
+    hipLaunchKernelGGL((k_ic0_tile<false, TX, TY, TZ, T, T, double>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D,
+                       (const T*)y, z, P, j0, k0, nj, nk, tg.n[0], st);
   }
+}
+
+template <class T>
+void launch_ic0_tiles_t(const Geo& g, const uint8_t* codes, const double* D, const double* r, T* y, double* z,
+                        const PcgState* st, hipStream_t s) {
+  // an axis of at most 2 cells: flat 16 x 16 tiles across the other two axes
+  if (g.n[1] <= 2)
+    launch_ic0_tiles_shaped<16, 2, 16, T>(g, codes, D, r, y, z, st, s);
+  else if (g.n[2] <= 2)
+    launch_ic0_tiles_shaped<16, 16, 2, T>(g, codes, D, r, y, z, st, s);
+  else if (g.n[0] <= 2)
+    launch_ic0_tiles_shaped<2, 16, 16, T>(g, codes, D, r, y, z, st, s);
+  else
+    launch_ic0_tiles_shaped<8, 8, 8, T>(g, codes, D, r, y, z, st, s);
 }
 
 void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
                       const PcgState* st, hipStream_t s) {
-  // an axis of at most 2 cells: flat 16 x 16 tiles across the other two axes
-  if (g.n[1] <= 2)
-    launch_ic0_tiles_shaped<16, 2, 16>(g, codes, D, r, z, st, s);
-  else if (g.n[2] <= 2)
-    launch_ic0_tiles_shaped<16, 16, 2>(g, codes, D, r, z, st, s);
-  else if (g.n[0] <= 2)
-    launch_ic0_tiles_shaped<2, 16, 16>(g, codes, D, r, z, st, s);
-  else
-    launch_ic0_tiles_shaped<8, 8, 8>(g, codes, D, r, z, st, s);
+  launch_ic0_tiles_t<double>(g, codes, D, r, z, z, st, s);
+}
+
+void launch_ic0_tiles_mixed(const Geo& g, const uint8_t* codes, const double* D, const double* r, float* y, double* z,
+                            const PcgState* st, hipStream_t s) {
+  launch_ic0_tiles_t<float>(g, codes, D, r, y, z, st, s);
 }
 
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,

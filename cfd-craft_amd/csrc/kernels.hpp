@@ -37,6 +37,11 @@ void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const doubl
 // The same sweeps as a tiled wavefront (one launch per tile diagonal).
 void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
                       const PcgState* st, hipStream_t s);
+// The same sweeps in FP32 (the mixed-precision preconditioner): the factor
+// entries formed in FP64 and rounded, the forward result held in y (FP32), z
+// written in FP64.
+void launch_ic0_tiles_mixed(const Geo& g, const uint8_t* codes, const double* D, const double* r, float* y, double* z,
+                            const PcgState* st, hipStream_t s);
 // One V-cycle: fine level from the codes, coarse levels lv[0..nlev) (device copy dlv),
 // levels from tail_first on inside one workgroup.
 template <class T, class TB>

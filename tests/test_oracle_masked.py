@@ -87,14 +87,12 @@ def porous_periodic():
 
 
 def courtyards():
-    """Three sealed courtyards and an open street in one grid: two full-height
-    closed rings and a closed box, plus an outlet-anchored remainder."""
+    """Three sealed courtyards (full-height rings of wall) beside an
+    outlet-anchored street in one grid."""
     blocks = []
-    for x0, y0, w in ((3, 3, 8), (14, 4, 7)):
+    for x0, y0, w in ((3, 3, 8), (14, 4, 7), (4, 12, 7)):
         blocks += [Block([x0, y0, 0.0], [x0 + w, y0 + 1.0, 9.0]), Block([x0, y0 + w - 1.0, 0.0], [x0 + w, y0 + w, 9.0]),
                    Block([x0, y0, 0.0], [x0 + 1.0, y0 + w, 9.0]), Block([x0 + w - 1.0, y0, 0.0], [x0 + w, y0 + w, 9.0])]
-    blocks.append(Block([4.0, 13.0, 0.0], [9.0, 18.0, 9.0]))
-    blocks.append(Block([5.0, 14.0, 0.0], [8.0, 17.0, 4.0]))
     blocks = [Block([float(v) for v in b.min_m], [float(v) for v in b.max_m]) for b in blocks]
     return Case(cells=[24, 20, 9], size_m=[24.0, 20.0, 9.0], boundaries=[OPEN, WALL, WALL], velocity_m_s=[1.0, 0, 0],
                 blocks=blocks, preconditioner=MULTIGRID)
