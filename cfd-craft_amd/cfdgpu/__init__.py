@@ -177,6 +177,7 @@ def lib():
             "cfd_center_nullspaces": (I, [P, P]),
             "cfd_solve_scaled": (I, [P, P, P, D, I64, POLL_FN, P, P, P]),
             "cfd_project": (I, [P, P, P, D, D, I64, POLL_FN, P, P]),
+            "cfd_project_static": (I, [P, P, P, D, D, I64, POLL_FN, P, P]),
             "cfd_projection_rhs": (I, [P, P, D, P, P]),
             "cfd_precondition": (I, [P, P, P]),
             "cfd_apply_device": (I, [P, P, P]),
@@ -517,9 +518,10 @@ class Grid:
         _check(lib().cfd_projection_rhs(self.h, _ptr(u), float(dt), _ptr(b), _ptr(t)))
         return b, t
 
-    def project(self, u, p, dt, tol=None, max_iterations=None, control=None):
+    def project(self, u, p, dt, tol=None, max_iterations=None, control=None, static=False):
         """Returns (u, p, info); the inputs are not modified (the reference mutates in place
-        and only on success, pressure.rs:468)."""
+        and only on success, pressure.rs:468). static=True calls cfd_project_static:
+        no device allocation (u and p staged in the library's static device array)."""
         tol = self.tolerance if tol is None else tol
         max_iterations = self.max_iterations if max_iterations is None else max_iterations
         u = _host_vector(u, self.face_count, "u", copy=True)
@@ -531,8 +533,8 @@ class Grid:
         with outer:
             info = self._measured(
                 control,
-                lambda info: lib().cfd_project(self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn,
-                                               None, ctypes.byref(info)),
+                lambda info: (lib().cfd_project_static if static else lib().cfd_project)(
+                    self.h, _ptr(u), _ptr(p), float(dt), float(tol), int(max_iterations), fn, None, ctypes.byref(info)),
             )
         return u, p, info
 

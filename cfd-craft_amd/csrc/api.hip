@@ -220,6 +220,7 @@ struct cfd_grid {
     double tol = 0.0;
     int64_t max = 0;
     int rc = 0;
+    bool graphs = true;  // false: iterations launched kernel by kernel (nothing instantiated)
     std::string error;
     cfd_linear_info info{};
   } job;
@@ -1044,7 +1045,7 @@ void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local);
 int enqueue_chunk(cfd_grid* g, int len) {
   cfd_grid::Job& J = g->job;
   const int parity = (int)(J.enq & 1);
-  if (g->use_graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
+  if (g->use_graphs && J.graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
     hipGraphExec_t seg = nullptr;
     int rc = segment_graph(g, len, parity, &seg);
     if (rc) return rc;
@@ -1087,7 +1088,7 @@ int64_t expected_iterations(const PcgState& s, double tol, int64_t queued) {
 }
 
 int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations, double* d_x,
-              cfd_poll_fn poll = nullptr, void* user = nullptr) {
+              cfd_poll_fn poll = nullptr, void* user = nullptr, bool graphs = true) {
   if (!(tol > 0.0) || max_iterations <= 0) {
     set_error("pressure tolerance must be positive and the iteration limit nonzero (CFD-E008/E009)");
     return CFD_EINVAL;
@@ -1095,6 +1096,7 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   HIP_TRY(hipSetDevice(g->device));
   cfd_grid::Job& J = g->job;
   J = cfd_grid::Job();
+  J.graphs = graphs;
   J.b = d_b;
   J.scale = d_scale;
   J.x = d_x;
@@ -1265,8 +1267,8 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
 }
 
 int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
-                 cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info) {
-  int rc = job_begin(g, d_b, d_scale, tol, max_iterations, d_x, poll, user);
+                 cfd_poll_fn poll, void* user, double* d_x, cfd_linear_info* info, bool graphs = true) {
+  int rc = job_begin(g, d_b, d_scale, tol, max_iterations, d_x, poll, user, graphs);
   if (rc) return rc;
   for (;;) {
     rc = job_poll(g, true, poll, user);
@@ -1613,17 +1615,90 @@ int cfd_projection_rhs(cfd_grid* g, const double* u, double dt, double* b, doubl
   return CFD_OK;
 }
 
-int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
-                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info) {
+}  // extern "C"
+
+namespace {
+// pressure::project on device buffers u (faces) and p (cells); graphs false
+// launches the iterations kernel by kernel.
+int project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
+                   cfd_poll_fn poll, void* poll_user, cfd_linear_info* info, bool graphs) {
   HIP_TRY(hipSetDevice(g->device));
   launch_projection_rhs(g->geo, g->fgeo, g->codes.p, d_u, dt, g->b.p, g->scale.p, g->stream);
-  int rc = solve_device(g, g->b.p, g->scale.p, tol, max_iterations, poll, poll_user, g->phi.p, info);
+  int rc = solve_device(g, g->b.p, g->scale.p, tol, max_iterations, poll, poll_user, g->phi.p, info, graphs);
   if (rc) return rc;
   launch_project_faces(g->geo, g->fgeo, g->codes.p, g->phi.p, dt, d_u, g->stream);
   launch_add(g->geo.N, d_p, g->phi.p, g->stream);
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
   enqueue_gauge(g, d_p, nullptr, 0, g->mean_x.p, nullptr);
   launch_center(g->geo.N, g->codes.p, d_p, g->mean_x.p, cidx, d_p, g->stream);
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
+// Staging memory of cfd_project_static: one array per GPU (every device that
+// loads the library gets its own copy of a __device__ variable, held until the
+// process ends): the faces and then the cells of one grid, kStaticDoubles
+// values at most (48 MiB; about 1.4 M cells).
+constexpr size_t kStaticDoubles = size_t(6) << 20;
+static __device__ __align__(256) double scratch[kStaticDoubles];
+constexpr int kMaxDevices = 64;
+// the device address of each GPU's array, from hipGetSymbolAddress with that
+// GPU current (the host-side name `scratch` is not a device address)
+double* scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("cfd_project_static: device ordinal out of range");
+    return CFD_EINVAL;
+  }
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("cfd_project_static: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return CFD_EDEVICE;
+    }
+    scratch_ptr[device] = (double*)p;
+  }
+  *out = scratch_ptr[device];
+  return CFD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double tol, int64_t max_iterations,
+                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info) {
+  return project_device(g, d_u, d_p, dt, tol, max_iterations, poll, poll_user, info, true);
+}
+
+int cfd_project_static(cfd_grid* g, double* u, double* p, double dt, double tol, int64_t max_iterations,
+                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info) {
+  if (!g || !u || !p) {
+    set_error("cfd_project_static: null argument");
+    return CFD_EINVAL;
+  }
+  const size_t faces = (size_t)g->fgeo.total, cells = (size_t)g->geo.N;
+  if (faces + cells > kStaticDoubles) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "cfd_project_static: the grid's %zu faces and %zu cells need %zu doubles of staging; the static "
+                  "scratch holds %zu (use cfd_project)",
+                  faces, cells, faces + cells, kStaticDoubles);
+    set_error(buf);
+    return CFD_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  double* base = nullptr;
+  if (int rc = static_scratch(g->device, &base)) return rc;
+  double* du = base;
+  double* dp = base + faces;
+  HIP_TRY(hipMemcpyAsync(du, u, faces * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(dp, p, cells * sizeof(double), hipMemcpyHostToDevice, g->stream));
+  int rc = project_device(g, du, dp, dt, tol, max_iterations, poll, poll_user, info, false);
+  if (rc) return rc;  // u and p untouched on error
+  HIP_TRY(hipMemcpyAsync(u, du, faces * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(p, dp, cells * sizeof(double), hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   return CFD_OK;
 }

@@ -232,6 +232,20 @@ int cfd_solve_scaled(cfd_grid* g, const double* b, const double* roundoff_scale,
  * reference's face order (x-faces, y-faces, z-faces, each x fastest). */
 int cfd_project(cfd_grid* g, double* u, double* p, double dt, double tol, int64_t max_iterations, cfd_poll_fn poll,
                 void* poll_user, cfd_linear_info* info);
+/* The same projection with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture elsewhere in the process, tight
+ * serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. cfd_project stages u and p in device buffers it allocates per
+ * call; this entry point stages them in a static device array of 6 Mi doubles
+ * (48 MiB) that the library holds on every GPU it runs on until the process
+ * ends, and launches the PCG iterations kernel by kernel (no graph is
+ * instantiated). A grid whose faces plus cells exceed 6 Mi values (about 1.4 M
+ * cells) is refused with CFD_EINVAL before any device work. All calls on one
+ * GPU share that GPU's array: calls that overlap on one GPU (two threads, two
+ * grids) race and must be serialized by the caller. Same contract as
+ * cfd_project: u and p are left unchanged on any error. */
+int cfd_project_static(cfd_grid* g, double* u, double* p, double dt, double tol, int64_t max_iterations,
+                       cfd_poll_fn poll, void* poll_user, cfd_linear_info* info);
 /* projection_rhs (pressure.rs:440-459). */
 int cfd_projection_rhs(cfd_grid* g, const double* u, double dt, double* b, double* terms);
 /* One preconditioner application z = M r: Factor::apply (pressure.rs:223-241)
