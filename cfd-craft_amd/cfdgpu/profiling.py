@@ -15,14 +15,9 @@ selected, not lazily inside the first solve as the reference does (pressure.rs:7
 The first solve after a build reports it (``hierarchy_builds`` = 1) and its span
 receives a nested ``pressure_multigrid_setup`` phase with the build's measured host
 seconds, so the setup calls still equal the reported builds.
-
-:func:`check_pressure` restates the pressure part of the reference's profile checker
-(validation/check_profile.py:127-180, 213-323, 387-393) for a profile of pressure
-spans alone.
 """
 from __future__ import annotations
 
-import math
 import threading
 import time
 
@@ -218,122 +213,3 @@ def record_solve(span: Span, info: dict, multigrid: bool, build_began: float | N
     if work["hierarchy_builds"]:
         began = build_began if build_began is not None else time.perf_counter()
         span.nested("pressure_multigrid_setup", float(info["hierarchy_build_seconds"]), began)
-
-
-# ------------------------------------------------------------------ the checker
-def _require(ok, field, message):
-    if not ok:
-        raise ValueError(f"CFD-B003 profile.{field}: {message}")
-
-
-def _number(value, field, integer=False):
-    _require(type(value) is int if integer else type(value) in (int, float), field, "invalid numeric type")
-    _require((type(value) is int or math.isfinite(value)) and value >= 0, field, "finite nonnegative value required")
-    return value
-
-
-def check_pressure(profile: dict, selected: str, stopped: bool = False, roots=None) -> None:
-    """Raise ``ValueError`` (``CFD-B003 profile.<field>: ...``) unless ``profile`` keeps
-    the reference checker's rules for pressure work: the timing accounting
-    (check_profile.py:127-180, with ``roots`` the outermost phases, by default those
-    of the pressure path present), the pressure operator count
-    (check_profile.py:387-393) and the multigrid work (check_profile.py:213-323).
-    ``selected`` is the case's ``numerics.pressure_preconditioner``; ``stopped`` a
-    run that ended cancelled, at its wall-time limit or diverged."""
-    _require(profile.get("schema_version") == SCHEMA, "timing.schema_version", "unsupported")
-    elapsed = _number(profile["elapsed_s"], "timing.elapsed_s")
-    phases = profile["phases"]
-    _require(isinstance(phases, dict), "timing.phases", "expected object")
-    for name, row in phases.items():
-        for field in ("calls",) + COUNTERS:
-            _number(row[field], f"timing.phases.{name}.{field}", integer=True)
-        for field in ("inclusive_s", "exclusive_s"):
-            _number(row[field], f"timing.phases.{name}.{field}")
-        _require(row["exclusive_s"] <= row["inclusive_s"], f"timing.phases.{name}", "exclusive exceeds inclusive")
-    epsilon = max(1e-9, elapsed * 1e-9)
-    total = sum(row["exclusive_s"] for row in phases.values())
-    if roots is None:
-        roots = ["pressure_projection"] if "pressure_projection" in phases else ["pressure_solve"]
-    top = sum(phases.get(r, {}).get("inclusive_s", 0.0) for r in roots)
-    _require(abs(total - top) <= epsilon, "timing.accounting", "nested time is counted twice or missing")
-    _require(abs(total - profile["accounted_exclusive_s"]) <= epsilon, "timing.accounted_exclusive_s", "sum differs")
-    _require(
-        _number(profile["unattributed_s"], "timing.unattributed_s") + total <= elapsed + epsilon,
-        "timing.elapsed_s",
-        "accounting exceeds elapsed",
-    )
-    _require(abs(total + profile["unattributed_s"] - elapsed) <= epsilon, "timing.unattributed_s", "remainder differs")
-
-    calls = {name: row.get("calls", 0) for name, row in phases.items()}
-    linear = phases.get("pressure_solve", {})
-    multigrid = linear.get("multigrid")
-    if multigrid is not None:
-        _require(selected == "multigrid", "work.multigrid", "case selects another preconditioner")
-        _require(isinstance(multigrid, dict), "work.multigrid", "expected object")
-        _require(
-            set(MULTIGRID_FIELDS).issubset(multigrid) and set(multigrid).issubset(set(MULTIGRID_FIELDS) | set(BLOCK_FIELDS)),
-            "work.multigrid",
-            "unexpected or missing fields",
-        )
-        for field in MULTIGRID_FIELDS:
-            _number(multigrid[field], f"work.multigrid.{field}", integer=True)
-        _require(
-            all(f in multigrid for f in BLOCK_FIELDS) or all(f not in multigrid for f in BLOCK_FIELDS),
-            "work.multigrid.block",
-            "block counters must occur together",
-        )
-        for field in BLOCK_FIELDS:
-            _number(multigrid.get(field, 0), f"work.multigrid.{field}", integer=True)
-        updates = multigrid.get("block_smoothing_cell_updates", 0)
-        solves = multigrid.get("block_solves", 0)
-        visits = multigrid.get("block_level_cell_visits", 0)
-        _require(
-            (updates == solves == 0) or (0 < 2 * solves <= updates <= multigrid["smoothing_cell_updates"]),
-            "work.multigrid.block",
-            "block work exceeds total smoothing or has no solves",
-        )
-        _require(
-            updates == 2 * visits and visits <= multigrid["residual_cell_updates"],
-            "work.multigrid.block",
-            "block passes differ from fixed two-sweep work",
-        )
-        _require(
-            multigrid["cycles"] == linear.get("preconditioner_applications", 0) == multigrid["coarse_solves"],
-            "work.multigrid.cycles",
-            "cycle/application/coarse-solve counts differ",
-        )
-        _require(
-            multigrid["hierarchy_builds"] == calls.get("pressure_multigrid_setup", 0)
-            and not calls.get("pressure_factor_setup", 0),
-            "work.multigrid.hierarchy_builds",
-            "setup counts differ or IC(0) fallback occurred",
-        )
-        _require(
-            multigrid["residual_cell_updates"]
-            == multigrid["restriction_cell_updates"]
-            == multigrid["prolongation_cell_updates"]
-            and multigrid["smoothing_cell_updates"] == 4 * multigrid["residual_cell_updates"] - 2 * visits,
-            "work.multigrid.updates",
-            "fixed-cycle work differs",
-        )
-        _require(
-            (
-                multigrid["hierarchy_levels"] > 0
-                and multigrid["hierarchy_nonzeros"] >= multigrid["hierarchy_cells"] > 0
-                and multigrid["hierarchy_bytes"] > 0
-            )
-            or (stopped and multigrid["cycles"] == 0),
-            "work.multigrid.hierarchy",
-            "missing hierarchy dimensions/storage",
-        )
-    else:
-        _require(
-            selected != "multigrid" or not calls.get("pressure_solve", 0),
-            "work.multigrid",
-            "missing selected preconditioner work",
-        )
-    _require(
-        linear.get("operator_applications", 0) == linear.get("work_units", 0) + linear.get("true_residual_checks", 0),
-        "work.operator_applications",
-        "operator applications differ from iterations plus true-residual checks",
-    )

@@ -100,7 +100,7 @@ def test_warm_start_reuses_the_resident_solution(gpu_available):
 def test_admitted_grid_takes_options_and_profiles(gpu_available):
     """A grid admitted by the server is a full Grid: options, direct solves with a
     profiling control, and the profile checker's rules."""
-    from cfdgpu import profiling
+    from profile_rules import check
 
     case, b, t, tol = W.served_request(3, 2)
     server = cfdgpu.Server(max_active=2)
@@ -109,6 +109,6 @@ def test_admitted_grid_takes_options_and_profiles(gpu_available):
     ctl = cfdgpu.Control(profile=True)
     x, info = g.solve_scaled(b, t, tol, 2000, control=ctl)
     assert info["relative_residual"] <= 10 * tol + 1e-12
-    profiling.check_pressure(ctl.profile(), "multigrid" if case.preconditioner == cfdgpu.MULTIGRID else "ic0")
+    check(ctl.profile(), "multigrid" if case.preconditioner == cfdgpu.MULTIGRID else "ic0")
     server.release(g)
     server.close()

@@ -1,10 +1,10 @@
-"""The phase profile of pressure work (profiling.rs, multigrid.rs:74-130) and the
-restated profile checker (validation/check_profile.py:127-180, 213-323, 387-393).
+"""The phase profile of pressure work (profiling.rs, multigrid.rs:74-130) and its
+consistency rules (tests/profile_rules.py, after validation/check_profile.py).
 
 CPU: the collector's nesting arithmetic; the oracle, which counts MultigridWork
 where the reference's cycles increment it (multigrid.rs:672-725, masked.rs:301-341),
-yields profiles that the checker accepts, and the reference's own corruptions of them
-(validation/test_profiling.py:46-185) are rejected.
+yields profiles that keep the rules, and the reference's own corruptions of them
+(validation/test_profiling.py:46-185) break the rule they should.
 
 GPU: the device library's work counters equal the oracle's for the reference cycle
 (the same hierarchy and iteration count), its profiles pass the checker for every
@@ -20,6 +20,7 @@ import oracle
 from cfdgpu import profiling
 from cfdgpu.case import IC0, MULTIGRID, OPEN, PERIODIC, WALL, Case
 
+from profile_rules import broken, check
 from test_oracle_goldens import aniso_case, aniso_rhs
 
 ANISO = {"x": "wall", "cells": [12, 4, 12], "size_m": [1.0, 1.0, 0.1]}
@@ -73,7 +74,7 @@ def test_nested_spans_account_exclusive_time_once():
     assert abs(p["accounted_exclusive_s"] - outer["inclusive_s"]) < 1e-12
     assert abs(p["unattributed_s"] + p["accounted_exclusive_s"] - p["elapsed_s"]) < 1e-12
     assert inner["work_units"] == 4 and "multigrid" not in inner
-    profiling.check_pressure(p, "ic0")
+    check(p, "ic0")
 
 
 def test_setup_reported_from_before_a_span_extends_it():
@@ -113,57 +114,56 @@ def test_merge_keeps_hierarchy_maxima_and_sums_work():
 def test_oracle_point_work_passes_and_corruptions_fail():
     """validation/test_profiling.py:46-87 on the 8x2x8 lid cavity with multigrid."""
     p = oracle_profile(lid([8, 2, 8], WALL))
-    profiling.check_pressure(p, "multigrid")
+    check(p, "multigrid")
     work = p["phases"]["pressure_solve"]["multigrid"]
     assert work["hierarchy_builds"] == 1 and work["cycles"] > 0
     assert not any(f in work for f in profiling.BLOCK_FIELDS)
-    for field in ("cycles", "hierarchy_builds", "smoothing_cell_updates", "coarse_solves", "restriction_cell_updates"):
+    for field, rule in (("cycles", "mg_cycles"), ("hierarchy_builds", "mg_setup"),
+                        ("smoothing_cell_updates", "mg_updates"), ("coarse_solves", "mg_cycles"),
+                        ("restriction_cell_updates", "mg_updates")):
         changed = copy.deepcopy(p)
         changed["phases"]["pressure_solve"]["multigrid"][field] += 1
-        with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-            profiling.check_pressure(changed, "multigrid")
+        assert broken(changed, "multigrid") == {rule}, field
     changed = copy.deepcopy(p)
     del changed["phases"]["pressure_solve"]["multigrid"]
-    with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-        profiling.check_pressure(changed, "multigrid")
-    with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-        profiling.check_pressure(p, "ic0")
+    assert broken(changed, "multigrid") == {"choice"}
+    assert broken(p, "ic0") == {"choice"}
+    with pytest.raises(AssertionError, match="choice"):
+        check(p, "ic0")
 
 
 def test_oracle_block_work_passes_and_corruptions_fail():
     """validation/test_profiling.py:152-184 and tests/multigrid.rs:297-331 on the
     12x4x12 cavity of size [1, 1, 0.1]: every smoothed level takes line blocks."""
     p = oracle_profile(aniso_case(ANISO, "mg"), tol=1e-14)
-    profiling.check_pressure(p, "multigrid")
+    check(p, "multigrid")
     work = p["phases"]["pressure_solve"]["multigrid"]
     assert work["block_solves"] > 0
     assert work["block_smoothing_cell_updates"] == 2 * work["block_level_cell_visits"]
     assert work["smoothing_cell_updates"] == work["block_smoothing_cell_updates"]
-    for field, value in (("block_solves", 0), ("block_smoothing_cell_updates", True),
-                         ("block_smoothing_cell_updates", work["smoothing_cell_updates"] + 1)):
+    for field, value, rule in (("block_solves", 0, "mg_blocks"), ("block_smoothing_cell_updates", True, "mg_fields"),
+                               ("block_smoothing_cell_updates", work["smoothing_cell_updates"] + 1, "mg_blocks")):
         changed = copy.deepcopy(p)
         changed["phases"]["pressure_solve"]["multigrid"][field] = value
-        with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-            profiling.check_pressure(changed, "multigrid")
+        assert rule in broken(changed, "multigrid"), (field, value)
     changed = copy.deepcopy(p)
     del changed["phases"]["pressure_solve"]["multigrid"]["block_solves"]
-    with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-        profiling.check_pressure(changed, "multigrid")
+    assert broken(changed, "multigrid") == {"mg_fields"}
 
 
 def test_oracle_ic0_work_and_operator_count():
     p = oracle_profile(lid([9, 2, 7], OPEN, IC0))
-    profiling.check_pressure(p, "ic0")
+    check(p, "ic0")
     changed = copy.deepcopy(p)
     changed["phases"]["pressure_solve"]["operator_applications"] += 1
-    with pytest.raises(ValueError, match="CFD-B003 profile.work.operator_applications"):
-        profiling.check_pressure(changed, "ic0")
+    assert broken(changed, "ic0") == {"operators"}
     changed = copy.deepcopy(p)
     changed["phases"]["pressure_solve"]["exclusive_s"] = changed["phases"]["pressure_solve"]["inclusive_s"] * 2 + 1
-    with pytest.raises(ValueError, match="CFD-B003 profile.timing"):
-        profiling.check_pressure(changed, "ic0")
-    with pytest.raises(ValueError, match="CFD-B003 profile.work.multigrid"):
-        profiling.check_pressure(p, "multigrid")
+    assert broken(changed, "ic0") == {"format"}
+    changed = copy.deepcopy(p)
+    changed["unattributed_s"] += 1.0
+    assert broken(changed, "ic0") == {"time"}
+    assert broken(p, "multigrid") == {"choice"}
 
 
 def test_oracle_work_follows_the_hierarchy():
@@ -220,7 +220,7 @@ def test_device_reference_cycle_work_equals_oracle(gpu_available, spec_name):
     assert [{k: v for k, v in b.items() if k != "allocated_bytes"} for b in dev.get("block_smoothers", [])] == \
         shape["block_smoothers"]
     assert dev["allocated_bytes"] == got["hierarchy_bytes"] > 0
-    profiling.check_pressure(ctl.profile(), "multigrid")
+    check(ctl.profile(), "multigrid")
 
 
 @pytest.mark.gpu
@@ -245,7 +245,7 @@ def test_device_profile_reuse_and_projection(gpu_available, cycle):
         u = np.cos(0.21 * np.arange(g.face_count))
         g.project(u, np.zeros(g.count), 0.125, 1e-12, 2000, control=ctl)
     p = ctl.profile()
-    profiling.check_pressure(p, "multigrid", roots=["solver_run"])
+    check(p, "multigrid", roots=["solver_run"])
     assert p["phases"]["pressure_multigrid_setup"]["calls"] == 1
     assert p["phases"]["pressure_projection"]["calls"] == 1
     solve = p["phases"]["pressure_solve"]
@@ -269,6 +269,6 @@ def test_device_ic0_profile_has_no_multigrid_work(gpu_available):
     i = np.arange(g.count, dtype=float)
     g.solve(g.apply(np.sin(0.37 * i)), 1e-12, 2000, control=ctl)
     p = ctl.profile()
-    profiling.check_pressure(p, "ic0")
+    check(p, "ic0")
     assert "multigrid" not in p["phases"]["pressure_solve"]
     assert "pressure_multigrid_setup" not in p["phases"]
