@@ -1,26 +1,35 @@
 #!/usr/bin/env python3
 """Pressure-solve benchmark on one MI355X.
 
-Workload: the urban block (128 x 128 x 64 cells, h = 2 m, 24 seeded ground boxes,
-about 15 % solid, open x with a log-law inflow, outlet anchoring), right-hand
-side -div(u) of a seeded perturbed log-law face velocity. A step is one full
-pressure solve (PCG with the red-black multigrid V(2,2) preconditioner on the
-reference's aggregation hierarchy, FP64) from the right-hand side resident in
-HBM to the solution in HBM.
+Headline workload: one direction (200 degrees) of the street-canyon wind
+campaign at its full size: 384x384x160 cells (23.6M, h = 1 m), 400 convex
+4-8-vertex prisms on 20 x 20 lots between 3 m streets (about 20 % solid), x and
+y open with the log-law wind, ground_open_top z. A step is one pressure solve
+(PCG with the red-black multigrid V(2,2) preconditioner on the reference's
+aggregation hierarchy, plain FP64, from x = 0) of one right-hand side of the
+campaign's chain b_{k+1} = b_k + 0.02 rms(b_k) xi, every right-hand side
+distinct and resident in HBM before timing; value = solves/s at relative
+residual 1e-8. Mixed precision (FP32 V-cycle inside FP64 PCG) and warm starts
+from the resident previous solution run on the same chain beside it.
 
-value = solves/s at relative residual 1e-8 (plain FP64 path); the same workload
-at 1e-10 and in mixed precision (FP32 V-cycle inside FP64 PCG) is reported
-beside it. roofline: the fine-level smoother half-sweep, timed with HIP events
-on the grid's stream. cpu_baseline: the oracle (serial FP64 restatement of the
-reference's own multigrid PCG) on the same workload, one core.
+roofline: the kernel class with the largest share of a solve's device time,
+measured inside solves of the same chain (every launch of the class bracketed
+by a pair of HIP events on the grid's stream): algorithmic bytes over summed
+launch durations. traffic: HBM bytes per launch of that kernel from the
+committed rocprofv3 --pmc summary. cpu_baseline: the oracle (the reference's
+serial FP64 multigrid PCG restated in C++), one core of this host, on the
+urban block: one cold solve and the median of three reused ones.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
 import argparse
 import json
 import math
+import multiprocessing
 import os
+import statistics
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -31,37 +40,376 @@ for p in (ROOT, os.path.join(ROOT, "cfd-craft_amd")):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+HEADLINE_DIRECTION = 200.0
+HEADLINE_TOL = 1e-8
+METRIC = "Poisson solves/s at rel-res 1e-8 + HBM GB/s %peak; p50/p99 latency served vs plain"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_rb_gs2_street_canyon_384x384x160.txt")
+# The reference's published single-thread CPU times (ms, reused solve; AMD EPYC 9V74):
+# docs/evidence/pressure-multigrid.md:34-38 (manufactured kernels, tol 1e-11, IC(0) / MG)
+# and docs/evidence/native-cpu-performance.md:87 (canyon-93k IC(0) solve, median warm).
+PUBLISHED_CPU_MS = {
+    "cavity24": ([24, 2, 24], 1.292, 0.928),
+    "cavity48": ([48, 2, 48], 13.007, 3.646),
+    "cavity96": ([96, 2, 96], 102.410, 16.864),
+    "cube24": ([24, 24, 24], 19.565, 13.059),
+    "cube48": ([48, 48, 48], 390.576, 152.088),
+}
+CUBE48_IC0_MS_PER_ITERATION = 390.576 / 113
+CANYON93K_IC0_CPU_MS = 393.146
 
 
-def cpu_baseline(case, b, t, tol):
-    """The oracle on the same system: the reference's multigrid PCG, serial FP64,
-    pinned to one core; hierarchy built by a first (cold) solve, then one reused
-    solve is timed (performance_probe.rs:110-136 protocol, one sample)."""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(math.ceil(q * len(v))) - 1)]
+
+
+# ---------------------------------------------------------------- CPU side (no GPU touched)
+def host_cpu():
+    model = None
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def _timed_protocol(make_grid, solve, reused=3):
+    """performance_probe.rs:110-136: build the grid, one cold solve (factor or
+    hierarchy built inside it), then `reused` solves; ms of the cold one and the
+    median of the reused ones."""
+    t0 = time.perf_counter()
+    g = make_grid()
+    info = solve(g)
+    cold = time.perf_counter() - t0
+    times = []
+    for _ in range(reused):
+        t0 = time.perf_counter()
+        info = solve(g)
+        times.append(time.perf_counter() - t0)
+    return cold, statistics.median(times), info
+
+
+def cpu_baseline():
+    """The oracle pinned to one core (the reference's pressure path is serial).
+    value: reused multigrid-PCG solves/s of the urban block at rel-res 1e-8; beside
+    it the lid cavity (IC(0), 1e-10) and the cube48 IC(0) time per iteration
+    against the reference's published 3.46 ms on an EPYC 9V74."""
     import oracle
+    from cfdgpu import workloads as W
+    from cfdgpu.case import IC0
 
     old = os.sched_getaffinity(0)
-    core = min(old)
-    os.sched_setaffinity(0, {core})
+    os.sched_setaffinity(0, {min(old)})
+    rows = {}
     try:
-        g = oracle.Grid(case)
-        t0 = time.perf_counter()
-        g.mg_stats()  # hierarchy build (cold set-up, not part of the reused solve)
-        setup = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        x, info = g.solve_scaled(b, t, tol, 2000)
-        dt = time.perf_counter() - t0
+        case, b, t = W.urban_block()
+        cold, warm, info = _timed_protocol(lambda: oracle.Grid(case), lambda g: g.solve_scaled(b, t, 1e-8, 3000)[1])
+        rows["urban_block_128x128x64_mg_1e-8"] = {"cold_s": round(cold, 3), "reused_median_s": round(warm, 3),
+                                                  "iterations": info["iterations"]}
+        urban = (warm, info["iterations"])
+        c1, b1, t1 = W.lid_cavity_48()
+        cold, warm, info = _timed_protocol(lambda: oracle.Grid(c1), lambda g: g.solve_scaled(b1, t1, 1e-10, 1200)[1])
+        rows["lid_cavity_48_ic0_1e-10"] = {"cold_s": round(cold, 3), "reused_median_s": round(warm, 3),
+                                           "iterations": info["iterations"]}
+        cube = W.cavity([48, 48, 48], IC0)
+        g = oracle.Grid(cube)
+        ex = g.center_nullspaces(W.manufactured(cube, g.fluid()))
+        bc = g.apply(ex)
+        cold, warm, info = _timed_protocol(lambda: oracle.Grid(cube), lambda g: g.solve(bc, 1e-11, 10000)[1])
+        rows["cube48_ic0_1e-11"] = {"cold_s": round(cold, 3), "reused_median_s": round(warm, 3),
+                                    "iterations": info["iterations"],
+                                    "ms_per_iteration": round(1e3 * warm / info["iterations"], 3),
+                                    "published_ms_per_iteration_epyc_9v74": round(CUBE48_IC0_MS_PER_ITERATION, 3)}
+        del case, b, t
     finally:
         os.sched_setaffinity(0, old)
-    return x, dict(
-        value=1.0 / dt,
-        unit="solves/s",
-        cores=1,
-        kind="port",
-        sample=f"one reused multigrid-PCG solve of the 128x128x64 urban block at rel-res {tol:g} "
-        f"(reference's masked-aggregation MG, {info['iterations']} iterations, {dt:.2f} s; hierarchy set-up "
-        f"{setup:.2f} s excluded)",
-        iterations=info["iterations"],
-    )
+    warm, its = urban
+    return {
+        "value": round(1.0 / warm, 4),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"urban block 128x128x64 (24 boxes, outlet x), the oracle's multigrid PCG (the reference's masked "
+                  f"aggregation cycle) at rel-res 1e-8, {its} iterations: median of 3 reused solves after a cold one "
+                  f"(the 384x384x160 canyon grid needs about 7 GB of host face records and minutes per solve on one "
+                  f"core)",
+        "host": host_cpu(),
+        "rows": rows,
+    }
+
+
+def _c3_oracle_child(path, core):
+    """Background process: the oracle's FP64 field of the prism city at rel-res
+    1e-10 (one core), saved for the GPU fields' error."""
+    try:
+        os.sched_setaffinity(0, {core})
+    except OSError:
+        pass
+    import oracle
+    from cfdgpu import workloads as W
+
+    case, b, t = W.urban_prisms()
+    t0 = time.perf_counter()
+    x, info = oracle.Grid(case).solve_scaled(b, t, 1e-10, 3000)
+    np.save(path, x)
+    with open(path + ".json", "w") as f:
+        json.dump({"iterations": info["iterations"], "seconds": time.perf_counter() - t0,
+                   "true_rel_residual": info["relative_residual"]}, f)
+
+
+# ---------------------------------------------------------------- GPU legs
+def chain(b, fluid, count, seed=5):
+    """b_0 = b, b_{k+1} = b_k + 0.02 rms(b_k) xi on the fluid cells (solid cells
+    have no equation; they stay zero)."""
+    rng = np.random.default_rng(seed)
+    out = [b]
+    for _ in range(count - 1):
+        prev = out[-1]
+        rms = math.sqrt(float(np.mean(prev[fluid] ** 2)))
+        out.append(prev + np.where(fluid, 0.02 * rms * rng.standard_normal(prev.size), 0.0))
+    return out
+
+
+def timed_chain(cfdgpu, g, dbs, dx, tol, precision, warm, warmup):
+    """Solves of dbs[:warmup] untimed, then of dbs[warmup:], each timed on the host;
+    the device is synchronized before and after the timed region."""
+    g.set_options(precision, warm_start=warm)
+    for db in dbs[:warmup]:
+        g.solve_device(db, dx, tol, 3000)
+    cfdgpu.synchronize()
+    lat, its = [], []
+    t0 = time.perf_counter()
+    for db in dbs[warmup:]:
+        s0 = time.perf_counter()
+        its.append(g.solve_device(db, dx, tol, 3000)["iterations"])
+        lat.append(time.perf_counter() - s0)
+    cfdgpu.synchronize()
+    return time.perf_counter() - t0, lat, its
+
+
+def probe_table(g, db, dx, tol, precision=0):
+    """Every kernel class of one solve, timed inside the solve: launches, device
+    seconds, algorithmic bytes and GB/s; the classes sorted by time."""
+    import cfdgpu
+
+    g.set_options(precision)
+    g.solve_device(db, dx, tol, 3000)  # the iteration count is learned: nothing queued past convergence
+    rows = {}
+    for name in cfdgpu.PROBE_KERNELS:
+        r = g.profile_solve(db, dx, tol, name)
+        if r["launches"] == 0:
+            continue
+        rows[name] = {"launches": r["launches"], "device_ms": round(1e3 * r["seconds"], 4),
+                      "avg_us": round(1e6 * r["seconds"] / r["launches"], 3),
+                      "bytes_per_launch": round(r["bytes"] / r["launches"]),
+                      "GBps": round(r["bytes"] / r["seconds"] / 1e9, 1) if r["bytes"] else None,
+                      "iterations": r["info"]["iterations"]}
+    return dict(sorted(rows.items(), key=lambda kv: -kv[1]["device_ms"]))
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel factor)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def headline(steps, warmup, extra):
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    log("street canyon 384x384x160: workload")
+    case, b, t = W.street_canyon(HEADLINE_DIRECTION)
+    fluid = W.fluid_mask(case)
+    t0 = time.perf_counter()
+    g = cfdgpu.Grid(case)
+    cfdgpu.synchronize()
+    setup = time.perf_counter() - t0
+    log(f"set-up {setup:.2f} s; chain of {warmup + steps} right-hand sides")
+    rhs = chain(b, fluid, warmup + steps)
+    dbs = [cfdgpu.DeviceArray.from_host(v) for v in rhs]
+    del rhs
+    dx = cfdgpu.DeviceArray(g.count)
+    el, lat, its = timed_chain(cfdgpu, g, dbs, dx, HEADLINE_TOL, cfdgpu.PRECISION_FP64, False, warmup)
+    x_plain = dx.numpy()
+    out = {"setup_s": setup, "elapsed": el, "lat": lat, "its": its, "cells": g.count,
+           "solid_fraction": 1.0 - float(fluid.mean())}
+    log(f"headline {steps / el:.2f} solves/s")
+    modes = {}
+    if extra:
+        for name, prec, warm in (("mixed_cold", cfdgpu.PRECISION_MIXED, False),
+                                 ("fp64_warm", cfdgpu.PRECISION_FP64, True),
+                                 ("mixed_warm", cfdgpu.PRECISION_MIXED, True)):
+            e2, l2, i2 = timed_chain(cfdgpu, g, dbs, dx, HEADLINE_TOL, prec, warm, warmup)
+            modes[name] = {"solves_per_s": round(steps / e2, 2), "mean_iterations": round(float(np.mean(i2)), 2),
+                           "p50_ms": round(1e3 * pct(l2, 0.5), 3), "p99_ms": round(1e3 * pct(l2, 0.99), 3)}
+            if name == "mixed_warm":
+                x_mixed_warm = dx.numpy()
+        # errors on the last right-hand side against a tight FP64 solve of it
+        g.set_options(cfdgpu.PRECISION_FP64)
+        ref = cfdgpu.DeviceArray(g.count)
+        g.solve_device(dbs[-1], ref, 1e-12, 3000)
+        xr = ref.numpy()
+        rel = lambda a: float(np.linalg.norm(a - xr) / np.linalg.norm(xr))  # noqa: E731
+        modes["field_rel_error_vs_fp64_1e-12"] = {"fp64_cold_1e-8": rel(x_plain), "mixed_warm_1e-8": rel(x_mixed_warm)}
+        del ref
+    log("kernel classes inside the solve")
+    out["probe"] = probe_table(g, dbs[-1], dx, HEADLINE_TOL)
+    out["modes"] = modes
+    return out, g, dbs, dx
+
+
+def second_direction():
+    """Another direction of the campaign (its own operator): cold set-up and one cold
+    FP64 solve."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.street_canyon(290.0)
+    t0 = time.perf_counter()
+    g = cfdgpu.Grid(case)
+    cfdgpu.synchronize()
+    setup = time.perf_counter() - t0
+    db = cfdgpu.DeviceArray.from_host(b)
+    dx = cfdgpu.DeviceArray(g.count)
+    info = g.solve_device(db, dx, HEADLINE_TOL, 3000)
+    return {"direction": 290.0, "setup_s": round(setup, 3), "iterations": info["iterations"],
+            "solve_ms": round(1e3 * info["seconds"], 3)}
+
+
+def prisms(c3_path):
+    """The 256x256x128 prism city (8.4M cells, 200 convex 4-8-vertex prisms, about
+    20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10, each field's
+    error against the oracle's FP64 solve at 1e-10 (computed on one core beside
+    the GPU legs) and against the GPU's own FP64 solve at 1e-12."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.urban_prisms()
+    t0 = time.perf_counter()
+    g = cfdgpu.Grid(case)
+    setup = time.perf_counter() - t0
+    db = cfdgpu.DeviceArray.from_host(b)
+    ds = cfdgpu.DeviceArray.from_host(t)
+    dx = cfdgpu.DeviceArray(g.count)
+    g.set_options(cfdgpu.PRECISION_FP64)
+    g.solve_device(db, dx, 1e-12, 3000, d_scale=ds)
+    ref = dx.numpy()
+    out = {"cells": g.count, "solid_fraction": round(1 - g.fluid_count / g.count, 4), "cold_setup_s": round(setup, 3)}
+    fields = {}
+    for tol in (1e-8, 1e-10):
+        for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
+            g.set_options(prec)
+            g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            el = time.perf_counter() - t0
+            x = dx.numpy()
+            key = f"{name}_tol_{tol:g}"
+            fields[key] = x
+            out[key] = {"solves_per_s": round(3 / el, 2), "iterations": info["iterations"],
+                        "true_rel_residual": info["relative_residual"],
+                        "field_rel_error_vs_gpu_fp64_1e-12": float(np.linalg.norm(x - ref) / np.linalg.norm(ref))}
+    g.set_options(cfdgpu.PRECISION_FP64)
+    return out, fields
+
+
+def urban(steps):
+    """The urban block (128x128x64, 24 boxes): red-black multigrid at the
+    configuration's 1e-10 and at 1e-8, plain and mixed; IC(0) and the reference's
+    own cycle on the same system."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+    from cfdgpu.case import IC0
+
+    case, b, t = W.urban_block()
+    g = cfdgpu.Grid(case)
+    db = cfdgpu.DeviceArray.from_host(b)
+    ds = cfdgpu.DeviceArray.from_host(t)
+    dx = cfdgpu.DeviceArray(g.count)
+    out = {}
+    for tol in (1e-10, 1e-8):
+        for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
+            g.set_options(prec)
+            for _ in range(2):
+                g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            cfdgpu.synchronize()
+            out[f"{name}_tol_{tol:g}"] = {"solves_per_s": round(steps / (time.perf_counter() - t0), 1),
+                                          "iterations": info["iterations"]}
+    g.set_options(cfdgpu.PRECISION_FP64, cycle=cfdgpu.CYCLE_REFERENCE)
+    g.solve_device(db, dx, 1e-8, 3000, d_scale=ds)
+    cfdgpu.synchronize()
+    t0 = time.perf_counter()
+    info = g.solve_device(db, dx, 1e-8, 3000, d_scale=ds)
+    out["reference_cycle_tol_1e-8"] = {"solves_per_s": round(1.0 / (time.perf_counter() - t0), 3),
+                                       "iterations": info["iterations"]}
+    case.preconditioner = IC0
+    gi = cfdgpu.Grid(case)
+    for prec, name in ((cfdgpu.PRECISION_FP64, "ic0_tol_1e-10"), (cfdgpu.PRECISION_MIXED, "ic0_mixed_tol_1e-10")):
+        gi.set_options(prec)
+        gi.solve_device(db, dx, 1e-10, 3000, d_scale=ds)
+        cfdgpu.synchronize()
+        t0 = time.perf_counter()
+        info = gi.solve_device(db, dx, 1e-10, 3000, d_scale=ds)
+        out[name] = {"solves_per_s": round(1.0 / (time.perf_counter() - t0), 3), "iterations": info["iterations"]}
+    return out
+
+
+def lid_cavity(steps=5):
+    """The closed 48^3 lid cavity (the reference's CPU-runnable case): IC(0)-PCG at
+    rel-res 1e-10, FP64 and with IC(0) applied in FP32."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    case, b, t = W.lid_cavity_48()
+    g = cfdgpu.Grid(case)
+    db = cfdgpu.DeviceArray.from_host(b)
+    ds = cfdgpu.DeviceArray.from_host(t)
+    dx = cfdgpu.DeviceArray(g.count)
+    out = {}
+    for name, prec in (("ic0_fp64", cfdgpu.PRECISION_FP64), ("ic0_mixed", cfdgpu.PRECISION_MIXED)):
+        g.set_options(prec)
+        g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
+        cfdgpu.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            info = g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
+        cfdgpu.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        out[name] = {"solves_per_s": round(1.0 / el, 2), "ms_per_solve": round(el * 1e3, 2),
+                     "iterations": info["iterations"]}
+    return out
 
 
 def served(requests=512, n_grids=16, max_active=8, seed=4):
@@ -83,12 +431,6 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
     rng = np.random.default_rng(seed)
     reqs = [(int(rng.integers(0, n_grids)), W.SERVED_TOLERANCES[int(rng.integers(0, 3))]) for _ in range(requests)]
     cells = sum(grids[k].count for k, _ in reqs)
-
-    def pct(v, q):
-        v = sorted(v)
-        return v[min(len(v) - 1, int(math.ceil(q * len(v))) - 1)]
-
-    # warm every grid in both precisions (hierarchy and graphs ready)
     for prec in (cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED):
         for k, g in enumerate(grids):
             g.set_options(prec)
@@ -97,7 +439,6 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
     out = {"requests": requests, "grids": n_grids, "max_active": max_active,
            "shapes": [list(s) for s in W.SERVED_SHAPES], "tolerances": W.SERVED_TOLERANCES,
            "mean_cells": round(cells / requests)}
-    # (1) plain: sequential FP64 solves
     for g in grids:
         g.set_options(cfdgpu.PRECISION_FP64)
     lat = []
@@ -109,7 +450,6 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
     el = time.perf_counter() - t0
     out["plain_fp64"] = {"solves_per_s": round(requests / el, 1), "p50_ms": round(1e3 * pct(lat, 0.5), 3),
                          "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
-    # (2) served, closed loop
     cap = {}
     for name, prec in (("served_fp64", cfdgpu.PRECISION_FP64), ("served_mixed", cfdgpu.PRECISION_MIXED)):
         t0 = time.perf_counter()
@@ -120,7 +460,6 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
             server.result(tk)
         cap[name] = requests / el
         out[name] = {"solves_per_s": round(cap[name], 1)}
-    # (3) open loop at 80 % of served mixed capacity
     lam = 0.8 * cap["served_mixed"]
     arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
     tks = []
@@ -144,42 +483,6 @@ def served(requests=512, n_grids=16, max_active=8, seed=4):
     return out
 
 
-def cavity48(steps=5):
-    """The closed 48^3 lid cavity of the reference's CPU-runnable case: plain FP64
-    IC(0)-PCG at rel-res 1e-10, right-hand side -div(u) of seeded U(-1,1) face
-    velocities (the parity tests hold its iteration count to the oracle's)."""
-    import cfdgpu
-    from cfdgpu import workloads as W
-
-    case, b, t = W.lid_cavity_48()
-    g = cfdgpu.Grid(case)
-    db = cfdgpu.DeviceArray.from_host(b)
-    ds = cfdgpu.DeviceArray.from_host(t)
-    dx = cfdgpu.DeviceArray(g.count)
-    g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
-    cfdgpu.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        info = g.solve_device(db, dx, 1e-10, 1200, d_scale=ds)
-    cfdgpu.synchronize()
-    el = (time.perf_counter() - t0) / steps
-    return {"solves_per_s": round(1.0 / el, 2), "ms_per_solve": round(el * 1e3, 2), "iterations": info["iterations"],
-            "true_rel_residual": info["relative_residual"]}
-
-
-# The reference's published single-thread CPU times (ms, reused solve; AMD EPYC 9V74):
-# docs/evidence/pressure-multigrid.md:34-38 (manufactured kernels, tol 1e-11, IC(0) / MG)
-# and docs/evidence/native-cpu-performance.md:87 (canyon-93k IC(0) solve, median warm).
-PUBLISHED_CPU_MS = {
-    "cavity24": ([24, 2, 24], 1.292, 0.928),
-    "cavity48": ([48, 2, 48], 13.007, 3.646),
-    "cavity96": ([96, 2, 96], 102.410, 16.864),
-    "cube24": ([24, 24, 24], 19.565, 13.059),
-    "cube48": ([48, 48, 48], 390.576, 152.088),
-}
-CANYON93K_IC0_CPU_MS = 393.146
-
-
 def published_fixtures(reps=3):
     """The fixtures whose reference CPU times are published, on the GPU with the
     reference's protocol: the manufactured field sin(2 pi x/Lx) + 0.3 cos(4 pi z/Lz)
@@ -187,8 +490,6 @@ def published_fixtures(reps=3):
     median of `reps` reused solves (inputs resident in HBM). IC(0), the reference's
     multigrid cycle (its iteration counts) and the red-black cycle; canyon-93k
     with IC(0) (its published 166 iterations)."""
-    import json as _json
-
     import cfdgpu
     from cfdgpu import workloads as W
     from cfdgpu.case import IC0, MULTIGRID, Case
@@ -200,7 +501,7 @@ def published_fixtures(reps=3):
         ex = g.center_nullspaces(W.manufactured(case, W.fluid_mask(case)))
         db = cfdgpu.DeviceArray.from_host(g.apply(ex))
         dx = cfdgpu.DeviceArray(g.count)
-        g.solve_device(db, dx, 1e-11, 10000)  # cold
+        g.solve_device(db, dx, 1e-11, 10000)
         cfdgpu.synchronize()
         times = []
         for _ in range(reps):
@@ -226,7 +527,7 @@ def published_fixtures(reps=3):
         row["mg_rb_cycle"] = r
         out[name] = row
     with open(os.path.join(ROOT, "tests", "golden", "canyon-93k.json")) as f:
-        case = Case.from_json(_json.load(f))
+        case = Case.from_json(json.load(f))
     case.preconditioner = IC0
     r = timed(case)
     r.update(reference_cpu_ms=CANYON93K_IC0_CPU_MS, speedup=round(CANYON93K_IC0_CPU_MS / r["gpu_ms"], 1))
@@ -234,204 +535,16 @@ def published_fixtures(reps=3):
     return out
 
 
-def anisotropic(steps=3):
-    """A flat solid-free lid cavity (96x16x96 cells over 1 x 1 x 0.1 m: z couplings
-    100x stiffer) at rel-res 1e-10 with the reference's own multigrid cycle (z-line
-    block smoothers on semicoarsened levels, multigrid/line.rs), the zebra cycle
-    (the same hierarchy and blocks swept by colour classes), the red-black cycle
-    and IC(0); right-hand side A applied to sin(0.73 i) + 0.3 cos(0.17 i),
-    centered, as in the reference's anisotropy suite."""
-    import cfdgpu
-    from cfdgpu.case import IC0, MULTIGRID, PERIODIC, WALL, Case
-
-    out = {"cells": [96, 16, 96], "size_m": [1.0, 1.0, 0.1]}
-    for name, pc, cycle in (("reference_cycle", MULTIGRID, cfdgpu.CYCLE_REFERENCE),
-                            ("zebra_cycle", MULTIGRID, cfdgpu.CYCLE_ZEBRA), ("rb_cycle", MULTIGRID, cfdgpu.CYCLE_RB),
-                            ("ic0", IC0, None)):
-        case = Case(cells=[96, 16, 96], size_m=[1.0, 1.0, 0.1], boundaries=[WALL, PERIODIC, WALL],
-                    velocity_m_s=[1.0, 0.0, 0.0], preconditioner=pc)
-        g = cfdgpu.Grid(case)
-        if cycle is not None:
-            g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
-        i = np.arange(g.count, dtype=float)
-        ex = g.center_nullspaces(np.sin(0.73 * i) + 0.3 * np.cos(0.17 * i))
-        db = cfdgpu.DeviceArray.from_host(g.apply(ex))
-        dx = cfdgpu.DeviceArray(g.count)
-        for _ in range(2):
-            g.solve_device(db, dx, 1e-10, 3000)
-        cfdgpu.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            info = g.solve_device(db, dx, 1e-10, 3000)
-        cfdgpu.synchronize()
-        el = (time.perf_counter() - t0) / steps
-        x = dx.numpy()
-        out[name] = {"solves_per_s": round(1.0 / el, 2), "iterations": info["iterations"],
-                     "true_rel_residual": info["relative_residual"],
-                     "recovery_rms": float(np.sqrt(np.mean((x - ex) ** 2)))}
-        if name == "reference_cycle":
-            out["block_smoothers"] = len(g.multigrid_stats().get("block_smoothers", []))
-    return out
-
-
-def campaign(directions=(0.0, 90.0, 200.0, 290.0), solves=20, tol=1e-8, seed=5):
-    """A bounded sample of the street-canyon wind-direction campaign at its full
-    size (384x384x160, 23.6M cells, 400 convex prisms between streets, about
-    24 % solid, x and y open): 4 of the 36 directions, 20 of the 200 chained
-    right-hand sides each. Per direction (its own operator) the chain
-    b_{k+1} = b_k + 0.02 rms(b_k) xi, xi ~ N(0,1) on the fluid cells (solid cells
-    have no equation; their entries stay zero) is solved cold (x = 0) and warm
-    (from the grid's resident previous solution), plain FP64 and mixed. The
-    right-hand sides are resident before timing. The last warm mixed field of
-    each direction is compared with a cold FP64 solve at 1e-12 (the oracle itself
-    is too slow at this size; the parity tests hold the FP64 path to it)."""
-    import cfdgpu
-    from cfdgpu import workloads as W
-
-    rng = np.random.default_rng(seed)
-    totals = {}
-    errors = []
-    setups = []
-    solid = None
-    for d in directions:
-        case, b, t = W.street_canyon(d)
-        fluid = W.fluid_mask(case)
-        solid = 1.0 - float(fluid.mean())
-        began = time.perf_counter()
-        g = cfdgpu.Grid(case)
-        cfdgpu.synchronize()
-        setups.append(time.perf_counter() - began)
-        chain = [b]
-        for _ in range(solves - 1):
-            prev = chain[-1]
-            rms = math.sqrt(float(np.mean(prev[fluid] ** 2)))
-            chain.append(prev + np.where(fluid, 0.02 * rms * rng.standard_normal(prev.size), 0.0))
-        dbs = [cfdgpu.DeviceArray.from_host(v) for v in chain]
-        del chain
-        dx = cfdgpu.DeviceArray(g.count)
-        for name, prec, warm in (("cold_fp64", cfdgpu.PRECISION_FP64, False), ("warm_fp64", cfdgpu.PRECISION_FP64, True),
-                                 ("cold_mixed", cfdgpu.PRECISION_MIXED, False),
-                                 ("warm_mixed", cfdgpu.PRECISION_MIXED, True)):
-            g.set_options(prec, warm_start=warm)
-            g.solve_device(dbs[0], dx, tol, 3000)  # warm-up (and the first resident solution)
-            cfdgpu.synchronize()
-            its = 0
-            t0 = time.perf_counter()
-            for db in dbs:
-                its += g.solve_device(db, dx, tol, 3000)["iterations"]
-            cfdgpu.synchronize()
-            el = time.perf_counter() - t0
-            tot = totals.setdefault(name, [0.0, 0, 0])
-            tot[0] += el
-            tot[1] += its
-            tot[2] += len(dbs)
-        x_warm = dx.numpy()
-        g.set_options(cfdgpu.PRECISION_FP64)
-        ref = cfdgpu.DeviceArray(g.count)
-        g.solve_device(dbs[-1], ref, 1e-12, 3000)
-        xr = ref.numpy()
-        errors.append(float(np.linalg.norm(x_warm - xr) / np.linalg.norm(xr)))
-        del g, dbs, dx, ref
-        print(f"[campaign] direction {d}: set-up {setups[-1]:.2f} s", file=sys.stderr, flush=True)
-    out = {"directions": list(directions), "solves_per_direction": solves, "tol": tol,
-           "grid": "384x384x160 street canyon, 400 prisms, x/y open", "cells": 384 * 384 * 160,
-           "solid_fraction": round(solid, 4), "cold_setup_s_mean": round(float(np.mean(setups)), 3)}
-    for name, (el, its, k) in totals.items():
-        out[name] = {"solves_per_s": round(k / el, 2), "mean_iterations": round(its / k, 2)}
-    out["last_warm_mixed_field_rel_error_max"] = max(errors)
-    return out
-
-
-def large(steps=3):
-    """The 256x256x128 prism city (8.4M cells, 200 convex 4-8-vertex prisms, about
-    20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10; each field is
-    compared with the FP64 solve at 1e-12 (itself held to the oracle by the parity
-    tests); fine smoother and operator bandwidth at a size beyond the 256 MB MALL."""
-    import cfdgpu
-    from cfdgpu import workloads as W
-
-    case, b, t = W.urban_prisms()
-    # Grid::build's topology (rasterization, face codes, components) on the host
-    # and on the device, and a whole cold set-up each way
-    t0 = time.perf_counter()
-    cfdgpu.case_topology(case)
-    topo_host = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    cfdgpu.case_topology_device(case)
-    topo_dev = time.perf_counter() - t0
-    os.environ["CFD_GPU_SETUP"] = "0"
-    try:
-        t0 = time.perf_counter()
-        gh = cfdgpu.Grid(case)
-        setup_host = time.perf_counter() - t0
-    finally:
-        os.environ.pop("CFD_GPU_SETUP", None)
-    del gh
-    t0 = time.perf_counter()
-    g = cfdgpu.Grid(case)
-    setup = time.perf_counter() - t0
-    db = cfdgpu.DeviceArray.from_host(b)
-    ds = cfdgpu.DeviceArray.from_host(t)
-    dx = cfdgpu.DeviceArray(g.count)
-    g.set_options(cfdgpu.PRECISION_FP64)
-    ref_info = g.solve_device(db, dx, 1e-12, 3000, d_scale=ds)
-    ref = dx.numpy()
-    out = {"cells": g.count, "fluid_cells": g.fluid_count, "solid_fraction": round(1 - g.fluid_count / g.count, 4),
-           "cold_setup_s": round(setup, 3), "cold_setup_host_topology_s": round(setup_host, 3),
-           "topology_host_s": round(topo_host, 3), "topology_device_s": round(topo_dev, 3),
-           "reference_iterations_1e-12": ref_info["iterations"]}
-    for tol in (1e-8, 1e-10):
-        for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
-            g.set_options(prec)
-            info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
-            cfdgpu.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
-            cfdgpu.synchronize()
-            el = time.perf_counter() - t0
-            x = dx.numpy()
-            out[f"{name}_tol_{tol:g}"] = {"solves_per_s": round(steps / el, 2), "iterations": info["iterations"],
-                                          "true_rel_residual": info["relative_residual"],
-                                          "field_rel_error": float(np.linalg.norm(x - ref) / np.linalg.norm(ref))}
-    g.set_options(cfdgpu.PRECISION_FP64)
-    for name, k in (("fine_gs_fp64", 1), ("operator", 0), ("axpy", 2)):
-        sec, nb = g.profile_kernel(k, reps=20)
-        out[name] = {"avg_us": round(sec * 1e6, 2), "algorithmic_GBps": round(nb / sec / 1e9, 1),
-                     "frac": round(nb / sec / 1e9 / HBM_PEAK_GBS, 4)}
-    g.set_options(cfdgpu.PRECISION_MIXED)
-    sec, nb = g.profile_kernel(1, reps=20)
-    out["fine_gs_mixed"] = {"avg_us": round(sec * 1e6, 2), "algorithmic_GBps": round(nb / sec / 1e9, 1),
-                            "frac": round(nb / sec / 1e9 / HBM_PEAK_GBS, 4)}
-    return out
-
-
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_rb_gs_urban_block_128x128x64.txt")
-
-
-def measured_traffic():
-    """HBM bytes per launch of the fine half-sweep from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its copy-kernel factor)."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
-
-
 def pmc_probe():
-    """The kernels the --pmc passes count: 20 fine half-sweeps of the urban block
-    (FP64) and 5 elementwise products c = 2 a over 64 Mi floats (256 MiB read,
-    256 MiB written), which calibrate FETCH_SIZE and WRITE_SIZE."""
+    """The kernels the --pmc passes count: 20 fine half-sweeps of the street canyon
+    (FP64, 384x384x160) and 5 elementwise products c = 2 a over 64 Mi floats
+    (256 MiB read, 256 MiB written), which calibrate FETCH_SIZE and WRITE_SIZE."""
     import torch
 
     import cfdgpu
     from cfdgpu import workloads as W
 
-    case, b, t = W.urban_block()
+    case, b, t = W.street_canyon(HEADLINE_DIRECTION)
     g = cfdgpu.Grid(case)
     g.set_options(cfdgpu.PRECISION_FP64)
     sec, nb = g.profile_kernel(1, reps=20)
@@ -448,10 +561,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--no-large", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="only the headline solve and its roofline kernel")
+    ap.add_argument("--no-cpu", action="store_true", help="no CPU baseline and no oracle field of the prism city")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline and its kernel table")
     ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -459,146 +570,100 @@ def main():
               "several GPUs would each hold their own grids (one wind direction or served grid per GPU)",
               file=sys.stderr)
         sys.exit(2)
-
     if args.pmc_probe:
         pmc_probe()
         return
 
-    from cfdgpu import workloads as W
-
-    case, b, t = W.urban_block()
-    tol = 1e-8
     base = None
-    x_cpu = None
+    c3_job = None
+    tmp = tempfile.mkdtemp(prefix="cfdbench")
+    c3_path = os.path.join(tmp, "c3_oracle.npy")
     if not args.no_cpu:
-        x_cpu, base = cpu_baseline(case, b, t, tol)
+        log("cpu baseline (oracle, one core)")
+        base = cpu_baseline()
+        if not args.no_extra:
+            # the oracle's prism-city field, on one core beside the GPU legs (no GPU in the child)
+            cores = sorted(os.sched_getaffinity(0))
+            ctx = multiprocessing.get_context("fork")
+            c3_job = ctx.Process(target=_c3_oracle_child, args=(c3_path, cores[-1]), daemon=True)
+            c3_job.start()
 
-    import cfdgpu
+    import cfdgpu  # noqa: F401  (the first GPU use comes after the fork)
 
-    g = cfdgpu.Grid(case)
-    db = cfdgpu.DeviceArray.from_host(b)
-    dt_ = cfdgpu.DeviceArray.from_host(t)
-    dx = cfdgpu.DeviceArray(g.count)
-
-    def run(steps, warmup, tol_, precision):
-        g.set_options(precision)
-        for _ in range(warmup):
-            g.solve_device(db, dx, tol_, 2000, d_scale=dt_)
-        cfdgpu.synchronize()
-        lat = []
-        infos = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            s0 = time.perf_counter()
-            infos.append(g.solve_device(db, dx, tol_, 2000, d_scale=dt_))
-            lat.append(time.perf_counter() - s0)
-        cfdgpu.synchronize()
-        el = time.perf_counter() - t0
-        return el, lat, infos
-
-    el, lat, infos = run(args.steps, args.warmup, tol, cfdgpu.PRECISION_FP64)
-    x_gpu = dx.numpy()
-    ms = 1e3 * el / args.steps
-    value = args.steps / el
-    # beside the headline: 1e-10 plain, and mixed precision at 1e-8 and 1e-10
-    extra = {}
-    variants = () if args.no_extra else (("plain_tol_1e-10", 1e-10, cfdgpu.PRECISION_FP64),
-                                         ("mixed_tol_1e-8", 1e-8, cfdgpu.PRECISION_MIXED),
-                                         ("mixed_tol_1e-10", 1e-10, cfdgpu.PRECISION_MIXED))
-    for name, tol_, prec in variants:
-        k = max(3, args.steps // 2)
-        e2, l2, i2 = run(k, 2, tol_, prec)
-        extra[name] = dict(solves_per_s=k / e2, iterations=i2[-1]["iterations"],
-                           true_rel_residual=i2[-1]["relative_residual"])
-    g.set_options(cfdgpu.PRECISION_FP64)
-    sec, nbytes = g.profile_kernel(1, reps=50)
-    achieved = nbytes / sec / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(),
-                    kernel="k_rb_gs<double> (fine red-black half-sweep, colour-separated layout)",
-                    avg_launch_us=round(sec * 1e6, 3), algorithmic_bytes_per_launch=nbytes)
-    if args.no_extra:
-        args.no_served = args.no_large = True
-    # the whole V-cycle and the operator for context
-    sv, bv = g.profile_kernel(4, reps=20)
-    so, bo = g.profile_kernel(0, reps=50)
-    extra["vcycle"] = dict(avg_us=round(sv * 1e6, 2), algorithmic_GBps=round(bv / sv / 1e9, 1))
-    extra["operator"] = dict(avg_us=round(so * 1e6, 2), algorithmic_GBps=round(bo / so / 1e9, 1))
-    # the same system with the IC(0) preconditioner (plain FP64, 1e-10)
-    from cfdgpu.case import IC0
-
-    cycles = () if args.no_extra else (("geometric_cycle", cfdgpu.CYCLE_GEOMETRIC, 5),
-                                       ("reference_cycle", cfdgpu.CYCLE_REFERENCE, 1))
-    case_ic = W.urban_block()[0]
-    case_ic.preconditioner = IC0
+    h, g, dbs, dx = headline(args.steps, args.warmup, not args.no_extra)
+    value = args.steps / h["elapsed"]
+    probe = h["probe"]
+    top, row = next(iter(probe.items()))
+    kernel_names = {"fine_half_sweep": "k_rb_gs2<double> (fine red-black half-sweep, two cells per thread)",
+                    "coarse_half_sweep": "k_agg_gs<double> (coarse half-sweep)", "operator": "k_pcg_op",
+                    "axpy": "k_pcg_axpy"}
+    prow = probe[top]
+    achieved = prow["GBps"]
+    solve_s = h["elapsed"] / args.steps
+    total_bytes = sum(r["bytes_per_launch"] * r["launches"] for r in probe.values())
+    device_ms = sum(r["device_ms"] for r in probe.values())
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": measured_traffic(),
+                "kernel": kernel_names.get(top, top), "grid": "street canyon 384x384x160",
+                "avg_launch_us": prow["avg_us"], "launches_per_solve": prow["launches"],
+                "algorithmic_bytes_per_launch": prow["bytes_per_launch"],
+                "share_of_solve_device_time": round(prow["device_ms"] / device_ms, 4),
+                "solve_algorithmic_GBps": round(total_bytes / solve_s / 1e9, 1),
+                "solve_frac": round(total_bytes / solve_s / 1e9 / HBM_PEAK_GBS, 4)}
+    extra = {"street_canyon_modes": h["modes"], "kernel_classes_in_solve": probe,
+             "headline_setup_s": round(h["setup_s"], 3)}
+    del g, dbs, dx
     if not args.no_extra:
-        gi = cfdgpu.Grid(case_ic)
-        for _ in range(2):
-            gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
-        cfdgpu.synchronize()
-        t0 = time.perf_counter()
-        ii = gi.solve_device(db, dx, 1e-10, 3000, d_scale=dt_)
-        cfdgpu.synchronize()
-        extra["ic0_tol_1e-10"] = dict(solves_per_s=round(1.0 / (time.perf_counter() - t0), 3),
-                                      iterations=ii["iterations"], true_rel_residual=ii["relative_residual"])
-        del gi
-    # the same system on the other multigrid cycles (plain FP64, 1e-8): the structured
-    # geometric one and the reference's own (its iteration count is the oracle's)
-    for name, cyc, k in cycles:
-        g.set_options(cfdgpu.PRECISION_FP64, cycle=cyc)
-        for _ in range(2):  # the second solve captures the iteration graphs it queues
-            g.solve_device(db, dx, tol, 3000, d_scale=dt_)
-        cfdgpu.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(k):
-            ic = g.solve_device(db, dx, tol, 3000, d_scale=dt_)
-        cfdgpu.synchronize()
-        extra[name] = dict(solves_per_s=round(k / (time.perf_counter() - t0), 3), iterations=ic["iterations"],
-                           true_rel_residual=ic["relative_residual"])
-    g.set_options(cfdgpu.PRECISION_FP64)
-    if not args.no_extra:
-        extra["cavity_48_ic0"] = cavity48()
-        extra["anisotropic_96x16x96"] = anisotropic()
-        extra["published_fixtures"] = published_fixtures()
-        extra["wind_campaign"] = campaign()
-    if not args.no_served:
+        log("second direction")
+        extra["street_canyon_second_direction"] = second_direction()
+        log("prism city")
+        extra["prisms_256x256x128"], fields = prisms(c3_path)
+        log("urban block")
+        extra["urban_block_128x128x64"] = urban(max(3, args.steps // 2))
+        log("lid cavity")
+        extra["lid_cavity_48"] = lid_cavity()
+        log("served mix")
         extra["served"] = served()
-    if not args.no_large:
-        extra["prisms_256x256x128"] = large()
-    lat_ms = sorted(1e3 * v for v in lat)
-    p = lambda q: lat_ms[min(len(lat_ms) - 1, int(math.ceil(q * len(lat_ms))) - 1)]  # noqa: E731
+        log("published fixtures")
+        extra["published_fixtures"] = published_fixtures()
+        if c3_job is not None:
+            log("waiting for the oracle's prism-city field")
+            c3_job.join(timeout=900)
+            if os.path.exists(c3_path + ".json"):
+                xo = np.load(c3_path)
+                with open(c3_path + ".json") as f:
+                    meta = json.load(f)
+                errs = {k: float(np.linalg.norm(v - xo) / np.linalg.norm(xo)) for k, v in fields.items()}
+                extra["prisms_256x256x128"]["field_rel_error_vs_oracle_1e-10"] = errs
+                extra["prisms_256x256x128"]["oracle"] = meta
+    lat_ms = [1e3 * v for v in h["lat"]]
     out = {
-        "metric": "Poisson solves/s at rel-res 1e-8 + HBM GB/s %peak; p50/p99 latency served vs plain",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(1e3 * solve_s, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded urban block and log-law velocity field)",
+        "data": "synthetic (seeded street-canyon city, log-law velocity field, chained perturbed right-hand sides)",
         "config": {
-            "workload": "urban block 128x128x64, h=2 m, 24 seeded boxes (~15% solid), open x outlet, plain FP64 "
-            "PCG + multigrid V(2,2) red-black on the split 2x2x2 aggregation hierarchy, rel-res 1e-8",
-            "cells": case.count,
-            "iterations": infos[-1]["iterations"],
-            "true_rel_residual": infos[-1]["relative_residual"],
-            "latency_ms": {"p50": round(p(0.5), 3), "p99": round(p(0.99), 3)},
+            "workload": "street-canyon wind campaign, one direction (200 deg): 384x384x160 cells h=1 m, 400 convex "
+                        "4-8-vertex prisms, x/y open, plain FP64 PCG + red-black multigrid V(2,2) on the split "
+                        "aggregation hierarchy from x=0, rel-res 1e-8, a distinct chained right-hand side per solve",
+            "cells": h["cells"],
+            "solid_fraction": round(h["solid_fraction"], 4),
+            "iterations_mean": round(float(np.mean(h["its"])), 2),
+            "iterations_range": [min(h["its"]), max(h["its"])],
+            "latency_ms": {"p50": round(pct(lat_ms, 0.5), 3), "p99": round(pct(lat_ms, 0.99), 3)},
         },
         "roofline": roofline,
+        "cpu_baseline": base,
         "extra": extra,
     }
-    if base is not None:
-        err = float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu))
-        base_out = {k: base[k] for k in ("value", "unit", "cores", "kind", "sample")}
-        out["cpu_baseline"] = base_out
-        out["config"]["field_rel_error_vs_oracle"] = err
-        out["config"]["oracle_iterations"] = base["iterations"]
-    else:
-        out["cpu_baseline"] = None
     print(json.dumps(out))
 
 

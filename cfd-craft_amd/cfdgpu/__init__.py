@@ -138,6 +138,25 @@ class MgStats(ctypes.Structure):
         return d
 
 
+class KernelProbe(ctypes.Structure):
+    _fields_ = [
+        ("launches", ctypes.c_int64),
+        ("seconds", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+        ("dropped", ctypes.c_int64),
+        ("truncated", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+# kernel classes of Grid.profile_solve (include/cfdgpu.h CFD_PROBE_*)
+PROBE_KERNELS = {
+    "fine_half_sweep": 1, "coarse_half_sweep": 2, "operator": 3, "axpy": 4, "fine_restrict": 5,
+    "fine_rz_half_sweep": 6, "fine_first_half_sweep": 7, "coarse_restrict": 8, "coarsest_solve": 9,
+    "fine_prolong": 10, "ic0_apply": 11, "true_residual": 12,
+}
+
+
 class Options(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int32),
@@ -193,6 +212,7 @@ def lib():
             "cfd_case_topology": (I, [P, P, P, P, P, I64, P]),
             "cfd_case_topology_device": (I, [P, P, P, P, P, I64]),
             "cfd_profile_kernel": (I, [P, I, I, P, P]),
+            "cfd_profile_solve": (I, [P, P, P, D, I64, P, I, P, P]),
             "cfd_server_create": (I, [I, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_server_destroy": (None, [P]),
             "cfd_server_admit": (I, [P, P, ctypes.POINTER(ctypes.c_void_p)]),
@@ -417,6 +437,22 @@ class Grid:
         nbytes = ctypes.c_double(0.0)
         _check(lib().cfd_profile_kernel(self.h, int(kernel), int(reps), ctypes.byref(sec), ctypes.byref(nbytes)))
         return sec.value, nbytes.value
+
+    def profile_solve(self, d_b: "DeviceArray", d_x: "DeviceArray", tol: float, kernel, max_iterations: int = 3000,
+                      d_scale: "DeviceArray" = None):
+        """One solve with the launches of one kernel class (a name or number of
+        PROBE_KERNELS) timed by HIP event pairs: dict with launches, seconds,
+        bytes (algorithmic) and the solve's info."""
+        pb = _device_vector(d_b, self.count, "d_b")
+        px = _device_vector(d_x, self.count, "d_x")
+        ps = _device_vector(d_scale, self.count, "d_scale", optional=True)
+        k = PROBE_KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+        out = KernelProbe()
+        info = LinearInfo()
+        _check(lib().cfd_profile_solve(self.h, pb, ps, float(tol), int(max_iterations), px, k, ctypes.byref(out),
+                                       ctypes.byref(info)))
+        return dict(launches=out.launches, seconds=out.seconds, bytes=out.bytes, dropped=out.dropped,
+                    truncated=bool(out.truncated), info=info.as_dict())
 
     def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False, cycle: int = 0):
         """precision: PRECISION_FP64 or PRECISION_MIXED; cycle (multigrid): CYCLE_RB,

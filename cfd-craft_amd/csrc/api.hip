@@ -227,6 +227,7 @@ struct cfd_grid {
   hipEvent_t evs = nullptr;  // recorded after each state read-back
   // CFD_GPU_GRAPHS=0 launches the segments kernel by kernel (for profilers that do not follow graphs).
   bool use_graphs = true;
+  LaunchProbe probe;  // event pool of cfd_profile_solve
   int64_t coarse_max = kCoarseMax, tail_cells = kTailCells;
   // CFD_MG_FUSED: 0 one kernel per half-sweep on every level; 1 coarse levels fused
   // (one kernel per level and phase), the default; 2 every level fused.
@@ -244,6 +245,7 @@ struct cfd_grid {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (evs) (void)hipEventDestroy(evs);
+    for (hipEvent_t e : probe.events) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -351,6 +353,7 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
     AggDevLevel<T> d{};
     d.n = sh.n;
     d.ncol = sh.ncol;
+    d.nnz = sh.nnz;
     for (int c = 0; c <= kMaxColours; ++c) d.cptr[c] = sh.cptr[c];
     d.ptr = sh.ptr;
     d.col = sh.col;
@@ -433,6 +436,7 @@ int build_agg(cfd_grid* g) {
     AggDevLevel<double> sh{};
     sh.n = (int)L.n;
     sh.ncol = (int)L.colour_ptr.size() - 1;
+    sh.nnz = (int)L.ptr.back();
     if (sh.ncol > kMaxColours) {
       set_error("multigrid: a coarse level needs more colour classes than supported");
       return CFD_EUNSUPPORTED;
@@ -1051,7 +1055,10 @@ int enqueue_chunk(cfd_grid* g, int len) {
     if (rc) return rc;
     HIP_TRY(hipGraphLaunch(seg, g->stream));
   } else {
-    for (int i = 0; i < len; ++i) enqueue_iteration(g, (parity + i) & 1);
+    for (int i = 0; i < len; ++i) {
+      if (t_probe) t_probe->tag = J.enq + i + 1;
+      enqueue_iteration(g, (parity + i) & 1);
+    }
   }
   J.enq += len;
   const int slot = J.slot;
@@ -1110,6 +1117,7 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   launch_pcg_init(g->geo.N, g->codes.p, d_b, g->mean_b.p, cidx, g->r.p, g->bc.p, g->x.p, st, g->part.p, g->stream);
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
+  if (t_probe) t_probe->tag = 0;
   enqueue_precondition(g, g->r.p, g->z.p, st);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   if (!rz_fused(g))
@@ -1787,8 +1795,15 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
     case 0:  // z, p_old, code in; p_new, Ap out
       *bytes_per_launch = n * (8 + 8 + 1 + 8 + 8);
       break;
-    case 1:  // half the cells: neighbour x, own b (FP64 r), own code in; own x out
-      *bytes_per_launch = mixed ? 0.5 * n * (4 + 8 + 1 + 4) : 0.5 * n * (8 + 8 + 1 + 8);
+    case 1:
+      if (g->opt.cycle == CFD_CYCLE_RB && g->ash.has_rb) {
+        // colour-separated: the swept half's code, b and new x, the other colour's x
+        const double sz = mixed ? 4.0 : 8.0;
+        *bytes_per_launch = (double)g->ash.rb64.half * (1.0 + 3.0 * sz);
+      } else {
+        // natural layout: half the cells; neighbour x, own b (the FP64 r), own code in; own x out
+        *bytes_per_launch = mixed ? 0.5 * n * (4 + 8 + 1 + 4) : 0.5 * n * (8 + 8 + 1 + 8);
+      }
       break;
     case 2:  // x, p, r, Ap in; x, r out
       *bytes_per_launch = n * 48;
@@ -1801,6 +1816,53 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
       *bytes_per_launch = n * fine * 8.0 / 7.0;
     }
   }
+  return CFD_OK;
+}
+
+int cfd_profile_solve(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                      double* d_x, int kernel, cfd_kernel_probe* out, cfd_linear_info* info) {
+  if (!g || !out || kernel < 1 || kernel >= PROBE_KERNELS) {
+    set_error("cfd_profile_solve: kernel must be one of CFD_PROBE_*");
+    return CFD_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  LaunchProbe& P = g->probe;
+  if (P.events.empty()) {
+    P.events.resize(8192, nullptr);
+    for (hipEvent_t& e : P.events) HIP_TRY(hipEventCreate(&e));
+  }
+  P.kernel = kernel;
+  P.used = 0;
+  P.tag = 0;
+  P.bytes.clear();
+  P.tags.clear();
+  t_probe = &P;
+  cfd_linear_info local{};
+  int rc = solve_device(g, d_b, d_scale, tol, max_iterations, nullptr, nullptr, d_x, &local, false);
+  t_probe = nullptr;
+  if (info) *info = local;
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  // launches queued past the converging iteration end at once (the device state
+  // is no longer RUNNING): only the solve's own work is counted. The PCG vector
+  // kernels of iteration `it` run before its convergence test; its preconditioner
+  // does not.
+  const long long it = local.iterations;
+  const bool before_test = kernel == PROBE_OPERATOR || kernel == PROBE_AXPY || kernel == PROBE_TRUE_RESIDUAL;
+  std::memset(out, 0, sizeof *out);
+  for (size_t k = 0; 2 * k + 1 < P.used; ++k) {
+    const long long tag = P.tags[k];
+    if (before_test ? tag > it : tag >= it && it > 0) {
+      ++out->dropped;
+      continue;
+    }
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, P.events[2 * k], P.events[2 * k + 1]));
+    out->launches += 1;
+    out->seconds += (double)ms * 1e-3;
+    out->bytes += P.bytes[k];
+  }
+  out->truncated = P.used + 2 > P.events.size() ? 1 : 0;
   return CFD_OK;
 }
 

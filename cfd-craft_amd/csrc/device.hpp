@@ -81,6 +81,7 @@ template <class T>
 struct AggDevLevel {
   int n;
   int ncol;
+  int nnz;  // stored off-diagonal entries (the CSR form)
   int cptr[kMaxColours + 1];
   const int* ptr;  // off-diagonal entries of each row
   const int* col;

@@ -17,6 +17,8 @@
 
 namespace cfdgpu {
 
+thread_local LaunchProbe* t_probe = nullptr;
+
 constexpr int BS = 256;
 constexpr int RB = kReduceBlock;  // threads of a reduction block
 constexpr int RP = kReduceGrid;   // blocks of a reduction grid
@@ -2126,12 +2128,16 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
                     double* out64, const PcgState* st, hipStream_t s) {
   // the pass carrying the prolongation keeps one cell per thread (its neighbour
   // reads gather the coarse correction cell by cell)
+  probe_begin(PROBE_FINE_HALF, s);
   if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
     hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
                        F, colour, xc, fparent, cs, out64, st);
   else
     hipLaunchKernelGGL((k_rb_gs<T>), dim3(rb_blocks(g0, F)), dim3(BS), 0, s, g0, F, colour, xc, fparent, cs, out64,
                        st);
+  // per swept cell: its code, b and new x, and the other colour's x (each value
+  // read once), and z when written
+  probe_end(PROBE_FINE_HALF, s, (double)F.half * (1 + 3 * sizeof(T) + (out64 ? 8 : 0)));
 }
 
 // Fine residual of cell q from the colour-separated layout.
@@ -2627,19 +2633,28 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 }
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s) {
+  probe_begin(PROBE_OPERATOR, s);
   if (closed)
     hipLaunchKernelGGL(k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
   else
     hipLaunchKernelGGL(k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+  // z, p_old, code in; p_new, Ap out
+  probe_end(PROBE_OPERATOR, s, 33.0 * (double)g.N);
 }
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s) {
+  probe_begin(PROBE_AXPY, s);
   hipLaunchKernelGGL(k_pcg_axpy, dim3(RP), dim3(RB), 0, s, N, p, ap, x, r, st, part);
+  probe_end(PROBE_AXPY, s, 48.0 * (double)N);  // x, p, r, Ap in; x, r out
 }
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s) {
+  probe_begin(PROBE_TRUE_RESIDUAL, s);
   hipLaunchKernelGGL(k_pcg_true, dim3(RP), dim3(RB), 0, s, g, codes, x, bc, st, part);
   hipLaunchKernelGGL(k_pcg_fin, dim3(1), dim3(RB), 0, s, st, (const double*)part);
+  // the residual is recomputed (17 B per cell) only after the recurrent test has
+  // passed, which the host does not know at launch: no bytes are counted
+  probe_end(PROBE_TRUE_RESIDUAL, s, 0.0);
 }
 void launch_pcg_rz(long long N, const uint8_t* codes, const double* r, const double* z, const double* mean,
                    const int* cidx, PcgState* st, double* part, hipStream_t s) {
@@ -2723,6 +2738,7 @@ void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D
 template <class T>
 void launch_ic0_tiles_t(const Geo& g, const uint8_t* codes, const double* D, const double* r, T* y, double* z,
                         const PcgState* st, hipStream_t s) {
+  probe_begin(PROBE_IC0, s);
   // an axis of at most 2 cells: flat 16 x 16 tiles across the other two axes
   if (g.n[1] <= 2)
     launch_ic0_tiles_shaped<16, 2, 16, T>(g, codes, D, r, y, z, st, s);
@@ -2732,6 +2748,8 @@ void launch_ic0_tiles_t(const Geo& g, const uint8_t* codes, const double* D, con
     launch_ic0_tiles_shaped<2, 16, 16, T>(g, codes, D, r, y, z, st, s);
   else
     launch_ic0_tiles_shaped<8, 8, 8, T>(g, codes, D, r, y, z, st, s);
+  // forward: r 8, D 8, code 1 in, y out; backward: y, D 8, code 1 in, z 8 out
+  probe_end(PROBE_IC0, s, (double)g.N * (8 + 8 + 1 + sizeof(T) + sizeof(T) + 8 + 1 + 8));
 }
 
 void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
@@ -2908,7 +2926,9 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const unsigned ab = blocks_for(g0.N);
   auto direct = [&](const auto* b, T* x, double* o64) {
     using TI = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
+    probe_begin(PROBE_COARSEST, s);
     hipLaunchKernelGGL((k_agg_direct<TI, T>), dim3(D.big_rows + (D.nsmall + 3) / 4), dim3(BS), 0, s, D, b, x, o64, st);
+    probe_end(PROBE_COARSEST, s, 0.0);
   };
   if (nlev == 0) {
     // the fine level is the coarsest: solid cells stay zero
@@ -2917,9 +2937,20 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     direct(b0, x0, out64);
     return;
   }
+  // algorithmic bytes of a coarse half-sweep of colour c: per row its padded
+  // (or CSR) entries, diagonal and right-hand side in and its value out, and the
+  // other classes' values read once
+  auto coarse_bytes = [&](const AggDevLevel<T>& L, int c) {
+    const double rows = (double)(L.cptr[c + 1] - L.cptr[c]);
+    const double per_row = L.ecol ? (double)L.K * (4.0 + sizeof(T)) : (double)L.nnz / (double)L.n * (4.0 + sizeof(T)) + 4.0;
+    return rows * (per_row + 3.0 * sizeof(T)) + ((double)L.n - rows) * sizeof(T);
+  };
   auto relax = [&](const AggDevLevel<T>& L, int c) {
     const int rows = L.cptr[c + 1] - L.cptr[c];
-    if (rows > 0) hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, c, st);
+    if (rows <= 0) return;
+    probe_begin(PROBE_COARSE_HALF, s);
+    hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, c, st);
+    probe_end(PROBE_COARSE_HALF, s, coarse_bytes(L, c));
   };
   const Geo none{};
   const T* nox = nullptr;
@@ -2929,17 +2960,25 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
     // pairs of natural cells when every row starts a pair and r allows 16-byte loads
+    probe_begin(PROBE_FINE_FIRST, s);
     if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
       hipLaunchKernelGGL((k_rb_first2<T>), dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
                          g0, codes, (const double*)b0, *rb, st);
     else
       hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
                          g0, codes, (const double*)b0, *rb, st);
+    // r (FP64) and codes in; b of both colours and the red x out
+    probe_end(PROBE_FINE_FIRST, s, (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T));
     for (int pass = 1; pass < 2 * fr; ++pass)
       launch_rb_half<T>(g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr, st, s);
+    probe_begin(PROBE_FINE_RESTRICT, s);
     hipLaunchKernelGGL((k_rb_restrict<T>),
                        dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)), dim3(BS), 0, s,
                        g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
+    // red cells: code, b, x; black x; one parent per 2x2x2 block; the first
+    // coarse level's b and x written and its diagonal read
+    probe_end(PROBE_FINE_RESTRICT, s,
+              (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N / 8 * 4 + (double)lv[0].n * 3 * sizeof(T));
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
@@ -2962,8 +3001,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
       for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) relax(L, c);
     // pre-smoothing ended with the last colour class (without rounds nothing is taken as zero)
     const int zero_from = cr >= 1 ? L.cptr[L.ncol - 1] : L.n;
+    probe_begin(PROBE_COARSE_RESTRICT, s);
     hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st,
                        zero_from);
+    probe_end(PROBE_COARSE_RESTRICT, s, 0.0);
   }
   direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
   for (int l = stop - 1; l >= 0; --l) {
@@ -2972,9 +3013,11 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     if (L.ncol == 2) {
       // prolongation fused into the first half-sweep
       const int rows = L.cptr[2] - L.cptr[1];
-      if (rows > 0)
+      if (rows > 0) {
+        probe_begin(PROBE_COARSE_HALF, s);
         hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, 1, st, xc, cs);
-      else
+        probe_end(PROBE_COARSE_HALF, s, coarse_bytes(L, 1));
+      } else
         hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
       for (int pass = 1; pass < 2 * cr; ++pass) relax(L, (pass + 1) & 1);
     } else {
@@ -2989,17 +3032,23 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     for (int pass = 0; pass < 2 * fr; ++pass) {
       if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
         // the last half-sweep also forms the PCG's r . z partials
+        probe_begin(PROBE_FINE_RZ, s);
         if (rb->two_cells && g0.n[0] % 4 == 0)
           hipLaunchKernelGGL((k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         else
           hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
+        // the red half-sweep, both colours' z out in FP64 and r in
+        probe_end(PROBE_FINE_RZ, s, (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N * 16);
         continue;
       }
       const T* xcp = pass == 0 ? (const T*)lv[0].x : nox;
       if (xcp && rb->rpar) {
         // the correction added to the red cells first, then a plain half-sweep
+        probe_begin(PROBE_FINE_PROLONG, s);
         hipLaunchKernelGGL((k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0, s, rb->half, rb->x, rb->rpar,
                            xcp, cs, st);
+        // red x in and out, its parent; the first coarse level's values
+        probe_end(PROBE_FINE_PROLONG, s, (double)rb->half * (2 * sizeof(T) + 4) + (double)lv[0].n * sizeof(T));
         xcp = nox;
       }
       // z in the natural layout from the last two half-sweeps (the r . z kernel

@@ -9,6 +9,48 @@
 
 namespace cfdgpu {
 
+// Launch probe: during an instrumented solve the host names one kernel class,
+// and every launch of that class is bracketed by a pair of HIP events on its
+// stream, with the launch's algorithmic bytes (each array crossing HBM once).
+// Inactive (null) otherwise: the launch sites then only test a pointer.
+enum ProbeKernel : int {
+  PROBE_FINE_HALF = 1,        // fine red-black half-sweep (k_rb_gs, k_rb_gs2)
+  PROBE_COARSE_HALF = 2,      // coarse half-sweep of one colour class (k_agg_gs)
+  PROBE_OPERATOR = 3,         // p update, A p and p . Ap (k_pcg_op)
+  PROBE_AXPY = 4,             // x and r update with |r|^2 (k_pcg_axpy)
+  PROBE_FINE_RESTRICT = 5,    // fine residual, restriction, first coarse half-sweep (k_rb_restrict)
+  PROBE_FINE_RZ = 6,          // last fine half-sweep with z and r . z (k_rb_gs_rz, k_rb_gs_rz2)
+  PROBE_FINE_FIRST = 7,       // first fine half-sweep from zero (k_rb_first, k_rb_first2)
+  PROBE_COARSE_RESTRICT = 8,  // coarse restriction (k_agg_restrict)
+  PROBE_COARSEST = 9,         // dense coarsest solve (k_agg_direct)
+  PROBE_FINE_PROLONG = 10,    // coarse correction of the red cells (k_rb_prolong)
+  PROBE_IC0 = 11,             // one IC(0) application (both tiled sweeps)
+  PROBE_TRUE_RESIDUAL = 12,   // recomputed residual and the iteration's decisions (k_pcg_true, k_pcg_fin)
+  PROBE_KERNELS = 13
+};
+struct LaunchProbe {
+  int kernel = 0;
+  long long tag = 0;               // the PCG iteration being enqueued (0: the set-up before the first)
+  std::vector<hipEvent_t> events;  // begin, end, begin, end, ...
+  std::vector<double> bytes;       // per completed pair
+  std::vector<long long> tags;     // per completed pair
+  size_t used = 0;
+};
+extern thread_local LaunchProbe* t_probe;
+inline void probe_begin(int kernel, hipStream_t s) {
+  LaunchProbe* p = t_probe;
+  if (p && p->kernel == kernel && (p->used & 1) == 0 && p->used + 2 <= p->events.size())
+    (void)hipEventRecord(p->events[p->used++], s);
+}
+inline void probe_end(int kernel, hipStream_t s, double bytes) {
+  LaunchProbe* p = t_probe;
+  if (p && p->kernel == kernel && (p->used & 1) == 1) {
+    (void)hipEventRecord(p->events[p->used++], s);
+    p->bytes.push_back(bytes);
+    p->tags.push_back(p->tag);
+  }
+}
+
 void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y, hipStream_t s);
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s);

@@ -270,6 +270,37 @@ int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double 
  * update with |r|^2; 3 one IC(0) application (both sweeps); 4 one V-cycle. */
 int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch);
 
+/* Kernel classes of cfd_profile_solve. */
+enum {
+  CFD_PROBE_FINE_HALF = 1,        /* fine red-black half-sweep */
+  CFD_PROBE_COARSE_HALF = 2,      /* coarse half-sweep of one colour class */
+  CFD_PROBE_OPERATOR = 3,         /* p update, A p and p . Ap */
+  CFD_PROBE_AXPY = 4,             /* x and r update with |r|^2 */
+  CFD_PROBE_FINE_RESTRICT = 5,    /* fine residual and restriction */
+  CFD_PROBE_FINE_RZ = 6,          /* last fine half-sweep with z and r . z */
+  CFD_PROBE_FINE_FIRST = 7,       /* first fine half-sweep from zero */
+  CFD_PROBE_COARSE_RESTRICT = 8,  /* coarse restriction */
+  CFD_PROBE_COARSEST = 9,         /* dense coarsest solve */
+  CFD_PROBE_FINE_PROLONG = 10,    /* coarse correction of the fine red cells */
+  CFD_PROBE_IC0 = 11,             /* one IC(0) application */
+  CFD_PROBE_TRUE_RESIDUAL = 12    /* recomputed residual and convergence decision */
+};
+typedef struct {
+  int64_t launches;  /* launches of the class that did the solve's work */
+  double seconds;    /* their summed device duration (a HIP event pair around each) */
+  double bytes;      /* their summed algorithmic HBM bytes (0 where not counted) */
+  int64_t dropped;   /* launches queued past convergence (they end at once), not counted */
+  int32_t truncated; /* 1 when the event pool ran out (later launches not counted) */
+  int32_t reserved;
+} cfd_kernel_probe;
+/* Measurement hook (the GPU counterpart of the reference's phase spans,
+ * profiling.rs:217-281): one solve_scaled on device buffers with the iterations
+ * launched kernel by kernel and every launch of one kernel class bracketed by a
+ * pair of HIP events on the grid's stream; the class's launch count, summed
+ * durations and algorithmic bytes. */
+int cfd_profile_solve(cfd_grid* g, const double* d_b, const double* d_scale, double tol, int64_t max_iterations,
+                      double* d_x, int kernel, cfd_kernel_probe* out, cfd_linear_info* info);
+
 /* ------------------------------------------------------------------ serving */
 /* Continuous batching of independent pressure solves (wind-direction campaigns,
  * optimisation jobs): no counterpart in the reference, whose concurrent solves

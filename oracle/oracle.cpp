@@ -1308,8 +1308,14 @@ void orc_apply(const orc_grid* g, const double* x, double* y) {
   }
 }
 
-void orc_operator_apply(const orc_grid* g, const double* x, double* y) {
+}  // extern "C"
+
+namespace {
+// pressure.rs:254-272 Operator::build: the interior and outlet faces in face order
+// (built once per solve, as the reference's solve_scaled does at 383-386)
+vector<OperatorEntry> operator_build(const orc_grid* g) {
   vector<OperatorEntry> entries;
+  entries.reserve(g->faces.size());
   for (auto& f : g->faces) {
     double w = g->weight(f);
     if (f.kind == ORC_INTERIOR)
@@ -1317,7 +1323,11 @@ void orc_operator_apply(const orc_grid* g, const double* x, double* y) {
     else if (f.kind == ORC_OUTLET)
       entries.push_back({false, f.lo != NONE ? f.lo : f.hi, 0, w});
   }
-  std::fill(y, y + g->count(), 0.0);
+  return entries;
+}
+// pressure.rs:273-285 Operator::apply
+void operator_entries_apply(const vector<OperatorEntry>& entries, size_t count, const double* x, double* y) {
+  std::fill(y, y + count, 0.0);
   for (auto& e : entries) {
     if (e.interior) {
       double d = e.weight * (x[e.lo] - x[e.hi]);
@@ -1327,6 +1337,13 @@ void orc_operator_apply(const orc_grid* g, const double* x, double* y) {
       y[e.lo] += e.weight * x[e.lo];
     }
   }
+}
+}  // namespace
+
+extern "C" {
+
+void orc_operator_apply(const orc_grid* g, const double* x, double* y) {
+  operator_entries_apply(operator_build(g), g->count(), x, y);
 }
 
 void orc_center_nullspaces(const orc_grid* g, double* x) {
@@ -1521,8 +1538,9 @@ int orc_solve_scaled(orc_grid* g, const double* b, const double* scale, double t
   vector<double> p = z;
   double rz = dot(r, z);
   vector<double> ap(count, 0.0);
+  const vector<OperatorEntry> op = operator_build(g);
   for (int64_t it = 0; it < max; ++it) {
-    orc_operator_apply(g, p.data(), ap.data());
+    operator_entries_apply(op, count, p.data(), ap.data());
     local.operator_applications += 1;
     double denom = dot(p, ap);
     if (denom <= 0.0 || !std::isfinite(denom)) return finish(fail(32, "CFD-E032: pressure CG lost positive definiteness"));
