@@ -1731,8 +1731,8 @@ int cfd_project(cfd_grid* g, double* u, double* p, double dt, double tol, int64_
 }
 
 int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch) {
-  if (reps < 1 || kernel < 0 || kernel > 4) {
-    set_error("cfd_profile_kernel: kernel must be 0..4 and reps positive");
+  if (reps < 1 || kernel < 0 || kernel > 5) {
+    set_error("cfd_profile_kernel: kernel must be 0..5 and reps positive");
     return CFD_EINVAL;
   }
   if ((kernel == 1 || kernel == 4) && !g->has_hier) {
@@ -1755,10 +1755,23 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
   if (g->n_closed > 0) launch_fill(std::max(1, g->n_closed), g->mean_z.p, 0.0, g->stream);
+  if (kernel == 5) {
+    // a later iteration: p = z + beta p_old with beta = (sum of the r . z partials) / rz = 1
+    launch_fill(kReduceGrid, g->part.p + P_RZ, 1.0, g->stream);
+    PcgState h{};
+    h.rz = (double)kReduceGrid;
+    h.tol = 1.0;
+    h.max_it = 1LL << 60;
+    h.status = ST_RUNNING;
+    h.first = 0;
+    HIP_TRY(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
   const bool mixed = g->opt.precision == CFD_PRECISION_MIXED;
   auto once = [&]() {
     switch (kernel) {
       case 0:
+      case 5:
         launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,
                       g->stream);
         break;
@@ -1792,7 +1805,10 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
   *avg_seconds = (double)ms * 1e-3 / reps;
   const double n = (double)N;
   switch (kernel) {
-    case 0:  // z, p_old, code in; p_new, Ap out
+    case 0:  // first iteration: z, code in; p_new, Ap out
+      *bytes_per_launch = n * (8 + 1 + 8 + 8);
+      break;
+    case 5:  // z, p_old, code in; p_new, Ap out
       *bytes_per_launch = n * (8 + 8 + 1 + 8 + 8);
       break;
     case 1:

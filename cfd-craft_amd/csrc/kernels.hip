@@ -22,6 +22,7 @@ thread_local LaunchProbe* t_probe = nullptr;
 constexpr int BS = 256;
 constexpr int RB = kReduceBlock;  // threads of a reduction block
 constexpr int RP = kReduceGrid;   // blocks of a reduction grid
+static_assert(kReduceGrid % 8 == 0, "the reduction grid deals whole rounds of blocks to the 8 XCDs");
 constexpr uint8_t FLUID = 1u << 6;
 constexpr uint8_t CLOSED = 1u << 7;
 
@@ -165,17 +166,36 @@ __global__ void __launch_bounds__(BS) k_apply(Geo g, const uint8_t* __restrict__
   y[c] = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
 }
 
+// The share of `items` work units (in order) of the block's XCD: the blocks of
+// a launch are dealt to the 8 XCDs in turn (block b runs on XCD b % 8), so XCD x
+// takes the x-th contiguous eighth of the items and its blocks take them in turn
+// (first, first + step, ...). Neighbouring rows and layers of a stencil are then
+// read by the same XCD, from its own L2. The grid must be a multiple of 8 blocks.
+struct XcdShare {
+  unsigned begin, end, first, step;
+};
+__device__ __forceinline__ XcdShare xcd_share(unsigned items, unsigned per_block, unsigned rank_in_block) {
+  const unsigned x = blockIdx.x & 7, blocks = gridDim.x >> 3;
+  const unsigned chunk = (items + 7) >> 3;
+  XcdShare s;
+  s.begin = min(items, x * chunk);
+  s.end = min(items, s.begin + chunk);
+  s.first = s.begin + (blockIdx.x >> 3) * per_block + rank_in_block;
+  s.step = blocks * per_block;
+  return s;
+}
+
 // Every cell of the grid, once, over a fixed reduction grid: each wave takes
-// segments of 64 cells of one x row in turn; a segment's row is wave-uniform
-// (scalar arithmetic), so no cell needs a division.
+// segments of 64 cells of one x row in turn, from its XCD's contiguous share;
+// a segment's row is wave-uniform (scalar arithmetic), so no cell needs a
+// division.
 template <class F>
 __device__ __forceinline__ void for_cells(const Geo& g, F f) {
   const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
   const unsigned sx = (nx + 63) >> 6, segs = sx * ny * (unsigned)g.n[2];
   const unsigned lane = threadIdx.x & 63;
-  const unsigned waves = gridDim.x * (blockDim.x >> 6);
-  for (unsigned sg = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); sg < segs;
-       sg += waves) {
+  const XcdShare sh = xcd_share(segs, blockDim.x >> 6, threadIdx.x >> 6);
+  for (unsigned sg = __builtin_amdgcn_readfirstlane(sh.first); sg < sh.end; sg += sh.step) {
     const unsigned row = sg / sx;
     const unsigned i = (sg - row * sx) * 64 + lane;
     if (i >= nx) continue;
@@ -183,6 +203,35 @@ __device__ __forceinline__ void for_cells(const Geo& g, F f) {
     const int q[3] = {(int)i, (int)(row - k * ny), (int)k};
     f((long long)i + (long long)nx * row, q);
   }
+}
+
+// The cells of the operator kernel: p_new = z_c (FIRST) or z_c + beta p_old,
+// Ap = A p_new, and the thread's part of p_new . Ap. FIRST is a template
+// argument so that the seven p values of a cell are loaded without a branch
+// between the loads (they are then all in flight at once).
+template <bool HAS_CLOSED, bool FIRST>
+__device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __restrict__ codes,
+                                               const double* __restrict__ z, const double* __restrict__ pold,
+                                               double* __restrict__ pnew, double* __restrict__ ap,
+                                               const double* __restrict__ mean, const int* __restrict__ cidx,
+                                               double beta) {
+  auto pv = [&](long long i) {
+    double zc = z[i];
+    // closed components: z centered (without any, the code load is skipped)
+    if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
+    if (FIRST) return zc;
+    return zc + beta * pold[i];
+  };
+  double acc = 0.0;
+  for_cells(g, [&](long long c, const int q[3]) {
+    uint8_t code = codes[c];
+    double pc = pv(c);
+    double y = apply_cell(g, code, c, q, pv);
+    pnew[c] = pc;
+    ap[c] = y;
+    acc += pc * y;
+  });
+  return acc;
 }
 
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
@@ -196,21 +245,8 @@ __global__ void __launch_bounds__(RB)
   const bool first = st->first;
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
-  auto pv = [&](long long i) {
-    double zc = z[i];
-    // closed components: z centered (without any, the code load is skipped)
-    if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
-    return first ? zc : zc + beta * pold[i];
-  };
-  double acc = 0.0;
-  for_cells(g, [&](long long c, const int q[3]) {
-    uint8_t code = codes[c];
-    double pc = pv(c);
-    double y = apply_cell(g, code, c, q, pv);
-    pnew[c] = pc;
-    ap[c] = y;
-    acc += pc * y;
-  });
+  const double acc = first ? pcg_op_cells<HAS_CLOSED, true>(g, codes, z, pold, pnew, ap, mean, cidx, beta)
+                           : pcg_op_cells<HAS_CLOSED, false>(g, codes, z, pold, pnew, ap, mean, cidx, beta);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -2023,7 +2059,8 @@ __global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict
 // (final since the previous half-sweep), writes both into z (out64, natural
 // layout: each wave fills whole cache lines) and adds both to r . z; the fixed
 // reduction grid (RP blocks of RB threads, tiles of 64 x 16 cells taken in turn)
-// writes the partials that k_pcg_rz would.
+// writes the partials that k_pcg_rz would; each XCD takes a contiguous share of
+// the tiles.
 template <class T>
 __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                  const double* __restrict__ r, const PcgState* st, double* part) {
@@ -2031,7 +2068,8 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __r
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1], nx = (unsigned)g.n[0];
   const unsigned gx = (hx + 63) >> 6, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
   double acc = 0.0;
-  for (unsigned tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  const XcdShare sh = xcd_share(tiles, 1, 0);
+  for (unsigned tile = sh.first; tile < sh.end; tile += sh.step) {
     const unsigned txy = tile % (gx * gy);
     const int i2 = (int)((txy % gx) * 64 + (threadIdx.x & 63));
     const int j = (int)((txy / gx) * 16 + (threadIdx.x >> 6));
@@ -2070,7 +2108,8 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned gx = (hx + 127) >> 7, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
   double acc = 0.0;
-  for (unsigned tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  const XcdShare sh = xcd_share(tiles, 1, 0);
+  for (unsigned tile = sh.first; tile < sh.end; tile += sh.step) {
     const unsigned txy = tile % (gx * gy);
     const int i2 = (int)((txy % gx) * 128 + 2 * (threadIdx.x & 63));
     const int j = (int)((txy / gx) * 16 + (threadIdx.x >> 6));

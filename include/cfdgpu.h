@@ -266,8 +266,10 @@ int cfd_project_device(cfd_grid* g, double* d_u, double* d_p, double dt, double 
  * on the grid's stream, from HIP events around `reps` back-to-back launches over
  * the grid's own work vectors, and the kernel's algorithmic HBM bytes per
  * launch (each array crossing HBM once). kernel: 0 fused p update + operator +
- * p.Ap; 1 fine-level red-black half-sweep of the multigrid smoother; 2 x/r
- * update with |r|^2; 3 one IC(0) application (both sweeps); 4 one V-cycle. */
+ * p.Ap of a first iteration (p = z); 1 fine-level red-black half-sweep of the
+ * multigrid smoother; 2 x/r update with |r|^2; 3 one IC(0) application (both
+ * sweeps); 4 one V-cycle; 5 the operator kernel of a later iteration
+ * (p = z + beta p_old). */
 int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, double* bytes_per_launch);
 
 /* Kernel classes of cfd_profile_solve. */
