@@ -236,6 +236,11 @@ struct cfd_grid {
   // red-black cycle: smoothing rounds (sweeps over all colours) before and after
   // the correction on the fine level and on the coarse levels (CFD_MG_SWEEPS=f,c)
   int fine_rounds = 2, coarse_rounds = 2;
+  // the coarse restriction takes the residual of the class relaxed last as zero
+  // (CFD_MG_ZERO_LAST=0: every residual is formed)
+  bool zero_last = true;
+  // stencil kernels marching through z on large grids (CFD_MARCH: 0 never, 1 by size, 2 always)
+  int march = 1;
   ~cfd_grid() {
     for (auto& kv : seg)
       if (kv.second) (void)hipGraphExecDestroy(kv.second);
@@ -838,6 +843,8 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
+  if (const char* env = std::getenv("CFD_MG_ZERO_LAST")) g->zero_last = std::string(env) != "0";
+  if (const char* env = std::getenv("CFD_MARCH")) g->march = std::atoi(env);
   if (const char* env = std::getenv("CFD_MG_SWEEPS")) {
     int f = 2, c = 2;
     if (std::sscanf(env, "%d,%d", &f, &c) == 2 && f >= 1 && f <= 8 && c >= 1 && c <= 8) {
@@ -936,12 +943,12 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
                                        g->a32.levels.data(), A.D, z, st, g->omega, g->stream,
                                        A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
-                                       g->part.p);
+                                       g->part.p, g->zero_last);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb ? z : nullptr, st, g->omega, g->stream,
                                         A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
-                                        g->part.p);
+                                        g->part.p, g->zero_last);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -971,7 +978,8 @@ void enqueue_iteration(cfd_grid* g, int parity) {
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p, g->stream);
+  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p, g->stream,
+                g->march);
   launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
   launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st);
@@ -1773,7 +1781,7 @@ int cfd_profile_kernel(cfd_grid* g, int kernel, int reps, double* avg_seconds, d
       case 0:
       case 5:
         launch_pcg_op(g->geo, g->codes.p, g->z.p, g->p0.p, g->p1.p, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,
-                      g->stream);
+                      g->stream, g->march);
         break;
       case 1:
         if (g->opt.cycle == CFD_CYCLE_RB && g->ash.has_rb) {

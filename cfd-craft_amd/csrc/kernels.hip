@@ -107,16 +107,9 @@ __device__ __forceinline__ long long nb_or_self(const Geo& g, long long c, int a
   return t < n - 1 ? c + s : (g.per[a] ? c - (long long)(n - 1) * s : c);
 }
 
-template <class XV>
-__device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long long c, const int q[3], XV xv) {
-  // every value first (independent loads), then the reference's operation order
-  const double xc = xv(c);
-  double nv[6];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    nv[2 * a] = xv(nb_or_self(g, c, a, q[a], 0));
-    nv[2 * a + 1] = xv(nb_or_self(g, c, a, q[a], 1));
-  }
+// (A x)_c from the cell's value xc and its six neighbour values nv (axis-major,
+// lower side first; entries of uncoupled sides are not read).
+__device__ __forceinline__ double apply_vals(const Geo& g, uint8_t code, const int q[3], double xc, const double nv[6]) {
   double y = 0.0;
   if (!(code & 0x3f)) return y;
 #pragma unroll
@@ -149,6 +142,19 @@ __device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long lo
     }
   }
   return y;
+}
+
+template <class XV>
+__device__ __forceinline__ double apply_cell(const Geo& g, uint8_t code, long long c, const int q[3], XV xv) {
+  // every value first (independent loads), then the reference's operation order
+  const double xc = xv(c);
+  double nv[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nv[2 * a] = xv(nb_or_self(g, c, a, q[a], 0));
+    nv[2 * a + 1] = xv(nb_or_self(g, c, a, q[a], 1));
+  }
+  return apply_vals(g, code, q, xc, nv);
 }
 
 __device__ __forceinline__ double mean_of(uint8_t code, long long c, const double* mean, const int* cidx) {
@@ -247,6 +253,117 @@ __global__ void __launch_bounds__(RB)
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc = first ? pcg_op_cells<HAS_CLOSED, true>(g, codes, z, pold, pnew, ap, mean, cidx, beta)
                            : pcg_op_cells<HAS_CLOSED, false>(g, codes, z, pold, pnew, ap, mean, cidx, beta);
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) {
+    part[P_PAP + blockIdx.x] = s;
+    if (blockIdx.x == 0 && !first) st->beta = beta;
+  }
+}
+
+// The same kernel for large grids, marching through z: work items are tiles of
+// MX x MY columns (x, y) over MZ layers, dealt to the blocks of the fixed grid
+// from each XCD's contiguous share (tiles of one layer range side by side). A
+// thread keeps its column's p at layers k - 1, k and k + 1 in registers and
+// writes layer k into an LDS tile, so each p value of the grid is formed once
+// per item (the z neighbours come from registers, the x and y neighbours inside
+// the tile from LDS; only those across the tile's edge are formed again from
+// global memory). Every p value and every (A p)_c repeats pcg_op_cells'
+// arithmetic bit for bit.
+constexpr int MX = 64, MY = 16, MZ = 16;
+static_assert(MX * MY == RB, "one thread per column of a tile");
+struct MarchShape {
+  int tx, ty, tz;  // tiles in x, y and layer ranges in z
+};
+inline MarchShape march_shape(const Geo& g) {
+  return MarchShape{(g.n[0] + MX - 1) / MX, (g.n[1] + MY - 1) / MY, (g.n[2] + MZ - 1) / MZ};
+}
+
+template <bool HAS_CLOSED, bool FIRST>
+__device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
+                                               const double* __restrict__ z, const double* __restrict__ pold,
+                                               double* __restrict__ pnew, double* __restrict__ ap,
+                                               const double* __restrict__ mean, const int* __restrict__ cidx,
+                                               double beta, double (*tile)[MY][MX]) {
+  auto pv = [&](long long i) {
+    double zc = z[i];
+    if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
+    if (FIRST) return zc;
+    return zc + beta * pold[i];
+  };
+  const int lx = (int)threadIdx.x % MX, ly = (int)threadIdx.x / MX;
+  const unsigned items = (unsigned)(ms.tx * ms.ty * ms.tz);
+  const XcdShare sh = xcd_share(items, 1, 0);
+  double acc = 0.0;
+  for (unsigned item = sh.first; item < sh.end; item += sh.step) {
+    const int txi = (int)(item % (unsigned)ms.tx);
+    const int tyi = (int)((item / (unsigned)ms.tx) % (unsigned)ms.ty);
+    const int tzi = (int)(item / (unsigned)(ms.tx * ms.ty));
+    const int ox = txi * MX, oy = tyi * MY;
+    const int i = ox + lx, j = oy + ly;
+    const int k0 = tzi * MZ, k1 = min(g.n[2], k0 + MZ);
+    const bool live = i < g.n[0] && j < g.n[1];
+    const long long col = (long long)i + (long long)j * g.sy;
+    double pm = 0.0, pc = 0.0;
+    if (live) {
+      const long long c0 = col + (long long)k0 * g.sz;
+      pc = pv(c0);
+      pm = pv(nb_or_self(g, c0, 2, k0, 0));
+    }
+    for (int k = k0; k < k1; ++k) {
+      const long long c = col + (long long)k * g.sz;
+      const int q[3] = {i, j, k};
+      double pn = 0.0;
+      uint8_t code = 0;
+      if (live) {
+        pn = pv(nb_or_self(g, c, 2, k, 1));
+        code = codes[c];
+      }
+      double(*t)[MX] = tile[k & 1];
+      t[ly][lx] = pc;
+      __syncthreads();
+      if (live) {
+        double nv[6];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            // the neighbour's coordinate on axis a (nb_or_self: wrapping, or the
+            // cell itself where the axis ends)
+            const int n = g.n[a], t0 = q[a];
+            const int tn = side == 0 ? (t0 > 0 ? t0 - 1 : (g.per[a] ? n - 1 : t0))
+                                     : (t0 < n - 1 ? t0 + 1 : (g.per[a] ? 0 : t0));
+            const int ti = (a == 0 ? tn : i) - ox, tj = (a == 1 ? tn : j) - oy;
+            const long long nb = c + (long long)(tn - t0) * (a == 0 ? 1 : g.sy);
+            // inside the tile (and then live: a neighbour index is inside the grid)
+            nv[2 * a + side] = (ti >= 0 && ti < MX && tj >= 0 && tj < MY) ? t[tj][ti] : pv(nb);
+          }
+        nv[4] = pm;
+        nv[5] = pn;
+        const double y = apply_vals(g, code, q, pc, nv);
+        pnew[c] = pc;
+        ap[c] = y;
+        acc += pc * y;
+      }
+      pm = pc;
+      pc = pn;
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
+template <bool HAS_CLOSED>
+__global__ void __launch_bounds__(RB)
+    k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const double* __restrict__ z,
+                   const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
+                   const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part) {
+  __shared__ double tile[2][MY][MX];
+  if (!block_running(st)) return;
+  const bool first = st->first;
+  double beta = 0.0;
+  if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
+  const double acc = first ? pcg_op_march<HAS_CLOSED, true>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile)
+                           : pcg_op_march<HAS_CLOSED, false>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -2671,12 +2788,23 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
   hipLaunchKernelGGL(k_apply, dim3(blocks_for(g.N)), dim3(BS), 0, s, g, codes, x, y);
 }
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
-                   const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s) {
+                   const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
+                   int march) {
   probe_begin(PROBE_OPERATOR, s);
-  if (closed)
+  const MarchShape ms = march_shape(g);
+  // marching pays once the grid gives every block of the fixed grid an item
+  if (march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP)) {
+    if (closed)
+      hipLaunchKernelGGL(k_pcg_op_march<true>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew, ap, mean, cidx, st,
+                         part);
+    else
+      hipLaunchKernelGGL(k_pcg_op_march<false>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew, ap, mean, cidx,
+                         st, part);
+  } else if (closed) {
     hipLaunchKernelGGL(k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
-  else
+  } else {
     hipLaunchKernelGGL(k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+  }
   // z, p_old, code in; p_new, Ap out
   probe_end(PROBE_OPERATOR, s, 33.0 * (double)g.N);
 }
@@ -2957,7 +3085,7 @@ template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds,
-                       const double* rz_r, double* rz_part) {
+                       const double* rz_r, double* rz_part, bool zero_last) {
   // a round is one sweep over every colour class; pre-smoothing runs the classes in
   // ascending order, post-smoothing in descending order (the adjoint)
   const int fr = fine_rounds, cr = coarse_rounds;
@@ -3038,8 +3166,9 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // the first half-sweep (colour 0 from zero) happened in the restriction
     for (int round = 0; round < cr; ++round)
       for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) relax(L, c);
-    // pre-smoothing ended with the last colour class (without rounds nothing is taken as zero)
-    const int zero_from = cr >= 1 ? L.cptr[L.ncol - 1] : L.n;
+    // pre-smoothing ended with the last colour class, whose residual is zero to
+    // rounding (without rounds, or with zero_last off, nothing is taken as zero)
+    const int zero_from = cr >= 1 && zero_last ? L.cptr[L.ncol - 1] : L.n;
     probe_begin(PROBE_COARSE_RESTRICT, s);
     hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st,
                        zero_from);
@@ -3111,11 +3240,11 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
                                                 const PcgState*, double, hipStream_t, const FineRB<double>*, int, int,
-                                                const double*, double*);
+                                                const double*, double*, bool);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
                                                double, hipStream_t, const FineRB<float>*, int, int, const double*,
-                                               double*);
+                                               double*, bool);
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {

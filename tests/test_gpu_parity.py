@@ -283,6 +283,72 @@ def test_odd_periodic_multigrid_is_deterministic(gpu_available, cycle):
     assert info["relative_residual"] <= 1.01e-9
 
 
+MARCH_CASES = [
+    ("urban", lambda: W.urban_block(n=(70, 50, 40), boxes=8)[0]),
+    ("periodic-odd", lambda: Case(cells=[67, 33, 21], size_m=[6.7, 3.3, 2.1], boundaries=[PERIODIC, PERIODIC, PERIODIC],
+                                  blocks=[Block([1.0, 1.0, 0.0], [2.0, 2.0, 2.1])], preconditioner=MULTIGRID)),
+    ("closed", lambda: W.cavity([40, 18, 36], MULTIGRID)),
+    ("ic0-closed", lambda: W.cavity([36, 20, 18], IC0)),
+    ("courtyards", None),
+]
+
+
+@pytest.mark.parametrize("name,make", MARCH_CASES, ids=[n for n, _ in MARCH_CASES])
+def test_marching_operator_solves_like_cell_by_cell(gpu_available, name, make):
+    """The operator kernel that marches through z (tiles of 64 x 16 columns over
+    16 layers, p of the layers above and below in registers, the layer in LDS;
+    used on large grids, forced here) forms every p and (A p)_c with the
+    cell-by-cell kernel's arithmetic: the same iteration counts and the same
+    fields up to the order of the p . Ap sums, on partial tiles, periodic axes
+    and closed components."""
+    if make is None:
+        from test_gpu_components import large_courtyards
+
+        case = large_courtyards()
+    else:
+        case = make()
+    o = oracle.Grid(case)
+    ex = o.center_nullspaces(np.where(o.fluid(), np.random.default_rng(4).standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    out = []
+    for march in (2, 0):
+        with env(CFD_MARCH=march):
+            g = cfdgpu.Grid(case)
+        print(f"[march] {name} march={march}", flush=True)
+        out.append(g.solve(b, 1e-11, 3000))
+    (x2, i2), (x0, i0) = out
+    assert i2["iterations"] == i0["iterations"], (i2["iterations"], i0["iterations"])
+    assert rel(x2, x0) <= 1e-11, rel(x2, x0)
+    assert rel(x2, ex) <= 1e-8
+
+
+@pytest.mark.parametrize("name", ["odd-periodic-coarse", "odd-periodic-fine", "urban"])
+def test_coarse_restriction_zero_residual_of_the_last_class(gpu_available, name):
+    """The coarse restriction takes the residual of the colour class relaxed last
+    as zero (after its relaxation it is zero to rounding). On levels coloured
+    greedily because of odd periodic seams (36 -> 18 -> 9 cells) as on the
+    two-coloured ones, one FP64 V-cycle with that shortcut equals the V-cycle
+    that forms every residual to rounding; taking any other class as zero would
+    change it at order one."""
+    if name == "odd-periodic-coarse":
+        case = W.cavity([40, 36, 28], MULTIGRID)
+    elif name == "odd-periodic-fine":
+        case = Case(cells=[27, 20, 18], size_m=[2.7, 2.0, 1.8], boundaries=[PERIODIC, PERIODIC, WALL],
+                    blocks=[Block([0.5, 0.5, 0.0], [1.1, 1.2, 0.9])], preconditioner=MULTIGRID)
+    else:
+        case = W.urban_block(n=(64, 48, 40), boxes=8)[0]
+    rng = np.random.default_rng(13)
+    r = rng.standard_normal(case.count)
+    out = []
+    for zero_last in (1, 0):
+        with env(CFD_MG_DENSE=8, CFD_MG_ZERO_LAST=zero_last):
+            g = cfdgpu.Grid(case)
+        if name != "urban":
+            r = g.center_nullspaces(r)
+        out.append(g.precondition(r))
+    assert rel(out[0], out[1]) <= 1e-12, rel(out[0], out[1])
+
+
 @pytest.mark.parametrize("cells", [[27, 20, 18], [33, 21, 16]])
 def test_odd_periodic_fine_level_colours(gpu_available, cells):
     """Periodic axes of odd length on the fine level (seam colour classes): the
