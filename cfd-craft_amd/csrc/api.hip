@@ -191,6 +191,7 @@ struct cfd_grid {
   int64_t agg_dense = kAggDense;
   bool rb_layout = true;  // CFD_MG_RB_LAYOUT=0: the fine level in the natural layout
   int rb_pairs = -1;      // CFD_MG_RB_PAIRS: two fine cells per thread (-1 by size, 0 never, 1 when hx is even)
+  bool rb_lin = true;     // CFD_MG_RB_LIN=0: the two-cell half-sweep in tiles instead of linear pairs
   bool ic0_tiles = true;  // CFD_IC0_TILES=0: one launch per hyperplane
   // a hierarchy built since the last solve of its cycle (MultigridWork::hierarchy_builds)
   bool unreported_build[4] = {false, false, false, false};
@@ -567,8 +568,14 @@ int build_agg(cfd_grid* g) {
     // bytes in flight (2M cells: 10.5 against 8.8 us, 3.5M: 14.2 against 11.4,
     // 8.4M: 23.0 against 18.2). CFD_MG_RB_PAIRS=0 / 1 forces.
     const int two = h.n[0] % 4 == 0 && (g->rb_pairs == 1 || (g->rb_pairs < 0 && half >= 3 * (1LL << 18)));
-    A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p, two, A.rb_rpar.p, A.rb_bpar.p};
-    A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p, two, A.rb_rpar.p, A.rb_bpar.p};
+    // two cells per thread: consecutive threads on consecutive pairs (CFD_MG_RB_LIN=0: tiles of 128 x 4)
+    const int pairs_mode = two ? (g->rb_lin ? 2 : 1) : 0;
+    A.rb64 = FineRB<double>{half, hx, A.rb_codes.p, A.rb_b64.p, A.rb_x64.p, pairs_mode, A.rb_rpar.p, A.rb_bpar.p};
+    A.rb32 = FineRB<float>{half, hx, A.rb_codes.p, A.rb_b32.p, A.rb_x32.p, pairs_mode, A.rb_rpar.p, A.rb_bpar.p};
+    const unsigned long long two40 = 1ULL << 40;
+    for (unsigned long long* m : {&A.rb64.mhx, &A.rb32.mhx}) *m = (two40 + (unsigned long long)hx - 1) / (unsigned long long)hx;
+    for (unsigned long long* m : {&A.rb64.mny, &A.rb32.mny})
+      *m = (two40 + (unsigned long long)h.n[1] - 1) / (unsigned long long)h.n[1];
     A.has_rb = true;
   }
   // the host copies of the coarse matrices are no longer needed
@@ -840,6 +847,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_RB_LAYOUT")) g->rb_layout = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_RB_PAIRS")) g->rb_pairs = std::atoi(env) ? 1 : 0;
+  if (const char* env = std::getenv("CFD_MG_RB_LIN")) g->rb_lin = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);

@@ -111,10 +111,12 @@ struct FineRB {
   const uint8_t* codes;
   T* b;
   T* x;
-  int two_cells;     // half-sweeps take two cells per thread (nx a multiple of 4; chosen at set-up)
+  int two_cells;     // half-sweeps take two cells per thread (nx a multiple of 4; chosen at set-up): 1 in
+                     // tiles of 128 x 4 entries, 2 consecutive threads on consecutive pairs of a colour
   const int* rpar;   // per red cell (red layout): node of the first coarse level, -1 without one
   const int* bpar;   // per 2x2x2 block of cells (x fastest): the common node of the first coarse
                      // level of its cells in the grid (-1: all solid), kMixedParent when they differ
+  unsigned long long mhx = 0, mny = 0;  // ceil(2^40 / hx), ceil(2^40 / ny): row and layer of a linear entry
 };
 constexpr int kMixedParent = -2147483647 - 1;
 

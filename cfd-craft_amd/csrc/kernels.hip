@@ -242,8 +242,9 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
 
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
+// (registers sized for 8 waves per SIMD: two blocks per CU)
 template <bool HAS_CLOSED>
-__global__ void __launch_bounds__(RB)
+__global__ void __launch_bounds__(RB, 8)
     k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ z, const double* __restrict__ pold,
              double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
              const int* __restrict__ cidx, PcgState* st, double* part) {
@@ -264,11 +265,10 @@ __global__ void __launch_bounds__(RB)
 // MX x MY columns (x, y) over MZ layers, dealt to the blocks of the fixed grid
 // from each XCD's contiguous share (tiles of one layer range side by side). A
 // thread keeps its column's p at layers k - 1, k and k + 1 in registers and
-// writes layer k into an LDS tile, so each p value of the grid is formed once
-// per item (the z neighbours come from registers, the x and y neighbours inside
-// the tile from LDS; only those across the tile's edge are formed again from
-// global memory). Every p value and every (A p)_c repeats pcg_op_cells'
-// arithmetic bit for bit.
+// writes layer k into an LDS tile with a one-cell rim (the x and y neighbours
+// of the edge cells, formed again by the first threads), so each p value of the
+// grid is formed once per item and every neighbour comes from registers or LDS.
+// Every p value and every (A p)_c repeats pcg_op_cells' arithmetic bit for bit.
 constexpr int MX = 64, MY = 16, MZ = 16;
 static_assert(MX * MY == RB, "one thread per column of a tile");
 struct MarchShape {
@@ -283,7 +283,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
                                                const double* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
-                                               double beta, double (*tile)[MY][MX]) {
+                                               double beta, double (*tile)[MY + 2][MX + 2]) {
   auto pv = [&](long long i) {
     double zc = z[i];
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
@@ -299,44 +299,87 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
     const int tyi = (int)((item / (unsigned)ms.tx) % (unsigned)ms.ty);
     const int tzi = (int)(item / (unsigned)(ms.tx * ms.ty));
     const int ox = txi * MX, oy = tyi * MY;
+    const int wx = min(MX, g.n[0] - ox), wy = min(MY, g.n[1] - oy);  // live columns of the tile
     const int i = ox + lx, j = oy + ly;
     const int k0 = tzi * MZ, k1 = min(g.n[2], k0 + MZ);
-    const bool live = i < g.n[0] && j < g.n[1];
+    const bool live = lx < wx && ly < wy;
     const long long col = (long long)i + (long long)j * g.sy;
-    double pm = 0.0, pc = 0.0;
+    // The rim: the x and y neighbours (nb_or_self: wrapping, or the edge cell
+    // itself where the axis ends) of the tile's edge cells, one slot each,
+    // formed by thread r < 2 (wx + wy): the bottom and top rows, then the left
+    // and right columns.
+    const int r = (int)threadIdx.x;
+    const bool rim = r < 2 * (wx + wy);
+    int sx = 0, sy = 0;
+    long long rcol = 0;
+    if (rim) {
+      int ex, ey, axis, side;
+      if (r < wx) {
+        ex = r, ey = 0, axis = 1, side = 0, sx = r + 1, sy = 0;
+      } else if (r < 2 * wx) {
+        ex = r - wx, ey = wy - 1, axis = 1, side = 1, sx = r - wx + 1, sy = wy + 1;
+      } else if (r < 2 * wx + wy) {
+        ex = 0, ey = r - 2 * wx, axis = 0, side = 0, sx = 0, sy = r - 2 * wx + 1;
+      } else {
+        ex = wx - 1, ey = r - 2 * wx - wy, axis = 0, side = 1, sx = wx + 1, sy = r - 2 * wx - wy + 1;
+      }
+      const int eq[2] = {ox + ex, oy + ey};
+      const int n = g.n[axis], t0 = eq[axis];
+      const int tn = side == 0 ? (t0 > 0 ? t0 - 1 : (g.per[axis] ? n - 1 : t0))
+                               : (t0 < n - 1 ? t0 + 1 : (g.per[axis] ? 0 : t0));
+      const int nxy[2] = {axis == 0 ? tn : eq[0], axis == 1 ? tn : eq[1]};
+      rcol = (long long)nxy[0] + (long long)nxy[1] * g.sy;
+    }
+    // software pipeline: the values of the next PD layers (the column's p, its
+    // code and the rim slot's p) are in flight while the current layer is
+    // worked on. p of layer L = k0 - 1 .. k1 is the column's z neighbour at the
+    // ends (nb_or_self), the column's own value in between.
+    constexpr int PD = 3;
+    auto p_at = [&](int L) {
+      if (L < k0) return pv(nb_or_self(g, col + (long long)k0 * g.sz, 2, k0, 0));
+      if (L >= k1) return pv(nb_or_self(g, col + (long long)(k1 - 1) * g.sz, 2, k1 - 1, 1));
+      return pv(col + (long long)L * g.sz);
+    };
+    double pm = 0.0, pc = 0.0, rv = 0.0;
+    double pq[PD] = {}, rq[PD] = {};
+    uint8_t code = 0, cq[PD] = {};
     if (live) {
-      const long long c0 = col + (long long)k0 * g.sz;
-      pc = pv(c0);
-      pm = pv(nb_or_self(g, c0, 2, k0, 0));
+      pm = p_at(k0 - 1);
+      pc = p_at(k0);
+      code = codes[col + (long long)k0 * g.sz];
+#pragma unroll
+      for (int d = 0; d < PD; ++d) {
+        if (k0 + 1 + d <= k1) pq[d] = p_at(k0 + 1 + d);
+        if (k0 + 1 + d < k1) cq[d] = codes[col + (long long)(k0 + 1 + d) * g.sz];
+      }
+    }
+    if (rim) {
+      rv = pv(rcol + (long long)k0 * g.sz);
+#pragma unroll
+      for (int d = 0; d < PD; ++d)
+        if (k0 + 1 + d < k1) rq[d] = pv(rcol + (long long)(k0 + 1 + d) * g.sz);
     }
     for (int k = k0; k < k1; ++k) {
       const long long c = col + (long long)k * g.sz;
       const int q[3] = {i, j, k};
-      double pn = 0.0;
-      uint8_t code = 0;
-      if (live) {
-        pn = pv(nb_or_self(g, c, 2, k, 1));
-        code = codes[c];
-      }
-      double(*t)[MX] = tile[k & 1];
-      t[ly][lx] = pc;
+      // layer k + 1 + PD enters the pipeline
+      const int L = k + 1 + PD;
+      double pin = 0.0, rin = 0.0;
+      uint8_t cin = 0;
+      if (live && L <= k1) pin = p_at(L);
+      if (live && L < k1) cin = codes[col + (long long)L * g.sz];
+      if (rim && L < k1) rin = pv(rcol + (long long)L * g.sz);
+      const double pn = pq[0];
+      double(*t)[MX + 2] = tile[k & 1];
+      if (live) t[ly + 1][lx + 1] = pc;
+      if (rim) t[sy][sx] = rv;
       __syncthreads();
       if (live) {
         double nv[6];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int side = 0; side < 2; ++side) {
-            // the neighbour's coordinate on axis a (nb_or_self: wrapping, or the
-            // cell itself where the axis ends)
-            const int n = g.n[a], t0 = q[a];
-            const int tn = side == 0 ? (t0 > 0 ? t0 - 1 : (g.per[a] ? n - 1 : t0))
-                                     : (t0 < n - 1 ? t0 + 1 : (g.per[a] ? 0 : t0));
-            const int ti = (a == 0 ? tn : i) - ox, tj = (a == 1 ? tn : j) - oy;
-            const long long nb = c + (long long)(tn - t0) * (a == 0 ? 1 : g.sy);
-            // inside the tile (and then live: a neighbour index is inside the grid)
-            nv[2 * a + side] = (ti >= 0 && ti < MX && tj >= 0 && tj < MY) ? t[tj][ti] : pv(nb);
-          }
+        nv[0] = t[ly + 1][lx];
+        nv[1] = t[ly + 1][lx + 2];
+        nv[2] = t[ly][lx + 1];
+        nv[3] = t[ly + 2][lx + 1];
         nv[4] = pm;
         nv[5] = pn;
         const double y = apply_vals(g, code, q, pc, nv);
@@ -346,6 +389,17 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
       }
       pm = pc;
       pc = pn;
+      rv = rq[0];
+      code = cq[0];
+#pragma unroll
+      for (int d = 0; d + 1 < PD; ++d) {
+        pq[d] = pq[d + 1];
+        rq[d] = rq[d + 1];
+        cq[d] = cq[d + 1];
+      }
+      pq[PD - 1] = pin;
+      rq[PD - 1] = rin;
+      cq[PD - 1] = cin;
     }
     __syncthreads();
   }
@@ -353,11 +407,11 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
 }
 
 template <bool HAS_CLOSED>
-__global__ void __launch_bounds__(RB)
+__global__ void __launch_bounds__(RB, 8)
     k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const double* __restrict__ z,
                    const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
                    const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part) {
-  __shared__ double tile[2][MY][MX];
+  __shared__ double tile[2][MY + 2][MX + 2];
   if (!block_running(st)) return;
   const bool first = st->first;
   double beta = 0.0;
@@ -408,7 +462,7 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
 
 // When the recurrent residual |r|/|b_c| reached tol: partials of |A x - b_c|^2,
 // the independent recomputed residual (pressure.rs:403-414).
-__global__ void __launch_bounds__(RB) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
+__global__ void __launch_bounds__(RB, 8) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
                                                  const double* __restrict__ bc, PcgState* st, double* part) {
   if (!block_running(st)) return;
   const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
@@ -2073,13 +2127,60 @@ __global__ void __launch_bounds__(BS) k_rb_gs(Geo g, FineRB<T> F, int colour, co
   if (out64) out64[(long long)i + j * g.sy + k * g.sz] = (double)v;
 }
 
+// Gauss-Seidel values of the colour-`colour` pair at entries i2 and i2 + 1 of row
+// (j, k) (offset u in the colour's array; nx a multiple of 4, so the pair is
+// aligned and complete): its right-hand sides, its neighbours in y and z and
+// the row segment around it move as 16-byte (FP64) or 8-byte (FP32) pairs;
+// each cell's arithmetic is rb_update's.
+template <class T>
+__device__ __forceinline__ void rb_pair(const Geo& g, const FineRB<T>& F, int colour, int i2, int j, int k, unsigned u,
+                                        T& va, T& vb) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const int par = (j + k + colour) & 1;
+  const int ia = 2 * i2 + par, ib = ia + 2;
+  const T* __restrict__ xo = F.x + (1 - colour) * F.half;
+  const long long own = colour * F.half + u;
+  const T2 bv = *reinterpret_cast<const T2*>(F.b + own);
+  const uint8_t ca = F.codes[own], cbc = F.codes[own + 1];
+  // x neighbours: entries i2 - 1 + par .. i2 + 1 + par of the other colour's row
+  const unsigned row = hx * ((unsigned)j + ny * (unsigned)k);
+  const T2 mid = *reinterpret_cast<const T2*>(xo + row + (unsigned)i2);
+  int e = par ? i2 + 2 : i2 - 1;
+  if (e < 0) e = g.per[0] ? (int)hx - 1 : 0;
+  if (e >= (int)hx) e = g.per[0] ? 0 : (int)hx - 1;
+  const T edge = xo[row + (unsigned)e];
+  T yz[4][2];
+#pragma unroll
+  for (int a = 1; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int qa[3] = {ia, j, k};
+      int qn[3];
+      nb_coords(g, qa, a, side, qn);
+      const T2 v = *reinterpret_cast<const T2*>(xo + (unsigned)i2 + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2]));
+      yz[2 * (a - 1) + side][0] = v.x;
+      yz[2 * (a - 1) + side][1] = v.y;
+    }
+  T xa[6], xb[6];
+  xa[0] = par ? mid.x : edge;
+  xa[1] = par ? mid.y : mid.x;
+  xb[0] = par ? mid.y : mid.x;
+  xb[1] = par ? edge : mid.y;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    xa[2 + f] = yz[f][0];
+    xb[2 + f] = yz[f][1];
+  }
+  const int qa[3] = {ia, j, k}, qb[3] = {ib, j, k};
+  va = rb_relax<T>(g, ca, bv.x, qa, xa);
+  vb = rb_relax<T>(g, cbc, bv.y, qb, xb);
+}
+
 // One half-sweep with two cells of the colour per thread (entries i2 and i2 + 1
-// of a colour row, tiles of 128 x 4): the pair's right-hand sides, its
-// neighbours in y and z and its new values move as 16-byte (FP64) or 8-byte
-// (FP32) pairs. Needs nx a multiple of 4, so that every pair is aligned and
-// complete. Each cell's
-// arithmetic is rb_update's. With xc (the prolongation riding on the reads) each
-// cell takes rb_update itself; the V-cycle launches that pass one cell per thread.
+// of a colour row, tiles of 128 x 4). Needs nx a multiple of 4. With xc (the
+// prolongation riding on the reads) each cell takes rb_update itself; the
+// V-cycle launches that pass one cell per thread.
 template <class T>
 __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, const T* __restrict__ xc,
                                                const int* __restrict__ fparent, T cs, double* __restrict__ out64,
@@ -2103,42 +2204,7 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
     va = rb_update<T>(g, F, colour, qa, u, xc, fparent, cs);
     vb = rb_update<T>(g, F, colour, qb, u + 1, xc, fparent, cs);
   } else {
-    const T* __restrict__ xo = F.x + (1 - colour) * F.half;
-    const long long own = colour * F.half + u;
-    const T2 bv = *reinterpret_cast<const T2*>(F.b + own);
-    const uint8_t ca = F.codes[own], cbc = F.codes[own + 1];
-    // x neighbours: entries i2 - 1 + par .. i2 + 1 + par of the other colour's row
-    const unsigned row = hx * ((unsigned)j + ny * (unsigned)k);
-    const T2 mid = *reinterpret_cast<const T2*>(xo + row + (unsigned)i2);
-    int e = par ? i2 + 2 : i2 - 1;
-    if (e < 0) e = g.per[0] ? (int)hx - 1 : 0;
-    if (e >= (int)hx) e = g.per[0] ? 0 : (int)hx - 1;
-    const T edge = xo[row + (unsigned)e];
-    T yz[4][2];
-#pragma unroll
-    for (int a = 1; a < 3; ++a)
-#pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        const int qa[3] = {ia, j, k};
-        int qn[3];
-        nb_coords(g, qa, a, side, qn);
-        const T2 v = *reinterpret_cast<const T2*>(xo + (unsigned)i2 + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2]));
-        yz[2 * (a - 1) + side][0] = v.x;
-        yz[2 * (a - 1) + side][1] = v.y;
-      }
-    T xa[6], xb[6];
-    xa[0] = par ? mid.x : edge;
-    xa[1] = par ? mid.y : mid.x;
-    xb[0] = par ? mid.y : mid.x;
-    xb[1] = par ? edge : mid.y;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      xa[2 + f] = yz[f][0];
-      xb[2 + f] = yz[f][1];
-    }
-    const int qa[3] = {ia, j, k}, qb[3] = {ib, j, k};
-    va = rb_relax<T>(g, ca, bv.x, qa, xa);
-    vb = rb_relax<T>(g, cbc, bv.y, qb, xb);
+    rb_pair<T>(g, F, colour, i2, j, k, u, va, vb);
   }
   if (status != ST_RUNNING) return;
   T2 out;
@@ -2147,6 +2213,43 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
   *reinterpret_cast<T2*>(F.x + colour * F.half + u) = out;
   if (out64) {
     const long long c = (long long)ia + j * g.sy + k * g.sz;
+    out64[c] = (double)va;
+    out64[c + 2] = (double)vb;
+  }
+}
+
+// floor(n / d) for n < 2^31 and d < 2^20 from m = ceil(2^40 / d) (exact there:
+// the error n (m - 2^40 / d) / 2^40 stays below 1 / d).
+__device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m) {
+  return (unsigned)(((unsigned long long)n * m) >> 40);
+}
+
+// The same half-sweep with consecutive threads on consecutive pairs of the
+// colour's array (no idle lanes where a row is not a multiple of the tile
+// width, whole-wave linear streams); each XCD takes a contiguous range of blocks.
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colour, double* __restrict__ out64,
+                                                   const PcgState* st) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned pair = (unsigned)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned u = 2 * pair;
+  if (u >= (unsigned)F.half) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned row = div_magic(u, F.mhx);
+  const int i2 = (int)(u - row * hx);
+  const unsigned k = div_magic(row, F.mny);
+  const int j = (int)(row - k * ny);
+  T va, vb;
+  rb_pair<T>(g, F, colour, i2, j, (int)k, u, va, vb);
+  if (status != ST_RUNNING) return;
+  T2 out;
+  out.x = va;
+  out.y = vb;
+  *reinterpret_cast<T2*>(F.x + colour * F.half + u) = out;
+  if (out64) {
+    const int ia = 2 * i2 + ((j + (int)k + colour) & 1);
+    const long long c = (long long)ia + j * g.sy + (long long)k * g.sz;
     out64[c] = (double)va;
     out64[c + 2] = (double)vb;
   }
@@ -2179,7 +2282,7 @@ __global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict
 // writes the partials that k_pcg_rz would; each XCD takes a contiguous share of
 // the tiles.
 template <class T>
-__global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __restrict__ out64,
+__global__ void __launch_bounds__(RB, 8) k_rb_gs_rz(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                  const double* __restrict__ r, const PcgState* st, double* part) {
   if (!block_running(st)) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1], nx = (unsigned)g.n[0];
@@ -2218,7 +2321,7 @@ __global__ void __launch_bounds__(RB) k_rb_gs_rz(Geo g, FineRB<T> F, double* __r
 // have r read and z written as two 16-byte pairs. Each red cell's arithmetic is
 // rb_update's; r . z adds, per thread, red a, black a, red b, black b.
 template <class T>
-__global__ void __launch_bounds__(RB) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
+__global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                   const double* __restrict__ r, const PcgState* st, double* part) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   if (!block_running(st)) return;
@@ -2285,7 +2388,10 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
   // the pass carrying the prolongation keeps one cell per thread (its neighbour
   // reads gather the coarse correction cell by cell)
   probe_begin(PROBE_FINE_HALF, s);
-  if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
+  if (F.two_cells == 2 && g0.n[0] % 4 == 0 && !xc)
+    hipLaunchKernelGGL((k_rb_gs2_lin<T>), dim3((unsigned)((F.half / 2 + BS - 1) / BS)), dim3(BS), 0, s, g0, F, colour,
+                       out64, st);
+  else if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
     hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
                        F, colour, xc, fparent, cs, out64, st);
   else

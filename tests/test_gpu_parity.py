@@ -381,9 +381,10 @@ LAYOUT_CASES = [("urban", [64, 48, 40]), ("urban", [33, 27, 21]), ("urban", [130
 @pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
 def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, precision):
     """The red-black cycle with the fine level stored colour by colour, with one
-    or (nx a multiple of 4) two cells per thread, gives the same bits as with the
-    natural layout (same per-cell arithmetic), odd sizes and periodic axes of 1
-    and 2 cells included."""
+    or (nx a multiple of 4) two cells per thread (in tiles, or consecutive
+    threads on consecutive pairs), gives the same bits as with the natural
+    layout (same per-cell arithmetic), odd sizes and periodic axes of 1 and 2
+    cells included."""
     if kind == "urban":
         case = W.urban_block(n=tuple(cells), boxes=6)[0]
     else:
@@ -392,13 +393,13 @@ def test_colour_separated_fine_level_is_bitwise(gpu_available, kind, cells, prec
     if kind == "cavity":
         r -= r.mean()  # one closed component
     out = []
-    for layout, pairs in ((1, 0), (1, 1), (0, 0)):
-        with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_RB_PAIRS=pairs, CFD_MG_DENSE=64):
+    for layout, pairs, lin in ((1, 0, 1), (1, 1, 1), (0, 0, 1), (1, 1, 0)):
+        with env(CFD_MG_RB_LAYOUT=layout, CFD_MG_RB_PAIRS=pairs, CFD_MG_RB_LIN=lin, CFD_MG_DENSE=64):
             g = cfdgpu.Grid(case)
         g.set_options(precision)
-        print(f"[layout] {kind} {cells} precision={precision} layout={layout} pairs={pairs}", flush=True)
+        print(f"[layout] {kind} {cells} precision={precision} layout={layout} pairs={pairs} linear={lin}", flush=True)
         out.append(g.precondition(r))
-    for k in (1, 2):
+    for k in (1, 2, 3):
         assert np.array_equal(out[0], out[k]), (k, float(np.max(np.abs(out[0] - out[k]))))
 
 
