@@ -2315,15 +2315,55 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz(Geo g, FineRB<T> F, double* 
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
 }
 
+// The red pair at entries i2, i2 + 1 of row (j, k) (offset u) relaxed, and the
+// four natural cells 2 i2 .. 2 i2 + 3 (red and black alternating) with z written
+// as two 16-byte pairs and r . z added to acc in the order red a, black a, red
+// b, black b.
+template <class T>
+__device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
+                                        const double* __restrict__ r, int i2, int j, int k, unsigned u, double& acc) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const long long row = (long long)j * g.sy + k * g.sz;
+  const int par = (j + k) & 1;  // red at 2 i2 + par and 2 i2 + 2 + par
+  T va, vb;
+  rb_pair<T>(g, F, 0, i2, j, k, u, va, vb);
+  T2 red;
+  red.x = va;
+  red.y = vb;
+  *reinterpret_cast<T2*>(F.x + u) = red;
+  const T2 black = *reinterpret_cast<const T2*>(F.x + F.half + u);
+  const double2 r01 = *reinterpret_cast<const double2*>(r + row + 2 * i2);
+  const double2 r23 = *reinterpret_cast<const double2*>(r + row + 2 * i2 + 2);
+  double2 z01, z23;
+  if (par == 0) {
+    z01.x = (double)va;
+    z01.y = (double)black.x;
+    z23.x = (double)vb;
+    z23.y = (double)black.y;
+    acc += r01.x * z01.x;
+    acc += r01.y * z01.y;
+    acc += r23.x * z23.x;
+    acc += r23.y * z23.y;
+  } else {
+    z01.x = (double)black.x;
+    z01.y = (double)va;
+    z23.x = (double)black.y;
+    z23.y = (double)vb;
+    acc += r01.y * z01.y;
+    acc += r01.x * z01.x;
+    acc += r23.y * z23.y;
+    acc += r23.x * z23.x;
+  }
+  *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
+  *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+}
+
 // k_rb_gs_rz with two red cells per thread (entries i2 and i2 + 1 of a colour
 // row, tiles of 128 x 16 taken in turn by the fixed reduction grid; needs nx a
-// multiple of 4): the four natural cells 2 i2 .. 2 i2 + 3 (red and black alternating)
-// have r read and z written as two 16-byte pairs. Each red cell's arithmetic is
-// rb_update's; r . z adds, per thread, red a, black a, red b, black b.
+// multiple of 4).
 template <class T>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                   const double* __restrict__ r, const PcgState* st, double* part) {
-  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   if (!block_running(st)) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned gx = (hx + 127) >> 7, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
@@ -2336,40 +2376,31 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
     const int k = (int)(tile / (gx * gy));
     if (i2 >= (int)hx || j >= (int)ny) continue;
     const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
-    const long long row = (long long)j * g.sy + k * g.sz;
-    const int par = (j + k) & 1;  // red at 2 i2 + par and 2 i2 + 2 + par
-    const int qa[3] = {2 * i2 + par, j, k}, qb[3] = {2 * i2 + 2 + par, j, k};
-    const T va = rb_update<T>(g, F, 0, qa, u, (const T*)nullptr, (const int*)nullptr, T(0));
-    const T vb = rb_update<T>(g, F, 0, qb, u + 1, (const T*)nullptr, (const int*)nullptr, T(0));
-    T2 red;
-    red.x = va;
-    red.y = vb;
-    *reinterpret_cast<T2*>(F.x + u) = red;
-    const T2 black = *reinterpret_cast<const T2*>(F.x + F.half + u);
-    const double2 r01 = *reinterpret_cast<const double2*>(r + row + 2 * i2);
-    const double2 r23 = *reinterpret_cast<const double2*>(r + row + 2 * i2 + 2);
-    double2 z01, z23;
-    if (par == 0) {
-      z01.x = (double)va;
-      z01.y = (double)black.x;
-      z23.x = (double)vb;
-      z23.y = (double)black.y;
-      acc += r01.x * z01.x;
-      acc += r01.y * z01.y;
-      acc += r23.x * z23.x;
-      acc += r23.y * z23.y;
-    } else {
-      z01.x = (double)black.x;
-      z01.y = (double)va;
-      z23.x = (double)black.y;
-      z23.y = (double)vb;
-      acc += r01.y * z01.y;
-      acc += r01.x * z01.x;
-      acc += r23.y * z23.y;
-      acc += r23.x * z23.x;
-    }
-    *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
-    *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+    rz_pair<T>(g, F, out64, r, i2, j, k, u, acc);
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
+}
+
+// The same with consecutive threads on consecutive red pairs: items of RB pairs
+// in order, each XCD taking a contiguous share of them (no idle lanes where a
+// row is not a multiple of the tile width).
+template <class T>
+__global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
+                                                      const double* __restrict__ r, const PcgState* st, double* part) {
+  if (!block_running(st)) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned pairs = (unsigned)(F.half / 2);
+  double acc = 0.0;
+  const XcdShare sh = xcd_share((pairs + RB - 1) / RB, 1, 0);
+  for (unsigned item = sh.first; item < sh.end; item += sh.step) {
+    const unsigned pair = item * RB + threadIdx.x;
+    if (pair >= pairs) continue;
+    const unsigned u = 2 * pair;
+    const unsigned row = div_magic(u, F.mhx);
+    const int i2 = (int)(u - row * hx);
+    const unsigned k = div_magic(row, F.mny);
+    rz_pair<T>(g, F, out64, r, i2, (int)(row - k * ny), (int)k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -3307,7 +3338,9 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
       if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
         // the last half-sweep also forms the PCG's r . z partials
         probe_begin(PROBE_FINE_RZ, s);
-        if (rb->two_cells && g0.n[0] % 4 == 0)
+        if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
+          hipLaunchKernelGGL((k_rb_gs_rz2_lin<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
+        else if (rb->two_cells && g0.n[0] % 4 == 0)
           hipLaunchKernelGGL((k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         else
           hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
