@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 HEADLINE_DIRECTION = 200.0
 HEADLINE_TOL = 1e-8
 METRIC = "Poisson solves/s at rel-res 1e-8 + HBM GB/s %peak; p50/p99 latency served vs plain"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_rb_gs2_street_canyon_384x384x160.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_rb_gs2_lin_street_canyon_384x384x160.txt")
 # The reference's published single-thread CPU times (ms, reused solve; AMD EPYC 9V74):
 # docs/evidence/pressure-multigrid.md:34-38 (manufactured kernels, tol 1e-11, IC(0) / MG)
 # and docs/evidence/native-cpu-performance.md:87 (canyon-93k IC(0) solve, median warm).
@@ -594,9 +594,9 @@ def main():
     value = args.steps / h["elapsed"]
     probe = h["probe"]
     top, row = next(iter(probe.items()))
-    kernel_names = {"fine_half_sweep": "k_rb_gs2<double> (fine red-black half-sweep, two cells per thread)",
-                    "coarse_half_sweep": "k_agg_gs<double> (coarse half-sweep)", "operator": "k_pcg_op",
-                    "axpy": "k_pcg_axpy"}
+    kernel_names = {"fine_half_sweep": "k_rb_gs2_lin<double> (fine red-black half-sweep, a pair of cells per thread)",
+                    "coarse_half_sweep": "k_agg_gs<double> (coarse half-sweep)",
+                    "operator": "k_pcg_op_march (operator marching through z)", "axpy": "k_pcg_axpy"}
     prow = probe[top]
     achieved = prow["GBps"]
     solve_s = h["elapsed"] / args.steps

@@ -222,6 +222,7 @@ struct cfd_grid {
     int64_t max = 0;
     int rc = 0;
     bool graphs = true;  // false: iterations launched kernel by kernel (nothing instantiated)
+    int64_t learned = 0;  // iterations of the last converged solve with these settings (0: none)
     std::string error;
     cfd_linear_info info{};
   } job;
@@ -980,24 +981,33 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
 }
 
 // One PCG iteration (pressure.rs:386-434). p alternates between p0 and p1.
-void enqueue_iteration(cfd_grid* g, int parity) {
+// One PCG iteration: the preconditioner applied to the residual the previous
+// iteration left (`precondition`; the job's set-up applies it before the first
+// iteration), then the search direction, its operator product, the update of x
+// and r, and the convergence test. The preconditioner leads the iteration so
+// that the iteration that converges queues no cycle after its test.
+void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
   PcgState* st = g->st.p;
   double* pold = parity ? g->p1.p : g->p0.p;
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  if (precondition) {
+    enqueue_precondition(g, g->r.p, g->z.p, st);
+    enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
+    if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
+  }
   launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p, g->stream,
                 g->march);
   launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
   launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
-  enqueue_precondition(g, g->r.p, g->z.p, st);
-  enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
-  if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
 }
 
-// The captured graph of `len` iterations whose first one has parity `parity`.
-int segment_graph(cfd_grid* g, int len, int parity, hipGraphExec_t* out) {
-  const int key = (len << 2) | (parity << 1) | (g->opt.precision == CFD_PRECISION_MIXED ? 1 : 0);
+// The captured graph of `len` iterations whose first one has parity `parity`
+// (`first`: it is the job's first iteration, whose preconditioner the set-up
+// applied).
+int segment_graph(cfd_grid* g, int len, int parity, bool first, hipGraphExec_t* out) {
+  const int key = (len << 3) | (first ? 4 : 0) | (parity << 1) | (g->opt.precision == CFD_PRECISION_MIXED ? 1 : 0);
   auto it = g->seg.find(key);
   if (it != g->seg.end()) {
     *out = it->second;
@@ -1005,7 +1015,7 @@ int segment_graph(cfd_grid* g, int len, int parity, hipGraphExec_t* out) {
   }
   hipGraph_t graph;
   HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < len; ++i) enqueue_iteration(g, (parity + i) & 1);
+  for (int i = 0; i < len; ++i) enqueue_iteration(g, (parity + i) & 1, !first || i > 0);
   HIP_TRY(hipStreamEndCapture(g->stream, &graph));
   hipGraphExec_t exec = nullptr;
   hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1067,13 +1077,13 @@ int enqueue_chunk(cfd_grid* g, int len) {
   const int parity = (int)(J.enq & 1);
   if (g->use_graphs && J.graphs && g->opt.cycle != CFD_CYCLE_REFERENCE) {
     hipGraphExec_t seg = nullptr;
-    int rc = segment_graph(g, len, parity, &seg);
+    int rc = segment_graph(g, len, parity, J.enq == 0, &seg);
     if (rc) return rc;
     HIP_TRY(hipGraphLaunch(seg, g->stream));
   } else {
     for (int i = 0; i < len; ++i) {
       if (t_probe) t_probe->tag = J.enq + i + 1;
-      enqueue_iteration(g, (parity + i) & 1);
+      enqueue_iteration(g, (parity + i) & 1, J.enq + i > 0);
     }
   }
   J.enq += len;
@@ -1154,7 +1164,8 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   // the first iterations go into the queue at once: as many as the last such solve
   // took, else one segment; kernels queued past the end of a solve do nothing
   auto known = g->learned.find(learn_key(g, tol));
-  int rc = queue_ahead(g, known != g->learned.end() ? known->second : kSegment);
+  J.learned = known != g->learned.end() ? known->second : 0;
+  int rc = queue_ahead(g, J.learned > 0 ? J.learned : kSegment);
   if (rc) return rc;
   J.phase = 2;
   return 0;
@@ -1260,7 +1271,11 @@ int job_poll(cfd_grid* g, bool block, cfd_poll_fn poll, void* user) {
         return JOB_DONE;
       }
     }
-    int rc = queue_ahead(g, expected_iterations(s, J.tol, J.enq));
+    // no further than the count the last such solve converged in until a
+    // read-back shows that count reached
+    int64_t target = expected_iterations(s, J.tol, J.enq);
+    if (J.learned > s.it) target = std::min(target, J.learned);
+    int rc = queue_ahead(g, target);
     if (rc) return rc;
     return JOB_PENDING;
   }
@@ -1876,15 +1891,13 @@ int cfd_profile_solve(cfd_grid* g, const double* d_b, const double* d_scale, dou
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   // launches queued past the converging iteration end at once (the device state
-  // is no longer RUNNING): only the solve's own work is counted. The PCG vector
-  // kernels of iteration `it` run before its convergence test; its preconditioner
-  // does not.
+  // is no longer RUNNING): only the solve's own work is counted, the launches of
+  // the set-up (tag 0) and of iterations 1..it
   const long long it = local.iterations;
-  const bool before_test = kernel == PROBE_OPERATOR || kernel == PROBE_AXPY || kernel == PROBE_TRUE_RESIDUAL;
   std::memset(out, 0, sizeof *out);
   for (size_t k = 0; 2 * k + 1 < P.used; ++k) {
     const long long tag = P.tags[k];
-    if (before_test ? tag > it : tag >= it && it > 0) {
+    if (tag > it) {
       ++out->dropped;
       continue;
     }

@@ -2418,19 +2418,18 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
                     double* out64, const PcgState* st, hipStream_t s) {
   // the pass carrying the prolongation keeps one cell per thread (its neighbour
   // reads gather the coarse correction cell by cell)
-  probe_begin(PROBE_FINE_HALF, s);
-  if (F.two_cells == 2 && g0.n[0] % 4 == 0 && !xc)
-    hipLaunchKernelGGL((k_rb_gs2_lin<T>), dim3((unsigned)((F.half / 2 + BS - 1) / BS)), dim3(BS), 0, s, g0, F, colour,
-                       out64, st);
-  else if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
-    hipLaunchKernelGGL((k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
-                       F, colour, xc, fparent, cs, out64, st);
-  else
-    hipLaunchKernelGGL((k_rb_gs<T>), dim3(rb_blocks(g0, F)), dim3(BS), 0, s, g0, F, colour, xc, fparent, cs, out64,
-                       st);
   // per swept cell: its code, b and new x, and the other colour's x (each value
   // read once), and z when written
-  probe_end(PROBE_FINE_HALF, s, (double)F.half * (1 + 3 * sizeof(T) + (out64 ? 8 : 0)));
+  const double bytes = (double)F.half * (1 + 3 * sizeof(T) + (out64 ? 8 : 0));
+  if (F.two_cells == 2 && g0.n[0] % 4 == 0 && !xc)
+    PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs2_lin<T>), dim3((unsigned)((F.half / 2 + BS - 1) / BS)), dim3(BS), 0,
+                  s, g0, F, colour, out64, st);
+  else if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
+    PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]),
+                  dim3(BS), 0, s, g0, F, colour, xc, fparent, cs, out64, st);
+  else
+    PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs<T>), dim3(rb_blocks(g0, F)), dim3(BS), 0, s, g0, F, colour, xc,
+                  fparent, cs, out64, st);
 }
 
 // Fine residual of cell q from the colour-separated layout.
@@ -2927,29 +2926,28 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
                    int march) {
-  probe_begin(PROBE_OPERATOR, s);
   const MarchShape ms = march_shape(g);
+  const double bytes = 33.0 * (double)g.N;  // z, p_old, code in; p_new, Ap out
   // marching pays once the grid gives every block of the fixed grid an item
   if (march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP)) {
     if (closed)
-      hipLaunchKernelGGL(k_pcg_op_march<true>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew, ap, mean, cidx, st,
-                         part);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op_march<true>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew,
+                    ap, mean, cidx, st, part);
     else
-      hipLaunchKernelGGL(k_pcg_op_march<false>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew, ap, mean, cidx,
-                         st, part);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op_march<false>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold,
+                    pnew, ap, mean, cidx, st, part);
   } else if (closed) {
-    hipLaunchKernelGGL(k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean,
+                  cidx, st, part);
   } else {
-    hipLaunchKernelGGL(k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean, cidx, st, part);
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean,
+                  cidx, st, part);
   }
-  // z, p_old, code in; p_new, Ap out
-  probe_end(PROBE_OPERATOR, s, 33.0 * (double)g.N);
 }
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s) {
-  probe_begin(PROBE_AXPY, s);
-  hipLaunchKernelGGL(k_pcg_axpy, dim3(RP), dim3(RB), 0, s, N, p, ap, x, r, st, part);
-  probe_end(PROBE_AXPY, s, 48.0 * (double)N);  // x, p, r, Ap in; x, r out
+  // x, p, r, Ap in; x, r out
+  PROBED_LAUNCH(PROBE_AXPY, 48.0 * (double)N, k_pcg_axpy, dim3(RP), dim3(RB), 0, s, N, p, ap, x, r, st, part);
 }
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s) {
@@ -3230,9 +3228,8 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   const unsigned ab = blocks_for(g0.N);
   auto direct = [&](const auto* b, T* x, double* o64) {
     using TI = std::remove_cv_t<std::remove_pointer_t<decltype(b)>>;
-    probe_begin(PROBE_COARSEST, s);
-    hipLaunchKernelGGL((k_agg_direct<TI, T>), dim3(D.big_rows + (D.nsmall + 3) / 4), dim3(BS), 0, s, D, b, x, o64, st);
-    probe_end(PROBE_COARSEST, s, 0.0);
+    PROBED_LAUNCH(PROBE_COARSEST, 0.0, (k_agg_direct<TI, T>), dim3(D.big_rows + (D.nsmall + 3) / 4), dim3(BS), 0, s,
+                  D, b, x, o64, st);
   };
   if (nlev == 0) {
     // the fine level is the coarsest: solid cells stay zero
@@ -3252,9 +3249,8 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   auto relax = [&](const AggDevLevel<T>& L, int c) {
     const int rows = L.cptr[c + 1] - L.cptr[c];
     if (rows <= 0) return;
-    probe_begin(PROBE_COARSE_HALF, s);
-    hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, c, st);
-    probe_end(PROBE_COARSE_HALF, s, coarse_bytes(L, c));
+    PROBED_LAUNCH(PROBE_COARSE_HALF, coarse_bytes(L, c), (k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, c,
+                  st, (const T*)nullptr, T(0));
   };
   const Geo none{};
   const T* nox = nullptr;
@@ -3264,25 +3260,24 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
     // pairs of natural cells when every row starts a pair and r allows 16-byte loads
-    probe_begin(PROBE_FINE_FIRST, s);
-    if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
-      hipLaunchKernelGGL((k_rb_first2<T>), dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
-                         g0, codes, (const double*)b0, *rb, st);
-    else
-      hipLaunchKernelGGL((k_rb_first<T>), dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s,
-                         g0, codes, (const double*)b0, *rb, st);
     // r (FP64) and codes in; b of both colours and the red x out
-    probe_end(PROBE_FINE_FIRST, s, (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T));
+    const double first_bytes = (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
+    if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
+      PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first2<T>),
+                    dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0, codes,
+                    (const double*)b0, *rb, st);
+    else
+      PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first<T>),
+                    dim3(((g0.n[0] + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0, codes,
+                    (const double*)b0, *rb, st);
     for (int pass = 1; pass < 2 * fr; ++pass)
       launch_rb_half<T>(g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr, st, s);
-    probe_begin(PROBE_FINE_RESTRICT, s);
-    hipLaunchKernelGGL((k_rb_restrict<T>),
-                       dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)), dim3(BS), 0, s,
-                       g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
     // red cells: code, b, x; black x; one parent per 2x2x2 block; the first
     // coarse level's b and x written and its diagonal read
-    probe_end(PROBE_FINE_RESTRICT, s,
-              (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N / 8 * 4 + (double)lv[0].n * 3 * sizeof(T));
+    PROBED_LAUNCH(PROBE_FINE_RESTRICT,
+                  (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N / 8 * 4 + (double)lv[0].n * 3 * sizeof(T),
+                  (k_rb_restrict<T>), dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)),
+                  dim3(BS), 0, s, g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
@@ -3306,10 +3301,8 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // pre-smoothing ended with the last colour class, whose residual is zero to
     // rounding (without rounds, or with zero_last off, nothing is taken as zero)
     const int zero_from = cr >= 1 && zero_last ? L.cptr[L.ncol - 1] : L.n;
-    probe_begin(PROBE_COARSE_RESTRICT, s);
-    hipLaunchKernelGGL((k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64), dim3(kRestrictThreads), 0, s, L, lv[l + 1], st,
-                       zero_from);
-    probe_end(PROBE_COARSE_RESTRICT, s, 0.0);
+    PROBED_LAUNCH(PROBE_COARSE_RESTRICT, 0.0, (k_agg_restrict<T>), dim3((lv[l + 1].n + 63) / 64),
+                  dim3(kRestrictThreads), 0, s, L, lv[l + 1], st, zero_from);
   }
   direct((const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr);
   for (int l = stop - 1; l >= 0; --l) {
@@ -3318,11 +3311,10 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     if (L.ncol == 2) {
       // prolongation fused into the first half-sweep
       const int rows = L.cptr[2] - L.cptr[1];
-      if (rows > 0) {
-        probe_begin(PROBE_COARSE_HALF, s);
-        hipLaunchKernelGGL((k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, 1, st, xc, cs);
-        probe_end(PROBE_COARSE_HALF, s, coarse_bytes(L, 1));
-      } else
+      if (rows > 0)
+        PROBED_LAUNCH(PROBE_COARSE_HALF, coarse_bytes(L, 1), (k_agg_gs<T>), dim3(blocks_for(rows)), dim3(BS), 0, s, L, 1,
+                      st, xc, cs);
+      else
         hipLaunchKernelGGL((k_agg_prolong<T>), dim3(blocks_for(L.n)), dim3(BS), 0, s, L, xc, cs, st);
       for (int pass = 1; pass < 2 * cr; ++pass) relax(L, (pass + 1) & 1);
     } else {
@@ -3337,25 +3329,26 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     for (int pass = 0; pass < 2 * fr; ++pass) {
       if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
         // the last half-sweep also forms the PCG's r . z partials
-        probe_begin(PROBE_FINE_RZ, s);
-        if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
-          hipLaunchKernelGGL((k_rb_gs_rz2_lin<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
-        else if (rb->two_cells && g0.n[0] % 4 == 0)
-          hipLaunchKernelGGL((k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
-        else
-          hipLaunchKernelGGL((k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st, rz_part);
         // the red half-sweep, both colours' z out in FP64 and r in
-        probe_end(PROBE_FINE_RZ, s, (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N * 16);
+        const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N * 16;
+        if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r,
+                        st, rz_part);
+        else if (rb->two_cells && g0.n[0] % 4 == 0)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st,
+                        rz_part);
+        else
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st,
+                        rz_part);
         continue;
       }
       const T* xcp = pass == 0 ? (const T*)lv[0].x : nox;
       if (xcp && rb->rpar) {
         // the correction added to the red cells first, then a plain half-sweep
-        probe_begin(PROBE_FINE_PROLONG, s);
-        hipLaunchKernelGGL((k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0, s, rb->half, rb->x, rb->rpar,
-                           xcp, cs, st);
         // red x in and out, its parent; the first coarse level's values
-        probe_end(PROBE_FINE_PROLONG, s, (double)rb->half * (2 * sizeof(T) + 4) + (double)lv[0].n * sizeof(T));
+        PROBED_LAUNCH(PROBE_FINE_PROLONG, (double)rb->half * (2 * sizeof(T) + 4) + (double)lv[0].n * sizeof(T),
+                      (k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0, s, rb->half, rb->x, rb->rpar, xcp, cs,
+                      st);
         xcp = nox;
       }
       // z in the natural layout from the last two half-sweeps (the r . z kernel

@@ -1,5 +1,6 @@
 // Host-side launchers of the kernels in kernels.hip.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <functional>
@@ -42,6 +43,30 @@ inline void probe_begin(int kernel, hipStream_t s) {
   if (p && p->kernel == kernel && (p->used & 1) == 0 && p->used + 2 <= p->events.size())
     (void)hipEventRecord(p->events[p->used++], s);
 }
+// The probe when it times `kernel` and has a pair of events left (a launch of
+// one kernel then records them as its own start and end: hipExtLaunchKernelGGL
+// takes them into the dispatch, so they time the kernel's execution alone).
+inline LaunchProbe* probe_slot(int kernel) {
+  LaunchProbe* p = t_probe;
+  return p && p->kernel == kernel && (p->used & 1) == 0 && p->used + 2 <= p->events.size() ? p : nullptr;
+}
+inline void probe_done(LaunchProbe* p, double bytes) {
+  p->used += 2;
+  p->bytes.push_back(bytes);
+  p->tags.push_back(p->tag);
+}
+#define PROBED_LAUNCH(ID, BYTES, K, G, B, SHM, S, ...)                                                           \
+  do {                                                                                                           \
+    if (LaunchProbe* probe_ = probe_slot(ID)) {                                                                  \
+      hipExtLaunchKernelGGL(K, G, B, SHM, S, probe_->events[probe_->used], probe_->events[probe_->used + 1], 0, \
+                            __VA_ARGS__);                                                                        \
+      probe_done(probe_, BYTES);                                                                                 \
+    } else {                                                                                                     \
+      hipLaunchKernelGGL(K, G, B, SHM, S, __VA_ARGS__);                                                          \
+    }                                                                                                            \
+  } while (0)
+// A pair of events around several launches of a class (a V-cycle's IC(0)
+// tiles, the residual kernel and its decision kernel).
 inline void probe_end(int kernel, hipStream_t s, double bytes) {
   LaunchProbe* p = t_probe;
   if (p && p->kernel == kernel && (p->used & 1) == 1) {
