@@ -243,6 +243,9 @@ struct cfd_grid {
   bool zero_last = true;
   // stencil kernels marching through z on large grids (CFD_MARCH: 0 never, 1 by size, 2 always)
   int march = 1;
+  // the red-black cycle in FP64 keeps the PCG's r and z in the fine level's
+  // colour-separated layout where it can (CFD_RB_RESIDENT=0: never)
+  bool rb_resident = true;
   ~cfd_grid() {
     for (auto& kv : seg)
       if (kv.second) (void)hipGraphExecDestroy(kv.second);
@@ -854,6 +857,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   if (const char* env = std::getenv("CFD_MG_ZERO_LAST")) g->zero_last = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MARCH")) g->march = std::atoi(env);
+  if (const char* env = std::getenv("CFD_RB_RESIDENT")) g->rb_resident = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_SWEEPS")) {
     int f = 2, c = 2;
     if (std::sscanf(env, "%d,%d", &f, &c) == 2 && f >= 1 && f <= 8 && c >= 1 && c <= 8) {
@@ -928,7 +932,18 @@ bool rz_fused(const cfd_grid* g) {
          g->fine_rounds >= 1;
 }
 
-void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st) {
+// The solve keeps r in the fine level's right-hand side and z in its x (both in
+// the colour-separated layout, FP64) instead of natural-layout vectors: the
+// fused r . z half-sweep, two cells per thread (nx a multiple of 4) and the
+// divisions of the colour-separated PCG update exact (N nx < 2^40).
+bool resident(const cfd_grid* g) {
+  return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_FP64 && g->ash.rb64.two_cells >= 1 &&
+         g->geo.n[0] % 4 == 0 && g->geo.N < (1LL << 31) && (double)g->geo.N * g->geo.n[0] < 1099511627776.0;
+}
+
+// z = M r. resident (see resident()): 0 r and z natural; 1 r natural, copied
+// into the fine level's right-hand side, z left in its x; 2 r already there.
+void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0) {
   if (g->host.preconditioner == CFD_IC0) {
     // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
     if (g->opt.precision == CFD_PRECISION_MIXED)
@@ -955,9 +970,9 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
                                        g->part.p, g->zero_last);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
-                                        A.D, A.has_rb ? z : nullptr, st, g->omega, g->stream,
-                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
-                                        g->part.p, g->zero_last);
+                                        A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
+                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds,
+                                        resident ? nullptr : rz_r, g->part.p, g->zero_last, resident);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -980,26 +995,34 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st)
   }
 }
 
-// One PCG iteration (pressure.rs:386-434). p alternates between p0 and p1.
-// One PCG iteration: the preconditioner applied to the residual the previous
-// iteration left (`precondition`; the job's set-up applies it before the first
-// iteration), then the search direction, its operator product, the update of x
-// and r, and the convergence test. The preconditioner leads the iteration so
-// that the iteration that converges queues no cycle after its test.
+// One PCG iteration (pressure.rs:386-434): the preconditioner applied to the
+// residual the previous iteration left (`precondition`; the job's set-up applies
+// it before the first iteration), then the search direction, its operator
+// product, the update of x and r, and the convergence test. The preconditioner
+// leads the iteration so that the iteration that converges queues no cycle
+// after its test. p alternates between p0 and p1.
 void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
   PcgState* st = g->st.p;
   double* pold = parity ? g->p1.p : g->p0.p;
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
+  const bool res = resident(g);
   if (precondition) {
-    enqueue_precondition(g, g->r.p, g->z.p, st);
+    enqueue_precondition(g, g->r.p, g->z.p, st, res ? 2 : 0);
     enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
     if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
   }
-  launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p, g->stream,
-                g->march);
-  launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+  if (res) {
+    const FineRB<double>& F = g->ash.rb64;
+    launch_pcg_op(g->geo, g->codes.p, F.x, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream, g->march,
+                  F.half);
+    launch_pcg_axpy_rb(g->geo, pnew, g->ap.p, g->x.p, F.b, st, g->part.p, g->stream);
+  } else {
+    launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,
+                  g->stream, g->march);
+    launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
+  }
   launch_pcg_true(g->geo, g->codes.p, g->x.p, g->bc.p, st, g->part.p, g->stream);
 }
 
@@ -1144,7 +1167,7 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
   if (t_probe) t_probe->tag = 0;
-  enqueue_precondition(g, g->r.p, g->z.p, st);
+  enqueue_precondition(g, g->r.p, g->z.p, st, resident(g) ? 1 : 0);
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   if (!rz_fused(g))
     launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);

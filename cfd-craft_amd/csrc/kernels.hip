@@ -177,6 +177,13 @@ __global__ void __launch_bounds__(BS) k_apply(Geo g, const uint8_t* __restrict__
 // takes the x-th contiguous eighth of the items and its blocks take them in turn
 // (first, first + step, ...). Neighbouring rows and layers of a stencil are then
 // read by the same XCD, from its own L2. The grid must be a multiple of 8 blocks.
+// floor(n / d) from m = ceil(2^40 / d), exact for n d < 2^40 (n < 2^32): the
+// error n (m - 2^40 / d) / 2^40 < n / 2^40 stays below 1 / d, less than the
+// distance from n / d up to the next integer.
+__device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m) {
+  return (unsigned)(((unsigned long long)n * m) >> 40);
+}
+
 struct XcdShare {
   unsigned begin, end, first, step;
 };
@@ -215,14 +222,19 @@ __device__ __forceinline__ void for_cells(const Geo& g, F f) {
 // Ap = A p_new, and the thread's part of p_new . Ap. FIRST is a template
 // argument so that the seven p values of a cell are loaded without a branch
 // between the loads (they are then all in flight at once).
-template <bool HAS_CLOSED, bool FIRST>
+// RES: z in the colour-separated layout of the fine red-black level (cells with
+// x + y + z even first, then the others, each in natural order: cell c at c / 2
+// of its colour's half, nx even), zhalf entries per colour. Every neighbour of
+// a cell has the other colour (periodic axes have even lengths there), except
+// the cell itself standing in at a closed boundary.
+template <bool HAS_CLOSED, bool FIRST, bool RES = false>
 __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __restrict__ codes,
                                                const double* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
-                                               double beta) {
-  auto pv = [&](long long i) {
-    double zc = z[i];
+                                               double beta, long long zhalf = 0) {
+  auto pv = [&](long long i, int colour) {
+    double zc = RES ? z[(colour ? zhalf : 0) + (i >> 1)] : z[i];
     // closed components: z centered (without any, the code load is skipped)
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
     if (FIRST) return zc;
@@ -230,9 +242,11 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
   };
   double acc = 0.0;
   for_cells(g, [&](long long c, const int q[3]) {
+    const int cc = (q[0] + q[1] + q[2]) & 1;
+    auto pvc = [&](long long i) { return pv(i, i == c ? cc : cc ^ 1); };
     uint8_t code = codes[c];
-    double pc = pv(c);
-    double y = apply_cell(g, code, c, q, pv);
+    double pc = pvc(c);
+    double y = apply_cell(g, code, c, q, pvc);
     pnew[c] = pc;
     ap[c] = y;
     acc += pc * y;
@@ -243,17 +257,18 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
 // (registers sized for 8 waves per SIMD: two blocks per CU)
-template <bool HAS_CLOSED>
+template <bool HAS_CLOSED, bool RES>
 __global__ void __launch_bounds__(RB, 8)
     k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ z, const double* __restrict__ pold,
              double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
-             const int* __restrict__ cidx, PcgState* st, double* part) {
+             const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf) {
   if (!block_running(st)) return;
   const bool first = st->first;
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
-  const double acc = first ? pcg_op_cells<HAS_CLOSED, true>(g, codes, z, pold, pnew, ap, mean, cidx, beta)
-                           : pcg_op_cells<HAS_CLOSED, false>(g, codes, z, pold, pnew, ap, mean, cidx, beta);
+  const double acc =
+      first ? pcg_op_cells<HAS_CLOSED, true, RES>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf)
+            : pcg_op_cells<HAS_CLOSED, false, RES>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -278,14 +293,15 @@ inline MarchShape march_shape(const Geo& g) {
   return MarchShape{(g.n[0] + MX - 1) / MX, (g.n[1] + MY - 1) / MY, (g.n[2] + MZ - 1) / MZ};
 }
 
-template <bool HAS_CLOSED, bool FIRST>
+template <bool HAS_CLOSED, bool FIRST, bool RES>
 __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
                                                const double* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
-                                               double beta, double (*tile)[MY + 2][MX + 2]) {
-  auto pv = [&](long long i) {
-    double zc = z[i];
+                                               double beta, double (*tile)[MY + 2][MX + 2], long long zhalf) {
+  // colour: that of cell i (used with RES, z in the colour-separated layout)
+  auto pv = [&](long long i, int colour) {
+    double zc = RES ? z[(colour ? zhalf : 0) + (i >> 1)] : z[i];
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
     if (FIRST) return zc;
     return zc + beta * pold[i];
@@ -310,7 +326,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
     // and right columns.
     const int r = (int)threadIdx.x;
     const bool rim = r < 2 * (wx + wy);
-    int sx = 0, sy = 0;
+    int sx = 0, sy = 0, rpar = 0;
     long long rcol = 0;
     if (rim) {
       int ex, ey, axis, side;
@@ -329,16 +345,24 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
                                : (t0 < n - 1 ? t0 + 1 : (g.per[axis] ? 0 : t0));
       const int nxy[2] = {axis == 0 ? tn : eq[0], axis == 1 ? tn : eq[1]};
       rcol = (long long)nxy[0] + (long long)nxy[1] * g.sy;
+      rpar = (nxy[0] + nxy[1]) & 1;
     }
+    const int cpar = (i + j) & 1;
     // software pipeline: the values of the next PD layers (the column's p, its
     // code and the rim slot's p) are in flight while the current layer is
     // worked on. p of layer L = k0 - 1 .. k1 is the column's z neighbour at the
     // ends (nb_or_self), the column's own value in between.
     constexpr int PD = 3;
     auto p_at = [&](int L) {
-      if (L < k0) return pv(nb_or_self(g, col + (long long)k0 * g.sz, 2, k0, 0));
-      if (L >= k1) return pv(nb_or_self(g, col + (long long)(k1 - 1) * g.sz, 2, k1 - 1, 1));
-      return pv(col + (long long)L * g.sz);
+      if (L < k0) {
+        const long long c0 = col + (long long)k0 * g.sz, n = nb_or_self(g, c0, 2, k0, 0);
+        return pv(n, (cpar + k0 + (n == c0 ? 0 : 1)) & 1);
+      }
+      if (L >= k1) {
+        const long long c1 = col + (long long)(k1 - 1) * g.sz, n = nb_or_self(g, c1, 2, k1 - 1, 1);
+        return pv(n, (cpar + k1 - 1 + (n == c1 ? 0 : 1)) & 1);
+      }
+      return pv(col + (long long)L * g.sz, (cpar + L) & 1);
     };
     double pm = 0.0, pc = 0.0, rv = 0.0;
     double pq[PD] = {}, rq[PD] = {};
@@ -354,10 +378,10 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
       }
     }
     if (rim) {
-      rv = pv(rcol + (long long)k0 * g.sz);
+      rv = pv(rcol + (long long)k0 * g.sz, (rpar + k0) & 1);
 #pragma unroll
       for (int d = 0; d < PD; ++d)
-        if (k0 + 1 + d < k1) rq[d] = pv(rcol + (long long)(k0 + 1 + d) * g.sz);
+        if (k0 + 1 + d < k1) rq[d] = pv(rcol + (long long)(k0 + 1 + d) * g.sz, (rpar + k0 + 1 + d) & 1);
     }
     for (int k = k0; k < k1; ++k) {
       const long long c = col + (long long)k * g.sz;
@@ -368,7 +392,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
       uint8_t cin = 0;
       if (live && L <= k1) pin = p_at(L);
       if (live && L < k1) cin = codes[col + (long long)L * g.sz];
-      if (rim && L < k1) rin = pv(rcol + (long long)L * g.sz);
+      if (rim && L < k1) rin = pv(rcol + (long long)L * g.sz, (rpar + L) & 1);
       const double pn = pq[0];
       double(*t)[MX + 2] = tile[k & 1];
       if (live) t[ly + 1][lx + 1] = pc;
@@ -406,18 +430,20 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   return acc;
 }
 
-template <bool HAS_CLOSED>
+template <bool HAS_CLOSED, bool RES>
 __global__ void __launch_bounds__(RB, 8)
     k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const double* __restrict__ z,
                    const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
-                   const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part) {
+                   const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part,
+                   long long zhalf) {
   __shared__ double tile[2][MY + 2][MX + 2];
   if (!block_running(st)) return;
   const bool first = st->first;
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
-  const double acc = first ? pcg_op_march<HAS_CLOSED, true>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile)
-                           : pcg_op_march<HAS_CLOSED, false>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile);
+  const double acc =
+      first ? pcg_op_march<HAS_CLOSED, true, RES>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf)
+            : pcg_op_march<HAS_CLOSED, false, RES>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -446,6 +472,50 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
     x[i] += alpha * p[i];
     double ri = r[i] - alpha * ap[i];
     r[i] = ri;
+    acc += ri * ri;
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) {
+    part[P_RR + blockIdx.x] = s;
+    if (blockIdx.x == 0) {
+      st->rz = rz;
+      st->alpha = alpha;
+      st->denom = denom;
+      st->first = 0;
+    }
+  }
+}
+
+// The same with r in the colour-separated layout of the fine red-black level
+// (cell c at c / 2 of the half of its colour, (x + y + z) & 1; nx even): the
+// same cells per thread in the same order, so x, r and the partials of |r|^2
+// are those of k_pcg_axpy bit for bit. mnx, mny: ceil(2^40 / nx), ceil(2^40 / ny)
+// (N nx < 2^40).
+__global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny, unsigned long long mnx,
+                                                    unsigned long long mny, const double* __restrict__ p,
+                                                    const double* __restrict__ ap, double* __restrict__ x,
+                                                    double* __restrict__ r, PcgState* st, double* part) {
+  if (!block_running(st)) return;
+  const double rz = reduce_part(part + P_RZ, RP);
+  const double denom = reduce_part(part + P_PAP, RP);
+  if (denom <= 0.0 || !isfinite(denom)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->denom = denom;
+      st->status = ST_E032;
+    }
+    return;
+  }
+  const double alpha = rz / denom;
+  const long long half = N >> 1;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
+    const unsigned c = (unsigned)i;
+    const unsigned row = div_magic(c, mnx), k = div_magic(row, mny);
+    const unsigned colour = (c - row * (unsigned)nx + row - k * (unsigned)ny + k) & 1u;
+    double* rc = r + (colour ? half : 0) + (c >> 1);
+    x[i] += alpha * p[i];
+    double ri = *rc - alpha * ap[i];
+    *rc = ri;
     acc += ri * ri;
   }
   double s = block_sum(acc);
@@ -2044,6 +2114,34 @@ __global__ void __launch_bounds__(BS) k_rb_first2(Geo g, const uint8_t* __restri
   F.x[u] = v;
 }
 
+// The first red half-sweep (from x = 0) when the right-hand side is already in
+// the colour-separated layout (F.b, where the PCG keeps its residual): x_red =
+// b / diagonal, the arithmetic of k_rb_first2; consecutive threads on
+// consecutive pairs of red entries (nx a multiple of 4).
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first_lin(Geo g, FineRB<T> F, const PcgState* st) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned pair = (unsigned)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned u = 2 * pair;
+  if (u >= (unsigned)F.half) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned row = div_magic(u, F.mhx);
+  const int i2 = (int)(u - row * hx);
+  const unsigned k = div_magic(row, F.mny);
+  const int j = (int)(row - k * ny);
+  const int par = (j + (int)k) & 1;
+  const T2 bv = *reinterpret_cast<const T2*>(F.b + u);
+  const uint8_t ca = F.codes[u], cb = F.codes[u + 1];
+  const int qa[3] = {2 * i2 + par, j, (int)k}, qb[3] = {2 * i2 + 2 + par, j, (int)k};
+  const T da = fine_diag<T>(g, ca, qa), db = fine_diag<T>(g, cb, qb);
+  if (status != ST_RUNNING) return;
+  T2 out;
+  out.x = (ca & FLUID) && da > T(0) ? bv.x / da : T(0);
+  out.y = (cb & FLUID) && db > T(0) ? bv.y / db : T(0);
+  *reinterpret_cast<T2*>(F.x + u) = out;
+}
+
 // The Gauss-Seidel value of a cell from its code, right-hand side and the six
 // neighbour values xn (axis-major, lower side first; entries of uncoupled or
 // boundary faces are not read).
@@ -2218,11 +2316,6 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
   }
 }
 
-// floor(n / d) for n < 2^31 and d < 2^20 from m = ceil(2^40 / d) (exact there:
-// the error n (m - 2^40 / d) / 2^40 stays below 1 / d).
-__device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m) {
-  return (unsigned)(((unsigned long long)n * m) >> 40);
-}
 
 // The same half-sweep with consecutive threads on consecutive pairs of the
 // colour's array (no idle lanes where a row is not a multiple of the tile
@@ -2318,8 +2411,10 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz(Geo g, FineRB<T> F, double* 
 // The red pair at entries i2, i2 + 1 of row (j, k) (offset u) relaxed, and the
 // four natural cells 2 i2 .. 2 i2 + 3 (red and black alternating) with z written
 // as two 16-byte pairs and r . z added to acc in the order red a, black a, red
-// b, black b.
-template <class T>
+// b, black b. RES: r is the fine level's right-hand side F.b (both colours; the
+// PCG keeps its residual there) and z stays in F.x (nothing written in the
+// natural layout).
+template <class T, bool RES>
 __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
                                         const double* __restrict__ r, int i2, int j, int k, unsigned u, double& acc) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
@@ -2332,8 +2427,18 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
   red.y = vb;
   *reinterpret_cast<T2*>(F.x + u) = red;
   const T2 black = *reinterpret_cast<const T2*>(F.x + F.half + u);
-  const double2 r01 = *reinterpret_cast<const double2*>(r + row + 2 * i2);
-  const double2 r23 = *reinterpret_cast<const double2*>(r + row + 2 * i2 + 2);
+  double2 r01, r23;
+  if (RES) {
+    const T2 rr = *reinterpret_cast<const T2*>(F.b + u);
+    const T2 rk = *reinterpret_cast<const T2*>(F.b + F.half + u);
+    r01.x = (double)(par ? rk.x : rr.x);
+    r01.y = (double)(par ? rr.x : rk.x);
+    r23.x = (double)(par ? rk.y : rr.y);
+    r23.y = (double)(par ? rr.y : rk.y);
+  } else {
+    r01 = *reinterpret_cast<const double2*>(r + row + 2 * i2);
+    r23 = *reinterpret_cast<const double2*>(r + row + 2 * i2 + 2);
+  }
   double2 z01, z23;
   if (par == 0) {
     z01.x = (double)va;
@@ -2354,14 +2459,16 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
     acc += r23.y * z23.y;
     acc += r23.x * z23.x;
   }
-  *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
-  *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+  if (!RES) {
+    *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
+    *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+  }
 }
 
 // k_rb_gs_rz with two red cells per thread (entries i2 and i2 + 1 of a colour
 // row, tiles of 128 x 16 taken in turn by the fixed reduction grid; needs nx a
 // multiple of 4).
-template <class T>
+template <class T, bool RES>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                   const double* __restrict__ r, const PcgState* st, double* part) {
   if (!block_running(st)) return;
@@ -2376,7 +2483,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
     const int k = (int)(tile / (gx * gy));
     if (i2 >= (int)hx || j >= (int)ny) continue;
     const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
-    rz_pair<T>(g, F, out64, r, i2, j, k, u, acc);
+    rz_pair<T, RES>(g, F, out64, r, i2, j, k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -2385,7 +2492,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
 // The same with consecutive threads on consecutive red pairs: items of RB pairs
 // in order, each XCD taking a contiguous share of them (no idle lanes where a
 // row is not a multiple of the tile width).
-template <class T>
+template <class T, bool RES>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                       const double* __restrict__ r, const PcgState* st, double* part) {
   if (!block_running(st)) return;
@@ -2400,7 +2507,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, dou
     const unsigned row = div_magic(u, F.mhx);
     const int i2 = (int)(u - row * hx);
     const unsigned k = div_magic(row, F.mny);
-    rz_pair<T>(g, F, out64, r, i2, (int)(row - k * ny), (int)k, u, acc);
+    rz_pair<T, RES>(g, F, out64, r, i2, (int)(row - k * ny), (int)k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -2925,29 +3032,43 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 }
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
-                   int march) {
+                   int march, long long zhalf) {
   const MarchShape ms = march_shape(g);
   const double bytes = 33.0 * (double)g.N;  // z, p_old, code in; p_new, Ap out
   // marching pays once the grid gives every block of the fixed grid an item
   if (march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP)) {
     if (closed)
-      PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op_march<true>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold, pnew,
-                    ap, mean, cidx, st, part);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op_march<true, false>), dim3(RP), dim3(RB), 0, s, g, ms, codes, z,
+                    pold, pnew, ap, mean, cidx, st, part, 0LL);
+    else if (zhalf)
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op_march<false, true>), dim3(RP), dim3(RB), 0, s, g, ms, codes, z,
+                    pold, pnew, ap, mean, cidx, st, part, zhalf);
     else
-      PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op_march<false>, dim3(RP), dim3(RB), 0, s, g, ms, codes, z, pold,
-                    pnew, ap, mean, cidx, st, part);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op_march<false, false>), dim3(RP), dim3(RB), 0, s, g, ms, codes, z,
+                    pold, pnew, ap, mean, cidx, st, part, 0LL);
   } else if (closed) {
-    PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op<true>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean,
-                  cidx, st, part);
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op<true, false>), dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap,
+                  mean, cidx, st, part, 0LL);
+  } else if (zhalf) {
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op<false, true>), dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap,
+                  mean, cidx, st, part, zhalf);
   } else {
-    PROBED_LAUNCH(PROBE_OPERATOR, bytes, k_pcg_op<false>, dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew, ap, mean,
-                  cidx, st, part);
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op<false, false>), dim3(RP), dim3(RB), 0, s, g, codes, z, pold, pnew,
+                  ap, mean, cidx, st, part, 0LL);
   }
 }
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s) {
   // x, p, r, Ap in; x, r out
   PROBED_LAUNCH(PROBE_AXPY, 48.0 * (double)N, k_pcg_axpy, dim3(RP), dim3(RB), 0, s, N, p, ap, x, r, st, part);
+}
+void launch_pcg_axpy_rb(const Geo& g, const double* p, const double* ap, double* x, double* r, PcgState* st,
+                        double* part, hipStream_t s) {
+  const unsigned long long two40 = 1ULL << 40;
+  const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
+  const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
+  PROBED_LAUNCH(PROBE_AXPY, 48.0 * (double)g.N, k_pcg_axpy_rb, dim3(RP), dim3(RB), 0, s, g.N, g.n[0], g.n[1], mnx, mny,
+                p, ap, x, r, st, part);
 }
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s) {
@@ -3215,12 +3336,15 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 // coupling codes (red-black, or 2^(1+seams) colour classes across odd periodic
 // seams), lv[0..nlev) the coarse levels (lv[nlev-1] solved by the dense
 // blocks), levels from tail_first on inside one workgroup (dlv: device copy of
-// lv). The correction of each level is scaled by omega.
+// lv). The correction of each level is scaled by omega. resident (colour-
+// separated fine level with two cells per thread): 1 b0 copied into the fine
+// level's b and z left in its x, the last half-sweep forming r . z from them;
+// 2 the same with b already there (b0 unused).
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds,
-                       const double* rz_r, double* rz_part, bool zero_last) {
+                       const double* rz_r, double* rz_part, bool zero_last, int resident) {
   // a round is one sweep over every colour class; pre-smoothing runs the classes in
   // ascending order, post-smoothing in descending order (the adjoint)
   const int fr = fine_rounds, cr = coarse_rounds;
@@ -3260,9 +3384,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   if (rb) {
     // colour-separated fine level: scatter + red, black, red, black; restriction
     // pairs of natural cells when every row starts a pair and r allows 16-byte loads
-    // r (FP64) and codes in; b of both colours and the red x out
-    const double first_bytes = (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
-    if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
+    // r (FP64) and codes in; b of both colours and the red x out (from b kept
+    // in the layout: red codes and b in, red x out)
+    const double first_bytes = resident == 2 ? (double)rb->half * (1 + 2 * sizeof(T))
+                                             : (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
+    if (resident == 2)
+      PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first_lin<T>), dim3((unsigned)((rb->half / 2 + BS - 1) / BS)),
+                    dim3(BS), 0, s, g0, *rb, st);
+    else if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
       PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first2<T>),
                     dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0, codes,
                     (const double*)b0, *rb, st);
@@ -3327,16 +3456,24 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // layout) the final half-sweeps also write z in FP64
   if (rb) {
     for (int pass = 0; pass < 2 * fr; ++pass) {
-      if (pass == 2 * fr - 1 && pass > 0 && rz_r) {
-        // the last half-sweep also forms the PCG's r . z partials
-        // the red half-sweep, both colours' z out in FP64 and r in
-        const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N * 16;
-        if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
-          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r,
-                        st, rz_part);
+      if (pass == 2 * fr - 1 && pass > 0 && (rz_r || resident)) {
+        // the last half-sweep also forms the PCG's r . z partials: the red
+        // half-sweep, both colours' z out in FP64 and r in (with r and z kept in
+        // the layout: the black b read besides)
+        const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) +
+                                (resident ? (double)rb->half * sizeof(T) : (double)g0.N * 16);
+        if (resident && rb->two_cells == 2)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, true>), dim3(RP), dim3(RB), 0, s, g0, *rb,
+                        (double*)nullptr, (const double*)nullptr, st, rz_part);
+        else if (resident)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, true>), dim3(RP), dim3(RB), 0, s, g0, *rb,
+                        (double*)nullptr, (const double*)nullptr, st, rz_part);
+        else if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64,
+                        rz_r, st, rz_part);
         else if (rb->two_cells && g0.n[0] % 4 == 0)
-          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st,
-                        rz_part);
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64,
+                        rz_r, st, rz_part);
         else
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st,
                         rz_part);
@@ -3372,11 +3509,11 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
                                                 const PcgState*, double, hipStream_t, const FineRB<double>*, int, int,
-                                                const double*, double*, bool);
+                                                const double*, double*, bool, int);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
                                                double, hipStream_t, const FineRB<float>*, int, int, const double*,
-                                               double*, bool);
+                                               double*, bool, int);
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {

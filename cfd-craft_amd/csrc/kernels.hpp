@@ -79,11 +79,16 @@ inline void probe_end(int kernel, hipStream_t s, double bytes) {
 void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y, hipStream_t s);
 // march: 0 cell by cell; 1 marching through z once the grid has an item per
 // block of the fixed grid; 2 always marching
+// zhalf > 0: z in the fine red-black level's colour-separated layout (cell c at
+// c / 2 of its colour's half of zhalf entries; nx even, no closed component)
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
-                   int march = 1);
+                   int march = 1, long long zhalf = 0);
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s);
+// the same with r in the colour-separated layout (nx even, N nx < 2^40)
+void launch_pcg_axpy_rb(const Geo& g, const double* p, const double* ap, double* x, double* r, PcgState* st,
+                        double* part, hipStream_t s);
 // the recomputed-residual kernel and the one-block decision kernel of an iteration
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s);
@@ -127,7 +132,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb = nullptr, int fine_rounds = 2,
                        int coarse_rounds = 2, const double* rz_r = nullptr, double* rz_part = nullptr,
-                       bool zero_last = true);
+                       bool zero_last = true, int resident = 0);
 // One colour-separated fine half-sweep (timing hook).
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);

@@ -17,7 +17,7 @@ import pytest
 import oracle
 import cfdgpu
 from cfdgpu import workloads as W
-from cfdgpu.case import IC0, MULTIGRID, OPEN, PERIODIC, SLIP, WALL, Block, Case
+from cfdgpu.case import GROUND_OPEN_TOP, IC0, MULTIGRID, OPEN, PERIODIC, SLIP, WALL, Block, Case
 
 pytestmark = pytest.mark.gpu
 
@@ -673,6 +673,75 @@ def test_red_black_cycle_on_stretched_cells(gpu_available, cells, size, limit):
     assert info["iterations"] <= limit, info
     assert info["relative_residual"] <= 1.01e-10
     assert rel(x, ex) <= 1e-8
+
+
+RESIDENT_CASES = [
+    ("urban64", lambda: W.urban_block(n=(64, 48, 40), boxes=4)[0]),
+    ("urban128", lambda: W.urban_block(n=(128, 40, 12), boxes=4)[0]),
+    ("periodic-x", lambda: _with_wind(Case(cells=[32, 20, 16], size_m=[3.2, 2.0, 1.6],
+                                           boundaries=[PERIODIC, OPEN, GROUND_OPEN_TOP],
+                                           blocks=[Block([0.6, 0.5, 0.0], [1.4, 1.3, 0.9])],
+                                           preconditioner=MULTIGRID), 180.0)),
+    ("periodic-y", lambda: _with_wind(Case(cells=[32, 24, 20], size_m=[3.2, 2.4, 2.0],
+                                           boundaries=[OPEN, PERIODIC, GROUND_OPEN_TOP],
+                                           blocks=[Block([0.6, 0.5, 0.0], [1.4, 1.3, 0.9])],
+                                           preconditioner=MULTIGRID), 270.0)),
+    ("periodic-z", lambda: _with_wind(Case(cells=[36, 20, 16], size_m=[3.6, 2.0, 1.6],
+                                           boundaries=[OPEN, WALL, PERIODIC],
+                                           blocks=[Block([0.8, 0.4, 0.3], [1.6, 1.2, 0.9])],
+                                           preconditioner=MULTIGRID), 270.0)),
+]
+
+
+def _with_wind(case, direction):
+    """The log-law wind from `direction`: its outlet faces fix the pressure level
+    (no closed component)."""
+    W._neutral_rans(case, direction=direction)
+    return case
+
+
+@pytest.mark.parametrize("name,make", RESIDENT_CASES, ids=[n for n, _ in RESIDENT_CASES])
+def test_resident_residual_solves_bitwise(gpu_available, name, make):
+    """With two cells per thread and no closed component, the FP64 red-black
+    solve keeps the PCG's r and z in the fine level's colour-separated arrays
+    (the residual update scatters into them, the operator reads z from them,
+    the first half-sweep starts from them and the last one forms r . z there)
+    instead of natural-layout vectors. Every value and every partial sum keeps
+    its arithmetic and order: the field, the iteration count and the residuals
+    equal those of the natural-layout solve bit for bit, with the marching and
+    the cell-by-cell operator kernels, pairs in tiles or linear, periodic axes
+    included, and on a warm-started second solve."""
+    case = make()
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(21)
+    ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    b2 = o.apply(o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0)))
+    for march, lin in ((2, 1), (0, 1), (2, 0)):
+        out = []
+        for resident in (1, 0):
+            with env(CFD_MG_RB_PAIRS=1, CFD_MG_RB_LIN=lin, CFD_MARCH=march, CFD_RB_RESIDENT=resident):
+                g = cfdgpu.Grid(case)
+            print(f"[resident] {name} march={march} linear={lin} resident={resident}", flush=True)
+            # the path taken: the first half-sweep of a cycle reads r in the
+            # natural layout (21 bytes a cell) only in the set-up's cycle when r
+            # is kept colour by colour (8.5 bytes a cell in the others)
+            d_b, d_x = cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray(case.count)
+            first = g.profile_solve(d_b, d_x, 1e-11, "fine_first_half_sweep")
+            n = first["launches"]
+            want = (21.0 + 8.5 * (n - 1)) if resident else 21.0 * n
+            assert abs(first["bytes"] / case.count - want) < 1e-6 * n, (first, want)
+            x, info = g.solve(b, 1e-11, 3000)
+            g.set_options(cfdgpu.PRECISION_FP64, warm_start=True)
+            xw, infow = g.solve(b2, 1e-11, 3000)
+            out.append((x, info, xw, infow))
+        (x1, i1, w1, j1), (x0, i0, w0, j0) = out
+        for a, c in ((i1, i0), (j1, j0)):
+            assert a["iterations"] == c["iterations"], (a, c)
+            assert a["relative_residual"] == c["relative_residual"], (a, c)
+        assert np.array_equal(x1, x0), float(np.max(np.abs(x1 - x0)))
+        assert np.array_equal(w1, w0), float(np.max(np.abs(w1 - w0)))
+        assert rel(x1, ex) <= 1e-8
 
 
 @pytest.mark.parametrize("cells", [[64, 48, 40], [63, 20, 9], [128, 40, 12]])
