@@ -125,6 +125,9 @@ struct DevAggShared {
   int seams = 0;       // periodic axes of odd length on the fine level
   int nlev = 0;        // coarse levels
   std::vector<AggDevLevel<double>> shape;  // integer pointers and colour ranges per coarse level
+  // per coarse level: the distinct off-diagonal values its padded rows index
+  // (empty without padded rows)
+  std::vector<std::vector<double>> ell_values;
   bool built = false;
 };
 // The reference's cycle on the device (FP64).
@@ -381,15 +384,11 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
     d.x = x;
     d.K = sh.K;
     d.ecol = sh.ecol;
-    d.eval = nullptr;
+    d.etab = nullptr;
     if (sh.ecol) {
-      std::vector<double> ev((size_t)sh.K * (size_t)L.n, 0.0);
-      for (int64_t i = 0; i < L.n; ++i)
-        for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
-          ev[(size_t)(e - L.ptr[(size_t)i]) * (size_t)L.n + (size_t)i] = L.val[(size_t)e];
-      T* evp = nullptr;
-      if ((rc = up(ev, &evp))) return rc;
-      d.eval = evp;
+      T* tab = nullptr;
+      if ((rc = up(g->ash.ell_values[(size_t)l], &tab))) return rc;
+      d.etab = tab;
     }
     da.levels.push_back(d);
   }
@@ -441,6 +440,7 @@ int build_agg(cfd_grid* g) {
     return upi(t.data(), t.size(), out);
   };
   A.shape.clear();
+  A.ell_values.clear();
   for (int l = 1; l <= A.nlev; ++l) {
     const AggLevel& L = H.levels[(size_t)l];
     AggDevLevel<double> sh{};
@@ -456,22 +456,52 @@ int build_agg(cfd_grid* g) {
         (rc = upi64(L.child_ptr, &sh.child_ptr)) || (rc = upi(L.child.data(), L.child.size(), &sh.child)))
       return rc;
     if (!L.parent.empty() && (rc = upi(L.parent.data(), L.parent.size(), &sh.parent))) return rc;
-    // padded rows when every row has at most kEllWidth entries
+    // padded rows when every row has at most kEllWidth entries, the columns fit
+    // 24 bits and the off-diagonal values take at most 256 distinct bit patterns:
+    // each entry then packs its column and the index of its value
     int64_t K = 0;
     for (int64_t i = 0; i < L.n; ++i) K = std::max<int64_t>(K, L.ptr[(size_t)i + 1] - L.ptr[(size_t)i]);
     sh.K = 0;
     sh.ecol = nullptr;
-    if (K <= kEllWidth) {
+    std::vector<double> table;
+    if (K > 0 && K <= kEllWidth && L.n < (1LL << 24)) {
+      std::unordered_map<uint64_t, int32_t> index;
+      auto slot = [&](double v) -> int32_t {
+        uint64_t key;
+        std::memcpy(&key, &v, sizeof key);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        if (table.size() == 256) return -1;
+        index.emplace(key, (int32_t)table.size());
+        table.push_back(v);
+        return (int32_t)table.size() - 1;
+      };
+      const int32_t pad = slot(0.0);
       std::vector<int32_t> ec((size_t)K * (size_t)L.n);
-      for (int64_t i = 0; i < L.n; ++i)
+      bool fits = true;
+      for (int64_t i = 0; i < L.n && fits; ++i)
         for (int64_t k = 0; k < K; ++k) {
           const int64_t e = L.ptr[(size_t)i] + k;
-          ec[(size_t)k * (size_t)L.n + (size_t)i] = e < L.ptr[(size_t)i + 1] ? L.col[(size_t)e] : (int32_t)i;
+          const bool entry = e < L.ptr[(size_t)i + 1];
+          const int32_t v = entry ? slot(L.val[(size_t)e]) : pad;
+          if (v < 0) {
+            fits = false;
+            break;
+          }
+          const uint32_t w = (uint32_t)(entry ? L.col[(size_t)e] : (int32_t)i) | ((uint32_t)v << 24);
+          ec[(size_t)k * (size_t)L.n + (size_t)i] = (int32_t)w;
         }
-      sh.K = (int)K;
-      if (K > 0 && (rc = upi(ec.data(), ec.size(), &sh.ecol))) return rc;
+      if (fits) {
+        const int* ep = nullptr;
+        if ((rc = upi(ec.data(), ec.size(), &ep))) return rc;
+        sh.ecol = reinterpret_cast<const unsigned*>(ep);
+        sh.K = (int)K;
+      } else {
+        table.clear();
+      }
     }
     A.shape.push_back(sh);
+    A.ell_values.push_back(std::move(table));
   }
   // coarsest blocks (on the fine level itself the rows are cells)
   std::vector<int32_t> rows = H.block_rows;

@@ -92,11 +92,14 @@ struct AggDevLevel {
   const int* parent;     // node of the next level (null at the coarsest)
   const int* child_ptr;  // finer nodes of each node
   const int* child;
-  // the same rows padded to K <= kEllWidth entries (null when a row is longer):
-  // entry k of row i at [k * n + i], padding points at the row itself with value 0
+  // the same rows padded to K <= kEllWidth entries (null when a row is longer,
+  // the level has 2^24 nodes or more, or more than 256 distinct off-diagonal
+  // values): entry k of row i at [k * n + i] holds the column in its low 24 bits
+  // and the index of its value in etab in the high 8; padding points at the row
+  // itself with the value 0
   int K;
-  const int* ecol;
-  const T* eval;
+  const unsigned* ecol;
+  const T* etab;
 };
 constexpr int kEllWidth = 8;
 

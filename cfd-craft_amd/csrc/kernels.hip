@@ -1629,8 +1629,9 @@ __device__ __forceinline__ T ell_offdiag(const AggDevLevel<T>& L, int i, T v, bo
     cols[k] = i;
     vals[k] = T(0);
     if (k < L.K) {
-      cols[k] = L.ecol[(long long)k * L.n + i];
-      vals[k] = L.eval[(long long)k * L.n + i];
+      const unsigned w = L.ecol[(long long)k * L.n + i];
+      cols[k] = (int)(w & 0xFFFFFFu);
+      vals[k] = L.etab[w >> 24];
     }
   }
 #pragma unroll
@@ -3367,7 +3368,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
   // other classes' values read once
   auto coarse_bytes = [&](const AggDevLevel<T>& L, int c) {
     const double rows = (double)(L.cptr[c + 1] - L.cptr[c]);
-    const double per_row = L.ecol ? (double)L.K * (4.0 + sizeof(T)) : (double)L.nnz / (double)L.n * (4.0 + sizeof(T)) + 4.0;
+    const double per_row = L.ecol ? (double)L.K * 4.0 : (double)L.nnz / (double)L.n * (4.0 + sizeof(T)) + 4.0;
     return rows * (per_row + 3.0 * sizeof(T)) + ((double)L.n - rows) * sizeof(T);
   };
   auto relax = [&](const AggDevLevel<T>& L, int c) {
