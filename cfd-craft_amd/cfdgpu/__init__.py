@@ -157,6 +157,66 @@ PROBE_KERNELS = {
 }
 
 
+class CFace(ctypes.Structure):
+    """cfd_face: one entry of Grid::faces (grid.rs:85-108)."""
+    _fields_ = [("axis", ctypes.c_int32), ("kind", ctypes.c_int32), ("lo", ctypes.c_int64), ("hi", ctypes.c_int64),
+                ("area", ctypes.c_double), ("distance", ctypes.c_double)]
+
+
+class GridDesc(ctypes.Structure):
+    """cfd_grid_desc: the built Grid as the pressure path reads it (grid.rs:114-141)."""
+    _fields_ = [("n", ctypes.c_int64 * 3), ("dx", ctypes.c_double * 3), ("volume", ctypes.c_double),
+                ("periodic", ctypes.c_int32 * 3), ("fluid", ctypes.c_void_p), ("face_count", ctypes.c_int64),
+                ("faces", ctypes.c_void_p), ("component", ctypes.c_void_p), ("component_count", ctypes.c_int64),
+                ("anchored", ctypes.c_void_p), ("preconditioner", ctypes.c_int32)]
+
+
+FACE_INTERIOR, FACE_WALL, FACE_SLIP, FACE_INLET, FACE_OUTLET, FACE_BLOCKED = range(6)
+FACE_DTYPE = np.dtype([("axis", np.int32), ("kind", np.int32), ("lo", np.int64), ("hi", np.int64),
+                       ("area", np.float64), ("distance", np.float64)], align=True)
+
+
+class GridDescriptor:
+    """A caller's built Grid for cfd_grid_create_desc: cell counts n, cell sizes dx,
+    volume, periodic flags, the fluid mask (n0 n1 n2 flags) and the faces as a
+    FACE_DTYPE record array in Grid::faces order; optionally component (-1 for
+    solid) and anchored, which the library checks against the faces."""
+
+    def __init__(self, n, dx, volume, periodic, fluid, faces, preconditioner=MULTIGRID, component=None,
+                 anchored=None, tolerance=1e-10, max_iterations=1200):
+        self.n = [int(v) for v in n]
+        self.dx = [float(v) for v in dx]
+        self.volume = float(volume)
+        self.periodic = [1 if p else 0 for p in periodic]
+        self.fluid = np.ascontiguousarray(fluid, dtype=np.uint8)
+        self.faces = np.ascontiguousarray(faces, dtype=FACE_DTYPE)
+        self.preconditioner = int(preconditioner)
+        self.component = None if component is None else np.ascontiguousarray(component, dtype=np.int64)
+        self.anchored = None if anchored is None else np.ascontiguousarray(anchored, dtype=np.uint8)
+        self.tolerance = tolerance
+        self.max_iterations = max_iterations
+        if self.fluid.size != self.n[0] * self.n[1] * self.n[2]:
+            raise ValueError("fluid: one flag per cell")
+
+    def to_c(self) -> GridDesc:
+        d = GridDesc()
+        for a in range(3):
+            d.n[a], d.dx[a], d.periodic[a] = self.n[a], self.dx[a], self.periodic[a]
+        d.volume = self.volume
+        d.fluid = self.fluid.ctypes.data
+        d.face_count = self.faces.size
+        d.faces = self.faces.ctypes.data
+        if self.component is not None:
+            if self.component.size != self.fluid.size:
+                raise ValueError("component: one entry per cell")
+            d.component = self.component.ctypes.data
+            d.component_count = 0 if self.anchored is None else self.anchored.size
+            if self.anchored is not None:
+                d.anchored = self.anchored.ctypes.data
+        d.preconditioner = self.preconditioner
+        return d
+
+
 class Options(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int32),
@@ -186,6 +246,8 @@ def lib():
             "cfd_grid_create": (I, [P, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_grid_destroy": (None, [P]),
             "cfd_grid_matches": (I, [P, P, P]),
+            "cfd_grid_create_desc": (I, [P, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_grid_matches_desc": (I, [P, P, P]),
             "cfd_grid_counts": (I, [P, P]),
             "cfd_grid_codes": (I, [P, P]),
             "cfd_grid_components": (I, [P, P, P]),
@@ -383,6 +445,24 @@ class Grid:
         began = time.perf_counter()
         _check(lib().cfd_grid_create(ctypes.byref(c), ctypes.byref(h)))
         self._attach(h, case, began)
+
+    @classmethod
+    def from_descriptor(cls, desc: "GridDescriptor") -> "Grid":
+        """The grid of a Grid the caller built (cfd_grid_create_desc)."""
+        g = cls.__new__(cls)
+        d = desc.to_c()
+        h = ctypes.c_void_p()
+        began = time.perf_counter()
+        _check(lib().cfd_grid_create_desc(ctypes.byref(d), ctypes.byref(h)))
+        g._attach(h, desc, began)
+        return g
+
+    def matches_descriptor(self, desc: "GridDescriptor") -> bool:
+        """Signature::matches against a caller's Grid (cfd_grid_matches_desc)."""
+        d = desc.to_c()
+        same = ctypes.c_int(0)
+        _check(lib().cfd_grid_matches_desc(self.h, ctypes.byref(d), ctypes.byref(same)))
+        return bool(same.value)
 
     def _attach(self, h, case: Case, began: float):
         self.case = case

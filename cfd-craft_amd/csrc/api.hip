@@ -1413,17 +1413,33 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
   return CFD_OK;
 }
 
-void cfd_grid_destroy(cfd_grid* g) { delete g; }
-
-int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same) {
-  if (!g || !c || !same) {
-    set_error("cfd_grid_matches: null argument");
+int cfd_grid_create_desc(const cfd_grid_desc* d, cfd_grid** out) {
+  if (!d || !out) {
+    set_error("cfd_grid_create_desc: null argument");
     return CFD_EINVAL;
   }
-  *same = 0;
-  HostGrid h;
-  if (int rc = build_host_grid(*c, h)) return rc;
-  const HostGrid& o = g->host;
+  *out = nullptr;
+  std::unique_ptr<cfd_grid> g(new cfd_grid());
+  int rc = grid_from_desc(*d, g->host, true);
+  if (rc) return rc;
+  if (d->preconditioner == CFD_IC0) {
+    bool fill = false;
+    for (int a = 0; a < 3; ++a) fill |= g->host.periodic[a] && g->host.n[a] == 3;
+    if (fill) {
+      set_error("ic0: a 3-cell periodic axis gives the IC(0) factor fill coupling, which the device sweeps do not carry");
+      return CFD_EUNSUPPORTED;
+    }
+  }
+  rc = setup_device(g.get());
+  if (rc) return rc;
+  *out = g.release();
+  return CFD_OK;
+}
+
+void cfd_grid_destroy(cfd_grid* g) { delete g; }
+
+// Signature::matches over two host grids (multigrid.rs:158-222).
+static bool same_operator(const HostGrid& h, const HostGrid& o) {
   auto bits = [](double v) {
     uint64_t u;
     std::memcpy(&u, &v, sizeof u);
@@ -1437,7 +1453,33 @@ int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same) {
   // fluid bit and the closed bit, the fluid mask and the anchoring of each cell's
   // component; components are numbered by their lowest cell in both builds
   eq = eq && h.fluid == o.fluid && h.anchored == o.anchored && h.codes == o.codes;
-  *same = eq ? 1 : 0;
+  return eq;
+}
+
+int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same) {
+  if (!g || !c || !same) {
+    set_error("cfd_grid_matches: null argument");
+    return CFD_EINVAL;
+  }
+  *same = 0;
+  HostGrid h;
+  if (int rc = build_host_grid(*c, h)) return rc;
+  *same = same_operator(h, g->host) ? 1 : 0;
+  return CFD_OK;
+}
+
+int cfd_grid_matches_desc(const cfd_grid* g, const cfd_grid_desc* d, int* same) {
+  if (!g || !d || !same) {
+    set_error("cfd_grid_matches_desc: null argument");
+    return CFD_EINVAL;
+  }
+  *same = 0;
+  HostGrid h;
+  if (int rc = grid_from_desc(*d, h, false)) return rc;
+  uint64_t a = 0, b = 0;
+  std::memcpy(&a, &h.volume, sizeof a);
+  std::memcpy(&b, &g->host.volume, sizeof b);
+  *same = a == b && same_operator(h, g->host) ? 1 : 0;
   return CFD_OK;
 }
 

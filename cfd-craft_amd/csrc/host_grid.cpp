@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <deque>
 
 #include "internal.hpp"
@@ -227,8 +228,18 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
         g.codes[(size_t)c0] = code;
       }
 
-  // Fluid components by breadth-first search from the lowest unassigned cell
-  // (grid.rs:430-462): the same partition and numbering as the reference.
+  find_components(g);
+  closed_lists(g);
+  return 0;
+}
+
+// Fluid components by breadth-first search from the lowest unassigned cell
+// (grid.rs:430-462): the same partition and numbering as the reference; a
+// component with a coupled domain face (an outlet) is anchored.
+void find_components(HostGrid& g) {
+  const int64_t total = g.count;
+  const int64_t* n = g.n;
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
   g.component.assign((size_t)total, -1);
   g.anchored.clear();
   std::vector<int64_t> queue;
@@ -263,6 +274,165 @@ int build_host_grid(const cfd_case& c, HostGrid& g) {
         }
     }
     g.anchored.push_back(anchored ? 1 : 0);
+  }
+}
+
+// The grid of a caller's built Grid (cfd_grid_desc): the mask, the periodic axes
+// and the faces give the coupling codes (an Interior face couples its two cells
+// unless they are one cell of a one-cell periodic axis; an Outlet face couples
+// its cell to the outside), the weights come from the faces' own area and
+// distance (pressure.rs:257), and the components from the codes as in
+// build_host_grid. strict: weights that differ within an axis and face kind are
+// CFD_EUNSUPPORTED; otherwise they leave that weight NaN (it then matches no grid).
+int grid_from_desc(const cfd_grid_desc& d, HostGrid& g, bool strict) {
+  char buf[256];
+  int64_t total = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t minimum = d.periodic[a] ? 1 : 2;
+    if (d.n[a] < minimum || d.n[a] > 4096) {
+      set_error("CFD-E004: grid.cells needs 2..4096 cells per axis (1..4096 on a periodic axis)");
+      return CFD_EINVAL;
+    }
+    if (!(d.dx[a] > 0.0) || !std::isfinite(d.dx[a])) {
+      set_error("cfd_grid_desc: cell sizes must be finite and positive");
+      return CFD_EINVAL;
+    }
+    total *= d.n[a];
+  }
+  if (total > 100000000) {
+    set_error("CFD-E006: at most 1e8 cells");
+    return CFD_EINVAL;
+  }
+  if (!(d.volume > 0.0) || !std::isfinite(d.volume) || !d.fluid || !d.faces) {
+    set_error("cfd_grid_desc: a positive finite volume, the fluid mask and the faces are required");
+    return CFD_EINVAL;
+  }
+  if (d.preconditioner != CFD_IC0 && d.preconditioner != CFD_MULTIGRID) {
+    set_error("numerics.pressure_preconditioner must be ic0 or multigrid");
+    return CFD_EINVAL;
+  }
+  int64_t fdim[3][3], faces = 0, first[3];
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = d.n[a];
+    g.dx[a] = d.dx[a];
+    g.periodic[a] = d.periodic[a] != 0;
+    g.boundary[a] = g.periodic[a] ? CFD_PERIODIC : CFD_WALL;
+    for (int b = 0; b < 3; ++b) fdim[a][b] = d.n[b] + (b == a && !g.periodic[a] ? 1 : 0);
+    first[a] = faces;
+    g.face_count[a] = fdim[a][0] * fdim[a][1] * fdim[a][2];
+    faces += g.face_count[a];
+  }
+  if (d.face_count != faces) {
+    std::snprintf(buf, sizeof buf, "cfd_grid_desc: %lld faces given, the grid has %lld", (long long)d.face_count,
+                  (long long)faces);
+    set_error(buf);
+    return CFD_EINVAL;
+  }
+  g.count = total;
+  g.volume = d.volume;
+  g.preconditioner = d.preconditioner;
+  g.fluid.assign(d.fluid, d.fluid + total);
+  for (uint8_t& f : g.fluid) f = f ? 1 : 0;
+  if (int rc = count_fluid(g)) return rc;
+  const int64_t* n = g.n;
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
+  g.codes.assign((size_t)total, 0);
+  for (int64_t c0 = 0; c0 < total; ++c0)
+    if (g.fluid[(size_t)c0]) g.codes[(size_t)c0] = kFluid;
+  const uint64_t unset = ~0ULL;
+  uint64_t wbits[2][3] = {{unset, unset, unset}, {unset, unset, unset}};
+  bool mixed[2][3] = {{false, false, false}, {false, false, false}};
+  int64_t wface[2][3] = {{NONE, NONE, NONE}, {NONE, NONE, NONE}};  // the face that set each weight
+  for (int a = 0; a < 3; ++a) {
+    for (int64_t e = 0; e < g.face_count[a]; ++e) {
+      const int64_t f = first[a] + e;
+      const cfd_face& F = d.faces[f];
+      const int64_t q[3] = {e % fdim[a][0], e / fdim[a][0] % fdim[a][1], e / (fdim[a][0] * fdim[a][1])};
+      const int64_t t = q[a];
+      const int64_t base = q[0] * stride[0] + q[1] * stride[1] + q[2] * stride[2] - t * stride[a];
+      const int64_t above = t < n[a] ? base + t * stride[a] : NONE;
+      const int64_t below = t > 0 ? base + (t - 1) * stride[a] : (g.periodic[a] ? base + (n[a] - 1) * stride[a] : NONE);
+      const int64_t lo = below != NONE && g.fluid[(size_t)below] ? below : NONE;
+      const int64_t hi = above != NONE && g.fluid[(size_t)above] ? above : NONE;
+      if (F.axis != a || F.lo != lo || F.hi != hi || F.kind < CFD_FACE_INTERIOR || F.kind > CFD_FACE_BLOCKED) {
+        std::snprintf(buf, sizeof buf,
+                      "cfd_grid_desc: face %lld is not the Grid's (axis %d, lo %lld, hi %lld expected; kind 0..5)",
+                      (long long)f, a, (long long)lo, (long long)hi);
+        set_error(buf);
+        return CFD_EINVAL;
+      }
+      const bool domain = below == NONE || above == NONE;
+      int role = -1;  // 0 interior weight, 1 outlet weight
+      if (F.kind == CFD_FACE_INTERIOR) {
+        if (lo == NONE || hi == NONE) {
+          std::snprintf(buf, sizeof buf, "cfd_grid_desc: interior face %lld lacks a fluid cell on one side", (long long)f);
+          set_error(buf);
+          return CFD_EINVAL;
+        }
+        if (lo != hi) {
+          g.codes[(size_t)lo] |= side_bit(a, 1);
+          g.codes[(size_t)hi] |= side_bit(a, 0);
+          role = 0;
+        }
+      } else if (F.kind == CFD_FACE_OUTLET) {
+        if (!domain || (lo == NONE) == (hi == NONE)) {
+          std::snprintf(buf, sizeof buf, "cfd_grid_desc: outlet face %lld is not a domain face of one fluid cell",
+                        (long long)f);
+          set_error(buf);
+          return CFD_EINVAL;
+        }
+        if (hi != NONE)
+          g.codes[(size_t)hi] |= side_bit(a, 0);
+        else
+          g.codes[(size_t)lo] |= side_bit(a, 1);
+        role = 1;
+      }
+      if (role < 0) continue;
+      const double w = F.area / (F.distance * g.volume);
+      uint64_t bits;
+      std::memcpy(&bits, &w, sizeof bits);
+      if (!std::isfinite(w) || !(w > 0.0)) {
+        std::snprintf(buf, sizeof buf, "cfd_grid_desc: face %lld has no positive finite pressure weight", (long long)f);
+        set_error(buf);
+        return CFD_EINVAL;
+      }
+      if (wbits[role][a] == unset) {
+        wbits[role][a] = bits;
+        wface[role][a] = f;
+      } else if (wbits[role][a] != bits) {
+        if (strict) {
+          std::snprintf(buf, sizeof buf,
+                        "cfd_grid_desc: faces %lld and %lld (axis %d) carry different %s pressure weights; the device "
+                        "operator holds one interior and one outlet weight per axis",
+                        (long long)wface[role][a], (long long)f, a, role ? "outlet" : "interior");
+          set_error(buf);
+          return CFD_EUNSUPPORTED;
+        }
+        mixed[role][a] = true;
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    // an axis without faces of a kind keeps the weight Grid::build gives them
+    const double area = g.volume / g.dx[a];
+    double w[2] = {area / (g.dx[a] * g.volume), area / ((0.5 * g.dx[a]) * g.volume)};
+    for (int r = 0; r < 2; ++r) {
+      if (wbits[r][a] != unset) std::memcpy(&w[r], &wbits[r][a], sizeof(double));
+      if (mixed[r][a]) w[r] = std::nan("");
+    }
+    g.w_int[a] = w[0];
+    g.w_out[a] = w[1];
+  }
+  find_components(g);
+  if (d.component) {
+    bool same = d.component_count == (int64_t)g.anchored.size();
+    for (int64_t c0 = 0; c0 < total && same; ++c0) same = d.component[c0] == (int64_t)g.component[(size_t)c0];
+    for (int64_t id = 0; id < d.component_count && same && d.anchored; ++id)
+      same = (d.anchored[id] != 0) == (g.anchored[(size_t)id] != 0);
+    if (!same) {
+      set_error("cfd_grid_desc: the components or anchored flags differ from those its faces give (grid.rs:430-462)");
+      return CFD_EINVAL;
+    }
   }
   closed_lists(g);
   return 0;

@@ -60,6 +60,11 @@ int block_error(const cfd_case& c, int b);
 int count_fluid(HostGrid& g);
 void open_outlets(const cfd_case& c, const HostGrid& g, std::vector<uint8_t>& out);
 void closed_lists(HostGrid& g);
+// Components and anchoring from the codes (grid.rs:430-462).
+void find_components(HostGrid& g);
+// The grid of a caller's built Grid; strict: CFD_EUNSUPPORTED for weights that
+// vary within an axis and face kind (else those weights become NaN).
+int grid_from_desc(const cfd_grid_desc& d, HostGrid& g, bool strict);
 void build_ic0_diagonal(HostGrid& g);
 
 // Geometric hierarchy: 2x2x2 bins of the structured grid with Galerkin

@@ -185,6 +185,56 @@ void cfd_grid_destroy(cfd_grid* g);
  * Builds the case's topology on the host (O(cells)); errors of Grid::build are
  * returned as by cfd_grid_create. */
 int cfd_grid_matches(const cfd_grid* g, const cfd_case* c, int* same);
+
+/* FaceKind (grid.rs:60-81). */
+enum { CFD_FACE_INTERIOR = 0, CFD_FACE_WALL = 1, CFD_FACE_SLIP = 2, CFD_FACE_INLET = 3, CFD_FACE_OUTLET = 4,
+       CFD_FACE_BLOCKED = 5 };
+/* One entry of Grid::faces (grid.rs:85-108): normal axis, kind, the fluid cells
+ * below and above (-1 for None; across a periodic boundary the wrapped cell),
+ * area (m^2) and gradient distance (m). A caller may have edited area or
+ * distance after Grid::build (pressure.rs:244). */
+typedef struct {
+  int32_t axis;
+  int32_t kind;
+  int64_t lo, hi;
+  double area;
+  double distance;
+} cfd_face;
+/* The built Grid as the pressure path reads it (grid.rs:114-141): cell counts,
+ * cell sizes and volume, periodic axes (face_dims[a][a] == n[a]), the fluid
+ * mask, every face in Grid::faces order (x-faces, y-faces, z-faces, each x
+ * fastest over face_dims), and optionally Grid::component (-1 for solid
+ * cells) with Grid::anchored, which are then checked against the faces. */
+typedef struct {
+  int64_t n[3];
+  double dx[3];
+  double volume;
+  int32_t periodic[3];
+  const uint8_t* fluid;
+  int64_t face_count;
+  const cfd_face* faces;
+  const int64_t* component;  /* optional (NULL) */
+  int64_t component_count;
+  const uint8_t* anchored;   /* optional (NULL), component_count entries */
+  int32_t preconditioner;    /* CFD_IC0 or CFD_MULTIGRID */
+} cfd_grid_desc;
+/* The grid context from a Grid the caller already built, in place of
+ * cfd_grid_create's own Grid::build (the pressure path's view of the Rust
+ * Grid: Operator::build reads the faces, pressure.rs:255-279). The faces'
+ * order, kinds and cells are checked against the mask and the periodic axes
+ * (CFD_EINVAL naming the first face that differs). The device operator holds
+ * one interior and one outlet weight area / (distance * volume) per axis: a
+ * grid whose interior (or outlet) faces of one axis carry different weight
+ * bits, as after a caller's edit of one face, is refused with
+ * CFD_EUNSUPPORTED naming the face; Wall, Slip, Inlet and Blocked faces carry
+ * no pressure weight and may be edited freely. */
+int cfd_grid_create_desc(const cfd_grid_desc* d, cfd_grid** out);
+/* Signature::matches (multigrid.rs:158-222) against a descriptor: *same = 1
+ * when the descriptor gives the grid's operator exactly (cell counts, spacing
+ * and volume bits, fluid mask, anchoring, every interior and outlet face with
+ * its weight bits, periodic axes, preconditioner), else 0 (also when its
+ * weights vary within an axis, which cfd_grid_create_desc then refuses). */
+int cfd_grid_matches_desc(const cfd_grid* g, const cfd_grid_desc* d, int* same);
 /* counts: cells, faces, fluid cells, components, anchored components, x/y/z face counts. */
 int cfd_grid_counts(const cfd_grid* g, int64_t counts[8]);
 /* Per cell coupling code: bit 2a = lower face on axis a couples (interior or
