@@ -116,6 +116,7 @@ class MgStats(ctypes.Structure):
         ("allocated_bytes", ctypes.c_int64),
         ("n_block_smoothers", ctypes.c_int64),
         ("block_smoothers", BlockSmootherStats * 32),
+        ("digest", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -364,6 +365,7 @@ def case_multigrid_stats(case: Case, cycle: int = 0):
     _check(lib().cfd_case_multigrid_stats(ctypes.byref(c), int(cycle), ctypes.byref(s)))
     d = s.as_dict()
     del d["allocated_bytes"]  # nothing is allocated on a device here
+    d["digest"] = int(s.digest)
     return d
 
 

@@ -569,22 +569,24 @@ int build_agg(cfd_grid* g) {
     const int64_t bx = (h.n[0] + 1) / 2, by = (h.n[1] + 1) / 2, bz = (h.n[2] + 1) / 2;
     std::vector<int> bpar((size_t)(bx * by * bz), 0);
     std::vector<uint8_t> bseen(bpar.size(), 0);
-    for (int64_t c = 0; c < h.count; ++c) {
-      const int64_t i = c % h.n[0], j = c / h.n[0] % h.n[1], k = c / (h.n[0] * h.n[1]);
-      const int colour = (int)((i + j + k) & 1);
-      const size_t u = (size_t)((i >> 1) + (long long)hx * (j + h.n[1] * k));
-      rc8[(size_t)colour * (size_t)half + u] = h.codes[(size_t)c];
-      const int32_t node = H.node_of_cell[(size_t)c];
-      if (colour == 0 && node >= 0) rpar[u] = H.levels[0].parent[(size_t)node];
-      const int par = node >= 0 ? H.levels[0].parent[(size_t)node] : -1;
-      const size_t blk = (size_t)((i >> 1) + bx * ((j >> 1) + by * (k >> 1)));
-      if (!bseen[blk]) {
-        bseen[blk] = 1;
-        bpar[blk] = par;
-      } else if (bpar[blk] != par) {
-        bpar[blk] = kMixedParent;
-      }
-    }
+    const std::vector<int32_t>& parent0 = H.levels[0].parent;
+    for (int64_t k = 0, c = 0; k < h.n[2]; ++k)
+      for (int64_t j = 0; j < h.n[1]; ++j)
+        for (int64_t i = 0; i < h.n[0]; ++i, ++c) {
+          const int colour = (int)((i + j + k) & 1);
+          const size_t u = (size_t)((i >> 1) + (long long)hx * (j + h.n[1] * k));
+          rc8[(size_t)colour * (size_t)half + u] = h.codes[(size_t)c];
+          const int32_t node = H.node_of_cell[(size_t)c];
+          const int par = node >= 0 ? parent0[(size_t)node] : -1;
+          if (colour == 0 && node >= 0) rpar[u] = par;
+          const size_t blk = (size_t)((i >> 1) + bx * ((j >> 1) + by * (k >> 1)));
+          if (!bseen[blk]) {
+            bseen[blk] = 1;
+            bpar[blk] = par;
+          } else if (bpar[blk] != par) {
+            bpar[blk] = kMixedParent;
+          }
+        }
     if ((rc = A.rb_codes.upload(rc8.data(), rc8.size())) || (rc = A.rb_rpar.upload(rpar.data(), rpar.size())) ||
         (rc = A.rb_bpar.upload(bpar.data(), bpar.size())) ||
         (rc = A.rb_b64.alloc((size_t)(2 * half))) ||
@@ -1557,6 +1559,7 @@ int cfd_case_multigrid_stats(const cfd_case* c, int32_t cycle, cfd_mg_stats* s) 
   }
   s->levels = L;
   agg_hierarchy_shape(H, s);
+  s->digest = agg_hierarchy_digest(H);
   return CFD_OK;
 }
 

@@ -15,14 +15,31 @@
 // the nodes of one independent set are contiguous (the red-black cycle).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 #include <numeric>
+#include <mutex>
+#include <thread>
 
 #include "internal.hpp"
 
 namespace cfdgpu {
 namespace {
 
+
+// Runs f(begin, end) over [0, n) split among the host's threads (at most 16).
+template <class F>
+void parallel_ranges(int64_t n, F f) {
+  const int64_t threads = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), n}));
+  if (threads == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int64_t t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] { f(n * t / threads, n * (t + 1) / threads); });
+  for (auto& th : pool) th.join();
+}
 
 struct UnionFind {
   std::vector<int32_t> p;
@@ -87,11 +104,13 @@ void for_each_face(const HostGrid& g, E edge, O outlet, bool self_loops = false)
 // Connected components of a level by breadth-first search from the lowest
 // unvisited node (masked.rs:90-126); members are stored in ascending order.
 void level_components(const AggLevel& L, std::vector<std::vector<int32_t>>& comps, std::vector<uint8_t>& anchored) {
-  std::vector<uint8_t> seen((size_t)L.n, 0);
+  std::vector<int32_t> comp((size_t)L.n, -1);
+  std::vector<int64_t> size;
   std::vector<int32_t> queue;
   for (int64_t s = 0; s < L.n; ++s) {
-    if (seen[(size_t)s]) continue;
-    seen[(size_t)s] = 1;
+    if (comp[(size_t)s] >= 0) continue;
+    const int32_t id = (int32_t)size.size();
+    comp[(size_t)s] = id;
     queue.assign(1, (int32_t)s);
     bool fixed = false;
     for (size_t head = 0; head < queue.size(); ++head) {
@@ -99,16 +118,19 @@ void level_components(const AggLevel& L, std::vector<std::vector<int32_t>>& comp
       fixed |= L.anchor[(size_t)i] > 0.0;
       for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
         const int32_t j = L.col[(size_t)e];
-        if (!seen[(size_t)j]) {
-          seen[(size_t)j] = 1;
+        if (comp[(size_t)j] < 0) {
+          comp[(size_t)j] = id;
           queue.push_back(j);
         }
       }
     }
-    std::sort(queue.begin(), queue.end());
-    comps.push_back(queue);
+    size.push_back((int64_t)queue.size());
     anchored.push_back(fixed ? 1 : 0);
   }
+  // members ascending: the nodes in order, each to its component
+  comps.resize(size.size());
+  for (size_t r = 0; r < size.size(); ++r) comps[r].reserve((size_t)size[r]);
+  for (int64_t i = 0; i < L.n; ++i) comps[(size_t)comp[(size_t)i]].push_back((int32_t)i);
 }
 
 // CSR of a level from its edge list (a < b, sorted by (a, b)) and anchors, in
@@ -176,22 +198,11 @@ void build_children(AggHierarchy& h) {
   }
 }
 
-// Sums fine couplings between distinct aggregates into the coarse edge list,
-// keyed by (min, max) and accumulated in the order the fine edges come.
+// Coarse pair sums of the trilinear Galerkin product, keyed by (min, max) and
+// accumulated in arrival order, emitted by row with columns ascending.
 struct EdgeSums {
   std::vector<std::vector<std::pair<int32_t, double>>> rows;
   explicit EdgeSums(int64_t n) : rows((size_t)n) {}
-  void add(int32_t i, int32_t j, double w) {
-    if (i == j) return;
-    const int32_t a = std::min(i, j), b = std::max(i, j);
-    auto& r = rows[(size_t)a];
-    for (auto& e : r)
-      if (e.first == b) {
-        e.second += w;
-        return;
-      }
-    r.emplace_back(b, w);
-  }
   void emit(AggLevel& C) {
     for (size_t a = 0; a < rows.size(); ++a) {
       auto& r = rows[a];
@@ -206,10 +217,66 @@ struct EdgeSums {
   }
 };
 
+// Sums fine couplings between distinct aggregates into the coarse edge list,
+// keyed by (min, max) and accumulated in the order the fine edges come; the
+// coarse edges are emitted by row, columns ascending. `edges(f)` calls
+// f(a, b, w) for every fine edge in order and is run twice: once to size each
+// coarse row's slots (its count of contributions bounds its distinct columns),
+// once to add each contribution to the entry of its column in the row, or to
+// the row's next slot for a new column.
+template <class Edges>
+void coarse_edges(int64_t m, const std::vector<int32_t>& parent, Edges edges, AggLevel& C) {
+  std::vector<int64_t> start((size_t)m + 1, 0);
+  edges([&](int32_t a, int32_t b, double) {
+    const int32_t i = parent[(size_t)a], j = parent[(size_t)b];
+    if (i != j) start[(size_t)std::min(i, j) + 1] += 1;
+  });
+  for (int64_t r = 0; r < m; ++r) start[(size_t)r + 1] += start[(size_t)r];
+  std::vector<int32_t> col((size_t)start[(size_t)m]);
+  std::vector<double> val((size_t)start[(size_t)m]);
+  std::vector<int64_t> end(start.begin(), start.end() - 1);
+  edges([&](int32_t a, int32_t b, double w) {
+    const int32_t i = parent[(size_t)a], j = parent[(size_t)b];
+    if (i == j) return;
+    const int32_t r = std::min(i, j), c = std::max(i, j);
+    int64_t e = start[(size_t)r];
+    const int64_t stop = end[(size_t)r];
+    while (e < stop && col[(size_t)e] != c) ++e;
+    if (e < stop) {
+      val[(size_t)e] += w;
+    } else {
+      col[(size_t)e] = c;
+      val[(size_t)e] = w;
+      end[(size_t)r] = e + 1;
+    }
+  });
+  int64_t total = 0;
+  for (int64_t r = 0; r < m; ++r) total += end[(size_t)r] - start[(size_t)r];
+  C.ea.reserve((size_t)total);
+  C.eb.reserve((size_t)total);
+  C.ew.reserve((size_t)total);
+  for (int64_t r = 0; r < m; ++r) {
+    // the row's distinct columns in ascending order (a row holds a handful)
+    const int64_t b0 = start[(size_t)r], b1 = end[(size_t)r];
+    for (int64_t x = b0 + 1; x < b1; ++x)
+      for (int64_t y = x; y > b0 && col[(size_t)y - 1] > col[(size_t)y]; --y) {
+        std::swap(col[(size_t)y - 1], col[(size_t)y]);
+        std::swap(val[(size_t)y - 1], val[(size_t)y]);
+      }
+    for (int64_t e = b0; e < b1; ++e) {
+      C.ea.push_back((int32_t)r);
+      C.eb.push_back(col[(size_t)e]);
+      C.ew.push_back(val[(size_t)e]);
+    }
+  }
+}
+
 // Groups the nodes of level L into aggregates: bins of coordinates >> shift[d]
 // per axis, split into connected groups, numbered by their lowest member.
-int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, const int shift[3],
-                  const std::function<void(const std::function<void(int32_t, int32_t)>&)>& edges,
+// `edges(f)` calls f(a, b) for every edge of the level (any order: only the
+// partition matters).
+template <class Edges>
+int64_t aggregate(int64_t n, const std::vector<int32_t>& coords, const int shift[3], Edges edges,
                   std::vector<int32_t>& parent, std::vector<int32_t>& coarse_coords) {
   UnionFind uf(n);
   auto same_bin = [&](int32_t i, int32_t j) {
@@ -308,6 +375,177 @@ int coarsest_block(const AggLevel& L, const std::vector<int32_t>& members, bool 
   return 0;
 }
 
+// The first aggregation of level 0 with bins of 2x2x2 cells, straight from the
+// codes, and the coarse edge list it gives: the same partition, numbering and
+// sums as aggregate() and coarse_edges() over level 0's face-ordered edge list.
+// Within a bin the fluid cells, in node order, are joined along their coupled
+// faces; an aggregate is numbered by its lowest member. Two aggregates in
+// different bins meet only on the interface of their two bins, whose faces all
+// belong to one axis, so the sum of each coarse edge is formed over that
+// interface's faces in face order. Returns false (nothing built) where a
+// periodic axis has exactly two bins (a bin pair would meet twice) or where the
+// bins reduce nothing.
+bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggLevel& L0, std::vector<int32_t>& parent,
+                 std::vector<int32_t>& coarse_coords, AggLevel& C) {
+  const int64_t* n = g.n;
+  const int64_t st[3] = {1, n[0], n[0] * n[1]};
+  int64_t nb[3];
+  for (int a = 0; a < 3; ++a) {
+    nb[a] = (n[a] + 1) / 2;
+    if (g.periodic[a] && nb[a] == 2) return false;
+  }
+  // lowest member of each node's aggregate (bins of one layer per task: they
+  // hold disjoint nodes)
+  std::vector<int32_t> low((size_t)L0.n);
+  parallel_ranges(nb[2], [&](int64_t z0, int64_t z1) {
+  for (int64_t bz = z0; bz < z1; ++bz)
+    for (int64_t by = 0; by < nb[1]; ++by)
+      for (int64_t bx = 0; bx < nb[0]; ++bx) {
+        int loc[2][2][2];
+        int64_t cell[8];
+        int up[8];
+        int cnt = 0;
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              loc[dz][dy][dx] = -1;
+              const int64_t x = 2 * bx + dx, y = 2 * by + dy, z = 2 * bz + dz;
+              if (x >= n[0] || y >= n[1] || z >= n[2]) continue;
+              const int64_t c = x + y * st[1] + z * st[2];
+              if (!g.fluid[(size_t)c]) continue;
+              loc[dz][dy][dx] = cnt;
+              cell[cnt] = c;
+              up[cnt] = cnt;
+              ++cnt;
+            }
+        if (cnt == 0) continue;
+        auto find = [&](int u) {
+          while (up[u] != u) u = up[u];
+          return u;
+        };
+        auto unite = [&](int a, int b) {
+          a = find(a);
+          b = find(b);
+          if (a < b)
+            up[b] = a;
+          else if (b < a)
+            up[a] = b;
+        };
+        for (int dz = 0; dz < 2; ++dz)
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              const int u = loc[dz][dy][dx];
+              if (u < 0) continue;
+              const uint8_t code = g.codes[(size_t)cell[u]];
+              if (dx == 0 && loc[dz][dy][1] >= 0 && (code & side_bit(0, 1))) unite(u, loc[dz][dy][1]);
+              if (dy == 0 && loc[dz][1][dx] >= 0 && (code & side_bit(1, 1))) unite(u, loc[dz][1][dx]);
+              if (dz == 0 && loc[1][dy][dx] >= 0 && (code & side_bit(2, 1))) unite(u, loc[1][dy][dx]);
+            }
+        // the root of a group is its smallest local index, the lowest node
+        for (int u = 0; u < cnt; ++u) low[(size_t)node[(size_t)cell[u]]] = node[(size_t)cell[find(u)]];
+      }
+  });
+  parent.assign((size_t)L0.n, -1);
+  coarse_coords.clear();
+  int64_t m = 0;
+  for (int64_t i = 0; i < L0.n; ++i) {
+    if (low[(size_t)i] == (int32_t)i) {
+      parent[(size_t)i] = (int32_t)m++;
+      for (int d = 0; d < 3; ++d) coarse_coords.push_back(L0.coords[3 * (size_t)i + d] >> 1);
+    } else {
+      parent[(size_t)i] = parent[(size_t)low[(size_t)i]];
+    }
+  }
+  if (m == L0.n) return false;
+  // coarse edges: each interface between a bin and its upper neighbour along an
+  // axis (wrapping on a periodic axis of three or more bins), its faces in face
+  // order (the two other axes, the higher one outer)
+  struct Entry {
+    int32_t r, c;
+    double w;
+  };
+  // (layers of bins per task, each with its own list: every pair is formed on
+  // one interface, so the lists only need joining and sorting)
+  std::vector<std::vector<Entry>> lists(16);
+  std::vector<int64_t> count((size_t)m + 1, 0);
+  std::mutex join;
+  size_t next_list = 0;
+  parallel_ranges(nb[2], [&](int64_t z0, int64_t z1) {
+  std::vector<Entry> list;
+  for (int a = 0; a < 3; ++a) {
+    if (nb[a] == 1) continue;
+    const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+    const double w = g.w_int[a];
+    for (int64_t bz = z0; bz < z1; ++bz)
+      for (int64_t by = 0; by < nb[1]; ++by)
+        for (int64_t bx = 0; bx < nb[0]; ++bx) {
+          const int64_t B[3] = {bx, by, bz};
+          int64_t next = B[a] + 1;
+          if (next == nb[a]) {
+            if (!g.periodic[a]) continue;
+            next = 0;
+          }
+          const int64_t t_lo = std::min(2 * B[a] + 1, n[a] - 1), t_hi = 2 * next;
+          Entry e[4];
+          int ne = 0;
+          for (int s2 = 0; s2 < 2; ++s2) {
+            const int64_t q2 = 2 * B[b2] + s2;
+            if (q2 >= n[b2]) continue;
+            for (int s1 = 0; s1 < 2; ++s1) {
+              const int64_t q1 = 2 * B[b1] + s1;
+              if (q1 >= n[b1]) continue;
+              const int64_t lo = t_lo * st[a] + q1 * st[b1] + q2 * st[b2];
+              const int64_t hi = t_hi * st[a] + q1 * st[b1] + q2 * st[b2];
+              if (!g.fluid[(size_t)lo] || !(g.codes[(size_t)lo] & side_bit(a, 1))) continue;
+              const int32_t P = parent[(size_t)node[(size_t)lo]], Q = parent[(size_t)node[(size_t)hi]];
+              const int32_t r = std::min(P, Q), c = std::max(P, Q);
+              int k = 0;
+              while (k < ne && !(e[k].r == r && e[k].c == c)) ++k;
+              if (k < ne) {
+                e[k].w += w;
+              } else {
+                e[ne++] = Entry{r, c, w};
+              }
+            }
+          }
+          for (int k = 0; k < ne; ++k) list.push_back(e[k]);
+        }
+  }
+  std::lock_guard<std::mutex> lock(join);
+  lists[next_list++] = std::move(list);
+  });
+  // by row, columns ascending (each pair comes from one interface)
+  size_t total = 0;
+  for (const auto& l : lists) {
+    total += l.size();
+    for (const Entry& e : l) count[(size_t)e.r + 1] += 1;
+  }
+  for (int64_t r = 0; r < m; ++r) count[(size_t)r + 1] += count[(size_t)r];
+  std::vector<Entry> sorted(total);
+  {
+    std::vector<int64_t> at(count.begin(), count.end() - 1);
+    for (auto& l : lists) {
+      for (const Entry& e : l) sorted[(size_t)at[(size_t)e.r]++] = e;
+      std::vector<Entry>().swap(l);
+    }
+  }
+  C.ea.resize(sorted.size());
+  C.eb.resize(sorted.size());
+  C.ew.resize(sorted.size());
+  for (int64_t r = 0; r < m; ++r) {
+    const int64_t b0 = count[(size_t)r], b1e = count[(size_t)r + 1];
+    for (int64_t x = b0 + 1; x < b1e; ++x)
+      for (int64_t y = x; y > b0 && sorted[(size_t)y - 1].c > sorted[(size_t)y].c; --y)
+        std::swap(sorted[(size_t)y - 1], sorted[(size_t)y]);
+    for (int64_t e = b0; e < b1e; ++e) {
+      C.ea[(size_t)e] = sorted[(size_t)e].r;
+      C.eb[(size_t)e] = sorted[(size_t)e].c;
+      C.ew[(size_t)e] = sorted[(size_t)e].w;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells, int64_t host_block_max,
@@ -321,32 +559,75 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
   // Level 0: fluid cells in index order.
   h.node_of_cell.assign((size_t)g.count, -1);
   AggLevel L0;
-  for (int64_t c = 0; c < g.count; ++c)
-    if (g.fluid[(size_t)c]) {
-      h.node_of_cell[(size_t)c] = (int32_t)L0.n;
-      h.cell_of_node.push_back(c);
-      L0.coords.push_back((int32_t)(c % n[0]));
-      L0.coords.push_back((int32_t)(c / n[0] % n[1]));
-      L0.coords.push_back((int32_t)(c / (n[0] * n[1])));
-      ++L0.n;
-    }
-  L0.anchor.assign((size_t)L0.n, 0.0);
+  h.cell_of_node.reserve((size_t)g.fluid_count);
+  L0.coords.reserve(3 * (size_t)g.fluid_count);
+  for (int64_t k = 0, c = 0; k < n[2]; ++k)
+    for (int64_t j = 0; j < n[1]; ++j)
+      for (int64_t i = 0; i < n[0]; ++i, ++c)
+        if (g.fluid[(size_t)c]) {
+          h.node_of_cell[(size_t)c] = (int32_t)L0.n;
+          h.cell_of_node.push_back(c);
+          L0.coords.push_back((int32_t)i);
+          L0.coords.push_back((int32_t)j);
+          L0.coords.push_back((int32_t)k);
+          ++L0.n;
+        }
   const auto& node = h.node_of_cell;
-  for_each_face(
-      g, [&](int64_t lo, int64_t hi, double w) {
-        L0.ea.push_back(node[(size_t)lo]);
-        L0.eb.push_back(node[(size_t)hi]);
-        L0.ew.push_back(w);
-      },
-      [&](int64_t c, double w) { L0.anchor[(size_t)node[(size_t)c]] += w; });
-  h.level0_entries = 2 * (int64_t)L0.ea.size();
-  // Level 0's diagonal: anchors, then the face weights in face order.
-  L0.diag = L0.anchor;
-  for (size_t e = 0; e < L0.ea.size(); ++e) {
-    L0.diag[(size_t)L0.ea[e]] += L0.ew[e];
-    L0.diag[(size_t)L0.eb[e]] += L0.ew[e];
-  }
+  std::vector<int64_t> comp_size(g.anchored.size(), 0);
+  for (int64_t c = 0; c < g.count; ++c)
+    if (g.component[(size_t)c] >= 0) comp_size[(size_t)g.component[(size_t)c]] += 1;
+  // Per node, in face order (x faces, then y, then z; every face of an axis
+  // carries that axis's weight): the anchor sums its outlet sides (a coupled
+  // side on a non-periodic domain face), the diagonal is the anchor plus the
+  // weights of its interior sides; each interior face has two such sides.
+  L0.anchor.assign((size_t)L0.n, 0.0);
+  L0.diag.assign((size_t)L0.n, 0.0);
+  int64_t sides0 = 0;
+  for (int64_t k = 0, c = 0; k < n[2]; ++k)
+    for (int64_t j = 0; j < n[1]; ++j)
+      for (int64_t i = 0; i < n[0]; ++i, ++c) {
+        if (!g.fluid[(size_t)c]) continue;
+        const uint8_t code = g.codes[(size_t)c];
+        const int64_t q[3] = {i, j, k};
+        int inner[3] = {0, 0, 0};
+        double anchor = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int side = 0; side < 2; ++side) {
+            if (!(code & side_bit(a, side))) continue;
+            if (g.periodic[a] || (side == 0 ? q[a] > 0 : q[a] + 1 < n[a]))
+              ++inner[a];
+            else
+              anchor += g.w_out[a];
+          }
+        double d = anchor;
+        for (int a = 0; a < 3; ++a)
+          for (int t = 0; t < inner[a]; ++t) d += g.w_int[a];
+        const int32_t v = node[(size_t)c];
+        L0.anchor[(size_t)v] = anchor;
+        L0.diag[(size_t)v] = d;
+        sides0 += inner[0] + inner[1] + inner[2];
+      }
+  const int64_t faces0 = sides0 / 2;
+  h.level0_entries = 2 * faces0;
   h.levels.push_back(std::move(L0));
+  // level 0's edge list in face order, formed only where it is read: the CSR of
+  // a coarsest level 0 and the general aggregation (bins other than 2x2x2)
+  const auto list_level0 = [&](AggLevel& L) {
+    if (!L.ea.empty() || faces0 == 0) return;
+    L.ea.resize((size_t)faces0);
+    L.eb.resize((size_t)faces0);
+    L.ew.resize((size_t)faces0);
+    size_t e = 0;
+    for_each_face(
+        g,
+        [&](int64_t lo, int64_t hi, double w) {
+          L.ea[e] = node[(size_t)lo];
+          L.eb[e] = node[(size_t)hi];
+          L.ew[e++] = w;
+        },
+        [](int64_t, double) {});
+  };
+  const bool general = std::getenv("CFD_AGG_GENERAL") != nullptr;
   // coupling strength per axis of the current level's matrix: a coarse face sums
   // the fine faces of its bin, so w_a grows by the bin's extent on the other axes
   double w[3] = {g.w_int[0], g.w_int[1], g.w_int[2]};
@@ -358,12 +639,16 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     if (fine) {
       // the grid's own components (the same BFS numbering), in node numbers
       comps.resize(g.anchored.size());
+      for (size_t r = 0; r < comps.size(); ++r) comps[r].reserve((size_t)comp_size[r]);
       for (int64_t c = 0; c < g.count; ++c)
         if (g.component[(size_t)c] >= 0) comps[(size_t)g.component[(size_t)c]].push_back(node[(size_t)c]);
       anch.assign(g.anchored.begin(), g.anchored.end());
       bool small = true;
       for (auto& m : comps) small &= (int64_t)m.size() <= coarse_cells;
-      if (small) csr_level0(L);
+      if (small) {
+        list_level0(L);
+        csr_level0(L);
+      }
     } else {
       csr_from_edges(L);
       level_components(L, comps, anch);
@@ -424,10 +709,22 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     int64_t m = 0;
     int s = 1;
     int shift[3];
-    const auto edges = [&](const std::function<void(int32_t, int32_t)>& f) {
-      for (size_t e = 0; e < L.ea.size(); ++e) f(L.ea[e], L.eb[e]);
+    AggLevel C;
+    bool binned = false;
+    if (fine && coarsen[0] && coarsen[1] && coarsen[2] && !general) {
+      binned = bins_level0(g, node, L, parent, cc, C);
+      if (binned) {
+        m = (int64_t)cc.size() / 3;
+        shift[0] = shift[1] = shift[2] = 1;
+      }
+    }
+    if (!binned && fine) list_level0(L);
+    // the level's edges in order
+    const auto weighted = [&](auto&& f) {
+      for (size_t e = 0; e < L.ea.size(); ++e) f(L.ea[e], L.eb[e], L.ew[e]);
     };
-    for (;;) {
+    const auto edges = [&](auto&& f) { weighted([&](int32_t a, int32_t b, double) { f(a, b); }); };
+    while (!binned) {
       for (int d = 0; d < 3; ++d) shift[d] = coarsen[d] ? s : 0;
       m = aggregate(L.n, L.coords, shift, edges, parent, cc);
       if (m < L.n) break;
@@ -452,14 +749,11 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
       for (int d = 0; d < 3; ++d) wn[d] = w[d] * f[(d + 1) % 3] * f[(d + 2) % 3];
       for (int d = 0; d < 3; ++d) w[d] = wn[d];
     }
-    AggLevel C;
     C.n = m;
     C.coords = std::move(cc);
     C.anchor.assign((size_t)m, 0.0);
     for (int64_t i = 0; i < L.n; ++i) C.anchor[(size_t)parent[(size_t)i]] += L.anchor[(size_t)i];
-    EdgeSums sums(m);
-    for (size_t e = 0; e < L.ea.size(); ++e) sums.add(parent[(size_t)L.ea[e]], parent[(size_t)L.eb[e]], L.ew[e]);
-    sums.emit(C);
+    if (!binned) coarse_edges(m, parent, weighted, C);
     L.parent = std::move(parent);
     // the finer edge list is not needed once the next level exists
     std::vector<int32_t>().swap(L.ea);
@@ -1076,6 +1370,43 @@ void colour_agg_hierarchy(AggHierarchy& h) {
       for (auto& r : h.block_rows) r = perm[(size_t)r];
   }
   build_children(h);
+}
+
+// FNV-1a over the hierarchy's host arrays: a digest of every value and index
+// the device copies, for checks that two builds are the same bit for bit.
+uint64_t agg_hierarchy_digest(const AggHierarchy& h) {
+  uint64_t x = 1469598103934665603ULL;
+  auto bytes = [&](const void* p, size_t len) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < len; ++i) {
+      x ^= b[i];
+      x *= 1099511628211ULL;
+    }
+  };
+  auto vec = [&](const auto& v) {
+    const uint64_t len = v.size();
+    bytes(&len, sizeof len);
+    if (len) bytes(v.data(), v.size() * sizeof(v[0]));
+  };
+  vec(h.node_of_cell);
+  bytes(&h.level0_entries, sizeof h.level0_entries);
+  for (const AggLevel& L : h.levels) {
+    bytes(&L.n, sizeof L.n);
+    vec(L.ptr);
+    vec(L.col);
+    vec(L.val);
+    vec(L.diag);
+    vec(L.anchor);
+    vec(L.coords);
+    vec(L.parent);
+    vec(L.closed_ptr);
+    vec(L.closed_members);
+  }
+  vec(h.block_ptr);
+  vec(h.block_rows);
+  vec(h.block_G);
+  for (const auto& d : h.deferred) vec(d.K);
+  return x;
 }
 
 void agg_hierarchy_stats(const AggHierarchy& h, std::vector<int64_t>& cells, std::vector<int64_t>& nonzeros) {

@@ -153,6 +153,8 @@ struct AggHierarchy {
 // MultigridStats shape of an aggregation or unmasked hierarchy: level dimensions
 // (unmasked only) and the block smoothers (multigrid.rs:610-617, line.rs:200-209).
 void agg_hierarchy_shape(const AggHierarchy& h, cfd_mg_stats* s);
+// FNV-1a digest of the hierarchy's host arrays (indices, values, coarsest blocks).
+uint64_t agg_hierarchy_digest(const AggHierarchy& h);
 // Coarsening stops when every component has at most coarse_cells nodes (the
 // reference: 32); blocks above host_block_max nodes are left to the device.
 // semicoarsen: bin an axis only while its coupling is within kSemiRatio of the

@@ -134,6 +134,9 @@ typedef struct {
   int64_t allocated_bytes;
   int64_t n_block_smoothers;
   cfd_block_smoother_stats block_smoothers[32];
+  /* cfd_case_multigrid_stats only (0 elsewhere): FNV-1a digest of the built
+   * hierarchy's indices and values, equal for two builds that agree bit for bit */
+  uint64_t digest;
 } cfd_mg_stats;
 
 /* Control::check as a callback (solver.rs:122-139): 0 continue, 1 cancelled,
