@@ -246,6 +246,10 @@ struct cfd_grid {
   bool zero_last = true;
   // stencil kernels marching through z on large grids (CFD_MARCH: 0 never, 1 by size, 2 always)
   int march = 1;
+  // the reference and zebra cycles sweep the fine level over tiles and small
+  // coarse levels in one workgroup (CFD_REF_TILES=0: a launch per hyperplane
+  // and per dependency level)
+  bool ref_tiles = true;
   // the red-black cycle in FP64 keeps the PCG's r and z in the fine level's
   // colour-separated layout where it can (CFD_RB_RESIDENT=0: never)
   bool rb_resident = true;
@@ -782,6 +786,8 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
     rc2.bwd_rows = upi(L.bwd_rows);
     rc2.fwd_ptr.assign(L.fwd_ptr.begin(), L.fwd_ptr.end());
     rc2.bwd_ptr.assign(L.bwd_ptr.begin(), L.bwd_ptr.end());
+    rc2.fwd_dptr = upi64(L.fwd_ptr);
+    rc2.bwd_dptr = upi64(L.bwd_ptr);
     if ((rc = lines_of(L, false, rc2.lines))) return rc;
     if (!d.x || !d.b || !rc2.r || !d.diag) return CFD_ENOMEM;
     C.lv.push_back(std::move(rc2));
@@ -821,6 +827,7 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
   D.nsmall = (int)sb.size();
   D.small_block = R.small_block.p;
   if (!g->ref_mean.p && (rc = g->ref_mean.alloc((size_t)std::max(3, g->n_closed)))) return rc;
+  C.tiles = g->ref_tiles;
   C.center0 = [g](const double* src, double* dst) {
     if (g->n_closed == 0) {
       if (src != dst)
@@ -890,6 +897,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_ZERO_LAST")) g->zero_last = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MARCH")) g->march = std::atoi(env);
   if (const char* env = std::getenv("CFD_RB_RESIDENT")) g->rb_resident = std::string(env) != "0";
+  if (const char* env = std::getenv("CFD_REF_TILES")) g->ref_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_SWEEPS")) {
     int f = 2, c = 2;
     if (std::sscanf(env, "%d,%d", &f, &c) == 2 && f >= 1 && f <= 8 && c >= 1 && c <= 8) {

@@ -926,6 +926,9 @@ __global__ void __launch_bounds__(TX * TY * TZ)
   // the tile's last local hyperplane holding a cell of the grid (partial tiles
   // at the grid's far faces walk fewer steps)
   const int S = min(TX, g.n[0] - ox) + min(TY, g.n[1] - oy) + min(TZ, g.n[2] - oz) - 3;
+  // the steps exchange values through LDS only; each cell stores its value once
+  // after the last step (no global store outstanding at the steps' barriers)
+  T v = 0;
   for (int step = 0; step <= S; ++step) {
     const int sp = FWD ? step : S - step;
     if (live && plane == sp) {
@@ -936,12 +939,12 @@ __global__ void __launch_bounds__(TX * TY * TZ)
         const T ym = lix[m] >= 0 ? ys[lix[m]] : ext[m];
         value -= co[m] * ym;
       }
-      const T v = value / dc;
+      v = value / dc;
       ys[li] = v;
-      out[c] = (TOUT)v;
     }
     __syncthreads();
   }
+  if (live) out[c] = (TOUT)v;
 }
 
 // ---------------------------------------------------------------- multigrid (geometric)
@@ -2655,15 +2658,8 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
 // the lower face first, the wrap face first on the last cell of a periodic
 // axis). acc + sign * sum w x_j over the interior faces; a periodic axis of one
 // cell (bit a of self_mask) contributes its self face twice.
-__device__ __forceinline__ double ref_fine_row(const Geo& g, uint8_t code, long long c, const int q[3],
-                                               const double* __restrict__ x, double acc, bool subtract, int self_mask) {
-  double nv[6];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    nv[2 * a] = x[nb_or_self(g, c, a, q[a], 0)];
-    nv[2 * a + 1] = x[nb_or_self(g, c, a, q[a], 1)];
-  }
-  const double xc = x[c];
+__device__ __forceinline__ double ref_fine_acc(const Geo& g, uint8_t code, const int q[3], const double nv[6],
+                                               double xc, double acc, bool subtract, int self_mask) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double w = g.wi[a];
@@ -2685,6 +2681,16 @@ __device__ __forceinline__ double ref_fine_row(const Geo& g, uint8_t code, long 
     }
   }
   return acc;
+}
+__device__ __forceinline__ double ref_fine_row(const Geo& g, uint8_t code, long long c, const int q[3],
+                                               const double* __restrict__ x, double acc, bool subtract, int self_mask) {
+  double nv[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    nv[2 * a] = x[nb_or_self(g, c, a, q[a], 0)];
+    nv[2 * a + 1] = x[nb_or_self(g, c, a, q[a], 1)];
+  }
+  return ref_fine_acc(g, code, q, nv, x[c], acc, subtract, self_mask);
 }
 
 // Gauss-Seidel on hyperplane i + j + k = L: x = (rhs + sum w x_j) / a_ii, which
@@ -2710,6 +2716,77 @@ __global__ void __launch_bounds__(BS) k_ref_fine_gs(Geo g, const uint8_t* __rest
   // a sealed singleton (no couplings, no outlet) has no unknown: it stays zero
   // (masked.rs:161-166)
   x[c] = d != 0.0 ? v / d : 0.0;
+}
+
+// The same sweep over the tiles of k_ic0_tile: one launch per tile diagonal
+// (forward ascending, backward descending), and in a tile one wavefront walks
+// its local hyperplanes with a barrier between them. Every neighbour of a tile
+// lies in the tile or on a neighbouring diagonal, a wrap neighbour of a
+// periodic axis on an earlier diagonal for the sweep that takes it as updated,
+// so a cell finds updated (lower index forward, higher backward) the
+// neighbours the serial sweep has updated: those inside the tile in LDS, the
+// others, and everything not yet updated, as the launch found them. Each cell
+// then repeats ref_fine_row's operations (bitwise equal to k_ref_fine_gs).
+template <bool FWD, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(TX * TY * TZ)
+    k_ref_fine_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ diag,
+                    const double* __restrict__ rhs, double* __restrict__ x, int self_mask, int P, int J0, int K0,
+                    int nJ, int ntx, const PcgState* st) {
+  constexpr int NT = TX * TY * TZ;
+  __shared__ double xs[NT];
+  if (st && st->status != ST_RUNNING) return;
+  const int J = J0 + (int)blockIdx.x % nJ, K = K0 + (int)blockIdx.x / nJ;
+  const int I = P - J - K;
+  if (I < 0 || I >= ntx) return;
+  const int ox = I * TX, oy = J * TY, oz = K * TZ;
+  const int li = threadIdx.x;
+  const int a = li % TX, b = (li / TX) % TY, cz = li / (TX * TY);
+  const int q[3] = {ox + a, oy + b, oz + cz};
+  const bool live = q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2];
+  double nv[6] = {0, 0, 0, 0, 0, 0}, xc = 0.0, rv = 0.0, d = 0.0;
+  int lix[6] = {-1, -1, -1, -1, -1, -1};
+  uint8_t code = 0;
+  long long c = 0;
+  if (live) {
+    c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
+    code = codes[c];
+    if (code & FLUID) {
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          const int m = 2 * ax + side;
+          const long long nb = nb_or_self(g, c, ax, q[ax], side);
+          nv[m] = x[nb];
+          if (!(code & (1u << m)) || !(FWD ? nb < c : nb > c)) continue;
+          const int ni = (int)(nb % g.n[0]), nj = (int)((nb / g.n[0]) % g.n[1]), nk = (int)(nb / g.sz);
+          if (ni >= ox && ni < ox + TX && nj >= oy && nj < oy + TY && nk >= oz && nk < oz + TZ)
+            lix[m] = (ni - ox) + TX * ((nj - oy) + TY * (nk - oz));
+        }
+      xc = x[c];
+      rv = rhs[c];
+      d = diag[c];
+    }
+  }
+  const int plane = a + b + cz;
+  const int S = min(TX, g.n[0] - ox) + min(TY, g.n[1] - oy) + min(TZ, g.n[2] - oz) - 3;
+  // the steps exchange values through LDS only; each cell stores its value once
+  // after the last step (no global store outstanding at the steps' barriers)
+  double v = 0.0;
+  for (int step = 0; step <= S; ++step) {
+    const int sp = FWD ? step : S - step;
+    if (live && plane == sp && (code & FLUID)) {
+      double v6[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v6[m] = lix[m] >= 0 ? xs[lix[m]] : nv[m];
+      const double v0 = ref_fine_acc(g, code, q, v6, xc, rv, false, self_mask);
+      // a sealed singleton (no couplings, no outlet) has no unknown: it stays zero
+      v = d != 0.0 ? v0 / d : 0.0;
+      xs[li] = v;
+    }
+    __syncthreads();
+  }
+  if (live && (code & FLUID)) x[c] = v;
 }
 
 // r = rhs - (a_ii x + sum a_ij x_j) (Matrix::product); zero on solid cells.
@@ -2762,6 +2839,58 @@ __global__ void __launch_bounds__(BS) k_ref_gs(AggDevLevel<double> L, const int*
   for (int e = e0; e < e1; ++e) v -= L.val[e] * L.x[L.col[e]];
   if (status != ST_RUNNING) return;
   L.x[i] = d != 0.0 ? v / d : 0.0;
+}
+
+// A whole sweep of a CSR level inside one workgroup: its dependency levels in
+// order, a barrier between them (the workgroup's stores are visible to its
+// loads after the barrier), each row as in k_ref_gs: b_i - sum a_ij x_j in
+// entry order, divided by a_ii. A row's entries are loaded 16 at a time before
+// they are summed, so the loads of a chain of dependency levels overlap within
+// each level. LDSX: the level's iterate (n <= kRefWgLds) is held in LDS for
+// the sweep and written back at its end. For levels whose launches would each
+// hold a few rows.
+template <bool LDSX>
+__global__ void __launch_bounds__(1024) k_ref_gs_wg(AggDevLevel<double> L, const int* __restrict__ rows,
+                                                    const int* __restrict__ ptr, int nlev, const PcgState* st) {
+  __shared__ double xs[LDSX ? kRefWgLds : 1];
+  if (st && st->status != ST_RUNNING) return;
+  if (LDSX) {
+    for (int i = (int)threadIdx.x; i < L.n; i += (int)blockDim.x) xs[i] = L.x[i];
+    __syncthreads();
+  }
+  for (int dl = 0; dl < nlev; ++dl) {
+    const int b = ptr[dl], e = ptr[dl + 1];
+    for (int t = b + (int)threadIdx.x; t < e; t += (int)blockDim.x) {
+      const int i = rows[t];
+      const int e0 = L.ptr[i], e1 = L.ptr[i + 1];
+      const double dg = L.diag[i];
+      double v = L.b[i];
+      for (int k = e0; k < e1; k += 16) {
+        double av[16], xv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          av[u] = 0.0;
+          xv[u] = 0.0;
+          if (k + u < e1) {
+            const int col = L.col[k + u];
+            av[u] = L.val[k + u];
+            xv[u] = LDSX ? xs[col] : L.x[col];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          if (k + u < e1) v -= av[u] * xv[u];
+      }
+      const double out = dg != 0.0 ? v / dg : 0.0;
+      if (LDSX)
+        xs[i] = out;
+      else
+        L.x[i] = out;
+    }
+    __syncthreads();
+  }
+  if (LDSX)
+    for (int i = (int)threadIdx.x; i < L.n; i += (int)blockDim.x) L.x[i] = xs[i];
 }
 
 __global__ void __launch_bounds__(BS) k_ref_residual(AggDevLevel<double> L, double* __restrict__ r, const PcgState* st) {
@@ -3561,7 +3690,42 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   const Geo& g = R.g0;
   const unsigned ab = blocks_for(g.N);
   const int planes = g.n[0] + g.n[1] + g.n[2] - 2;
+  auto tiled = [&](auto tx, auto ty, auto tz, bool forward) {
+    constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value, TZ = decltype(tz)::value;
+    Geo tg{};
+    tg.n[0] = (g.n[0] + TX - 1) / TX;
+    tg.n[1] = (g.n[1] + TY - 1) / TY;
+    tg.n[2] = (g.n[2] + TZ - 1) / TZ;
+    const int diagonals = tg.n[0] + tg.n[1] + tg.n[2] - 2;
+    for (int k = 0; k < diagonals; ++k) {
+      const int P = forward ? k : diagonals - 1 - k;
+      int j0, k0, nj, nk;
+      level_box(tg, P, j0, k0, nj, nk);
+      if (nj * nk == 0) continue;
+      if (forward)
+        hipLaunchKernelGGL((k_ref_fine_tile<true, TX, TY, TZ>), dim3(nj * nk), dim3(TX * TY * TZ), 0, s, g, R.codes,
+                           R.diag0, (const double*)R.rhs0, z, R.self_mask, P, j0, k0, nj, tg.n[0], st);
+      else
+        hipLaunchKernelGGL((k_ref_fine_tile<false, TX, TY, TZ>), dim3(nj * nk), dim3(TX * TY * TZ), 0, s, g, R.codes,
+                           R.diag0, (const double*)R.rhs0, z, R.self_mask, P, j0, k0, nj, tg.n[0], st);
+    }
+  };
+  using I2 = std::integral_constant<int, 2>;
+  using I8 = std::integral_constant<int, 8>;
+  using I16 = std::integral_constant<int, 16>;
   auto fine_sweep = [&](bool forward) {
+    if (R.tiles) {
+      // tiles as the IC(0) sweeps take them: flat ones across an axis of at most 2 cells
+      if (g.n[1] <= 2)
+        tiled(I16{}, I2{}, I16{}, forward);
+      else if (g.n[2] <= 2)
+        tiled(I16{}, I16{}, I2{}, forward);
+      else if (g.n[0] <= 2)
+        tiled(I2{}, I16{}, I16{}, forward);
+      else
+        tiled(I8{}, I8{}, I8{}, forward);
+      return;
+    }
     for (int k = 0; k < planes; ++k) {
       const int L = forward ? k : planes - 1 - k;
       int j0, k0, nj, nk;
@@ -3581,6 +3745,15 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   auto coarse_sweep = [&](const RefCoarse& C, bool forward) {
     const std::vector<int>& ptr = forward ? C.fwd_ptr : C.bwd_ptr;
     const int* rows = forward ? C.fwd_rows : C.bwd_rows;
+    if (R.tiles && C.L.n <= kRefWgRows && ptr.size() > 2) {
+      if (C.L.n <= kRefWgLds)
+        hipLaunchKernelGGL(k_ref_gs_wg<true>, dim3(1), dim3(1024), 0, s, C.L, rows, forward ? C.fwd_dptr : C.bwd_dptr,
+                           (int)ptr.size() - 1, st);
+      else
+        hipLaunchKernelGGL(k_ref_gs_wg<false>, dim3(1), dim3(1024), 0, s, C.L, rows, forward ? C.fwd_dptr : C.bwd_dptr,
+                           (int)ptr.size() - 1, st);
+      return;
+    }
     for (size_t d = 0; d + 1 < ptr.size(); ++d) {
       const int count = ptr[d + 1] - ptr[d];
       if (count > 0)

@@ -176,7 +176,13 @@ struct RefCoarse {
   const int* fwd_rows;  // sweep schedules: rows of dependency level d at [ptr[d], ptr[d+1])
   const int* bwd_rows;
   std::vector<int> fwd_ptr, bwd_ptr;
+  const int* fwd_dptr;  // device copies of fwd_ptr / bwd_ptr
+  const int* bwd_dptr;
 };
+// levels of at most this many rows sweep inside one workgroup (k_ref_gs_wg),
+// with the iterate in LDS up to kRefWgLds rows
+constexpr int kRefWgRows = 65536;
+constexpr int kRefWgLds = 6144;
 struct RefCycle {
   Geo g0;
   const uint8_t* codes;
@@ -193,6 +199,9 @@ struct RefCycle {
   std::vector<RefCoarse> lv;
   AggCoarsest D;
   std::function<void(const double*, double*)> center0;  // fine-level centering (closed components)
+  // fine sweeps over tiles and small coarse levels in one workgroup per sweep
+  // (false: one launch per hyperplane and per dependency level)
+  bool tiles = true;
 };
 void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgState* st, hipStream_t s);
 

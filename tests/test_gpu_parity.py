@@ -463,6 +463,44 @@ def test_reference_cycle_matches_oracle_vcycle(gpu_available, name, case):
     assert rel(z, zo) <= 1e-12, (name, rel(z, zo))
 
 
+def _tile_cases():
+    out = [(n, c) for n, c in _ref_cases()]
+    out.append(("periodic3", Case(cells=[12, 3, 10], size_m=[1.2, 0.3, 1.0], boundaries=[WALL, PERIODIC, WALL],
+                                  blocks=[Block([0.3, 0.0, 0.2], [0.6, 0.3, 0.5])])))
+    out.append(("odd-periodic", Case(cells=[27, 20, 18], size_m=[2.7, 2.0, 1.8], boundaries=[PERIODIC, PERIODIC, WALL],
+                                     blocks=[Block([0.5, 0.5, 0.0], [1.1, 1.2, 0.9])])))
+    out.append(("urban", W.urban_block(n=(48, 40, 24), boxes=6)[0]))
+    return out
+
+
+@pytest.mark.parametrize("cycle", [cfdgpu.CYCLE_REFERENCE, cfdgpu.CYCLE_ZEBRA])
+@pytest.mark.parametrize("name,case", _tile_cases(), ids=[n for n, _ in _tile_cases()])
+def test_reference_cycle_tiles_are_bitwise(gpu_available, name, case, cycle):
+    """The reference (and zebra) cycle's fine sweeps over 8x8x8 tiles (flat 16x2x16
+    ones across an axis of at most 2 cells), one launch per tile diagonal, and its
+    small coarse levels swept in one workgroup give the bits of one launch per
+    hyperplane and per dependency level: one V-cycle and a whole solve, on
+    periodic axes of 1, 2, 3 and odd length, closed components and the
+    trilinear hierarchy."""
+    case.preconditioner = MULTIGRID
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(31)
+    r = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0)))
+    out = []
+    for tiles in (1, 0):
+        with env(CFD_REF_TILES=tiles, CFD_MG_CYCLE=cycle):
+            g = cfdgpu.Grid(case)
+        g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
+        print(f"[ref tiles] {name} cycle={cycle} tiles={tiles}", flush=True)
+        z = g.precondition(r)
+        x, info = g.solve(b, 1e-11, 3000)
+        out.append((z, x, info["iterations"]))
+    assert np.array_equal(out[0][0], out[1][0]), float(np.max(np.abs(out[0][0] - out[1][0])))
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("name,case", _ref_cases(), ids=[n for n, _ in _ref_cases()])
 def test_reference_cycle_iterations_match_oracle(gpu_available, name, case):
     case.preconditioner = MULTIGRID
