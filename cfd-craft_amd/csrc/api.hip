@@ -980,9 +980,18 @@ bool resident(const cfd_grid* g) {
   return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_FP64 && g->ash.rb64.two_cells >= 1 &&
          g->geo.n[0] % 4 == 0 && g->geo.N < (1LL << 31) && (double)g->geo.N * g->geo.n[0] < 1099511627776.0;
 }
+// The mixed-precision solve on such a grid keeps z in the FP32 x arrays of the
+// fine level (the FP64 residual stays in the natural layout).
+bool resident32(const cfd_grid* g) {
+  return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_MIXED && g->ash.rb32.two_cells >= 1 &&
+         g->geo.n[0] % 4 == 0 && g->geo.N < (1LL << 31);
+}
+// The mode enqueue_precondition takes in a solve (job set-up: first).
+int resident_mode(const cfd_grid* g, bool first) { return resident(g) ? (first ? 1 : 2) : resident32(g) ? 3 : 0; }
 
 // z = M r. resident (see resident()): 0 r and z natural; 1 r natural, copied
-// into the fine level's right-hand side, z left in its x; 2 r already there.
+// into the fine level's right-hand side, z left in its x; 2 r already there;
+// 3 (mixed) r natural, z left in the FP32 x arrays.
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0) {
   if (g->host.preconditioner == CFD_IC0) {
     // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
@@ -1005,9 +1014,9 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st,
     const double* rz_r = rz_fused(g) ? r : nullptr;
     if (g->opt.precision == CFD_PRECISION_MIXED)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
-                                       g->a32.levels.data(), A.D, z, st, g->omega, g->stream,
-                                       A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds, rz_r,
-                                       g->part.p, g->zero_last);
+                                       g->a32.levels.data(), A.D, resident == 3 ? nullptr : z, st, g->omega,
+                                       g->stream, A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
+                                       rz_r, g->part.p, g->zero_last, resident == 3 ? 3 : 0);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
@@ -1047,9 +1056,9 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  const bool res = resident(g);
+  const bool res = resident(g), res32 = resident32(g);
   if (precondition) {
-    enqueue_precondition(g, g->r.p, g->z.p, st, res ? 2 : 0);
+    enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, false));
     enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
     if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
   }
@@ -1058,6 +1067,11 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
     launch_pcg_op(g->geo, g->codes.p, F.x, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream, g->march,
                   F.half);
     launch_pcg_axpy_rb(g->geo, pnew, g->ap.p, g->x.p, F.b, st, g->part.p, g->stream);
+  } else if (res32) {
+    const FineRB<float>& F = g->ash.rb32;
+    launch_pcg_op(g->geo, g->codes.p, nullptr, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream,
+                  g->march, F.half, F.x);
+    launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
   } else {
     launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,
                   g->stream, g->march);
@@ -1207,7 +1221,7 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
   if (t_probe) t_probe->tag = 0;
-  enqueue_precondition(g, g->r.p, g->z.p, st, resident(g) ? 1 : 0);
+  enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, true));
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   if (!rz_fused(g))
     launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, g->mean_z.p, cidx, st, g->part.p, g->stream);

@@ -227,14 +227,14 @@ __device__ __forceinline__ void for_cells(const Geo& g, F f) {
 // of its colour's half, nx even), zhalf entries per colour. Every neighbour of
 // a cell has the other colour (periodic axes have even lengths there), except
 // the cell itself standing in at a closed boundary.
-template <bool HAS_CLOSED, bool FIRST, bool RES = false>
+template <bool HAS_CLOSED, bool FIRST, bool RES = false, class TZ = double>
 __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __restrict__ codes,
-                                               const double* __restrict__ z, const double* __restrict__ pold,
+                                               const TZ* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
                                                double beta, long long zhalf = 0) {
   auto pv = [&](long long i, int colour) {
-    double zc = RES ? z[(colour ? zhalf : 0) + (i >> 1)] : z[i];
+    double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
     // closed components: z centered (without any, the code load is skipped)
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
     if (FIRST) return zc;
@@ -257,9 +257,9 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
 // (registers sized for 8 waves per SIMD: two blocks per CU)
-template <bool HAS_CLOSED, bool RES>
+template <bool HAS_CLOSED, bool RES, class TZ = double>
 __global__ void __launch_bounds__(RB, 8)
-    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ z, const double* __restrict__ pold,
+    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z, const double* __restrict__ pold,
              double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
              const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf) {
   if (!block_running(st)) return;
@@ -267,8 +267,8 @@ __global__ void __launch_bounds__(RB, 8)
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_cells<HAS_CLOSED, true, RES>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf)
-            : pcg_op_cells<HAS_CLOSED, false, RES>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf);
+      first ? pcg_op_cells<HAS_CLOSED, true, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf)
+            : pcg_op_cells<HAS_CLOSED, false, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -293,15 +293,15 @@ inline MarchShape march_shape(const Geo& g) {
   return MarchShape{(g.n[0] + MX - 1) / MX, (g.n[1] + MY - 1) / MY, (g.n[2] + MZ - 1) / MZ};
 }
 
-template <bool HAS_CLOSED, bool FIRST, bool RES>
+template <bool HAS_CLOSED, bool FIRST, bool RES, class TZ>
 __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
-                                               const double* __restrict__ z, const double* __restrict__ pold,
+                                               const TZ* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
                                                double beta, double (*tile)[MY + 2][MX + 2], long long zhalf) {
   // colour: that of cell i (used with RES, z in the colour-separated layout)
   auto pv = [&](long long i, int colour) {
-    double zc = RES ? z[(colour ? zhalf : 0) + (i >> 1)] : z[i];
+    double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
     if (FIRST) return zc;
     return zc + beta * pold[i];
@@ -430,9 +430,9 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   return acc;
 }
 
-template <bool HAS_CLOSED, bool RES>
+template <bool HAS_CLOSED, bool RES, class TZ = double>
 __global__ void __launch_bounds__(RB, 8)
-    k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const double* __restrict__ z,
+    k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
                    const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
                    const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part,
                    long long zhalf) {
@@ -442,8 +442,8 @@ __global__ void __launch_bounds__(RB, 8)
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_march<HAS_CLOSED, true, RES>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf)
-            : pcg_op_march<HAS_CLOSED, false, RES>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf);
+      first ? pcg_op_march<HAS_CLOSED, true, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf)
+            : pcg_op_march<HAS_CLOSED, false, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + blockIdx.x] = s;
@@ -2463,7 +2463,7 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
     acc += r23.y * z23.y;
     acc += r23.x * z23.x;
   }
-  if (!RES) {
+  if (!RES && out64) {
     *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
     *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
   }
@@ -3162,11 +3162,23 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 }
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
-                   int march, long long zhalf) {
+                   int march, long long zhalf, const float* z32) {
   const MarchShape ms = march_shape(g);
+  const bool marching = march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP);
+  if (z32) {
+    // z of the mixed-precision cycle, FP32 in the colour-separated layout
+    const double bytes32 = 29.0 * (double)g.N;  // z 4, p_old 8, code 1 in; p_new, Ap 16 out
+    if (marching)
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op_march<false, true, float>), dim3(RP), dim3(RB), 0, s, g, ms,
+                    codes, z32, pold, pnew, ap, mean, cidx, st, part, zhalf);
+    else
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op<false, true, float>), dim3(RP), dim3(RB), 0, s, g, codes, z32,
+                    pold, pnew, ap, mean, cidx, st, part, zhalf);
+    return;
+  }
   const double bytes = 33.0 * (double)g.N;  // z, p_old, code in; p_new, Ap out
   // marching pays once the grid gives every block of the fixed grid an item
-  if (march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP)) {
+  if (marching) {
     if (closed)
       PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op_march<true, false>), dim3(RP), dim3(RB), 0, s, g, ms, codes, z,
                     pold, pnew, ap, mean, cidx, st, part, 0LL);
@@ -3469,7 +3481,8 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 // lv). The correction of each level is scaled by omega. resident (colour-
 // separated fine level with two cells per thread): 1 b0 copied into the fine
 // level's b and z left in its x, the last half-sweep forming r . z from them;
-// 2 the same with b already there (b0 unused).
+// 2 the same with b already there (b0 unused); 3 b0 (FP64) rounded into the
+// level's b as without, z left in its x, r . z formed from b0.
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
@@ -3591,8 +3604,15 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         // half-sweep, both colours' z out in FP64 and r in (with r and z kept in
         // the layout: the black b read besides)
         const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) +
-                                (resident ? (double)rb->half * sizeof(T) : (double)g0.N * 16);
-        if (resident && rb->two_cells == 2)
+                                (resident == 3 ? (double)g0.N * 8
+                                               : resident ? (double)rb->half * sizeof(T) : (double)g0.N * 16);
+        if (resident == 3 && rb->two_cells == 2)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb,
+                        (double*)nullptr, rz_r, st, rz_part);
+        else if (resident == 3)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb,
+                        (double*)nullptr, rz_r, st, rz_part);
+        else if (resident && rb->two_cells == 2)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, true>), dim3(RP), dim3(RB), 0, s, g0, *rb,
                         (double*)nullptr, (const double*)nullptr, st, rz_part);
         else if (resident)

@@ -80,10 +80,11 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 // march: 0 cell by cell; 1 marching through z once the grid has an item per
 // block of the fixed grid; 2 always marching
 // zhalf > 0: z in the fine red-black level's colour-separated layout (cell c at
-// c / 2 of its colour's half of zhalf entries; nx even, no closed component)
+// c / 2 of its colour's half of zhalf entries; nx even, no closed component),
+// FP64 in z or, with z32, FP32 in z32
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
-                   int march = 1, long long zhalf = 0);
+                   int march = 1, long long zhalf = 0, const float* z32 = nullptr);
 void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, double* r, PcgState* st, double* part,
                      hipStream_t s);
 // the same with r in the colour-separated layout (nx even, N nx < 2^40)

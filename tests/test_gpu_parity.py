@@ -738,17 +738,19 @@ def _with_wind(case, direction):
     return case
 
 
+@pytest.mark.parametrize("precision", [cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED])
 @pytest.mark.parametrize("name,make", RESIDENT_CASES, ids=[n for n, _ in RESIDENT_CASES])
-def test_resident_residual_solves_bitwise(gpu_available, name, make):
+def test_resident_residual_solves_bitwise(gpu_available, name, make, precision):
     """With two cells per thread and no closed component, the FP64 red-black
     solve keeps the PCG's r and z in the fine level's colour-separated arrays
     (the residual update scatters into them, the operator reads z from them,
     the first half-sweep starts from them and the last one forms r . z there)
-    instead of natural-layout vectors. Every value and every partial sum keeps
-    its arithmetic and order: the field, the iteration count and the residuals
-    equal those of the natural-layout solve bit for bit, with the marching and
-    the cell-by-cell operator kernels, pairs in tiles or linear, periodic axes
-    included, and on a warm-started second solve."""
+    instead of natural-layout vectors; the mixed-precision solve keeps z in the
+    FP32 arrays (its residual stays FP64 in the natural layout). Every value and
+    every partial sum keeps its arithmetic and order: the field, the iteration
+    count and the residuals equal those of the natural-layout solve bit for bit,
+    with the marching and the cell-by-cell operator kernels, pairs in tiles or
+    linear, periodic axes included, and on a warm-started second solve."""
     case = make()
     o = oracle.Grid(case)
     rng = np.random.default_rng(21)
@@ -760,17 +762,27 @@ def test_resident_residual_solves_bitwise(gpu_available, name, make):
         for resident in (1, 0):
             with env(CFD_MG_RB_PAIRS=1, CFD_MG_RB_LIN=lin, CFD_MARCH=march, CFD_RB_RESIDENT=resident):
                 g = cfdgpu.Grid(case)
-            print(f"[resident] {name} march={march} linear={lin} resident={resident}", flush=True)
-            # the path taken: the first half-sweep of a cycle reads r in the
-            # natural layout (21 bytes a cell) only in the set-up's cycle when r
-            # is kept colour by colour (8.5 bytes a cell in the others)
+            g.set_options(precision)
+            print(f"[resident] {name} precision={precision} march={march} linear={lin} resident={resident}",
+                  flush=True)
             d_b, d_x = cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray(case.count)
-            first = g.profile_solve(d_b, d_x, 1e-11, "fine_first_half_sweep")
-            n = first["launches"]
-            want = (21.0 + 8.5 * (n - 1)) if resident else 21.0 * n
-            assert abs(first["bytes"] / case.count - want) < 1e-6 * n, (first, want)
+            if precision == cfdgpu.PRECISION_FP64:
+                # the path taken: the first half-sweep of a cycle reads r in the
+                # natural layout (21 bytes a cell) only in the set-up's cycle when
+                # r is kept colour by colour (8.5 bytes a cell in the others)
+                first = g.profile_solve(d_b, d_x, 1e-11, "fine_first_half_sweep")
+                n = first["launches"]
+                want = (21.0 + 8.5 * (n - 1)) if resident else 21.0 * n
+                assert abs(first["bytes"] / case.count - want) < 1e-6 * n, (first, want)
+            else:
+                # the last half-sweep writes z in FP64 (8 bytes a cell) only
+                # when z is not kept in the FP32 arrays
+                last = g.profile_solve(d_b, d_x, 1e-11, "fine_rz_half_sweep")
+                n = last["launches"]
+                want = (14.5 if resident else 22.5) * n
+                assert abs(last["bytes"] / case.count - want) < 1e-6 * n, (last, want)
             x, info = g.solve(b, 1e-11, 3000)
-            g.set_options(cfdgpu.PRECISION_FP64, warm_start=True)
+            g.set_options(precision, warm_start=True)
             xw, infow = g.solve(b2, 1e-11, 3000)
             out.append((x, info, xw, infow))
         (x1, i1, w1, j1), (x0, i0, w0, j0) = out
