@@ -185,6 +185,7 @@ struct cfd_grid {
   DevBuf<double> G;
   bool has_geo = false;  // geometric hierarchy built (CFD_CYCLE_GEOMETRIC)
   DevBuf<float> x0f;     // FP32 fine iterate of the mixed-precision cycles
+  DevBuf<float> z32;     // z of the mixed-precision red-black cycle, FP32 values in the natural layout
   DevBuf<float> ic0_y32;  // FP32 forward result of the mixed-precision IC(0) application
   // aggregation hierarchy (CFD_CYCLE_RB)
   AggHierarchy agg;
@@ -950,7 +951,7 @@ int setup_device_inner(cfd_grid* g) {
     if ((rc = g->ic0_y32.alloc(N))) return rc;
   } else {
     g->has_hier = true;
-    if ((rc = g->x0f.alloc(N))) return rc;
+    if ((rc = g->x0f.alloc(N)) || (rc = g->z32.alloc(N))) return rc;
     int cycle = CFD_CYCLE_RB;
     if (const char* env = std::getenv("CFD_MG_CYCLE")) cycle = std::atoi(env);
     if ((rc = ensure_cycle(g, cycle))) return rc;
@@ -980,18 +981,19 @@ bool resident(const cfd_grid* g) {
   return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_FP64 && g->ash.rb64.two_cells >= 1 &&
          g->geo.n[0] % 4 == 0 && g->geo.N < (1LL << 31) && (double)g->geo.N * g->geo.n[0] < 1099511627776.0;
 }
-// The mixed-precision solve on such a grid keeps z in the FP32 x arrays of the
-// fine level (the FP64 residual stays in the natural layout).
+// The mixed-precision solve on such a grid keeps z as the FP32 values it is
+// (written once in the natural layout, read so by the operator) instead of an
+// FP64 copy.
 bool resident32(const cfd_grid* g) {
   return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_MIXED && g->ash.rb32.two_cells >= 1 &&
-         g->geo.n[0] % 4 == 0 && g->geo.N < (1LL << 31);
+         g->geo.n[0] % 4 == 0 && g->z32.p;
 }
 // The mode enqueue_precondition takes in a solve (job set-up: first).
 int resident_mode(const cfd_grid* g, bool first) { return resident(g) ? (first ? 1 : 2) : resident32(g) ? 3 : 0; }
 
 // z = M r. resident (see resident()): 0 r and z natural; 1 r natural, copied
 // into the fine level's right-hand side, z left in its x; 2 r already there;
-// 3 (mixed) r natural, z left in the FP32 x arrays.
+// 3 (mixed) r natural, z written as FP32 to z32.
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0) {
   if (g->host.preconditioner == CFD_IC0) {
     // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
@@ -1016,7 +1018,8 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st,
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
                                        g->a32.levels.data(), A.D, resident == 3 ? nullptr : z, st, g->omega,
                                        g->stream, A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
-                                       rz_r, g->part.p, g->zero_last, resident == 3 ? 3 : 0);
+                                       rz_r, g->part.p, g->zero_last, resident == 3 ? 3 : 0,
+                                       resident == 3 ? g->z32.p : nullptr);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
@@ -1068,9 +1071,8 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
                   F.half);
     launch_pcg_axpy_rb(g->geo, pnew, g->ap.p, g->x.p, F.b, st, g->part.p, g->stream);
   } else if (res32) {
-    const FineRB<float>& F = g->ash.rb32;
     launch_pcg_op(g->geo, g->codes.p, nullptr, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream,
-                  g->march, F.half, F.x);
+                  g->march, 0, g->z32.p);
     launch_pcg_axpy(g->geo.N, pnew, g->ap.p, g->x.p, g->r.p, st, g->part.p, g->stream);
   } else {
     launch_pcg_op(g->geo, g->codes.p, g->z.p, pold, pnew, g->ap.p, mz, cidx, g->n_closed > 0, st, g->part.p,

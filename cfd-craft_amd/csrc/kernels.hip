@@ -2420,7 +2420,8 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz(Geo g, FineRB<T> F, double* 
 // natural layout).
 template <class T, bool RES>
 __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
-                                        const double* __restrict__ r, int i2, int j, int k, unsigned u, double& acc) {
+                                        float* __restrict__ out32, const double* __restrict__ r, int i2, int j, int k,
+                                        unsigned u, double& acc) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   const long long row = (long long)j * g.sy + k * g.sz;
   const int par = (j + k) & 1;  // red at 2 i2 + par and 2 i2 + 2 + par
@@ -2466,6 +2467,14 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
   if (!RES && out64) {
     *reinterpret_cast<double2*>(out64 + row + 2 * i2) = z01;
     *reinterpret_cast<double2*>(out64 + row + 2 * i2 + 2) = z23;
+  } else if (!RES && out32) {
+    // z of an FP32 cycle: every value is an FP32 value, stored exactly
+    float4 zf;
+    zf.x = (float)z01.x;
+    zf.y = (float)z01.y;
+    zf.z = (float)z23.x;
+    zf.w = (float)z23.y;
+    *reinterpret_cast<float4*>(out32 + row + 2 * i2) = zf;
   }
 }
 
@@ -2474,7 +2483,8 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
 // multiple of 4).
 template <class T, bool RES>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double* __restrict__ out64,
-                                                  const double* __restrict__ r, const PcgState* st, double* part) {
+                                                  float* __restrict__ out32, const double* __restrict__ r,
+                                                  const PcgState* st, double* part) {
   if (!block_running(st)) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned gx = (hx + 127) >> 7, gy = (ny + 15) >> 4, tiles = gx * gy * (unsigned)g.n[2];
@@ -2487,7 +2497,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
     const int k = (int)(tile / (gx * gy));
     if (i2 >= (int)hx || j >= (int)ny) continue;
     const unsigned u = (unsigned)i2 + hx * ((unsigned)j + ny * (unsigned)k);
-    rz_pair<T, RES>(g, F, out64, r, i2, j, k, u, acc);
+    rz_pair<T, RES>(g, F, out64, out32, r, i2, j, k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -2498,7 +2508,8 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
 // row is not a multiple of the tile width).
 template <class T, bool RES>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
-                                                      const double* __restrict__ r, const PcgState* st, double* part) {
+                                                      float* __restrict__ out32, const double* __restrict__ r,
+                                                      const PcgState* st, double* part) {
   if (!block_running(st)) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned pairs = (unsigned)(F.half / 2);
@@ -2511,7 +2522,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, dou
     const unsigned row = div_magic(u, F.mhx);
     const int i2 = (int)(u - row * hx);
     const unsigned k = div_magic(row, F.mny);
-    rz_pair<T, RES>(g, F, out64, r, i2, (int)(row - k * ny), (int)k, u, acc);
+    rz_pair<T, RES>(g, F, out64, out32, r, i2, (int)(row - k * ny), (int)k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
@@ -3166,14 +3177,14 @@ void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const do
   const MarchShape ms = march_shape(g);
   const bool marching = march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP);
   if (z32) {
-    // z of the mixed-precision cycle, FP32 in the colour-separated layout
+    // z of the mixed-precision cycle, its FP32 values in the natural layout
     const double bytes32 = 29.0 * (double)g.N;  // z 4, p_old 8, code 1 in; p_new, Ap 16 out
     if (marching)
-      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op_march<false, true, float>), dim3(RP), dim3(RB), 0, s, g, ms,
-                    codes, z32, pold, pnew, ap, mean, cidx, st, part, zhalf);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op_march<false, false, float>), dim3(RP), dim3(RB), 0, s, g, ms,
+                    codes, z32, pold, pnew, ap, mean, cidx, st, part, 0LL);
     else
-      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op<false, true, float>), dim3(RP), dim3(RB), 0, s, g, codes, z32,
-                    pold, pnew, ap, mean, cidx, st, part, zhalf);
+      PROBED_LAUNCH(PROBE_OPERATOR, bytes32, (k_pcg_op<false, false, float>), dim3(RP), dim3(RB), 0, s, g, codes,
+                    z32, pold, pnew, ap, mean, cidx, st, part, 0LL);
     return;
   }
   const double bytes = 33.0 * (double)g.N;  // z, p_old, code in; p_new, Ap out
@@ -3481,13 +3492,14 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 // lv). The correction of each level is scaled by omega. resident (colour-
 // separated fine level with two cells per thread): 1 b0 copied into the fine
 // level's b and z left in its x, the last half-sweep forming r . z from them;
-// 2 the same with b already there (b0 unused); 3 b0 (FP64) rounded into the
-// level's b as without, z left in its x, r . z formed from b0.
+// 2 the same with b already there (b0 unused); 3 (an FP32 level) b0 rounded into
+// the level's b as without, r . z formed from b0 and z written in FP32 (the
+// values of the level's x) to out32 in the natural layout.
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds,
-                       const double* rz_r, double* rz_part, bool zero_last, int resident) {
+                       const double* rz_r, double* rz_part, bool zero_last, int resident, float* out32) {
   // a round is one sweep over every colour class; pre-smoothing runs the classes in
   // ascending order, post-smoothing in descending order (the adjoint)
   const int fr = fine_rounds, cr = coarse_rounds;
@@ -3604,26 +3616,26 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         // half-sweep, both colours' z out in FP64 and r in (with r and z kept in
         // the layout: the black b read besides)
         const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) +
-                                (resident == 3 ? (double)g0.N * 8
+                                (resident == 3 ? (double)g0.N * 12
                                                : resident ? (double)rb->half * sizeof(T) : (double)g0.N * 16);
         if (resident == 3 && rb->two_cells == 2)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb,
-                        (double*)nullptr, rz_r, st, rz_part);
+                        (double*)nullptr, out32, rz_r, st, rz_part);
         else if (resident == 3)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb,
-                        (double*)nullptr, rz_r, st, rz_part);
+                        (double*)nullptr, out32, rz_r, st, rz_part);
         else if (resident && rb->two_cells == 2)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, true>), dim3(RP), dim3(RB), 0, s, g0, *rb,
-                        (double*)nullptr, (const double*)nullptr, st, rz_part);
+                        (double*)nullptr, (float*)nullptr, (const double*)nullptr, st, rz_part);
         else if (resident)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, true>), dim3(RP), dim3(RB), 0, s, g0, *rb,
-                        (double*)nullptr, (const double*)nullptr, st, rz_part);
+                        (double*)nullptr, (float*)nullptr, (const double*)nullptr, st, rz_part);
         else if (rb->two_cells == 2 && g0.n[0] % 4 == 0)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64,
-                        rz_r, st, rz_part);
+                        (float*)nullptr, rz_r, st, rz_part);
         else if (rb->two_cells && g0.n[0] % 4 == 0)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64,
-                        rz_r, st, rz_part);
+                        (float*)nullptr, rz_r, st, rz_part);
         else
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz<T>), dim3(RP), dim3(RB), 0, s, g0, *rb, out64, rz_r, st,
                         rz_part);
@@ -3659,11 +3671,11 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
                                                 const PcgState*, double, hipStream_t, const FineRB<double>*, int, int,
-                                                const double*, double*, bool, int);
+                                                const double*, double*, bool, int, float*);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
                                                double, hipStream_t, const FineRB<float>*, int, int, const double*,
-                                               double*, bool, int);
+                                               double*, bool, int, float*);
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {

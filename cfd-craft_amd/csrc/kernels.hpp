@@ -80,8 +80,8 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 // march: 0 cell by cell; 1 marching through z once the grid has an item per
 // block of the fixed grid; 2 always marching
 // zhalf > 0: z in the fine red-black level's colour-separated layout (cell c at
-// c / 2 of its colour's half of zhalf entries; nx even, no closed component),
-// FP64 in z or, with z32, FP32 in z32
+// c / 2 of its colour's half of zhalf entries; nx even, no closed component);
+// z32: z as FP32 values in the natural layout (no closed component)
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
                    int march = 1, long long zhalf = 0, const float* z32 = nullptr);
@@ -133,7 +133,7 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb = nullptr, int fine_rounds = 2,
                        int coarse_rounds = 2, const double* rz_r = nullptr, double* rz_part = nullptr,
-                       bool zero_last = true, int resident = 0);
+                       bool zero_last = true, int resident = 0, float* out32 = nullptr);
 // One colour-separated fine half-sweep (timing hook).
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
