@@ -350,9 +350,12 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
     const int cpar = (i + j) & 1;
     // software pipeline: the values of the next PD layers (the column's p, its
     // code and the rim slot's p) are in flight while the current layer is
-    // worked on. p of layer L = k0 - 1 .. k1 is the column's z neighbour at the
-    // ends (nb_or_self), the column's own value in between.
-    constexpr int PD = 3;
+    // worked on (one layer ahead: with eight waves per SIMD the other waves hide
+    // the latency, and deeper pipelines spill registers: 185 against 191 and
+    // 215 us a launch for 1, 2 and 3 layers on the 384x384x160 canyon). p of
+    // layer L = k0 - 1 .. k1 is the column's z neighbour at the ends
+    // (nb_or_self), the column's own value in between.
+    constexpr int PD = 1;
     auto p_at = [&](int L) {
       if (L < k0) {
         const long long c0 = col + (long long)k0 * g.sz, n = nb_or_self(g, c0, 2, k0, 0);
