@@ -167,6 +167,7 @@ struct cfd_grid {
   DevBuf<uint8_t> codes;
   DevBuf<int> cidx;
   DevBuf<double> D;
+  DevBuf<double> ic0_fill;  // factor entries with their dot terms (a periodic axis of 3 cells), else empty
   int n_closed = 0;
   double inv_count1 = 0.0;
   DevBuf<int> members, chunk_begin, comp_first_chunk;
@@ -948,6 +949,8 @@ int setup_device_inner(cfd_grid* g) {
   if (h.preconditioner == CFD_IC0) {
     build_ic0_diagonal(h);
     if ((rc = g->D.upload(h.ic0_diag.data(), N))) return rc;
+    if (!h.ic0_fill_coef.empty() && (rc = g->ic0_fill.upload(h.ic0_fill_coef.data(), h.ic0_fill_coef.size())))
+      return rc;
     if ((rc = g->ic0_y32.alloc(N))) return rc;
   } else {
     g->has_hier = true;
@@ -997,12 +1000,13 @@ int resident_mode(const cfd_grid* g, bool first) { return resident(g) ? (first ?
 void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0) {
   if (g->host.preconditioner == CFD_IC0) {
     // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
+    const double* fill = g->ic0_fill.p;
     if (g->opt.precision == CFD_PRECISION_MIXED)
-      launch_ic0_tiles_mixed(g->geo, g->codes.p, g->D.p, r, g->ic0_y32.p, z, st, g->stream);
+      launch_ic0_tiles_mixed(g->geo, g->codes.p, g->D.p, r, g->ic0_y32.p, z, st, g->stream, fill);
     else if (g->ic0_tiles)
-      launch_ic0_tiles(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+      launch_ic0_tiles(g->geo, g->codes.p, g->D.p, r, z, st, g->stream, fill);
     else
-      launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream);
+      launch_ic0(g->geo, g->codes.p, g->D.p, r, z, st, g->stream, fill);
     return;
   }
   if (g->opt.cycle == CFD_CYCLE_REFERENCE || g->opt.cycle == CFD_CYCLE_ZEBRA) {
@@ -1425,14 +1429,6 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
   const char* env = std::getenv("CFD_GPU_SETUP");
   int rc = env && std::string(env) == "0" ? build_host_grid(*c, g->host) : build_device_grid(*c, g->host);
   if (rc) return rc;
-  if (c->preconditioner == CFD_IC0) {
-    bool fill = false;
-    for (int a = 0; a < 3; ++a) fill |= g->host.periodic[a] && g->host.n[a] == 3;
-    if (fill) {
-      set_error("ic0: a 3-cell periodic axis gives the IC(0) factor fill coupling, which the device sweeps do not carry");
-      return CFD_EUNSUPPORTED;
-    }
-  }
   rc = setup_device(g.get());
   if (rc) return rc;
   *out = g.release();
@@ -1448,14 +1444,6 @@ int cfd_grid_create_desc(const cfd_grid_desc* d, cfd_grid** out) {
   std::unique_ptr<cfd_grid> g(new cfd_grid());
   int rc = grid_from_desc(*d, g->host, true);
   if (rc) return rc;
-  if (d->preconditioner == CFD_IC0) {
-    bool fill = false;
-    for (int a = 0; a < 3; ++a) fill |= g->host.periodic[a] && g->host.n[a] == 3;
-    if (fill) {
-      set_error("ic0: a 3-cell periodic axis gives the IC(0) factor fill coupling, which the device sweeps do not carry");
-      return CFD_EUNSUPPORTED;
-    }
-  }
   rc = setup_device(g.get());
   if (rc) return rc;
   *out = g.release();

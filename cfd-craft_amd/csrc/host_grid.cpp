@@ -528,6 +528,88 @@ void build_ic0_diagonal(HostGrid& g) {
     }
     D[(size_t)c0] = std::sqrt(std::fmax(raw[(size_t)c0] + 1e-6 * maxdiag - squared, 1e-12 * maxdiag));
   }
+  if (g.ic0_fill) build_ic0_fill(g, raw, maxdiag);
+}
+
+// The factor of a grid with a periodic axis of exactly 3 cells, where two lower
+// neighbours of a cell can be joined themselves (the three cells of the axis)
+// and an entry takes the dot term of Factor::build (pressure.rs:197-209): the
+// lower entries of each row (merged per column in face order, sorted by
+// column), each l = (a - sum_k l_row,k l_col,k) / D[col] over the row's earlier
+// columns k that the column's row also holds, then the diagonal as above. Also
+// the entries the device sweeps gather, per cell in their slot order (per axis
+// z, y, x: the wrap neighbour, then the neighbour on the axis): 6 forward
+// (l of the cell's row at its lower neighbours) and 6 backward (l of its upper
+// neighbours' rows at the cell).
+void build_ic0_fill(HostGrid& g, const std::vector<double>& raw, double maxdiag) {
+  const int64_t* n = g.n;
+  const int64_t total = g.count;
+  const int64_t stride[3] = {1, n[0], n[0] * n[1]};
+  struct Entry {
+    int64_t col;
+    double l;
+  };
+  std::vector<std::vector<Entry>> rows((size_t)total);
+  // lower neighbours in ascending index order (z wrap, z lo, y wrap, y lo, x wrap, x lo)
+  auto lower_of = [&](int64_t c0, uint8_t code, const int64_t q[3], auto f) {
+    for (int a = 2; a >= 0; --a) {
+      if (!g.periodic[a] || n[a] == 1) {
+        if (q[a] > 0 && (code & side_bit(a, 0))) f(c0 - stride[a], -g.w_int[a], a, 1);
+        continue;
+      }
+      if (n[a] == 2) {
+        if (q[a] == 1 && (code & side_bit(a, 0))) f(c0 - stride[a], -g.w_int[a] - g.w_int[a], a, 1);
+        continue;
+      }
+      if (q[a] == n[a] - 1 && (code & side_bit(a, 1))) f(c0 - (n[a] - 1) * stride[a], -g.w_int[a], a, 0);
+      if (q[a] > 0 && (code & side_bit(a, 0))) f(c0 - stride[a], -g.w_int[a], a, 1);
+    }
+  };
+  double* D = g.ic0_diag.data();
+  for (int64_t c0 = 0; c0 < total; ++c0) {
+    const uint8_t code = g.codes[(size_t)c0];
+    const int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+    auto& row = rows[(size_t)c0];
+    lower_of(c0, code, q, [&](int64_t col, double a, int, int) { row.push_back(Entry{col, a}); });
+    for (size_t e = 0; e < row.size(); ++e) {
+      const auto& other = rows[(size_t)row[e].col];
+      double dot = 0.0;
+      size_t o = 0;
+      for (size_t p = 0; p < e; ++p) {
+        while (o < other.size() && other[o].col < row[p].col) ++o;
+        if (o < other.size() && other[o].col == row[p].col) dot += row[p].l * other[o].l;
+      }
+      row[e].l = (row[e].l - dot) / D[(size_t)row[e].col];
+    }
+    double squared = -0.0;
+    for (const Entry& e : row) squared += e.l * e.l;
+    D[(size_t)c0] = std::sqrt(std::fmax(raw[(size_t)c0] + 1e-6 * maxdiag - squared, 1e-12 * maxdiag));
+  }
+  auto entry = [&](int64_t r, int64_t col) {
+    for (const Entry& e : rows[(size_t)r])
+      if (e.col == col) return e.l;
+    return 0.0;
+  };
+  g.ic0_fill_coef.assign((size_t)total * 12, 0.0);
+  for (int64_t c0 = 0; c0 < total; ++c0) {
+    const uint8_t code = g.codes[(size_t)c0];
+    const int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
+    double* fwd = &g.ic0_fill_coef[(size_t)c0 * 12];
+    double* bwd = fwd + 6;
+    lower_of(c0, code, q, [&](int64_t col, double, int a, int slot) { fwd[2 * (2 - a) + slot] = entry(c0, col); });
+    // upper neighbours: the wrap neighbour, then the neighbour on the axis
+    for (int a = 2; a >= 0; --a) {
+      const int m = 2 * (2 - a);
+      if (!g.periodic[a] || n[a] == 1) {
+        if (q[a] < n[a] - 1 && (code & side_bit(a, 1))) bwd[m + 1] = entry(c0 + stride[a], c0);
+      } else if (n[a] == 2) {
+        if (q[a] == 0 && (code & side_bit(a, 1))) bwd[m + 1] = entry(c0 + stride[a], c0);
+      } else {
+        if (q[a] == 0 && (code & side_bit(a, 0))) bwd[m] = entry(c0 + (n[a] - 1) * stride[a], c0);
+        if (q[a] < n[a] - 1 && (code & side_bit(a, 1))) bwd[m + 1] = entry(c0 + stride[a], c0);
+      }
+    }
+  }
 }
 
 namespace {

@@ -44,6 +44,9 @@ struct HostGrid {
   // topology whose factor has fill coupling (a periodic axis of 3 cells).
   std::vector<double> ic0_diag;
   bool ic0_fill = false;
+  // with fill coupling: per cell 6 forward and 6 backward factor entries in the
+  // device sweeps' slot order (empty otherwise)
+  std::vector<double> ic0_fill_coef;
 };
 
 // Builds the grid; returns a CFD error code (0 on success) and sets the error text.
@@ -66,6 +69,8 @@ void find_components(HostGrid& g);
 // vary within an axis and face kind (else those weights become NaN).
 int grid_from_desc(const cfd_grid_desc& d, HostGrid& g, bool strict);
 void build_ic0_diagonal(HostGrid& g);
+// The factor's dot terms of a grid with fill coupling (called by build_ic0_diagonal).
+void build_ic0_fill(HostGrid& g, const std::vector<double>& raw, double maxdiag);
 
 // Geometric hierarchy: 2x2x2 bins of the structured grid with Galerkin
 // piecewise-constant coarse operators (7-point with per-face weights).

@@ -800,7 +800,8 @@ __device__ __forceinline__ Ic0Nbrs ic0_nbrs(const Geo& g, uint8_t code, long lon
 // before the arithmetic.
 __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
                                                 const double* __restrict__ r, double* __restrict__ y, int L, int j0,
-                                                int k0, int nj, int nk, const PcgState* st) {
+                                                int k0, int nj, int nk, const PcgState* st,
+                                                const double* __restrict__ fill) {
   const int status = st ? st->status : ST_RUNNING;
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= nj * nk) return;
@@ -823,7 +824,7 @@ __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict
 #pragma unroll
   for (int m = 0; m < 6; ++m) {
     if (N.a[m] == 0.0) continue;
-    double l = N.a[m] / dn[m];
+    double l = fill ? fill[12 * c + m] : N.a[m] / dn[m];
     value -= l * yn[m];
   }
   y[c] = value / dc;
@@ -835,7 +836,7 @@ __global__ void __launch_bounds__(BS) k_ic0_fwd(Geo g, const uint8_t* __restrict
 // the division by D[c]; l = a / D[c].
 __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D,
                                                 double* __restrict__ z, int L, int j0, int k0, int nj, int nk,
-                                                const PcgState* st) {
+                                                const PcgState* st, const double* __restrict__ fill) {
   const int status = st ? st->status : ST_RUNNING;
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= nj * nk) return;
@@ -855,7 +856,7 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 #pragma unroll
   for (int m = 0; m < 6; ++m) {
     if (N.a[m] == 0.0) continue;
-    double l = N.a[m] / dc;
+    double l = fill ? fill[12 * c + 6 + m] : N.a[m] / dc;
     value -= l * zn[m];
   }
   z[c] = value / dc;
@@ -883,7 +884,7 @@ __global__ void __launch_bounds__(BS) k_ic0_bwd(Geo g, const uint8_t* __restrict
 template <bool FWD, int TX, int TY, int TZ, class T, class TOWN, class TOUT>
 __global__ void __launch_bounds__(TX * TY * TZ)
     k_ic0_tile(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ D, const TOWN* own, TOUT* out,
-               int P, int J0, int K0, int nJ, int nK, int ntx, const PcgState* st) {
+               int P, int J0, int K0, int nJ, int nK, int ntx, const PcgState* st, const double* __restrict__ fill) {
   constexpr int NT = TX * TY * TZ;
   __shared__ T ys[NT];
   if (st && st->status != ST_RUNNING) return;
@@ -914,7 +915,7 @@ __global__ void __launch_bounds__(TX * TY * TZ)
     for (int m = 0; m < 6; ++m) {
       if (N.a[m] == 0.0) continue;
       const unsigned nb = (unsigned)N.idx[m];
-      co[m] = (T)(FWD ? N.a[m] / D[nb] : N.a[m] / d64);
+      co[m] = (T)(fill ? fill[12 * c + (FWD ? 0 : 6) + m] : FWD ? N.a[m] / D[nb] : N.a[m] / d64);
       const unsigned ni = nb % (unsigned)g.n[0], nj = (nb / (unsigned)g.n[0]) % (unsigned)g.n[1];
       const unsigned nk = nb / (unsigned)g.sz;
       if ((int)ni >= ox && (int)ni < ox + TX && (int)nj >= oy && (int)nj < oy + TY && (int)nk >= oz && (int)nk < oz + TZ) {
@@ -3291,7 +3292,7 @@ static void level_box(const Geo& g, int L, int& j0, int& k0, int& nj, int& nk) {
 // y: the forward result (z itself in FP64; an FP32 buffer for the mixed sweeps).
 template <int TX, int TY, int TZ, class T>
 void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D, const double* r, T* y, double* z,
-                             const PcgState* st, hipStream_t s) {
+                             const PcgState* st, hipStream_t s, const double* fill) {
   constexpr int NT = TX * TY * TZ;
   Geo tg{};
   tg.n[0] = (g.n[0] + TX - 1) / TX;
@@ -3303,60 +3304,60 @@ void launch_ic0_tiles_shaped(const Geo& g, const uint8_t* codes, const double* D
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
     hipLaunchKernelGGL((k_ic0_tile<true, TX, TY, TZ, T, double, T>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D, r, y,
-                       P, j0, k0, nj, nk, tg.n[0], st);
+                       P, j0, k0, nj, nk, tg.n[0], st, fill);
   }
   for (int P = diagonals - 1; P >= 0; --P) {
     int j0, k0, nj, nk;
     level_box(tg, P, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
     hipLaunchKernelGGL((k_ic0_tile<false, TX, TY, TZ, T, T, double>), dim3(nj * nk), dim3(NT), 0, s, g, codes, D,
-                       (const T*)y, z, P, j0, k0, nj, nk, tg.n[0], st);
+                       (const T*)y, z, P, j0, k0, nj, nk, tg.n[0], st, fill);
   }
 }
 
 template <class T>
 void launch_ic0_tiles_t(const Geo& g, const uint8_t* codes, const double* D, const double* r, T* y, double* z,
-                        const PcgState* st, hipStream_t s) {
+                        const PcgState* st, hipStream_t s, const double* fill) {
   probe_begin(PROBE_IC0, s);
   // an axis of at most 2 cells: flat 16 x 16 tiles across the other two axes
   if (g.n[1] <= 2)
-    launch_ic0_tiles_shaped<16, 2, 16, T>(g, codes, D, r, y, z, st, s);
+    launch_ic0_tiles_shaped<16, 2, 16, T>(g, codes, D, r, y, z, st, s, fill);
   else if (g.n[2] <= 2)
-    launch_ic0_tiles_shaped<16, 16, 2, T>(g, codes, D, r, y, z, st, s);
+    launch_ic0_tiles_shaped<16, 16, 2, T>(g, codes, D, r, y, z, st, s, fill);
   else if (g.n[0] <= 2)
-    launch_ic0_tiles_shaped<2, 16, 16, T>(g, codes, D, r, y, z, st, s);
+    launch_ic0_tiles_shaped<2, 16, 16, T>(g, codes, D, r, y, z, st, s, fill);
   else
-    launch_ic0_tiles_shaped<8, 8, 8, T>(g, codes, D, r, y, z, st, s);
+    launch_ic0_tiles_shaped<8, 8, 8, T>(g, codes, D, r, y, z, st, s, fill);
   // forward: r 8, D 8, code 1 in, y out; backward: y, D 8, code 1 in, z 8 out
   probe_end(PROBE_IC0, s, (double)g.N * (8 + 8 + 1 + sizeof(T) + sizeof(T) + 8 + 1 + 8));
 }
 
 void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
-                      const PcgState* st, hipStream_t s) {
-  launch_ic0_tiles_t<double>(g, codes, D, r, z, z, st, s);
+                      const PcgState* st, hipStream_t s, const double* fill) {
+  launch_ic0_tiles_t<double>(g, codes, D, r, z, z, st, s, fill);
 }
 
 void launch_ic0_tiles_mixed(const Geo& g, const uint8_t* codes, const double* D, const double* r, float* y, double* z,
-                            const PcgState* st, hipStream_t s) {
-  launch_ic0_tiles_t<float>(g, codes, D, r, y, z, st, s);
+                            const PcgState* st, hipStream_t s, const double* fill) {
+  launch_ic0_tiles_t<float>(g, codes, D, r, y, z, st, s, fill);
 }
 
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,
-                hipStream_t s) {
+                hipStream_t s, const double* fill) {
   const int levels = g.n[0] + g.n[1] + g.n[2] - 2;
   for (int L = 0; L < levels; ++L) {
     int j0, k0, nj, nk;
     level_box(g, L, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
     hipLaunchKernelGGL(k_ic0_fwd, dim3(blocks_for((long long)nj * nk)), dim3(BS), 0, s, g, codes, D, r, z, L, j0, k0, nj,
-                       nk, st);
+                       nk, st, fill);
   }
   for (int L = levels - 1; L >= 0; --L) {
     int j0, k0, nj, nk;
     level_box(g, L, j0, k0, nj, nk);
     if (nj * nk == 0) continue;
     hipLaunchKernelGGL(k_ic0_bwd, dim3(blocks_for((long long)nj * nk)), dim3(BS), 0, s, g, codes, D, z, L, j0, k0, nj, nk,
-                       st);
+                       st, fill);
   }
 }
 

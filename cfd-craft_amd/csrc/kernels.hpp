@@ -109,15 +109,15 @@ void launch_center(long long N, const uint8_t* codes, const double* src, const d
 void launch_fill(long long N, double* dst, double v, hipStream_t s);
 void launch_add(long long N, double* p, const double* q, hipStream_t s);
 void launch_ic0(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z, const PcgState* st,
-                hipStream_t s);
+                hipStream_t s, const double* fill = nullptr);
 // The same sweeps as a tiled wavefront (one launch per tile diagonal).
 void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const double* r, double* z,
-                      const PcgState* st, hipStream_t s);
+                      const PcgState* st, hipStream_t s, const double* fill = nullptr);
 // The same sweeps in FP32 (the mixed-precision preconditioner): the factor
 // entries formed in FP64 and rounded, the forward result held in y (FP32), z
 // written in FP64.
 void launch_ic0_tiles_mixed(const Geo& g, const uint8_t* codes, const double* D, const double* r, float* y, double* z,
-                            const PcgState* st, hipStream_t s);
+                            const PcgState* st, hipStream_t s, const double* fill = nullptr);
 // One V-cycle: fine level from the codes, coarse levels lv[0..nlev) (device copy dlv),
 // levels from tail_first on inside one workgroup.
 template <class T, class TB>

@@ -91,6 +91,38 @@ def test_ic0_tiles_equal_hyperplanes(gpu_available, name, case):
     assert np.array_equal(out[0], out[1])
 
 
+def _fill_cases():
+    from test_host_topology import fill_cases
+
+    return fill_cases()
+
+
+@pytest.mark.parametrize("name,case", _fill_cases(), ids=[n for n, _ in _fill_cases()])
+def test_ic0_with_fill_coupling(gpu_available, name, case):
+    """A periodic axis of exactly 3 cells: the IC(0) entries with their dot terms
+    (pressure.rs:197-209) come from the host; the application is bitwise the
+    oracle's (tiled and per hyperplane), FP64 PCG takes the oracle's iterations
+    and field, and the FP32 application (mixed) reaches the same field."""
+    case.preconditioner = IC0
+    o = oracle.Grid(case)
+    rng = np.random.default_rng(17)
+    r = rng.standard_normal(o.count)
+    for tiles in (1, 0):
+        with env(CFD_IC0_TILES=tiles):
+            g = cfdgpu.Grid(case)
+        print(f"[ic0 fill] {name} tiles={tiles}", flush=True)
+        assert np.array_equal(g.precondition(r), o.ic0_apply(r)), (name, tiles)
+    ex = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
+    b = o.apply(ex)
+    xo, io = o.solve(b, 1e-13, 5000)
+    x, info = g.solve(b, 1e-13, 5000)
+    assert abs(info["iterations"] - io["iterations"]) <= 1, (info, io)
+    assert rel(x, xo) <= 1e-10
+    g.set_options(cfdgpu.PRECISION_MIXED)
+    xm, im = g.solve(b, 1e-13, 5000)
+    assert rel(xm, xo) <= 1e-10, (im, io)
+
+
 def _manufactured(case):
     o = oracle.Grid(case)
     ex = W.manufactured(case, o.fluid())

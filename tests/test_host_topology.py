@@ -71,6 +71,32 @@ def test_topology_matches_oracle(name, case):
     assert np.array_equal(t["ic0_diagonal"].view(np.uint64), o.ic0_diagonal().view(np.uint64))
 
 
+def fill_cases():
+    """Grids with a periodic axis of exactly 3 cells, the topology where two lower
+    neighbours of a cell are joined themselves and IC(0) entries take the dot
+    term of Factor::build (pressure.rs:197-209)."""
+    return [
+        ("fill-x", Case(cells=[3, 8, 6], size_m=[0.3, 0.8, 0.6], boundaries=[PERIODIC, WALL, WALL])),
+        ("fill-y", Case(cells=[7, 3, 5], size_m=[0.7, 0.3, 0.5], boundaries=[OPEN, PERIODIC, WALL],
+                        velocity_m_s=[1.0, 0.0, 0.0])),
+        ("fill-z", Case(cells=[6, 5, 3], size_m=[0.6, 0.5, 0.3], boundaries=[WALL, WALL, PERIODIC],
+                        blocks=[Block([0.2, 0.2, 0.0], [0.4, 0.3, 0.1])])),
+        ("fill-xyz", Case(cells=[3, 3, 3], size_m=[0.3, 0.3, 0.3], boundaries=[PERIODIC, PERIODIC, PERIODIC])),
+        ("fill-3-2", Case(cells=[3, 2, 9], size_m=[0.3, 0.2, 0.9], boundaries=[PERIODIC, PERIODIC, OPEN],
+                          velocity_m_s=[0.0, 0.0, 1.0])),
+        ("fill-block", Case(cells=[9, 3, 8], size_m=[0.9, 0.3, 0.8], boundaries=[OPEN, PERIODIC, WALL],
+                            velocity_m_s=[1.0, 0.0, 0.0], blocks=[Block([0.3, 0.0, 0.2], [0.5, 0.1, 0.5])])),
+    ]
+
+
+@pytest.mark.parametrize("name,case", fill_cases(), ids=[n for n, _ in fill_cases()])
+def test_ic0_fill_diagonal_matches_oracle(name, case):
+    """The host factor with the dot terms gives the oracle's IC(0) diagonal bit for bit."""
+    o = oracle.Grid(case)
+    t = cfdgpu.case_topology(case, ic0=True)
+    assert np.array_equal(t["ic0_diagonal"].view(np.uint64), o.ic0_diagonal().view(np.uint64))
+
+
 @pytest.mark.parametrize("name,case", cases(), ids=[n for n, _ in cases()])
 def test_numpy_workload_helpers_match_oracle(name, case):
     o = oracle.Grid(case)
