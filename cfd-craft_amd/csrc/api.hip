@@ -252,6 +252,9 @@ struct cfd_grid {
   // coarse levels in one workgroup (CFD_REF_TILES=0: a launch per hyperplane
   // and per dependency level)
   bool ref_tiles = true;
+  // ... and sweep coarse levels of at most 65536 rows from sweep-ordered copies
+  // of their rows (CFD_REF_PACKED=0: from the CSR by dependency level)
+  bool ref_packed = true;
   // the red-black cycle in FP64 keeps the PCG's r and z in the fine level's
   // colour-separated layout where it can (CFD_RB_RESIDENT=0: never)
   bool rb_resident = true;
@@ -678,6 +681,67 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
     if (R.dbls.back()->upload(v.data(), v.size())) return nullptr;
     return R.dbls.back()->p;
   };
+  // the sweep of a coarse level in one workgroup (k_ref_gs_pk): its rows in the
+  // sweep's order, each with its entries, in sub-levels of at most kRefSubRows
+  // rows and `cap` entries; the iterate in LDS when it fits beside a product
+  // buffer of 2048 entries or more. Not built (nsub 0) for a row longer than cap.
+  int lds_max = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->device));
+  auto pack_sweep = [&](const AggLevel& L, const std::vector<int64_t>& ptr, const std::vector<int32_t>& order,
+                        double* bs, RefSweep& P) -> int {
+    P = RefSweep();
+    const int64_t n = L.n, words = lds_max / (int64_t)sizeof(double);
+    const bool ldsx = n + 2048 <= words;
+    const int64_t cap = std::min<int64_t>((int64_t)kRefSubEpt * kRefSubRows, ldsx ? words - n : words);
+    std::vector<int4> meta, rinfo((size_t)n);
+    std::vector<int32_t> col;
+    std::vector<double> val, dsw((size_t)n);
+    col.reserve(L.col.size());
+    val.reserve(L.val.size());
+    int64_t most = 0;
+    for (size_t d = 0; d + 1 < ptr.size(); ++d) {
+      int4 m = make_int4((int)ptr[d], (int)ptr[d], (int)col.size(), (int)col.size());
+      auto close = [&]() {
+        if (m.y == m.x) return;
+        meta.push_back(m);
+        most = std::max<int64_t>(most, m.w - m.z);
+      };
+      for (int64_t t = ptr[d]; t < ptr[d + 1]; ++t) {
+        const int32_t i = order[(size_t)t];
+        const int64_t e0 = L.ptr[(size_t)i], e1 = L.ptr[(size_t)i + 1];
+        if (e1 - e0 > cap) return 0;
+        if (m.y - m.x == kRefSubRows || (m.w - m.z) + (e1 - e0) > cap) {
+          close();
+          m = make_int4((int)t, (int)t, (int)col.size(), (int)col.size());
+        }
+        rinfo[(size_t)t] = make_int4(i, m.w - m.z, m.w - m.z + (int)(e1 - e0), 0);
+        dsw[(size_t)t] = L.diag[(size_t)i];
+        for (int64_t e = e0; e < e1; ++e) {
+          col.push_back(L.col[(size_t)e]);
+          val.push_back(L.val[(size_t)e]);
+        }
+        m.y += 1;
+        m.w += (int)(e1 - e0);
+      }
+      close();
+    }
+    if (col.empty()) col.push_back(0), val.push_back(0.0);  // a level without couplings
+    R.ints.emplace_back(new DevBuf<int>());
+    if (R.ints.back()->upload((const int*)meta.data(), meta.size() * 4)) return CFD_ENOMEM;
+    P.meta = (const int4*)R.ints.back()->p;
+    R.ints.emplace_back(new DevBuf<int>());
+    if (R.ints.back()->upload((const int*)rinfo.data(), rinfo.size() * 4)) return CFD_ENOMEM;
+    P.rinfo = (const int4*)R.ints.back()->p;
+    P.dsw = upd(dsw);
+    P.col = upi(col);
+    P.val = upd(val);
+    P.bs = bs;
+    if (!P.dsw || !P.col || !P.val) return CFD_ENOMEM;
+    P.nsub = (int)meta.size();
+    P.cap = (int)std::max<int64_t>(most, 1);
+    P.ldsx = ldsx;
+    return 0;
+  };
   const int64_t N = hg.count;
   RefCycle& C = R.cyc;
   C.g0 = g->geo;
@@ -790,6 +854,13 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
     rc2.bwd_ptr.assign(L.bwd_ptr.begin(), L.bwd_ptr.end());
     rc2.fwd_dptr = upi64(L.fwd_ptr);
     rc2.bwd_dptr = upi64(L.bwd_ptr);
+    if (g->ref_packed && L.n <= kRefWgRows && L.fwd_ptr.size() > 2) {
+      double* bs = upd(zero);
+      if (!bs) return CFD_ENOMEM;
+      if ((rc = pack_sweep(L, L.fwd_ptr, L.fwd_rows, bs, rc2.fwd_pk)) ||
+          (rc = pack_sweep(L, L.bwd_ptr, L.bwd_rows, bs, rc2.bwd_pk)))
+        return rc;
+    }
     if ((rc = lines_of(L, false, rc2.lines))) return rc;
     if (!d.x || !d.b || !rc2.r || !d.diag) return CFD_ENOMEM;
     C.lv.push_back(std::move(rc2));
@@ -830,6 +901,7 @@ int build_ref(cfd_grid* g, DevRef& R, bool zebra) {
   D.small_block = R.small_block.p;
   if (!g->ref_mean.p && (rc = g->ref_mean.alloc((size_t)std::max(3, g->n_closed)))) return rc;
   C.tiles = g->ref_tiles;
+  C.packed = g->ref_packed;
   C.center0 = [g](const double* src, double* dst) {
     if (g->n_closed == 0) {
       if (src != dst)
@@ -900,6 +972,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MARCH")) g->march = std::atoi(env);
   if (const char* env = std::getenv("CFD_RB_RESIDENT")) g->rb_resident = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_REF_TILES")) g->ref_tiles = std::string(env) != "0";
+  if (const char* env = std::getenv("CFD_REF_PACKED")) g->ref_packed = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_SWEEPS")) {
     int f = 2, c = 2;
     if (std::sscanf(env, "%d,%d", &f, &c) == 2 && f >= 1 && f <= 8 && c >= 1 && c <= 8) {

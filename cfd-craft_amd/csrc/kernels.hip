@@ -2908,6 +2908,98 @@ __global__ void __launch_bounds__(1024) k_ref_gs_wg(AggDevLevel<double> L, const
     for (int i = (int)threadIdx.x; i < L.n; i += (int)blockDim.x) L.x[i] = xs[i];
 }
 
+// A uniform int4 loaded through the scalar cache (the constant address space).
+__device__ __forceinline__ int4 scalar_load(const int4* p, int i) {
+  const __attribute__((address_space(4))) int* q = (const __attribute__((address_space(4))) int*)p;
+  return make_int4(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
+}
+
+// The operands of one sub-level a thread holds: its entries (column -1: none)
+// and, when it has one, its row with the row's entry range, rhs and diagonal.
+struct RefSubOps {
+  int col[kRefSubEpt];
+  double val[kRefSubEpt];
+  int4 ri;
+  double b, d;
+};
+
+// A whole sweep of a CSR level inside one workgroup of NT threads from its
+// sweep-ordered copy (RefSweep), a sub-level at a time: the workgroup's
+// threads form the products a_ij x_j of every entry of the sub-level at once
+// (each rounded, as the serial loop rounds it before subtracting) into LDS;
+// after a barrier one thread a row subtracts its products from b_i in entry
+// order and divides by a_ii, the serial row's arithmetic in its order, so the
+// sweep is bitwise k_ref_gs's. Every load of a sub-level's operands is
+// addressed from its meta entry alone (no load waits for another) and is
+// issued while the previous sub-level is worked. LDSX: the iterate is held in
+// LDS (the barriers wait for LDS only); else it stays in global memory.
+template <bool LDSX, int NT>
+__global__ void __launch_bounds__(NT) k_ref_gs_pk(AggDevLevel<double> L, RefSweep S, const PcgState* st) {
+  extern __shared__ double shm[];
+  if (st && st->status != ST_RUNNING) return;
+  double* xs = shm;
+  double* prod = shm + (LDSX ? L.n : 0);
+  const int tid = (int)threadIdx.x;
+  // the right-hand side in the sweep's order (read by this workgroup only)
+  for (int r = tid; r < L.n; r += NT) S.bs[r] = L.b[S.rinfo[r].x];
+  if (LDSX)
+    for (int i = tid; i < L.n; i += NT) xs[i] = L.x[i];
+  __syncthreads();
+  auto sync = [&]() {
+    if (LDSX)
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else
+      __syncthreads();
+  };
+  auto fetch = [&](const int4 m, RefSubOps& o) {
+    const int r = m.x + tid;
+    const bool has = r < m.y;
+    o.ri = has ? S.rinfo[r] : make_int4(-1, 0, 0, 0);
+    o.b = has ? S.bs[r] : 0.0;
+    o.d = has ? S.dsw[r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kRefSubEpt; ++u) {
+      const int e = m.z + tid + u * NT;
+      o.col[u] = e < m.w ? S.col[e] : -1;
+      o.val[u] = e < m.w ? S.val[e] : 0.0;
+    }
+  };
+  RefSubOps cur;
+  fetch(scalar_load(S.meta, 0), cur);
+  int4 mnext = S.nsub > 1 ? scalar_load(S.meta, 1) : make_int4(0, 0, 0, 0);
+  for (int s = 0; s < S.nsub; ++s) {
+    RefSubOps nxt;
+    if (s + 1 < S.nsub) fetch(mnext, nxt);
+    const int4 mafter = s + 2 < S.nsub ? scalar_load(S.meta, s + 2) : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < kRefSubEpt; ++u)
+      if (cur.col[u] >= 0) prod[tid + u * NT] = cur.val[u] * (LDSX ? xs[cur.col[u]] : L.x[cur.col[u]]);
+    sync();
+    if (cur.ri.x >= 0) {
+      double v = cur.b;
+      int k = cur.ri.y;
+      for (; k + 16 <= cur.ri.z; k += 16) {
+        double p[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) p[u] = prod[k + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v -= p[u];
+      }
+      for (; k < cur.ri.z; ++k) v -= prod[k];
+      const double out = cur.d != 0.0 ? v / cur.d : 0.0;
+      if (LDSX)
+        xs[cur.ri.x] = out;
+      else
+        L.x[cur.ri.x] = out;
+    }
+    sync();
+    cur = nxt;
+    mnext = mafter;
+  }
+  if (LDSX)
+    for (int i = tid; i < L.n; i += NT) L.x[i] = xs[i];
+}
+
 __global__ void __launch_bounds__(BS) k_ref_residual(AggDevLevel<double> L, double* __restrict__ r, const PcgState* st) {
   const int status = st ? st->status : ST_RUNNING;
   const int i = (int)(blockIdx.x * BS + threadIdx.x);
@@ -3781,6 +3873,15 @@ void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgS
   auto coarse_sweep = [&](const RefCoarse& C, bool forward) {
     const std::vector<int>& ptr = forward ? C.fwd_ptr : C.bwd_ptr;
     const int* rows = forward ? C.fwd_rows : C.bwd_rows;
+    const RefSweep& P = forward ? C.fwd_pk : C.bwd_pk;
+    if (R.tiles && R.packed && P.nsub > 0) {
+      const size_t shm = ((size_t)(P.ldsx ? C.L.n : 0) + (size_t)P.cap) * sizeof(double);
+      if (P.ldsx)
+        hipLaunchKernelGGL((k_ref_gs_pk<true, kRefSubRows>), dim3(1), dim3(kRefSubRows), shm, s, C.L, P, st);
+      else
+        hipLaunchKernelGGL((k_ref_gs_pk<false, kRefSubRows>), dim3(1), dim3(kRefSubRows), shm, s, C.L, P, st);
+      return;
+    }
     if (R.tiles && C.L.n <= kRefWgRows && ptr.size() > 2) {
       if (C.L.n <= kRefWgLds)
         hipLaunchKernelGGL(k_ref_gs_wg<true>, dim3(1), dim3(1024), 0, s, C.L, rows, forward ? C.fwd_dptr : C.bwd_dptr,

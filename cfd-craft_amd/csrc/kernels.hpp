@@ -155,6 +155,27 @@ struct RefLines {
   const int* bwd = nullptr;
   std::vector<int> fwd_ptr, bwd_ptr;
 };
+// One direction of a CSR level's sweep inside one workgroup with the rows in
+// the sweep's order and their off-diagonal entries copied in that order.
+// Sub-level s (one dependency level or a part of one: at most kRefSubRows rows
+// and `cap` entries) is meta[s] = {first position, end position, first entry,
+// end entry}; position r of the order is rinfo[r] = {row, first entry, end
+// entry (both counted from the sub-level's first), 0} with the row's diagonal
+// at dsw[r]; bs holds the right-hand side in the order (written by the sweep).
+struct RefSweep {
+  const int4* meta = nullptr;
+  const int4* rinfo = nullptr;
+  const double* dsw = nullptr;
+  const int* col = nullptr;
+  const double* val = nullptr;
+  double* bs = nullptr;
+  int nsub = 0;
+  int cap = 0;        // entries of the largest sub-level (the products' LDS buffer)
+  bool ldsx = false;  // the level's iterate held in LDS for the sweep
+};
+constexpr int kRefSubRows = 1024;  // the workgroup: one thread a row of a sub-level
+constexpr int kRefSubEpt = 8;      // entries a thread of a sub-level: cap <= 8 * 1024
+
 struct RefCoarse {
   AggDevLevel<double> L;  // n, CSR, b, x, parent (aggregation) and children (restriction into this level)
   RefLines lines;         // anisotropic levels: block smoother instead of point sweeps
@@ -179,6 +200,7 @@ struct RefCoarse {
   std::vector<int> fwd_ptr, bwd_ptr;
   const int* fwd_dptr;  // device copies of fwd_ptr / bwd_ptr
   const int* bwd_dptr;
+  RefSweep fwd_pk, bwd_pk;  // both sweeps in one workgroup (nsub == 0: not built)
 };
 // levels of at most this many rows sweep inside one workgroup (k_ref_gs_wg),
 // with the iterate in LDS up to kRefWgLds rows
@@ -203,6 +225,8 @@ struct RefCycle {
   // fine sweeps over tiles and small coarse levels in one workgroup per sweep
   // (false: one launch per hyperplane and per dependency level)
   bool tiles = true;
+  // coarse sweeps from the sweep-ordered copies where built (k_ref_gs_pk)
+  bool packed = true;
 };
 void launch_ref_vcycle(const RefCycle& R, const double* r, double* z, const PcgState* st, hipStream_t s);
 

@@ -502,6 +502,9 @@ def _tile_cases():
     out.append(("odd-periodic", Case(cells=[27, 20, 18], size_m=[2.7, 2.0, 1.8], boundaries=[PERIODIC, PERIODIC, WALL],
                                      blocks=[Block([0.5, 0.5, 0.0], [1.1, 1.2, 0.9])])))
     out.append(("urban", W.urban_block(n=(48, 40, 24), boxes=6)[0]))
+    # trilinear levels of 4096 rows (iterate in LDS) and 32768 rows (in global memory)
+    out.append(("cube32", W.cavity([32, 32, 32], MULTIGRID)))
+    out.append(("cube64", W.cavity([64, 64, 64], MULTIGRID)))
     return out
 
 
@@ -510,7 +513,8 @@ def _tile_cases():
 def test_reference_cycle_tiles_are_bitwise(gpu_available, name, case, cycle):
     """The reference (and zebra) cycle's fine sweeps over 8x8x8 tiles (flat 16x2x16
     ones across an axis of at most 2 cells), one launch per tile diagonal, and its
-    small coarse levels swept in one workgroup give the bits of one launch per
+    coarse levels of at most 65536 rows swept in one workgroup (from sweep-ordered
+    copies of their rows, or from the CSR) give the bits of one launch per
     hyperplane and per dependency level: one V-cycle and a whole solve, on
     periodic axes of 1, 2, 3 and odd length, closed components and the
     trilinear hierarchy."""
@@ -520,17 +524,18 @@ def test_reference_cycle_tiles_are_bitwise(gpu_available, name, case, cycle):
     r = o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0))
     b = o.apply(o.center_nullspaces(np.where(o.fluid(), rng.standard_normal(o.count), 0.0)))
     out = []
-    for tiles in (1, 0):
-        with env(CFD_REF_TILES=tiles, CFD_MG_CYCLE=cycle):
+    for tiles, packed in ((1, 1), (1, 0), (0, 0)):
+        with env(CFD_REF_TILES=tiles, CFD_REF_PACKED=packed, CFD_MG_CYCLE=cycle):
             g = cfdgpu.Grid(case)
         g.set_options(cfdgpu.PRECISION_FP64, cycle=cycle)
-        print(f"[ref tiles] {name} cycle={cycle} tiles={tiles}", flush=True)
+        print(f"[ref tiles] {name} cycle={cycle} tiles={tiles} packed={packed}", flush=True)
         z = g.precondition(r)
         x, info = g.solve(b, 1e-11, 3000)
         out.append((z, x, info["iterations"]))
-    assert np.array_equal(out[0][0], out[1][0]), float(np.max(np.abs(out[0][0] - out[1][0])))
-    assert out[0][2] == out[1][2]
-    assert np.array_equal(out[0][1], out[1][1])
+    for k in (0, 1):
+        assert np.array_equal(out[k][0], out[2][0]), (k, float(np.max(np.abs(out[k][0] - out[2][0]))))
+        assert out[k][2] == out[2][2]
+        assert np.array_equal(out[k][1], out[2][1])
 
 
 @pytest.mark.parametrize("name,case", _ref_cases(), ids=[n for n, _ in _ref_cases()])
