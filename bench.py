@@ -412,74 +412,89 @@ def lid_cavity(steps=5):
     return out
 
 
-def served(requests=512, n_grids=16, max_active=8, seed=4):
+def served(requests=512, n_grids=16, lanes=2, seed=4):
     """Served mix: resident grids of 64^3 .. 192x192x96 cells (seeded boxes, 5-25 %
     solid), tolerances 1e-6/1e-8/1e-10. The same request list runs (1) plain:
-    one FP64 solve after another on the caller's thread, (2) served: FP64 and
-    mixed precision through the continuous-batching server, closed loop (all
-    requests queued at once: capacity) and (3) open loop: Poisson arrivals at
-    80 % of the served mixed capacity, for per-request latency p50/p99 measured
+    one FP64 solve after another on the caller's thread, on grids built alone
+    (cfd_grid_create); (2) served, batched: FP64 and mixed precision through the
+    continuous-batching server, `lanes` concurrent lanes of up to 8 solves per
+    precision (one launch per iteration stage for a lane's batch), closed loop
+    (all requests queued at once: capacity); (3) the same server without
+    batching (every solve on its grid's own stream); (4) open loop: Poisson
+    arrivals at 80 % of the batched mixed capacity, per-request latency p50/p99
     from arrival to completion. Grid set-up is excluded (resident grids)."""
     import cfdgpu
     from cfdgpu import workloads as W
 
     pool = [W.served_request(seed, k) for k in range(n_grids)]
-    server = cfdgpu.Server(max_active=max_active)
-    grids = [server.admit(c) for c, _, _, _ in pool]
-    bufs = [(cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray.from_host(t), cfdgpu.DeviceArray(g.count))
-            for (_, b, t, _), g in zip(pool, grids)]
     rng = np.random.default_rng(seed)
     reqs = [(int(rng.integers(0, n_grids)), W.SERVED_TOLERANCES[int(rng.integers(0, 3))]) for _ in range(requests)]
-    cells = sum(grids[k].count for k, _ in reqs)
-    for prec in (cfdgpu.PRECISION_FP64, cfdgpu.PRECISION_MIXED):
-        for k, g in enumerate(grids):
-            g.set_options(prec)
-            g.solve_device(bufs[k][0], bufs[k][2], 1e-8, 3000, d_scale=bufs[k][1])
-    cfdgpu.synchronize()
-    out = {"requests": requests, "grids": n_grids, "max_active": max_active,
+    plain = [cfdgpu.Grid(c) for c, _, _, _ in pool]
+    bufs = [(cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray.from_host(t), cfdgpu.DeviceArray(g.count))
+            for (_, b, t, _), g in zip(pool, plain)]
+    cells = sum(plain[k].count for k, _ in reqs)
+    out = {"requests": requests, "grids": n_grids, "lanes": lanes, "batch_max": 8,
            "shapes": [list(s) for s in W.SERVED_SHAPES], "tolerances": W.SERVED_TOLERANCES,
            "mean_cells": round(cells / requests)}
-    for g in grids:
+    for k, g in enumerate(plain):
         g.set_options(cfdgpu.PRECISION_FP64)
+        g.solve_device(bufs[k][0], bufs[k][2], 1e-8, 3000, d_scale=bufs[k][1])
+    cfdgpu.synchronize()
     lat = []
     t0 = time.perf_counter()
     for k, tol in reqs:
         s0 = time.perf_counter()
-        grids[k].solve_device(bufs[k][0], bufs[k][2], tol, 3000, d_scale=bufs[k][1])
+        plain[k].solve_device(bufs[k][0], bufs[k][2], tol, 3000, d_scale=bufs[k][1])
         lat.append(time.perf_counter() - s0)
     el = time.perf_counter() - t0
     out["plain_fp64"] = {"solves_per_s": round(requests / el, 1), "p50_ms": round(1e3 * pct(lat, 0.5), 3),
                          "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
-    cap = {}
-    for name, prec in (("served_fp64", cfdgpu.PRECISION_FP64), ("served_mixed", cfdgpu.PRECISION_MIXED)):
+    del plain
+
+    def closed_loop(server, grids, prec):
+        tks = [server.submit(grids[k], bufs[k][0], bufs[k][2], 1e-8, 3000, prec, d_scale=bufs[k][1])
+               for k in range(n_grids)]
+        server.drain()
+        for tk in tks:
+            server.result(tk)
         t0 = time.perf_counter()
         tks = [server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, prec, d_scale=bufs[k][1]) for k, tol in reqs]
         server.drain()
         el = time.perf_counter() - t0
-        for tk in tks:
-            server.result(tk)
-        cap[name] = requests / el
-        out[name] = {"solves_per_s": round(cap[name], 1)}
-    lam = 0.8 * cap["served_mixed"]
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
-    tks = []
-    t0 = time.perf_counter()
-    i = 0
-    while i < requests:
-        now = time.perf_counter() - t0
-        while i < requests and arrivals[i] <= now:
-            k, tol = reqs[i]
-            tks.append(server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, cfdgpu.PRECISION_MIXED,
-                                     d_scale=bufs[k][1]))
-            i += 1
-        server.progress()
-    server.drain()
-    el = time.perf_counter() - t0
-    lat = [server.result(tk)[1] for tk in tks]
-    out["open_loop_mixed"] = {"lambda_per_s": round(lam, 1), "achieved_per_s": round(requests / el, 1),
-                              "p50_ms": round(1e3 * pct(lat, 0.5), 3), "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
+        lat = [server.result(tk)[1] for tk in tks]
+        return requests / el, lat
+
+    cap = {}
+    for name, nl in (("served", lanes), ("streams", 0)):
+        server = cfdgpu.Server(max_active=n_grids, lanes=nl)
+        grids = [server.admit(c) for c, _, _, _ in pool]
+        for pname, prec in (("fp64", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
+            rate, lat = closed_loop(server, grids, prec)
+            cap[f"{name}_{pname}"] = rate
+            out[f"{name}_{pname}"] = {"solves_per_s": round(rate, 1), "p50_ms": round(1e3 * pct(lat, 0.5), 3),
+                                      "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
+        if name == "served":
+            lam = 0.8 * cap["served_mixed"]
+            arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+            tks = []
+            t0 = time.perf_counter()
+            i = 0
+            while i < requests:
+                now = time.perf_counter() - t0
+                while i < requests and arrivals[i] <= now:
+                    k, tol = reqs[i]
+                    tks.append(server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, cfdgpu.PRECISION_MIXED,
+                                             d_scale=bufs[k][1]))
+                    i += 1
+                server.progress()
+            server.drain()
+            el = time.perf_counter() - t0
+            lat = [server.result(tk)[1] for tk in tks]
+            out["open_loop_mixed"] = {"lambda_per_s": round(lam, 1), "achieved_per_s": round(requests / el, 1),
+                                      "p50_ms": round(1e3 * pct(lat, 0.5), 3), "p99_ms": round(1e3 * pct(lat, 0.99), 3)}
+        server.close()
     out["served_mixed_over_plain"] = round(cap["served_mixed"] / out["plain_fp64"]["solves_per_s"], 3)
-    server.close()
+    out["served_fp64_over_plain"] = round(cap["served_fp64"] / out["plain_fp64"]["solves_per_s"], 3)
     return out
 
 

@@ -278,6 +278,7 @@ def lib():
             "cfd_profile_solve": (I, [P, P, P, D, I64, P, I, P, P]),
             "cfd_server_create": (I, [I, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_server_destroy": (None, [P]),
+            "cfd_server_set_batching": (I, [P, I]),
             "cfd_server_admit": (I, [P, P, ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_server_release": (I, [P, P]),
             "cfd_server_usage": (I, [P, P, P]),
@@ -676,13 +677,19 @@ class GridCache:
 
 
 class Server:
-    """Continuous batching of pressure solves on resident grids (cfd_server_*)."""
+    """Continuous batching of pressure solves on resident grids (cfd_server_*).
 
-    def __init__(self, max_active: int = 8, hbm_budget_bytes: int = 0):
+    lanes: requests on red-black multigrid grids without closed components run
+    in that many concurrent lanes, each a batch of up to 8 solves per precision
+    (one launch per iteration stage for the batch); 0: each request on its
+    grid's own stream."""
+
+    def __init__(self, max_active: int = 8, hbm_budget_bytes: int = 0, lanes: int = 2):
         h = ctypes.c_void_p()
         _check(lib().cfd_server_create(int(max_active), int(hbm_budget_bytes), ctypes.byref(h)))
         self.h = h
         self.grids = []
+        _check(lib().cfd_server_set_batching(self.h, int(lanes)))
 
     def admit(self, case: Case) -> "Grid":
         """Grid::build under the server's HBM budget (CfdError ENOMEM when it does not fit)."""
@@ -718,7 +725,10 @@ class Server:
         return t.value
 
     def progress(self) -> int:
-        return lib().cfd_server_progress(self.h)
+        n = lib().cfd_server_progress(self.h)
+        if n < 0:
+            _check(-n)
+        return n
 
     def wait(self, ticket: int):
         _check(lib().cfd_server_wait(self.h, ctypes.c_uint64(ticket)))

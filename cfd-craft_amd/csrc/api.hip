@@ -196,6 +196,7 @@ struct cfd_grid {
   int64_t agg_dense = kAggDense;
   bool rb_layout = true;  // CFD_MG_RB_LAYOUT=0: the fine level in the natural layout
   int rb_pairs = -1;      // CFD_MG_RB_PAIRS: two fine cells per thread (-1 by size, 0 never, 1 when hx is even)
+  bool force_pairs = false;  // two fine cells per thread whatever the size (grids of a batching server)
   bool rb_lin = true;     // CFD_MG_RB_LIN=0: the two-cell half-sweep in tiles instead of linear pairs
   bool ic0_tiles = true;  // CFD_IC0_TILES=0: one launch per hyperplane
   // a hierarchy built since the last solve of its cycle (MultigridWork::hierarchy_builds)
@@ -964,6 +965,7 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_RB_LAYOUT")) g->rb_layout = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_RB_PAIRS")) g->rb_pairs = std::atoi(env) ? 1 : 0;
+  if (g->force_pairs) g->rb_pairs = 1;
   if (const char* env = std::getenv("CFD_MG_RB_LIN")) g->rb_lin = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
@@ -1480,14 +1482,9 @@ int solve_device(cfd_grid* g, const double* d_b, const double* d_scale, double t
   return CFD_OK;
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------- C ABI
-extern "C" {
-
-const char* cfd_last_error(void) { return g_error.c_str(); }
-
-int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
+// A grid context from a case; force_pairs: the fine red-black level takes two
+// cells per thread whatever its size (the layout of the batched iterations).
+int create_grid(const cfd_case* c, cfd_grid** out, bool force_pairs) {
   if (!c || !out) {
     set_error("cfd_grid_create: null argument");
     return CFD_EINVAL;
@@ -1498,6 +1495,7 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
     return CFD_EINVAL;
   }
   std::unique_ptr<cfd_grid> g(new cfd_grid());
+  g->force_pairs = force_pairs;
   // rasterization, face codes and components on the device (CFD_GPU_SETUP=0: on the host)
   const char* env = std::getenv("CFD_GPU_SETUP");
   int rc = env && std::string(env) == "0" ? build_host_grid(*c, g->host) : build_device_grid(*c, g->host);
@@ -1507,6 +1505,15 @@ int cfd_grid_create(const cfd_case* c, cfd_grid** out) {
   *out = g.release();
   return CFD_OK;
 }
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* cfd_last_error(void) { return g_error.c_str(); }
+
+int cfd_grid_create(const cfd_case* c, cfd_grid** out) { return create_grid(c, out, false); }
 
 int cfd_grid_create_desc(const cfd_grid_desc* d, cfd_grid** out) {
   if (!d || !out) {
@@ -2119,14 +2126,20 @@ int cfd_device_synchronize(void) {
 
 // ---------------------------------------------------------------- serving
 // Continuous batching of independent pressure solves: each request names a
-// resident grid; up to max_active requests run at once, each on its grid's own
-// stream, and a request that arrives while others are on the GPU starts as soon
-// as a slot and its grid are free. Grids are admitted against an HBM budget by
-// their measured footprint.
+// resident grid; up to max_active requests run at once, and a request that
+// arrives while others are on the GPU starts as soon as a slot and its grid are
+// free. Requests whose grid and precision allow it (red-black cycle,
+// colour-separated fine level with two cells per thread, no closed component)
+// join a batch of their precision: the iterations of all its solves go out as
+// one launch per stage (launch_batch_iteration), and a solve joins or leaves
+// the batch at any iteration. Other requests run on their grid's own stream.
+// Grids are admitted against an HBM budget by their measured footprint.
 namespace {
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+constexpr int kStepRing = 4;   // host slots for the states read back after each batched step
+constexpr int kStepsAhead = 2;  // batched steps queued beyond the last one seen finished
 }  // namespace
 
 struct cfd_server {
@@ -2140,58 +2153,339 @@ struct cfd_server {
     int precision = 0;
     int grid_precision = 0;  // the grid's own setting, restored when the request ends
     double t_submit = 0.0, t_start = 0.0, t_end = 0.0;
-    int state = 0;  // 0 queued, 1 running, 2 done
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 batched and finishing (final kernels queued)
+    int lane = -1;  // batched: the lane it runs in
+    int64_t enq = 0;  // batched: iterations queued
     int rc = 0;
     std::string error;
     cfd_linear_info info{};
+  };
+  // A lane: a stream running batches (one per precision) step after step. Lanes
+  // run concurrently, so one lane's latency-bound coarse levels overlap another's
+  // streaming fine-level kernels.
+  struct Step {
+    int slot = 0;
+    std::vector<uint64_t> ids;  // the solves whose states the step reads back, in slot order
+  };
+  struct Lane {
+    hipStream_t bs = nullptr;
+    std::vector<uint64_t> group[2];  // solves in the FP64 and the mixed-precision batch
+    std::deque<Step> steps;
+    int next_slot = 0;
+    PcgState* d_states = nullptr;  // kStepRing x 2 kBatchMax
+    PcgState* h_states = nullptr;  // pinned copy of the same
+    hipEvent_t step_ev[kStepRing] = {};
   };
   int max_active = 8;
   size_t budget = 0, used = 0;
   uint64_t next = 1;
   std::unordered_map<uint64_t, Req> reqs;
   std::deque<uint64_t> queued;
-  std::vector<uint64_t> running;
+  std::vector<uint64_t> running;  // on their grids' streams
   std::unordered_map<cfd_grid*, int> busy;
   std::unordered_map<cfd_grid*, size_t> grids;  // admitted grids and their footprints
+  int n_lanes = 2;  // 0: no batching
+  std::vector<Lane> lanes;
+  std::vector<uint64_t> finishing;  // batched solves whose final kernels are queued
+  ~cfd_server() {
+    for (Lane& L : lanes) {
+      if (L.bs) (void)hipStreamSynchronize(L.bs);
+      for (hipEvent_t e : L.step_ev)
+        if (e) (void)hipEventDestroy(e);
+      if (L.d_states) (void)hipFree(L.d_states);
+      if (L.h_states) (void)hipHostFree(L.h_states);
+      if (L.bs) (void)hipStreamDestroy(L.bs);
+    }
+  }
 };
 
 namespace {
-// Drives every running job one step and starts queued requests whose grid is idle.
-int server_progress(cfd_server* s) {
-  for (size_t i = 0; i < s->running.size();) {
-    cfd_server::Req& R = s->reqs[s->running[i]];
-    int rc = job_poll(R.g, false, nullptr, nullptr);
-    if (rc == JOB_PENDING) {
+// The request's grid and precision run in a batch: the red-black cycle on the
+// colour-separated fine level with two cells per thread (nx a multiple of 4), no
+// closed component, and the FP64 solve's r and z kept in that level's arrays
+// (or the mixed solve's z as FP32 values).
+bool batchable(const cfd_grid* g, int precision) {
+  if (!g->has_hier || g->host.preconditioner == CFD_IC0 || g->opt.cycle != CFD_CYCLE_RB) return false;
+  const DevAggShared& A = g->ash;
+  if (!A.has_rb || A.nlev < 1 || A.seams != 0 || g->n_closed != 0 || g->fine_rounds < 1 || g->coarse_rounds < 1)
+    return false;
+  if (g->geo.n[0] % 4 != 0 || g->geo.N >= (1LL << 31) || (double)g->geo.N * g->geo.n[0] >= 1099511627776.0)
+    return false;
+  if (precision == CFD_PRECISION_MIXED) return A.rb32.two_cells == 2 && g->z32.p;
+  return A.rb64.two_cells == 2;
+}
+
+int ensure_lanes(cfd_server* s) {
+  if ((int)s->lanes.size() == s->n_lanes) return 0;
+  s->lanes.resize((size_t)s->n_lanes);
+  for (cfd_server::Lane& L : s->lanes) {
+    if (L.bs) continue;
+    HIP_TRY(hipStreamCreateWithFlags(&L.bs, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc((void**)&L.d_states, (size_t)kStepRing * 2 * kBatchMax * sizeof(PcgState)));
+    HIP_TRY(hipHostMalloc((void**)&L.h_states, (size_t)kStepRing * 2 * kBatchMax * sizeof(PcgState),
+                          hipHostMallocDefault));
+    for (hipEvent_t& e : L.step_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  return 0;
+}
+
+// The solve of grid g as the batched launches see it; pold / pnew by the parity
+// of its next iteration.
+template <class T>
+BatchSolve<T> batch_solve(cfd_grid* g, const FineRB<T>& F, const std::vector<AggDevLevel<T>>& lv, int64_t enq) {
+  BatchSolve<T> b;
+  const DevAggShared& A = g->ash;
+  b.g = g->geo;
+  b.codes = g->codes.p;
+  b.F = F;
+  b.fparent = A.fparent.p;
+  b.nlev = A.nlev;
+  b.lv = lv.data();
+  b.D = A.D;
+  b.st = g->st.p;
+  b.part = g->part.p;
+  b.x = g->x.p;
+  b.r = g->r.p;
+  b.bc = g->bc.p;
+  b.ap = g->ap.p;
+  const bool odd = enq & 1;
+  b.pold = odd ? g->p1.p : g->p0.p;
+  b.pnew = odd ? g->p0.p : g->p1.p;
+  b.z32 = g->z32.p;
+  b.cs = (T)g->omega;
+  b.zero_last = g->zero_last;
+  b.fine_rounds = g->fine_rounds;
+  b.coarse_rounds = g->coarse_rounds;
+  const int tiles = ((g->geo.n[0] + 63) / 64) * ((g->geo.n[1] + 15) / 16) * ((g->geo.n[2] + 15) / 16);
+  b.march = g->march == 2 || (g->march == 1 && tiles >= kReduceGrid);
+  return b;
+}
+
+// A request joins a batch of its precision in lane `lane`: the set-up kernels
+// of solve_scaled (pressure.rs:342-375) on the lane's stream, and for the FP64
+// solve the residual scattered into the fine level's colour-separated arrays,
+// where the batched iterations keep it. Its first batched iteration applies the
+// preconditioner to that residual.
+int batch_begin(cfd_server* s, int lane, cfd_server::Req& R) {
+  cfd_grid* g = R.g;
+  hipStream_t st = s->lanes[(size_t)lane].bs;
+  PcgState* sp = g->st.p;
+  HIP_TRY(hipEventRecord(g->ev0, st));
+  launch_state_init(sp, R.tol, R.max, st);
+  launch_pcg_init(g->geo.N, g->codes.p, R.b, g->mean_b.p, nullptr, g->r.p, g->bc.p, g->x.p, sp, g->part.p, st);
+  if (g->opt.warm_start && g->warm_valid)
+    launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, sp, st);
+  if (R.precision == CFD_PRECISION_FP64) launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, sp, st);
+  R.enq = 0;
+  R.lane = lane;
+  return 0;
+}
+
+// One batched step of a lane: an iteration of every solve in each of its
+// batches, then their states copied into the lane's next host slot.
+int batch_step(cfd_server* s, cfd_server::Lane& L) {
+  cfd_server::Step step;
+  step.slot = L.next_slot;
+  L.next_slot = (L.next_slot + 1) % kStepRing;
+  std::vector<PcgState*> sts;
+  for (int grp = 0; grp < 2; ++grp) {
+    std::vector<uint64_t>& ids = L.group[grp];
+    if (ids.empty()) continue;
+    if (grp == 0) {
+      std::vector<BatchSolve<double>> items;
+      for (uint64_t id : ids) {
+        cfd_server::Req& R = s->reqs[id];
+        items.push_back(batch_solve<double>(R.g, R.g->ash.rb64, R.g->a64.levels, R.enq));
+      }
+      launch_batch_iteration<double>(items.data(), (int)items.size(), L.bs);
+    } else {
+      std::vector<BatchSolve<float>> items;
+      for (uint64_t id : ids) {
+        cfd_server::Req& R = s->reqs[id];
+        items.push_back(batch_solve<float>(R.g, R.g->ash.rb32, R.g->a32.levels, R.enq));
+      }
+      launch_batch_iteration<float>(items.data(), (int)items.size(), L.bs);
+    }
+    for (uint64_t id : ids) {
+      cfd_server::Req& R = s->reqs[id];
+      R.enq += 1;
+      step.ids.push_back(id);
+      sts.push_back(R.g->st.p);
+    }
+  }
+  PcgState* d = L.d_states + (size_t)step.slot * 2 * kBatchMax;
+  for (size_t k = 0; k < sts.size(); k += kBatchMax)
+    launch_batch_states(sts.data() + k, (int)std::min<size_t>(kBatchMax, sts.size() - k), d + k, L.bs);
+  HIP_TRY(hipMemcpyAsync(L.h_states + (size_t)step.slot * 2 * kBatchMax, d, sts.size() * sizeof(PcgState),
+                         hipMemcpyDeviceToHost, L.bs));
+  HIP_TRY(hipEventRecord(L.step_ev[step.slot], L.bs));
+  L.steps.push_back(std::move(step));
+  return 0;
+}
+
+void end_request(cfd_server* s, cfd_server::Req& R, int rc, const std::string& error) {
+  R.t_end = now_s();
+  R.state = 2;
+  R.rc = rc;
+  R.error = error;
+  R.g->opt.precision = R.grid_precision;
+  s->busy.erase(R.g);
+}
+
+// A batched solve whose state shows it ended: its LinearInfo and code, and the
+// final kernels of solve_scaled (x centered into the caller's buffer, the warm
+// start kept) on its lane's stream. The request ends once that stream has
+// passed them, and with them every queued step that still names its grid.
+int batch_finish(cfd_server* s, uint64_t id, const PcgState& st) {
+  cfd_server::Req& R = s->reqs[id];
+  cfd_grid* g = R.g;
+  cfd_server::Lane& L = s->lanes[(size_t)R.lane];
+  std::vector<uint64_t>& grp = L.group[R.precision == CFD_PRECISION_MIXED ? 1 : 0];
+  grp.erase(std::find(grp.begin(), grp.end(), id));
+  if (st.status == ST_CONVERGED) g->learned[learn_key(g, R.tol)] = st.it;
+  fill_info(g, st, R.info);
+  R.rc = finish_status(g, st, R.max);
+  R.error = R.rc ? g_error : std::string();
+  if (!R.rc) {
+    if (st.status == ST_ZERO) {
+      launch_fill(g->geo.N, R.x, 0.0, L.bs);
+    } else {
+      launch_center(g->geo.N, g->codes.p, g->x.p, g->mean_x.p, nullptr, R.x, L.bs);
+    }
+    if (g->opt.warm_start) {
+      HIP_TRY(hipMemcpyAsync(g->warm.p, R.x, (size_t)g->geo.N * sizeof(double), hipMemcpyDeviceToDevice, L.bs));
+      g->warm_valid = true;
+    }
+  }
+  HIP_TRY(hipEventRecord(g->ev1, L.bs));
+  HIP_TRY(hipEventRecord(g->evs, L.bs));
+  R.state = 3;
+  s->finishing.push_back(id);
+  return 0;
+}
+
+// Steps whose read-back landed: solves that ended leave their batch; finished
+// final kernels complete their requests.
+int batch_progress(cfd_server* s) {
+  for (cfd_server::Lane& L : s->lanes) {
+    while (!L.steps.empty()) {
+      const cfd_server::Step& step = L.steps.front();
+      hipError_t q = hipEventQuery(L.step_ev[step.slot]);
+      if (q == hipErrorNotReady) break;
+      if (q != hipSuccess) {
+        set_error(std::string("HIP error: ") + hipGetErrorString(q));
+        return CFD_EDEVICE;
+      }
+      const PcgState* h = L.h_states + (size_t)step.slot * 2 * kBatchMax;
+      for (size_t k = 0; k < step.ids.size(); ++k) {
+        cfd_server::Req& R = s->reqs[step.ids[k]];
+        if (R.state != 1) continue;  // already ended in an earlier step
+        if (h[k].status != ST_RUNNING) {
+          int rc = batch_finish(s, step.ids[k], h[k]);
+          if (rc) return rc;
+        }
+      }
+      L.steps.pop_front();
+    }
+  }
+  for (size_t i = 0; i < s->finishing.size();) {
+    cfd_server::Req& R = s->reqs[s->finishing[i]];
+    hipError_t q = hipEventQuery(R.g->evs);
+    if (q == hipErrorNotReady) {
       ++i;
       continue;
     }
-    R.t_end = now_s();
-    R.state = 2;
-    R.g->opt.precision = R.grid_precision;
-    if (rc == JOB_DONE) {
-      R.rc = R.g->job.rc;
-      R.error = R.g->job.error;
-      R.info = R.g->job.info;
-    } else {
-      R.rc = rc;
-      R.error = g_error;
+    if (q != hipSuccess) {
+      set_error(std::string("HIP error: ") + hipGetErrorString(q));
+      return CFD_EDEVICE;
     }
-    s->busy.erase(R.g);
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, R.g->ev0, R.g->ev1));
+    R.info.seconds = ms * 1e-3;
+    end_request(s, R, R.rc, R.error);
+    s->finishing[i] = s->finishing.back();
+    s->finishing.pop_back();
+  }
+  return 0;
+}
+
+// Steps are queued in every lane with a solve, at most kStepsAhead past the last
+// step seen finished.
+int batch_fill(cfd_server* s) {
+  for (cfd_server::Lane& L : s->lanes)
+    while ((!L.group[0].empty() || !L.group[1].empty()) && (int)L.steps.size() < kStepsAhead) {
+      int rc = batch_step(s, L);
+      if (rc) return rc;
+    }
+  return 0;
+}
+
+int active_count(const cfd_server* s) {
+  size_t n = s->running.size() + s->finishing.size();
+  for (const cfd_server::Lane& L : s->lanes) n += L.group[0].size() + L.group[1].size();
+  return (int)n;
+}
+
+// The lane a batched request of precision group grp joins: the one with the
+// fewest solves that has room in that group (-1: none).
+int pick_lane(const cfd_server* s, int grp) {
+  int best = -1;
+  size_t best_n = 0;
+  for (size_t i = 0; i < s->lanes.size(); ++i) {
+    const cfd_server::Lane& L = s->lanes[i];
+    if ((int)L.group[grp].size() >= kBatchMax) continue;
+    const size_t n = L.group[0].size() + L.group[1].size();
+    if (best < 0 || n < best_n) {
+      best = (int)i;
+      best_n = n;
+    }
+  }
+  return best;
+}
+
+// Drives every running job one step and starts queued requests whose grid is idle.
+int server_progress(cfd_server* s) {
+  int rc = batch_progress(s);
+  if (rc) return -rc;
+  for (size_t i = 0; i < s->running.size();) {
+    cfd_server::Req& R = s->reqs[s->running[i]];
+    int jr = job_poll(R.g, false, nullptr, nullptr);
+    if (jr == JOB_PENDING) {
+      ++i;
+      continue;
+    }
+    if (jr == JOB_DONE) {
+      R.info = R.g->job.info;
+      end_request(s, R, R.g->job.rc, R.g->job.error);
+    } else {
+      end_request(s, R, jr, g_error);
+    }
     s->running[i] = s->running.back();
     s->running.pop_back();
   }
-  for (size_t i = 0; i < s->queued.size() && (int)s->running.size() < s->max_active;) {
+  for (size_t i = 0; i < s->queued.size() && active_count(s) < s->max_active;) {
     const uint64_t id = s->queued[i];
     cfd_server::Req& R = s->reqs[id];
     if (s->busy.count(R.g)) {
       ++i;
       continue;
     }
+    const bool batch = s->n_lanes > 0 && batchable(R.g, R.precision);
+    const int grp = R.precision == CFD_PRECISION_MIXED ? 1 : 0;
+    int lane = -1;
+    if (batch) {
+      if ((rc = ensure_lanes(s))) return -rc;
+      lane = pick_lane(s, grp);
+      if (lane < 0) {
+        ++i;
+        continue;
+      }
+    }
     // the request's precision for the duration of the job (the iteration graphs
     // and learned counts are keyed by it); the grid's own setting comes back after
     R.grid_precision = R.g->opt.precision;
     R.g->opt.precision = R.precision;
-    int rc = job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
+    rc = batch ? batch_begin(s, lane, R) : job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
     R.t_start = now_s();
     s->queued.erase(s->queued.begin() + (long)i);
     if (rc) {
@@ -2204,9 +2498,13 @@ int server_progress(cfd_server* s) {
     }
     R.state = 1;
     s->busy[R.g] = 1;
-    s->running.push_back(id);
+    if (batch)
+      s->lanes[(size_t)lane].group[grp].push_back(id);
+    else
+      s->running.push_back(id);
   }
-  return (int)(s->running.size() + s->queued.size());
+  if ((rc = batch_fill(s))) return -rc;
+  return (int)(active_count(s) + s->queued.size());
 }
 }  // namespace
 
@@ -2228,6 +2526,19 @@ int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out)
   return CFD_OK;
 }
 
+int cfd_server_set_batching(cfd_server* s, int lanes) {
+  if (!s || lanes < 0 || lanes > 8) {
+    set_error("cfd_server_set_batching: lanes must be 0 .. 8");
+    return CFD_EINVAL;
+  }
+  if (!s->lanes.empty() && lanes != (int)s->lanes.size()) {
+    set_error("cfd_server_set_batching: the lanes are set before the first request");
+    return CFD_EINVAL;
+  }
+  s->n_lanes = lanes;
+  return CFD_OK;
+}
+
 void cfd_server_destroy(cfd_server* s) {
   if (!s) return;
   while (server_progress(s) > 0) std::this_thread::yield();
@@ -2236,7 +2547,8 @@ void cfd_server_destroy(cfd_server* s) {
 
 int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out) {
   cfd_grid* g = nullptr;
-  int rc = cfd_grid_create(c, &g);
+  // a batching server's grids take two fine cells per thread (the batched layout)
+  int rc = create_grid(c, &g, s->n_lanes > 0);
   if (rc) return rc;
   if (s->used + g->footprint > s->budget) {
     char buf[200];
@@ -2259,7 +2571,7 @@ int cfd_server_release(cfd_server* s, cfd_grid* g) {
     return CFD_EINVAL;
   }
   while (s->busy.count(g)) {
-    server_progress(s);
+    if (server_progress(s) < 0) return CFD_EDEVICE;
     std::this_thread::yield();
   }
   for (const uint64_t id : s->queued)
@@ -2267,6 +2579,9 @@ int cfd_server_release(cfd_server* s, cfd_grid* g) {
       set_error("cfd_server_release: requests on this grid are still queued");
       return CFD_EINVAL;
     }
+  // batched steps queued before the last solve on the grid ended still name it
+  for (cfd_server::Lane& L : s->lanes)
+    if (L.bs) HIP_TRY(hipStreamSynchronize(L.bs));
   s->used -= it->second;
   s->grids.erase(it);
   cfd_grid_destroy(g);
@@ -2303,8 +2618,8 @@ int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const doubl
   R.precision = precision;
   R.t_submit = now_s();
   s->queued.push_back(id);
-  server_progress(s);
   if (ticket) *ticket = id;
+  if (server_progress(s) < 0) return CFD_EDEVICE;
   return CFD_OK;
 }
 
@@ -2317,14 +2632,19 @@ int cfd_server_wait(cfd_server* s, uint64_t ticket) {
     return CFD_EINVAL;
   }
   while (it->second.state != 2) {
-    server_progress(s);
+    if (server_progress(s) < 0) return CFD_EDEVICE;
     if (it->second.state != 2) std::this_thread::yield();
   }
   return CFD_OK;
 }
 
 int cfd_server_drain(cfd_server* s) {
-  while (server_progress(s) > 0) std::this_thread::yield();
+  for (;;) {
+    const int n = server_progress(s);
+    if (n < 0) return CFD_EDEVICE;
+    if (n == 0) break;
+    std::this_thread::yield();
+  }
   return CFD_OK;
 }
 

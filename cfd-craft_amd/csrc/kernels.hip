@@ -187,15 +187,21 @@ __device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m) 
 struct XcdShare {
   unsigned begin, end, first, step;
 };
-__device__ __forceinline__ XcdShare xcd_share(unsigned items, unsigned per_block, unsigned rank_in_block) {
-  const unsigned x = blockIdx.x & 7, blocks = gridDim.x >> 3;
+// (bid, nblk: the block's index and the block count of its grid; a batched
+// launch gives each grid its own range of blocks, starting at a multiple of 8)
+__device__ __forceinline__ XcdShare xcd_share_at(unsigned bid, unsigned nblk, unsigned items, unsigned per_block,
+                                                 unsigned rank_in_block) {
+  const unsigned x = bid & 7, blocks = nblk >> 3;
   const unsigned chunk = (items + 7) >> 3;
   XcdShare s;
   s.begin = min(items, x * chunk);
   s.end = min(items, s.begin + chunk);
-  s.first = s.begin + (blockIdx.x >> 3) * per_block + rank_in_block;
+  s.first = s.begin + (bid >> 3) * per_block + rank_in_block;
   s.step = blocks * per_block;
   return s;
+}
+__device__ __forceinline__ XcdShare xcd_share(unsigned items, unsigned per_block, unsigned rank_in_block) {
+  return xcd_share_at(blockIdx.x, gridDim.x, items, per_block, rank_in_block);
 }
 
 // Every cell of the grid, once, over a fixed reduction grid: each wave takes
@@ -203,11 +209,11 @@ __device__ __forceinline__ XcdShare xcd_share(unsigned items, unsigned per_block
 // a segment's row is wave-uniform (scalar arithmetic), so no cell needs a
 // division.
 template <class F>
-__device__ __forceinline__ void for_cells(const Geo& g, F f) {
+__device__ __forceinline__ void for_cells_at(const Geo& g, unsigned bid, unsigned nblk, F f) {
   const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
   const unsigned sx = (nx + 63) >> 6, segs = sx * ny * (unsigned)g.n[2];
   const unsigned lane = threadIdx.x & 63;
-  const XcdShare sh = xcd_share(segs, blockDim.x >> 6, threadIdx.x >> 6);
+  const XcdShare sh = xcd_share_at(bid, nblk, segs, blockDim.x >> 6, threadIdx.x >> 6);
   for (unsigned sg = __builtin_amdgcn_readfirstlane(sh.first); sg < sh.end; sg += sh.step) {
     const unsigned row = sg / sx;
     const unsigned i = (sg - row * sx) * 64 + lane;
@@ -216,6 +222,10 @@ __device__ __forceinline__ void for_cells(const Geo& g, F f) {
     const int q[3] = {(int)i, (int)(row - k * ny), (int)k};
     f((long long)i + (long long)nx * row, q);
   }
+}
+template <class F>
+__device__ __forceinline__ void for_cells(const Geo& g, F f) {
+  for_cells_at(g, blockIdx.x, gridDim.x, f);
 }
 
 // The cells of the operator kernel: p_new = z_c (FIRST) or z_c + beta p_old,
@@ -232,7 +242,7 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
                                                const TZ* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
-                                               double beta, long long zhalf = 0) {
+                                               double beta, long long zhalf, unsigned bid, unsigned nblk) {
   auto pv = [&](long long i, int colour) {
     double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
     // closed components: z centered (without any, the code load is skipped)
@@ -241,7 +251,7 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
     return zc + beta * pold[i];
   };
   double acc = 0.0;
-  for_cells(g, [&](long long c, const int q[3]) {
+  for_cells_at(g, bid, nblk, [&](long long c, const int q[3]) {
     const int cc = (q[0] + q[1] + q[2]) & 1;
     auto pvc = [&](long long i) { return pv(i, i == c ? cc : cc ^ 1); };
     uint8_t code = codes[c];
@@ -257,23 +267,31 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
 // (registers sized for 8 waves per SIMD: two blocks per CU)
-template <bool HAS_CLOSED, bool RES, class TZ = double>
-__global__ void __launch_bounds__(RB, 8)
-    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z, const double* __restrict__ pold,
-             double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
-             const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf) {
+template <bool HAS_CLOSED, bool RES, class TZ>
+__device__ __forceinline__ void pcg_op_body(const Geo& g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
+                                            const double* __restrict__ pold, double* __restrict__ pnew,
+                                            double* __restrict__ ap, const double* __restrict__ mean,
+                                            const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf,
+                                            unsigned bid, unsigned nblk) {
   if (!block_running(st)) return;
   const bool first = st->first;
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_cells<HAS_CLOSED, true, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf)
-            : pcg_op_cells<HAS_CLOSED, false, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf);
+      first ? pcg_op_cells<HAS_CLOSED, true, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid, nblk)
+            : pcg_op_cells<HAS_CLOSED, false, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid, nblk);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
-    part[P_PAP + blockIdx.x] = s;
-    if (blockIdx.x == 0 && !first) st->beta = beta;
+    part[P_PAP + bid] = s;
+    if (bid == 0 && !first) st->beta = beta;
   }
+}
+template <bool HAS_CLOSED, bool RES, class TZ = double>
+__global__ void __launch_bounds__(RB, 8)
+    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z, const double* __restrict__ pold,
+             double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
+             const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf) {
+  pcg_op_body<HAS_CLOSED, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x, gridDim.x);
 }
 
 // The same kernel for large grids, marching through z: work items are tiles of
@@ -298,7 +316,8 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
                                                const TZ* __restrict__ z, const double* __restrict__ pold,
                                                double* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
-                                               double beta, double (*tile)[MY + 2][MX + 2], long long zhalf) {
+                                               double beta, double (*tile)[MY + 2][MX + 2], long long zhalf,
+                                               unsigned bid, unsigned nblk) {
   // colour: that of cell i (used with RES, z in the colour-separated layout)
   auto pv = [&](long long i, int colour) {
     double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
@@ -308,7 +327,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   };
   const int lx = (int)threadIdx.x % MX, ly = (int)threadIdx.x / MX;
   const unsigned items = (unsigned)(ms.tx * ms.ty * ms.tz);
-  const XcdShare sh = xcd_share(items, 1, 0);
+  const XcdShare sh = xcd_share_at(bid, nblk, items, 1, 0);
   double acc = 0.0;
   for (unsigned item = sh.first; item < sh.end; item += sh.step) {
     const int txi = (int)(item % (unsigned)ms.tx);
@@ -433,37 +452,50 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   return acc;
 }
 
-template <bool HAS_CLOSED, bool RES, class TZ = double>
-__global__ void __launch_bounds__(RB, 8)
-    k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
-                   const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
-                   const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part,
-                   long long zhalf) {
+template <bool HAS_CLOSED, bool RES, class TZ>
+__device__ __forceinline__ void pcg_op_march_body(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
+                                                  const TZ* __restrict__ z, const double* __restrict__ pold,
+                                                  double* __restrict__ pnew, double* __restrict__ ap,
+                                                  const double* __restrict__ mean, const int* __restrict__ cidx,
+                                                  PcgState* st, double* part, long long zhalf, unsigned bid,
+                                                  unsigned nblk) {
   __shared__ double tile[2][MY + 2][MX + 2];
   if (!block_running(st)) return;
   const bool first = st->first;
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_march<HAS_CLOSED, true, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf)
-            : pcg_op_march<HAS_CLOSED, false, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf);
+      first ? pcg_op_march<HAS_CLOSED, true, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf,
+                                                       bid, nblk)
+            : pcg_op_march<HAS_CLOSED, false, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf,
+                                                        bid, nblk);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
-    part[P_PAP + blockIdx.x] = s;
-    if (blockIdx.x == 0 && !first) st->beta = beta;
+    part[P_PAP + bid] = s;
+    if (bid == 0 && !first) st->beta = beta;
   }
+}
+template <bool HAS_CLOSED, bool RES, class TZ = double>
+__global__ void __launch_bounds__(RB, 8)
+    k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
+                   const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
+                   const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part,
+                   long long zhalf) {
+  pcg_op_march_body<HAS_CLOSED, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x,
+                                         gridDim.x);
 }
 
 // alpha = rz / (p . Ap) with CFD-E032 (pressure.rs:393-397); x += alpha p;
 // r -= alpha Ap; partials of |r|^2 (pressure.rs:398-402).
-__global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __restrict__ p, const double* __restrict__ ap,
-                                                 double* __restrict__ x, double* __restrict__ r, PcgState* st,
-                                                 double* part) {
+__device__ __forceinline__ void pcg_axpy_body(long long N, const double* __restrict__ p,
+                                              const double* __restrict__ ap, double* __restrict__ x,
+                                              double* __restrict__ r, PcgState* st, double* part, unsigned bid,
+                                              unsigned nblk) {
   if (!block_running(st)) return;
   const double rz = reduce_part(part + P_RZ, RP);
   const double denom = reduce_part(part + P_PAP, RP);
   if (denom <= 0.0 || !isfinite(denom)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       st->denom = denom;
       st->status = ST_E032;
     }
@@ -471,7 +503,7 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
   }
   const double alpha = rz / denom;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
+  for (long long i = (long long)bid * RB + threadIdx.x; i < N; i += (long long)nblk * RB) {
     x[i] += alpha * p[i];
     double ri = r[i] - alpha * ap[i];
     r[i] = ri;
@@ -479,8 +511,8 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
   }
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
-    part[P_RR + blockIdx.x] = s;
-    if (blockIdx.x == 0) {
+    part[P_RR + bid] = s;
+    if (bid == 0) {
       st->rz = rz;
       st->alpha = alpha;
       st->denom = denom;
@@ -488,21 +520,27 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
     }
   }
 }
+__global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __restrict__ p, const double* __restrict__ ap,
+                                                 double* __restrict__ x, double* __restrict__ r, PcgState* st,
+                                                 double* part) {
+  pcg_axpy_body(N, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
+}
 
 // The same with r in the colour-separated layout of the fine red-black level
 // (cell c at c / 2 of the half of its colour, (x + y + z) & 1; nx even): the
 // same cells per thread in the same order, so x, r and the partials of |r|^2
 // are those of k_pcg_axpy bit for bit. mnx, mny: ceil(2^40 / nx), ceil(2^40 / ny)
 // (N nx < 2^40).
-__global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny, unsigned long long mnx,
-                                                    unsigned long long mny, const double* __restrict__ p,
-                                                    const double* __restrict__ ap, double* __restrict__ x,
-                                                    double* __restrict__ r, PcgState* st, double* part) {
+__device__ __forceinline__ void pcg_axpy_rb_body(long long N, int nx, int ny, unsigned long long mnx,
+                                                 unsigned long long mny, const double* __restrict__ p,
+                                                 const double* __restrict__ ap, double* __restrict__ x,
+                                                 double* __restrict__ r, PcgState* st, double* part, unsigned bid,
+                                                 unsigned nblk) {
   if (!block_running(st)) return;
   const double rz = reduce_part(part + P_RZ, RP);
   const double denom = reduce_part(part + P_PAP, RP);
   if (denom <= 0.0 || !isfinite(denom)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       st->denom = denom;
       st->status = ST_E032;
     }
@@ -511,7 +549,7 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny,
   const double alpha = rz / denom;
   const long long half = N >> 1;
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
+  for (long long i = (long long)bid * RB + threadIdx.x; i < N; i += (long long)nblk * RB) {
     const unsigned c = (unsigned)i;
     const unsigned row = div_magic(c, mnx), k = div_magic(row, mny);
     const unsigned colour = (c - row * (unsigned)nx + row - k * (unsigned)ny + k) & 1u;
@@ -523,8 +561,8 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny,
   }
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
-    part[P_RR + blockIdx.x] = s;
-    if (blockIdx.x == 0) {
+    part[P_RR + bid] = s;
+    if (bid == 0) {
       st->rz = rz;
       st->alpha = alpha;
       st->denom = denom;
@@ -532,28 +570,39 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny,
     }
   }
 }
+__global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny, unsigned long long mnx,
+                                                    unsigned long long mny, const double* __restrict__ p,
+                                                    const double* __restrict__ ap, double* __restrict__ x,
+                                                    double* __restrict__ r, PcgState* st, double* part) {
+  pcg_axpy_rb_body(N, nx, ny, mnx, mny, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
+}
 
 // When the recurrent residual |r|/|b_c| reached tol: partials of |A x - b_c|^2,
 // the independent recomputed residual (pressure.rs:403-414).
-__global__ void __launch_bounds__(RB, 8) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
-                                                 const double* __restrict__ bc, PcgState* st, double* part) {
+__device__ __forceinline__ void pcg_true_body(const Geo& g, const uint8_t* __restrict__ codes,
+                                              const double* __restrict__ x, const double* __restrict__ bc, PcgState* st,
+                                              double* part, unsigned bid, unsigned nblk) {
   if (!block_running(st)) return;
   const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
   if (!(rel <= st->tol)) return;
   double acc = 0.0;
-  for_cells(g, [&](long long c, const int q[3]) {
+  for_cells_at(g, bid, nblk, [&](long long c, const int q[3]) {
     double y = apply_cell(g, codes[c], c, q, [&](long long i) { return x[i]; });
     double d = y - bc[c];
     acc += d * d;
   });
   double s = block_sum(acc);
-  if (threadIdx.x == 0) part[P_TR + blockIdx.x] = s;
+  if (threadIdx.x == 0) part[P_TR + bid] = s;
+}
+__global__ void __launch_bounds__(RB, 8) k_pcg_true(Geo g, const uint8_t* __restrict__ codes, const double* __restrict__ x,
+                                                 const double* __restrict__ bc, PcgState* st, double* part) {
+  pcg_true_body(g, codes, x, bc, st, part, blockIdx.x, gridDim.x);
 }
 
 // One block: the iteration's decisions. Acceptance needs the recurrent and the
 // recomputed residual (<= 10 tol + 1e-12, pressure.rs:415); the limit raises
 // CFD-E033 (pressure.rs:435).
-__global__ void __launch_bounds__(RB) k_pcg_fin(PcgState* st, const double* part) {
+__device__ __forceinline__ void pcg_fin_body(PcgState* st, const double* part) {
   if (!block_running(st)) return;
   const double rel = sqrt(reduce_part(part + P_RR, RP)) / st->norm;
   double tr = 0.0;
@@ -572,6 +621,7 @@ __global__ void __launch_bounds__(RB) k_pcg_fin(PcgState* st, const double* part
       st->status = ST_E033;
   }
 }
+__global__ void __launch_bounds__(RB) k_pcg_fin(PcgState* st, const double* part) { pcg_fin_body(st, part); }
 
 // Partials of r . z_c (pressure.rs:428); the next iteration's kernels reduce them.
 __global__ void __launch_bounds__(RB) k_pcg_rz(long long N, const uint8_t* __restrict__ codes, const double* __restrict__ r,
@@ -1690,14 +1740,19 @@ __device__ __forceinline__ T agg_row_relax(const AggDevLevel<T>& L, int i, const
 // reads: this class is overwritten and the other one is recomputed from scratch
 // by the next half-sweep, so no node needs the correction stored.
 template <class T>
-__global__ void __launch_bounds__(BS) k_agg_gs(AggDevLevel<T> L, int colour, const PcgState* st,
-                                               const T* __restrict__ xc = nullptr, T cs = T(0)) {
+__device__ __forceinline__ void agg_gs_body(const AggDevLevel<T>& L, int colour, const PcgState* st,
+                                            const T* __restrict__ xc, T cs, unsigned bid) {
   const int status = st ? st->status : ST_RUNNING;
-  const int i = L.cptr[colour] + (int)(blockIdx.x * BS + threadIdx.x);
+  const int i = L.cptr[colour] + (int)(bid * BS + threadIdx.x);
   if (i >= L.cptr[colour + 1]) return;
   const T v = agg_row_relax<T>(L, i, xc, cs);
   if (status != ST_RUNNING) return;
   L.x[i] = v;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_gs(AggDevLevel<T> L, int colour, const PcgState* st,
+                                               const T* __restrict__ xc = nullptr, T cs = T(0)) {
+  agg_gs_body<T>(L, colour, st, xc, cs, blockIdx.x);
 }
 
 // The first pre-smoothing half-sweep of a coarse level starts from a zero
@@ -1720,12 +1775,12 @@ __device__ __forceinline__ T first_relax(int node, int red_end, T b, const T* __
 // The coarse iterate starts at zero.
 constexpr int kRestrictThreads = 512;
 template <class T>
-__global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C,
-                                                                   const PcgState* st, int zero_from) {
+__device__ __forceinline__ void agg_restrict_body(const AggDevLevel<T>& F, const AggDevLevel<T>& C, const PcgState* st,
+                                                  int zero_from, unsigned bid) {
   __shared__ T part[8][64];
   const int status = st ? st->status : ST_RUNNING;
   const int lane = (int)(threadIdx.x & 63), sub = (int)(threadIdx.x >> 6);
-  const int J = (int)blockIdx.x * 64 + lane;
+  const int J = (int)bid * 64 + lane;
   T r = T(0);
   if (J < C.n) {
     const int k0 = C.child_ptr[J], k1 = C.child_ptr[J + 1];
@@ -1743,16 +1798,26 @@ __global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T
   C.b[J] = sum;
   C.x[J] = first_relax<T>(J, C.cptr[1], sum, C.diag);
 }
+template <class T>
+__global__ void __launch_bounds__(kRestrictThreads) k_agg_restrict(AggDevLevel<T> F, AggDevLevel<T> C,
+                                                                   const PcgState* st, int zero_from) {
+  agg_restrict_body<T>(F, C, st, zero_from, blockIdx.x);
+}
 
 template <class T>
-__global__ void __launch_bounds__(BS) k_agg_prolong(AggDevLevel<T> F, const T* __restrict__ xc, T cs,
-                                                    const PcgState* st) {
+__device__ __forceinline__ void agg_prolong_body(const AggDevLevel<T>& F, const T* __restrict__ xc, T cs,
+                                                 const PcgState* st, unsigned bid) {
   const int status = st ? st->status : ST_RUNNING;
-  const int i = (int)(blockIdx.x * BS + threadIdx.x);
+  const int i = (int)(bid * BS + threadIdx.x);
   if (i >= F.n) return;
   const T v = F.x[i] + cs * xc[F.parent[i]];
   if (status != ST_RUNNING) return;
   F.x[i] = v;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_agg_prolong(AggDevLevel<T> F, const T* __restrict__ xc, T cs,
+                                                    const PcgState* st) {
+  agg_prolong_body<T>(F, xc, cs, st, blockIdx.x);
 }
 
 // Coarsest level: x = G_k b per component block, one wavefront per block.
@@ -1774,12 +1839,12 @@ __device__ __forceinline__ void agg_direct_block(const AggCoarsest& D, int k, in
 // big block each (the four wavefronts split the columns), the following ones
 // four small blocks each (one wavefront per block).
 template <class TI, class TO>
-__global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __restrict__ b, TO* __restrict__ x,
-                                                   double* __restrict__ out64, const PcgState* st) {
+__device__ __forceinline__ void agg_direct_body(const AggCoarsest& D, const TI* __restrict__ b, TO* __restrict__ x,
+                                                double* __restrict__ out64, const PcgState* st, unsigned bid) {
   __shared__ double part[BS / 64];
   const int status = st ? st->status : ST_RUNNING;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int w = (int)blockIdx.x;
+  const int w = (int)bid;
   if (w >= D.big_rows) {
     const int k = (w - D.big_rows) * (BS / 64) + wave;
     if (k < D.nsmall) agg_direct_block<TI, TO>(D, D.small_block[k], lane, b, x, out64, status == ST_RUNNING);
@@ -1804,6 +1869,11 @@ __global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __re
   const int r = D.rows[r0 + row];
   x[r] = (TO)total;
   if (out64) out64[r] = (double)(TO)total;
+}
+template <class TI, class TO>
+__global__ void __launch_bounds__(BS) k_agg_direct(AggCoarsest D, const TI* __restrict__ b, TO* __restrict__ x,
+                                                   double* __restrict__ out64, const PcgState* st) {
+  agg_direct_body<TI, TO>(D, b, x, out64, st, blockIdx.x);
 }
 
 // Set-up of a big coarsest block on the device: in-place blocked Gauss-Jordan
@@ -2096,12 +2166,13 @@ __global__ void __launch_bounds__(BS) k_rb_first(Geo g, const uint8_t* __restric
 // one red, one black; nx even): r and the codes are read as pairs and each
 // colour array is written by whole wavefronts (tiles of 64 pairs x 4 rows).
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_first2(Geo g, const uint8_t* __restrict__ codes,
-                                                  const double* __restrict__ r, FineRB<T> F, const PcgState* st) {
+__device__ __forceinline__ void rb_first2_body(const Geo& g, const uint8_t* __restrict__ codes,
+                                               const double* __restrict__ r, const FineRB<T>& F, const PcgState* st,
+                                               unsigned bid) {
   const int status = st ? st->status : ST_RUNNING;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned gx = (hx + 63) >> 6, gy = (ny + 3) >> 2;
-  const unsigned tile = blockIdx.x, txy = tile % (gx * gy);
+  const unsigned tile = bid, txy = tile % (gx * gy);
   const int i2 = (int)((txy % gx) * 64 + (threadIdx.x & 63));
   const int j = (int)((txy / gx) * 4 + (threadIdx.x >> 6));
   const int k = (int)(tile / (gx * gy));
@@ -2121,16 +2192,22 @@ __global__ void __launch_bounds__(BS) k_rb_first2(Geo g, const uint8_t* __restri
   F.b[F.half + u] = bblack;
   F.x[u] = v;
 }
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first2(Geo g, const uint8_t* __restrict__ codes,
+                                                  const double* __restrict__ r, FineRB<T> F, const PcgState* st) {
+  rb_first2_body<T>(g, codes, r, F, st, blockIdx.x);
+}
 
 // The first red half-sweep (from x = 0) when the right-hand side is already in
 // the colour-separated layout (F.b, where the PCG keeps its residual): x_red =
 // b / diagonal, the arithmetic of k_rb_first2; consecutive threads on
 // consecutive pairs of red entries (nx a multiple of 4).
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_first_lin(Geo g, FineRB<T> F, const PcgState* st) {
+__device__ __forceinline__ void rb_first_lin_body(const Geo& g, const FineRB<T>& F, const PcgState* st, unsigned bid,
+                                                  unsigned nblk) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   const int status = st ? st->status : ST_RUNNING;
-  const unsigned pair = (unsigned)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned pair = (unsigned)xcd_tile(bid, nblk) * BS + threadIdx.x;
   const unsigned u = 2 * pair;
   if (u >= (unsigned)F.half) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
@@ -2148,6 +2225,10 @@ __global__ void __launch_bounds__(BS) k_rb_first_lin(Geo g, FineRB<T> F, const P
   out.x = (ca & FLUID) && da > T(0) ? bv.x / da : T(0);
   out.y = (cb & FLUID) && db > T(0) ? bv.y / db : T(0);
   *reinterpret_cast<T2*>(F.x + u) = out;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first_lin(Geo g, FineRB<T> F, const PcgState* st) {
+  rb_first_lin_body<T>(g, F, st, blockIdx.x, gridDim.x);
 }
 
 // The Gauss-Seidel value of a cell from its code, right-hand side and the six
@@ -2329,11 +2410,12 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
 // colour's array (no idle lanes where a row is not a multiple of the tile
 // width, whole-wave linear streams); each XCD takes a contiguous range of blocks.
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colour, double* __restrict__ out64,
-                                                   const PcgState* st) {
+__device__ __forceinline__ void rb_gs2_lin_body(const Geo& g, const FineRB<T>& F, int colour,
+                                                double* __restrict__ out64, const PcgState* st, unsigned bid,
+                                                unsigned nblk) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   const int status = st ? st->status : ST_RUNNING;
-  const unsigned pair = (unsigned)xcd_tile(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  const unsigned pair = (unsigned)xcd_tile(bid, nblk) * BS + threadIdx.x;
   const unsigned u = 2 * pair;
   if (u >= (unsigned)F.half) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
@@ -2355,6 +2437,11 @@ __global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colou
     out64[c + 2] = (double)vb;
   }
 }
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colour, double* __restrict__ out64,
+                                                   const PcgState* st) {
+  rb_gs2_lin_body<T>(g, F, colour, out64, st, blockIdx.x, gridDim.x);
+}
 
 // The coarse correction of the red cells, x += cs e[parent] (the value that the
 // black half-sweep after the prolongation would otherwise form at each of its
@@ -2362,16 +2449,21 @@ __global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colou
 // scratch by the next red half-sweep, so only that black half-sweep reads these
 // values.
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
-                                                   const T* __restrict__ xc, T cs, const PcgState* st) {
+__device__ __forceinline__ void rb_prolong_body(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
+                                                const T* __restrict__ xc, T cs, const PcgState* st, unsigned bid) {
   const int status = st ? st->status : ST_RUNNING;
-  const long long u = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long u = (long long)bid * BS + threadIdx.x;
   if (u >= half) return;
   const int p = rpar[u];
   if (p < 0 || status != ST_RUNNING) return;
   T v = xr[u];
   v += cs * xc[p];
   xr[u] = v;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
+                                                   const T* __restrict__ xc, T cs, const PcgState* st) {
+  rb_prolong_body<T>(half, xr, rpar, xc, cs, st, blockIdx.x);
 }
 
 // The last post-smoothing half-sweep (colour 0) together with the PCG's r . z
@@ -2511,14 +2603,14 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
 // in order, each XCD taking a contiguous share of them (no idle lanes where a
 // row is not a multiple of the tile width).
 template <class T, bool RES>
-__global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
-                                                      float* __restrict__ out32, const double* __restrict__ r,
-                                                      const PcgState* st, double* part) {
+__device__ __forceinline__ void rb_gs_rz2_lin_body(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
+                                                   float* __restrict__ out32, const double* __restrict__ r,
+                                                   const PcgState* st, double* part, unsigned bid, unsigned nblk) {
   if (!block_running(st)) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
   const unsigned pairs = (unsigned)(F.half / 2);
   double acc = 0.0;
-  const XcdShare sh = xcd_share((pairs + RB - 1) / RB, 1, 0);
+  const XcdShare sh = xcd_share_at(bid, nblk, (pairs + RB - 1) / RB, 1, 0);
   for (unsigned item = sh.first; item < sh.end; item += sh.step) {
     const unsigned pair = item * RB + threadIdx.x;
     if (pair >= pairs) continue;
@@ -2529,7 +2621,13 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, dou
     rz_pair<T, RES>(g, F, out64, out32, r, i2, (int)(row - k * ny), (int)k, u, acc);
   }
   const double s = block_sum(acc);
-  if (threadIdx.x == 0) part[P_RZ + blockIdx.x] = s;
+  if (threadIdx.x == 0) part[P_RZ + bid] = s;
+}
+template <class T, bool RES>
+__global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
+                                                      float* __restrict__ out32, const double* __restrict__ r,
+                                                      const PcgState* st, double* part) {
+  rb_gs_rz2_lin_body<T, RES>(g, F, out64, out32, r, st, part, blockIdx.x, gridDim.x);
 }
 
 // Workgroups of a half-sweep: tiles of 64 x 4 cells of a colour (x, y) per z layer.
@@ -2604,15 +2702,15 @@ __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const
 // 4, 7 (sub = dx + 2 dy + 4 dz); the four red residuals and eight parents meet in
 // LDS and the first wavefront sums each parent's share in sub order.
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const int* __restrict__ fparent,
-                                                     T* __restrict__ bc, T* __restrict__ xc,
-                                                     const T* __restrict__ cdiag, int red_end, const PcgState* st) {
+__device__ __forceinline__ void rb_restrict_body(const Geo& g, const FineRB<T>& F, const int* __restrict__ fparent,
+                                                 T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ cdiag,
+                                                 int red_end, const PcgState* st, unsigned bid) {
   __shared__ T sr[4][64];
   __shared__ int sp[4][64], spb[4][64];
   const int status = st ? st->status : ST_RUNNING;
   const unsigned bx = (unsigned)(g.n[0] + 1) >> 1, by = (unsigned)(g.n[1] + 1) >> 1;
   const unsigned gbx = (bx + 63) >> 6;
-  const unsigned tile = blockIdx.x, rest = tile / gbx;
+  const unsigned tile = bid, rest = tile / gbx;
   const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const unsigned Bx = (tile % gbx) * 64 + lane, By = rest % by, Bz = rest / by;
   const int dy = (int)(w & 1), dz = (int)(w >> 1);
@@ -2658,6 +2756,12 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
     bc[ps[s2]] = sum;
     xc[ps[s2]] = first_relax<T>(ps[s2], red_end, sum, cdiag);
   }
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const int* __restrict__ fparent,
+                                                     T* __restrict__ bc, T* __restrict__ xc,
+                                                     const T* __restrict__ cdiag, int red_end, const PcgState* st) {
+  rb_restrict_body<T>(g, F, fparent, bc, xc, cdiag, red_end, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- multigrid (the reference's cycle)
@@ -3774,6 +3878,15 @@ template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const
                                                double*, bool, int, float*);
 
 template <class T>
+void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
+                       hipStream_t s) {
+  hipLaunchKernelGGL((k_rb_first2<T>), dim3(((rb.hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0,
+                     codes, r, rb, st);
+}
+template void launch_rb_scatter<double>(const Geo&, const uint8_t*, const double*, const FineRB<double>&,
+                                        const PcgState*, hipStream_t);
+
+template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {
   launch_rb_half<T>(g0, rb, colour, (const T*)nullptr, (const int*)nullptr, T(0), (double*)nullptr,
                     (const PcgState*)nullptr, s);
@@ -3981,6 +4094,491 @@ void launch_projection_rhs(const Geo& g, const FaceGeo& fg, const uint8_t* codes
 void launch_project_faces(const Geo& g, const FaceGeo& fg, const uint8_t* codes, const double* phi, double dt, double* u,
                           hipStream_t s) {
   hipLaunchKernelGGL(k_project_faces, dim3(blocks_for(fg.total)), dim3(BS), 0, s, g, fg, codes, phi, dt, u);
+}
+
+// ---------------------------------------------------------------- batched iterations
+// A batched launch carries the arguments of up to kBatchMax single-grid
+// launches in its kernel arguments: item i owns the blocks [first[i],
+// first[i] + nblk) (first[i] a multiple of 8, so a block's XCD, bid & 7, is the
+// one it would have alone) and runs the single-grid kernel's body on them.
+template <class A>
+struct Batch {
+  A a[kBatchMax];
+  unsigned first[kBatchMax];
+  int n;
+};
+template <class A>
+__device__ __forceinline__ const A& batch_item(const Batch<A>& B, unsigned& bid) {
+  int i = 0;
+#pragma unroll
+  for (int k = 1; k < kBatchMax; ++k)
+    if (k < B.n && blockIdx.x >= B.first[k]) i = k;
+  bid = blockIdx.x - B.first[i];
+  return B.a[i];
+}
+
+template <class T>
+struct BFirstLin {
+  Geo g;
+  FineRB<T> F;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BFirst2 {
+  Geo g;
+  const uint8_t* codes;
+  const double* r;
+  FineRB<T> F;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BHalf {
+  Geo g;
+  FineRB<T> F;
+  int colour;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BFineRestrict {
+  Geo g;
+  FineRB<T> F;
+  const int* fparent;
+  T* bc;
+  T* xc;
+  const T* cdiag;
+  int red_end;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BAggGs {
+  AggDevLevel<T> L;
+  int colour;
+  const PcgState* st;
+  const T* xc;
+  T cs;
+  unsigned nblk;
+};
+template <class T>
+struct BAggRestrict {
+  AggDevLevel<T> F, C;
+  const PcgState* st;
+  int zero_from;
+  unsigned nblk;
+};
+template <class T>
+struct BAggProlong {
+  AggDevLevel<T> F;
+  const T* xc;
+  T cs;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BDirect {
+  AggCoarsest D;
+  const T* b;
+  T* x;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BRbProlong {
+  long long half;
+  T* xr;
+  const int* rpar;
+  const T* xc;
+  T cs;
+  const PcgState* st;
+  unsigned nblk;
+};
+template <class T>
+struct BRz {
+  Geo g;
+  FineRB<T> F;
+  float* out32;
+  const double* r;
+  const PcgState* st;
+  double* part;
+  unsigned nblk;
+};
+template <class TZ>
+struct BOp {
+  Geo g;
+  MarchShape ms;
+  const uint8_t* codes;
+  const TZ* z;
+  const double* pold;
+  double* pnew;
+  double* ap;
+  PcgState* st;
+  double* part;
+  long long zhalf;
+  unsigned nblk;
+};
+struct BAxpy {
+  long long N;
+  int nx, ny;
+  unsigned long long mnx, mny;
+  const double* p;
+  const double* ap;
+  double* x;
+  double* r;
+  PcgState* st;
+  double* part;
+  unsigned nblk;
+};
+struct BTrue {
+  Geo g;
+  const uint8_t* codes;
+  const double* x;
+  const double* bc;
+  PcgState* st;
+  double* part;
+  unsigned nblk;
+};
+struct BState {
+  const PcgState* src;
+  PcgState* dst;
+  unsigned nblk;
+};
+
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_first_lin(Batch<BFirstLin<T>> B) {
+  unsigned bid;
+  const BFirstLin<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_first_lin_body<T>(a.g, a.F, a.st, bid, a.nblk);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_first2(Batch<BFirst2<T>> B) {
+  unsigned bid;
+  const BFirst2<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_first2_body<T>(a.g, a.codes, a.r, a.F, a.st, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_gs2_lin(Batch<BHalf<T>> B) {
+  unsigned bid;
+  const BHalf<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_gs2_lin_body<T>(a.g, a.F, a.colour, (double*)nullptr, a.st, bid, a.nblk);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_restrict(Batch<BFineRestrict<T>> B) {
+  unsigned bid;
+  const BFineRestrict<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_restrict_body<T>(a.g, a.F, a.fparent, a.bc, a.xc, a.cdiag, a.red_end, a.st, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_agg_gs(Batch<BAggGs<T>> B) {
+  unsigned bid;
+  const BAggGs<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  agg_gs_body<T>(a.L, a.colour, a.st, a.xc, a.cs, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(kRestrictThreads) kb_agg_restrict(Batch<BAggRestrict<T>> B) {
+  unsigned bid;
+  const BAggRestrict<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  agg_restrict_body<T>(a.F, a.C, a.st, a.zero_from, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_agg_prolong(Batch<BAggProlong<T>> B) {
+  unsigned bid;
+  const BAggProlong<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  agg_prolong_body<T>(a.F, a.xc, a.cs, a.st, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_agg_direct(Batch<BDirect<T>> B) {
+  unsigned bid;
+  const BDirect<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  agg_direct_body<T, T>(a.D, a.b, a.x, (double*)nullptr, a.st, bid);
+}
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_prolong(Batch<BRbProlong<T>> B) {
+  unsigned bid;
+  const BRbProlong<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_prolong_body<T>(a.half, a.xr, a.rpar, a.xc, a.cs, a.st, bid);
+}
+template <class T, bool RES>
+__global__ void __launch_bounds__(RB, 8) kb_rb_gs_rz2_lin(Batch<BRz<T>> B) {
+  unsigned bid;
+  const BRz<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_gs_rz2_lin_body<T, RES>(a.g, a.F, (double*)nullptr, a.out32, a.r, a.st, a.part, bid, a.nblk);
+}
+template <class TZ, bool RES>
+__global__ void __launch_bounds__(RB, 8) kb_pcg_op(Batch<BOp<TZ>> B) {
+  unsigned bid;
+  const BOp<TZ>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  pcg_op_body<false, RES, TZ>(a.g, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr, (const int*)nullptr,
+                              a.st, a.part, a.zhalf, bid, a.nblk);
+}
+template <class TZ, bool RES>
+__global__ void __launch_bounds__(RB, 8) kb_pcg_op_march(Batch<BOp<TZ>> B) {
+  unsigned bid;
+  const BOp<TZ>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  pcg_op_march_body<false, RES, TZ>(a.g, a.ms, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr,
+                                    (const int*)nullptr, a.st, a.part, a.zhalf, bid, a.nblk);
+}
+__global__ void __launch_bounds__(RB) kb_pcg_axpy(Batch<BAxpy> B) {
+  unsigned bid;
+  const BAxpy& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  pcg_axpy_body(a.N, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
+}
+__global__ void __launch_bounds__(RB) kb_pcg_axpy_rb(Batch<BAxpy> B) {
+  unsigned bid;
+  const BAxpy& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  pcg_axpy_rb_body(a.N, a.nx, a.ny, a.mnx, a.mny, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
+}
+__global__ void __launch_bounds__(RB, 8) kb_pcg_true(Batch<BTrue> B) {
+  unsigned bid;
+  const BTrue& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  pcg_true_body(a.g, a.codes, a.x, a.bc, a.st, a.part, bid, a.nblk);
+}
+__global__ void __launch_bounds__(RB) kb_pcg_fin(Batch<BTrue> B) {
+  unsigned bid;
+  const BTrue& a = batch_item(B, bid);
+  if (bid != 0) return;
+  pcg_fin_body(a.st, a.part);
+}
+// One wavefront per state: the words of the PcgState copied in order.
+__global__ void __launch_bounds__(64) kb_states(Batch<BState> B) {
+  unsigned bid;
+  const BState& a = batch_item(B, bid);
+  if (bid != 0) return;
+  static_assert(sizeof(PcgState) % 8 == 0, "copied as 64-bit words");
+  const unsigned long long* src = reinterpret_cast<const unsigned long long*>(a.src);
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(a.dst);
+  for (unsigned w = threadIdx.x; w < sizeof(PcgState) / 8; w += 64) dst[w] = src[w];
+}
+
+// The host side of a batched launch: items whose block count is zero are left
+// out, and nothing is launched when no item is left.
+template <class A>
+struct Batcher {
+  Batch<A> B;
+  unsigned total = 0;
+  Batcher() { B.n = 0; }
+  void add(const A& a) {
+    if (a.nblk == 0) return;
+    B.a[B.n] = a;
+    B.first[B.n] = total;
+    total += (a.nblk + 7u) & ~7u;
+    ++B.n;
+  }
+};
+static_assert(sizeof(Batch<BAggRestrict<double>>) <= 4096, "kernel arguments of a batched launch");
+#define BATCH_LAUNCH(K, BT, THREADS, S)                                               \
+  do {                                                                                \
+    if ((BT).B.n > 0) hipLaunchKernelGGL(K, dim3((BT).total), dim3(THREADS), 0, S, (BT).B); \
+  } while (0)
+
+template <class T>
+void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
+  constexpr bool MIXED = std::is_same<T, float>::value;
+  if (n <= 0 || n > kBatchMax) return;
+  auto pair_blocks = [](const FineRB<T>& F) { return (unsigned)((F.half / 2 + BS - 1) / BS); };
+  const int fr = it[0].fine_rounds;
+  // the first fine half-sweep from a zero iterate
+  if constexpr (MIXED) {
+    Batcher<BFirst2<T>> L;
+    for (int i = 0; i < n; ++i) {
+      const Geo& g = it[i].g;
+      L.add({g, it[i].codes, it[i].r, it[i].F, it[i].st,
+             (unsigned)(((it[i].F.hx + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[2])});
+    }
+    BATCH_LAUNCH(kb_rb_first2<T>, L, BS, s);
+  } else {
+    Batcher<BFirstLin<T>> L;
+    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].st, pair_blocks(it[i].F)});
+    BATCH_LAUNCH(kb_rb_first_lin<T>, L, BS, s);
+  }
+  for (int pass = 1; pass < 2 * fr; ++pass) {
+    Batcher<BHalf<T>> L;
+    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, pass & 1, it[i].st, pair_blocks(it[i].F)});
+    BATCH_LAUNCH(kb_rb_gs2_lin<T>, L, BS, s);
+  }
+  {
+    Batcher<BFineRestrict<T>> L;
+    for (int i = 0; i < n; ++i) {
+      const Geo& g = it[i].g;
+      const AggDevLevel<T>& L0 = it[i].lv[0];
+      L.add({g, it[i].F, it[i].fparent, L0.b, L0.x, L0.diag, L0.cptr[1], it[i].st,
+             (unsigned)((((g.n[0] + 1) / 2 + 63) / 64) * ((g.n[1] + 1) / 2) * ((g.n[2] + 1) / 2))});
+    }
+    BATCH_LAUNCH(kb_rb_restrict<T>, L, BS, s);
+  }
+  int max_stop = 0;
+  for (int i = 0; i < n; ++i) max_stop = std::max(max_stop, it[i].nlev - 1);
+  auto rows_of = [](const AggDevLevel<T>& L, int c) { return L.cptr[c + 1] - L.cptr[c]; };
+  // coarse pre-smoothing: each level's colour classes in ascending order per
+  // round (the first class of the first round was relaxed by the restriction),
+  // step k of every solve's sequence in one launch; then the restrictions
+  for (int l = 0; l < max_stop; ++l) {
+    std::vector<int> seq[kBatchMax];
+    size_t steps = 0;
+    for (int i = 0; i < n; ++i) {
+      if (l >= it[i].nlev - 1) continue;
+      const AggDevLevel<T>& L = it[i].lv[l];
+      for (int round = 0; round < it[i].coarse_rounds; ++round)
+        for (int c = round == 0 ? 1 : 0; c < L.ncol; ++c) seq[i].push_back(c);
+      steps = std::max(steps, seq[i].size());
+    }
+    for (size_t k = 0; k < steps; ++k) {
+      Batcher<BAggGs<T>> L;
+      for (int i = 0; i < n; ++i) {
+        if (k >= seq[i].size()) continue;
+        const AggDevLevel<T>& Lv = it[i].lv[l];
+        const int c = seq[i][k];
+        L.add({Lv, c, it[i].st, (const T*)nullptr, T(0), blocks_for(rows_of(Lv, c))});
+      }
+      BATCH_LAUNCH(kb_agg_gs<T>, L, BS, s);
+    }
+    Batcher<BAggRestrict<T>> R;
+    for (int i = 0; i < n; ++i) {
+      if (l >= it[i].nlev - 1) continue;
+      const AggDevLevel<T>& Lv = it[i].lv[l];
+      const int zero_from = it[i].coarse_rounds >= 1 && it[i].zero_last ? Lv.cptr[Lv.ncol - 1] : Lv.n;
+      R.add({Lv, it[i].lv[l + 1], it[i].st, zero_from, (unsigned)((it[i].lv[l + 1].n + 63) / 64)});
+    }
+    BATCH_LAUNCH(kb_agg_restrict<T>, R, kRestrictThreads, s);
+  }
+  {
+    Batcher<BDirect<T>> L;
+    for (int i = 0; i < n; ++i) {
+      const AggCoarsest& D = it[i].D;
+      const AggDevLevel<T>& C = it[i].lv[it[i].nlev - 1];
+      L.add({D, C.b, C.x, it[i].st, (unsigned)(D.big_rows + (D.nsmall + 3) / 4)});
+    }
+    BATCH_LAUNCH(kb_agg_direct<T>, L, BS, s);
+  }
+  // coarse post-smoothing: the prolongation (riding on the first half-sweep of a
+  // two-colour level), then the classes in the adjoint order
+  struct Op {
+    int kind;  // 0 half-sweep, 1 half-sweep with the prolongation, 2 prolongation alone
+    int colour;
+  };
+  for (int l = max_stop - 1; l >= 0; --l) {
+    std::vector<Op> seq[kBatchMax];
+    size_t steps = 0;
+    for (int i = 0; i < n; ++i) {
+      if (l >= it[i].nlev - 1) continue;
+      const AggDevLevel<T>& L = it[i].lv[l];
+      const int cr = it[i].coarse_rounds;
+      if (L.ncol == 2) {
+        seq[i].push_back(rows_of(L, 1) > 0 ? Op{1, 1} : Op{2, 0});
+        for (int pass = 1; pass < 2 * cr; ++pass) seq[i].push_back(Op{0, (pass + 1) & 1});
+      } else {
+        seq[i].push_back(Op{2, 0});
+        for (int round = 0; round < cr; ++round)
+          for (int c = L.ncol - 1; c >= 0; --c) seq[i].push_back(Op{0, c});
+      }
+      steps = std::max(steps, seq[i].size());
+    }
+    for (size_t k = 0; k < steps; ++k) {
+      Batcher<BAggGs<T>> G;
+      Batcher<BAggProlong<T>> P;
+      for (int i = 0; i < n; ++i) {
+        if (k >= seq[i].size()) continue;
+        const AggDevLevel<T>& Lv = it[i].lv[l];
+        const T* xc = it[i].lv[l + 1].x;
+        const Op op = seq[i][k];
+        if (op.kind == 2)
+          P.add({Lv, xc, it[i].cs, it[i].st, blocks_for(Lv.n)});
+        else
+          G.add({Lv, op.colour, it[i].st, op.kind == 1 ? xc : (const T*)nullptr, op.kind == 1 ? it[i].cs : T(0),
+                 blocks_for(rows_of(Lv, op.colour))});
+      }
+      BATCH_LAUNCH(kb_agg_prolong<T>, P, BS, s);
+      BATCH_LAUNCH(kb_agg_gs<T>, G, BS, s);
+    }
+  }
+  // fine post-smoothing: the correction added to the red cells, black, red, ...,
+  // and the last red half-sweep with the partials of r . z
+  {
+    Batcher<BRbProlong<T>> L;
+    for (int i = 0; i < n; ++i)
+      L.add({it[i].F.half, it[i].F.x, it[i].F.rpar, (const T*)it[i].lv[0].x, it[i].cs, it[i].st,
+             blocks_for(it[i].F.half)});
+    BATCH_LAUNCH(kb_rb_prolong<T>, L, BS, s);
+  }
+  for (int pass = 0; pass + 1 < 2 * fr; ++pass) {
+    Batcher<BHalf<T>> L;
+    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, (pass + 1) & 1, it[i].st, pair_blocks(it[i].F)});
+    BATCH_LAUNCH(kb_rb_gs2_lin<T>, L, BS, s);
+  }
+  {
+    Batcher<BRz<T>> L;
+    for (int i = 0; i < n; ++i)
+      L.add({it[i].g, it[i].F, MIXED ? it[i].z32 : (float*)nullptr, MIXED ? (const double*)it[i].r : nullptr,
+             it[i].st, it[i].part, (unsigned)RP});
+    if constexpr (MIXED)
+      BATCH_LAUNCH((kb_rb_gs_rz2_lin<T, false>), L, RB, s);
+    else
+      BATCH_LAUNCH((kb_rb_gs_rz2_lin<T, true>), L, RB, s);
+  }
+  // p = z + beta p, A p, p . Ap
+  {
+    using TZ = std::conditional_t<MIXED, float, double>;
+    Batcher<BOp<TZ>> C, M;
+    for (int i = 0; i < n; ++i) {
+      const TZ* z = MIXED ? (const TZ*)it[i].z32 : (const TZ*)it[i].F.x;
+      const BOp<TZ> a{it[i].g, march_shape(it[i].g), it[i].codes, z, it[i].pold, it[i].pnew, it[i].ap, it[i].st,
+                      it[i].part, MIXED ? 0LL : it[i].F.half, (unsigned)RP};
+      (it[i].march ? M : C).add(a);
+    }
+    BATCH_LAUNCH((kb_pcg_op<TZ, !MIXED>), C, RB, s);
+    BATCH_LAUNCH((kb_pcg_op_march<TZ, !MIXED>), M, RB, s);
+  }
+  // alpha, x += alpha p, r -= alpha A p, |r|^2
+  {
+    Batcher<BAxpy> L;
+    const unsigned long long two40 = 1ULL << 40;
+    for (int i = 0; i < n; ++i) {
+      const Geo& g = it[i].g;
+      const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
+      const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
+      L.add({g.N, g.n[0], g.n[1], mnx, mny, it[i].pnew, it[i].ap, it[i].x, MIXED ? it[i].r : (double*)it[i].F.b,
+             it[i].st, it[i].part, (unsigned)RP});
+    }
+    if constexpr (MIXED)
+      BATCH_LAUNCH(kb_pcg_axpy, L, RB, s);
+    else
+      BATCH_LAUNCH(kb_pcg_axpy_rb, L, RB, s);
+  }
+  // the recomputed residual and the iteration's decisions
+  {
+    Batcher<BTrue> L, F1;
+    for (int i = 0; i < n; ++i) {
+      L.add({it[i].g, it[i].codes, it[i].x, it[i].bc, it[i].st, it[i].part, (unsigned)RP});
+      F1.add({it[i].g, it[i].codes, it[i].x, it[i].bc, it[i].st, it[i].part, 1u});
+    }
+    BATCH_LAUNCH(kb_pcg_true, L, RB, s);
+    BATCH_LAUNCH(kb_pcg_fin, F1, RB, s);
+  }
+}
+template void launch_batch_iteration<double>(const BatchSolve<double>*, int, hipStream_t);
+template void launch_batch_iteration<float>(const BatchSolve<float>*, int, hipStream_t);
+
+void launch_batch_states(PcgState* const* st, int n, PcgState* dst, hipStream_t s) {
+  Batcher<BState> L;
+  for (int i = 0; i < n && i < kBatchMax; ++i) L.add({st[i], dst + i, 1u});
+  BATCH_LAUNCH(kb_states, L, 64, s);
 }
 
 }  // namespace cfdgpu

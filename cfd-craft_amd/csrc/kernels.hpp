@@ -137,6 +137,44 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 // One colour-separated fine half-sweep (timing hook).
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
+
+// Batched iterations: one PCG iteration (red-black V(2,2) preconditioner led,
+// then p, A p, the x/r update and the convergence decision) of up to
+// kBatchMax independent solves, each on its own grid, issued as one launch per
+// stage of the iteration; every launch gives each solve the blocks its own
+// single-grid launch would have (from a multiple of 8, so XCD placement is
+// unchanged), so each solve computes exactly what it computes alone. T double:
+// the FP64 solve with r and z in the fine level's colour-separated arrays; T
+// float: the mixed-precision solve (FP32 cycle, r natural, z as FP32 values in
+// z32). The fine level takes two cells per thread (nx a multiple of 4).
+constexpr int kBatchMax = 8;
+template <class T>
+struct BatchSolve {
+  Geo g;
+  const uint8_t* codes;
+  FineRB<T> F;
+  const int* fparent;
+  int nlev;                   // coarse levels, the last solved by the dense blocks
+  const AggDevLevel<T>* lv;   // host array of nlev levels
+  AggCoarsest D;
+  PcgState* st;
+  double* part;
+  double *x, *r, *bc, *ap, *pold, *pnew;
+  float* z32;
+  T cs;             // coarse correction scale
+  bool zero_last;   // the restriction takes the class relaxed last as zero residual
+  int fine_rounds, coarse_rounds;
+  bool march;       // the operator marches through z
+};
+template <class T>
+void launch_batch_iteration(const BatchSolve<T>* items, int n, hipStream_t s);
+// r (natural layout) scattered into the fine level's colour-separated b arrays,
+// and the red x set to b / a_ii (the first half-sweep's values).
+template <class T>
+void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
+                       hipStream_t s);
+// The PcgState of each solve copied to dst[i] (device memory).
+void launch_batch_states(PcgState* const* st, int n, PcgState* dst, hipStream_t s);
 // The reference's cycle: fine level from the codes (diagonal diag0 in the
 // reference's summation order), coarse CSR levels in the reference's numbering.
 // Block smoother of one level of the reference cycle (multigrid/line.rs): lines

@@ -368,6 +368,14 @@ int cfd_profile_solve(cfd_grid* g, const double* d_b, const double* d_scale, dou
 typedef struct cfd_server cfd_server;
 /* hbm_budget_bytes 0: 90 % of the free device memory. */
 int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out);
+/* Batching: requests on red-black multigrid grids without closed components
+ * (nx a multiple of 4) run in lanes, each a stream running a batch of up to 8
+ * solves per precision with one launch per stage of a PCG iteration for the
+ * whole batch; lanes run concurrently. Each solve's result is bitwise the one
+ * its grid gives alone. lanes 0: every request on its grid's own stream; the
+ * default is 2. Set before the first request. Grids admitted while lanes > 0
+ * take two fine cells per thread (the batched layout; the same values). */
+int cfd_server_set_batching(cfd_server* s, int lanes);
 void cfd_server_destroy(cfd_server* s);
 /* cfd_grid_create under the budget; CFD_ENOMEM (nothing allocated) when it does not fit. */
 int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out);
@@ -377,7 +385,8 @@ size_t cfd_grid_footprint(const cfd_grid* g);
 /* Queues a solve of grid g (device pointers); precision CFD_PRECISION_FP64 or MIXED. */
 int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const double* d_scale, double tol,
                       int64_t max_iterations, int precision, double* d_x, uint64_t* ticket);
-/* Advances every running request without blocking; returns the number running or queued. */
+/* Advances every running request without blocking; returns the number running or queued
+ * (a negative CFD error code when the device failed). */
 int cfd_server_progress(cfd_server* s);
 int cfd_server_wait(cfd_server* s, uint64_t ticket);
 int cfd_server_drain(cfd_server* s);
