@@ -412,7 +412,7 @@ def lid_cavity(steps=5):
     return out
 
 
-def served(requests=512, n_grids=16, lanes=2, seed=4):
+def served(requests=512, n_grids=16, lanes=3, seed=4):
     """Served mix: resident grids of 64^3 .. 192x192x96 cells (seeded boxes, 5-25 %
     solid), tolerances 1e-6/1e-8/1e-10. The same request list runs (1) plain:
     one FP64 solve after another on the caller's thread, on grids built alone

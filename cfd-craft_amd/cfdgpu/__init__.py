@@ -684,7 +684,7 @@ class Server:
     (one launch per iteration stage for the batch); 0: each request on its
     grid's own stream."""
 
-    def __init__(self, max_active: int = 8, hbm_budget_bytes: int = 0, lanes: int = 2):
+    def __init__(self, max_active: int = 8, hbm_budget_bytes: int = 0, lanes: int = 3):
         h = ctypes.c_void_p()
         _check(lib().cfd_server_create(int(max_active), int(hbm_budget_bytes), ctypes.byref(h)))
         self.h = h

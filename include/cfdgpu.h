@@ -373,7 +373,7 @@ int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out)
  * solves per precision with one launch per stage of a PCG iteration for the
  * whole batch; lanes run concurrently. Each solve's result is bitwise the one
  * its grid gives alone. lanes 0: every request on its grid's own stream; the
- * default is 2. Set before the first request. Grids admitted while lanes > 0
+ * default is 3. Set before the first request. Grids admitted while lanes > 0
  * take two fine cells per thread (the batched layout; the same values). */
 int cfd_server_set_batching(cfd_server* s, int lanes);
 void cfd_server_destroy(cfd_server* s);
