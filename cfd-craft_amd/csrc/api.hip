@@ -1066,8 +1066,17 @@ bool resident32(const cfd_grid* g) {
   return g->rb_resident && rz_fused(g) && g->opt.precision == CFD_PRECISION_MIXED && g->ash.rb32.two_cells >= 1 &&
          g->geo.n[0] % 4 == 0 && g->z32.p;
 }
+// ... and with consecutive pairs of cells per thread it keeps its FP64 residual
+// in the FP64 level's colour-separated b arrays, z in the FP32 level's x arrays
+// and p in FP32 (no natural-layout r, z or FP64 p per iteration).
+bool resident_mixed(const cfd_grid* g) {
+  return resident32(g) && g->ash.rb32.two_cells == 2 && g->geo.N < (1LL << 31) &&
+         (double)g->geo.N * g->geo.n[0] < 1099511627776.0;
+}
 // The mode enqueue_precondition takes in a solve (job set-up: first).
-int resident_mode(const cfd_grid* g, bool first) { return resident(g) ? (first ? 1 : 2) : resident32(g) ? 3 : 0; }
+int resident_mode(const cfd_grid* g, bool first) {
+  return resident(g) ? (first ? 1 : 2) : resident_mixed(g) ? 4 : resident32(g) ? 3 : 0;
+}
 
 // z = M r. resident (see resident()): 0 r and z natural; 1 r natural, copied
 // into the fine level's right-hand side, z left in its x; 2 r already there;
@@ -1095,10 +1104,10 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st,
     const double* rz_r = rz_fused(g) ? r : nullptr;
     if (g->opt.precision == CFD_PRECISION_MIXED)
       launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
-                                       g->a32.levels.data(), A.D, resident == 3 ? nullptr : z, st, g->omega,
+                                       g->a32.levels.data(), A.D, resident >= 3 ? nullptr : z, st, g->omega,
                                        g->stream, A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
-                                       rz_r, g->part.p, g->zero_last, resident == 3 ? 3 : 0,
-                                       resident == 3 ? g->z32.p : nullptr);
+                                       resident == 4 ? A.rb64.b : rz_r, g->part.p, g->zero_last,
+                                       resident >= 3 ? resident : 0, resident == 3 ? g->z32.p : nullptr);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
@@ -1138,13 +1147,19 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
   double* pnew = parity ? g->p0.p : g->p1.p;
   const double* mz = g->mean_z.p;
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
-  const bool res = resident(g), res32 = resident32(g);
+  const bool res = resident(g), res32 = resident32(g), resm = resident_mixed(g);
   if (precondition) {
     enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, false));
     enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
     if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
   }
-  if (res) {
+  if (resm) {
+    // p in FP32 in the space of the FP64 vectors p0 / p1
+    const FineRB<float>& F = g->ash.rb32;
+    launch_pcg_op_mixed(g->geo, g->codes.p, F.x, F.half, (const float*)pold, (float*)pnew, g->ap.p, st, g->part.p,
+                        g->stream, g->march);
+    launch_pcg_axpy_rb(g->geo, (const float*)pnew, g->ap.p, g->x.p, g->ash.rb64.b, st, g->part.p, g->stream);
+  } else if (res) {
     const FineRB<double>& F = g->ash.rb64;
     launch_pcg_op(g->geo, g->codes.p, F.x, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream, g->march,
                   F.half);
@@ -1302,6 +1317,7 @@ int job_begin(cfd_grid* g, const double* d_b, const double* d_scale, double tol,
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, st, g->stream);
   if (t_probe) t_probe->tag = 0;
+  if (resident_mixed(g)) launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, st, g->stream);
   enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, true));
   enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
   if (!rz_fused(g))
@@ -2216,7 +2232,7 @@ bool batchable(const cfd_grid* g, int precision) {
     return false;
   if (g->geo.n[0] % 4 != 0 || g->geo.N >= (1LL << 31) || (double)g->geo.N * g->geo.n[0] >= 1099511627776.0)
     return false;
-  if (precision == CFD_PRECISION_MIXED) return A.rb32.two_cells == 2 && g->z32.p;
+  if (precision == CFD_PRECISION_MIXED) return A.rb32.two_cells == 2 && g->rb_resident;
   return A.rb64.two_cells == 2;
 }
 
@@ -2250,13 +2266,12 @@ BatchSolve<T> batch_solve(cfd_grid* g, const FineRB<T>& F, const std::vector<Agg
   b.st = g->st.p;
   b.part = g->part.p;
   b.x = g->x.p;
-  b.r = g->r.p;
   b.bc = g->bc.p;
   b.ap = g->ap.p;
   const bool odd = enq & 1;
   b.pold = odd ? g->p1.p : g->p0.p;
   b.pnew = odd ? g->p0.p : g->p1.p;
-  b.z32 = g->z32.p;
+  b.rres = A.rb64.b;
   b.cs = (T)g->omega;
   b.zero_last = g->zero_last;
   b.fine_rounds = g->fine_rounds;
@@ -2267,9 +2282,9 @@ BatchSolve<T> batch_solve(cfd_grid* g, const FineRB<T>& F, const std::vector<Agg
 }
 
 // A request joins a batch of its precision in lane `lane`: the set-up kernels
-// of solve_scaled (pressure.rs:342-375) on the lane's stream, and for the FP64
-// solve the residual scattered into the fine level's colour-separated arrays,
-// where the batched iterations keep it. Its first batched iteration applies the
+// of solve_scaled (pressure.rs:342-375) on the lane's stream, and the residual
+// scattered into the FP64 fine level's colour-separated arrays, where the
+// batched iterations keep it. Its first batched iteration applies the
 // preconditioner to that residual.
 int batch_begin(cfd_server* s, int lane, cfd_server::Req& R) {
   cfd_grid* g = R.g;
@@ -2280,7 +2295,7 @@ int batch_begin(cfd_server* s, int lane, cfd_server::Req& R) {
   launch_pcg_init(g->geo.N, g->codes.p, R.b, g->mean_b.p, nullptr, g->r.p, g->bc.p, g->x.p, sp, g->part.p, st);
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, sp, st);
-  if (R.precision == CFD_PRECISION_FP64) launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, sp, st);
+  launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, sp, st);
   R.enq = 0;
   R.lane = lane;
   return 0;

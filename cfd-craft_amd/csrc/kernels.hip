@@ -237,18 +237,21 @@ __device__ __forceinline__ void for_cells(const Geo& g, F f) {
 // of its colour's half, nx even), zhalf entries per colour. Every neighbour of
 // a cell has the other colour (periodic axes have even lengths there), except
 // the cell itself standing in at a closed boundary.
-template <bool HAS_CLOSED, bool FIRST, bool RES = false, class TZ = double>
+// TP: the type p is stored in (float in the mixed-precision solve: each p value
+// is rounded once, and A p and p . Ap are formed from the rounded values, so x
+// += alpha p and r -= alpha A p stay consistent).
+template <bool HAS_CLOSED, bool FIRST, bool RES = false, class TZ = double, class TP = double>
 __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __restrict__ codes,
-                                               const TZ* __restrict__ z, const double* __restrict__ pold,
-                                               double* __restrict__ pnew, double* __restrict__ ap,
+                                               const TZ* __restrict__ z, const TP* __restrict__ pold,
+                                               TP* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
                                                double beta, long long zhalf, unsigned bid, unsigned nblk) {
   auto pv = [&](long long i, int colour) {
     double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
     // closed components: z centered (without any, the code load is skipped)
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
-    if (FIRST) return zc;
-    return zc + beta * pold[i];
+    if (FIRST) return (double)(TP)zc;
+    return (double)(TP)(zc + beta * (double)pold[i]);
   };
   double acc = 0.0;
   for_cells_at(g, bid, nblk, [&](long long c, const int q[3]) {
@@ -257,7 +260,7 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
     uint8_t code = codes[c];
     double pc = pvc(c);
     double y = apply_cell(g, code, c, q, pvc);
-    pnew[c] = pc;
+    pnew[c] = (TP)pc;
     ap[c] = y;
     acc += pc * y;
   });
@@ -267,9 +270,9 @@ __device__ __forceinline__ double pcg_op_cells(const Geo& g, const uint8_t* __re
 // p_new = z_c + beta p (p_new = z_c on the first iteration) with beta = rz_next / rz,
 // Ap = A p_new, and per-block partials of p_new . Ap (pressure.rs:391-393,426-432).
 // (registers sized for 8 waves per SIMD: two blocks per CU)
-template <bool HAS_CLOSED, bool RES, class TZ>
+template <bool HAS_CLOSED, bool RES, class TZ, class TP = double>
 __device__ __forceinline__ void pcg_op_body(const Geo& g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
-                                            const double* __restrict__ pold, double* __restrict__ pnew,
+                                            const TP* __restrict__ pold, TP* __restrict__ pnew,
                                             double* __restrict__ ap, const double* __restrict__ mean,
                                             const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf,
                                             unsigned bid, unsigned nblk) {
@@ -278,20 +281,23 @@ __device__ __forceinline__ void pcg_op_body(const Geo& g, const uint8_t* __restr
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_cells<HAS_CLOSED, true, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid, nblk)
-            : pcg_op_cells<HAS_CLOSED, false, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid, nblk);
+      first ? pcg_op_cells<HAS_CLOSED, true, RES, TZ, TP>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid,
+                                                          nblk)
+            : pcg_op_cells<HAS_CLOSED, false, RES, TZ, TP>(g, codes, z, pold, pnew, ap, mean, cidx, beta, zhalf, bid,
+                                                           nblk);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + bid] = s;
     if (bid == 0 && !first) st->beta = beta;
   }
 }
-template <bool HAS_CLOSED, bool RES, class TZ = double>
+template <bool HAS_CLOSED, bool RES, class TZ = double, class TP = double>
 __global__ void __launch_bounds__(RB, 8)
-    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z, const double* __restrict__ pold,
-             double* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
+    k_pcg_op(Geo g, const uint8_t* __restrict__ codes, const TZ* __restrict__ z, const TP* __restrict__ pold,
+             TP* __restrict__ pnew, double* __restrict__ ap, const double* __restrict__ mean,
              const int* __restrict__ cidx, PcgState* st, double* part, long long zhalf) {
-  pcg_op_body<HAS_CLOSED, RES, TZ>(g, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x, gridDim.x);
+  pcg_op_body<HAS_CLOSED, RES, TZ, TP>(g, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x,
+                                       gridDim.x);
 }
 
 // The same kernel for large grids, marching through z: work items are tiles of
@@ -311,10 +317,10 @@ inline MarchShape march_shape(const Geo& g) {
   return MarchShape{(g.n[0] + MX - 1) / MX, (g.n[1] + MY - 1) / MY, (g.n[2] + MZ - 1) / MZ};
 }
 
-template <bool HAS_CLOSED, bool FIRST, bool RES, class TZ>
+template <bool HAS_CLOSED, bool FIRST, bool RES, class TZ, class TP>
 __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
-                                               const TZ* __restrict__ z, const double* __restrict__ pold,
-                                               double* __restrict__ pnew, double* __restrict__ ap,
+                                               const TZ* __restrict__ z, const TP* __restrict__ pold,
+                                               TP* __restrict__ pnew, double* __restrict__ ap,
                                                const double* __restrict__ mean, const int* __restrict__ cidx,
                                                double beta, double (*tile)[MY + 2][MX + 2], long long zhalf,
                                                unsigned bid, unsigned nblk) {
@@ -322,8 +328,8 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   auto pv = [&](long long i, int colour) {
     double zc = RES ? (double)z[(colour ? zhalf : 0) + (i >> 1)] : (double)z[i];
     if (HAS_CLOSED && (codes[i] & CLOSED)) zc -= mean[cidx ? cidx[i] : 0];
-    if (FIRST) return zc;
-    return zc + beta * pold[i];
+    if (FIRST) return (double)(TP)zc;
+    return (double)(TP)(zc + beta * (double)pold[i]);
   };
   const int lx = (int)threadIdx.x % MX, ly = (int)threadIdx.x / MX;
   const unsigned items = (unsigned)(ms.tx * ms.ty * ms.tz);
@@ -429,7 +435,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
         nv[4] = pm;
         nv[5] = pn;
         const double y = apply_vals(g, code, q, pc, nv);
-        pnew[c] = pc;
+        pnew[c] = (TP)pc;
         ap[c] = y;
         acc += pc * y;
       }
@@ -452,10 +458,10 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
   return acc;
 }
 
-template <bool HAS_CLOSED, bool RES, class TZ>
+template <bool HAS_CLOSED, bool RES, class TZ, class TP = double>
 __device__ __forceinline__ void pcg_op_march_body(const Geo& g, MarchShape ms, const uint8_t* __restrict__ codes,
-                                                  const TZ* __restrict__ z, const double* __restrict__ pold,
-                                                  double* __restrict__ pnew, double* __restrict__ ap,
+                                                  const TZ* __restrict__ z, const TP* __restrict__ pold,
+                                                  TP* __restrict__ pnew, double* __restrict__ ap,
                                                   const double* __restrict__ mean, const int* __restrict__ cidx,
                                                   PcgState* st, double* part, long long zhalf, unsigned bid,
                                                   unsigned nblk) {
@@ -465,29 +471,30 @@ __device__ __forceinline__ void pcg_op_march_body(const Geo& g, MarchShape ms, c
   double beta = 0.0;
   if (!first) beta = reduce_part(part + P_RZ, RP) / st->rz;
   const double acc =
-      first ? pcg_op_march<HAS_CLOSED, true, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf,
-                                                       bid, nblk)
-            : pcg_op_march<HAS_CLOSED, false, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile, zhalf,
-                                                        bid, nblk);
+      first ? pcg_op_march<HAS_CLOSED, true, RES, TZ, TP>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile,
+                                                           zhalf, bid, nblk)
+            : pcg_op_march<HAS_CLOSED, false, RES, TZ, TP>(g, ms, codes, z, pold, pnew, ap, mean, cidx, beta, tile,
+                                                            zhalf, bid, nblk);
   double s = block_sum(acc);
   if (threadIdx.x == 0) {
     part[P_PAP + bid] = s;
     if (bid == 0 && !first) st->beta = beta;
   }
 }
-template <bool HAS_CLOSED, bool RES, class TZ = double>
+template <bool HAS_CLOSED, bool RES, class TZ = double, class TP = double>
 __global__ void __launch_bounds__(RB, 8)
     k_pcg_op_march(Geo g, MarchShape ms, const uint8_t* __restrict__ codes, const TZ* __restrict__ z,
-                   const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ ap,
+                   const TP* __restrict__ pold, TP* __restrict__ pnew, double* __restrict__ ap,
                    const double* __restrict__ mean, const int* __restrict__ cidx, PcgState* st, double* part,
                    long long zhalf) {
-  pcg_op_march_body<HAS_CLOSED, RES, TZ>(g, ms, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x,
-                                         gridDim.x);
+  pcg_op_march_body<HAS_CLOSED, RES, TZ, TP>(g, ms, codes, z, pold, pnew, ap, mean, cidx, st, part, zhalf, blockIdx.x,
+                                             gridDim.x);
 }
 
 // alpha = rz / (p . Ap) with CFD-E032 (pressure.rs:393-397); x += alpha p;
 // r -= alpha Ap; partials of |r|^2 (pressure.rs:398-402).
-__device__ __forceinline__ void pcg_axpy_body(long long N, const double* __restrict__ p,
+template <class TP>
+__device__ __forceinline__ void pcg_axpy_body(long long N, const TP* __restrict__ p,
                                               const double* __restrict__ ap, double* __restrict__ x,
                                               double* __restrict__ r, PcgState* st, double* part, unsigned bid,
                                               unsigned nblk) {
@@ -504,7 +511,7 @@ __device__ __forceinline__ void pcg_axpy_body(long long N, const double* __restr
   const double alpha = rz / denom;
   double acc = 0.0;
   for (long long i = (long long)bid * RB + threadIdx.x; i < N; i += (long long)nblk * RB) {
-    x[i] += alpha * p[i];
+    x[i] += alpha * (double)p[i];
     double ri = r[i] - alpha * ap[i];
     r[i] = ri;
     acc += ri * ri;
@@ -523,7 +530,7 @@ __device__ __forceinline__ void pcg_axpy_body(long long N, const double* __restr
 __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __restrict__ p, const double* __restrict__ ap,
                                                  double* __restrict__ x, double* __restrict__ r, PcgState* st,
                                                  double* part) {
-  pcg_axpy_body(N, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
+  pcg_axpy_body<double>(N, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
 }
 
 // The same with r in the colour-separated layout of the fine red-black level
@@ -531,8 +538,9 @@ __global__ void __launch_bounds__(RB) k_pcg_axpy(long long N, const double* __re
 // same cells per thread in the same order, so x, r and the partials of |r|^2
 // are those of k_pcg_axpy bit for bit. mnx, mny: ceil(2^40 / nx), ceil(2^40 / ny)
 // (N nx < 2^40).
+template <class TP>
 __device__ __forceinline__ void pcg_axpy_rb_body(long long N, int nx, int ny, unsigned long long mnx,
-                                                 unsigned long long mny, const double* __restrict__ p,
+                                                 unsigned long long mny, const TP* __restrict__ p,
                                                  const double* __restrict__ ap, double* __restrict__ x,
                                                  double* __restrict__ r, PcgState* st, double* part, unsigned bid,
                                                  unsigned nblk) {
@@ -554,7 +562,7 @@ __device__ __forceinline__ void pcg_axpy_rb_body(long long N, int nx, int ny, un
     const unsigned row = div_magic(c, mnx), k = div_magic(row, mny);
     const unsigned colour = (c - row * (unsigned)nx + row - k * (unsigned)ny + k) & 1u;
     double* rc = r + (colour ? half : 0) + (c >> 1);
-    x[i] += alpha * p[i];
+    x[i] += alpha * (double)p[i];
     double ri = *rc - alpha * ap[i];
     *rc = ri;
     acc += ri * ri;
@@ -570,11 +578,12 @@ __device__ __forceinline__ void pcg_axpy_rb_body(long long N, int nx, int ny, un
     }
   }
 }
+template <class TP = double>
 __global__ void __launch_bounds__(RB) k_pcg_axpy_rb(long long N, int nx, int ny, unsigned long long mnx,
-                                                    unsigned long long mny, const double* __restrict__ p,
+                                                    unsigned long long mny, const TP* __restrict__ p,
                                                     const double* __restrict__ ap, double* __restrict__ x,
                                                     double* __restrict__ r, PcgState* st, double* part) {
-  pcg_axpy_rb_body(N, nx, ny, mnx, mny, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
+  pcg_axpy_rb_body<TP>(N, nx, ny, mnx, mny, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
 }
 
 // When the recurrent residual |r|/|b_c| reached tol: partials of |A x - b_c|^2,
@@ -2231,6 +2240,46 @@ __global__ void __launch_bounds__(BS) k_rb_first_lin(Geo g, FineRB<T> F, const P
   rb_first_lin_body<T>(g, F, st, blockIdx.x, gridDim.x);
 }
 
+// The same from an FP64 right-hand side kept in the same colour-separated
+// layout (rres, the mixed-precision solve's residual): both colours' b rounded
+// into F.b and the red x = b / diagonal, k_rb_first2's arithmetic.
+template <class T>
+__device__ __forceinline__ void rb_first_cvt_body(const Geo& g, const FineRB<T>& F, const double* __restrict__ rres,
+                                                  const PcgState* st, unsigned bid, unsigned nblk) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned pair = (unsigned)xcd_tile(bid, nblk) * BS + threadIdx.x;
+  const unsigned u = 2 * pair;
+  if (u >= (unsigned)F.half) return;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const unsigned row = div_magic(u, F.mhx);
+  const int i2 = (int)(u - row * hx);
+  const unsigned k = div_magic(row, F.mny);
+  const int j = (int)(row - k * ny);
+  const int par = (j + (int)k) & 1;
+  const double2 rr = *reinterpret_cast<const double2*>(rres + u);
+  const double2 rk = *reinterpret_cast<const double2*>(rres + F.half + u);
+  const uint8_t ca = F.codes[u], cb = F.codes[u + 1];
+  const int qa[3] = {2 * i2 + par, j, (int)k}, qb[3] = {2 * i2 + 2 + par, j, (int)k};
+  const T da = fine_diag<T>(g, ca, qa), db = fine_diag<T>(g, cb, qb);
+  T2 red, black, out;
+  red.x = (T)rr.x;
+  red.y = (T)rr.y;
+  black.x = (T)rk.x;
+  black.y = (T)rk.y;
+  out.x = (ca & FLUID) && da > T(0) ? red.x / da : T(0);
+  out.y = (cb & FLUID) && db > T(0) ? red.y / db : T(0);
+  if (status != ST_RUNNING) return;
+  *reinterpret_cast<T2*>(F.b + u) = red;
+  *reinterpret_cast<T2*>(F.b + F.half + u) = black;
+  *reinterpret_cast<T2*>(F.x + u) = out;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_first_cvt(Geo g, FineRB<T> F, const double* __restrict__ rres,
+                                                     const PcgState* st) {
+  rb_first_cvt_body<T>(g, F, rres, st, blockIdx.x, gridDim.x);
+}
+
 // The Gauss-Seidel value of a cell from its code, right-hand side and the six
 // neighbour values xn (axis-major, lower side first; entries of uncoupled or
 // boundary faces are not read).
@@ -2514,7 +2563,11 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz(Geo g, FineRB<T> F, double* 
 // b, black b. RES: r is the fine level's right-hand side F.b (both colours; the
 // PCG keeps its residual there) and z stays in F.x (nothing written in the
 // natural layout).
-template <class T, bool RES>
+// TR (RES): the type of the colour-separated residual; TR == T reads it from
+// F.b (the FP64 solve), TR double with T float from r (the mixed-precision
+// solve keeps its FP64 residual in the FP64 level's b arrays, in the same
+// layout).
+template <class T, bool RES, class TR = T>
 __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
                                         float* __restrict__ out32, const double* __restrict__ r, int i2, int j, int k,
                                         unsigned u, double& acc) {
@@ -2530,8 +2583,14 @@ __device__ __forceinline__ void rz_pair(const Geo& g, const FineRB<T>& F, double
   const T2 black = *reinterpret_cast<const T2*>(F.x + F.half + u);
   double2 r01, r23;
   if (RES) {
-    const T2 rr = *reinterpret_cast<const T2*>(F.b + u);
-    const T2 rk = *reinterpret_cast<const T2*>(F.b + F.half + u);
+    using R2 = std::conditional_t<std::is_same<TR, double>::value, double2, float2>;
+    const TR* rs;
+    if constexpr (std::is_same<TR, T>::value)
+      rs = F.b;
+    else
+      rs = r;
+    const R2 rr = *reinterpret_cast<const R2*>(rs + u);
+    const R2 rk = *reinterpret_cast<const R2*>(rs + F.half + u);
     r01.x = (double)(par ? rk.x : rr.x);
     r01.y = (double)(par ? rr.x : rk.x);
     r23.x = (double)(par ? rk.y : rr.y);
@@ -2602,7 +2661,7 @@ __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2(Geo g, FineRB<T> F, double*
 // The same with consecutive threads on consecutive red pairs: items of RB pairs
 // in order, each XCD taking a contiguous share of them (no idle lanes where a
 // row is not a multiple of the tile width).
-template <class T, bool RES>
+template <class T, bool RES, class TR = T>
 __device__ __forceinline__ void rb_gs_rz2_lin_body(const Geo& g, const FineRB<T>& F, double* __restrict__ out64,
                                                    float* __restrict__ out32, const double* __restrict__ r,
                                                    const PcgState* st, double* part, unsigned bid, unsigned nblk) {
@@ -2618,16 +2677,16 @@ __device__ __forceinline__ void rb_gs_rz2_lin_body(const Geo& g, const FineRB<T>
     const unsigned row = div_magic(u, F.mhx);
     const int i2 = (int)(u - row * hx);
     const unsigned k = div_magic(row, F.mny);
-    rz_pair<T, RES>(g, F, out64, out32, r, i2, (int)(row - k * ny), (int)k, u, acc);
+    rz_pair<T, RES, TR>(g, F, out64, out32, r, i2, (int)(row - k * ny), (int)k, u, acc);
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0) part[P_RZ + bid] = s;
 }
-template <class T, bool RES>
+template <class T, bool RES, class TR = T>
 __global__ void __launch_bounds__(RB, 8) k_rb_gs_rz2_lin(Geo g, FineRB<T> F, double* __restrict__ out64,
                                                       float* __restrict__ out32, const double* __restrict__ r,
                                                       const PcgState* st, double* part) {
-  rb_gs_rz2_lin_body<T, RES>(g, F, out64, out32, r, st, part, blockIdx.x, gridDim.x);
+  rb_gs_rz2_lin_body<T, RES, TR>(g, F, out64, out32, r, st, part, blockIdx.x, gridDim.x);
 }
 
 // Workgroups of a half-sweep: tiles of 64 x 4 cells of a colour (x, y) per z layer.
@@ -3420,8 +3479,31 @@ void launch_pcg_axpy_rb(const Geo& g, const double* p, const double* ap, double*
   const unsigned long long two40 = 1ULL << 40;
   const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
   const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
-  PROBED_LAUNCH(PROBE_AXPY, 48.0 * (double)g.N, k_pcg_axpy_rb, dim3(RP), dim3(RB), 0, s, g.N, g.n[0], g.n[1], mnx, mny,
-                p, ap, x, r, st, part);
+  PROBED_LAUNCH(PROBE_AXPY, 48.0 * (double)g.N, k_pcg_axpy_rb<double>, dim3(RP), dim3(RB), 0, s, g.N, g.n[0], g.n[1],
+                mnx, mny, p, ap, x, r, st, part);
+}
+void launch_pcg_axpy_rb(const Geo& g, const float* p, const double* ap, double* x, double* r, PcgState* st,
+                        double* part, hipStream_t s) {
+  const unsigned long long two40 = 1ULL << 40;
+  const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
+  const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
+  // x, r, Ap 8 and p 4 in; x, r out
+  PROBED_LAUNCH(PROBE_AXPY, 44.0 * (double)g.N, k_pcg_axpy_rb<float>, dim3(RP), dim3(RB), 0, s, g.N, g.n[0], g.n[1],
+                mnx, mny, p, ap, x, r, st, part);
+}
+void launch_pcg_op_mixed(const Geo& g, const uint8_t* codes, const float* z, long long zhalf, const float* pold,
+                         float* pnew, double* ap, PcgState* st, double* part, hipStream_t s, int march) {
+  const MarchShape ms = march_shape(g);
+  const bool marching = march == 2 || (march == 1 && ms.tx * ms.ty * ms.tz >= RP);
+  const double bytes = 21.0 * (double)g.N;  // z 4, p_old 4, code 1 in; p_new 4, Ap 8 out
+  const double* none = nullptr;
+  const int* nidx = nullptr;
+  if (marching)
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op_march<false, true, float, float>), dim3(RP), dim3(RB), 0, s, g, ms,
+                  codes, z, pold, pnew, ap, none, nidx, st, part, zhalf);
+  else
+    PROBED_LAUNCH(PROBE_OPERATOR, bytes, (k_pcg_op<false, true, float, float>), dim3(RP), dim3(RB), 0, s, g, codes, z,
+                  pold, pnew, ap, none, nidx, st, part, zhalf);
 }
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s) {
@@ -3694,7 +3776,10 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 // level's b and z left in its x, the last half-sweep forming r . z from them;
 // 2 the same with b already there (b0 unused); 3 (an FP32 level) b0 rounded into
 // the level's b as without, r . z formed from b0 and z written in FP32 (the
-// values of the level's x) to out32 in the natural layout.
+// values of the level's x) to out32 in the natural layout; 4 (an FP32 level,
+// two cells per thread) the FP64 right-hand side rz_r in the colour-separated
+// layout (b0 unused) rounded into the level's b, r . z formed from rz_r and z
+// left in the level's x.
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
@@ -3741,11 +3826,16 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
     // pairs of natural cells when every row starts a pair and r allows 16-byte loads
     // r (FP64) and codes in; b of both colours and the red x out (from b kept
     // in the layout: red codes and b in, red x out)
-    const double first_bytes = resident == 2 ? (double)rb->half * (1 + 2 * sizeof(T))
-                                             : (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
+    const double first_bytes =
+        resident == 2   ? (double)rb->half * (1 + 2 * sizeof(T))
+        : resident == 4 ? (double)g0.N * (8 + sizeof(T)) + (double)rb->half * (1 + sizeof(T))
+                        : (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
     if (resident == 2)
       PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first_lin<T>), dim3((unsigned)((rb->half / 2 + BS - 1) / BS)),
                     dim3(BS), 0, s, g0, *rb, st);
+    else if (resident == 4)
+      PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first_cvt<T>),
+                    dim3((unsigned)((rb->half / 2 + BS - 1) / BS)), dim3(BS), 0, s, g0, *rb, rz_r, st);
     else if (g0.n[0] % 2 == 0 && ((uintptr_t)b0 & 15) == 0)
       PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first2<T>),
                     dim3(((rb->hx + 63) / 64) * ((g0.n[1] + 3) / 4) * g0.n[2]), dim3(BS), 0, s, g0, codes,
@@ -3816,9 +3906,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         // half-sweep, both colours' z out in FP64 and r in (with r and z kept in
         // the layout: the black b read besides)
         const double rz_bytes = (double)rb->half * (1 + 3 * sizeof(T)) +
-                                (resident == 3 ? (double)g0.N * 12
-                                               : resident ? (double)rb->half * sizeof(T) : (double)g0.N * 16);
-        if (resident == 3 && rb->two_cells == 2)
+                                (resident == 3   ? (double)g0.N * 12
+                                 : resident == 4 ? (double)g0.N * 8
+                                 : resident      ? (double)rb->half * sizeof(T)
+                                                 : (double)g0.N * 16);
+        if (resident == 4)
+          PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, true, double>), dim3(RP), dim3(RB), 0, s, g0,
+                        *rb, (double*)nullptr, (float*)nullptr, rz_r, st, rz_part);
+        else if (resident == 3 && rb->two_cells == 2)
           PROBED_LAUNCH(PROBE_FINE_RZ, rz_bytes, (k_rb_gs_rz2_lin<T, false>), dim3(RP), dim3(RB), 0, s, g0, *rb,
                         (double*)nullptr, out32, rz_r, st, rz_part);
         else if (resident == 3)
@@ -4125,11 +4220,10 @@ struct BFirstLin {
   unsigned nblk;
 };
 template <class T>
-struct BFirst2 {
+struct BFirstCvt {
   Geo g;
-  const uint8_t* codes;
-  const double* r;
   FineRB<T> F;
+  const double* rres;
   const PcgState* st;
   unsigned nblk;
 };
@@ -4199,31 +4293,31 @@ template <class T>
 struct BRz {
   Geo g;
   FineRB<T> F;
-  float* out32;
   const double* r;
   const PcgState* st;
   double* part;
   unsigned nblk;
 };
-template <class TZ>
+template <class TZ, class TP>
 struct BOp {
   Geo g;
   MarchShape ms;
   const uint8_t* codes;
   const TZ* z;
-  const double* pold;
-  double* pnew;
+  const TP* pold;
+  TP* pnew;
   double* ap;
   PcgState* st;
   double* part;
   long long zhalf;
   unsigned nblk;
 };
+template <class TP>
 struct BAxpy {
   long long N;
   int nx, ny;
   unsigned long long mnx, mny;
-  const double* p;
+  const TP* p;
   const double* ap;
   double* x;
   double* r;
@@ -4254,11 +4348,11 @@ __global__ void __launch_bounds__(BS) kb_rb_first_lin(Batch<BFirstLin<T>> B) {
   rb_first_lin_body<T>(a.g, a.F, a.st, bid, a.nblk);
 }
 template <class T>
-__global__ void __launch_bounds__(BS) kb_rb_first2(Batch<BFirst2<T>> B) {
+__global__ void __launch_bounds__(BS) kb_rb_first_cvt(Batch<BFirstCvt<T>> B) {
   unsigned bid;
-  const BFirst2<T>& a = batch_item(B, bid);
+  const BFirstCvt<T>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  rb_first2_body<T>(a.g, a.codes, a.r, a.F, a.st, bid);
+  rb_first_cvt_body<T>(a.g, a.F, a.rres, a.st, bid, a.nblk);
 }
 template <class T>
 __global__ void __launch_bounds__(BS) kb_rb_gs2_lin(Batch<BHalf<T>> B) {
@@ -4309,40 +4403,35 @@ __global__ void __launch_bounds__(BS) kb_rb_prolong(Batch<BRbProlong<T>> B) {
   if (bid >= a.nblk) return;
   rb_prolong_body<T>(a.half, a.xr, a.rpar, a.xc, a.cs, a.st, bid);
 }
-template <class T, bool RES>
+template <class T, class TR>
 __global__ void __launch_bounds__(RB, 8) kb_rb_gs_rz2_lin(Batch<BRz<T>> B) {
   unsigned bid;
   const BRz<T>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  rb_gs_rz2_lin_body<T, RES>(a.g, a.F, (double*)nullptr, a.out32, a.r, a.st, a.part, bid, a.nblk);
+  rb_gs_rz2_lin_body<T, true, TR>(a.g, a.F, (double*)nullptr, (float*)nullptr, a.r, a.st, a.part, bid, a.nblk);
 }
-template <class TZ, bool RES>
-__global__ void __launch_bounds__(RB, 8) kb_pcg_op(Batch<BOp<TZ>> B) {
+template <class TZ, class TP>
+__global__ void __launch_bounds__(RB, 8) kb_pcg_op(Batch<BOp<TZ, TP>> B) {
   unsigned bid;
-  const BOp<TZ>& a = batch_item(B, bid);
+  const BOp<TZ, TP>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  pcg_op_body<false, RES, TZ>(a.g, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr, (const int*)nullptr,
-                              a.st, a.part, a.zhalf, bid, a.nblk);
+  pcg_op_body<false, true, TZ, TP>(a.g, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr,
+                                   (const int*)nullptr, a.st, a.part, a.zhalf, bid, a.nblk);
 }
-template <class TZ, bool RES>
-__global__ void __launch_bounds__(RB, 8) kb_pcg_op_march(Batch<BOp<TZ>> B) {
+template <class TZ, class TP>
+__global__ void __launch_bounds__(RB, 8) kb_pcg_op_march(Batch<BOp<TZ, TP>> B) {
   unsigned bid;
-  const BOp<TZ>& a = batch_item(B, bid);
+  const BOp<TZ, TP>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  pcg_op_march_body<false, RES, TZ>(a.g, a.ms, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr,
-                                    (const int*)nullptr, a.st, a.part, a.zhalf, bid, a.nblk);
+  pcg_op_march_body<false, true, TZ, TP>(a.g, a.ms, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr,
+                                         (const int*)nullptr, a.st, a.part, a.zhalf, bid, a.nblk);
 }
-__global__ void __launch_bounds__(RB) kb_pcg_axpy(Batch<BAxpy> B) {
+template <class TP>
+__global__ void __launch_bounds__(RB) kb_pcg_axpy_rb(Batch<BAxpy<TP>> B) {
   unsigned bid;
-  const BAxpy& a = batch_item(B, bid);
+  const BAxpy<TP>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  pcg_axpy_body(a.N, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
-}
-__global__ void __launch_bounds__(RB) kb_pcg_axpy_rb(Batch<BAxpy> B) {
-  unsigned bid;
-  const BAxpy& a = batch_item(B, bid);
-  if (bid >= a.nblk) return;
-  pcg_axpy_rb_body(a.N, a.nx, a.ny, a.mnx, a.mny, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
+  pcg_axpy_rb_body<TP>(a.N, a.nx, a.ny, a.mnx, a.mny, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
 }
 __global__ void __launch_bounds__(RB, 8) kb_pcg_true(Batch<BTrue> B) {
   unsigned bid;
@@ -4396,13 +4485,9 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
   const int fr = it[0].fine_rounds;
   // the first fine half-sweep from a zero iterate
   if constexpr (MIXED) {
-    Batcher<BFirst2<T>> L;
-    for (int i = 0; i < n; ++i) {
-      const Geo& g = it[i].g;
-      L.add({g, it[i].codes, it[i].r, it[i].F, it[i].st,
-             (unsigned)(((it[i].F.hx + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[2])});
-    }
-    BATCH_LAUNCH(kb_rb_first2<T>, L, BS, s);
+    Batcher<BFirstCvt<T>> L;
+    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].rres, it[i].st, pair_blocks(it[i].F)});
+    BATCH_LAUNCH(kb_rb_first_cvt<T>, L, BS, s);
   } else {
     Batcher<BFirstLin<T>> L;
     for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].st, pair_blocks(it[i].F)});
@@ -4523,43 +4608,34 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
     BATCH_LAUNCH(kb_rb_gs2_lin<T>, L, BS, s);
   }
   {
+    // r . z from the FP64 residual: F.b itself (FP64), the FP64 level's b (mixed)
     Batcher<BRz<T>> L;
-    for (int i = 0; i < n; ++i)
-      L.add({it[i].g, it[i].F, MIXED ? it[i].z32 : (float*)nullptr, MIXED ? (const double*)it[i].r : nullptr,
-             it[i].st, it[i].part, (unsigned)RP});
-    if constexpr (MIXED)
-      BATCH_LAUNCH((kb_rb_gs_rz2_lin<T, false>), L, RB, s);
-    else
-      BATCH_LAUNCH((kb_rb_gs_rz2_lin<T, true>), L, RB, s);
+    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].rres, it[i].st, it[i].part, (unsigned)RP});
+    BATCH_LAUNCH((kb_rb_gs_rz2_lin<T, double>), L, RB, s);
   }
-  // p = z + beta p, A p, p . Ap
+  // p = z + beta p (stored as T), A p, p . Ap; z is the fine level's x
   {
-    using TZ = std::conditional_t<MIXED, float, double>;
-    Batcher<BOp<TZ>> C, M;
+    Batcher<BOp<T, T>> C, M;
     for (int i = 0; i < n; ++i) {
-      const TZ* z = MIXED ? (const TZ*)it[i].z32 : (const TZ*)it[i].F.x;
-      const BOp<TZ> a{it[i].g, march_shape(it[i].g), it[i].codes, z, it[i].pold, it[i].pnew, it[i].ap, it[i].st,
-                      it[i].part, MIXED ? 0LL : it[i].F.half, (unsigned)RP};
+      const BOp<T, T> a{it[i].g, march_shape(it[i].g), it[i].codes, it[i].F.x, (const T*)it[i].pold, (T*)it[i].pnew,
+                        it[i].ap, it[i].st, it[i].part, it[i].F.half, (unsigned)RP};
       (it[i].march ? M : C).add(a);
     }
-    BATCH_LAUNCH((kb_pcg_op<TZ, !MIXED>), C, RB, s);
-    BATCH_LAUNCH((kb_pcg_op_march<TZ, !MIXED>), M, RB, s);
+    BATCH_LAUNCH((kb_pcg_op<T, T>), C, RB, s);
+    BATCH_LAUNCH((kb_pcg_op_march<T, T>), M, RB, s);
   }
-  // alpha, x += alpha p, r -= alpha A p, |r|^2
+  // alpha, x += alpha p, r -= alpha A p, |r|^2 (r in the FP64 colour-separated arrays)
   {
-    Batcher<BAxpy> L;
+    Batcher<BAxpy<T>> L;
     const unsigned long long two40 = 1ULL << 40;
     for (int i = 0; i < n; ++i) {
       const Geo& g = it[i].g;
       const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
       const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
-      L.add({g.N, g.n[0], g.n[1], mnx, mny, it[i].pnew, it[i].ap, it[i].x, MIXED ? it[i].r : (double*)it[i].F.b,
-             it[i].st, it[i].part, (unsigned)RP});
+      L.add({g.N, g.n[0], g.n[1], mnx, mny, (const T*)it[i].pnew, it[i].ap, it[i].x, it[i].rres, it[i].st, it[i].part,
+             (unsigned)RP});
     }
-    if constexpr (MIXED)
-      BATCH_LAUNCH(kb_pcg_axpy, L, RB, s);
-    else
-      BATCH_LAUNCH(kb_pcg_axpy_rb, L, RB, s);
+    BATCH_LAUNCH(kb_pcg_axpy_rb<T>, L, RB, s);
   }
   // the recomputed residual and the iteration's decisions
   {

@@ -90,6 +90,14 @@ void launch_pcg_axpy(long long N, const double* p, const double* ap, double* x, 
 // the same with r in the colour-separated layout (nx even, N nx < 2^40)
 void launch_pcg_axpy_rb(const Geo& g, const double* p, const double* ap, double* x, double* r, PcgState* st,
                         double* part, hipStream_t s);
+// ... with p in FP32 (the mixed-precision solve)
+void launch_pcg_axpy_rb(const Geo& g, const float* p, const double* ap, double* x, double* r, PcgState* st,
+                        double* part, hipStream_t s);
+// The operator kernel of the mixed-precision solve: z the FP32 values of the
+// fine level's x (colour-separated, zhalf entries per colour), p stored in FP32
+// (each value rounded once; A p and p . Ap formed from the rounded values).
+void launch_pcg_op_mixed(const Geo& g, const uint8_t* codes, const float* z, long long zhalf, const float* pold,
+                         float* pnew, double* ap, PcgState* st, double* part, hipStream_t s, int march = 1);
 // the recomputed-residual kernel and the one-block decision kernel of an iteration
 void launch_pcg_true(const Geo& g, const uint8_t* codes, const double* x, const double* bc, PcgState* st, double* part,
                      hipStream_t s);
@@ -143,10 +151,10 @@ void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s)
 // kBatchMax independent solves, each on its own grid, issued as one launch per
 // stage of the iteration; every launch gives each solve the blocks its own
 // single-grid launch would have (from a multiple of 8, so XCD placement is
-// unchanged), so each solve computes exactly what it computes alone. T double:
-// the FP64 solve with r and z in the fine level's colour-separated arrays; T
-// float: the mixed-precision solve (FP32 cycle, r natural, z as FP32 values in
-// z32). The fine level takes two cells per thread (nx a multiple of 4).
+// unchanged), so each solve computes exactly what it computes alone. The fine
+// level takes two cells per thread (nx a multiple of 4); r stays in the FP64
+// colour-separated b arrays and z in the fine level's x. T double: the FP64
+// solve; T float: the mixed-precision solve (FP32 cycle, p in FP32).
 constexpr int kBatchMax = 8;
 template <class T>
 struct BatchSolve {
@@ -159,8 +167,9 @@ struct BatchSolve {
   AggCoarsest D;
   PcgState* st;
   double* part;
-  double *x, *r, *bc, *ap, *pold, *pnew;
-  float* z32;
+  double *x, *bc, *ap;
+  void *pold, *pnew;  // p, stored as T
+  double* rres;       // the FP64 residual, colour-separated (F.b for T double)
   T cs;             // coarse correction scale
   bool zero_last;   // the restriction takes the class relaxed last as zero residual
   int fine_rounds, coarse_rounds;

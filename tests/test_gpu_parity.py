@@ -787,7 +787,14 @@ def test_resident_residual_solves_bitwise(gpu_available, name, make, precision):
     every partial sum keeps its arithmetic and order: the field, the iteration
     count and the residuals equal those of the natural-layout solve bit for bit,
     with the marching and the cell-by-cell operator kernels, pairs in tiles or
-    linear, periodic axes included, and on a warm-started second solve."""
+    linear, periodic axes included, and on a warm-started second solve.
+
+    With linear pairs the mixed-precision solve also keeps its FP64 residual
+    colour by colour and stores p in FP32 (each p value rounded once, A p and
+    p . Ap formed from the rounded values): not bitwise the natural-layout
+    solve, so it is held to the same iteration count within one and the same
+    accuracy (both fields within 1e-9 of each other and 1e-8 of the exact
+    field at rel-res 1e-11)."""
     case = make()
     o = oracle.Grid(case)
     rng = np.random.default_rng(21)
@@ -813,16 +820,24 @@ def test_resident_residual_solves_bitwise(gpu_available, name, make, precision):
                 assert abs(first["bytes"] / case.count - want) < 1e-6 * n, (first, want)
             else:
                 # the last half-sweep writes z as its FP32 values (4 bytes a
-                # cell) instead of an FP64 copy (8)
+                # cell) instead of an FP64 copy (8), or (linear pairs) leaves z
+                # in the level and reads r colour by colour (no z written)
                 last = g.profile_solve(d_b, d_x, 1e-11, "fine_rz_half_sweep")
                 n = last["launches"]
-                want = (18.5 if resident else 22.5) * n
+                want = (14.5 if lin else 18.5) * n if resident else 22.5 * n
                 assert abs(last["bytes"] / case.count - want) < 1e-6 * n, (last, want)
             x, info = g.solve(b, 1e-11, 3000)
             g.set_options(precision, warm_start=True)
             xw, infow = g.solve(b2, 1e-11, 3000)
             out.append((x, info, xw, infow))
         (x1, i1, w1, j1), (x0, i0, w0, j0) = out
+        if precision == cfdgpu.PRECISION_MIXED and lin:
+            for a, c in ((i1, i0), (j1, j0)):
+                assert abs(a["iterations"] - c["iterations"]) <= 1, (a, c)
+                assert a["relative_residual"] <= 1e-10 + 1e-12, a
+            assert rel(x1, x0) <= 1e-9 and rel(w1, w0) <= 1e-9, (rel(x1, x0), rel(w1, w0))
+            assert rel(x1, ex) <= 1e-8
+            continue
         for a, c in ((i1, i0), (j1, j0)):
             assert a["iterations"] == c["iterations"], (a, c)
             assert a["relative_residual"] == c["relative_residual"], (a, c)
