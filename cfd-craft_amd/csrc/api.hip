@@ -13,6 +13,9 @@
 #include <map>
 #include <tuple>
 #include <thread>
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <unordered_map>
 #include <cstdlib>
 #include <cstdio>
@@ -2173,49 +2176,69 @@ struct cfd_server {
     int precision = 0;
     int grid_precision = 0;  // the grid's own setting, restored when the request ends
     double t_submit = 0.0, t_start = 0.0, t_end = 0.0;
-    int state = 0;  // 0 queued, 1 running, 2 done, 3 batched and finishing (final kernels queued)
-    int lane = -1;  // batched: the lane it runs in
-    int64_t enq = 0;  // batched: iterations queued
+    // 0 queued, 1 running, 2 done; a batched request is driven by its lane's
+    // thread, which publishes 2 (release) after its results
+    std::atomic<int> state{0};
+    int lane = -1;    // batched: the lane it runs in
+    int64_t enq = 0;  // batched: iterations queued (lane thread)
     int rc = 0;
     std::string error;
     cfd_linear_info info{};
   };
-  // A lane: a stream running batches (one per precision) step after step. Lanes
-  // run concurrently, so one lane's latency-bound coarse levels overlap another's
-  // streaming fine-level kernels.
+  // A lane: a stream running batches (one per precision) step after step, driven
+  // by a host thread of its own (the kernel launches of a step take about 3 us
+  // each on the host). Lanes run concurrently, so one lane's latency-bound coarse
+  // levels overlap another's streaming fine-level kernels.
   struct Step {
     int slot = 0;
-    std::vector<uint64_t> ids;  // the solves whose states the step reads back, in slot order
+    std::vector<Req*> items;  // the solves whose states the step reads back, in slot order
   };
   struct Lane {
     hipStream_t bs = nullptr;
-    std::vector<uint64_t> group[2];  // solves in the FP64 and the mixed-precision batch
+    std::vector<Req*> group[2];  // solves in the FP64 and the mixed-precision batch (lane thread)
+    std::vector<Req*> finishing;  // solves whose final kernels are queued (lane thread)
     std::deque<Step> steps;
     int next_slot = 0;
-    PcgState* d_states = nullptr;  // kStepRing x 2 kBatchMax
-    PcgState* h_states = nullptr;  // pinned copy of the same
+    // kStepRing x 2 kBatchMax states in coherent pinned host memory, written by
+    // the step's last kernel (no copy call per step)
+    PcgState* h_states = nullptr;
+    PcgState* hd_states = nullptr;  // the same memory as the device addresses it
     hipEvent_t step_ev[kStepRing] = {};
+    std::vector<Req*> joining;  // handed over by the driving thread (under the server's lock)
+    std::atomic<bool> has_joining{false};
+    int load = 0;               // solves assigned and not yet collected (driving thread)
+    int failed = 0;             // a device error stopped the lane (its code)
+    std::thread worker;
   };
   int max_active = 8;
   size_t budget = 0, used = 0;
   uint64_t next = 1;
-  std::unordered_map<uint64_t, Req> reqs;
+  std::unordered_map<uint64_t, Req> reqs;  // node-based: a Req keeps its address while others come and go
   std::deque<uint64_t> queued;
   std::vector<uint64_t> running;  // on their grids' streams
+  std::vector<uint64_t> batched;  // handed to lanes, not yet collected
   std::unordered_map<cfd_grid*, int> busy;
   std::unordered_map<cfd_grid*, size_t> grids;  // admitted grids and their footprints
   int n_lanes = 3;  // 0: no batching
   int ahead = kStepsAhead;
-  std::vector<Lane> lanes;
-  std::vector<uint64_t> finishing;  // batched solves whose final kernels are queued
+  std::vector<std::unique_ptr<Lane>> lanes;
+  std::mutex m;
+  std::condition_variable cv;
+  bool stop = false;
   ~cfd_server() {
-    for (Lane& L : lanes) {
-      if (L.bs) (void)hipStreamSynchronize(L.bs);
-      for (hipEvent_t e : L.step_ev)
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto& L : lanes)
+      if (L->worker.joinable()) L->worker.join();
+    for (auto& L : lanes) {
+      if (L->bs) (void)hipStreamSynchronize(L->bs);
+      for (hipEvent_t e : L->step_ev)
         if (e) (void)hipEventDestroy(e);
-      if (L.d_states) (void)hipFree(L.d_states);
-      if (L.h_states) (void)hipHostFree(L.h_states);
-      if (L.bs) (void)hipStreamDestroy(L.bs);
+      if (L->h_states) (void)hipHostFree(L->h_states);
+      if (L->bs) (void)hipStreamDestroy(L->bs);
     }
   }
 };
@@ -2234,20 +2257,6 @@ bool batchable(const cfd_grid* g, int precision) {
     return false;
   if (precision == CFD_PRECISION_MIXED) return A.rb32.two_cells == 2 && g->rb_resident;
   return A.rb64.two_cells == 2;
-}
-
-int ensure_lanes(cfd_server* s) {
-  if ((int)s->lanes.size() == s->n_lanes) return 0;
-  s->lanes.resize((size_t)s->n_lanes);
-  for (cfd_server::Lane& L : s->lanes) {
-    if (L.bs) continue;
-    HIP_TRY(hipStreamCreateWithFlags(&L.bs, hipStreamNonBlocking));
-    HIP_TRY(hipMalloc((void**)&L.d_states, (size_t)kStepRing * 2 * kBatchMax * sizeof(PcgState)));
-    HIP_TRY(hipHostMalloc((void**)&L.h_states, (size_t)kStepRing * 2 * kBatchMax * sizeof(PcgState),
-                          hipHostMallocDefault));
-    for (hipEvent_t& e : L.step_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
-  return 0;
 }
 
 // The solve of grid g as the batched launches see it; pold / pnew by the parity
@@ -2281,14 +2290,14 @@ BatchSolve<T> batch_solve(cfd_grid* g, const FineRB<T>& F, const std::vector<Agg
   return b;
 }
 
-// A request joins a batch of its precision in lane `lane`: the set-up kernels
-// of solve_scaled (pressure.rs:342-375) on the lane's stream, and the residual
+// A request joins a batch of its precision (lane thread): the set-up kernels of
+// solve_scaled (pressure.rs:342-375) on the lane's stream, and the residual
 // scattered into the FP64 fine level's colour-separated arrays, where the
 // batched iterations keep it. Its first batched iteration applies the
 // preconditioner to that residual.
-int batch_begin(cfd_server* s, int lane, cfd_server::Req& R) {
+int batch_begin(cfd_server::Lane& L, cfd_server::Req& R) {
   cfd_grid* g = R.g;
-  hipStream_t st = s->lanes[(size_t)lane].bs;
+  hipStream_t st = L.bs;
   PcgState* sp = g->st.p;
   HIP_TRY(hipEventRecord(g->ev0, st));
   launch_state_init(sp, R.tol, R.max, st);
@@ -2297,71 +2306,51 @@ int batch_begin(cfd_server* s, int lane, cfd_server::Req& R) {
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, sp, st);
   launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, sp, st);
   R.enq = 0;
-  R.lane = lane;
   return 0;
 }
 
 // One batched step of a lane: an iteration of every solve in each of its
-// batches, then their states copied into the lane's next host slot.
-int batch_step(cfd_server* s, cfd_server::Lane& L) {
+// batches, then their states written into the lane's next host slot.
+int batch_step(cfd_server::Lane& L) {
   cfd_server::Step step;
   step.slot = L.next_slot;
   L.next_slot = (L.next_slot + 1) % kStepRing;
   std::vector<PcgState*> sts;
   for (int grp = 0; grp < 2; ++grp) {
-    std::vector<uint64_t>& ids = L.group[grp];
-    if (ids.empty()) continue;
+    std::vector<cfd_server::Req*>& rs = L.group[grp];
+    if (rs.empty()) continue;
     if (grp == 0) {
       std::vector<BatchSolve<double>> items;
-      for (uint64_t id : ids) {
-        cfd_server::Req& R = s->reqs[id];
-        items.push_back(batch_solve<double>(R.g, R.g->ash.rb64, R.g->a64.levels, R.enq));
-      }
+      for (cfd_server::Req* R : rs) items.push_back(batch_solve<double>(R->g, R->g->ash.rb64, R->g->a64.levels, R->enq));
       launch_batch_iteration<double>(items.data(), (int)items.size(), L.bs);
     } else {
       std::vector<BatchSolve<float>> items;
-      for (uint64_t id : ids) {
-        cfd_server::Req& R = s->reqs[id];
-        items.push_back(batch_solve<float>(R.g, R.g->ash.rb32, R.g->a32.levels, R.enq));
-      }
+      for (cfd_server::Req* R : rs) items.push_back(batch_solve<float>(R->g, R->g->ash.rb32, R->g->a32.levels, R->enq));
       launch_batch_iteration<float>(items.data(), (int)items.size(), L.bs);
     }
-    for (uint64_t id : ids) {
-      cfd_server::Req& R = s->reqs[id];
-      R.enq += 1;
-      step.ids.push_back(id);
-      sts.push_back(R.g->st.p);
+    for (cfd_server::Req* R : rs) {
+      R->enq += 1;
+      step.items.push_back(R);
+      sts.push_back(R->g->st.p);
     }
   }
-  PcgState* d = L.d_states + (size_t)step.slot * 2 * kBatchMax;
+  PcgState* d = L.hd_states + (size_t)step.slot * 2 * kBatchMax;
   for (size_t k = 0; k < sts.size(); k += kBatchMax)
     launch_batch_states(sts.data() + k, (int)std::min<size_t>(kBatchMax, sts.size() - k), d + k, L.bs);
-  HIP_TRY(hipMemcpyAsync(L.h_states + (size_t)step.slot * 2 * kBatchMax, d, sts.size() * sizeof(PcgState),
-                         hipMemcpyDeviceToHost, L.bs));
   HIP_TRY(hipEventRecord(L.step_ev[step.slot], L.bs));
   L.steps.push_back(std::move(step));
   return 0;
 }
 
-void end_request(cfd_server* s, cfd_server::Req& R, int rc, const std::string& error) {
-  R.t_end = now_s();
-  R.state = 2;
-  R.rc = rc;
-  R.error = error;
-  R.g->opt.precision = R.grid_precision;
-  s->busy.erase(R.g);
-}
-
-// A batched solve whose state shows it ended: its LinearInfo and code, and the
-// final kernels of solve_scaled (x centered into the caller's buffer, the warm
-// start kept) on its lane's stream. The request ends once that stream has
-// passed them, and with them every queued step that still names its grid.
-int batch_finish(cfd_server* s, uint64_t id, const PcgState& st) {
-  cfd_server::Req& R = s->reqs[id];
+// A batched solve whose state shows it ended (lane thread): its LinearInfo and
+// code, and the final kernels of solve_scaled (x centered into the caller's
+// buffer, the warm start kept) on the lane's stream. The request ends once that
+// stream has passed them, and with them every queued step that still names its
+// grid.
+int batch_finish(cfd_server::Lane& L, cfd_server::Req& R, const PcgState& st) {
   cfd_grid* g = R.g;
-  cfd_server::Lane& L = s->lanes[(size_t)R.lane];
-  std::vector<uint64_t>& grp = L.group[R.precision == CFD_PRECISION_MIXED ? 1 : 0];
-  grp.erase(std::find(grp.begin(), grp.end(), id));
+  std::vector<cfd_server::Req*>& grp = L.group[R.precision == CFD_PRECISION_MIXED ? 1 : 0];
+  grp.erase(std::find(grp.begin(), grp.end(), &R));
   if (st.status == ST_CONVERGED) g->learned[learn_key(g, R.tol)] = st.it;
   fill_info(g, st, R.info);
   R.rc = finish_status(g, st, R.max);
@@ -2379,38 +2368,55 @@ int batch_finish(cfd_server* s, uint64_t id, const PcgState& st) {
   }
   HIP_TRY(hipEventRecord(g->ev1, L.bs));
   HIP_TRY(hipEventRecord(g->evs, L.bs));
-  R.state = 3;
-  s->finishing.push_back(id);
+  L.finishing.push_back(&R);
   return 0;
 }
 
-// Steps whose read-back landed: solves that ended leave their batch; finished
-// final kernels complete their requests.
-int batch_progress(cfd_server* s) {
-  for (cfd_server::Lane& L : s->lanes) {
-    while (!L.steps.empty()) {
-      const cfd_server::Step& step = L.steps.front();
-      hipError_t q = hipEventQuery(L.step_ev[step.slot]);
-      if (q == hipErrorNotReady) break;
-      if (q != hipSuccess) {
-        set_error(std::string("HIP error: ") + hipGetErrorString(q));
-        return CFD_EDEVICE;
-      }
-      const PcgState* h = L.h_states + (size_t)step.slot * 2 * kBatchMax;
-      for (size_t k = 0; k < step.ids.size(); ++k) {
-        cfd_server::Req& R = s->reqs[step.ids[k]];
-        if (R.state != 1) continue;  // already ended in an earlier step
-        if (h[k].status != ST_RUNNING) {
-          int rc = batch_finish(s, step.ids[k], h[k]);
-          if (rc) return rc;
-        }
-      }
-      L.steps.pop_front();
-    }
+// One pass of a lane: joining solves set up, steps whose states landed read
+// (solves that ended leave their batch), finished final kernels complete their
+// requests, and steps queued while a batch has a solve, at most `ahead` past
+// the last step seen finished. Returns whether anything happened.
+int lane_pass(cfd_server* s, cfd_server::Lane& L, bool& moved) {
+  std::vector<cfd_server::Req*> join;
+  if (L.has_joining.load(std::memory_order_acquire)) {
+    std::lock_guard<std::mutex> lk(s->m);
+    join.swap(L.joining);
+    L.has_joining.store(false, std::memory_order_relaxed);
   }
-  for (size_t i = 0; i < s->finishing.size();) {
-    cfd_server::Req& R = s->reqs[s->finishing[i]];
-    hipError_t q = hipEventQuery(R.g->evs);
+  for (cfd_server::Req* R : join) {
+    int rc = batch_begin(L, *R);
+    if (rc) return rc;
+    L.group[R->precision == CFD_PRECISION_MIXED ? 1 : 0].push_back(R);
+    moved = true;
+  }
+  while (!L.steps.empty()) {
+    const cfd_server::Step& step = L.steps.front();
+    hipError_t q = hipEventQuery(L.step_ev[step.slot]);
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) {
+      set_error(std::string("HIP error: ") + hipGetErrorString(q));
+      return CFD_EDEVICE;
+    }
+    const PcgState* h = L.h_states + (size_t)step.slot * 2 * kBatchMax;
+    for (size_t k = 0; k < step.items.size(); ++k) {
+      cfd_server::Req& R = *step.items[k];
+      const bool live = std::find(L.group[0].begin(), L.group[0].end(), &R) != L.group[0].end() ||
+                        std::find(L.group[1].begin(), L.group[1].end(), &R) != L.group[1].end();
+      if (live && h[k].status != ST_RUNNING) {
+        int rc = batch_finish(L, R, h[k]);
+        if (rc) return rc;
+      }
+    }
+    L.steps.pop_front();
+    moved = true;
+  }
+  for (size_t i = 0; i < L.finishing.size();) {
+    cfd_server::Req& R = *L.finishing[i];
+    // a step still to be read names the solve: it is handed back after that step
+    bool named = false;
+    for (const cfd_server::Step& st : L.steps)
+      named = named || std::find(st.items.begin(), st.items.end(), &R) != st.items.end();
+    hipError_t q = named ? hipErrorNotReady : hipEventQuery(R.g->evs);
     if (q == hipErrorNotReady) {
       ++i;
       continue;
@@ -2422,51 +2428,121 @@ int batch_progress(cfd_server* s) {
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, R.g->ev0, R.g->ev1));
     R.info.seconds = ms * 1e-3;
-    end_request(s, R, R.rc, R.error);
-    s->finishing[i] = s->finishing.back();
-    s->finishing.pop_back();
+    R.t_end = now_s();
+    R.state.store(2, std::memory_order_release);
+    L.finishing[i] = L.finishing.back();
+    L.finishing.pop_back();
+    moved = true;
+  }
+  while ((!L.group[0].empty() || !L.group[1].empty()) && (int)L.steps.size() < s->ahead) {
+    int rc = batch_step(L);
+    if (rc) return rc;
+    moved = true;
   }
   return 0;
 }
 
-// Steps are queued in every lane with a solve, at most kStepsAhead past the last
-// step seen finished.
-int batch_fill(cfd_server* s) {
-  for (cfd_server::Lane& L : s->lanes)
-    while ((!L.group[0].empty() || !L.group[1].empty()) && (int)L.steps.size() < s->ahead) {
-      int rc = batch_step(s, L);
-      if (rc) return rc;
+// A lane's thread: passes while the lane has work (taking the server's lock
+// only to pick up joining solves), else it waits for a joining solve (or the
+// server's end).
+void lane_worker(cfd_server* s, cfd_server::Lane* L, int device) {
+  (void)hipSetDevice(device);
+  for (;;) {
+    const bool idle = L->group[0].empty() && L->group[1].empty() && L->steps.empty() && L->finishing.empty();
+    if (idle && !L->has_joining.load(std::memory_order_acquire)) {
+      std::unique_lock<std::mutex> lk(s->m);
+      s->cv.wait(lk, [&] { return s->stop || !L->joining.empty(); });
+      if (s->stop && L->joining.empty()) return;
     }
+    bool moved = false;
+    const int rc = lane_pass(s, *L, moved);
+    if (rc) {
+      // the lane stops; its solves end with the device error
+      const std::string error = g_error;
+      std::vector<cfd_server::Req*> all;
+      for (int grp = 0; grp < 2; ++grp) all.insert(all.end(), L->group[grp].begin(), L->group[grp].end());
+      all.insert(all.end(), L->finishing.begin(), L->finishing.end());
+      {
+        std::lock_guard<std::mutex> lk(s->m);
+        all.insert(all.end(), L->joining.begin(), L->joining.end());
+        L->joining.clear();
+        L->failed = rc;
+      }
+      for (cfd_server::Req* R : all) {
+        R->rc = rc;
+        R->error = error;
+        R->t_end = now_s();
+        R->state.store(2, std::memory_order_release);
+      }
+      L->group[0].clear();
+      L->group[1].clear();
+      L->finishing.clear();
+      L->steps.clear();
+      continue;
+    }
+    if (!moved) std::this_thread::yield();
+  }
+}
+
+int ensure_lanes(cfd_server* s) {
+  if ((int)s->lanes.size() == s->n_lanes) return 0;
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  for (int i = (int)s->lanes.size(); i < s->n_lanes; ++i) {
+    std::unique_ptr<cfd_server::Lane> L(new cfd_server::Lane());
+    HIP_TRY(hipStreamCreateWithFlags(&L->bs, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void**)&L->h_states, (size_t)kStepRing * 2 * kBatchMax * sizeof(PcgState),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    HIP_TRY(hipHostGetDevicePointer((void**)&L->hd_states, L->h_states, 0));
+    for (hipEvent_t& e : L->step_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    L->worker = std::thread(lane_worker, s, L.get(), device);
+    s->lanes.push_back(std::move(L));
+  }
   return 0;
 }
 
-int active_count(const cfd_server* s) {
-  size_t n = s->running.size() + s->finishing.size();
-  for (const cfd_server::Lane& L : s->lanes) n += L.group[0].size() + L.group[1].size();
-  return (int)n;
+void end_request(cfd_server* s, cfd_server::Req& R, int rc, const std::string& error) {
+  R.t_end = now_s();
+  R.rc = rc;
+  R.error = error;
+  R.state.store(2, std::memory_order_relaxed);
+  R.g->opt.precision = R.grid_precision;
+  s->busy.erase(R.g);
 }
 
 // The lane a batched request of precision group grp joins: the one with the
-// fewest solves that has room in that group (-1: none).
-int pick_lane(const cfd_server* s, int grp) {
+// least load that has room in that group (-1: none). Load counts the solves
+// assigned and not yet collected (at most kBatchMax per precision, shared by
+// both groups here).
+int pick_lane(const cfd_server* s) {
   int best = -1;
-  size_t best_n = 0;
   for (size_t i = 0; i < s->lanes.size(); ++i) {
-    const cfd_server::Lane& L = s->lanes[i];
-    if ((int)L.group[grp].size() >= kBatchMax) continue;
-    const size_t n = L.group[0].size() + L.group[1].size();
-    if (best < 0 || n < best_n) {
-      best = (int)i;
-      best_n = n;
-    }
+    const cfd_server::Lane& L = *s->lanes[i];
+    if (L.failed || L.load >= kBatchMax) continue;
+    if (best < 0 || L.load < s->lanes[(size_t)best]->load) best = (int)i;
   }
   return best;
 }
 
-// Drives every running job one step and starts queued requests whose grid is idle.
+int active_count(const cfd_server* s) { return (int)(s->running.size() + s->batched.size()); }
+
+// Drives the requests on their grids' own streams one step, collects batched
+// requests that their lanes finished and starts queued requests whose grid is
+// idle (batched ones are handed to a lane's thread).
 int server_progress(cfd_server* s) {
-  int rc = batch_progress(s);
-  if (rc) return -rc;
+  std::lock_guard<std::mutex> lk(s->m);
+  for (size_t i = 0; i < s->batched.size();) {
+    cfd_server::Req& R = s->reqs[s->batched[i]];
+    if (R.state.load(std::memory_order_acquire) != 2) {
+      ++i;
+      continue;
+    }
+    s->lanes[(size_t)R.lane]->load -= 1;
+    R.g->opt.precision = R.grid_precision;
+    s->busy.erase(R.g);
+    s->batched[i] = s->batched.back();
+    s->batched.pop_back();
+  }
   for (size_t i = 0; i < s->running.size();) {
     cfd_server::Req& R = s->reqs[s->running[i]];
     int jr = job_poll(R.g, false, nullptr, nullptr);
@@ -2483,6 +2559,7 @@ int server_progress(cfd_server* s) {
     s->running[i] = s->running.back();
     s->running.pop_back();
   }
+  bool handed = false;
   for (size_t i = 0; i < s->queued.size() && active_count(s) < s->max_active;) {
     const uint64_t id = s->queued[i];
     cfd_server::Req& R = s->reqs[id];
@@ -2491,11 +2568,10 @@ int server_progress(cfd_server* s) {
       continue;
     }
     const bool batch = s->n_lanes > 0 && batchable(R.g, R.precision);
-    const int grp = R.precision == CFD_PRECISION_MIXED ? 1 : 0;
     int lane = -1;
     if (batch) {
-      if ((rc = ensure_lanes(s))) return -rc;
-      lane = pick_lane(s, grp);
+      if (int rc = ensure_lanes(s)) return -rc;
+      lane = pick_lane(s);
       if (lane < 0) {
         ++i;
         continue;
@@ -2505,25 +2581,33 @@ int server_progress(cfd_server* s) {
     // and learned counts are keyed by it); the grid's own setting comes back after
     R.grid_precision = R.g->opt.precision;
     R.g->opt.precision = R.precision;
-    rc = batch ? batch_begin(s, lane, R) : job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
+    int rc = 0;
+    if (!batch) rc = job_begin(R.g, R.b, R.scale, R.tol, R.max, R.x);
     R.t_start = now_s();
     s->queued.erase(s->queued.begin() + (long)i);
     if (rc) {
       R.g->opt.precision = R.grid_precision;
-      R.state = 2;
       R.rc = rc;
       R.error = g_error;
       R.t_end = R.t_start;
+      R.state.store(2, std::memory_order_relaxed);
       continue;
     }
-    R.state = 1;
+    R.state.store(1, std::memory_order_relaxed);
     s->busy[R.g] = 1;
-    if (batch)
-      s->lanes[(size_t)lane].group[grp].push_back(id);
-    else
+    if (batch) {
+      R.lane = lane;
+      cfd_server::Lane& L = *s->lanes[(size_t)lane];
+      L.load += 1;
+      L.joining.push_back(&R);
+      L.has_joining.store(true, std::memory_order_release);
+      s->batched.push_back(id);
+      handed = true;
+    } else {
       s->running.push_back(id);
+    }
   }
-  if ((rc = batch_fill(s))) return -rc;
+  if (handed) s->cv.notify_all();
   return (int)(active_count(s) + s->queued.size());
 }
 }  // namespace
@@ -2552,6 +2636,7 @@ int cfd_server_set_batching(cfd_server* s, int lanes) {
     set_error("cfd_server_set_batching: lanes must be 0 .. 8");
     return CFD_EINVAL;
   }
+  std::lock_guard<std::mutex> lk(s->m);
   if (!s->lanes.empty() && lanes != (int)s->lanes.size()) {
     set_error("cfd_server_set_batching: the lanes are set before the first request");
     return CFD_EINVAL;
@@ -2601,8 +2686,8 @@ int cfd_server_release(cfd_server* s, cfd_grid* g) {
       return CFD_EINVAL;
     }
   // batched steps queued before the last solve on the grid ended still name it
-  for (cfd_server::Lane& L : s->lanes)
-    if (L.bs) HIP_TRY(hipStreamSynchronize(L.bs));
+  for (auto& L : s->lanes)
+    if (L->bs) HIP_TRY(hipStreamSynchronize(L->bs));
   s->used -= it->second;
   s->grids.erase(it);
   cfd_grid_destroy(g);
@@ -2628,6 +2713,7 @@ int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const doubl
     set_error("cfd_server_submit: the reference and zebra cycles run in FP64 only");
     return CFD_EINVAL;
   }
+  std::unique_lock<std::mutex> lk(s->m);
   const uint64_t id = s->next++;
   cfd_server::Req& R = s->reqs[id];
   R.g = g;
@@ -2639,6 +2725,7 @@ int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const doubl
   R.precision = precision;
   R.t_submit = now_s();
   s->queued.push_back(id);
+  lk.unlock();
   if (ticket) *ticket = id;
   if (server_progress(s) < 0) return CFD_EDEVICE;
   return CFD_OK;
@@ -2652,9 +2739,15 @@ int cfd_server_wait(cfd_server* s, uint64_t ticket) {
     set_error("cfd_server_wait: unknown ticket");
     return CFD_EINVAL;
   }
-  while (it->second.state != 2) {
+  // collected: done, and (batched) handed back by its lane
+  auto collected = [&] {
+    std::lock_guard<std::mutex> lk(s->m);
+    return it->second.state.load(std::memory_order_acquire) == 2 &&
+           std::find(s->batched.begin(), s->batched.end(), ticket) == s->batched.end();
+  };
+  while (!collected()) {
     if (server_progress(s) < 0) return CFD_EDEVICE;
-    if (it->second.state != 2) std::this_thread::yield();
+    std::this_thread::yield();
   }
   return CFD_OK;
 }
@@ -2676,9 +2769,13 @@ int cfd_server_result(cfd_server* s, uint64_t ticket, cfd_linear_info* info, dou
     return CFD_EINVAL;
   }
   cfd_server::Req& R = it->second;
-  if (R.state != 2) {
-    set_error("cfd_server_result: the request has not finished");
-    return CFD_EINVAL;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    if (R.state.load(std::memory_order_acquire) != 2 ||
+        std::find(s->batched.begin(), s->batched.end(), ticket) != s->batched.end()) {
+      set_error("cfd_server_result: the request has not finished");
+      return CFD_EINVAL;
+    }
   }
   if (info) *info = R.info;
   if (latency_s) *latency_s = R.t_end - R.t_submit;

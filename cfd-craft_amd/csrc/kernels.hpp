@@ -182,7 +182,8 @@ void launch_batch_iteration(const BatchSolve<T>* items, int n, hipStream_t s);
 template <class T>
 void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
                        hipStream_t s);
-// The PcgState of each solve copied to dst[i] (device memory).
+// The PcgState of each solve copied to dst[i] (device memory or coherent pinned
+// host memory).
 void launch_batch_states(PcgState* const* st, int n, PcgState* dst, hipStream_t s);
 // The reference's cycle: fine level from the codes (diagonal diag0 in the
 // reference's summation order), coarse CSR levels in the reference's numbering.
