@@ -278,6 +278,20 @@ struct cfd_grid {
 
 namespace {
 
+// Set-up stage times on stderr (CFD_SETUP_TIMES=1): the seconds since the
+// previous mark of the same thread.
+struct StageClock {
+  bool on = std::getenv("CFD_SETUP_TIMES") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char* what) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[setup] %-28s %8.1f ms\n", what, 1e3 * std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+thread_local StageClock t_clock;
+
 int fill_geo(cfd_grid* g) {
   const HostGrid& h = g->host;
   Geo& G = g->geo;
@@ -412,8 +426,11 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
 int build_agg(cfd_grid* g) {
   if (g->ash.built) return 0;
   const HostGrid& h = g->host;
-  int rc = build_agg_hierarchy(h, g->agg, g->agg_dense, kHostBlock, true);
+  t_clock.mark("before hierarchy");
+  // (the colouring below forms the children lists in its numbering)
+  int rc = build_agg_hierarchy(h, g->agg, g->agg_dense, kHostBlock, true, false);
   if (rc) return rc;
+  t_clock.mark("host hierarchy");
   if (g->agg.fine_shift > 1) {
     // the fine-level restrictions sum each 2x2x2 bin's residuals per aggregate
     set_error("multigrid: no fluid cells of the grid share a 2x2x2 bin with a face neighbour; the red-black "
@@ -421,6 +438,7 @@ int build_agg(cfd_grid* g) {
     return CFD_EUNSUPPORTED;
   }
   colour_agg_hierarchy(g->agg);
+  t_clock.mark("colouring");
   AggHierarchy& H = g->agg;
   DevAggShared& A = g->ash;
   A.nlev = (int)H.levels.size() - 1;
@@ -428,11 +446,13 @@ int build_agg(cfd_grid* g) {
   for (int a = 0; a < 3; ++a)
     if (h.periodic[a] && h.n[a] >= 3 && (h.n[a] & 1)) A.seams |= 1 << a;
   if (A.nlev > 0) {
-    std::vector<int> fp((size_t)h.count, -1);
-    for (int64_t c = 0; c < h.count; ++c) {
-      const int32_t node = H.node_of_cell[(size_t)c];
-      if (node >= 0) fp[(size_t)c] = H.levels[0].parent[(size_t)node];
-    }
+    std::vector<int> fp((size_t)h.count);
+    parallel_ranges(h.count, [&](int64_t c0, int64_t c1) {
+      for (int64_t c = c0; c < c1; ++c) {
+        const int32_t node = H.node_of_cell[(size_t)c];
+        fp[(size_t)c] = node >= 0 ? H.levels[0].parent[(size_t)node] : -1;
+      }
+    });
     if ((rc = A.fparent.upload(fp.data(), fp.size()))) return rc;
   }
   auto upi = [&](const int32_t* v, size_t n, const int** out) -> int {
@@ -452,6 +472,7 @@ int build_agg(cfd_grid* g) {
     }
     return upi(t.data(), t.size(), out);
   };
+  t_clock.mark("fine parents");
   A.shape.clear();
   A.ell_values.clear();
   for (int l = 1; l <= A.nlev; ++l) {
@@ -516,6 +537,7 @@ int build_agg(cfd_grid* g) {
     A.shape.push_back(sh);
     A.ell_values.push_back(std::move(table));
   }
+  t_clock.mark("coarse levels (ELL)");
   // coarsest blocks (on the fine level itself the rows are cells)
   std::vector<int32_t> rows = H.block_rows;
   if (A.nlev == 0)
@@ -572,8 +594,10 @@ int build_agg(cfd_grid* g) {
     }
     H.deferred.clear();
   }
+  t_clock.mark("coarsest blocks");
   if ((rc = upload_agg_values<double>(g, g->a64))) return rc;
   if ((rc = upload_agg_values<float>(g, g->a32))) return rc;
+  t_clock.mark("level values");
   if (A.seams == 0 && A.nlev > 0 && g->rb_layout) {
     const int hx = (int)((h.n[0] + 1) / 2);
     const long long half = (long long)hx * h.n[1] * h.n[2];
@@ -583,23 +607,26 @@ int build_agg(cfd_grid* g) {
     std::vector<int> bpar((size_t)(bx * by * bz), 0);
     std::vector<uint8_t> bseen(bpar.size(), 0);
     const std::vector<int32_t>& parent0 = H.levels[0].parent;
-    for (int64_t k = 0, c = 0; k < h.n[2]; ++k)
-      for (int64_t j = 0; j < h.n[1]; ++j)
-        for (int64_t i = 0; i < h.n[0]; ++i, ++c) {
-          const int colour = (int)((i + j + k) & 1);
-          const size_t u = (size_t)((i >> 1) + (long long)hx * (j + h.n[1] * k));
-          rc8[(size_t)colour * (size_t)half + u] = h.codes[(size_t)c];
-          const int32_t node = H.node_of_cell[(size_t)c];
-          const int par = node >= 0 ? parent0[(size_t)node] : -1;
-          if (colour == 0 && node >= 0) rpar[u] = par;
-          const size_t blk = (size_t)((i >> 1) + bx * ((j >> 1) + by * (k >> 1)));
-          if (!bseen[blk]) {
-            bseen[blk] = 1;
-            bpar[blk] = par;
-          } else if (bpar[blk] != par) {
-            bpar[blk] = kMixedParent;
+    // layers in pairs (a 2x2x2 block's cells lie in one pair), in parallel
+    parallel_ranges(bz, [&](int64_t z0, int64_t z1) {
+      for (int64_t k = 2 * z0; k < std::min<int64_t>(h.n[2], 2 * z1); ++k)
+        for (int64_t j = 0, c = k * h.n[0] * h.n[1]; j < h.n[1]; ++j)
+          for (int64_t i = 0; i < h.n[0]; ++i, ++c) {
+            const int colour = (int)((i + j + k) & 1);
+            const size_t u = (size_t)((i >> 1) + (long long)hx * (j + h.n[1] * k));
+            rc8[(size_t)colour * (size_t)half + u] = h.codes[(size_t)c];
+            const int32_t node = H.node_of_cell[(size_t)c];
+            const int par = node >= 0 ? parent0[(size_t)node] : -1;
+            if (colour == 0 && node >= 0) rpar[u] = par;
+            const size_t blk = (size_t)((i >> 1) + bx * ((j >> 1) + by * (k >> 1)));
+            if (!bseen[blk]) {
+              bseen[blk] = 1;
+              bpar[blk] = par;
+            } else if (bpar[blk] != par) {
+              bpar[blk] = kMixedParent;
+            }
           }
-        }
+    });
     if ((rc = A.rb_codes.upload(rc8.data(), rc8.size())) || (rc = A.rb_rpar.upload(rpar.data(), rpar.size())) ||
         (rc = A.rb_bpar.upload(bpar.data(), bpar.size())) ||
         (rc = A.rb_b64.alloc((size_t)(2 * half))) ||
@@ -626,6 +653,7 @@ int build_agg(cfd_grid* g) {
     for (unsigned long long* m : {&A.rb64.mny, &A.rb32.mny})
       *m = (two40 + (unsigned long long)h.n[1] - 1) / (unsigned long long)h.n[1];
     A.has_rb = true;
+    t_clock.mark("fine colour layout");
   }
   // the host copies of the coarse matrices are no longer needed
   for (auto& L : H.levels) {
@@ -1515,12 +1543,15 @@ int create_grid(const cfd_case* c, cfd_grid** out, bool force_pairs) {
   }
   std::unique_ptr<cfd_grid> g(new cfd_grid());
   g->force_pairs = force_pairs;
+  t_clock.t = std::chrono::steady_clock::now();
   // rasterization, face codes and components on the device (CFD_GPU_SETUP=0: on the host)
   const char* env = std::getenv("CFD_GPU_SETUP");
   int rc = env && std::string(env) == "0" ? build_host_grid(*c, g->host) : build_device_grid(*c, g->host);
   if (rc) return rc;
+  t_clock.mark("topology");
   rc = setup_device(g.get());
   if (rc) return rc;
+  t_clock.mark("end");
   *out = g.release();
   return CFD_OK;
 }

@@ -14,6 +14,9 @@
 // numbering (the lexicographic cycle) and a colour-sorted numbering in which
 // the nodes of one independent set are contiguous (the red-black cycle).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstdlib>
 #include <functional>
@@ -27,19 +30,6 @@ namespace cfdgpu {
 namespace {
 
 
-// Runs f(begin, end) over [0, n) split among the host's threads (at most 16).
-template <class F>
-void parallel_ranges(int64_t n, F f) {
-  const int64_t threads = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), n}));
-  if (threads == 1) {
-    f((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> pool;
-  for (int64_t t = 0; t < threads; ++t)
-    pool.emplace_back([&, t] { f(n * t / threads, n * (t + 1) / threads); });
-  for (auto& th : pool) th.join();
-}
 
 struct UnionFind {
   std::vector<int32_t> p;
@@ -445,18 +435,37 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
         for (int u = 0; u < cnt; ++u) low[(size_t)node[(size_t)cell[u]]] = node[(size_t)cell[find(u)]];
       }
   });
+  // aggregates numbered by their lowest node: the roots counted per chunk,
+  // numbered from the chunk's offset, then every other node takes its root's
   parent.assign((size_t)L0.n, -1);
-  coarse_coords.clear();
-  int64_t m = 0;
-  for (int64_t i = 0; i < L0.n; ++i) {
-    if (low[(size_t)i] == (int32_t)i) {
-      parent[(size_t)i] = (int32_t)m++;
-      for (int d = 0; d < 3; ++d) coarse_coords.push_back(L0.coords[3 * (size_t)i + d] >> 1);
-    } else {
-      parent[(size_t)i] = parent[(size_t)low[(size_t)i]];
+  constexpr int64_t kChunks = 64;
+  std::vector<int64_t> roots(kChunks + 1, 0);
+  parallel_ranges(kChunks, [&](int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      int64_t r = 0;
+      for (int64_t i = L0.n * t / kChunks; i < L0.n * (t + 1) / kChunks; ++i) r += low[(size_t)i] == (int32_t)i;
+      roots[(size_t)t + 1] = r;
     }
-  }
+  });
+  for (int64_t t = 0; t < kChunks; ++t) roots[(size_t)t + 1] += roots[(size_t)t];
+  const int64_t m = roots[kChunks];
   if (m == L0.n) return false;
+  coarse_coords.assign(3 * (size_t)m, 0);
+  parallel_ranges(kChunks, [&](int64_t t0, int64_t t1) {
+    for (int64_t t = t0; t < t1; ++t) {
+      int64_t next = roots[(size_t)t];
+      for (int64_t i = L0.n * t / kChunks; i < L0.n * (t + 1) / kChunks; ++i)
+        if (low[(size_t)i] == (int32_t)i) {
+          parent[(size_t)i] = (int32_t)next;
+          for (int d = 0; d < 3; ++d) coarse_coords[3 * (size_t)next + d] = L0.coords[3 * (size_t)i + d] >> 1;
+          ++next;
+        }
+    }
+  });
+  parallel_ranges(L0.n, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i)
+      if (low[(size_t)i] != (int32_t)i) parent[(size_t)i] = parent[(size_t)low[(size_t)i]];
+  });
   // coarse edges: each interface between a bin and its upper neighbour along an
   // axis (wrapping on a periodic axis of three or more bins), its faces in face
   // order (the two other axes, the higher one outer)
@@ -548,68 +557,98 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
 
 }  // namespace
 
+namespace {
+// Stage times of the hierarchy build on stderr (CFD_SETUP_TIMES=1).
+struct AggClock {
+  bool on = std::getenv("CFD_SETUP_TIMES") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char* what, int64_t level = -1) {
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[setup]   %-24s %3lld %8.1f ms\n", what, (long long)level,
+                 1e3 * std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+}  // namespace
+
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells, int64_t host_block_max,
-                        bool semicoarsen) {
+                        bool semicoarsen, bool children) {
+  AggClock clk;
   h = AggHierarchy();
   if (g.fluid_count == 0) {
     set_error("CFD-E034: numerics.pressure_preconditioner: multigrid requires fluid cells");
     return CFD_E034;
   }
   const int64_t* n = g.n;
-  // Level 0: fluid cells in index order.
+  // Level 0: fluid cells in index order (numbered from per-layer counts, the
+  // layers filled in parallel).
   h.node_of_cell.assign((size_t)g.count, -1);
   AggLevel L0;
-  h.cell_of_node.reserve((size_t)g.fluid_count);
-  L0.coords.reserve(3 * (size_t)g.fluid_count);
-  for (int64_t k = 0, c = 0; k < n[2]; ++k)
-    for (int64_t j = 0; j < n[1]; ++j)
-      for (int64_t i = 0; i < n[0]; ++i, ++c)
-        if (g.fluid[(size_t)c]) {
-          h.node_of_cell[(size_t)c] = (int32_t)L0.n;
-          h.cell_of_node.push_back(c);
-          L0.coords.push_back((int32_t)i);
-          L0.coords.push_back((int32_t)j);
-          L0.coords.push_back((int32_t)k);
-          ++L0.n;
-        }
-  const auto& node = h.node_of_cell;
-  std::vector<int64_t> comp_size(g.anchored.size(), 0);
-  for (int64_t c = 0; c < g.count; ++c)
-    if (g.component[(size_t)c] >= 0) comp_size[(size_t)g.component[(size_t)c]] += 1;
+  const int64_t layer = n[0] * n[1];
+  std::vector<int64_t> first((size_t)n[2] + 1, 0);
+  parallel_ranges(n[2], [&](int64_t k0, int64_t k1) {
+    for (int64_t k = k0; k < k1; ++k) {
+      int64_t m = 0;
+      for (int64_t c = k * layer; c < (k + 1) * layer; ++c) m += g.fluid[(size_t)c] ? 1 : 0;
+      first[(size_t)k + 1] = m;
+    }
+  });
+  for (int64_t k = 0; k < n[2]; ++k) first[(size_t)k + 1] += first[(size_t)k];
+  L0.n = first[(size_t)n[2]];
+  h.cell_of_node.resize((size_t)L0.n);
+  L0.coords.resize(3 * (size_t)L0.n);
   // Per node, in face order (x faces, then y, then z; every face of an axis
   // carries that axis's weight): the anchor sums its outlet sides (a coupled
   // side on a non-periodic domain face), the diagonal is the anchor plus the
   // weights of its interior sides; each interior face has two such sides.
   L0.anchor.assign((size_t)L0.n, 0.0);
   L0.diag.assign((size_t)L0.n, 0.0);
+  std::vector<int64_t> sides_of((size_t)n[2], 0);
+  parallel_ranges(n[2], [&](int64_t k0, int64_t k1) {
+    for (int64_t k = k0; k < k1; ++k) {
+      int64_t v = first[(size_t)k], sides = 0;
+      for (int64_t j = 0, c = k * layer; j < n[1]; ++j)
+        for (int64_t i = 0; i < n[0]; ++i, ++c) {
+          if (!g.fluid[(size_t)c]) continue;
+          h.node_of_cell[(size_t)c] = (int32_t)v;
+          h.cell_of_node[(size_t)v] = c;
+          L0.coords[3 * (size_t)v] = (int32_t)i;
+          L0.coords[3 * (size_t)v + 1] = (int32_t)j;
+          L0.coords[3 * (size_t)v + 2] = (int32_t)k;
+          const uint8_t code = g.codes[(size_t)c];
+          const int64_t q[3] = {i, j, k};
+          int inner[3] = {0, 0, 0};
+          double anchor = 0.0;
+          for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) {
+              if (!(code & side_bit(a, side))) continue;
+              if (g.periodic[a] || (side == 0 ? q[a] > 0 : q[a] + 1 < n[a]))
+                ++inner[a];
+              else
+                anchor += g.w_out[a];
+            }
+          double d = anchor;
+          for (int a = 0; a < 3; ++a)
+            for (int t = 0; t < inner[a]; ++t) d += g.w_int[a];
+          L0.anchor[(size_t)v] = anchor;
+          L0.diag[(size_t)v] = d;
+          sides += inner[0] + inner[1] + inner[2];
+          ++v;
+        }
+      sides_of[(size_t)k] = sides;
+    }
+  });
+  const auto& node = h.node_of_cell;
+  std::vector<int64_t> comp_size(g.anchored.size(), 0);
+  for (int64_t c = 0; c < g.count; ++c)
+    if (g.component[(size_t)c] >= 0) comp_size[(size_t)g.component[(size_t)c]] += 1;
   int64_t sides0 = 0;
-  for (int64_t k = 0, c = 0; k < n[2]; ++k)
-    for (int64_t j = 0; j < n[1]; ++j)
-      for (int64_t i = 0; i < n[0]; ++i, ++c) {
-        if (!g.fluid[(size_t)c]) continue;
-        const uint8_t code = g.codes[(size_t)c];
-        const int64_t q[3] = {i, j, k};
-        int inner[3] = {0, 0, 0};
-        double anchor = 0.0;
-        for (int a = 0; a < 3; ++a)
-          for (int side = 0; side < 2; ++side) {
-            if (!(code & side_bit(a, side))) continue;
-            if (g.periodic[a] || (side == 0 ? q[a] > 0 : q[a] + 1 < n[a]))
-              ++inner[a];
-            else
-              anchor += g.w_out[a];
-          }
-        double d = anchor;
-        for (int a = 0; a < 3; ++a)
-          for (int t = 0; t < inner[a]; ++t) d += g.w_int[a];
-        const int32_t v = node[(size_t)c];
-        L0.anchor[(size_t)v] = anchor;
-        L0.diag[(size_t)v] = d;
-        sides0 += inner[0] + inner[1] + inner[2];
-      }
+  for (int64_t k = 0; k < n[2]; ++k) sides0 += sides_of[(size_t)k];
   const int64_t faces0 = sides0 / 2;
   h.level0_entries = 2 * faces0;
   h.levels.push_back(std::move(L0));
+  clk.mark("level 0 nodes, diagonal");
   // level 0's edge list in face order, formed only where it is read: the CSR of
   // a coarsest level 0 and the general aggregation (bins other than 2x2x2)
   const auto list_level0 = [&](AggLevel& L) {
@@ -651,8 +690,10 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
       }
     } else {
       csr_from_edges(L);
+      clk.mark("csr", (int64_t)h.levels.size() - 1);
       level_components(L, comps, anch);
     }
+    clk.mark("components", (int64_t)h.levels.size() - 1);
 
     L.closed_ptr.assign(1, 0);
     bool small = true;
@@ -697,8 +738,14 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     bool coarsen[3] = {true, true, true};
     if (semicoarsen) {
       int32_t ext[3] = {0, 0, 0};
-      for (int64_t i = 0; i < L.n; ++i)
-        for (int d = 0; d < 3; ++d) ext[d] = std::max(ext[d], L.coords[3 * (size_t)i + d] + 1);
+      std::mutex mx;
+      parallel_ranges(L.n, [&](int64_t i0, int64_t i1) {
+        int32_t e[3] = {0, 0, 0};
+        for (int64_t i = i0; i < i1; ++i)
+          for (int d = 0; d < 3; ++d) e[d] = std::max(e[d], L.coords[3 * (size_t)i + d] + 1);
+        std::lock_guard<std::mutex> lk(mx);
+        for (int d = 0; d < 3; ++d) ext[d] = std::max(ext[d], e[d]);
+      });
       double wmax = 0.0;
       for (int d = 0; d < 3; ++d)
         if (ext[d] > 1) wmax = std::max(wmax, w[d]);
@@ -712,7 +759,9 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     AggLevel C;
     bool binned = false;
     if (fine && coarsen[0] && coarsen[1] && coarsen[2] && !general) {
+      clk.mark("before bins", 0);
       binned = bins_level0(g, node, L, parent, cc, C);
+      clk.mark("bins level 0", 0);
       if (binned) {
         m = (int64_t)cc.size() / 3;
         shift[0] = shift[1] = shift[2] = 1;
@@ -753,7 +802,9 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
     C.coords = std::move(cc);
     C.anchor.assign((size_t)m, 0.0);
     for (int64_t i = 0; i < L.n; ++i) C.anchor[(size_t)parent[(size_t)i]] += L.anchor[(size_t)i];
+    clk.mark("aggregate", (int64_t)h.levels.size() - 1);
     if (!binned) coarse_edges(m, parent, weighted, C);
+    clk.mark("coarse edges", (int64_t)h.levels.size() - 1);
     L.parent = std::move(parent);
     // the finer edge list is not needed once the next level exists
     std::vector<int32_t>().swap(L.ea);
@@ -764,7 +815,8 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
   std::vector<int32_t>().swap(h.levels.back().ea);
   std::vector<int32_t>().swap(h.levels.back().eb);
   std::vector<double>().swap(h.levels.back().ew);
-  build_children(h);
+  if (children) build_children(h);
+  clk.mark("children");
   return 0;
 }
 
@@ -1295,15 +1347,21 @@ void colour_agg_hierarchy(AggHierarchy& h) {
     // face-adjacent bins) except across an odd periodic seam; fall back to a
     // greedy colouring in node order where it fails.
     std::vector<int32_t> colour((size_t)n);
-    bool ok = true;
-    for (int64_t i = 0; i < n; ++i)
-      colour[(size_t)i] = (L.coords[3 * (size_t)i] + L.coords[3 * (size_t)i + 1] + L.coords[3 * (size_t)i + 2]) & 1;
-    for (int64_t i = 0; i < n && ok; ++i)
-      for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
-        if (colour[(size_t)L.col[(size_t)e]] == colour[(size_t)i]) {
-          ok = false;
-          break;
-        }
+    parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        colour[(size_t)i] =
+            (L.coords[3 * (size_t)i] + L.coords[3 * (size_t)i + 1] + L.coords[3 * (size_t)i + 2]) & 1;
+    });
+    std::atomic<bool> clash{false};
+    parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1 && !clash.load(std::memory_order_relaxed); ++i)
+        for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e)
+          if (colour[(size_t)L.col[(size_t)e]] == colour[(size_t)i]) {
+            clash.store(true, std::memory_order_relaxed);
+            break;
+          }
+    });
+    bool ok = !clash.load();
     int ncol = 2;
     if (!ok) {
       ncol = 1;
@@ -1333,7 +1391,9 @@ void colour_agg_hierarchy(AggHierarchy& h) {
     // permute this level's rows, columns, diagonal, anchors, coordinates and parent
     std::vector<int64_t> ptr((size_t)n + 1, 0);
     std::vector<int32_t> inv((size_t)n);
-    for (int64_t i = 0; i < n; ++i) inv[(size_t)perm[(size_t)i]] = (int32_t)i;
+    parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) inv[(size_t)perm[(size_t)i]] = (int32_t)i;
+    });
     for (int64_t r = 0; r < n; ++r) {
       const int32_t o = inv[(size_t)r];
       ptr[(size_t)r + 1] = ptr[(size_t)r] + (L.ptr[(size_t)o + 1] - L.ptr[(size_t)o]);
@@ -1342,18 +1402,20 @@ void colour_agg_hierarchy(AggHierarchy& h) {
     std::vector<double> val((size_t)ptr[(size_t)n]), diag((size_t)n), anchor((size_t)n);
     std::vector<int32_t> coords((size_t)(3 * n));
     std::vector<int32_t> parent(L.parent.empty() ? 0 : (size_t)n);
-    for (int64_t r = 0; r < n; ++r) {
-      const int32_t o = inv[(size_t)r];
-      int64_t at = ptr[(size_t)r];
-      for (int64_t e = L.ptr[(size_t)o]; e < L.ptr[(size_t)o + 1]; ++e, ++at) {
-        col[(size_t)at] = perm[(size_t)L.col[(size_t)e]];
-        val[(size_t)at] = L.val[(size_t)e];
+    parallel_ranges(n, [&](int64_t r0, int64_t r1) {
+      for (int64_t r = r0; r < r1; ++r) {
+        const int32_t o = inv[(size_t)r];
+        int64_t at = ptr[(size_t)r];
+        for (int64_t e = L.ptr[(size_t)o]; e < L.ptr[(size_t)o + 1]; ++e, ++at) {
+          col[(size_t)at] = perm[(size_t)L.col[(size_t)e]];
+          val[(size_t)at] = L.val[(size_t)e];
+        }
+        diag[(size_t)r] = L.diag[(size_t)o];
+        anchor[(size_t)r] = L.anchor[(size_t)o];
+        for (int d = 0; d < 3; ++d) coords[3 * (size_t)r + d] = L.coords[3 * (size_t)o + d];
+        if (!parent.empty()) parent[(size_t)r] = L.parent[(size_t)o];
       }
-      diag[(size_t)r] = L.diag[(size_t)o];
-      anchor[(size_t)r] = L.anchor[(size_t)o];
-      for (int d = 0; d < 3; ++d) coords[3 * (size_t)r + d] = L.coords[3 * (size_t)o + d];
-      if (!parent.empty()) parent[(size_t)r] = L.parent[(size_t)o];
-    }
+    });
     L.ptr.swap(ptr);
     L.col.swap(col);
     L.val.swap(val);
@@ -1365,7 +1427,10 @@ void colour_agg_hierarchy(AggHierarchy& h) {
     for (size_t c = 0; c + 1 < L.closed_ptr.size(); ++c)
       std::sort(L.closed_members.begin() + L.closed_ptr[c], L.closed_members.begin() + L.closed_ptr[c + 1]);
     // the finer level's parents point at the new numbers
-    for (auto& p : h.levels[l - 1].parent) p = perm[(size_t)p];
+    std::vector<int32_t>& fpar = h.levels[l - 1].parent;
+    parallel_ranges((int64_t)fpar.size(), [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) fpar[(size_t)i] = perm[(size_t)fpar[(size_t)i]];
+    });
     if (l + 1 == nl)
       for (auto& r : h.block_rows) r = perm[(size_t)r];
   }

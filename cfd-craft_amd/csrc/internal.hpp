@@ -1,7 +1,9 @@
 // Internal host-side structures of the cfdgpu library.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cfdgpu.h"
@@ -14,6 +16,20 @@ constexpr uint8_t kClosed = 1u << 7;
 inline uint8_t side_bit(int axis, int side) { return (uint8_t)(1u << (2 * axis + side)); }
 
 void set_error(const std::string& text);
+
+// Runs f(begin, end) over [0, n) split among the host's threads (at most 16).
+template <class F>
+void parallel_ranges(int64_t n, F f) {
+  const int64_t threads = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), n}));
+  if (threads == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int64_t t = 0; t < threads; ++t)
+    pool.emplace_back([&, t] { f(n * t / threads, n * (t + 1) / threads); });
+  for (auto& th : pool) th.join();
+}
 
 // Host description of a uniform staggered grid as the pressure solve sees it:
 // the restatement of Grid::build (grid.rs:246-464) in per-cell form.
@@ -164,9 +180,11 @@ uint64_t agg_hierarchy_digest(const AggHierarchy& h);
 // reference: 32); blocks above host_block_max nodes are left to the device.
 // semicoarsen: bin an axis only while its coupling is within kSemiRatio of the
 // strongest axis (the red-black cycle; the reference bins every axis).
+// children: form each coarse node's list of finer nodes (false when a
+// renumbering that forms them follows).
 constexpr double kSemiRatio = 25.0;
 int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells = 32,
-                        int64_t host_block_max = INT64_MAX, bool semicoarsen = false);
+                        int64_t host_block_max = INT64_MAX, bool semicoarsen = false, bool children = true);
 // The hierarchy the reference builds for a grid (multigrid.rs:530-645): masked
 // aggregation when the grid has solids or several components, else trilinear
 // interpolation with Galerkin coarse operators (with line or small-plane block
