@@ -286,6 +286,8 @@ def lib():
             "cfd_server_submit": (I, [P, P, P, P, D, I64, I, P, ctypes.POINTER(ctypes.c_uint64)]),
             "cfd_server_progress": (I, [P]),
             "cfd_server_wait": (I, [P, ctypes.c_uint64]),
+            "cfd_server_finished": (I, [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+            "cfd_grid_perturb": (I, [P, P, P, D, ctypes.c_uint64]),
             "cfd_server_drain": (I, [P]),
             "cfd_server_result": (I, [P, ctypes.c_uint64, P, P, P]),
         }
@@ -630,6 +632,13 @@ class Grid:
                                                        ctypes.byref(info)),
         )
 
+    def perturb(self, d_src: DeviceArray, d_dst: DeviceArray, rel: float, seed: int):
+        """d_dst = d_src + rel rms(d_src) xi on the fluid cells, xi a seeded standard
+        normal field (cfd_grid_perturb): a chained campaign right-hand side in HBM."""
+        ps = _device_vector(d_src, self.count, "d_src")
+        pd = _device_vector(d_dst, self.count, "d_dst")
+        _check(lib().cfd_grid_perturb(self.h, ps, pd, float(rel), ctypes.c_uint64(int(seed) & ((1 << 64) - 1))))
+
     def projection_rhs(self, u, dt):
         u = _host_vector(u, self.face_count, "u")
         b = np.empty(self.count)
@@ -732,6 +741,12 @@ class Server:
 
     def wait(self, ticket: int):
         _check(lib().cfd_server_wait(self.h, ctypes.c_uint64(ticket)))
+
+    def finished(self, ticket: int) -> bool:
+        """True once the request's result can be taken (result() does not wait)."""
+        done = ctypes.c_int(0)
+        _check(lib().cfd_server_finished(self.h, ctypes.c_uint64(ticket), ctypes.byref(done)))
+        return bool(done.value)
 
     def drain(self):
         _check(lib().cfd_server_drain(self.h))

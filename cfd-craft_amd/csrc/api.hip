@@ -2151,6 +2151,18 @@ int cfd_profile_solve(cfd_grid* g, const double* d_b, const double* d_scale, dou
   return CFD_OK;
 }
 
+int cfd_grid_perturb(cfd_grid* g, const double* d_src, double* d_dst, double rel, uint64_t seed) {
+  if (!g || !d_src || !d_dst) {
+    set_error("cfd_grid_perturb: null argument");
+    return CFD_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  launch_perturb(g->geo.N, g->codes.p, d_src, d_dst, rel, seed, g->part.p, (double)std::max<int64_t>(1, g->host.fluid_count),
+                 g->stream);
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return CFD_OK;
+}
+
 int cfd_device_alloc(size_t bytes, void** d_ptr) {
   HIP_TRY(hipMalloc(d_ptr, bytes));
   return CFD_OK;
@@ -2780,6 +2792,22 @@ int cfd_server_wait(cfd_server* s, uint64_t ticket) {
     if (server_progress(s) < 0) return CFD_EDEVICE;
     std::this_thread::yield();
   }
+  return CFD_OK;
+}
+
+int cfd_server_finished(cfd_server* s, uint64_t ticket, int* done) {
+  if (!s || !done) {
+    set_error("cfd_server_finished: null argument");
+    return CFD_EINVAL;
+  }
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    set_error("cfd_server_finished: unknown ticket");
+    return CFD_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->m);
+  *done = it->second.state.load(std::memory_order_acquire) == 2 &&
+          std::find(s->batched.begin(), s->batched.end(), ticket) == s->batched.end();
   return CFD_OK;
 }
 

@@ -4657,4 +4657,46 @@ void launch_batch_states(PcgState* const* st, int n, PcgState* dst, hipStream_t 
   BATCH_LAUNCH(kb_states, L, 64, s);
 }
 
+// ---------------------------------------------------------------- workload helper
+// A standard normal deviate per (seed, index), the same on every run: two
+// 53-bit uniforms from 64-bit mixes of the seed and the index (the SplitMix64
+// finalizer, Steele, Lea and Flood 2014), then the Box-Muller transform.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double normal_at(unsigned long long seed, long long i) {
+  const unsigned long long a = mix64(seed ^ mix64(2ull * (unsigned long long)i));
+  const unsigned long long b = mix64(seed ^ mix64(2ull * (unsigned long long)i + 1ull));
+  const double u1 = ((double)(a >> 11) + 0.5) * 0x1.0p-53, u2 = (double)(b >> 11) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+// Partials of the sum of src^2 over the fluid cells.
+__global__ void __launch_bounds__(RB) k_sumsq_fluid(long long N, const uint8_t* __restrict__ codes,
+                                                    const double* __restrict__ src, double* part) {
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB)
+    if (codes[i] & FLUID) acc += src[i] * src[i];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// dst = src + rel rms xi on the fluid cells (src elsewhere), rms over the fluid cells.
+__global__ void __launch_bounds__(RB) k_perturb(long long N, const uint8_t* __restrict__ codes,
+                                                const double* __restrict__ src, double* __restrict__ dst, double rel,
+                                                unsigned long long seed, const double* part, double fluid_count) {
+  const double scale = rel * sqrt(reduce_part(part, RP) / fluid_count);
+  for (long long i = (long long)blockIdx.x * RB + threadIdx.x; i < N; i += (long long)gridDim.x * RB) {
+    const double v = src[i];
+    dst[i] = (codes[i] & FLUID) ? v + scale * normal_at(seed, i) : v;
+  }
+}
+void launch_perturb(long long N, const uint8_t* codes, const double* src, double* dst, double rel,
+                    unsigned long long seed, double* part, double fluid_count, hipStream_t s) {
+  hipLaunchKernelGGL(k_sumsq_fluid, dim3(RP), dim3(RB), 0, s, N, codes, src, part);
+  hipLaunchKernelGGL(k_perturb, dim3(RP), dim3(RB), 0, s, N, codes, src, dst, rel, seed, (const double*)part,
+                     fluid_count);
+}
+
 }  // namespace cfdgpu

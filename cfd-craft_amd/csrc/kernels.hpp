@@ -182,6 +182,10 @@ void launch_batch_iteration(const BatchSolve<T>* items, int n, hipStream_t s);
 template <class T>
 void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
                        hipStream_t s);
+// Workload helper: dst = src + rel rms(src) xi on the fluid cells, xi a seeded
+// standard normal field, rms over the fluid cells (part: RP partials).
+void launch_perturb(long long N, const uint8_t* codes, const double* src, double* dst, double rel,
+                    unsigned long long seed, double* part, double fluid_count, hipStream_t s);
 // The PcgState of each solve copied to dst[i] (device memory or coherent pinned
 // host memory).
 void launch_batch_states(PcgState* const* st, int n, PcgState* dst, hipStream_t s);

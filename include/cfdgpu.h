@@ -389,10 +389,21 @@ int cfd_server_submit(cfd_server* s, cfd_grid* g, const double* d_b, const doubl
  * (a negative CFD error code when the device failed). */
 int cfd_server_progress(cfd_server* s);
 int cfd_server_wait(cfd_server* s, uint64_t ticket);
+/* done = 1 when the request has finished and its result can be taken (no wait). */
+int cfd_server_finished(cfd_server* s, uint64_t ticket, int* done);
 int cfd_server_drain(cfd_server* s);
 /* Result of a finished request (then forgotten): the solve's code, info, latency
  * (submit to finish) and queueing time (submit to start), in seconds. */
 int cfd_server_result(cfd_server* s, uint64_t ticket, cfd_linear_info* info, double* latency_s, double* queued_s);
+
+/* Workload helper (no counterpart in the reference's pressure path, whose
+ * campaign scripts form their right-hand sides on the host): d_dst = d_src +
+ * rel * rms(d_src) * xi on the fluid cells of g (d_src elsewhere), rms over the
+ * fluid cells and xi a standard normal field fixed by seed (SplitMix64 mixes of
+ * the seed and the cell index, Box-Muller), the chained right-hand sides of a
+ * wind-direction campaign formed in HBM. Synchronous; not while a solve on g
+ * runs (it shares the grid's reduction buffer). d_src == d_dst is allowed. */
+int cfd_grid_perturb(cfd_grid* g, const double* d_src, double* d_dst, double rel, uint64_t seed);
 
 /* Device memory helpers for callers without their own allocator. */
 int cfd_device_alloc(size_t bytes, void** d_ptr);

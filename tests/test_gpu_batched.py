@@ -153,3 +153,29 @@ def test_unbatchable_requests_fall_back_beside_a_batch(gpu_available):
         assert info["iterations"] == solo[k][1]
         assert np.array_equal(outs[k].numpy(), solo[k][0])
     server.close()
+
+
+def test_campaign_perturbation_is_seeded_and_confined_to_fluid(gpu_available):
+    """cfd_grid_perturb (the campaign's chained right-hand sides): solid cells
+    keep their values, the fluid cells move by rel * rms(src) times a standard
+    normal field that the seed fixes (same seed: the same bits; another seed:
+    another field), src == dst allowed."""
+    case, b, t, tol = W.served_request(37, 2)
+    g = cfdgpu.Grid(case)
+    fluid = W.fluid_mask(case)
+    src = cfdgpu.DeviceArray.from_host(b)
+    d1, d2, d3 = (cfdgpu.DeviceArray(g.count) for _ in range(3))
+    g.perturb(src, d1, 0.02, 7)
+    g.perturb(src, d2, 0.02, 7)
+    g.perturb(src, d3, 0.02, 8)
+    x1, x2, x3 = d1.numpy(), d2.numpy(), d3.numpy()
+    assert np.array_equal(x1, x2) and not np.array_equal(x1, x3)
+    assert np.array_equal(x1[~fluid], b[~fluid])
+    rms = np.sqrt(np.mean(b[fluid] ** 2))
+    xi = (x1[fluid] - b[fluid]) / (0.02 * rms)
+    n = xi.size
+    assert abs(xi.mean()) < 5 / np.sqrt(n) and abs(xi.std() - 1.0) < 5 / np.sqrt(n), (xi.mean(), xi.std())
+    # a fourth moment of 3 (normal, not uniform)
+    assert abs(np.mean(xi ** 4) - 3.0) < 0.1
+    g.perturb(src, src, 0.02, 7)
+    assert np.array_equal(src.numpy(), x1)
