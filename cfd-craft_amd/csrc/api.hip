@@ -1111,8 +1111,11 @@ int resident_mode(const cfd_grid* g, bool first) {
 
 // z = M r. resident (see resident()): 0 r and z natural; 1 r natural, copied
 // into the fine level's right-hand side, z left in its x; 2 r already there;
-// 3 (mixed) r natural, z written as FP32 to z32.
-void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0) {
+// 3 (mixed) r natural, z written as FP32 to z32; 4 (mixed) r in the FP64
+// level's b arrays, z left in the FP32 level's x. first_done (2, 4): the x/r
+// update did the cycle's first half-sweep.
+void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st, int resident = 0,
+                          bool first_done = false) {
   if (g->host.preconditioner == CFD_IC0) {
     // mixed precision: both sweeps in FP32 over the FP64 pivots (tiled only)
     const double* fill = g->ic0_fill.p;
@@ -1138,12 +1141,14 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st,
                                        g->a32.levels.data(), A.D, resident >= 3 ? nullptr : z, st, g->omega,
                                        g->stream, A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
                                        resident == 4 ? A.rb64.b : rz_r, g->part.p, g->zero_last,
-                                       resident >= 3 ? resident : 0, resident == 3 ? g->z32.p : nullptr);
+                                       resident >= 3 ? resident : 0, resident == 3 ? g->z32.p : nullptr,
+                                       first_done);
     else
       launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
                                         A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
                                         A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds,
-                                        resident ? nullptr : rz_r, g->part.p, g->zero_last, resident);
+                                        resident ? nullptr : rz_r, g->part.p, g->zero_last, resident, nullptr,
+                                        first_done);
     return;
   }
   const int nlev = (int)g->h64.levels.size();
@@ -1180,7 +1185,8 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
   const int* cidx = g->n_closed > 1 ? g->cidx.p : nullptr;
   const bool res = resident(g), res32 = resident32(g), resm = resident_mixed(g);
   if (precondition) {
-    enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, false));
+    // the previous iteration's x/r update did the cycle's first half-sweep
+    enqueue_precondition(g, g->r.p, g->z.p, st, resident_mode(g, false), res || resm);
     enqueue_gauge(g, g->z.p, nullptr, 0, g->mean_z.p, st);
     if (!rz_fused(g)) launch_pcg_rz(g->geo.N, g->codes.p, g->r.p, g->z.p, mz, cidx, st, g->part.p, g->stream);
   }
@@ -1189,12 +1195,13 @@ void enqueue_iteration(cfd_grid* g, int parity, bool precondition) {
     const FineRB<float>& F = g->ash.rb32;
     launch_pcg_op_mixed(g->geo, g->codes.p, F.x, F.half, (const float*)pold, (float*)pnew, g->ap.p, st, g->part.p,
                         g->stream, g->march);
-    launch_pcg_axpy_rb(g->geo, (const float*)pnew, g->ap.p, g->x.p, g->ash.rb64.b, st, g->part.p, g->stream);
+    launch_pcg_axpy_first<float, float>(g->geo, F, (const float*)pnew, g->ap.p, g->x.p, g->ash.rb64.b, st, g->part.p,
+                                        g->stream);
   } else if (res) {
     const FineRB<double>& F = g->ash.rb64;
     launch_pcg_op(g->geo, g->codes.p, F.x, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream, g->march,
                   F.half);
-    launch_pcg_axpy_rb(g->geo, pnew, g->ap.p, g->x.p, F.b, st, g->part.p, g->stream);
+    launch_pcg_axpy_first<double, double>(g->geo, F, pnew, g->ap.p, g->x.p, F.b, st, g->part.p, g->stream);
   } else if (res32) {
     launch_pcg_op(g->geo, g->codes.p, nullptr, pold, pnew, g->ap.p, mz, cidx, false, st, g->part.p, g->stream,
                   g->march, 0, g->z32.p);
@@ -2347,7 +2354,10 @@ int batch_begin(cfd_server::Lane& L, cfd_server::Req& R) {
   launch_pcg_init(g->geo.N, g->codes.p, R.b, g->mean_b.p, nullptr, g->r.p, g->bc.p, g->x.p, sp, g->part.p, st);
   if (g->opt.warm_start && g->warm_valid)
     launch_warm_residual(g->geo, g->codes.p, g->warm.p, g->bc.p, g->x.p, g->r.p, sp, st);
+  // (FP64: the scatter is the cycle's first half-sweep; mixed: from the FP64
+  // residual into the FP32 level)
   launch_rb_scatter<double>(g->geo, g->codes.p, g->r.p, g->ash.rb64, sp, st);
+  if (R.precision == CFD_PRECISION_MIXED) launch_rb_first_cvt(g->geo, g->ash.rb32, g->ash.rb64.b, sp, st);
   R.enq = 0;
   return 0;
 }

@@ -2280,6 +2280,71 @@ __global__ void __launch_bounds__(BS) k_rb_first_cvt(Geo g, FineRB<T> F, const d
   rb_first_cvt_body<T>(g, F, rres, st, blockIdx.x, gridDim.x);
 }
 
+// The x/r update of k_pcg_axpy_rb (the same cells per thread in the same order:
+// x, r and the partials of |r|^2 bit for bit) with the next V-cycle's first
+// half-sweep from a zero iterate done on the way: each red cell's x in the fine
+// level F set to r / a_ii (T double: F's b arrays are r itself; T float: both
+// colours of r rounded into F's b first), the values of k_rb_first_lin and
+// k_rb_first_cvt.
+template <class TP, class T>
+__device__ __forceinline__ void pcg_axpy_first_body(const Geo& g, const FineRB<T>& F, unsigned long long mnx,
+                                                    unsigned long long mny, const TP* __restrict__ p,
+                                                    const double* __restrict__ ap, double* __restrict__ x,
+                                                    double* __restrict__ r, PcgState* st, double* part, unsigned bid,
+                                                    unsigned nblk) {
+  if (!block_running(st)) return;
+  const double rz = reduce_part(part + P_RZ, RP);
+  const double denom = reduce_part(part + P_PAP, RP);
+  if (denom <= 0.0 || !isfinite(denom)) {
+    if (bid == 0 && threadIdx.x == 0) {
+      st->denom = denom;
+      st->status = ST_E032;
+    }
+    return;
+  }
+  const double alpha = rz / denom;
+  const long long N = g.N, half = N >> 1;
+  const unsigned nx = (unsigned)g.n[0], ny = (unsigned)g.n[1];
+  double acc = 0.0;
+  for (long long i = (long long)bid * RB + threadIdx.x; i < N; i += (long long)nblk * RB) {
+    const unsigned c = (unsigned)i;
+    const unsigned row = div_magic(c, mnx), k = div_magic(row, mny);
+    const unsigned ci = c - row * nx, cj = row - k * ny;
+    const unsigned colour = (ci + cj + k) & 1u;
+    const long long at = (colour ? half : 0) + (c >> 1);
+    double* rc = r + at;
+    x[i] += alpha * (double)p[i];
+    double ri = *rc - alpha * ap[i];
+    *rc = ri;
+    acc += ri * ri;
+    const T bv = (T)ri;
+    if constexpr (!std::is_same<T, double>::value) F.b[at] = bv;
+    if (colour == 0) {
+      const uint8_t code = F.codes[c >> 1];
+      const int q[3] = {(int)ci, (int)cj, (int)k};
+      const T d = fine_diag<T>(g, code, q);
+      F.x[c >> 1] = (code & FLUID) && d > T(0) ? bv / d : T(0);
+    }
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) {
+    part[P_RR + bid] = s;
+    if (bid == 0) {
+      st->rz = rz;
+      st->alpha = alpha;
+      st->denom = denom;
+      st->first = 0;
+    }
+  }
+}
+template <class TP, class T>
+__global__ void __launch_bounds__(RB) k_pcg_axpy_first(Geo g, FineRB<T> F, unsigned long long mnx,
+                                                       unsigned long long mny, const TP* __restrict__ p,
+                                                       const double* __restrict__ ap, double* __restrict__ x,
+                                                       double* __restrict__ r, PcgState* st, double* part) {
+  pcg_axpy_first_body<TP, T>(g, F, mnx, mny, p, ap, x, r, st, part, blockIdx.x, gridDim.x);
+}
+
 // The Gauss-Seidel value of a cell from its code, right-hand side and the six
 // neighbour values xn (axis-major, lower side first; entries of uncoupled or
 // boundary faces are not read).
@@ -3491,6 +3556,23 @@ void launch_pcg_axpy_rb(const Geo& g, const float* p, const double* ap, double* 
   PROBED_LAUNCH(PROBE_AXPY, 44.0 * (double)g.N, k_pcg_axpy_rb<float>, dim3(RP), dim3(RB), 0, s, g.N, g.n[0], g.n[1],
                 mnx, mny, p, ap, x, r, st, part);
 }
+template <class TP, class T>
+void launch_pcg_axpy_first(const Geo& g, const FineRB<T>& F, const TP* p, const double* ap, double* x, double* r,
+                           PcgState* st, double* part, hipStream_t s) {
+  const unsigned long long two40 = 1ULL << 40;
+  const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
+  const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
+  // x, r, Ap and p in; x, r out; the red cells' code in and x out; (FP32
+  // level) both colours' b out
+  const double bytes = (double)g.N * (40.0 + sizeof(TP)) + (double)F.half * (1.0 + sizeof(T)) +
+                       (std::is_same<T, double>::value ? 0.0 : (double)g.N * sizeof(T));
+  PROBED_LAUNCH(PROBE_AXPY, bytes, (k_pcg_axpy_first<TP, T>), dim3(RP), dim3(RB), 0, s, g, F, mnx, mny, p, ap, x, r,
+                st, part);
+}
+template void launch_pcg_axpy_first<double, double>(const Geo&, const FineRB<double>&, const double*, const double*,
+                                                     double*, double*, PcgState*, double*, hipStream_t);
+template void launch_pcg_axpy_first<float, float>(const Geo&, const FineRB<float>&, const float*, const double*,
+                                                  double*, double*, PcgState*, double*, hipStream_t);
 void launch_pcg_op_mixed(const Geo& g, const uint8_t* codes, const float* z, long long zhalf, const float* pold,
                          float* pnew, double* ap, PcgState* st, double* part, hipStream_t s, int march) {
   const MarchShape ms = march_shape(g);
@@ -3779,12 +3861,15 @@ template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const dou
 // values of the level's x) to out32 in the natural layout; 4 (an FP32 level,
 // two cells per thread) the FP64 right-hand side rz_r in the colour-separated
 // layout (b0 unused) rounded into the level's b, r . z formed from rz_r and z
-// left in the level's x.
+// left in the level's x. first_done (resident 2 and 4): the x/r update that
+// formed the right-hand side also did the first half-sweep (the level's b and
+// red x are set); the cycle starts with the black half-sweep.
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb, int fine_rounds, int coarse_rounds,
-                       const double* rz_r, double* rz_part, bool zero_last, int resident, float* out32) {
+                       const double* rz_r, double* rz_part, bool zero_last, int resident, float* out32,
+                       bool first_done) {
   // a round is one sweep over every colour class; pre-smoothing runs the classes in
   // ascending order, post-smoothing in descending order (the adjoint)
   const int fr = fine_rounds, cr = coarse_rounds;
@@ -3830,7 +3915,9 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
         resident == 2   ? (double)rb->half * (1 + 2 * sizeof(T))
         : resident == 4 ? (double)g0.N * (8 + sizeof(T)) + (double)rb->half * (1 + sizeof(T))
                         : (double)g0.N * (8 + 1 + sizeof(T)) + (double)rb->half * sizeof(T);
-    if (resident == 2)
+    if (first_done && (resident == 2 || resident == 4)) {
+      // the x/r update that formed the right-hand side relaxed the red cells
+    } else if (resident == 2)
       PROBED_LAUNCH(PROBE_FINE_FIRST, first_bytes, (k_rb_first_lin<T>), dim3((unsigned)((rb->half / 2 + BS - 1) / BS)),
                     dim3(BS), 0, s, g0, *rb, st);
     else if (resident == 4)
@@ -3966,11 +4053,11 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
 template void launch_agg_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, const int*, int, int,
                                                 const AggDevLevel<double>*, const AggCoarsest&, double*,
                                                 const PcgState*, double, hipStream_t, const FineRB<double>*, int, int,
-                                                const double*, double*, bool, int, float*);
+                                                const double*, double*, bool, int, float*, bool);
 template void launch_agg_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, const int*, int, int,
                                                const AggDevLevel<float>*, const AggCoarsest&, double*, const PcgState*,
                                                double, hipStream_t, const FineRB<float>*, int, int, const double*,
-                                               double*, bool, int, float*);
+                                               double*, bool, int, float*, bool);
 
 template <class T>
 void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
@@ -3980,6 +4067,11 @@ void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, con
 }
 template void launch_rb_scatter<double>(const Geo&, const uint8_t*, const double*, const FineRB<double>&,
                                         const PcgState*, hipStream_t);
+void launch_rb_first_cvt(const Geo& g0, const FineRB<float>& rb, const double* rres, const PcgState* st,
+                         hipStream_t s) {
+  hipLaunchKernelGGL((k_rb_first_cvt<float>), dim3((unsigned)((rb.half / 2 + BS - 1) / BS)), dim3(BS), 0, s, g0, rb,
+                     rres, st);
+}
 
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s) {
@@ -4213,21 +4305,6 @@ __device__ __forceinline__ const A& batch_item(const Batch<A>& B, unsigned& bid)
 }
 
 template <class T>
-struct BFirstLin {
-  Geo g;
-  FineRB<T> F;
-  const PcgState* st;
-  unsigned nblk;
-};
-template <class T>
-struct BFirstCvt {
-  Geo g;
-  FineRB<T> F;
-  const double* rres;
-  const PcgState* st;
-  unsigned nblk;
-};
-template <class T>
 struct BHalf {
   Geo g;
   FineRB<T> F;
@@ -4312,19 +4389,6 @@ struct BOp {
   long long zhalf;
   unsigned nblk;
 };
-template <class TP>
-struct BAxpy {
-  long long N;
-  int nx, ny;
-  unsigned long long mnx, mny;
-  const TP* p;
-  const double* ap;
-  double* x;
-  double* r;
-  PcgState* st;
-  double* part;
-  unsigned nblk;
-};
 struct BTrue {
   Geo g;
   const uint8_t* codes;
@@ -4340,20 +4404,6 @@ struct BState {
   unsigned nblk;
 };
 
-template <class T>
-__global__ void __launch_bounds__(BS) kb_rb_first_lin(Batch<BFirstLin<T>> B) {
-  unsigned bid;
-  const BFirstLin<T>& a = batch_item(B, bid);
-  if (bid >= a.nblk) return;
-  rb_first_lin_body<T>(a.g, a.F, a.st, bid, a.nblk);
-}
-template <class T>
-__global__ void __launch_bounds__(BS) kb_rb_first_cvt(Batch<BFirstCvt<T>> B) {
-  unsigned bid;
-  const BFirstCvt<T>& a = batch_item(B, bid);
-  if (bid >= a.nblk) return;
-  rb_first_cvt_body<T>(a.g, a.F, a.rres, a.st, bid, a.nblk);
-}
 template <class T>
 __global__ void __launch_bounds__(BS) kb_rb_gs2_lin(Batch<BHalf<T>> B) {
   unsigned bid;
@@ -4426,12 +4476,25 @@ __global__ void __launch_bounds__(RB, 8) kb_pcg_op_march(Batch<BOp<TZ, TP>> B) {
   pcg_op_march_body<false, true, TZ, TP>(a.g, a.ms, a.codes, a.z, a.pold, a.pnew, a.ap, (const double*)nullptr,
                                          (const int*)nullptr, a.st, a.part, a.zhalf, bid, a.nblk);
 }
-template <class TP>
-__global__ void __launch_bounds__(RB) kb_pcg_axpy_rb(Batch<BAxpy<TP>> B) {
+template <class T>
+struct BAxpyFirst {
+  Geo g;
+  FineRB<T> F;
+  unsigned long long mnx, mny;
+  const T* p;
+  const double* ap;
+  double* x;
+  double* r;
+  PcgState* st;
+  double* part;
+  unsigned nblk;
+};
+template <class T>
+__global__ void __launch_bounds__(RB) kb_pcg_axpy_first(Batch<BAxpyFirst<T>> B) {
   unsigned bid;
-  const BAxpy<TP>& a = batch_item(B, bid);
+  const BAxpyFirst<T>& a = batch_item(B, bid);
   if (bid >= a.nblk) return;
-  pcg_axpy_rb_body<TP>(a.N, a.nx, a.ny, a.mnx, a.mny, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
+  pcg_axpy_first_body<T, T>(a.g, a.F, a.mnx, a.mny, a.p, a.ap, a.x, a.r, a.st, a.part, bid, a.nblk);
 }
 __global__ void __launch_bounds__(RB, 8) kb_pcg_true(Batch<BTrue> B) {
   unsigned bid;
@@ -4483,16 +4546,8 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
   if (n <= 0 || n > kBatchMax) return;
   auto pair_blocks = [](const FineRB<T>& F) { return (unsigned)((F.half / 2 + BS - 1) / BS); };
   const int fr = it[0].fine_rounds;
-  // the first fine half-sweep from a zero iterate
-  if constexpr (MIXED) {
-    Batcher<BFirstCvt<T>> L;
-    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].rres, it[i].st, pair_blocks(it[i].F)});
-    BATCH_LAUNCH(kb_rb_first_cvt<T>, L, BS, s);
-  } else {
-    Batcher<BFirstLin<T>> L;
-    for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, it[i].st, pair_blocks(it[i].F)});
-    BATCH_LAUNCH(kb_rb_first_lin<T>, L, BS, s);
-  }
+  // the first fine half-sweep from a zero iterate was done by the x/r update
+  // that formed the right-hand side (or by the solve's set-up)
   for (int pass = 1; pass < 2 * fr; ++pass) {
     Batcher<BHalf<T>> L;
     for (int i = 0; i < n; ++i) L.add({it[i].g, it[i].F, pass & 1, it[i].st, pair_blocks(it[i].F)});
@@ -4624,18 +4679,19 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
     BATCH_LAUNCH((kb_pcg_op<T, T>), C, RB, s);
     BATCH_LAUNCH((kb_pcg_op_march<T, T>), M, RB, s);
   }
-  // alpha, x += alpha p, r -= alpha A p, |r|^2 (r in the FP64 colour-separated arrays)
+  // alpha, x += alpha p, r -= alpha A p, |r|^2 (r in the FP64 colour-separated
+  // arrays), and the next cycle's first half-sweep
   {
-    Batcher<BAxpy<T>> L;
+    Batcher<BAxpyFirst<T>> L;
     const unsigned long long two40 = 1ULL << 40;
     for (int i = 0; i < n; ++i) {
       const Geo& g = it[i].g;
       const unsigned long long mnx = (two40 + (unsigned long long)g.n[0] - 1) / (unsigned long long)g.n[0];
       const unsigned long long mny = (two40 + (unsigned long long)g.n[1] - 1) / (unsigned long long)g.n[1];
-      L.add({g.N, g.n[0], g.n[1], mnx, mny, (const T*)it[i].pnew, it[i].ap, it[i].x, it[i].rres, it[i].st, it[i].part,
+      L.add({g, it[i].F, mnx, mny, (const T*)it[i].pnew, it[i].ap, it[i].x, it[i].rres, it[i].st, it[i].part,
              (unsigned)RP});
     }
-    BATCH_LAUNCH(kb_pcg_axpy_rb<T>, L, RB, s);
+    BATCH_LAUNCH(kb_pcg_axpy_first<T>, L, RB, s);
   }
   // the recomputed residual and the iteration's decisions
   {

@@ -93,6 +93,12 @@ void launch_pcg_axpy_rb(const Geo& g, const double* p, const double* ap, double*
 // ... with p in FP32 (the mixed-precision solve)
 void launch_pcg_axpy_rb(const Geo& g, const float* p, const double* ap, double* x, double* r, PcgState* st,
                         double* part, hipStream_t s);
+// The same x/r update (r in the FP64 colour-separated arrays, p stored as TP)
+// with the next V-cycle's first half-sweep from a zero iterate on the fine
+// level F done on the way (T float: r rounded into F's b arrays first).
+template <class TP, class T>
+void launch_pcg_axpy_first(const Geo& g, const FineRB<T>& F, const TP* p, const double* ap, double* x, double* r,
+                           PcgState* st, double* part, hipStream_t s);
 // The operator kernel of the mixed-precision solve: z the FP32 values of the
 // fine level's x (colour-separated, zhalf entries per colour), p stored in FP32
 // (each value rounded once; A p and p . Ap formed from the rounded values).
@@ -141,13 +147,14 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,
                        double omega, hipStream_t s, const FineRB<T>* rb = nullptr, int fine_rounds = 2,
                        int coarse_rounds = 2, const double* rz_r = nullptr, double* rz_part = nullptr,
-                       bool zero_last = true, int resident = 0, float* out32 = nullptr);
+                       bool zero_last = true, int resident = 0, float* out32 = nullptr, bool first_done = false);
 // One colour-separated fine half-sweep (timing hook).
 template <class T>
 void launch_rb_gs(const Geo& g0, const FineRB<T>& rb, int colour, hipStream_t s);
 
 // Batched iterations: one PCG iteration (red-black V(2,2) preconditioner led,
-// then p, A p, the x/r update and the convergence decision) of up to
+// then p, A p, the x/r update with the next cycle's first half-sweep, and the
+// convergence decision) of up to
 // kBatchMax independent solves, each on its own grid, issued as one launch per
 // stage of the iteration; every launch gives each solve the blocks its own
 // single-grid launch would have (from a multiple of 8, so XCD placement is
@@ -182,6 +189,10 @@ void launch_batch_iteration(const BatchSolve<T>* items, int n, hipStream_t s);
 template <class T>
 void launch_rb_scatter(const Geo& g0, const uint8_t* codes, const double* r, const FineRB<T>& rb, const PcgState* st,
                        hipStream_t s);
+// The mixed-precision first half-sweep: the FP64 residual rres (colour-separated)
+// rounded into the FP32 level's b arrays and its red x set to b / a_ii.
+void launch_rb_first_cvt(const Geo& g0, const FineRB<float>& rb, const double* rres, const PcgState* st,
+                         hipStream_t s);
 // Workload helper: dst = src + rel rms(src) xi on the fluid cells, xi a seeded
 // standard normal field, rms over the fluid cells (part: RP partials).
 void launch_perturb(long long N, const uint8_t* codes, const double* src, double* dst, double rel,
