@@ -626,6 +626,36 @@ def served(requests=512, n_grids=16, lanes=3, seed=4):
     return out
 
 
+def served_probe(lanes, requests=512, n_grids=16, seed=4):
+    """The served mix's batched FP64 closed loop alone (the request list of
+    `served`), for kernel traces of the batched launches: with one lane no two
+    batched kernels overlap on the device, so each traced duration is the
+    kernel's own."""
+    import cfdgpu
+    from cfdgpu import workloads as W
+
+    pool = [W.served_request(seed, k) for k in range(n_grids)]
+    rng = np.random.default_rng(seed)
+    reqs = [(int(rng.integers(0, n_grids)), W.SERVED_TOLERANCES[int(rng.integers(0, 3))]) for _ in range(requests)]
+    server = cfdgpu.Server(max_active=n_grids, lanes=lanes)
+    grids = [server.admit(c) for c, _, _, _ in pool]
+    bufs = [(cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray.from_host(t), cfdgpu.DeviceArray(g.count))
+            for (_, b, t, _), g in zip(pool, grids)]
+    prec = cfdgpu.PRECISION_FP64
+    for tk in [server.submit(grids[k], bufs[k][0], bufs[k][2], 1e-8, 3000, prec, d_scale=bufs[k][1])
+               for k in range(n_grids)]:
+        server.wait(tk)
+        server.result(tk)
+    t0 = time.perf_counter()
+    tks = [server.submit(grids[k], bufs[k][0], bufs[k][2], tol, 3000, prec, d_scale=bufs[k][1]) for k, tol in reqs]
+    server.drain()
+    el = time.perf_counter() - t0
+    its = [server.result(tk)[0]["iterations"] for tk in tks]
+    server.close()
+    print(json.dumps({"served_probe_lanes": lanes, "requests": requests, "solves_per_s": round(requests / el, 1),
+                      "iterations": int(sum(its))}), flush=True)
+
+
 def published_fixtures(reps=3):
     """The fixtures whose reference CPU times are published, on the GPU with the
     reference's protocol: the manufactured field sin(2 pi x/Lx) + 0.3 cos(4 pi z/Lz)
@@ -707,6 +737,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="no CPU baseline and no oracle field of the prism city")
     ap.add_argument("--no-extra", action="store_true", help="only the headline and its kernel table")
     ap.add_argument("--pmc-probe", action="store_true")
+    ap.add_argument("--served-probe", type=int, default=None, metavar="LANES",
+                    help="only the served mix's batched FP64 closed loop, on LANES lanes (kernel traces)")
     ap.add_argument("--campaign-directions", type=int, default=4,
                     help="wind directions of the served campaign (36: every 10 degrees)")
     ap.add_argument("--campaign-steps", type=int, default=20, help="chained solves per campaign direction")
@@ -718,6 +750,9 @@ def main():
         sys.exit(2)
     if args.pmc_probe:
         pmc_probe()
+        return
+    if args.served_probe is not None:
+        served_probe(args.served_probe)
         return
 
     base = None
