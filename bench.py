@@ -12,9 +12,10 @@ distinct and resident in HBM before timing; value = solves/s at relative
 residual 1e-8. Mixed precision (FP32 V-cycle inside FP64 PCG) and warm starts
 from the resident previous solution run on the same chain beside it.
 
-Beside the headline: the campaign served concurrently (several directions'
-grids resident at once, each a warm-started mixed-precision chain, every last
-step against the oracle), the prism city, the urban block, the lid cavity, the
+Beside the headline: the campaign served concurrently (directions' grids
+admitted under an HBM budget that holds some of them at once, the others
+queued and built while the admitted ones solve; each direction a warm-started
+mixed-precision chain, the last steps against the oracle), the prism city, the urban block, the lid cavity, the
 served mix of uneven grids and tolerances, and the published fixtures.
 
 roofline: the kernel class with the largest share of a solve's device time,
@@ -26,8 +27,8 @@ serial FP64 multigrid PCG restated in C++), one core of this host, on the
 urban block: one cold solve and the median of three reused ones.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
-                      [--campaign-directions D] [--campaign-steps S]
-(the full campaign: --campaign-directions 36 --campaign-steps 200)
+                      [--campaign-directions D] [--campaign-steps S] [--campaign-resident R]
+(the full campaign: --campaign-directions 36 --campaign-steps 200 --campaign-resident 0)
 """
 import argparse
 import json
@@ -65,8 +66,11 @@ CUBE48_IC0_MS_PER_ITERATION = 390.576 / 113
 CANYON93K_IC0_CPU_MS = 393.146
 
 
+_T0 = time.time()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.time() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def pct(v, q):
@@ -302,7 +306,7 @@ def _campaign_oracle_child(direction, path, core, timeout=1200.0):
     import oracle
     from cfdgpu import workloads as W
 
-    case, _, _ = W.street_canyon(direction)
+    case = W.street_canyon_case(direction)
     grid = oracle.Grid(case)
     del case
     end = time.time() + timeout
@@ -319,47 +323,106 @@ def _campaign_oracle_child(direction, path, core, timeout=1200.0):
                    "true_rel_residual": info["relative_residual"]}, f)
 
 
-def campaign(directions, steps, tmp, lanes=3, seed=11, oracle_paths=()):
+def _campaign_rhs_child(jobs, core):
+    """Background process (forked before the GPU is first used): the right-hand
+    sides of campaign directions, -div(u) of each direction's seeded log-law
+    field, written to their paths (with `path`.ready) for the serving loop."""
+    try:
+        os.sched_setaffinity(0, {core})
+    except OSError:
+        pass
+    from cfdgpu import workloads as W
+
+    for d, path in jobs:
+        _, b, _ = W.street_canyon(d)
+        np.save(path, b)
+        with open(path + ".ready", "w") as f:
+            f.write("1")
+        del b
+
+
+def _wait_file(path, timeout=600.0):
+    end = time.time() + timeout
+    while not os.path.exists(path + ".ready"):
+        if time.time() > end:
+            raise RuntimeError(f"{path} was not written")
+        time.sleep(0.01)
+    return np.load(path)
+
+
+def campaign(directions, steps, rhs_paths, resident, footprint, lanes=3, seed=11, oracle_paths=()):
     """The wind-direction campaign served concurrently: one street-canyon grid
     (384x384x160, h = 1 m) per inflow direction, admitted by the serving
-    library under its HBM budget and kept resident; each direction a chain of
-    `steps` projection solves whose right-hand side is the previous one plus a
-    seeded 2 % perturbation (formed in HBM between solves, cfd_grid_perturb),
-    each solve
-    warm-started from the direction's resident previous pressure, FP32
-    multigrid inside FP64 PCG at rel-res 1e-8. All directions run at once
-    through the server's batched lanes; a direction submits its next solve
-    when its previous one is done. The last right-hand side of every direction
-    is solved again in FP64 at 1e-12 on the GPU, and written to oracle_paths[i]
-    for the oracle's processes."""
+    library's queued admission under an HBM budget that holds `resident` such
+    grids (footprint: one canyon grid's measured bytes), so that a direction's
+    grid is built on the server's builder thread while the admitted directions
+    solve, and admitted when a finished direction's grid is released. Each
+    direction is a chain of `steps` projection solves whose right-hand side is
+    the previous one plus a seeded 2 % perturbation (formed in HBM between
+    solves, cfd_grid_perturb), each solve warm-started from the direction's
+    resident previous pressure, FP32 multigrid inside FP64 PCG at rel-res
+    1e-8, all admitted directions at once through the server's batched lanes.
+    The first right-hand side of each direction comes from rhs_paths (written
+    by background processes). The last right-hand side of every direction is
+    solved again in FP64 at 1e-12 on the GPU before its grid is released, and
+    written to oracle_paths[i] for the oracle's processes."""
     import cfdgpu
     from cfdgpu import workloads as W
 
-    server = cfdgpu.Server(max_active=len(directions), lanes=lanes)
-    runs = []
-    for d in directions:
-        case, b, t = W.street_canyon(d)
-        t0 = time.perf_counter()
-        g = server.admit(case)
-        cfdgpu.synchronize()
-        setup = time.perf_counter() - t0
-        g.set_options(cfdgpu.PRECISION_MIXED, warm_start=True)
-        runs.append({"d": d, "g": g, "b": [cfdgpu.DeviceArray.from_host(b), cfdgpu.DeviceArray(g.count)],
-                     "x": cfdgpu.DeviceArray(g.count), "setup": setup, "its": [], "lat": [], "k": 0})
-        del case, b, t
-    used, budget = server.usage()
+    n = len(directions)
+    resident = n if resident <= 0 else min(resident, n)
+    budget = int((resident + 0.5) * footprint)
+    server = cfdgpu.Server(max_active=n, hbm_budget_bytes=budget, lanes=lanes)
+    cases = [W.street_canyon_case(d) for d in directions]
+    t0 = time.perf_counter()
+    tickets = [server.admit_async(c) for c in cases]
+    del cases
+    runs = [{"i": i, "d": d, "its": [], "lat": [], "k": 0} for i, d in enumerate(directions)]
+    waiting = list(runs)
+    live, refs, last, jobs = [], [], [], []
+    max_used = 0
+    first_submit = None
 
     def submit(r):
         r["t"] = time.perf_counter()
         r["tk"] = server.submit(r["g"], r["b"][r["k"] % 2], r["x"], HEADLINE_TOL, 3000, cfdgpu.PRECISION_MIXED)
 
-    cfdgpu.synchronize()
-    t0 = time.perf_counter()
-    for r in runs:
-        submit(r)
-    live = list(runs)
-    while live:
+    def finish(r):
+        # the last right-hand side solved again in FP64 at 1e-12 (a request of its own)
+        r["ref"] = cfdgpu.DeviceArray(r["g"].count)
+        r["tk"] = server.submit(r["g"], r["b"][(steps - 1) % 2], r["ref"], 1e-12, 3000, cfdgpu.PRECISION_FP64)
+        refs.append(r)
+
+    def release(r):
+        # the fields go to the host (compared after the loop), then the grid makes room
+        r["fields"] = (r["b"][(steps - 1) % 2].numpy(), r["x"].numpy(), r["ref"].numpy())
+        for v in r["b"] + [r["x"], r["ref"]]:
+            v.free()
+        server.release(r["g"])
+        r["released_s"] = time.perf_counter() - t0
+
+    while waiting or live or refs:
+        for r in list(waiting):
+            g = server.admitted(tickets[r["i"]])
+            if g is None:
+                continue
+            waiting.remove(r)
+            r["admitted_s"] = time.perf_counter() - t0
+            max_used = max(max_used, server.usage()[0])
+            g.set_options(cfdgpu.PRECISION_MIXED, warm_start=True)
+            r["g"] = g
+            r["b"] = [cfdgpu.DeviceArray.from_host(_wait_file(rhs_paths[r["i"]])), cfdgpu.DeviceArray(g.count)]
+            r["x"] = cfdgpu.DeviceArray(g.count)
+            if first_submit is None:
+                first_submit = time.perf_counter()
+            submit(r)
+            live.append(r)
         server.progress()
+        for r in list(refs):
+            if server.finished(r["tk"]):
+                server.result(r["tk"])
+                refs.remove(r)
+                release(r)
         for r in list(live):
             if not server.finished(r["tk"]):
                 continue
@@ -369,45 +432,37 @@ def campaign(directions, steps, tmp, lanes=3, seed=11, oracle_paths=()):
             r["k"] += 1
             if r["k"] == steps:
                 live.remove(r)
+                finish(r)
                 continue
             # b_{k+1} = b_k + 0.02 rms(b_k) xi on the fluid cells
             r["g"].perturb(r["b"][(r["k"] - 1) % 2], r["b"][r["k"] % 2], 0.02,
                            (seed * 1000 + int(r["d"])) * 100000 + r["k"])
             submit(r)
     elapsed = time.perf_counter() - t0
-    lat_ms = [1e3 * v for r in runs for v in r["lat"]]
-    out = {"directions": [float(d) for d in directions], "steps": steps, "solves": len(runs) * steps,
-           "solves_per_s": round(len(runs) * steps / elapsed, 2), "elapsed_s": round(elapsed, 3), "lanes": lanes,
-           "hbm_admitted_gb": round(used / 1e9, 2), "hbm_budget_gb": round(budget / 1e9, 1),
-           "setup_s": [round(r["setup"], 3) for r in runs],
-           "iterations_mean": round(float(np.mean([i for r in runs for i in r["its"]])), 2),
-           "latency_ms": {"p50": round(pct(lat_ms, 0.5), 3), "p99": round(pct(lat_ms, 0.99), 3)}}
-    last = []
-    jobs = []
-    for i, r in enumerate(runs):
-        g = r["g"]
-        bl = r["b"][(steps - 1) % 2]
-        x = r["x"].numpy()
-        g.set_options(cfdgpu.PRECISION_FP64)
-        ref = cfdgpu.DeviceArray(g.count)
-        g.solve_device(bl, ref, 1e-12, 3000)
-        xr = ref.numpy()
+    for r in runs:
+        bl, x, xr = r.pop("fields")
         row = {"direction": r["d"], "iterations": r["its"][-1],
                "field_rel_error_vs_gpu_fp64_1e-12": float(np.linalg.norm(x - xr) / np.linalg.norm(xr))}
-        if i < len(oracle_paths):
-            path = oracle_paths[i]
-            np.save(path, bl.numpy())
+        if r["i"] < len(oracle_paths):
+            path = oracle_paths[r["i"]]
+            np.save(path, bl)
             np.save(path + ".gpu.npy", x)
             with open(path + ".ready", "w") as f:
                 f.write("1")
             jobs.append((row, path))
         last.append(row)
-        del ref
-    out["last_step"] = last
-    for r in runs:
-        for v in r["b"] + [r["x"]]:
-            v.free()
-        server.release(r["g"])
+        del bl, x, xr
+    lat_ms = [1e3 * v for r in runs for v in r["lat"]]
+    out = {"directions": [float(d) for d in directions], "steps": steps, "solves": n * steps, "lanes": lanes,
+           "resident_grids": resident, "hbm_budget_gb": round(budget / 1e9, 2),
+           "hbm_max_admitted_gb": round(max_used / 1e9, 2),
+           "solves_per_s": round(n * steps / elapsed, 2), "elapsed_s": round(elapsed, 3),
+           "first_solve_s": round(first_submit - t0, 3),
+           "admitted_s": [round(r["admitted_s"], 3) for r in runs],
+           "released_s": [round(r["released_s"], 3) for r in runs],
+           "iterations_mean": round(float(np.mean([i for r in runs for i in r["its"]])), 2),
+           "latency_ms": {"p50": round(pct(lat_ms, 0.5), 3), "p99": round(pct(lat_ms, 0.99), 3)},
+           "last_step": last}
     server.close()
     return out, jobs
 
@@ -431,7 +486,8 @@ def campaign_oracle_errors(jobs, procs, timeout):
 
 def prisms(c3_path):
     """The 256x256x128 prism city (8.4M cells, 200 convex 4-8-vertex prisms, about
-    20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10, each field's
+    20 % solid): plain FP64 and mixed precision at 1e-8 and 1e-10 (the median of
+    three rounds of 10 solves after two untimed ones), each field's
     error against the oracle's FP64 solve at 1e-10 (computed on one core beside
     the GPU legs) and against the GPU's own FP64 solve at 1e-12."""
     import cfdgpu
@@ -452,17 +508,21 @@ def prisms(c3_path):
     for tol in (1e-8, 1e-10):
         for name, prec in (("plain", cfdgpu.PRECISION_FP64), ("mixed", cfdgpu.PRECISION_MIXED)):
             g.set_options(prec)
-            g.solve_device(db, dx, tol, 3000, d_scale=ds)
+            for _ in range(2):
+                g.solve_device(db, dx, tol, 3000, d_scale=ds)
             cfdgpu.synchronize()
-            t0 = time.perf_counter()
+            rates = []
             for _ in range(3):
-                info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
-            cfdgpu.synchronize()
-            el = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    info = g.solve_device(db, dx, tol, 3000, d_scale=ds)
+                cfdgpu.synchronize()
+                rates.append(10 / (time.perf_counter() - t0))
             x = dx.numpy()
             key = f"{name}_tol_{tol:g}"
             fields[key] = x
-            out[key] = {"solves_per_s": round(3 / el, 2), "iterations": info["iterations"],
+            out[key] = {"solves_per_s": round(statistics.median(rates), 2),
+                        "rounds_of_10": [round(v, 1) for v in rates], "iterations": info["iterations"],
                         "true_rel_residual": info["relative_residual"],
                         "field_rel_error_vs_gpu_fp64_1e-12": float(np.linalg.norm(x - ref) / np.linalg.norm(ref))}
     g.set_options(cfdgpu.PRECISION_FP64)
@@ -739,9 +799,11 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--served-probe", type=int, default=None, metavar="LANES",
                     help="only the served mix's batched FP64 closed loop, on LANES lanes (kernel traces)")
-    ap.add_argument("--campaign-directions", type=int, default=4,
+    ap.add_argument("--campaign-directions", type=int, default=8,
                     help="wind directions of the served campaign (36: every 10 degrees)")
-    ap.add_argument("--campaign-steps", type=int, default=20, help="chained solves per campaign direction")
+    ap.add_argument("--campaign-steps", type=int, default=100, help="chained solves per campaign direction")
+    ap.add_argument("--campaign-resident", type=int, default=4,
+                    help="campaign grids the HBM budget holds at once (0: all); the others queue for admission")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the pressure solve of one grid is measured on one MI355X; "
@@ -762,17 +824,27 @@ def main():
     directions = [(20.0 + k * 360.0 / args.campaign_directions) % 360.0 for k in range(args.campaign_directions)]
     tmp = tempfile.mkdtemp(prefix="cfdbench")
     c3_path = os.path.join(tmp, "c3_oracle.npy")
+    cores = sorted(os.sched_getaffinity(0))
+    ctx = multiprocessing.get_context("fork")
+    rhs_procs, rhs_paths = [], []
+    if not args.no_extra:
+        # the campaign's first right-hand sides, formed on the host beside the GPU legs
+        rhs_paths = [os.path.join(tmp, f"rhs_{i}.npy") for i in range(len(directions))]
+        n_rhs = max(1, min(4, len(directions), len(cores) // 4))
+        for w in range(n_rhs):
+            jobs = [(d, rhs_paths[i]) for i, d in enumerate(directions) if i % n_rhs == w]
+            job = ctx.Process(target=_campaign_rhs_child, args=(jobs, cores[w % len(cores)]), daemon=True)
+            job.start()
+            rhs_procs.append(job)
     if not args.no_cpu:
         log("cpu baseline (oracle, one core)")
         base = cpu_baseline()
         if not args.no_extra:
             # the oracle's prism-city field, on one core beside the GPU legs (no GPU in the child)
-            cores = sorted(os.sched_getaffinity(0))
-            ctx = multiprocessing.get_context("fork")
             c3_job = ctx.Process(target=_c3_oracle_child, args=(c3_path, cores[-1]), daemon=True)
             c3_job.start()
             # the oracle's fields of the campaign's last steps, one core each
-            for i, d in enumerate(directions[:max(0, min(len(directions), len(cores) - 8))]):
+            for i, d in enumerate(directions[:max(0, min(4, len(directions), len(cores) - 8))]):
                 path = os.path.join(tmp, f"campaign_{i}.npy")
                 job = ctx.Process(target=_campaign_oracle_child, args=(d, path, cores[-2 - i]), daemon=True)
                 job.start()
@@ -803,10 +875,19 @@ def main():
                 "solve_frac": round(total_bytes / solve_s / 1e9 / HBM_PEAK_GBS, 4)}
     extra = {"street_canyon_modes": h["modes"], "kernel_classes_in_solve": probe,
              "headline_setup_s": round(h["setup_s"], 3)}
+    footprint = cfdgpu.lib().cfd_grid_footprint(g.h)
     del g, dbs, dx
     if not args.no_extra:
-        log(f"campaign: {len(directions)} directions x {args.campaign_steps} chained solves, served")
-        extra["campaign"], campaign_jobs = campaign(directions, args.campaign_steps, tmp, oracle_paths=campaign_paths)
+        log(f"campaign: {len(directions)} directions x {args.campaign_steps} chained solves, served, "
+            f"{args.campaign_resident or len(directions)} grids resident at once")
+        extra["campaign"], campaign_jobs = campaign(directions, args.campaign_steps, rhs_paths, args.campaign_resident,
+                                                    footprint, oracle_paths=campaign_paths)
+        for job in rhs_procs:
+            job.join(timeout=60)
+        for path in rhs_paths:
+            for f in (path, path + ".ready"):
+                if os.path.exists(f):
+                    os.remove(f)
         log("prism city")
         extra["prisms_256x256x128"], fields = prisms(c3_path)
         log("urban block")

@@ -280,6 +280,8 @@ def lib():
             "cfd_server_destroy": (None, [P]),
             "cfd_server_set_batching": (I, [P, I]),
             "cfd_server_admit": (I, [P, P, ctypes.POINTER(ctypes.c_void_p)]),
+            "cfd_server_admit_async": (I, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
+            "cfd_server_admitted": (I, [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_void_p)]),
             "cfd_server_release": (I, [P, P]),
             "cfd_server_usage": (I, [P, P, P]),
             "cfd_grid_footprint": (ctypes.c_size_t, [P]),
@@ -698,6 +700,7 @@ class Server:
         _check(lib().cfd_server_create(int(max_active), int(hbm_budget_bytes), ctypes.byref(h)))
         self.h = h
         self.grids = []
+        self.pending = {}
         _check(lib().cfd_server_set_batching(self.h, int(lanes)))
 
     def admit(self, case: Case) -> "Grid":
@@ -706,6 +709,32 @@ class Server:
         gh = ctypes.c_void_p()
         began = time.perf_counter()
         _check(lib().cfd_server_admit(self.h, ctypes.byref(c), ctypes.byref(gh)))
+        g = Grid.__new__(Grid)
+        g._attach(gh, case, began)
+        g.owner = self
+        self.grids.append(g)
+        return g
+
+    def admit_async(self, case: Case) -> int:
+        """Queued admission: the grid is built on the server's builder thread once
+        the HBM budget has room for it (admissions in order); a ticket for admitted()."""
+        c, keep = case.to_c()
+        t = ctypes.c_uint64(0)
+        _check(lib().cfd_server_admit_async(self.h, ctypes.byref(c), ctypes.byref(t)))
+        self.pending[t.value] = (case, time.perf_counter())
+        return t.value
+
+    def admitted(self, ticket: int) -> "Grid":
+        """The grid of a queued admission once built, else None; raises CfdError
+        if its build failed (ENOMEM: larger than the whole budget)."""
+        done = ctypes.c_int(0)
+        gh = ctypes.c_void_p()
+        rc = lib().cfd_server_admitted(self.h, ctypes.c_uint64(ticket), ctypes.byref(done), ctypes.byref(gh))
+        if done.value:
+            case, began = self.pending.pop(ticket)
+        _check(rc)
+        if not done.value:
+            return None
         g = Grid.__new__(Grid)
         g._attach(gh, case, began)
         g.owner = self

@@ -349,7 +349,19 @@ def street_canyon(direction_deg: float, seed: int = 5, n=(384, 384, 160), h: flo
     height of heights cells (about 25 % solid cells at 384x384x160); x and y open
     with the log-law wind blowing from direction_deg, ground_open_top z. The same
     seed gives the same city for every direction; the direction decides which
-    open faces are outlets, so each direction is its own operator."""
+    open faces are outlets, so each direction is its own operator. Returns the
+    case, the right-hand side -div(u) of the seeded log-law field and its scale."""
+    return _finish(*_street_canyon_city(direction_deg, seed, n, h, lots, street, heights, margin, preconditioner))
+
+
+def street_canyon_case(direction_deg: float, seed: int = 5, n=(384, 384, 160), h: float = 1.0, lots: int = 20,
+                       street: float = 3.0, heights=(30, 155), margin=4, preconditioner=MULTIGRID) -> Case:
+    """The case of street_canyon alone (the same city and wind), without forming
+    its right-hand side."""
+    return _street_canyon_city(direction_deg, seed, n, h, lots, street, heights, margin, preconditioner)[0]
+
+
+def _street_canyon_city(direction_deg, seed, n, h, lots, street, heights, margin, preconditioner):
     n = [int(v) for v in n]
     case = Case(cells=n, size_m=[n[0] * h, n[1] * h, n[2] * h], boundaries=[OPEN, OPEN, GROUND_OPEN_TOP],
                 preconditioner=preconditioner, tolerance=1e-8, max_iterations=2000)
@@ -382,7 +394,7 @@ def street_canyon(direction_deg: float, seed: int = 5, n=(384, 384, 160), h: flo
             hz = min(int(rng.integers(heights[0], heights[1] + 1)), n[2] - 1)
             case.blocks.append(Block(min_m=[min(xs), min(ys), 0.0], max_m=[max(xs), max(ys), hz * h], footprint=poly,
                                      id=f"c{a}_{b}"))
-    return _finish(case, rng)
+    return case, rng
 
 
 def _finish(case: Case, rng):

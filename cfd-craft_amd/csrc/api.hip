@@ -2260,8 +2260,21 @@ struct cfd_server {
     int failed = 0;             // a device error stopped the lane (its code)
     std::thread worker;
   };
+  // A queued admission (cfd_server_admit_async): the case copied, its grid built
+  // by the server's builder thread once the budget has room for it.
+  struct Admission {
+    std::vector<cfd_block> blocks;
+    std::vector<std::vector<double>> footprints;
+    cfd_case c{};
+    size_t need = 0;  // measured footprint of an earlier build that did not fit (0: estimate)
+    int state = 0;    // 0 waiting, 1 building, 2 admitted, 3 failed
+    cfd_grid* g = nullptr;
+    int rc = 0;
+    std::string error;
+  };
   int max_active = 8;
-  size_t budget = 0, used = 0;
+  int device = 0;
+  size_t budget = 0, used = 0;  // used: footprints of the admitted grids (under am)
   uint64_t next = 1;
   std::unordered_map<uint64_t, Req> reqs;  // node-based: a Req keeps its address while others come and go
   std::deque<uint64_t> queued;
@@ -2275,12 +2288,32 @@ struct cfd_server {
   std::mutex m;
   std::condition_variable cv;
   bool stop = false;
+  // admissions, grids, used and reserved are guarded by am
+  std::mutex am;
+  std::condition_variable acv;
+  std::map<uint64_t, Admission> admissions;  // by ticket: builds start in ticket order
+  uint64_t next_admission = 1;
+  size_t reserved = 0;         // bytes held for the admissions being built
+  double bytes_per_cell = 0;   // the largest measured footprint per cell so far
+  int n_builders = 2;          // builder threads (CFD_SERVE_BUILDERS)
+  std::vector<std::thread> builders;
+  bool builder_stop = false;
   ~cfd_server() {
     {
       std::lock_guard<std::mutex> lk(m);
       stop = true;
     }
     cv.notify_all();
+    {
+      std::lock_guard<std::mutex> lk(am);
+      builder_stop = true;
+    }
+    acv.notify_all();
+    for (std::thread& t : builders)
+      if (t.joinable()) t.join();
+    // grids admitted from the queue and never collected
+    for (auto& kv : admissions)
+      if (kv.second.state == 2 && kv.second.g) cfd_grid_destroy(kv.second.g);
     for (auto& L : lanes)
       if (L->worker.joinable()) L->worker.join();
     for (auto& L : lanes) {
@@ -2665,6 +2698,93 @@ int server_progress(cfd_server* s) {
 }
 }  // namespace
 
+namespace {
+// An admitted grid counted against the budget (the caller holds am).
+void admit_locked(cfd_server* s, cfd_grid* g) {
+  s->used += g->footprint;
+  s->grids[g] = g->footprint;
+  const double cells = (double)g->geo.N;
+  if (cells > 0) s->bytes_per_cell = std::max(s->bytes_per_cell, (double)g->footprint / cells);
+}
+
+// Bytes a waiting admission is expected to take: the measured footprint of an
+// earlier build that did not fit, else its cells times the largest footprint
+// per cell measured so far (2 % over), or 256 B a cell before any grid.
+size_t admission_need(const cfd_server& s, const cfd_server::Admission& A) {
+  if (A.need) return A.need;
+  const double cells = (double)A.c.cells[0] * (double)A.c.cells[1] * (double)A.c.cells[2];
+  const double per = s.bytes_per_cell > 0 ? 1.02 * s.bytes_per_cell : 256.0;
+  return (size_t)(cells * per);
+}
+
+// A builder thread: queued admissions taken in ticket order, each built once the
+// budget has room for it beside the admitted grids and the builds under way (an
+// empty server builds whatever comes next, so a grid larger than the whole
+// budget ends with CFD_ENOMEM instead of waiting forever). The grid set-up runs
+// outside the lock, beside the solves of the lanes and the other builder's set-up
+// (two builders: the host stages of one set-up overlap the other's device stages).
+void builder_loop(cfd_server* s) {
+  (void)hipSetDevice(s->device);
+  std::unique_lock<std::mutex> lk(s->am);
+  for (;;) {
+    cfd_server::Admission* A = nullptr;
+    size_t need = 0;
+    s->acv.wait(lk, [&] {
+      if (s->builder_stop) return true;
+      A = nullptr;
+      for (auto& kv : s->admissions)
+        if (kv.second.state == 0) {
+          A = &kv.second;
+          break;
+        }
+      if (!A) return false;
+      need = admission_need(*s, *A);
+      return s->used + s->reserved + need <= s->budget || (s->used == 0 && s->reserved == 0);
+    });
+    if (s->builder_stop) return;
+    A->state = 1;
+    s->reserved += need;
+    const bool pairs = s->n_lanes > 0;
+    lk.unlock();
+    cfd_grid* g = nullptr;
+    const int rc = create_grid(&A->c, &g, pairs);
+    const std::string error = rc ? g_error : std::string();
+    lk.lock();
+    s->reserved -= need;
+    if (rc) {
+      A->state = 3;
+      A->rc = rc;
+      A->error = error;
+      continue;
+    }
+    if (s->used + s->reserved + g->footprint > s->budget) {
+      if (s->used == 0 && s->reserved == 0) {
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "HBM admission: the grid needs %zu bytes, more than the whole budget of %zu",
+                      g->footprint, s->budget);
+        A->state = 3;
+        A->rc = CFD_ENOMEM;
+        A->error = buf;
+      } else {
+        // waits for room for its measured footprint
+        A->state = 0;
+        A->need = g->footprint;
+      }
+      lk.unlock();
+      cfd_grid_destroy(g);
+      lk.lock();
+      s->acv.notify_all();
+      continue;
+    }
+    admit_locked(s, g);
+    A->g = g;
+    A->state = 2;
+    // a smaller measured footprint than the estimate leaves room for another build
+    s->acv.notify_all();
+  }
+}
+}  // namespace
+
 extern "C" {
 
 int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out) {
@@ -2679,6 +2799,8 @@ int cfd_server_create(int max_active, size_t hbm_budget_bytes, cfd_server** out)
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) hbm_budget_bytes = free_b / 10 * 9;
   }
   s->budget = hbm_budget_bytes;
+  if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
+  if (const char* env = std::getenv("CFD_SERVE_BUILDERS")) s->n_builders = std::max(1, std::min(4, std::atoi(env)));
   if (const char* env = std::getenv("CFD_SERVE_AHEAD")) s->ahead = std::max(1, std::min(kStepRing - 1, std::atoi(env)));
   *out = s;
   return CFD_OK;
@@ -2709,25 +2831,90 @@ int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out) {
   // a batching server's grids take two fine cells per thread (the batched layout)
   int rc = create_grid(c, &g, s->n_lanes > 0);
   if (rc) return rc;
-  if (s->used + g->footprint > s->budget) {
+  std::unique_lock<std::mutex> lk(s->am);
+  if (s->used + s->reserved + g->footprint > s->budget) {
     char buf[200];
     std::snprintf(buf, sizeof buf, "HBM admission refused: the grid needs %zu bytes, %zu of %zu are in use", g->footprint,
-                  s->used, s->budget);
+                  s->used + s->reserved, s->budget);
+    lk.unlock();
     cfd_grid_destroy(g);
     set_error(buf);
     return CFD_ENOMEM;
   }
-  s->used += g->footprint;
-  s->grids[g] = g->footprint;
+  admit_locked(s, g);
   *out = g;
   return CFD_OK;
 }
 
-int cfd_server_release(cfd_server* s, cfd_grid* g) {
-  auto it = s->grids.find(g);
-  if (it == s->grids.end()) {
-    set_error("cfd_server_release: the grid was not admitted by this server");
+int cfd_server_admit_async(cfd_server* s, const cfd_case* c, uint64_t* ticket) {
+  if (!s || !c || !ticket || c->n_blocks < 0 || (c->n_blocks > 0 && !c->blocks)) {
+    set_error("cfd_server_admit_async: null argument or block list");
     return CFD_EINVAL;
+  }
+  for (int32_t i = 0; i < c->n_blocks; ++i)
+    if (c->blocks[i].n_vertices < 0 || (c->blocks[i].n_vertices > 0 && !c->blocks[i].footprint_xy)) {
+      set_error("cfd_server_admit_async: a prism without its footprint");
+      return CFD_EINVAL;
+    }
+  std::lock_guard<std::mutex> lk(s->am);
+  const uint64_t id = s->next_admission++;
+  cfd_server::Admission& A = s->admissions[id];
+  // the case is copied: the caller's arrays need not outlive this call
+  A.c = *c;
+  A.blocks.assign(c->blocks, c->blocks + c->n_blocks);
+  A.footprints.resize(A.blocks.size());
+  for (size_t i = 0; i < A.blocks.size(); ++i) {
+    cfd_block& b = A.blocks[i];
+    if (b.n_vertices > 0) {
+      A.footprints[i].assign(b.footprint_xy, b.footprint_xy + 2 * (size_t)b.n_vertices);
+      b.footprint_xy = A.footprints[i].data();
+    } else {
+      b.footprint_xy = nullptr;
+    }
+  }
+  A.c.blocks = A.blocks.empty() ? nullptr : A.blocks.data();
+  if (s->builders.empty())
+    for (int i = 0; i < s->n_builders; ++i) s->builders.emplace_back(builder_loop, s);
+  s->acv.notify_all();
+  *ticket = id;
+  return CFD_OK;
+}
+
+int cfd_server_admitted(cfd_server* s, uint64_t ticket, int* done, cfd_grid** out) {
+  if (!s || !done || !out) {
+    set_error("cfd_server_admitted: null argument");
+    return CFD_EINVAL;
+  }
+  *done = 0;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(s->am);
+  auto it = s->admissions.find(ticket);
+  if (it == s->admissions.end()) {
+    set_error("cfd_server_admitted: unknown admission ticket");
+    return CFD_EINVAL;
+  }
+  cfd_server::Admission& A = it->second;
+  if (A.state < 2) return CFD_OK;
+  *done = 1;
+  const int rc = A.rc;
+  if (A.state == 2)
+    *out = A.g;
+  else
+    set_error(A.error);
+  s->admissions.erase(it);
+  return rc;
+}
+
+int cfd_server_release(cfd_server* s, cfd_grid* g) {
+  size_t footprint = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->am);
+    auto it = s->grids.find(g);
+    if (it == s->grids.end()) {
+      set_error("cfd_server_release: the grid was not admitted by this server");
+      return CFD_EINVAL;
+    }
+    footprint = it->second;
   }
   while (s->busy.count(g)) {
     if (server_progress(s) < 0) return CFD_EDEVICE;
@@ -2741,13 +2928,19 @@ int cfd_server_release(cfd_server* s, cfd_grid* g) {
   // batched steps queued before the last solve on the grid ended still name it
   for (auto& L : s->lanes)
     if (L->bs) HIP_TRY(hipStreamSynchronize(L->bs));
-  s->used -= it->second;
-  s->grids.erase(it);
+  // the memory is freed before a queued admission may take its place
   cfd_grid_destroy(g);
+  {
+    std::lock_guard<std::mutex> lk(s->am);
+    s->used -= footprint;
+    s->grids.erase(g);
+  }
+  s->acv.notify_all();
   return CFD_OK;
 }
 
 int cfd_server_usage(const cfd_server* s, size_t* used, size_t* budget) {
+  std::lock_guard<std::mutex> lk(const_cast<cfd_server*>(s)->am);
   if (used) *used = s->used;
   if (budget) *budget = s->budget;
   return CFD_OK;

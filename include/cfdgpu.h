@@ -379,6 +379,21 @@ int cfd_server_set_batching(cfd_server* s, int lanes);
 void cfd_server_destroy(cfd_server* s);
 /* cfd_grid_create under the budget; CFD_ENOMEM (nothing allocated) when it does not fit. */
 int cfd_server_admit(cfd_server* s, const cfd_case* c, cfd_grid** out);
+/* Queued admission: the case is copied and its grid built by one of the
+ * server's builder threads (two; CFD_SERVE_BUILDERS 1..4) as soon as the budget
+ * has room for it beside the admitted grids and the builds under way; builds
+ * start in ticket order (a grid that does not fit waits for cfd_server_release
+ * of others, and the admissions behind it wait too) and may end out of order.
+ * Grid set-ups so overlap the running solves. Room is judged by an estimate (the cells
+ * times the largest footprint per cell measured so far, 256 B a cell before
+ * the first grid) and checked against the measured footprint after the build
+ * (a grid over the estimate that does not fit is freed and waits again with its
+ * measured size). cfd_server_admit does not wait for queued admissions. */
+int cfd_server_admit_async(cfd_server* s, const cfd_case* c, uint64_t* ticket);
+/* done = 0 (out NULL) while the admission waits or builds; done = 1 with the
+ * grid in out once admitted, or with the build's error code (CFD_ENOMEM when
+ * the grid alone exceeds the budget). The ticket is forgotten once done. */
+int cfd_server_admitted(cfd_server* s, uint64_t ticket, int* done, cfd_grid** out);
 int cfd_server_release(cfd_server* s, cfd_grid* g);
 int cfd_server_usage(const cfd_server* s, size_t* used, size_t* budget);
 size_t cfd_grid_footprint(const cfd_grid* g);
