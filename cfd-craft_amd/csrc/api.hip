@@ -2368,8 +2368,7 @@ BatchSolve<T> batch_solve(cfd_grid* g, const FineRB<T>& F, const std::vector<Agg
   b.zero_last = g->zero_last;
   b.fine_rounds = g->fine_rounds;
   b.coarse_rounds = g->coarse_rounds;
-  const int tiles = ((g->geo.n[0] + 63) / 64) * ((g->geo.n[1] + 15) / 16) * ((g->geo.n[2] + 15) / 16);
-  b.march = g->march == 2 || (g->march == 1 && tiles >= kReduceGrid);
+  b.march = g->march == 2 || (g->march == 1 && march_fits(g->geo));
   return b;
 }
 

@@ -312,9 +312,27 @@ constexpr int MX = 64, MY = 16, MZ = 16;
 static_assert(MX * MY == RB, "one thread per column of a tile");
 struct MarchShape {
   int tx, ty, tz;  // tiles in x, y and layer ranges in z
+  int mz;          // layers of a range (at most MZ)
 };
+// The shortest layer range tried (CFD_MARCH_MZ, 1 .. 16; 4: a 192x192x96 grid
+// marches in ranges of 4 layers, which took the served mix of 64^3 .. 192x192x96
+// grids from 1137 to 1167 solves/s against the cell-by-cell kernel; 2 gained
+// nothing more, and the 128x128x64 urban block solves at the same rate either way).
+inline int march_min_mz() {
+  static const int v = [] {
+    const char* e = std::getenv("CFD_MARCH_MZ");
+    const int m = e ? std::atoi(e) : 4;
+    return m >= 1 && m <= MZ ? m : MZ;
+  }();
+  return v;
+}
+// Ranges of MZ layers, halved while the grid gives fewer items than the fixed
+// grid has blocks (down to march_min_mz()).
 inline MarchShape march_shape(const Geo& g) {
-  return MarchShape{(g.n[0] + MX - 1) / MX, (g.n[1] + MY - 1) / MY, (g.n[2] + MZ - 1) / MZ};
+  const int tx = (g.n[0] + MX - 1) / MX, ty = (g.n[1] + MY - 1) / MY;
+  int mz = MZ;
+  while (mz / 2 >= march_min_mz() && (long long)tx * ty * ((g.n[2] + mz - 1) / mz) < RP) mz /= 2;
+  return MarchShape{tx, ty, (g.n[2] + mz - 1) / mz, mz};
 }
 
 template <bool HAS_CLOSED, bool FIRST, bool RES, class TZ, class TP>
@@ -342,7 +360,7 @@ __device__ __forceinline__ double pcg_op_march(const Geo& g, MarchShape ms, cons
     const int ox = txi * MX, oy = tyi * MY;
     const int wx = min(MX, g.n[0] - ox), wy = min(MY, g.n[1] - oy);  // live columns of the tile
     const int i = ox + lx, j = oy + ly;
-    const int k0 = tzi * MZ, k1 = min(g.n[2], k0 + MZ);
+    const int k0 = tzi * ms.mz, k1 = min(g.n[2], k0 + ms.mz);
     const bool live = lx < wx && ly < wy;
     const long long col = (long long)i + (long long)j * g.sy;
     // The rim: the x and y neighbours (nb_or_self: wrapping, or the edge cell
@@ -3487,6 +3505,13 @@ __global__ void __launch_bounds__(BS) k_project_faces(Geo g, FaceGeo fg, const u
 __global__ void __launch_bounds__(BS) k_add(long long N, double* __restrict__ p, const double* __restrict__ q) {
   long long i = (long long)blockIdx.x * BS + threadIdx.x;
   if (i < N) p[i] += q[i];
+}
+
+// Whether the operator marches through z on this grid: once every block of the
+// fixed grid gets a tile.
+bool march_fits(const Geo& g) {
+  const MarchShape ms = march_shape(g);
+  return (long long)ms.tx * ms.ty * ms.tz >= RP;
 }
 
 // ---------------------------------------------------------------- launchers

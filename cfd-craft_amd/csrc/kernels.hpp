@@ -82,6 +82,10 @@ void launch_apply(const Geo& g, const uint8_t* codes, const double* x, double* y
 // zhalf > 0: z in the fine red-black level's colour-separated layout (cell c at
 // c / 2 of its colour's half of zhalf entries; nx even, no closed component);
 // z32: z as FP32 values in the natural layout (no closed component)
+// Whether the PCG operator kernel marches through z on this grid (a tile of
+// 64 x 16 columns over 16 layers, or over 8, 4 ... down to CFD_MARCH_MZ layers,
+// for every block of the fixed reduction grid).
+bool march_fits(const Geo& g);
 void launch_pcg_op(const Geo& g, const uint8_t* codes, const double* z, const double* pold, double* pnew, double* ap,
                    const double* mean, const int* cidx, bool closed, PcgState* st, double* part, hipStream_t s,
                    int march = 1, long long zhalf = 0, const float* z32 = nullptr);
