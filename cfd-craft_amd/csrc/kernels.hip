@@ -326,6 +326,15 @@ inline int march_min_mz() {
   }();
   return v;
 }
+// The fine restriction with two bins per thread where nx is a multiple of 4
+// (CFD_RB_RESTRICT_PAIRS=0: one bin per thread).
+inline bool rb_restrict_pairs() {
+  static const bool v = [] {
+    const char* e = std::getenv("CFD_RB_RESTRICT_PAIRS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return v;
+}
 // Ranges of MZ layers, halved while the grid gives fewer items than the fixed
 // grid has blocks (down to march_min_mz()).
 inline MarchShape march_shape(const Geo& g) {
@@ -2800,6 +2809,8 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
 
 // Fine residual of cell q from the colour-separated layout.
 template <class T>
+__device__ __forceinline__ T rb_resid(const Geo& g, uint8_t code, T bv, T xv, const int q[3], const T xn[6]);
+template <class T>
 __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const int q[3]) {
   const int colour = (q[0] + q[1] + q[2]) & 1;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
@@ -2818,6 +2829,13 @@ __device__ __forceinline__ T rb_residual(const Geo& g, const FineRB<T>& F, const
       nb_coords(g, q, a, side, qn);
       xn[2 * a + side] = xo[at(qn)];
     }
+  return rb_resid<T>(g, code, bv, xv, q, xn);
+}
+
+// The residual b - A x of a cell from its code, b, x and the six neighbour
+// values (axis-major, lower side first; uncoupled sides not used).
+template <class T>
+__device__ __forceinline__ T rb_resid(const Geo& g, uint8_t code, T bv, T xv, const int q[3], const T xn[6]) {
   if (!(code & FLUID)) return T(0);
   T ax = T(0), off = T(0);
 #pragma unroll
@@ -2904,6 +2922,124 @@ __global__ void __launch_bounds__(BS) k_rb_restrict(Geo g, FineRB<T> F, const in
                                                      T* __restrict__ bc, T* __restrict__ xc,
                                                      const T* __restrict__ cdiag, int red_end, const PcgState* st) {
   rb_restrict_body<T>(g, F, fparent, bc, xc, cdiag, red_end, st, blockIdx.x);
+}
+
+// The residuals of the red cells at entries i2, i2 + 1 of colour row (j, k)
+// (offset u): rb_pair's loads (16-byte pairs of b, x and the black neighbours),
+// rb_residual's arithmetic, so the values are bitwise rb_residual's.
+template <class T>
+__device__ __forceinline__ void rb_residual_pair(const Geo& g, const FineRB<T>& F, int i2, int j, int k, unsigned u,
+                                                 T& ra, T& rb) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
+  const int par = (j + k) & 1;
+  const int ia = 2 * i2 + par, ib = ia + 2;
+  const T* __restrict__ xo = F.x + F.half;
+  const T2 bv = *reinterpret_cast<const T2*>(F.b + u);
+  const T2 xv = *reinterpret_cast<const T2*>(F.x + u);
+  const uint8_t ca = F.codes[u], cbc = F.codes[u + 1];
+  const unsigned row = hx * ((unsigned)j + ny * (unsigned)k);
+  const T2 mid = *reinterpret_cast<const T2*>(xo + row + (unsigned)i2);
+  int e = par ? i2 + 2 : i2 - 1;
+  if (e < 0) e = g.per[0] ? (int)hx - 1 : 0;
+  if (e >= (int)hx) e = g.per[0] ? 0 : (int)hx - 1;
+  const T edge = xo[row + (unsigned)e];
+  T yz[4][2];
+#pragma unroll
+  for (int a = 1; a < 3; ++a)
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int qa[3] = {ia, j, k};
+      int qn[3];
+      nb_coords(g, qa, a, side, qn);
+      const T2 v = *reinterpret_cast<const T2*>(xo + (unsigned)i2 + hx * ((unsigned)qn[1] + ny * (unsigned)qn[2]));
+      yz[2 * (a - 1) + side][0] = v.x;
+      yz[2 * (a - 1) + side][1] = v.y;
+    }
+  T xa[6], xb[6];
+  xa[0] = par ? mid.x : edge;
+  xa[1] = par ? mid.y : mid.x;
+  xb[0] = par ? mid.y : mid.x;
+  xb[1] = par ? edge : mid.y;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    xa[2 + f] = yz[f][0];
+    xb[2 + f] = yz[f][1];
+  }
+  const int qa[3] = {ia, j, k}, qb[3] = {ib, j, k};
+  ra = rb_resid<T>(g, ca, bv.x, xv.x, qa, xa);
+  rb = rb_resid<T>(g, cbc, bv.y, xv.y, qb, xb);
+}
+
+// rb_restrict_body with two bins per thread (nx a multiple of 4): a workgroup of
+// 128 threads takes one row of 64 2x2x2 bins, each half-wavefront one (dy, dz)
+// sub-row of it, a thread the bins 2 h and 2 h + 1 of the row (their red cells
+// at consecutive entries of the colour row: pair loads, as the pair
+// half-sweeps). The residuals and parents meet in LDS as before and the first
+// wavefront sums each bin's parents in sub order: bitwise rb_restrict_body.
+constexpr int RS2 = 128;
+template <class T>
+__device__ __forceinline__ void rb_restrict2_body(const Geo& g, const FineRB<T>& F, const int* __restrict__ fparent,
+                                                  T* __restrict__ bc, T* __restrict__ xc, const T* __restrict__ cdiag,
+                                                  int red_end, const PcgState* st, unsigned bid) {
+  __shared__ T sr[4][64];
+  __shared__ int sp[4][64], spb[4][64];
+  const int status = st ? st->status : ST_RUNNING;
+  const unsigned bx = (unsigned)g.n[0] >> 1, by = (unsigned)(g.n[1] + 1) >> 1;
+  const unsigned gbx = (bx + 63) >> 6;
+  const unsigned tile = bid, rest = tile / gbx;
+  const unsigned sub = threadIdx.x >> 5, h = threadIdx.x & 31;
+  const unsigned B0 = (tile % gbx) * 64 + 2 * h, By = rest % by, Bz = rest / by;
+  const int dy = (int)(sub & 1), dz = (int)(sub >> 1);
+  const int dxr = (dy + dz) & 1;
+  const int j = (int)(2 * By) + dy, k = (int)(2 * Bz) + dz;
+  T r0 = T(0), r1 = T(0);
+  int p0 = -1, p1 = -1, pb0 = -1, pb1 = -1;
+  // bx is even (nx a multiple of 4), so both bins of a pair are in the row or neither
+  if (B0 < bx && j < g.n[1] && k < g.n[2]) {
+    const int2 bp = *reinterpret_cast<const int2*>(F.bpar + B0 + bx * (By + by * Bz));
+    const unsigned row = (unsigned)g.sy * (unsigned)j + (unsigned)g.sz * (unsigned)k;
+    const int ir = (int)(2 * B0) + dxr, ib = (int)(2 * B0) + (1 - dxr);
+    p0 = bp.x != kMixedParent ? bp.x : fparent[row + (unsigned)ir];
+    p1 = bp.y != kMixedParent ? bp.y : fparent[row + (unsigned)ir + 2];
+    // the black cells were relaxed last: their residuals vanish (taken as zero)
+    pb0 = bp.x != kMixedParent ? bp.x : fparent[row + (unsigned)ib];
+    pb1 = bp.y != kMixedParent ? bp.y : fparent[row + (unsigned)ib + 2];
+    const unsigned u = B0 + (unsigned)F.hx * ((unsigned)j + (unsigned)g.n[1] * (unsigned)k);
+    rb_residual_pair<T>(g, F, (int)B0, j, k, u, r0, r1);
+  }
+  sr[sub][2 * h] = r0;
+  sr[sub][2 * h + 1] = r1;
+  sp[sub][2 * h] = p0;
+  sp[sub][2 * h + 1] = p1;
+  spb[sub][2 * h] = pb0;
+  spb[sub][2 * h + 1] = pb1;
+  __syncthreads();
+  const unsigned lane = threadIdx.x, Bx = (tile % gbx) * 64 + lane;
+  if (lane >= 64 || Bx >= bx || status != ST_RUNNING) return;
+  const T rs[8] = {sr[0][lane], T(0), T(0), sr[1][lane], T(0), sr[2][lane], sr[3][lane], T(0)};
+  const int ps[8] = {sp[0][lane], spb[0][lane], spb[1][lane], sp[1][lane], spb[2][lane], sp[2][lane], sp[3][lane],
+                     spb[3][lane]};
+  unsigned done = 0;
+#pragma unroll
+  for (int s2 = 0; s2 < 8; ++s2) {
+    if (ps[s2] < 0 || (done & (1u << s2))) continue;
+    T sum = T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u >= s2 && ps[u] == ps[s2]) {
+        sum += rs[u];
+        done |= 1u << u;
+      }
+    bc[ps[s2]] = sum;
+    xc[ps[s2]] = first_relax<T>(ps[s2], red_end, sum, cdiag);
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(RS2) k_rb_restrict2(Geo g, FineRB<T> F, const int* __restrict__ fparent,
+                                                       T* __restrict__ bc, T* __restrict__ xc,
+                                                       const T* __restrict__ cdiag, int red_end, const PcgState* st) {
+  rb_restrict2_body<T>(g, F, fparent, bc, xc, cdiag, red_end, st, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- multigrid (the reference's cycle)
@@ -3960,10 +4096,16 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
       launch_rb_half<T>(g0, *rb, pass & 1, nox, fparent, cs, (double*)nullptr, st, s);
     // red cells: code, b, x; black x; one parent per 2x2x2 block; the first
     // coarse level's b and x written and its diagonal read
-    PROBED_LAUNCH(PROBE_FINE_RESTRICT,
-                  (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N / 8 * 4 + (double)lv[0].n * 3 * sizeof(T),
-                  (k_rb_restrict<T>), dim3((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2)),
-                  dim3(BS), 0, s, g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
+    const double restrict_bytes =
+        (double)rb->half * (1 + 3 * sizeof(T)) + (double)g0.N / 8 * 4 + (double)lv[0].n * 3 * sizeof(T);
+    const unsigned restrict_blocks =
+        (unsigned)((((g0.n[0] + 1) / 2 + 63) / 64) * ((g0.n[1] + 1) / 2) * ((g0.n[2] + 1) / 2));
+    if (g0.n[0] % 4 == 0 && rb_restrict_pairs())
+      PROBED_LAUNCH(PROBE_FINE_RESTRICT, restrict_bytes, (k_rb_restrict2<T>), dim3(restrict_blocks), dim3(RS2), 0, s,
+                    g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
+    else
+      PROBED_LAUNCH(PROBE_FINE_RESTRICT, restrict_bytes, (k_rb_restrict<T>), dim3(restrict_blocks), dim3(BS), 0, s,
+                    g0, *rb, fparent, lv[0].b, lv[0].x, lv[0].diag, lv[0].cptr[1], st);
   } else if (!seams) {
     for (int pass = 0; pass < 2 * fr; ++pass)
       hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, nox,
@@ -4444,6 +4586,13 @@ __global__ void __launch_bounds__(BS) kb_rb_restrict(Batch<BFineRestrict<T>> B) 
   rb_restrict_body<T>(a.g, a.F, a.fparent, a.bc, a.xc, a.cdiag, a.red_end, a.st, bid);
 }
 template <class T>
+__global__ void __launch_bounds__(RS2) kb_rb_restrict2(Batch<BFineRestrict<T>> B) {
+  unsigned bid;
+  const BFineRestrict<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_restrict2_body<T>(a.g, a.F, a.fparent, a.bc, a.xc, a.cdiag, a.red_end, a.st, bid);
+}
+template <class T>
 __global__ void __launch_bounds__(BS) kb_agg_gs(Batch<BAggGs<T>> B) {
   unsigned bid;
   const BAggGs<T>& a = batch_item(B, bid);
@@ -4567,7 +4716,6 @@ static_assert(sizeof(Batch<BAggRestrict<double>>) <= 4096, "kernel arguments of 
 
 template <class T>
 void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
-  constexpr bool MIXED = std::is_same<T, float>::value;
   if (n <= 0 || n > kBatchMax) return;
   auto pair_blocks = [](const FineRB<T>& F) { return (unsigned)((F.half / 2 + BS - 1) / BS); };
   const int fr = it[0].fine_rounds;
@@ -4586,7 +4734,11 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
       L.add({g, it[i].F, it[i].fparent, L0.b, L0.x, L0.diag, L0.cptr[1], it[i].st,
              (unsigned)((((g.n[0] + 1) / 2 + 63) / 64) * ((g.n[1] + 1) / 2) * ((g.n[2] + 1) / 2))});
     }
-    BATCH_LAUNCH(kb_rb_restrict<T>, L, BS, s);
+    // batched grids have nx a multiple of 4
+    if (rb_restrict_pairs())
+      BATCH_LAUNCH(kb_rb_restrict2<T>, L, RS2, s);
+    else
+      BATCH_LAUNCH(kb_rb_restrict<T>, L, BS, s);
   }
   int max_stop = 0;
   for (int i = 0; i < n; ++i) max_stop = std::max(max_stop, it[i].nlev - 1);
