@@ -2606,6 +2606,27 @@ __global__ void __launch_bounds__(BS) k_rb_prolong(long long half, T* __restrict
                                                    const T* __restrict__ xc, T cs, const PcgState* st) {
   rb_prolong_body<T>(half, xr, rpar, xc, cs, st, blockIdx.x);
 }
+// The same with two red cells per thread (pair loads of x and the parents;
+// half even).
+template <class T>
+__device__ __forceinline__ void rb_prolong2_body(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
+                                                 const T* __restrict__ xc, T cs, const PcgState* st, unsigned bid) {
+  using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
+  const int status = st ? st->status : ST_RUNNING;
+  const long long u = 2 * ((long long)bid * BS + threadIdx.x);
+  if (u >= half || status != ST_RUNNING) return;
+  const int2 p = *reinterpret_cast<const int2*>(rpar + u);
+  if (p.x < 0 && p.y < 0) return;
+  T2 v = *reinterpret_cast<const T2*>(xr + u);
+  if (p.x >= 0) v.x += cs * xc[p.x];
+  if (p.y >= 0) v.y += cs * xc[p.y];
+  *reinterpret_cast<T2*>(xr + u) = v;
+}
+template <class T>
+__global__ void __launch_bounds__(BS) k_rb_prolong2(long long half, T* __restrict__ xr, const int* __restrict__ rpar,
+                                                    const T* __restrict__ xc, T cs, const PcgState* st) {
+  rb_prolong2_body<T>(half, xr, rpar, xc, cs, st, blockIdx.x);
+}
 
 // The last post-smoothing half-sweep (colour 0) together with the PCG's r . z
 // (pressure.rs:428) for a grid without closed components, whose z needs no
@@ -4194,9 +4215,13 @@ void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0,
       if (xcp && rb->rpar) {
         // the correction added to the red cells first, then a plain half-sweep
         // red x in and out, its parent; the first coarse level's values
-        PROBED_LAUNCH(PROBE_FINE_PROLONG, (double)rb->half * (2 * sizeof(T) + 4) + (double)lv[0].n * sizeof(T),
-                      (k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0, s, rb->half, rb->x, rb->rpar, xcp, cs,
-                      st);
+        const double prolong_bytes = (double)rb->half * (2 * sizeof(T) + 4) + (double)lv[0].n * sizeof(T);
+        if (rb->half % 2 == 0)
+          PROBED_LAUNCH(PROBE_FINE_PROLONG, prolong_bytes, (k_rb_prolong2<T>), dim3(blocks_for(rb->half / 2)), dim3(BS),
+                        0, s, rb->half, rb->x, rb->rpar, xcp, cs, st);
+        else
+          PROBED_LAUNCH(PROBE_FINE_PROLONG, prolong_bytes, (k_rb_prolong<T>), dim3(blocks_for(rb->half)), dim3(BS), 0,
+                        s, rb->half, rb->x, rb->rpar, xcp, cs, st);
         xcp = nox;
       }
       // z in the natural layout from the last two half-sweeps (the r . z kernel
@@ -4627,6 +4652,13 @@ __global__ void __launch_bounds__(BS) kb_rb_prolong(Batch<BRbProlong<T>> B) {
   if (bid >= a.nblk) return;
   rb_prolong_body<T>(a.half, a.xr, a.rpar, a.xc, a.cs, a.st, bid);
 }
+template <class T>
+__global__ void __launch_bounds__(BS) kb_rb_prolong2(Batch<BRbProlong<T>> B) {
+  unsigned bid;
+  const BRbProlong<T>& a = batch_item(B, bid);
+  if (bid >= a.nblk) return;
+  rb_prolong2_body<T>(a.half, a.xr, a.rpar, a.xc, a.cs, a.st, bid);
+}
 template <class T, class TR>
 __global__ void __launch_bounds__(RB, 8) kb_rb_gs_rz2_lin(Batch<BRz<T>> B) {
   unsigned bid;
@@ -4828,11 +4860,12 @@ void launch_batch_iteration(const BatchSolve<T>* it, int n, hipStream_t s) {
   // fine post-smoothing: the correction added to the red cells, black, red, ...,
   // and the last red half-sweep with the partials of r . z
   {
+    // batched grids have nx a multiple of 4: an even number of red cells
     Batcher<BRbProlong<T>> L;
     for (int i = 0; i < n; ++i)
       L.add({it[i].F.half, it[i].F.x, it[i].F.rpar, (const T*)it[i].lv[0].x, it[i].cs, it[i].st,
-             blocks_for(it[i].F.half)});
-    BATCH_LAUNCH(kb_rb_prolong<T>, L, BS, s);
+             blocks_for(it[i].F.half / 2)});
+    BATCH_LAUNCH(kb_rb_prolong2<T>, L, BS, s);
   }
   for (int pass = 0; pass + 1 < 2 * fr; ++pass) {
     Batcher<BHalf<T>> L;
