@@ -511,20 +511,38 @@ int build_agg(cfd_grid* g) {
         return (int32_t)table.size() - 1;
       };
       const int32_t pad = slot(0.0);
-      std::vector<int32_t> ec((size_t)K * (size_t)L.n);
+      // the distinct values in order of first appearance (runs of one value
+      // skip the lookup), then the packed entries over threads (read-only map)
       bool fits = true;
-      for (int64_t i = 0; i < L.n && fits; ++i)
-        for (int64_t k = 0; k < K; ++k) {
-          const int64_t e = L.ptr[(size_t)i] + k;
-          const bool entry = e < L.ptr[(size_t)i + 1];
-          const int32_t v = entry ? slot(L.val[(size_t)e]) : pad;
-          if (v < 0) {
-            fits = false;
-            break;
-          }
-          const uint32_t w = (uint32_t)(entry ? L.col[(size_t)e] : (int32_t)i) | ((uint32_t)v << 24);
-          ec[(size_t)k * (size_t)L.n + (size_t)i] = (int32_t)w;
+      {
+        uint64_t last = 0;
+        std::memcpy(&last, &L.val[0], sizeof last);
+        fits = slot(L.val[0]) >= 0;
+        for (size_t e = 1; e < L.val.size() && fits; ++e) {
+          uint64_t key;
+          std::memcpy(&key, &L.val[e], sizeof key);
+          if (key == last) continue;
+          last = key;
+          fits = slot(L.val[e]) >= 0;
         }
+      }
+      std::vector<int32_t> ec(fits ? (size_t)K * (size_t)L.n : 0);
+      if (fits)
+        parallel_ranges(L.n, [&](int64_t i0, int64_t i1) {
+          for (int64_t i = i0; i < i1; ++i)
+            for (int64_t k = 0; k < K; ++k) {
+              const int64_t e = L.ptr[(size_t)i] + k;
+              const bool entry = e < L.ptr[(size_t)i + 1];
+              int32_t v = pad;
+              if (entry) {
+                uint64_t key;
+                std::memcpy(&key, &L.val[(size_t)e], sizeof key);
+                v = index.find(key)->second;
+              }
+              const uint32_t w = (uint32_t)(entry ? L.col[(size_t)e] : (int32_t)i) | ((uint32_t)v << 24);
+              ec[(size_t)k * (size_t)L.n + (size_t)i] = (int32_t)w;
+            }
+        });
       if (fits) {
         const int* ep = nullptr;
         if ((rc = upi(ec.data(), ec.size(), &ep))) return rc;

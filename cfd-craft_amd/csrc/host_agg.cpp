@@ -386,7 +386,8 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
   }
   // lowest member of each node's aggregate (bins of one layer per task: they
   // hold disjoint nodes)
-  std::vector<int32_t> low((size_t)L0.n);
+  std::vector<int32_t> low;
+  concurrently([&] { low.resize((size_t)L0.n); }, [&] { parent.assign((size_t)L0.n, -1); });
   parallel_ranges(nb[2], [&](int64_t z0, int64_t z1) {
   for (int64_t bz = z0; bz < z1; ++bz)
     for (int64_t by = 0; by < nb[1]; ++by)
@@ -437,7 +438,6 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
   });
   // aggregates numbered by their lowest node: the roots counted per chunk,
   // numbered from the chunk's offset, then every other node takes its root's
-  parent.assign((size_t)L0.n, -1);
   constexpr int64_t kChunks = 64;
   std::vector<int64_t> roots(kChunks + 1, 0);
   parallel_ranges(kChunks, [&](int64_t t0, int64_t t1) {
@@ -538,20 +538,22 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
       std::vector<Entry>().swap(l);
     }
   }
-  C.ea.resize(sorted.size());
-  C.eb.resize(sorted.size());
-  C.ew.resize(sorted.size());
-  for (int64_t r = 0; r < m; ++r) {
-    const int64_t b0 = count[(size_t)r], b1e = count[(size_t)r + 1];
-    for (int64_t x = b0 + 1; x < b1e; ++x)
-      for (int64_t y = x; y > b0 && sorted[(size_t)y - 1].c > sorted[(size_t)y].c; --y)
-        std::swap(sorted[(size_t)y - 1], sorted[(size_t)y]);
-    for (int64_t e = b0; e < b1e; ++e) {
-      C.ea[(size_t)e] = sorted[(size_t)e].r;
-      C.eb[(size_t)e] = sorted[(size_t)e].c;
-      C.ew[(size_t)e] = sorted[(size_t)e].w;
+  concurrently([&] { C.ea.resize(sorted.size()); }, [&] { C.eb.resize(sorted.size()); },
+               [&] { C.ew.resize(sorted.size()); });
+  // rows are independent: sorted and copied over threads
+  parallel_ranges(m, [&](int64_t r0, int64_t r1) {
+    for (int64_t r = r0; r < r1; ++r) {
+      const int64_t b0 = count[(size_t)r], b1e = count[(size_t)r + 1];
+      for (int64_t x = b0 + 1; x < b1e; ++x)
+        for (int64_t y = x; y > b0 && sorted[(size_t)y - 1].c > sorted[(size_t)y].c; --y)
+          std::swap(sorted[(size_t)y - 1], sorted[(size_t)y]);
+      for (int64_t e = b0; e < b1e; ++e) {
+        C.ea[(size_t)e] = sorted[(size_t)e].r;
+        C.eb[(size_t)e] = sorted[(size_t)e].c;
+        C.ew[(size_t)e] = sorted[(size_t)e].w;
+      }
     }
-  }
+  });
   return true;
 }
 
@@ -583,27 +585,28 @@ int build_agg_hierarchy(const HostGrid& g, AggHierarchy& h, int64_t coarse_cells
   const int64_t* n = g.n;
   // Level 0: fluid cells in index order (numbered from per-layer counts, the
   // layers filled in parallel).
-  h.node_of_cell.assign((size_t)g.count, -1);
   AggLevel L0;
   const int64_t layer = n[0] * n[1];
   std::vector<int64_t> first((size_t)n[2] + 1, 0);
-  parallel_ranges(n[2], [&](int64_t k0, int64_t k1) {
-    for (int64_t k = k0; k < k1; ++k) {
-      int64_t m = 0;
-      for (int64_t c = k * layer; c < (k + 1) * layer; ++c) m += g.fluid[(size_t)c] ? 1 : 0;
-      first[(size_t)k + 1] = m;
-    }
-  });
+  concurrently(
+      [&] {
+        parallel_ranges(n[2], [&](int64_t k0, int64_t k1) {
+          for (int64_t k = k0; k < k1; ++k) {
+            int64_t m = 0;
+            for (int64_t c = k * layer; c < (k + 1) * layer; ++c) m += g.fluid[(size_t)c] ? 1 : 0;
+            first[(size_t)k + 1] = m;
+          }
+        });
+      },
+      [&] { h.node_of_cell.assign((size_t)g.count, -1); });
   for (int64_t k = 0; k < n[2]; ++k) first[(size_t)k + 1] += first[(size_t)k];
   L0.n = first[(size_t)n[2]];
-  h.cell_of_node.resize((size_t)L0.n);
-  L0.coords.resize(3 * (size_t)L0.n);
+  concurrently([&] { h.cell_of_node.resize((size_t)L0.n); }, [&] { L0.coords.resize(3 * (size_t)L0.n); },
+               [&] { L0.anchor.assign((size_t)L0.n, 0.0); }, [&] { L0.diag.assign((size_t)L0.n, 0.0); });
   // Per node, in face order (x faces, then y, then z; every face of an axis
   // carries that axis's weight): the anchor sums its outlet sides (a coupled
   // side on a non-periodic domain face), the diagonal is the anchor plus the
   // weights of its interior sides; each interior face has two such sides.
-  L0.anchor.assign((size_t)L0.n, 0.0);
-  L0.diag.assign((size_t)L0.n, 0.0);
   std::vector<int64_t> sides_of((size_t)n[2], 0);
   parallel_ranges(n[2], [&](int64_t k0, int64_t k1) {
     for (int64_t k = k0; k < k1; ++k) {

@@ -31,6 +31,17 @@ void parallel_ranges(int64_t n, F f) {
   for (auto& th : pool) th.join();
 }
 
+// Runs each callable on a thread of its own (the last on the calling thread)
+// and waits for all: large zero-filled allocations fault their pages in
+// parallel instead of one after another.
+template <class F, class... R>
+void concurrently(F&& f, R&&... rest) {
+  std::vector<std::thread> pool;
+  (pool.emplace_back(std::forward<R>(rest)), ...);
+  f();
+  for (auto& th : pool) th.join();
+}
+
 // Host description of a uniform staggered grid as the pressure solve sees it:
 // the restatement of Grid::build (grid.rs:246-464) in per-cell form.
 struct HostGrid {
