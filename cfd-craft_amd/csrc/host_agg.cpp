@@ -1401,10 +1401,12 @@ void colour_agg_hierarchy(AggHierarchy& h) {
       const int32_t o = inv[(size_t)r];
       ptr[(size_t)r + 1] = ptr[(size_t)r] + (L.ptr[(size_t)o + 1] - L.ptr[(size_t)o]);
     }
-    std::vector<int32_t> col((size_t)ptr[(size_t)n]);
-    std::vector<double> val((size_t)ptr[(size_t)n]), diag((size_t)n), anchor((size_t)n);
-    std::vector<int32_t> coords((size_t)(3 * n));
-    std::vector<int32_t> parent(L.parent.empty() ? 0 : (size_t)n);
+    std::vector<int32_t> col, coords, parent;
+    std::vector<double> val, diag, anchor;
+    concurrently([&] { col.resize((size_t)ptr[(size_t)n]); }, [&] { val.resize((size_t)ptr[(size_t)n]); },
+                 [&] { diag.resize((size_t)n); }, [&] { anchor.resize((size_t)n); },
+                 [&] { coords.resize((size_t)(3 * n)); },
+                 [&] { parent.resize(L.parent.empty() ? 0 : (size_t)n); });
     parallel_ranges(n, [&](int64_t r0, int64_t r1) {
       for (int64_t r = r0; r < r1; ++r) {
         const int32_t o = inv[(size_t)r];
