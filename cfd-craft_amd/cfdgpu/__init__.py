@@ -52,7 +52,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(_HERE, "libcfdgpu.so")
 
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-CYCLE_RB, CYCLE_REFERENCE, CYCLE_GEOMETRIC, CYCLE_ZEBRA = 0, 1, 2, 3
+CYCLE_RB, CYCLE_REFERENCE, CYCLE_ZEBRA = 0, 1, 3
 CANCELLED, WALL_TIME = 100, 101
 
 
@@ -544,7 +544,7 @@ class Grid:
     def set_options(self, precision: int = PRECISION_FP64, warm_start: bool = False, cycle: int = 0):
         """precision: PRECISION_FP64 or PRECISION_MIXED; cycle (multigrid): CYCLE_RB,
         CYCLE_REFERENCE (the reference's hierarchy and lexicographic smoothing, FP64),
-        CYCLE_GEOMETRIC or CYCLE_ZEBRA (the reference's hierarchy, its line and plane
+        or CYCLE_ZEBRA (the reference's hierarchy, its line and plane
         blocks included, smoothed by colour classes; FP64)."""
         o = Options(int(precision), int(bool(warm_start)), 8, int(cycle))
         began = time.perf_counter()

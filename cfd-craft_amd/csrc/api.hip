@@ -46,16 +46,12 @@ void set_error(const std::string& text) { g_error = text; }
 constexpr int kSegment = 8;          // PCG iterations per host poll (pressure.rs:387)
 constexpr int kRing = 4;             // host slots for state read-backs
 constexpr int kAhead = 2;            // segments queued beyond the last read-back
-// Multigrid shape: the coarsest level (solved by the dense pseudo-inverse) holds at
-// most kCoarseMax cells; coarse levels of at most kTailCells run inside one
-// workgroup (0: none); the coarse-grid correction is scaled by kOmega. A 2x2x2
-// piecewise-constant Galerkin operator is about twice as stiff as the
-// rediscretized one in 3-D, so the plain correction falls short by about half;
-// omega = 1.9 cut PCG iterations on the urban block from 36 to 11 (2.0 is the
-// optimum on every tested grid; 1.9 keeps a margin below the loss of
-// definiteness past 2). CFD_MG_COARSE / CFD_MG_TAIL / CFD_MG_OMEGA override.
-constexpr int64_t kCoarseMax = 512;
-constexpr int64_t kTailCells = 0;
+// The coarse-grid correction is scaled by kOmega. A 2x2x2 piecewise-constant
+// Galerkin operator is about twice as stiff as the rediscretized one in 3-D, so
+// the plain correction falls short by about half; omega = 1.9 cut PCG
+// iterations on the urban block from 36 to 11 (2.0 is the optimum on every
+// tested grid; 1.9 keeps a margin below the loss of definiteness past 2).
+// CFD_MG_OMEGA overrides.
 constexpr double kOmega = 1.9;
 // Aggregation hierarchy of the red-black cycle: coarsening stops once every
 // component has at most kAggDense nodes, and that level is solved exactly by a
@@ -94,15 +90,6 @@ struct DevBuf {
     if (count) HIP_TRY(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
     return 0;
   }
-};
-
-template <class T>
-struct DevHierarchy {
-  std::vector<DevLevel<T>> levels;
-  DevBuf<DevLevel<T>> dlevels;  // device copy of `levels` for the one-workgroup tail
-  int tail_first = 0;
-  std::vector<std::unique_ptr<DevBuf<T>>> store;
-  DevBuf<T> xpre;  // fine-level iterate after pre-smoothing (fused cycle)
 };
 
 // Aggregation hierarchy on the device. The integer arrays (CSR structure,
@@ -157,7 +144,6 @@ using namespace cfdgpu;
 
 struct cfd_grid {
   HostGrid host;
-  HostHierarchy hier;
   bool has_hier = false;
   int device = 0;
   hipStream_t stream = nullptr;
@@ -184,10 +170,6 @@ struct cfd_grid {
   PcgState* h_st = nullptr;  // kRing read-back slots
   hipEvent_t evr[kRing] = {};  // recorded after the read-back into each slot
   // multigrid
-  DevHierarchy<double> h64;
-  DevHierarchy<float> h32;
-  DevBuf<double> G;
-  bool has_geo = false;  // geometric hierarchy built (CFD_CYCLE_GEOMETRIC)
   DevBuf<float> x0f;     // FP32 fine iterate of the mixed-precision cycles
   DevBuf<float> z32;     // z of the mixed-precision red-black cycle, FP32 values in the natural layout
   DevBuf<float> ic0_y32;  // FP32 forward result of the mixed-precision IC(0) application
@@ -239,10 +221,6 @@ struct cfd_grid {
   // CFD_GPU_GRAPHS=0 launches the segments kernel by kernel (for profilers that do not follow graphs).
   bool use_graphs = true;
   LaunchProbe probe;  // event pool of cfd_profile_solve
-  int64_t coarse_max = kCoarseMax, tail_cells = kTailCells;
-  // CFD_MG_FUSED: 0 one kernel per half-sweep on every level; 1 coarse levels fused
-  // (one kernel per level and phase), the default; 2 every level fused.
-  int fused = 1;
   double omega = kOmega;
   // red-black cycle: smoothing rounds (sweeps over all colours) before and after
   // the correction on the fine level and on the coarse levels (CFD_MG_SWEEPS=f,c)
@@ -316,62 +294,6 @@ int fill_geo(cfd_grid* g) {
   F.total = off;
   long long nb = (h.count + 255) / 256;
   g->nb = (int)std::max<long long>(1, std::min<long long>(nb, kMaxReduceBlocks));
-  return 0;
-}
-
-template <class T>
-int upload_hierarchy(cfd_grid* g, DevHierarchy<T>& dh) {
-  const HostHierarchy& H = g->hier;
-  dh.levels.clear();
-  dh.store.clear();
-  auto buf = [&](const std::vector<double>& v) -> T* {
-    std::vector<T> t(v.begin(), v.end());
-    dh.store.emplace_back(new DevBuf<T>());
-    if (dh.store.back()->upload(t.data(), t.size())) return nullptr;
-    return dh.store.back()->p;
-  };
-  auto zeros = [&](size_t n) -> T* {
-    dh.store.emplace_back(new DevBuf<T>());
-    if (dh.store.back()->alloc(n)) return nullptr;
-    if (hipMemset(dh.store.back()->p, 0, n * sizeof(T)) != hipSuccess) return nullptr;
-    return dh.store.back()->p;
-  };
-  for (const HostLevel& L : H.coarse) {
-    DevLevel<T> d;
-    std::memset(&d.g, 0, sizeof d.g);
-    for (int a = 0; a < 3; ++a) {
-      d.g.n[a] = (int)L.n[a];
-      d.g.per[a] = g->host.periodic[a] ? 1 : 0;
-    }
-    d.g.N = L.count;
-    d.g.sy = L.n[0];
-    d.g.sz = L.n[0] * L.n[1];
-    for (int a = 0; a < 3; ++a) {
-      d.wp[a] = buf(L.wp[a]);
-      if (!d.wp[a]) return CFD_ENOMEM;
-    }
-    d.diag = buf(L.diag);
-    d.x = zeros((size_t)L.count);
-    d.b = zeros((size_t)L.count);
-    const bool coarsest = &L == &H.coarse.back();
-    d.xo = coarsest ? d.x : zeros((size_t)L.count);
-    if (!d.diag || !d.x || !d.b || !d.xo) return CFD_ENOMEM;
-    dh.levels.push_back(d);
-  }
-  if (!H.coarse.empty()) {
-    int rc = dh.xpre.alloc((size_t)g->host.count);
-    if (rc) return rc;
-  }
-  dh.tail_first = (int)dh.levels.size();
-  for (size_t l = 0; l < dh.levels.size(); ++l)
-    if (dh.levels[l].g.N <= g->tail_cells) {
-      dh.tail_first = (int)l;
-      break;
-    }
-  if (!dh.levels.empty()) {
-    int rc = dh.dlevels.upload(dh.levels.data(), dh.levels.size());
-    if (rc) return rc;
-  }
   return 0;
 }
 
@@ -684,30 +606,6 @@ int build_agg(cfd_grid* g) {
   return 0;
 }
 
-// Builds the geometric hierarchy of CFD_CYCLE_GEOMETRIC.
-int build_geometric(cfd_grid* g) {
-  if (g->has_geo) return 0;
-  HostGrid& h = g->host;
-  int rc = build_hierarchy(h, g->hier, g->coarse_max);
-  if (rc) return rc;
-  // A periodic axis of odd length >= 3 on some level puts two cells of one colour
-  // next to each other across the seam; only the fused kernels (which update a
-  // colour from the values before the half-sweep) handle that deterministically.
-  auto odd = [&](const int64_t* n) {
-    for (int a = 0; a < 3; ++a)
-      if (h.periodic[a] && n[a] >= 3 && (n[a] & 1)) return true;
-    return false;
-  };
-  bool odd_periodic = odd(h.n);
-  for (const HostLevel& L : g->hier.coarse) odd_periodic |= odd(L.n);
-  if (odd_periodic) g->fused = 2;
-  if ((rc = upload_hierarchy(g, g->h64))) return rc;
-  if ((rc = upload_hierarchy(g, g->h32))) return rc;
-  if ((rc = g->G.upload(g->hier.direct.data(), g->hier.direct.size()))) return rc;
-  g->has_geo = true;
-  return 0;
-}
-
 void enqueue_gauge(cfd_grid* g, const double* src, const double* scale, int e031, double* mean, PcgState* st);
 
 // Builds the reference's hierarchy (host) and the device arrays of its cycle.
@@ -972,15 +870,14 @@ const AggHierarchy& hierarchy_of(const cfd_grid* g) {
 }
 
 int ensure_cycle(cfd_grid* g, int cycle) {
-  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_GEOMETRIC && cycle != CFD_CYCLE_REFERENCE && cycle != CFD_CYCLE_ZEBRA) {
-    set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE, CFD_CYCLE_GEOMETRIC or CFD_CYCLE_ZEBRA");
+  if (cycle != CFD_CYCLE_RB && cycle != CFD_CYCLE_REFERENCE && cycle != CFD_CYCLE_ZEBRA) {
+    set_error("cfd_options.cycle must be CFD_CYCLE_RB, CFD_CYCLE_REFERENCE or CFD_CYCLE_ZEBRA");
     return CFD_EINVAL;
   }
   if (g->cycle_built[cycle]) return 0;
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = cycle == CFD_CYCLE_RB          ? build_agg(g)
-           : cycle == CFD_CYCLE_GEOMETRIC ? build_geometric(g)
-                                          : build_ref(g, cycle == CFD_CYCLE_ZEBRA ? g->zeb : g->ref, cycle == CFD_CYCLE_ZEBRA);
+  int rc = cycle == CFD_CYCLE_RB ? build_agg(g)
+                                 : build_ref(g, cycle == CFD_CYCLE_ZEBRA ? g->zeb : g->ref, cycle == CFD_CYCLE_ZEBRA);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->build_seconds[cycle] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1009,8 +906,6 @@ int setup_device_inner(cfd_grid* g) {
   for (hipEvent_t& e : g->evr) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   fill_geo(g);
   if (const char* env = std::getenv("CFD_GPU_GRAPHS")) g->use_graphs = std::string(env) != "0";
-  if (const char* env = std::getenv("CFD_MG_COARSE")) g->coarse_max = std::max<int64_t>(8, std::atoll(env));
-  if (const char* env = std::getenv("CFD_MG_TAIL")) g->tail_cells = std::atoll(env);
   if (const char* env = std::getenv("CFD_MG_DENSE")) g->agg_dense = std::max<int64_t>(1, std::atoll(env));
   if (const char* env = std::getenv("CFD_MG_RB_LAYOUT")) g->rb_layout = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_RB_PAIRS")) g->rb_pairs = std::atoi(env) ? 1 : 0;
@@ -1018,7 +913,6 @@ int setup_device_inner(cfd_grid* g) {
   if (const char* env = std::getenv("CFD_MG_RB_LIN")) g->rb_lin = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_IC0_TILES")) g->ic0_tiles = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MG_OMEGA")) g->omega = std::atof(env);
-  if (const char* env = std::getenv("CFD_MG_FUSED")) g->fused = std::atoi(env);
   if (const char* env = std::getenv("CFD_MG_ZERO_LAST")) g->zero_last = std::string(env) != "0";
   if (const char* env = std::getenv("CFD_MARCH")) g->march = std::atoi(env);
   if (const char* env = std::getenv("CFD_RB_RESIDENT")) g->rb_resident = std::string(env) != "0";
@@ -1149,44 +1043,21 @@ void enqueue_precondition(cfd_grid* g, const double* r, double* z, PcgState* st,
     launch_ref_vcycle((g->opt.cycle == CFD_CYCLE_ZEBRA ? g->zeb : g->ref).cyc, r, z, st, g->stream);
     return;
   }
-  if (g->opt.cycle == CFD_CYCLE_RB) {
-    const DevAggShared& A = g->ash;
-    // without closed components z needs no centering: the last fine half-sweep
-    // forms the r . z partials as well (no separate k_pcg_rz)
-    const double* rz_r = rz_fused(g) ? r : nullptr;
-    if (g->opt.precision == CFD_PRECISION_MIXED)
-      launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
-                                       g->a32.levels.data(), A.D, resident >= 3 ? nullptr : z, st, g->omega,
-                                       g->stream, A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
-                                       resident == 4 ? A.rb64.b : rz_r, g->part.p, g->zero_last,
-                                       resident >= 3 ? resident : 0, resident == 3 ? g->z32.p : nullptr,
-                                       first_done);
-    else
-      launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
-                                        A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
-                                        A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds,
-                                        resident ? nullptr : rz_r, g->part.p, g->zero_last, resident, nullptr,
-                                        first_done);
-    return;
-  }
-  const int nlev = (int)g->h64.levels.size();
-  const int dn = (int)g->hier.direct_n;
-  if (g->fused) {
-    if (g->opt.precision == CFD_PRECISION_MIXED)
-      launch_vcycle_fused<float, double>(g->geo, g->codes.p, r, g->x0f.p, g->h32.xpre.p, nlev, g->h32.levels.data(), dn,
-                                         g->G.p, z, st, g->omega, g->fused == 2, g->stream);
-    else
-      launch_vcycle_fused<double, double>(g->geo, g->codes.p, r, z, g->h64.xpre.p, nlev, g->h64.levels.data(), dn, g->G.p,
-                                          nullptr, st, g->omega, g->fused == 2, g->stream);
-    return;
-  }
-  if (g->opt.precision == CFD_PRECISION_MIXED) {
-    launch_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, nlev, g->h32.levels.data(), g->h32.dlevels.p,
-                                 g->h32.tail_first, dn, g->G.p, z, st, g->omega, g->stream);
-  } else {
-    launch_vcycle<double, double>(g->geo, g->codes.p, r, z, nlev, g->h64.levels.data(), g->h64.dlevels.p,
-                                  g->h64.tail_first, dn, g->G.p, nullptr, st, g->omega, g->stream);
-  }
+  const DevAggShared& A = g->ash;
+  // without closed components z needs no centering: the last fine half-sweep
+  // forms the r . z partials as well (no separate k_pcg_rz)
+  const double* rz_r = rz_fused(g) ? r : nullptr;
+  if (g->opt.precision == CFD_PRECISION_MIXED)
+    launch_agg_vcycle<float, double>(g->geo, g->codes.p, r, g->x0f.p, A.fparent.p, A.seams, A.nlev,
+                                     g->a32.levels.data(), A.D, resident >= 3 ? nullptr : z, st, g->omega, g->stream,
+                                     A.has_rb ? &A.rb32 : nullptr, g->fine_rounds, g->coarse_rounds,
+                                     resident == 4 ? A.rb64.b : rz_r, g->part.p, g->zero_last,
+                                     resident >= 3 ? resident : 0, resident == 3 ? g->z32.p : nullptr, first_done);
+  else
+    launch_agg_vcycle<double, double>(g->geo, g->codes.p, r, z, A.fparent.p, A.seams, A.nlev, g->a64.levels.data(),
+                                      A.D, A.has_rb && !resident ? z : nullptr, st, g->omega, g->stream,
+                                      A.has_rb ? &A.rb64 : nullptr, g->fine_rounds, g->coarse_rounds,
+                                      resident ? nullptr : rz_r, g->part.p, g->zero_last, resident, nullptr, first_done);
 }
 
 // One PCG iteration (pressure.rs:386-434): the preconditioner applied to the
@@ -1416,18 +1287,12 @@ void fill_info(cfd_grid* g, const PcgState& s, cfd_linear_info& local) {
     // residual, restriction and prolongation per cell and 2 x 2 point sweeps, or
     // one block sweep each way on a block-smoothed level
     int64_t cells = 0, block_cells = 0, block_lines = 0;
-    if (g->opt.cycle == CFD_CYCLE_GEOMETRIC) {
-      // without coarse levels the fine grid is the terminal (direct) level
-      cells = g->hier.coarse.empty() ? 0 : g->host.count;
-      for (size_t l = 0; l + 1 < g->hier.coarse.size(); ++l) cells += g->hier.coarse[l].count;
-    } else {
-      const AggHierarchy& H = hierarchy_of(g);
-      for (size_t l = 0; l + 1 < H.levels.size(); ++l) {
-        cells += H.levels[l].n;
-        if (H.levels[l].line_length > 0) {
-          block_cells += H.levels[l].n;
-          block_lines += (int64_t)H.levels[l].line_nodes.size() / H.levels[l].line_length;
-        }
+    const AggHierarchy& H = hierarchy_of(g);
+    for (size_t l = 0; l + 1 < H.levels.size(); ++l) {
+      cells += H.levels[l].n;
+      if (H.levels[l].line_length > 0) {
+        block_cells += H.levels[l].n;
+        block_lines += (int64_t)H.levels[l].line_nodes.size() / H.levels[l].line_length;
       }
     }
     cfd_mg_stats ms;
@@ -1792,7 +1657,7 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
     return CFD_EINVAL;
   }
   const HostGrid& h = g->host;
-  if (g->opt.cycle != CFD_CYCLE_GEOMETRIC) {
+  {
     // MultigridStats of the aggregation hierarchy (masked.rs:286-290): unknowns and
     // stored entries (off-diagonal plus diagonal) per level
     const AggHierarchy& H = hierarchy_of(g);
@@ -1813,28 +1678,6 @@ int cfd_multigrid_stats(const cfd_grid* g, cfd_mg_stats* s) {
     for (int64_t k = 0; k < s->n_block_smoothers; ++k) s->allocated_bytes += s->block_smoothers[k].allocated_bytes;
     return CFD_OK;
   }
-  int64_t fine_nnz = h.count;
-  for (uint8_t c : h.codes)
-    for (int b = 0; b < 6; ++b) fine_nnz += (c >> b) & 1;
-  s->level_cells[0] = h.count;
-  s->level_nonzeros[0] = fine_nnz;
-  for (int a = 0; a < 3; ++a) s->level_dims[0][a] = h.n[a];
-  int L = 1;
-  int64_t bytes = h.count * (int64_t)(sizeof(double) + sizeof(float));
-  for (const HostLevel& lv : g->hier.coarse) {
-    if (L >= 32) break;
-    int64_t nnz = lv.count;
-    for (int a = 0; a < 3; ++a)
-      for (double w : lv.wp[a]) nnz += w != 0.0 ? 2 : 0;
-    s->level_cells[L] = lv.count;
-    s->level_nonzeros[L] = nnz;
-    for (int a = 0; a < 3; ++a) s->level_dims[L][a] = lv.n[a];
-    bytes += lv.count * (int64_t)(6 * sizeof(double) + 6 * sizeof(float));
-    ++L;
-  }
-  s->levels = L;
-  s->allocated_bytes = bytes + (int64_t)g->hier.direct.size() * (int64_t)sizeof(double);
-  return CFD_OK;
 }
 
 int cfd_apply_device(cfd_grid* g, const double* d_x, double* d_y) {

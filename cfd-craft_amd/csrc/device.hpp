@@ -62,17 +62,6 @@ struct FaceGeo {
   long long total;
 };
 
-// One coarse level of the geometric hierarchy on the device.
-template <class T>
-struct DevLevel {
-  Geo g;
-  T* wp[3];   // coupling to the + neighbour per axis
-  T* diag;    // diagonal (0: inactive cell)
-  T* x;   // iterate after pre-smoothing (and the coarsest solution)
-  T* b;
-  T* xo;  // iterate after post-smoothing (fused cycle; equal to x on the coarsest level)
-};
-
 // One coarse level of the aggregation hierarchy on the device: a CSR matrix over
 // aggregates. In the colour-sorted numbering the rows of colour c are the range
 // [cptr[c], cptr[c+1]) and form an independent set.

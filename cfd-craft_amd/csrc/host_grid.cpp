@@ -1,5 +1,5 @@
 // Host set-up of the device grid: case -> per-cell coupling codes, fluid
-// components, the IC(0) factor diagonal and the geometric multigrid hierarchy.
+// components and the IC(0) factor diagonal.
 //
 // The topology must be the reference's (grid.rs:246-464), so the mask
 // rasterization (geometry.rs:113-193, case.rs:436-443) and the face
@@ -610,258 +610,6 @@ void build_ic0_fill(HostGrid& g, const std::vector<double>& raw, double maxdiag)
       }
     }
   }
-}
-
-namespace {
-
-// Coarsens level `f` (fine dims fn, weights from `wp_of`) into bins of 2x2x2.
-struct FineView {
-  int64_t n[3];
-  bool periodic[3];
-};
-
-int64_t half(int64_t v) { return v > 1 ? (v + 1) / 2 : 1; }
-
-// Dense Cholesky (lower, row-major) of an SPD matrix; false if not positive definite.
-bool cholesky(std::vector<double>& m, int64_t n) {
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t j = 0; j <= i; ++j) {
-      double v = m[(size_t)(i * n + j)];
-      for (int64_t k = 0; k < j; ++k) v -= m[(size_t)(i * n + k)] * m[(size_t)(j * n + k)];
-      if (i == j) {
-        if (!(v > 0.0) || !std::isfinite(v)) return false;
-        m[(size_t)(i * n + i)] = std::sqrt(v);
-      } else {
-        m[(size_t)(i * n + j)] = v / m[(size_t)(j * n + j)];
-      }
-    }
-  return true;
-}
-
-}  // namespace
-
-int build_hierarchy(const HostGrid& g, HostHierarchy& h, int64_t coarse_max) {
-  h.coarse.clear();
-  int64_t fn[3] = {g.n[0], g.n[1], g.n[2]};
-  const int64_t fine_stride[3] = {1, g.n[0], g.n[0] * g.n[1]};
-  // Level 1 from the fine grid's codes.
-  auto coarse_dims_ok = [&](const int64_t* dims) { return dims[0] * dims[1] * dims[2] > coarse_max; };
-  if (!coarse_dims_ok(fn) && g.count <= coarse_max) {
-    // The fine grid itself is the coarsest level: represent it as a weighted level.
-  }
-  // Build a weighted representation of the fine grid (level "0w") when needed as the coarsest.
-  auto fine_as_level = [&]() {
-    HostLevel L;
-    for (int a = 0; a < 3; ++a) L.n[a] = fn[a];
-    L.count = g.count;
-    for (int a = 0; a < 3; ++a) L.wp[a].assign((size_t)L.count, 0.0);
-    L.anchor.assign((size_t)L.count, 0.0);
-    for (int64_t c0 = 0; c0 < g.count; ++c0) {
-      uint8_t code = g.codes[(size_t)c0];
-      int64_t q[3] = {c0 % fn[0], c0 / fn[0] % fn[1], c0 / (fn[0] * fn[1])};
-      for (int a = 0; a < 3; ++a) {
-        for (int side = 0; side < 2; ++side) {
-          if (!(code & side_bit(a, side))) continue;
-          bool boundary = side == 0 ? q[a] == 0 : q[a] + 1 == fn[a];
-          if (boundary && !g.periodic[a]) L.anchor[(size_t)c0] += g.w_out[a];
-        }
-        if (code & side_bit(a, 1)) {
-          bool boundary = q[a] + 1 == fn[a];
-          if (!boundary || g.periodic[a]) L.wp[a][(size_t)c0] += g.w_int[a];
-        }
-      }
-    }
-    return L;
-  };
-  std::vector<HostLevel> levels;  // weighted levels: [fine-as-level?] then coarse
-  HostLevel current = fine_as_level();
-  bool current_is_fine = true;
-  while (current.count > coarse_max) {
-    int64_t cn[3];
-    bool can = false;
-    for (int a = 0; a < 3; ++a) {
-      cn[a] = half(current.n[a]);
-      can |= cn[a] != current.n[a];
-    }
-    if (!can) break;
-    HostLevel C;
-    for (int a = 0; a < 3; ++a) C.n[a] = cn[a];
-    C.count = cn[0] * cn[1] * cn[2];
-    for (int a = 0; a < 3; ++a) C.wp[a].assign((size_t)C.count, 0.0);
-    C.anchor.assign((size_t)C.count, 0.0);
-    const int64_t* n = current.n;
-    for (int64_t c0 = 0; c0 < current.count; ++c0) {
-      int64_t q[3] = {c0 % n[0], c0 / n[0] % n[1], c0 / (n[0] * n[1])};
-      int64_t b[3] = {q[0] / 2, q[1] / 2, q[2] / 2};
-      int64_t B = (b[2] * cn[1] + b[1]) * cn[0] + b[0];
-      C.anchor[(size_t)B] += current.anchor[(size_t)c0];
-      for (int a = 0; a < 3; ++a) {
-        double w = current.wp[a][(size_t)c0];
-        if (w == 0.0) continue;
-        int64_t qn = q[a] + 1 == n[a] ? 0 : q[a] + 1;  // + neighbour (wraps only when periodic: w is 0 otherwise)
-        int64_t bn = qn / 2;
-        if (bn == b[a]) continue;  // internal to the bin
-        // bn is the + neighbour of b along a (possibly by wrap)
-        C.wp[a][(size_t)B] += w;
-      }
-    }
-    if (current_is_fine) {
-      current_is_fine = false;
-    } else {
-      levels.push_back(std::move(current));
-    }
-    current = std::move(C);
-  }
-  if (current_is_fine) {
-    // No coarsening at all: the fine grid is solved directly.
-    levels.clear();
-  } else {
-    levels.push_back(std::move(current));
-  }
-  for (auto& L : levels) {
-    L.diag.assign((size_t)L.count, 0.0);
-    for (int64_t c0 = 0; c0 < L.count; ++c0) {
-      int64_t q[3] = {c0 % L.n[0], c0 / L.n[0] % L.n[1], c0 / (L.n[0] * L.n[1])};
-      double d = L.anchor[(size_t)c0];
-      const int64_t st[3] = {1, L.n[0], L.n[0] * L.n[1]};
-      for (int a = 0; a < 3; ++a) {
-        d += L.wp[a][(size_t)c0];
-        int64_t lo = q[a] > 0 ? c0 - st[a] : c0 + (L.n[a] - 1) * st[a];
-        if (q[a] > 0 || g.periodic[a]) d += L.wp[a][(size_t)lo];
-      }
-      L.diag[(size_t)c0] = d;
-    }
-  }
-  (void)fine_stride;
-  // The coarsest weighted level (or the fine grid) is solved by a dense pseudo-inverse.
-  HostLevel finest_direct;
-  const HostLevel* last;
-  if (levels.empty()) {
-    finest_direct = fine_as_level();
-    finest_direct.diag.assign((size_t)finest_direct.count, 0.0);
-    for (int64_t c0 = 0; c0 < finest_direct.count; ++c0) {
-      uint8_t code = g.codes[(size_t)c0];
-      int64_t q[3] = {c0 % fn[0], c0 / fn[0] % fn[1], c0 / (fn[0] * fn[1])};
-      double d = 0.0;
-      for (int a = 0; a < 3; ++a)
-        for (int side = 0; side < 2; ++side) {
-          if (!(code & side_bit(a, side))) continue;
-          bool boundary = side == 0 ? q[a] == 0 : q[a] + 1 == fn[a];
-          d += (boundary && !g.periodic[a]) ? g.w_out[a] : g.w_int[a];
-        }
-      finest_direct.diag[(size_t)c0] = d;
-    }
-    last = &finest_direct;
-  } else {
-    last = &levels.back();
-  }
-  const int64_t N = last->count;
-  if (N > 4096) {
-    set_error("multigrid: coarsest level too large for the dense solve");
-    return CFD_EUNSUPPORTED;
-  }
-  for (int a = 0; a < 3; ++a) h.coarsest_dims[a] = last->n[a];
-  std::vector<double> A((size_t)(N * N), 0.0);
-  const int64_t st[3] = {1, last->n[0], last->n[0] * last->n[1]};
-  for (int64_t c0 = 0; c0 < N; ++c0) {
-    int64_t q[3] = {c0 % last->n[0], c0 / last->n[0] % last->n[1], c0 / (last->n[0] * last->n[1])};
-    A[(size_t)(c0 * N + c0)] += last->diag[(size_t)c0];
-    for (int a = 0; a < 3; ++a) {
-      double w = last->wp[a][(size_t)c0];
-      if (w == 0.0) continue;
-      int64_t nb = q[a] + 1 < last->n[a] ? c0 + st[a] : c0 - (last->n[a] - 1) * st[a];
-      if (nb == c0) continue;
-      A[(size_t)(c0 * N + nb)] -= w;
-      A[(size_t)(nb * N + c0)] -= w;
-    }
-  }
-  // Components of the coarsest graph; closed ones lose their first member (gauge).
-  std::vector<int64_t> comp((size_t)N, -1);
-  std::vector<std::vector<int64_t>> members;
-  std::vector<bool> anchored;
-  for (int64_t s = 0; s < N; ++s) {
-    if (comp[(size_t)s] != -1 || last->diag[(size_t)s] == 0.0) continue;
-    int64_t id = (int64_t)members.size();
-    members.emplace_back();
-    bool anch = false;
-    std::vector<int64_t> q{s};
-    comp[(size_t)s] = id;
-    for (size_t head = 0; head < q.size(); ++head) {
-      int64_t i = q[head];
-      members.back().push_back(i);
-      double offsum = 0.0;
-      for (int64_t j = 0; j < N; ++j) {
-        if (j == i) continue;
-        double v = A[(size_t)(i * N + j)];
-        if (v == 0.0) continue;
-        offsum -= v;
-        if (comp[(size_t)j] == -1) {
-          comp[(size_t)j] = id;
-          q.push_back(j);
-        }
-      }
-      if (A[(size_t)(i * N + i)] - offsum > 1e-12 * A[(size_t)(i * N + i)]) anch = true;
-    }
-    anchored.push_back(anch);
-  }
-  std::vector<int64_t> active;
-  for (size_t id = 0; id < members.size(); ++id) {
-    auto& m = members[id];
-    std::sort(m.begin(), m.end());
-    for (size_t k = anchored[id] ? 0 : 1; k < m.size(); ++k) active.push_back(m[k]);
-  }
-  const int64_t M = (int64_t)active.size();
-  std::vector<double> R((size_t)(M * M), 0.0);
-  for (int64_t i = 0; i < M; ++i)
-    for (int64_t j = 0; j < M; ++j) R[(size_t)(i * M + j)] = A[(size_t)(active[(size_t)i] * N + active[(size_t)j])];
-  if (!cholesky(R, M)) {
-    set_error("CFD-E035: numerics.pressure_preconditioner: multigrid coarse Cholesky lost positive definiteness");
-    return CFD_E035;
-  }
-  // Inverse of the reduced matrix, column by column.
-  std::vector<double> Ginv((size_t)(M * M), 0.0), e((size_t)M);
-  for (int64_t col = 0; col < M; ++col) {
-    std::fill(e.begin(), e.end(), 0.0);
-    e[(size_t)col] = 1.0;
-    for (int64_t i = 0; i < M; ++i) {
-      double v = e[(size_t)i];
-      for (int64_t k = 0; k < i; ++k) v -= R[(size_t)(i * M + k)] * e[(size_t)k];
-      e[(size_t)i] = v / R[(size_t)(i * M + i)];
-    }
-    for (int64_t i = M - 1; i >= 0; --i) {
-      double v = e[(size_t)i];
-      for (int64_t k = i + 1; k < M; ++k) v -= R[(size_t)(k * M + i)] * e[(size_t)k];
-      e[(size_t)i] = v / R[(size_t)(i * M + i)];
-    }
-    for (int64_t i = 0; i < M; ++i) Ginv[(size_t)(i * M + col)] = e[(size_t)i];
-  }
-  std::vector<double> G((size_t)(N * N), 0.0);
-  for (int64_t i = 0; i < M; ++i)
-    for (int64_t j = 0; j < M; ++j) G[(size_t)(active[(size_t)i] * N + active[(size_t)j])] = Ginv[(size_t)(i * M + j)];
-  // Symmetric gauge: G <- C G C with C the per-closed-component centering projector.
-  for (size_t id = 0; id < members.size(); ++id) {
-    if (anchored[id]) continue;
-    const auto& m = members[id];
-    double inv = 1.0 / (double)m.size();
-    for (int pass = 0; pass < 2; ++pass) {
-      // pass 0: rows (C G), pass 1: columns (G C)
-      for (int64_t other = 0; other < N; ++other) {
-        double mean = 0.0;
-        for (int64_t i : m) mean += pass == 0 ? G[(size_t)(i * N + other)] : G[(size_t)(other * N + i)];
-        mean *= inv;
-        for (int64_t i : m) {
-          if (pass == 0)
-            G[(size_t)(i * N + other)] -= mean;
-          else
-            G[(size_t)(other * N + i)] -= mean;
-        }
-      }
-    }
-  }
-  h.direct_n = N;
-  h.direct = std::move(G);
-  h.coarse = std::move(levels);
-  return 0;
 }
 
 }  // namespace cfdgpu

@@ -99,24 +99,6 @@ void build_ic0_diagonal(HostGrid& g);
 // The factor's dot terms of a grid with fill coupling (called by build_ic0_diagonal).
 void build_ic0_fill(HostGrid& g, const std::vector<double>& raw, double maxdiag);
 
-// Geometric hierarchy: 2x2x2 bins of the structured grid with Galerkin
-// piecewise-constant coarse operators (7-point with per-face weights).
-struct HostLevel {
-  int64_t n[3];
-  int64_t count;
-  // coupling of cell c to its + neighbour on axis a (wrapping on periodic axes)
-  std::vector<double> wp[3];
-  std::vector<double> anchor;
-  std::vector<double> diag;
-};
-struct HostHierarchy {
-  std::vector<HostLevel> coarse;  // levels 1.. (level 0 is the fine grid itself)
-  // Coarsest level: dense symmetric pseudo-inverse with per-closed-component gauge centering.
-  int64_t direct_n = 0;
-  std::vector<double> direct;  // direct_n x direct_n, row-major
-  int64_t coarsest_dims[3];
-};
-int build_hierarchy(const HostGrid& g, HostHierarchy& h, int64_t coarse_max);
 
 // Aggregation hierarchy (multigrid/masked.rs): level 0 = fluid cells in index
 // order, coarser levels = connected groups of 2x2x2 bins of coordinates.

@@ -1,7 +1,7 @@
 // CDNA4 (gfx950) kernels of the pressure solve: the masked 7-point operator,
 // fused PCG vector updates with deterministic wave64 reductions, segmented
-// gauge means, IC(0) hyperplane sweeps, the geometric multigrid V-cycle and
-// the projection steps either side of the solve.
+// gauge means, IC(0) hyperplane sweeps, the multigrid V-cycles and the
+// projection steps either side of the solve.
 //
 // Built with -ffp-contract=off: the FP64 operator and IC(0) sweeps repeat the
 // reference's per-cell arithmetic (pressure.rs:223-241, 273-285) operation for
@@ -1037,7 +1037,7 @@ __global__ void __launch_bounds__(TX * TY * TZ)
   if (live) out[c] = (TOUT)v;
 }
 
-// ---------------------------------------------------------------- multigrid (geometric)
+// ---------------------------------------------------------------- fine level in the natural layout
 // Colour of a fine cell: its parity, and with `seams` (a bit per periodic axis
 // of odd length) two more classes per seam for the cells on the last slab of
 // that axis, whose wrap neighbour has the same parity.
@@ -1184,476 +1184,11 @@ __device__ __forceinline__ T fine_residual(const Geo& g, const uint8_t* __restri
 // Coefficients and neighbour values of a coarse cell, loaded together: the
 // coupling to the + neighbour (wp[a][c]) and to the - neighbour (wp[a][lo]) per
 // axis, zero where the axis ends.
-template <class T>
-struct CoarseStencil {
-  T d, bv, xv;
-  T wh[3], wl[3], xh[3], xl[3];
-};
-template <class T>
-__device__ __forceinline__ CoarseStencil<T> coarse_stencil(const DevLevel<T>& L, long long c, const int q[3], bool need_x,
-                                                           const T* __restrict__ xc, const Geo& gc, T cscale) {
-  const Geo& g = L.g;
-  CoarseStencil<T> S;
-  S.d = L.diag[c];
-  S.bv = L.b[c];
-  S.xv = need_x ? L.x[c] : T(0);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int n = g.n[a];
-    const bool has_hi = q[a] < n - 1 || g.per[a];
-    const bool has_lo = q[a] > 0 || g.per[a];
-    const long long hn = nb_or_self(g, c, a, q[a], 1);
-    const long long ln = nb_or_self(g, c, a, q[a], 0);
-    const T wh = L.wp[a][c];
-    const T wl = L.wp[a][ln];
-    S.wh[a] = has_hi ? wh : T(0);
-    S.wl[a] = has_lo ? wl : T(0);
-    T xh = T(0), xl = T(0);
-    if (need_x) {
-      xh = L.x[hn];
-      xl = L.x[ln];
-      if (xc) {
-        int qh[3] = {q[0], q[1], q[2]}, ql[3] = {q[0], q[1], q[2]};
-        qh[a] = q[a] < n - 1 ? q[a] + 1 : 0;
-        ql[a] = q[a] > 0 ? q[a] - 1 : n - 1;
-        xh += cscale * xc[(qh[0] >> 1) + (qh[1] >> 1) * gc.sy + (qh[2] >> 1) * gc.sz];
-        xl += cscale * xc[(ql[0] >> 1) + (ql[1] >> 1) * gc.sy + (ql[2] >> 1) * gc.sz];
-      }
-    }
-    S.xh[a] = xh;
-    S.xl[a] = xl;
-  }
-  return S;
-}
-
-// Coarse-level residual of one cell from the explicit face weights.
-template <class T>
-__device__ __forceinline__ T coarse_residual(const DevLevel<T>& L, const int q[3]) {
-  const Geo& g = L.g;
-  const long long c = (long long)q[0] + q[1] * g.sy + q[2] * g.sz;
-  const CoarseStencil<T> S = coarse_stencil<T>(L, c, q, true, nullptr, g, T(0));
-  if (!(S.d > T(0))) return T(0);
-  T off = T(0);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (S.wh[a] != T(0)) off += S.wh[a] * S.xh[a];
-    if (S.wl[a] != T(0)) off += S.wl[a] * S.xl[a];
-  }
-  return S.bv - (S.d * S.xv - off);
-}
-
-// One Gauss-Seidel update of a coarse cell; `xc` fuses the prolongation of the
-// next level's correction into the neighbour reads. Stores only while `run`.
-template <class T>
-__device__ __forceinline__ void coarse_gs_cell(const DevLevel<T>& L, int i, int j, int k, int first,
-                                               const T* __restrict__ xc, const Geo& gc, T cscale, bool run = true) {
-  const Geo& g = L.g;
-  const long long c = (long long)i + j * g.sy + k * g.sz;
-  const int q[3] = {i, j, k};
-  const CoarseStencil<T> S = coarse_stencil<T>(L, c, q, !first, xc, gc, cscale);
-  if (!run) return;
-  if (!(S.d > T(0))) {
-    L.x[c] = T(0);
-    return;
-  }
-  T value = S.bv;
-  if (!first) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      if (S.wh[a] != T(0)) value += S.wh[a] * S.xh[a];
-      if (S.wl[a] != T(0)) value += S.wl[a] * S.xl[a];
-    }
-  }
-  L.x[c] = value / S.d;
-}
-
-// Cell (i, j, k) of colour `color` for the u-th thread of a half-sweep; false past the end.
-__device__ __forceinline__ bool color_cell(const Geo& g, unsigned u, int color, int& i, int& j, int& k) {
-  const unsigned hx = (unsigned)(g.n[0] + 1) >> 1;
-  const unsigned row = u / hx;
-  if (row >= (unsigned)g.n[1] * (unsigned)g.n[2]) return false;
-  const int i2 = (int)(u - row * hx);
-  const unsigned kk = row / (unsigned)g.n[1];
-  j = (int)(row - kk * (unsigned)g.n[1]);
-  k = (int)kk;
-  i = 2 * i2 + ((j + k + color) & 1);
-  return i < g.n[0];
-}
-
-// Restriction P^T r: eight lanes per coarse cell each form one fine residual of
-// the 2x2x2 bin, and three lane exchanges sum them.
-template <class T, class TB>
-__global__ void __launch_bounds__(BS) k_fine_restrict(Geo g, const uint8_t* __restrict__ codes, const TB* __restrict__ b,
-                                                      const T* __restrict__ x, Geo gc, T* __restrict__ bc,
-                                                      const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;
-  const unsigned C = t >> 3;
-  const int sub = (int)(t & 7);
-  T r = T(0);
-  if (C < (unsigned)gc.N) {
-    int Q[3];
-    coords(gc, C, Q);
-    const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
-    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) r = fine_residual<T, TB>(g, codes, b, x, q);
-  }
-  r += __shfl_xor(r, 1, 64);
-  r += __shfl_xor(r, 2, 64);
-  r += __shfl_xor(r, 4, 64);
-  if (C < (unsigned)gc.N && sub == 0) bc[C] = r;
-}
-
-template <class T>
-__global__ void __launch_bounds__(BS) k_coarse_restrict(DevLevel<T> L, Geo gc, T* __restrict__ bc, const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
-  const Geo& g = L.g;
-  const unsigned t = blockIdx.x * BS + threadIdx.x;
-  const unsigned C = t >> 3;
-  const int sub = (int)(t & 7);
-  T r = T(0);
-  if (C < (unsigned)gc.N) {
-    int Q[3];
-    coords(gc, C, Q);
-    const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
-    if (q[0] < g.n[0] && q[1] < g.n[1] && q[2] < g.n[2]) r = coarse_residual<T>(L, q);
-  }
-  r += __shfl_xor(r, 1, 64);
-  r += __shfl_xor(r, 2, 64);
-  r += __shfl_xor(r, 4, 64);
-  if (C < (unsigned)gc.N && sub == 0) bc[C] = r;
-}
-
-// Coarse-level red-black half-sweep with explicit per-face weights.
-template <class T>
-__global__ void __launch_bounds__(BS) k_coarse_gs(DevLevel<T> L, int color, int first, const T* __restrict__ xc, Geo gc,
-                                                  const PcgState* st, T cscale) {
-  const int status = st ? st->status : ST_RUNNING;
-  int i, j, k;
-  if (!color_cell(L.g, blockIdx.x * BS + threadIdx.x, color, i, j, k)) return;
-  coarse_gs_cell<T>(L, i, j, k, first, xc, gc, cscale, status == ST_RUNNING);
-}
-
-// The small coarse levels of a V-cycle in one workgroup: smoothing,
-// restriction, the dense coarsest solve and the corrections, separated by
-// workgroup barriers instead of kernel launches.
-constexpr int TB_THREADS = 1024;
-template <class T>
-__global__ void __launch_bounds__(TB_THREADS)
-    k_tail(const DevLevel<T>* __restrict__ lv, int first, int nlev, int direct_n, const double* __restrict__ G,
-           const PcgState* st, T cscale) {
-  if (st && !block_running(st)) return;
-  const unsigned tid = threadIdx.x;
-  auto sweep = [&](int l, int color, int first_sweep, const T* xc) {
-    const DevLevel<T> L = lv[l];
-    Geo gc;
-    if (xc) gc = lv[l + 1].g;
-    const unsigned n = (unsigned)((L.g.n[0] + 1) >> 1) * (unsigned)L.g.n[1] * (unsigned)L.g.n[2];
-    for (unsigned u = tid; u < n; u += TB_THREADS) {
-      int i, j, k;
-      if (color_cell(L.g, u, color, i, j, k)) coarse_gs_cell<T>(L, i, j, k, first_sweep, xc, gc, cscale);
-    }
-    __syncthreads();
-  };
-  for (int l = first; l + 1 < nlev; ++l) {
-    sweep(l, 0, 1, nullptr);
-    sweep(l, 1, 0, nullptr);
-    sweep(l, 0, 0, nullptr);
-    sweep(l, 1, 0, nullptr);
-    const DevLevel<T> L = lv[l];
-    const DevLevel<T> N = lv[l + 1];
-    for (unsigned C = tid; C < (unsigned)N.g.N; C += TB_THREADS) {
-      int Q[3];
-      coords(N.g, C, Q);
-      T sum = T(0);
-      for (int sub = 0; sub < 8; ++sub) {
-        const int q[3] = {2 * Q[0] + (sub & 1), 2 * Q[1] + ((sub >> 1) & 1), 2 * Q[2] + (sub >> 2)};
-        if (q[0] < L.g.n[0] && q[1] < L.g.n[1] && q[2] < L.g.n[2]) sum += coarse_residual<T>(L, q);
-      }
-      N.b[C] = sum;
-    }
-    __syncthreads();
-  }
-  {
-    const DevLevel<T> D = lv[nlev - 1];
-    const int wave = (int)(tid >> 6), lane = (int)(tid & 63);
-    for (int row = wave; row < direct_n; row += TB_THREADS / 64) {
-      double acc = 0.0;
-      for (int j = lane; j < direct_n; j += 64) acc += G[(long long)row * direct_n + j] * (double)D.b[j];
-      acc = wave_sum(acc);
-      if (lane == 0) D.x[row] = (T)acc;
-    }
-    __syncthreads();
-  }
-  for (int l = nlev - 2; l >= first; --l) {
-    sweep(l, 1, 0, lv[l + 1].x);
-    sweep(l, 0, 0, nullptr);
-    sweep(l, 1, 0, nullptr);
-    sweep(l, 0, 0, nullptr);
-  }
-}
-
-// ---------------------------------------------------------------- fused tile smoother
-// Temporal blocking of a whole smoothing phase: one workgroup owns a tile of
-// TX x TY x TZ cells and loads the tile plus a halo of H cells into LDS. Half-sweep
-// s updates the cells of its colour at distance >= s from the region's edge, so
-// after four half-sweeps the tile holds exactly the values four separate global
-// half-sweeps would give (halo cells are recomputed redundantly by neighbouring
-// tiles, with the same arithmetic, hence bitwise the same numbers).
-//   PRE : x = 0, half-sweeps R B R B, then the tile's residual summed over its
-//         2x2x2 bins into the next level's right-hand side (H = 5);
-//   POST: x += omega P e over the region, half-sweeps B R B R (H = 4).
-// The tile's result goes to `xout`, never to the `x` other tiles read their halo
-// from (tiles that are not resident together would otherwise see finished values).
-// Within a half-sweep every update is computed from the values before the
-// half-sweep and written after a barrier (Jacobi within a colour), which is
-// Gauss-Seidel for a proper two-colouring and stays deterministic across the
-// seam of an odd periodic axis.
-constexpr int SM_THREADS = 1024;
-
 // Tile index for workgroup b of nb: each XCD (blocks b = x mod 8) gets a
 // contiguous range of tiles, so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int q = nb >> 3, r = nb & 7, x = b & 7;
   return x * q + (x < r ? x : r) + (b >> 3);
-}
-
-__device__ __forceinline__ bool wrap_coord(int& v, int n, int per) {
-  if (v >= 0 && v < n) return true;
-  if (!per) return false;
-  v %= n;
-  if (v < 0) v += n;
-  return true;
-}
-
-template <class T, class TB, bool FINE, bool PRE, int TX, int TY, int TZ>
-__global__ void __launch_bounds__(SM_THREADS)
-    k_smooth(Geo g, const uint8_t* __restrict__ codes, DevLevel<T> L, const TB* __restrict__ b, const T* __restrict__ x,
-             T* __restrict__ xout, Geo gc, T* __restrict__ bc, const T* __restrict__ xc, T cscale, double* __restrict__ out64,
-             const PcgState* st) {
-  constexpr int H = PRE ? 5 : 4;
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H, RZ = TZ + 2 * H, RN = RX * RY * RZ;
-  constexpr int SX = 1, SY = RX, SZ = RX * RY;
-  constexpr int PER = (RN + SM_THREADS - 1) / SM_THREADS;  // region cells owned by one thread
-  __shared__ T xs[RN];
-  __shared__ T rs[PRE ? TX * TY * TZ : 1];
-  const int ntx = (g.n[0] + TX - 1) / TX, nty = (g.n[1] + TY - 1) / TY, ntz = (g.n[2] + TZ - 1) / TZ;
-  const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
-  const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-  const int ox = tx * TX - H, oy = ty * TY - H, oz = tz * TZ - H;
-  const int lstride[3] = {SX, SY, SZ};
-  // Thread t owns region cells t, t + SM_THREADS, ...: their coordinates, colour,
-  // margin (distance to the region's edge) and every coefficient they need are
-  // loaded once, up front, in one batch of independent loads.
-  int own_c[PER];       // global index, -1 outside the domain or the region
-  uint8_t own_meta[PER];  // bit 0 colour, bits 1-3 margin (capped at 7), bit 4 fluid / active
-  uint8_t own_code[PER];
-  T own_b[PER], own_d[PER], own_wh[PER][3], own_wl[PER][3];
-  int own_q[PER][3];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int l = threadIdx.x + k * SM_THREADS;
-    own_c[k] = -1;
-    own_meta[k] = 0;
-    own_code[k] = 0;
-    own_b[k] = T(0);
-    own_d[k] = T(0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      own_wh[k][a] = T(0);
-      own_wl[k][a] = T(0);
-      own_q[k][a] = 0;
-    }
-    T v = T(0);
-    if (l < RN) {
-      const int li = l % RX, lj = (l / RX) % RY, lk = l / (RX * RY);
-      int q[3] = {ox + li, oy + lj, oz + lk};
-      const bool inside = wrap_coord(q[0], g.n[0], g.per[0]) && wrap_coord(q[1], g.n[1], g.per[1]) &&
-                          wrap_coord(q[2], g.n[2], g.per[2]);
-      if (inside) {
-        const int c = q[0] + q[1] * (int)g.sy + q[2] * (int)g.sz;
-        int m = li < RX - 1 - li ? li : RX - 1 - li;
-        m = min(m, min(lj, RY - 1 - lj));
-        m = min(m, min(lk, RZ - 1 - lk));
-        own_c[k] = c;
-        own_meta[k] = (uint8_t)(((q[0] + q[1] + q[2]) & 1) | (min(m, 7) << 1));
-#pragma unroll
-        for (int a = 0; a < 3; ++a) own_q[k][a] = q[a];
-        if constexpr (!PRE) {
-          v = x[c];
-          if (xc) v += cscale * xc[(q[0] >> 1) + (q[1] >> 1) * gc.sy + (q[2] >> 1) * gc.sz];
-        }
-        if (m >= 1) {
-          if constexpr (FINE) {
-            own_code[k] = codes[c];
-            own_b[k] = (T)b[c];
-          } else {
-            own_d[k] = L.diag[c];
-            own_b[k] = L.b[c];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const bool has_hi = q[a] < g.n[a] - 1 || g.per[a];
-              const bool has_lo = q[a] > 0 || g.per[a];
-              const T wh = L.wp[a][c];
-              const T wl = L.wp[a][lo_nb(g, c, a, q[a])];
-              own_wh[k][a] = has_hi ? wh : T(0);
-              own_wl[k][a] = has_lo ? wl : T(0);
-            }
-          }
-        }
-      }
-      xs[l] = v;
-    }
-  }
-  if (st && !block_running(st)) return;
-  __syncthreads();
-  for (int s = 1; s <= 4; ++s) {
-    const int color = PRE ? ((s - 1) & 1) : (s & 1);
-    const bool first = PRE && s == 1;
-    T newv[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      newv[k] = T(0);
-      const int m = own_meta[k] >> 1;
-      if (own_c[k] < 0 || m < s || (own_meta[k] & 1) != color) continue;
-      const int l = threadIdx.x + k * SM_THREADS;
-      T v;
-      if constexpr (FINE) {
-        const uint8_t code = own_code[k];
-        if (!(code & FLUID)) {
-          v = T(0);
-        } else {
-          T diag = T(0);
-          T value = own_b[k];
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-              if (!(code & (1u << (2 * a + side)))) continue;
-              const bool boundary = side == 0 ? own_q[k][a] == 0 : own_q[k][a] == g.n[a] - 1;
-              if (boundary && !g.per[a]) {
-                diag += (T)g.wo[a];
-                continue;
-              }
-              const T w = (T)g.wi[a];
-              diag += w;
-              if (first) continue;
-              value += w * xs[side == 0 ? l - lstride[a] : l + lstride[a]];
-            }
-          v = diag > T(0) ? value / diag : T(0);
-        }
-      } else {
-        const T d = own_d[k];
-        if (!(d > T(0))) {
-          v = T(0);
-        } else {
-          T value = own_b[k];
-          if (!first) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              if (own_wh[k][a] != T(0)) value += own_wh[k][a] * xs[l + lstride[a]];
-              if (own_wl[k][a] != T(0)) value += own_wl[k][a] * xs[l - lstride[a]];
-            }
-          }
-          v = value / d;
-        }
-      }
-      newv[k] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int m = own_meta[k] >> 1;
-      if (own_c[k] < 0 || m < s || (own_meta[k] & 1) != color) continue;
-      xs[threadIdx.x + k * SM_THREADS] = newv[k];
-    }
-    __syncthreads();
-  }
-  // the tile's final values, and for PRE the residual of each tile cell
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int l = threadIdx.x + k * SM_THREADS;
-    if (l >= RN) continue;
-    const int li = l % RX, lj = (l / RX) % RY, lk = l / (RX * RY);
-    if (li < H || li >= H + TX || lj < H || lj >= H + TY || lk < H || lk >= H + TZ) continue;
-    const int c = own_c[k];
-    const int u = (li - H) + (lj - H) * TX + (lk - H) * TX * TY;
-    const bool in_domain = ox + li < g.n[0] && oy + lj < g.n[1] && oz + lk < g.n[2] && c >= 0;
-    if (in_domain) {
-      xout[c] = xs[l];
-      if (out64) out64[c] = (double)xs[l];
-    }
-    if constexpr (PRE) {
-      T r = T(0);
-      if (in_domain) {
-        if constexpr (FINE) {
-          const uint8_t code = own_code[k];
-          if (code & FLUID) {
-            const T xv = xs[l];
-            T ax = T(0), off = T(0);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-              for (int side = 0; side < 2; ++side) {
-                if (!(code & (1u << (2 * a + side)))) continue;
-                const bool boundary = side == 0 ? own_q[k][a] == 0 : own_q[k][a] == g.n[a] - 1;
-                if (boundary && !g.per[a]) {
-                  ax += (T)g.wo[a];
-                  continue;
-                }
-                ax += (T)g.wi[a];
-                off += (T)g.wi[a] * xs[side == 0 ? l - lstride[a] : l + lstride[a]];
-              }
-            r = own_b[k] - (ax * xv - off);
-          }
-        } else {
-          const T d = own_d[k];
-          if (d > T(0)) {
-            T off = T(0);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              if (own_wh[k][a] != T(0)) off += own_wh[k][a] * xs[l + lstride[a]];
-              if (own_wl[k][a] != T(0)) off += own_wl[k][a] * xs[l - lstride[a]];
-            }
-            r = own_b[k] - (d * xs[l] - off);
-          }
-        }
-      }
-      rs[u] = r;
-    }
-  }
-  if constexpr (PRE) {
-    // P^T r over the tile's 2x2x2 bins, in the association order of the restriction kernels
-    __syncthreads();
-    constexpr int CX = TX / 2, CY = TY / 2, CZ = TZ / 2;
-    for (int u = threadIdx.x; u < CX * CY * CZ; u += SM_THREADS) {
-      const int ci = u % CX, cj = (u / CX) % CY, ck = u / (CX * CY);
-      const int I = tx * CX + ci, J = ty * CY + cj, K = tz * CZ + ck;
-      if (I >= gc.n[0] || J >= gc.n[1] || K >= gc.n[2]) continue;
-      T v[8];
-#pragma unroll
-      for (int sub = 0; sub < 8; ++sub)
-        v[sub] = rs[(2 * ci + (sub & 1)) + (2 * cj + ((sub >> 1) & 1)) * TX + (2 * ck + (sub >> 2)) * TX * TY];
-      const T sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-      bc[I + J * gc.sy + K * gc.sz] = sum;
-    }
-  }
-}
-
-// Coarsest level: x = G b with the dense symmetric pseudo-inverse (FP64), one wave per row.
-template <class TI, class TO>
-__global__ void __launch_bounds__(BS) k_direct(int n, const double* __restrict__ G, const TI* __restrict__ b,
-                                               TO* __restrict__ x, double* __restrict__ out64, const PcgState* st) {
-  if (st && st->status != ST_RUNNING) return;
-  int row = blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
-  int lane = threadIdx.x & 63;
-  double acc = 0.0;
-  if (row < n)
-    for (int j = lane; j < n; j += 64) acc += G[(long long)row * n + j] * (double)b[j];
-  acc = wave_sum(acc);
-  if (row < n && lane == 0) {
-    x[row] = (TO)acc;
-    if (out64) out64[row] = (double)(TO)acc;
-  }
 }
 
 // ---------------------------------------------------------------- multigrid (aggregation)
@@ -3910,126 +3445,12 @@ static unsigned color_blocks(const Geo& g) {
 }
 
 template <class T, class TB>
-void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv,
-                   const DevLevel<T>* dlv, int tail_first, int direct_n, const double* G, double* out64,
-                   const PcgState* st, double omega, hipStream_t s) {
-  const T cs = (T)omega;
-  // Level 0 is the fine grid; lv[0..nlev) are the coarse levels, lv[nlev-1] solved
-  // directly; levels from tail_first on run inside one workgroup (k_tail).
-  const unsigned fb = color_blocks<T>(g0);
-  if (nlev == 0) {
-    // No coarse level: the fine grid itself is tiny and solved directly.
-    hipLaunchKernelGGL((k_direct<TB, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, b0, x0, out64, st);
-    return;
-  }
-  const Geo& g1 = lv[0].g;
-  const T* none = nullptr;
-  for (int pass = 0; pass < 4; ++pass)
-    hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0, none,
-                       g1, (double*)nullptr, 0, st, cs);
-  hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(8 * g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0,
-                     g1, lv[0].b, st);
-  const int stop = nlev - 1;
-  for (int l = 0; l < stop; ++l) {
-    const unsigned cb = color_blocks<T>(lv[l].g);
-    const Geo& gn = lv[l + 1].g;
-    for (int pass = 0; pass < 4; ++pass)
-      hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], pass & 1, pass == 0 ? 1 : 0, none, gn, st, cs);
-    hipLaunchKernelGGL((k_coarse_restrict<T>), dim3(blocks_for(8 * gn.N)), dim3(BS), 0, s, lv[l], gn, lv[l + 1].b, st);
-  }
-  if (tail_first < nlev) {
-    hipLaunchKernelGGL((k_tail<T>), dim3(1), dim3(TB_THREADS), 0, s, dlv, tail_first, nlev, direct_n, G, st, cs);
-  } else {
-    hipLaunchKernelGGL((k_direct<T, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G,
-                       (const T*)lv[nlev - 1].b, lv[nlev - 1].x, (double*)nullptr, st);
-  }
-  for (int l = stop - 1; l >= 0; --l) {
-    const unsigned cb = color_blocks<T>(lv[l].g);
-    const Geo& gn = lv[l + 1].g;
-    for (int pass = 0; pass < 4; ++pass)
-      hipLaunchKernelGGL((k_coarse_gs<T>), dim3(cb), dim3(BS), 0, s, lv[l], (pass + 1) & 1, 0,
-                         pass == 0 ? (const T*)lv[l + 1].x : none, gn, st, cs);
-  }
-  // post-smoothing on the fine level; in mixed precision the last two half-sweeps also write z in FP64
-  for (int pass = 0; pass < 4; ++pass)
-    hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
-                       pass == 0 ? (const T*)lv[0].x : none, g1, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs);
-}
-
-template <class T, class TB, bool FINE, bool PRE, int TX, int TY, int TZ>
-static void smooth(const Geo& g, const uint8_t* codes, const DevLevel<T>& L, const TB* b, const T* x, T* xout,
-                   const Geo& gc, T* bc, const T* xc, T cs, double* out64, const PcgState* st, hipStream_t s) {
-  const unsigned nt = (unsigned)(((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * ((g.n[2] + TZ - 1) / TZ));
-  hipLaunchKernelGGL((k_smooth<T, TB, FINE, PRE, TX, TY, TZ>), dim3(nt), dim3(SM_THREADS), 0, s, g, codes, L, b, x, xout,
-                     gc, bc, xc, cs, out64, st);
-}
-
-// The same V-cycle with one fused kernel per level and phase (pre-smoothing +
-// restriction, prolongation + post-smoothing); bitwise equal to launch_vcycle.
-// The fine pre-smoothed iterate goes to `xpre`; each coarse level keeps its
-// pre-smoothed iterate in x and its post-smoothed one in xo. With fuse_fine
-// false the fine level keeps its streaming per-half-sweep kernels (they move
-// the fewest bytes) and only the latency-bound coarse levels are fused.
-template <class T, class TB>
-void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, T* xpre, int nlev, DevLevel<T>* lv,
-                         int direct_n, const double* G, double* out64, const PcgState* st, double omega, bool fuse_fine,
-                         hipStream_t s) {
-  const T cs = (T)omega;
-  if (nlev == 0) {
-    hipLaunchKernelGGL((k_direct<TB, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, b0, x0, out64, st);
-    return;
-  }
-  DevLevel<T> none{};
-  constexpr int FX = 16, FY = 8, FZ = 4;
-  const Geo& g1 = lv[0].g;
-  const T* nothing = nullptr;
-  const unsigned fb = color_blocks<T>(g0);
-  if (fuse_fine) {
-    smooth<T, TB, true, true, FX, FY, FZ>(g0, codes, none, b0, nullptr, xpre, g1, lv[0].b, nullptr, cs, nullptr, st, s);
-  } else {
-    for (int pass = 0; pass < 4; ++pass)
-      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, pass & 1, pass == 0 ? 1 : 0,
-                         nothing, g1, (double*)nullptr, 0, st, cs);
-    hipLaunchKernelGGL((k_fine_restrict<T, TB>), dim3(blocks_for(8 * g1.N)), dim3(BS), 0, s, g0, codes, b0, (const T*)x0,
-                       g1, lv[0].b, st);
-  }
-  for (int l = 0; l + 1 < nlev; ++l)
-    smooth<T, T, false, true, 8, 8, 4>(lv[l].g, nullptr, lv[l], lv[l].b, nullptr, lv[l].x, lv[l + 1].g, lv[l + 1].b,
-                                       nullptr, cs, nullptr, st, s);
-  hipLaunchKernelGGL((k_direct<T, T>), dim3((direct_n + 3) / 4), dim3(BS), 0, s, direct_n, G, (const T*)lv[nlev - 1].b,
-                     lv[nlev - 1].x, (double*)nullptr, st);
-  for (int l = nlev - 2; l >= 0; --l)
-    smooth<T, T, false, false, 8, 8, 8>(lv[l].g, nullptr, lv[l], lv[l].b, lv[l].x, lv[l].xo, lv[l + 1].g, nullptr,
-                                        lv[l + 1].xo, cs, nullptr, st, s);
-  if (fuse_fine) {
-    smooth<T, TB, true, false, FX, FY, FZ>(g0, codes, none, b0, xpre, x0, g1, nullptr, lv[0].xo, cs, out64, st, s);
-  } else {
-    for (int pass = 0; pass < 4; ++pass)
-      hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(fb), dim3(BS), 0, s, g0, codes, b0, x0, (pass + 1) & 1, 0,
-                         pass == 0 ? (const T*)lv[0].xo : nothing, g1, pass >= 2 ? out64 : (double*)nullptr, 0, st, cs);
-  }
-}
-template void launch_vcycle_fused<double, double>(const Geo&, const uint8_t*, const double*, double*, double*, int,
-                                                  DevLevel<double>*, int, const double*, double*, const PcgState*, double,
-                                                  bool, hipStream_t);
-template void launch_vcycle_fused<float, double>(const Geo&, const uint8_t*, const double*, float*, float*, int,
-                                                 DevLevel<float>*, int, const double*, double*, const PcgState*, double,
-                                                 bool, hipStream_t);
-
-template <class T, class TB>
 void launch_fine_gs(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int color, const Geo& g1, hipStream_t s) {
   hipLaunchKernelGGL((k_fine_gs<T, TB>), dim3(color_blocks<T>(g0)), dim3(BS), 0, s, g0, codes, b0, x0, color, 0,
                      (const T*)nullptr, g1, (double*)nullptr, 0, (const PcgState*)nullptr);
 }
 template void launch_fine_gs<double, double>(const Geo&, const uint8_t*, const double*, double*, int, const Geo&, hipStream_t);
 template void launch_fine_gs<float, double>(const Geo&, const uint8_t*, const double*, float*, int, const Geo&, hipStream_t);
-
-template void launch_vcycle<double, double>(const Geo&, const uint8_t*, const double*, double*, int, DevLevel<double>*,
-                                            const DevLevel<double>*, int, int, const double*, double*, const PcgState*,
-                                            double, hipStream_t);
-template void launch_vcycle<float, double>(const Geo&, const uint8_t*, const double*, float*, int, DevLevel<float>*,
-                                           const DevLevel<float>*, int, int, const double*, double*, const PcgState*,
-                                           double, hipStream_t);
 
 // One V(2,2) cycle on the aggregation hierarchy: the fine level from the
 // coupling codes (red-black, or 2^(1+seams) colour classes across odd periodic

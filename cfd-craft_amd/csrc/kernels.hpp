@@ -136,16 +136,6 @@ void launch_ic0_tiles(const Geo& g, const uint8_t* codes, const double* D, const
 // written in FP64.
 void launch_ic0_tiles_mixed(const Geo& g, const uint8_t* codes, const double* D, const double* r, float* y, double* z,
                             const PcgState* st, hipStream_t s, const double* fill = nullptr);
-// One V-cycle: fine level from the codes, coarse levels lv[0..nlev) (device copy dlv),
-// levels from tail_first on inside one workgroup.
-template <class T, class TB>
-void launch_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, int nlev, DevLevel<T>* lv,
-                   const DevLevel<T>* dlv, int tail_first, int direct_n, const double* G, double* out64,
-                   const PcgState* st, double omega, hipStream_t s);
-template <class T, class TB>
-void launch_vcycle_fused(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, T* xpre, int nlev, DevLevel<T>* lv,
-                         int direct_n, const double* G, double* out64, const PcgState* st, double omega, bool fuse_fine,
-                         hipStream_t s);
 template <class T, class TB>
 void launch_agg_vcycle(const Geo& g0, const uint8_t* codes, const TB* b0, T* x0, const int* fparent, int seams,
                        int nlev, const AggDevLevel<T>* lv, const AggCoarsest& D, double* out64, const PcgState* st,

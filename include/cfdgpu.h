@@ -123,7 +123,7 @@ typedef struct {
 } cfd_block_smoother_stats;
 
 /* MultigridStats (multigrid.rs:55-72). level_dims: cell dimensions per level of
- * the structured hierarchies (trilinear Galerkin, geometric), zero for the
+ * the structured hierarchy (trilinear Galerkin), zero for the
  * aggregation hierarchy, whose levels are irregular (masked.rs:262). */
 typedef struct {
   int32_t levels;
@@ -154,13 +154,12 @@ enum { CFD_PRECISION_FP64 = 0, CFD_PRECISION_MIXED = 1 };
  *                       (the throughput path, FP64 or mixed);
  *  CFD_CYCLE_REFERENCE  the reference's own cycle: its hierarchy and lexicographic
  *                       Gauss-Seidel (multigrid.rs:667-726, masked.rs:304-342), FP64;
- *  CFD_CYCLE_GEOMETRIC  structured 2x2x2 bins without the connectivity split;
  *  CFD_CYCLE_ZEBRA      the reference's hierarchy (with its line and small-plane
  *                       block smoothers and semicoarsening on anisotropic grids)
  *                       swept by colour classes instead of lexicographically: a
  *                       few launches per sweep; for anisotropic solid-free grids,
  *                       where point smoothers stall. FP64. */
-enum { CFD_CYCLE_RB = 0, CFD_CYCLE_REFERENCE = 1, CFD_CYCLE_GEOMETRIC = 2, CFD_CYCLE_ZEBRA = 3 };
+enum { CFD_CYCLE_RB = 0, CFD_CYCLE_REFERENCE = 1, CFD_CYCLE_ZEBRA = 3 };  /* 2: not a cycle */
 typedef struct {
   int32_t precision;   /* FP64 (plain) or FP32 multigrid/IC(0)-apply inside FP64 PCG */
   int32_t warm_start;  /* 1: start PCG from the grid's resident previous solution */

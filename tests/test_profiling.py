@@ -224,7 +224,7 @@ def test_device_reference_cycle_work_equals_oracle(gpu_available, spec_name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("cycle", [0, 1, 2], ids=["rb", "reference", "geometric"])
+@pytest.mark.parametrize("cycle", [0, 1], ids=["rb", "reference"])
 def test_device_profile_reuse_and_projection(gpu_available, cycle):
     """tests/multigrid.rs:175-208 on an unchanged grid: one build over several solves,
     setup calls equal builds, work follows the hierarchy; a projection nests its solve."""
