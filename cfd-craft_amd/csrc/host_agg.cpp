@@ -94,33 +94,61 @@ void for_each_face(const HostGrid& g, E edge, O outlet, bool self_loops = false)
 // Connected components of a level by breadth-first search from the lowest
 // unvisited node (masked.rs:90-126); members are stored in ascending order.
 void level_components(const AggLevel& L, std::vector<std::vector<int32_t>>& comps, std::vector<uint8_t>& anchored) {
-  std::vector<int32_t> comp((size_t)L.n, -1);
-  std::vector<int64_t> size;
-  std::vector<int32_t> queue;
-  for (int64_t s = 0; s < L.n; ++s) {
-    if (comp[(size_t)s] >= 0) continue;
-    const int32_t id = (int32_t)size.size();
-    comp[(size_t)s] = id;
-    queue.assign(1, (int32_t)s);
-    bool fixed = false;
-    for (size_t head = 0; head < queue.size(); ++head) {
-      const int32_t i = queue[head];
-      fixed |= L.anchor[(size_t)i] > 0.0;
+  // Components by union-find over threads: a root is only ever linked under a
+  // smaller root, so each component ends rooted at its smallest node, and the
+  // components numbered by that node in ascending order are the ones a
+  // breadth-first search from each unvisited node in order would number.
+  const int64_t n = L.n;
+  std::vector<std::atomic<int32_t>> up((size_t)n);
+  parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) up[(size_t)i].store((int32_t)i, std::memory_order_relaxed);
+  });
+  auto find = [&](int32_t x) {
+    for (;;) {
+      int32_t p = up[(size_t)x].load(std::memory_order_relaxed);
+      if (p == x) return x;
+      const int32_t q = up[(size_t)p].load(std::memory_order_relaxed);
+      if (q != p) up[(size_t)x].compare_exchange_weak(p, q, std::memory_order_relaxed);
+      x = q;
+    }
+  };
+  parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i)
       for (int64_t e = L.ptr[(size_t)i]; e < L.ptr[(size_t)i + 1]; ++e) {
-        const int32_t j = L.col[(size_t)e];
-        if (comp[(size_t)j] < 0) {
-          comp[(size_t)j] = id;
-          queue.push_back(j);
+        int32_t a = (int32_t)i, b = L.col[(size_t)e];
+        if (b <= a) continue;  // each coupling once
+        for (;;) {
+          a = find(a);
+          b = find(b);
+          if (a == b) break;
+          if (a > b) std::swap(a, b);
+          int32_t expect = b;
+          if (up[(size_t)b].compare_exchange_strong(expect, a, std::memory_order_relaxed)) break;
         }
       }
+  });
+  std::vector<int32_t> comp((size_t)n);
+  parallel_ranges(n, [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) comp[(size_t)i] = find((int32_t)i);
+  });
+  // number the roots in ascending order; members ascending
+  std::vector<int32_t> id((size_t)n, -1);
+  std::vector<int64_t> size;
+  for (int64_t i = 0; i < n; ++i)
+    if (comp[(size_t)i] == (int32_t)i) {
+      id[(size_t)i] = (int32_t)size.size();
+      size.push_back(0);
+      anchored.push_back(0);
     }
-    size.push_back((int64_t)queue.size());
-    anchored.push_back(fixed ? 1 : 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t r = id[(size_t)comp[(size_t)i]];
+    comp[(size_t)i] = r;
+    size[(size_t)r] += 1;
+    if (L.anchor[(size_t)i] > 0.0) anchored[(size_t)r] = 1;
   }
-  // members ascending: the nodes in order, each to its component
   comps.resize(size.size());
   for (size_t r = 0; r < size.size(); ++r) comps[r].reserve((size_t)size[r]);
-  for (int64_t i = 0; i < L.n; ++i) comps[(size_t)comp[(size_t)i]].push_back((int32_t)i);
+  for (int64_t i = 0; i < n; ++i) comps[(size_t)comp[(size_t)i]].push_back((int32_t)i);
 }
 
 // CSR of a level from its edge list (a < b, sorted by (a, b)) and anchors, in
