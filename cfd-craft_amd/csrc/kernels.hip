@@ -3455,8 +3455,7 @@ template void launch_fine_gs<float, double>(const Geo&, const uint8_t*, const do
 // One V(2,2) cycle on the aggregation hierarchy: the fine level from the
 // coupling codes (red-black, or 2^(1+seams) colour classes across odd periodic
 // seams), lv[0..nlev) the coarse levels (lv[nlev-1] solved by the dense
-// blocks), levels from tail_first on inside one workgroup (dlv: device copy of
-// lv). The correction of each level is scaled by omega. resident (colour-
+// blocks). The correction of each level is scaled by omega. resident (colour-
 // separated fine level with two cells per thread): 1 b0 copied into the fine
 // level's b and z left in its x, the last half-sweep forming r . z from them;
 // 2 the same with b already there (b0 unused); 3 (an FP32 level) b0 rounded into
