@@ -163,10 +163,9 @@ void csr_from_edges(AggLevel& L) {
     L.ptr[(size_t)L.eb[e] + 1] += 1;
   }
   for (int64_t i = 0; i < n; ++i) L.ptr[(size_t)i + 1] += L.ptr[(size_t)i];
-  L.col.assign((size_t)L.ptr[(size_t)n], 0);
-  L.val.assign((size_t)L.ptr[(size_t)n], 0.0);
-  std::vector<int64_t> next(L.ptr.begin(), L.ptr.end() - 1);
-  L.diag = L.anchor;
+  std::vector<int64_t> next;
+  concurrently([&] { L.col.assign((size_t)L.ptr[(size_t)n], 0); }, [&] { L.val.assign((size_t)L.ptr[(size_t)n], 0.0); },
+               [&] { next.assign(L.ptr.begin(), L.ptr.end() - 1); }, [&] { L.diag = L.anchor; });
   for (size_t e = 0; e < L.ea.size(); ++e) {
     const int32_t a = L.ea[e], b = L.eb[e];
     const double w = L.ew[e];
