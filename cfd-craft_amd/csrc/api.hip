@@ -324,7 +324,9 @@ int upload_agg_values(cfd_grid* g, DevAgg<T>& da) {
     d.child = sh.child;
     T *val = nullptr, *diag = nullptr, *b = nullptr, *x = nullptr;
     int rc;
-    if ((rc = up(L.val, &val)) || (rc = up(L.diag, &diag))) return rc;
+    // a level in packed rows reads its values from the table (its CSR values are
+    // not read on the device)
+    if ((!sh.ecol && (rc = up(L.val, &val))) || (rc = up(L.diag, &diag))) return rc;
     std::vector<double> zero((size_t)L.n, 0.0);
     if ((rc = up(zero, &b)) || (rc = up(zero, &x))) return rc;
     d.val = val;
