@@ -370,14 +370,15 @@ int build_agg(cfd_grid* g) {
   for (int a = 0; a < 3; ++a)
     if (h.periodic[a] && h.n[a] >= 3 && (h.n[a] & 1)) A.seams |= 1 << a;
   if (A.nlev > 0) {
-    std::vector<int> fp((size_t)h.count);
+    // every entry written below: no zero fill of the host array
+    std::unique_ptr<int[]> fp(new int[(size_t)h.count]);
     parallel_ranges(h.count, [&](int64_t c0, int64_t c1) {
       for (int64_t c = c0; c < c1; ++c) {
         const int32_t node = H.node_of_cell[(size_t)c];
         fp[(size_t)c] = node >= 0 ? H.levels[0].parent[(size_t)node] : -1;
       }
     });
-    if ((rc = A.fparent.upload(fp.data(), fp.size()))) return rc;
+    if ((rc = A.fparent.upload(fp.get(), (size_t)h.count))) return rc;
   }
   auto upi = [&](const int32_t* v, size_t n, const int** out) -> int {
     A.ints.emplace_back(new DevBuf<int>());
@@ -450,7 +451,8 @@ int build_agg(cfd_grid* g) {
           fits = slot(L.val[e]) >= 0;
         }
       }
-      std::vector<int32_t> ec(fits ? (size_t)K * (size_t)L.n : 0);
+      // every entry written below: no zero fill of the host array
+      std::unique_ptr<int32_t[]> ec(new int32_t[fits ? (size_t)K * (size_t)L.n : 1]);
       if (fits)
         parallel_ranges(L.n, [&](int64_t i0, int64_t i1) {
           for (int64_t i = i0; i < i1; ++i)
@@ -469,7 +471,7 @@ int build_agg(cfd_grid* g) {
         });
       if (fits) {
         const int* ep = nullptr;
-        if ((rc = upi(ec.data(), ec.size(), &ep))) return rc;
+        if ((rc = upi(ec.get(), (size_t)K * (size_t)L.n, &ep))) return rc;
         sh.ecol = reinterpret_cast<const unsigned*>(ep);
         sh.K = (int)K;
       } else {
