@@ -203,15 +203,33 @@ void csr_level0(AggLevel& L) {
 
 // Children lists: the finer nodes of each aggregate, ascending.
 void build_children(AggHierarchy& h) {
+  // children of each node ascending: counted and placed over threads (atomic
+  // counters), then each node's few children sorted
   for (size_t l = 1; l < h.levels.size(); ++l) {
     AggLevel& C = h.levels[l];
     const AggLevel& F = h.levels[l - 1];
+    std::vector<std::atomic<int32_t>> cnt((size_t)C.n);
+    parallel_ranges(C.n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) cnt[(size_t)i].store(0, std::memory_order_relaxed);
+    });
+    parallel_ranges(F.n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) cnt[(size_t)F.parent[(size_t)i]].fetch_add(1, std::memory_order_relaxed);
+    });
     C.child_ptr.assign((size_t)C.n + 1, 0);
-    for (int32_t p : F.parent) C.child_ptr[(size_t)p + 1] += 1;
-    for (int64_t i = 0; i < C.n; ++i) C.child_ptr[(size_t)i + 1] += C.child_ptr[(size_t)i];
-    C.child.assign((size_t)F.n, 0);
-    std::vector<int64_t> at(C.child_ptr.begin(), C.child_ptr.end() - 1);
-    for (int64_t i = 0; i < F.n; ++i) C.child[(size_t)at[(size_t)F.parent[(size_t)i]]++] = (int32_t)i;
+    for (int64_t i = 0; i < C.n; ++i)
+      C.child_ptr[(size_t)i + 1] = C.child_ptr[(size_t)i] + cnt[(size_t)i].load(std::memory_order_relaxed);
+    C.child.resize((size_t)F.n);
+    parallel_ranges(C.n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) cnt[(size_t)i].store((int32_t)C.child_ptr[(size_t)i], std::memory_order_relaxed);
+    });
+    parallel_ranges(F.n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        C.child[(size_t)cnt[(size_t)F.parent[(size_t)i]].fetch_add(1, std::memory_order_relaxed)] = (int32_t)i;
+    });
+    parallel_ranges(C.n, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        std::sort(C.child.begin() + C.child_ptr[(size_t)i], C.child.begin() + C.child_ptr[(size_t)i + 1]);
+    });
   }
 }
 
