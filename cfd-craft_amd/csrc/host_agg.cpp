@@ -568,21 +568,38 @@ bool bins_level0(const HostGrid& g, const std::vector<int32_t>& node, const AggL
   std::lock_guard<std::mutex> lock(join);
   lists[next_list++] = std::move(list);
   });
-  // by row, columns ascending (each pair comes from one interface)
+  // by row, columns ascending (each pair comes from one interface, so the
+  // order in which a row's entries are placed does not matter once the row is
+  // sorted): counted and placed over the lists' threads with atomic counters
   size_t total = 0;
-  for (const auto& l : lists) {
-    total += l.size();
-    for (const Entry& e : l) count[(size_t)e.r + 1] += 1;
-  }
-  for (int64_t r = 0; r < m; ++r) count[(size_t)r + 1] += count[(size_t)r];
-  std::vector<Entry> sorted(total);
-  {
-    std::vector<int64_t> at(count.begin(), count.end() - 1);
-    for (auto& l : lists) {
-      for (const Entry& e : l) sorted[(size_t)at[(size_t)e.r]++] = e;
-      std::vector<Entry>().swap(l);
+  for (const auto& l : lists) total += l.size();
+  std::vector<std::atomic<int64_t>> at((size_t)m);
+  parallel_ranges(m, [&](int64_t r0, int64_t r1) {
+    for (int64_t r = r0; r < r1; ++r) at[(size_t)r].store(0, std::memory_order_relaxed);
+  });
+  parallel_ranges((int64_t)lists.size(), [&](int64_t l0, int64_t l1) {
+    for (int64_t l = l0; l < l1; ++l)
+      for (const Entry& e : lists[(size_t)l]) at[(size_t)e.r].fetch_add(1, std::memory_order_relaxed);
+  });
+  for (int64_t r = 0; r < m; ++r) count[(size_t)r + 1] = count[(size_t)r] + at[(size_t)r].load(std::memory_order_relaxed);
+  parallel_ranges(m, [&](int64_t r0, int64_t r1) {
+    for (int64_t r = r0; r < r1; ++r) at[(size_t)r].store(count[(size_t)r], std::memory_order_relaxed);
+  });
+  // every entry placed below: no zero fill
+  std::unique_ptr<Entry[]> sorted_mem(new Entry[total ? total : 1]);
+  Entry* sorted_p = sorted_mem.get();
+  parallel_ranges((int64_t)lists.size(), [&](int64_t l0, int64_t l1) {
+    for (int64_t l = l0; l < l1; ++l) {
+      for (const Entry& e : lists[(size_t)l]) sorted_p[at[(size_t)e.r].fetch_add(1, std::memory_order_relaxed)] = e;
+      std::vector<Entry>().swap(lists[(size_t)l]);
     }
-  }
+  });
+  struct View {
+    Entry* p;
+    size_t n;
+    size_t size() const { return n; }
+    Entry& operator[](size_t i) const { return p[i]; }
+  } sorted{sorted_p, total};
   concurrently([&] { C.ea.resize(sorted.size()); }, [&] { C.eb.resize(sorted.size()); },
                [&] { C.ew.resize(sorted.size()); });
   // rows are independent: sorted and copied over threads
