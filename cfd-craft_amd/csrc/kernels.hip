@@ -2085,13 +2085,20 @@ __global__ void __launch_bounds__(BS) k_rb_gs2(Geo g, FineRB<T> F, int colour, c
 // The same half-sweep with consecutive threads on consecutive pairs of the
 // colour's array (no idle lanes where a row is not a multiple of the tile
 // width, whole-wave linear streams); each XCD takes a contiguous range of blocks.
+// Threads of a pair half-sweep workgroup: 512 for the FP64 level of one grid
+// (69.4 -> 67.8 us on the 384x384x160 canyon), 256 for the FP32 level (39.2
+// against 41.3 us with 512) and for batches of smaller grids.
+template <class T>
+constexpr int gs2_threads() {
+  return std::is_same<T, double>::value ? 512 : BS;
+}
 template <class T>
 __device__ __forceinline__ void rb_gs2_lin_body(const Geo& g, const FineRB<T>& F, int colour,
                                                 double* __restrict__ out64, const PcgState* st, unsigned bid,
                                                 unsigned nblk) {
   using T2 = std::conditional_t<std::is_same<T, double>::value, double2, float2>;
   const int status = st ? st->status : ST_RUNNING;
-  const unsigned pair = (unsigned)xcd_tile(bid, nblk) * BS + threadIdx.x;
+  const unsigned pair = (unsigned)xcd_tile(bid, nblk) * blockDim.x + threadIdx.x;
   const unsigned u = 2 * pair;
   if (u >= (unsigned)F.half) return;
   const unsigned hx = (unsigned)F.hx, ny = (unsigned)g.n[1];
@@ -2114,7 +2121,7 @@ __device__ __forceinline__ void rb_gs2_lin_body(const Geo& g, const FineRB<T>& F
   }
 }
 template <class T>
-__global__ void __launch_bounds__(BS) k_rb_gs2_lin(Geo g, FineRB<T> F, int colour, double* __restrict__ out64,
+__global__ void __launch_bounds__(512) k_rb_gs2_lin(Geo g, FineRB<T> F, int colour, double* __restrict__ out64,
                                                    const PcgState* st) {
   rb_gs2_lin_body<T>(g, F, colour, out64, st, blockIdx.x, gridDim.x);
 }
@@ -2353,7 +2360,7 @@ void launch_rb_half(const Geo& g0, const FineRB<T>& F, int colour, const T* xc, 
   // read once), and z when written
   const double bytes = (double)F.half * (1 + 3 * sizeof(T) + (out64 ? 8 : 0));
   if (F.two_cells == 2 && g0.n[0] % 4 == 0 && !xc)
-    PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs2_lin<T>), dim3((unsigned)((F.half / 2 + BS - 1) / BS)), dim3(BS), 0,
+    PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs2_lin<T>), dim3((unsigned)((F.half / 2 + gs2_threads<T>() - 1) / gs2_threads<T>())), dim3(gs2_threads<T>()), 0,
                   s, g0, F, colour, out64, st);
   else if (F.two_cells && g0.n[0] % 4 == 0 && !xc)
     PROBED_LAUNCH(PROBE_FINE_HALF, bytes, (k_rb_gs2<T>), dim3(((F.hx + 127) / 128) * ((g0.n[1] + 3) / 4) * g0.n[2]),
